@@ -1,0 +1,172 @@
+// Context, memory and error handling of the C-ABI (include/fh_hair.h).
+#include <cstring>
+
+#include "fh_internal.hpp"
+
+namespace {
+thread_local std::string g_last_error;
+
+// GPUCommandEncoder.clearBuffer as a kernel: graph-capture safe, unlike a
+// memset node.
+__global__ void fill_u32_kernel(uint32_t* __restrict__ dst, uint64_t n, uint32_t value) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = value;
+}
+}  // namespace
+
+namespace fh {
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(FH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return FH_OK;
+}
+
+int ensure_scratch(fh_ctx* ctx, size_t bytes) {
+  if (ctx->scratch_bytes >= bytes) return FH_OK;
+  if (ctx->scratch) {
+    FH_HIP(hipStreamSynchronize(ctx->stream));
+    FH_HIP(hipFree(ctx->scratch));
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+  }
+  size_t sz = align_up(bytes + bytes / 4, 1 << 20);
+  FH_HIP(hipMalloc(&ctx->scratch, sz));
+  ctx->scratch_bytes = sz;
+  return FH_OK;
+}
+
+int fill_u32(fh_ctx* ctx, void* dst, uint64_t n_words, uint32_t value) {
+  if (n_words == 0) return FH_OK;
+  uint64_t blocks = (n_words + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  fill_u32_kernel<<<(unsigned)blocks, 256, 0, ctx->stream>>>((uint32_t*)dst, n_words, value);
+  return check_launch("fill_u32_kernel");
+}
+
+}  // namespace fh
+
+extern "C" {
+
+const char* fh_last_error(void) { return g_last_error.c_str(); }
+
+int fh_abi_version(void) { return FH_ABI_VERSION; }
+
+int fh_create(int device, fh_ctx** out) {
+  FH_REQUIRE(out != nullptr, "fh_create: out is null");
+  *out = nullptr;
+  int count = 0;
+  FH_HIP(hipGetDeviceCount(&count));
+  FH_REQUIRE(device >= 0 && device < count, "fh_create: no such device");
+  FH_HIP(hipSetDevice(device));
+  fh_ctx* c = new fh_ctx();
+  c->device = device;
+  hipDeviceProp_t prop;
+  hipError_t e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess) {
+    delete c;
+    return fh::fail(FH_ERR_HIP, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
+  }
+  c->num_cus = prop.multiProcessorCount;
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fh::fail(FH_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  c->owns_stream = true;
+  *out = c;
+  return FH_OK;
+}
+
+int fh_destroy(fh_ctx* ctx) {
+  if (!ctx) return FH_OK;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  if (ctx->scratch) hipFree(ctx->scratch);
+  if (ctx->owns_stream && ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return FH_OK;
+}
+
+int fh_set_stream(fh_ctx* ctx, void* stream) {
+  FH_CHECK_CTX(ctx);
+  if (ctx->owns_stream && ctx->stream) {
+    FH_HIP(hipStreamSynchronize(ctx->stream));
+    FH_HIP(hipStreamDestroy(ctx->stream));
+  }
+  ctx->stream = (hipStream_t)stream;
+  ctx->owns_stream = false;
+  return FH_OK;
+}
+
+void* fh_get_stream(fh_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int fh_sync(fh_ctx* ctx) {
+  FH_CHECK_CTX(ctx);
+  FH_HIP(hipStreamSynchronize(ctx->stream));
+  return FH_OK;
+}
+
+int fh_alloc(fh_ctx* ctx, uint64_t bytes, void** out) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(out != nullptr, "fh_alloc: out is null");
+  FH_HIP(hipSetDevice(ctx->device));
+  FH_HIP(hipMalloc(out, bytes ? bytes : 4));
+  return FH_OK;
+}
+
+int fh_free(fh_ctx* ctx, void* ptr) {
+  FH_CHECK_CTX(ctx);
+  if (ptr) FH_HIP(hipFree(ptr));
+  return FH_OK;
+}
+
+int fh_upload(fh_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+  FH_CHECK_CTX(ctx);
+  FH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  FH_HIP(hipStreamSynchronize(ctx->stream));
+  return FH_OK;
+}
+
+int fh_download(fh_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+  FH_CHECK_CTX(ctx);
+  FH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  FH_HIP(hipStreamSynchronize(ctx->stream));
+  return FH_OK;
+}
+
+int fh_clear(fh_ctx* ctx, void* dst, uint64_t bytes) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(bytes % 4 == 0, "fh_clear: bytes must be a multiple of 4");
+  FH_REQUIRE(dst != nullptr || bytes == 0, "fh_clear: null buffer");
+  return fh::fill_u32(ctx, dst, bytes / 4, 0u);
+}
+
+uint32_t fh_tile_count(const fh_render_cfg* cfg, uint32_t vp_w, uint32_t vp_h) {
+  if (!cfg || cfg->tileSize == 0) return 0;
+  return fh::div_ceil(vp_w, cfg->tileSize) * fh::div_ceil(vp_h, cfg->tileSize);
+}
+
+uint32_t fh_segments_buffer_entries(const fh_render_cfg* cfg, uint32_t vp_w, uint32_t vp_h,
+                                    float avg_segments_per_tile) {
+  double tiles = (double)fh_tile_count(cfg, vp_w, vp_h);
+  double cnt = tiles * (double)avg_segments_per_tile;
+  uint64_t c = (uint64_t)cnt;
+  if ((double)c < cnt) c++;
+  return (uint32_t)(1 + c);
+}
+
+uint32_t fh_slice_data_per_processor(const fh_render_cfg* cfg) {
+  if (!cfg) return 0;
+  return cfg->avgFragmentsPerSlice * cfg->slicesPerPixel * cfg->tileSize * cfg->tileSize;
+}
+
+}  // extern "C"

@@ -1,0 +1,51 @@
+// Host-side internals shared by the translation units of libfhhip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/fh_hair.h"
+
+struct fh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool owns_stream = false;
+  int num_cus = 0;
+  // Grow-on-demand scratch owned by the context (fine-pass gather lists,
+  // per-bin offsets, prefix-scan temporaries).
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+};
+
+namespace fh {
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int check_launch(const char* what);
+// Ensure ctx->scratch holds at least `bytes`; frees and reallocates if not.
+int ensure_scratch(fh_ctx* ctx, size_t bytes);
+
+inline uint32_t div_ceil(uint32_t a, uint32_t b) { return (a + b - 1u) / b; }
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace fh
+
+#define FH_CHECK_CTX(ctx)                                              \
+  do {                                                                 \
+    if (!(ctx)) return fh::fail(FH_ERR_INVALID_ARG, "null fh_ctx");    \
+  } while (0)
+
+#define FH_REQUIRE(cond, msg)                                          \
+  do {                                                                 \
+    if (!(cond)) return fh::fail(FH_ERR_INVALID_ARG, (msg));           \
+  } while (0)
+
+#define FH_HIP(call)                                                   \
+  do {                                                                 \
+    hipError_t e_ = (call);                                            \
+    if (e_ != hipSuccess)                                              \
+      return fh::fail(FH_ERR_HIP, std::string(#call) + ": " +          \
+                                      hipGetErrorString(e_));          \
+  } while (0)

@@ -1,0 +1,768 @@
+// Software hair rasterizer on gfx950: tile pass (K4), tile sort (K5/K6) and
+// the fine OIT pass (K7), plus the raster-helper test kernel of the reference.
+// Reference: src/passes/swHair/*.
+#include "fh_internal.hpp"
+#include "fh_raster_common.hpp"
+
+namespace fh {
+int fill_u32(fh_ctx* ctx, void* dst, uint64_t n_words, uint32_t value);
+}
+
+namespace {
+
+using namespace fh;
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// number of set bits of `mask` below the calling lane
+__device__ __forceinline__ uint32_t mask_prefix(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
+  return v;
+}
+
+struct TileGeom {
+  uint32_t vpW, vpH, tilesX, tilesY;
+};
+
+__device__ __forceinline__ TileGeom tile_geom(f2 vp, uint32_t T) {
+  TileGeom g;
+  g.vpW = f2u(vp.x);
+  g.vpH = f2u(vp.y);
+  g.tilesX = (g.vpW + T - 1u) / T;
+  g.tilesY = (g.vpH + T - 1u) / T;
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+// K4 — hairTilesPass.wgsl.ts:70-246. One thread per segment (strand-major,
+// segment fastest). For every tile of the segment's rectangle, only the pixels
+// within the segment's bounding box (+1 px) are tested: the others fail the
+// edge tests anyway. Appends are compacted per wave: one atomicAdd on
+// drawnSegmentsCount per wave per tile round (ballot + mbcnt), then the
+// (tile, bin) head exchange and depth maxima as in the reference.
+// ---------------------------------------------------------------------------
+struct TilesArgs {
+  const fh_render_uniforms* u;
+  const fh_hair_data* hd;
+  const float4* pos;
+  const float4* tan;
+  fh_tile_result* tiles;
+  const float* depth;
+  uint32_t depthW, depthH;
+  uint32_t* segCounter;          // DrawnHairSegments.drawnSegmentsCount
+  fh_list_element* segData;      // DrawnHairSegments.data
+  uint32_t limit;                // min(maxDrawnHairSegments, capacity)
+  uint32_t* countPerTile;
+  uint32_t* slotKey;
+  uint32_t T, B, threshold;
+  uint32_t S, P;
+};
+
+__global__ __launch_bounds__(256) void tiles_draw_kernel(TilesArgs a) {
+  const fh_render_uniforms* __restrict__ u = a.u;
+  const uint32_t segsPerStrand = a.P - 1u;
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nSeg = a.S * segsPerStrand;
+  const bool active = gid < nSeg;
+  const uint32_t strand = active ? gid / segsPerStrand : 0u;
+  const uint32_t seg = active ? gid - strand * segsPerStrand : 0u;
+  const f2 vp = mk2(u->viewport[0], u->viewport[1]);
+  const TileGeom tg = tile_geom(vp, a.T);
+  const uint32_t T = a.T, B = a.B;
+  const uint32_t limit = min(a.limit, u->maxDrawnHairSegments);
+
+  // getHairDepthBoundsVS :242-246
+  const f4 bsC = mul(u->modelViewMat, mk4(a.hd->boundingSphere[0], a.hd->boundingSphere[1],
+                                          a.hd->boundingSphere[2], 1.0f));
+  const f2 dbVS = mk2(bsC.z - a.hd->boundingSphere[3], bsC.z + a.hd->boundingSphere[3]);
+
+  Seg ps;
+  uint32_t nTiles = 0, tminx = 0, tminy = 0, tsx = 1;
+  int cx0 = 1, cy0 = 1, cx1 = 0, cy1 = 0;
+  if (active) {
+    ps = project_segment(u, a.pos, a.tan, a.P, strand, seg, vp, u->fiberRadius);
+    // getRasterizedHairBounds — swRasterizeHair.wgsl.ts:111-123
+    f2 bmax = vmax(vmax(ps.v00, ps.v01), vmax(ps.v10, ps.v11));
+    f2 bmin = vmin(vmin(ps.v00, ps.v01), vmin(ps.v10, ps.v11));
+    bmax = vmin(mk2(ceilf(bmax.x), ceilf(bmax.y)), vp);
+    bmin = vmax(mk2(floorf(bmin.x), floorf(bmin.y)), mk2(0.0f, 0.0f));
+    tminx = f2u(bmin.x) / T; tminy = f2u(bmin.y) / T;
+    const uint32_t tmaxx = f2u(bmax.x) / T, tmaxy = f2u(bmax.y) / T;
+    tsx = (tmaxx - tminx) + 1u;
+    const uint32_t tsy = (tmaxy - tminy) + 1u;
+    // reject degenerate segments (:109-117); an inverted rectangle has no tile
+    if (!(tsx * tsy > a.threshold) && tmaxx >= tminx && tmaxy >= tminy) nTiles = tsx * tsy;
+    seg_pixel_bbox(ps, cx0, cy0, cx1, cy1);
+  }
+  const uint32_t rounds = wave_max_u32(nTiles);
+  for (uint32_t k = 0; k < rounds; k++) {
+    bool hit = false;
+    uint32_t tx = 0, ty = 0, dbin = B;
+    float dmin = 999.0f, dmax = -999.0f;
+    if (k < nTiles) {
+      ty = tminy + k / tsx;
+      tx = tminx + k % tsx;
+      // processTile :139-238 with getTileBoundsPx (tileUtils.wgsl.ts:44-54)
+      const uint32_t bx0 = clampu(tx * T, 0u, tg.vpW - 1u), by0 = clampu(ty * T, 0u, tg.vpH - 1u);
+      const uint32_t bx1 = clampu((tx + 1u) * T, 0u, tg.vpW - 1u), by1 = clampu((ty + 1u) * T, 0u, tg.vpH - 1u);
+      const int xs = max((int)bx0, cx0), xe = min((int)bx1, cx1 + 1);
+      const int ys = max((int)by0, cy0), ye = min((int)by1, cy1 + 1);
+      for (int y = ys; y < ye; y++) {
+        for (int x = xs; x < xe; x++) {
+          const f2 p = mk2((float)x, (float)y);
+          if (!inside_quad(ps, p)) continue;
+          const f2 w = interp_quad(ps, p);
+          const float hdp = interp_f32(w, ps.depths);
+          const float db = tex_load(a.depth, a.depthW, a.depthH, x, (int32_t)(tg.vpH - (uint32_t)y));
+          if (hdp > db) continue;
+          const f3 vs = project(u->projMatrixInv, mk4(p.x, p.y, hdp, 1.0f));
+          const uint32_t bin = (B - 1u) - depth_bin(B, dbVS, vs.z);
+          dmin = fmn(dmin, hdp);
+          dmax = fmx(dmax, hdp);
+          dbin = min(dbin, bin);
+        }
+      }
+      hit = !(dmin > 1.0f);
+    }
+    const uint64_t mask = __ballot(hit);
+    if (mask == 0) continue;
+    uint32_t base = 0;
+    const uint32_t leader = (uint32_t)__builtin_ctzll(mask);
+    if (lane_id() == leader) base = atomicAdd(a.segCounter, (uint32_t)__popcll(mask));
+    base = (uint32_t)__shfl((int)base, (int)leader);
+    if (hit) {
+      const uint32_t slot = base + mask_prefix(mask);
+      if (slot < limit) {
+        const uint32_t tb = ty * tg.tilesX * B + tx * B + dbin;  // getHairTileDepthBinIdx
+        fh_tile_result* t = &a.tiles[tb];
+        atomicMax(&t->maxDepth, f2u(dmax * 4294967296.0f));
+        atomicMax(&t->minDepth, f2u((1.0f - dmin) * 4294967296.0f));
+        const uint32_t last = atomicExch(&t->tileSegmentPtr, slot + 1u);
+        atomicAdd(&t->padding0, 1u);
+        const uint32_t prevPtr = last == 0u ? kInvalidPtr : last - 1u;
+        *reinterpret_cast<uint2*>(&a.segData[slot]) = make_uint2((seg << 24) | strand, prevPtr);
+        if (a.slotKey) a.slotKey[slot] = tb;
+        atomicAdd(&a.countPerTile[ty * tg.tilesX + tx], 1u);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// For lists written by another tile-pass implementation: one thread per
+// (tile, bin) walks its list, tags each slot and records the list length.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void rebuild_keys_kernel(fh_tile_result* __restrict__ tiles,
+                                                           const fh_list_element* __restrict__ data,
+                                                           uint32_t limit, uint32_t nTB,
+                                                           uint32_t* __restrict__ slotKey) {
+  const uint32_t tb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tb >= nTB) return;
+  uint32_t ptr = tiles[tb].tileSegmentPtr;
+  ptr = ptr == 0u ? kInvalidPtr : ptr - 1u;
+  uint32_t n = 0;
+  while (ptr != kInvalidPtr && ptr < limit && n < limit) {
+    slotKey[ptr] = tb;
+    n++;
+    ptr = data[ptr].next;
+  }
+  tiles[tb].padding0 = n;
+}
+
+// ---------------------------------------------------------------------------
+// K5 — hairTileSortPass.countTiles.wgsl.ts:63-97, with the bucket histogram
+// aggregated in LDS first (one global atomic per bucket per workgroup).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void sort_count_kernel(const uint32_t* __restrict__ counts,
+                                                         uint32_t nTiles, uint32_t nBuckets,
+                                                         uint32_t bucketSize,
+                                                         fh_sort_bucket* __restrict__ buckets) {
+  __shared__ uint32_t hist[64];
+  if (threadIdx.x < 64) hist[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nTiles) {
+    const uint32_t c = counts[t];
+    if (c > 0u) atomicAdd(&hist[min(c / bucketSize, nBuckets - 1u)], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < nBuckets && hist[threadIdx.x]) atomicAdd(&buckets[threadIdx.x].tileCount, hist[threadIdx.x]);
+}
+
+// K6 — hairTileSortPass.sort.wgsl.ts:37-75: heavier buckets first; each
+// workgroup reserves its in-bucket range with one atomic per bucket.
+__global__ __launch_bounds__(256) void sort_scatter_kernel(const uint32_t* __restrict__ counts,
+                                                           uint32_t nTiles, uint32_t nBuckets,
+                                                           uint32_t bucketSize,
+                                                           fh_sort_bucket* __restrict__ buckets,
+                                                           uint32_t* __restrict__ tileList) {
+  __shared__ uint32_t bucketOffset[64];
+  __shared__ uint32_t localCount[64];
+  __shared__ uint32_t localBase[64];
+  __shared__ uint32_t total;
+  if (threadIdx.x == 0) {
+    // calculateOffsetsForEachBucket :68-75
+    uint32_t off = 0;
+    for (uint32_t i = 0; i < nBuckets; i++) {
+      const uint32_t idx = nBuckets - 1u - i;
+      bucketOffset[idx] = off;
+      off += buckets[idx].tileCount;
+    }
+    total = 0;
+  }
+  if (threadIdx.x < 64) localCount[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t b = 0, rank = 0;
+  const bool has = t < nTiles && counts[t] > 0u;
+  if (has) {
+    b = min(counts[t] / bucketSize, nBuckets - 1u);
+    rank = atomicAdd(&localCount[b], 1u);
+    atomicAdd(&total, 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < nBuckets && localCount[threadIdx.x])
+    localBase[threadIdx.x] = atomicAdd(&buckets[threadIdx.x].writeOffset, localCount[threadIdx.x]);
+  if (threadIdx.x == 0 && total) atomicAdd(&tileList[0], total);
+  __syncthreads();
+  if (has) tileList[1 + bucketOffset[b] + localBase[b] + rank] = t;
+}
+
+// ---------------------------------------------------------------------------
+// Fine-pass gather: the per-(tile, bin) linked lists become one contiguous
+// range per (tile, bin) of a scratch array, so the fine pass streams each
+// list instead of chasing pointers. Step 1 is an inclusive scan of the list
+// lengths (HairTileResult.padding0); step 2 scatters every slot to
+// --end[key], leaving end[key] at the first element of the range.
+// ---------------------------------------------------------------------------
+constexpr int kScanBlock = 1024;
+
+__device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t v, uint32_t* warpSums) {
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t n = (uint32_t)__shfl_up((int)v, o);
+    if (lane >= (uint32_t)o) v += n;
+  }
+  if (lane == 63u) warpSums[wid] = v;
+  __syncthreads();
+  if (wid == 0) {
+    const uint32_t nw = blockDim.x >> 6;
+    uint32_t s = lane < nw ? warpSums[lane] : 0u;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+      const uint32_t n = (uint32_t)__shfl_up((int)s, o);
+      if (lane >= (uint32_t)o) s += n;
+    }
+    if (lane < nw) warpSums[lane] = s;
+  }
+  __syncthreads();
+  if (wid > 0) v += warpSums[wid - 1];
+  return v;
+}
+
+__global__ __launch_bounds__(kScanBlock) void bin_scan_local_kernel(const fh_tile_result* __restrict__ tiles,
+                                                                    uint32_t n, uint32_t* __restrict__ ends,
+                                                                    uint32_t* __restrict__ blockSums) {
+  __shared__ uint32_t warpSums[16];
+  const uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
+  const uint32_t v = i < n ? tiles[i].padding0 : 0u;
+  const uint32_t s = block_inclusive_scan(v, warpSums);
+  if (i < n) ends[i] = s;
+  if (threadIdx.x == kScanBlock - 1) blockSums[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kScanBlock) void bin_scan_add_kernel(uint32_t n, uint32_t* __restrict__ ends,
+                                                                  const uint32_t* __restrict__ blockSums) {
+  __shared__ uint32_t red[16];
+  // prefix of this block = sum of the sums of all earlier blocks
+  uint32_t acc = 0;
+  for (uint32_t j = threadIdx.x; j < blockIdx.x; j += kScanBlock) acc += blockSums[j];
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) acc += (uint32_t)__shfl_xor((int)acc, o);
+  if ((threadIdx.x & 63u) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  uint32_t prefix = 0;
+  for (uint32_t w = 0; w < kScanBlock / 64; w++) prefix += red[w];
+  const uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
+  if (i < n && prefix) ends[i] += prefix;
+}
+
+__global__ __launch_bounds__(256) void bin_scatter_kernel(const fh_render_uniforms* __restrict__ u,
+                                                          const uint32_t* __restrict__ segCounter,
+                                                          const fh_list_element* __restrict__ data,
+                                                          uint32_t capacity, const uint32_t* __restrict__ slotKey,
+                                                          uint32_t* __restrict__ ends,
+                                                          uint32_t* __restrict__ binSeg) {
+  const uint32_t n = min(*segCounter, min(capacity, u->maxDrawnHairSegments));
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    const uint32_t key = slotKey[s];
+    const uint32_t pos = atomicSub(&ends[key], 1u) - 1u;
+    binSeg[pos] = data[s].strandAndSegmentIdxs;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K7 — hairFinePass.wgsl.ts:126-335 with processHairSegment.wgsl.ts:5-70 and
+// reduceHairSlices.wgsl.ts:9-78. Persistent workgroups ("processors") pop
+// tiles from the sorted tile list; one thread per pixel. Per depth bin:
+//  * up to one segment per thread is projected into LDS, then every pixel
+//    tests the whole batch (2 barriers per batch instead of 2 per segment);
+//  * a covered pixel's fragment (f16 colour, coverage alpha, key) goes to
+//    the processor's fragment pool, LDS first and the per-processor global
+//    region beyond it, and is linked into its slice list in ascending
+//    (strand, segment) order, which makes the blend order independent of
+//    the atomic order in which the tile pass appended the segments;
+//  * slices are then blended front to back into the pixel.
+// ---------------------------------------------------------------------------
+struct FineArgs {
+  const fh_render_uniforms* u;
+  const float4* pos;
+  const float4* tan;
+  const fh_tile_result* tiles;
+  const uint32_t* binStart;  // first element of each (tile, bin) range
+  const uint32_t* binSeg;    // packed (segment << 24 | strand)
+  uint4* slicePool;          // processorCount * perProc entries
+  uint32_t* queue;           // HairRasterResult.tileQueueAtomicIdx
+  float4* fb;                // HairRasterResult.data
+  const float* depth;
+  uint32_t depthW, depthH;
+  const float4* shading;
+  const uint32_t* tileList;
+  uint32_t S, P;
+  uint32_t B, NS, PS;
+  uint32_t perProc, ldsPool, batch;
+  uint32_t alphaQuadratic;
+  float alphaMultiplier;
+};
+
+struct SegRec {
+  float v00x, v00y, v01x, v01y, v10x, v10y, v11x, v11y;
+  float d0, d1, d2, d3;
+  uint32_t packed;
+  int16_t bx0, by0, bx1, by1;
+  uint32_t pad;
+};
+static_assert(sizeof(SegRec) == 64, "SegRec must be 64 bytes");
+
+template <int TILE>
+__global__ __launch_bounds__(TILE* TILE) void fine_raster_kernel(FineArgs a) {
+  constexpr uint32_t NPX = TILE * TILE;
+  extern __shared__ __align__(16) unsigned char smem[];
+  SegRec* recs = reinterpret_cast<SegRec*>(smem);
+  uint32_t* heads = reinterpret_cast<uint32_t*>(smem + (size_t)a.batch * sizeof(SegRec));
+  uint4* pool = reinterpret_cast<uint4*>(heads + (size_t)a.NS * NPX);
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_sliceOff;
+
+  const fh_render_uniforms* __restrict__ u = a.u;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lx = tid % TILE, ly = tid / TILE;
+  const f2 vpf = mk2(u->viewport[0], u->viewport[1]);
+  const TileGeom tg = tile_geom(vpf, TILE);
+  const uint32_t tileCount = tg.tilesX * tg.tilesY;
+  const uint32_t drawnTiles = a.tileList[0];
+  const uint32_t fbLen = tg.vpW * tg.vpH - 1u;  // W*H*16 B minus the 16 B header
+  const uint32_t dispMode = u->displayMode & 0xffu;
+  const uint32_t dbgMax = dispMode == 4u ? ((u->displayMode >> 8) & 0xffffu) : 0u;  // USED_SLICES
+  const bool isDbg = dbgMax != 0u;
+  const float invSegs = (float)(a.P - 1u);
+  uint4* gpool = a.slicePool + (size_t)blockIdx.x * a.perProc;
+
+  for (uint32_t s = 0; s < a.NS; s++) heads[s * NPX + tid] = kInvalidPtr;
+
+  while (true) {
+    __syncthreads();
+    if (tid == 0) {
+      const uint32_t q = atomicAdd(a.queue, 1u);  // getNextTileIdx :294-303
+      s_tile = q < drawnTiles ? a.tileList[1 + q] : kInvalidPtr;
+    }
+    __syncthreads();
+    const uint32_t tileIdx = s_tile;
+    if (tileIdx == kInvalidPtr) break;
+    const uint32_t row = tileIdx / tg.tilesX;
+    const uint32_t tx = tileIdx - tg.tilesX * row, ty = row;
+    const uint32_t bx0 = clampu(tx * TILE, 0u, tg.vpW - 1u), by0 = clampu(ty * TILE, 0u, tg.vpH - 1u);
+    const uint32_t px = bx0 + lx, py = by0 + ly;
+    const f2 p = mk2((float)px, (float)py);
+    const int ipx = (int)px, ipy = (int)py;
+    const bool readable = px < tg.vpW && py < tg.vpH && (tg.vpW * py + px) < fbLen;
+    const uint32_t fbIdx = tg.vpW * py + px;
+    const int32_t depthY = f2i(vpf.y - p.y);
+    bool done = false;
+    f4 fc = mk4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (readable) { const float4 v = a.fb[fbIdx]; fc = mk4(v.x, v.y, v.z, v.w); }
+
+    for (uint32_t bin = 0; bin < a.B && tileIdx < tileCount; bin++) {
+      const uint32_t tb = tileIdx * a.B + bin;
+      const fh_tile_result tr = a.tiles[tb];
+      if (tr.maxDepth != 0u) {  // _getTileDepth == 0 means an empty bin (:215)
+        const f2 td = mk2((float)(0xffffffffu - tr.minDepth) / 4294967296.0f,
+                          (float)tr.maxDepth / 4294967296.0f);
+        const uint32_t n = tr.padding0;
+        const uint32_t start = a.binStart[tb];
+        if (tid == 0) s_sliceOff = 0;
+        for (uint32_t b0 = 0; b0 < n; b0 += a.batch) {
+          const uint32_t nb = min(a.batch, n - b0);
+          __syncthreads();
+          if (tid < nb) {
+            const uint32_t packed = a.binSeg[start + b0 + tid];
+            const uint32_t strand = packed & 0x00ffffffu, seg = packed >> 24;
+            const Seg sg = project_segment(u, a.pos, a.tan, a.P, strand, seg, vpf, u->fiberRadius);
+            int x0, y0, x1, y1;
+            seg_pixel_bbox(sg, x0, y0, x1, y1);
+            SegRec r;
+            r.v00x = sg.v00.x; r.v00y = sg.v00.y; r.v01x = sg.v01.x; r.v01y = sg.v01.y;
+            r.v10x = sg.v10.x; r.v10y = sg.v10.y; r.v11x = sg.v11.x; r.v11y = sg.v11.y;
+            r.d0 = sg.depths.x; r.d1 = sg.depths.y; r.d2 = sg.depths.z; r.d3 = sg.depths.w;
+            r.packed = packed;
+            // relative to the tile origin, clamped to the int16 range
+            r.bx0 = (int16_t)max(min(x0 - (int)bx0, 30000), -30000);
+            r.by0 = (int16_t)max(min(y0 - (int)by0, 30000), -30000);
+            r.bx1 = (int16_t)max(min(x1 - (int)bx0, 30000), -30000);
+            r.by1 = (int16_t)max(min(y1 - (int)by0, 30000), -30000);
+            r.pad = 0;
+            recs[tid] = r;
+          }
+          __syncthreads();
+          if (!done) {
+            for (uint32_t j = 0; j < nb; j++) {
+              const SegRec& r = recs[j];
+              if ((int)lx < r.bx0 || (int)lx > r.bx1 || (int)ly < r.by0 || (int)ly > r.by1) continue;
+              Seg sg;
+              sg.v00 = mk2(r.v00x, r.v00y); sg.v01 = mk2(r.v01x, r.v01y);
+              sg.v10 = mk2(r.v10x, r.v10y); sg.v11 = mk2(r.v11x, r.v11y);
+              if (!inside_quad(sg, p)) continue;
+              sg.depths = mk4(r.d0, r.d1, r.d2, r.d3);
+              const f2 w = interp_quad(sg, p);
+              const float hdp = interp_f32(w, sg.depths);
+              const float db = tex_load(a.depth, a.depthW, a.depthH, ipx, depthY);
+              if (hdp >= db) continue;
+              const uint32_t slot = atomicAdd(&s_sliceOff, 1u);
+              if (slot >= a.perProc) continue;
+              // getAlphaCoverage (processHairSegment.wgsl.ts:61-70)
+              float alpha = 1.0f - fabsf(w.x * 2.0f - 1.0f);
+              if (a.alphaQuadratic) alpha = sqrtf(alpha);
+              alpha = satf(alpha * a.alphaMultiplier);
+              const uint32_t strand = r.packed & 0x00ffffffu, seg = r.packed >> 24;
+              const float tFull = ((float)seg + w.y) / invSegs;
+              // _sampleShading — hairShadingBuffer.ts:29-38
+              const float sa = tFull * ((float)a.PS - 1.0f);
+              const uint32_t au = f2u(sa);
+              const uint32_t i0 = clampu(au, 0u, a.PS - 1u), i1 = clampu(au + 1u, 0u, a.PS - 1u);
+              const float fr = fractf(sa);
+              const float4 c0 = a.shading[(size_t)strand * a.PS + i0];
+              const float4 c1 = a.shading[(size_t)strand * a.PS + i1];
+              const float cr = mixf(c0.x, c1.x, fr), cg = mixf(c0.y, c1.y, fr);
+              const float cb = mixf(c0.z, c1.z, fr), ca = mixf(c0.w, c1.w, fr) * alpha;
+              const uint32_t sl = depth_bin(a.NS, td, hdp);
+              const uint32_t key = (strand << 8) | seg;
+              // sorted insert, ascending key from the head
+              uint32_t* headp = &heads[sl * NPX + tid];
+              uint32_t prevSlot = kInvalidPtr, cur = *headp;
+              while (cur != kInvalidPtr) {
+                const uint4 e = cur < a.ldsPool ? pool[cur] : gpool[cur];
+                if (e.w > key) break;
+                prevSlot = cur;
+                cur = e.z;
+              }
+              const uint4 entry = make_uint4(pack_half2(cr, cg), pack_half2(cb, ca), cur, key);
+              if (slot < a.ldsPool) pool[slot] = entry; else gpool[slot] = entry;
+              if (prevSlot == kInvalidPtr) {
+                *headp = slot;
+              } else if (prevSlot < a.ldsPool) {
+                pool[prevSlot].z = slot;
+              } else {
+                gpool[prevSlot].z = slot;
+              }
+            }
+          }
+          __syncthreads();
+          if (s_sliceOff >= a.perProc) break;  // slice memory exhausted (:252-264)
+        }
+        // reduceHairSlices — reduceHairSlices.wgsl.ts:9-73
+        if (!done) {
+          if (!readable) fc = mk4(0.0f, 0.0f, 0.0f, 0.0f);
+          uint32_t cnt = isDbg ? f2u(fc.x * (float)dbgMax) : 0u;
+          for (uint32_t s = 0; s < a.NS; s++) {
+            uint32_t cur = heads[s * NPX + tid];
+            heads[s * NPX + tid] = kInvalidPtr;
+            if (!isDbg && fc.w >= 0.999f) continue;
+            while (cur != kInvalidPtr) {
+              if (!isDbg && fc.w >= 0.999f) break;
+              const uint4 e = cur < a.ldsPool ? pool[cur] : gpool[cur];
+              cur = e.z;
+              cnt++;
+              const float oneMinusA = 1.0f - fc.w;
+              fc.x = fc.x + half_lo(e.x) * oneMinusA;
+              fc.y = fc.y + half_hi(e.x) * oneMinusA;
+              fc.z = fc.z + half_lo(e.y) * oneMinusA;
+              fc.w = fc.w + half_hi(e.y) * oneMinusA;
+            }
+          }
+          if (isDbg) {
+            const float c = satf((float)cnt / (float)dbgMax);
+            fc = mk4(c, 0.0f, 0.0f, 1.0f);
+          }
+          if (readable) a.fb[fbIdx] = make_float4(fc.x, fc.y, fc.z, fc.w);
+          done = !isDbg && fc.w >= 0.999f;
+        }
+      }
+      // checkAllPixelsInWkgrpDone :305-327
+      if (__syncthreads_and(done)) break;
+    }
+  }
+}
+
+// swRasterizer.test.ts:184-248 — the reference's raster-helper test kernel.
+__global__ __launch_bounds__(64) void debug_raster_kernel(const fh_render_uniforms* __restrict__ u,
+                                                          const float4* __restrict__ pos,
+                                                          const float4* __restrict__ tan, uint32_t S,
+                                                          uint32_t P, uint32_t* __restrict__ out) {
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nSeg = S * (P - 1u);
+  if (gid >= nSeg) return;
+  const uint32_t strand = gid / (P - 1u), seg = gid - strand * (P - 1u);
+  const f2 vp = mk2(u->viewport[0], u->viewport[1]);
+  const uint32_t W = f2u(vp.x), H = f2u(vp.y);
+  const Seg ps = project_segment(u, pos, tan, P, strand, seg, vp, u->fiberRadius);
+  f2 bmax = vmax(vmax(ps.v00, ps.v01), vmax(ps.v10, ps.v11));
+  f2 bmin = vmin(vmin(ps.v00, ps.v01), vmin(ps.v10, ps.v11));
+  bmax = vmin(mk2(ceilf(bmax.x), ceilf(bmax.y)), vp);
+  bmin = vmax(mk2(floorf(bmin.x), floorf(bmin.y)), mk2(0.0f, 0.0f));
+  for (float y = bmin.y; y < bmax.y; y += 1.0f)
+    for (float x = bmin.x; x < bmax.x; x += 1.0f) {
+      const f2 p = mk2(x, y);
+      if (!inside_quad(ps, p)) continue;
+      const f2 w = interp_quad(ps, p);
+      const uint32_t c0 = f2u(satf(w.x) * 255.0f), c1 = f2u(satf(w.y) * 255.0f);
+      const uint32_t c2 = f2u(satf(0.1f) * 255.0f);
+      const uint32_t value = 0xff000000u | c0 | (c1 << 8) | (c2 << 16);
+      const uint32_t ix = f2u(x), iy = f2u(y);
+      if (ix >= W || iy >= H) continue;
+      const size_t idx = (size_t)(H - iy) * W + ix;
+      if (idx >= (size_t)W * H) continue;
+      atomicMax(&out[idx], value);
+    }
+}
+
+int check_cfg(const fh_render_cfg* cfg, const char* fn) {
+  if (!cfg) return fail(FH_ERR_INVALID_ARG, std::string(fn) + ": null cfg");
+  if (!(cfg->tileSize == 8 || cfg->tileSize == 16 || cfg->tileSize == 32))
+    return fail(FH_ERR_UNSUPPORTED, std::string(fn) + ": tileSize must be 8, 16 or 32");
+  if (cfg->tileDepthBins < 1 || cfg->tileDepthBins > 64)
+    return fail(FH_ERR_INVALID_ARG, std::string(fn) + ": tileDepthBins must be in [1, 64]");
+  if (cfg->slicesPerPixel < 1 || cfg->slicesPerPixel > 32)
+    return fail(FH_ERR_INVALID_ARG, std::string(fn) + ": slicesPerPixel must be in [1, 32]");
+  if (cfg->sortBuckets < 1 || cfg->sortBuckets > 64 || cfg->sortBucketSize < 1)
+    return fail(FH_ERR_INVALID_ARG, std::string(fn) + ": sortBuckets must be in [1, 64], sortBucketSize >= 1");
+  if (cfg->shadingPoints < 2 || cfg->shadingPoints > 1024)
+    return fail(FH_ERR_INVALID_ARG, std::string(fn) + ": shadingPoints must be in [2, 1024]");
+  return FH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fh_tiles_clear(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t vp_h,
+                   fh_tile_result* tiles, void* tile_segments, uint32_t* segment_count_per_tile) {
+  FH_CHECK_CTX(ctx);
+  int rc = check_cfg(cfg, "fh_tiles_clear");
+  if (rc) return rc;
+  FH_REQUIRE(tiles && tile_segments && segment_count_per_tile, "fh_tiles_clear: null buffer");
+  const uint64_t nt = fh_tile_count(cfg, vp_w, vp_h);
+  if ((rc = fill_u32(ctx, tiles, nt * cfg->tileDepthBins * 4, 0u))) return rc;
+  if ((rc = fill_u32(ctx, segment_count_per_tile, nt, 0u))) return rc;
+  return fill_u32(ctx, tile_segments, 1, 0u);  // drawnSegmentsCount only (hairTilesPass.ts:51)
+}
+
+int fh_tiles_draw(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t vp_h,
+                  const fh_render_uniforms* uniforms, const fh_hair_data* hair_data,
+                  const fh_vec4* positions, const fh_vec4* tangents, fh_tile_result* tiles,
+                  fh_tex2d depth, void* tile_segments, uint32_t segments_capacity,
+                  uint32_t* segment_count_per_tile, uint32_t* slot_tile_bin,
+                  uint32_t rendered_strands, uint32_t points_per_strand) {
+  FH_CHECK_CTX(ctx);
+  int rc = check_cfg(cfg, "fh_tiles_draw");
+  if (rc) return rc;
+  FH_REQUIRE(uniforms && hair_data && positions && tangents && tiles && tile_segments &&
+                 segment_count_per_tile && depth.data,
+             "fh_tiles_draw: null buffer");
+  FH_REQUIRE(points_per_strand >= 2 && points_per_strand <= 256, "fh_tiles_draw: points_per_strand in [2, 256]");
+  FH_REQUIRE(rendered_strands < (1u << 24), "fh_tiles_draw: strand index must fit 24 bits (hairTileSegmentsBuffer.ts:26)");
+  FH_REQUIRE(vp_w > 0 && vp_h > 0, "fh_tiles_draw: empty viewport");
+  const uint64_t nSeg = (uint64_t)rendered_strands * (points_per_strand - 1u);
+  if (nSeg == 0) return FH_OK;
+  FH_REQUIRE(nSeg < (1ull << 31), "fh_tiles_draw: too many segments");
+  TilesArgs a;
+  a.u = uniforms; a.hd = hair_data;
+  a.pos = reinterpret_cast<const float4*>(positions);
+  a.tan = reinterpret_cast<const float4*>(tangents);
+  a.tiles = tiles; a.depth = depth.data; a.depthW = depth.width; a.depthH = depth.height;
+  a.segCounter = reinterpret_cast<uint32_t*>(tile_segments);
+  a.segData = reinterpret_cast<fh_list_element*>(reinterpret_cast<char*>(tile_segments) + 4);
+  a.limit = segments_capacity;  // the kernel also caps by uniforms->maxDrawnHairSegments
+  a.countPerTile = segment_count_per_tile; a.slotKey = slot_tile_bin;
+  a.T = cfg->tileSize; a.B = cfg->tileDepthBins; a.threshold = cfg->invalidTilesPerSegmentThreshold;
+  a.S = rendered_strands; a.P = points_per_strand;
+  tiles_draw_kernel<<<fh::div_ceil((uint32_t)nSeg, 256), 256, 0, ctx->stream>>>(a);
+  return check_launch("tiles_draw_kernel");
+}
+
+int fh_tiles_rebuild_slot_keys(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t vp_h,
+                               fh_tile_result* tiles, const void* tile_segments,
+                               uint32_t segments_capacity, uint32_t* slot_tile_bin) {
+  FH_CHECK_CTX(ctx);
+  int rc = check_cfg(cfg, "fh_tiles_rebuild_slot_keys");
+  if (rc) return rc;
+  FH_REQUIRE(tiles && tile_segments && slot_tile_bin, "fh_tiles_rebuild_slot_keys: null buffer");
+  const uint32_t nTB = fh_tile_count(cfg, vp_w, vp_h) * cfg->tileDepthBins;
+  rebuild_keys_kernel<<<fh::div_ceil(nTB, 256), 256, 0, ctx->stream>>>(
+      tiles, reinterpret_cast<const fh_list_element*>(reinterpret_cast<const char*>(tile_segments) + 4),
+      segments_capacity, nTB, slot_tile_bin);
+  return check_launch("rebuild_keys_kernel");
+}
+
+int fh_tiles_sort_clear(fh_ctx* ctx, const fh_render_cfg* cfg, void* tile_list, fh_sort_bucket* buckets) {
+  FH_CHECK_CTX(ctx);
+  int rc = check_cfg(cfg, "fh_tiles_sort_clear");
+  if (rc) return rc;
+  FH_REQUIRE(tile_list && buckets, "fh_tiles_sort_clear: null buffer");
+  if ((rc = fill_u32(ctx, tile_list, 1, 0u))) return rc;
+  return fill_u32(ctx, buckets, (uint64_t)cfg->sortBuckets * 2, 0u);
+}
+
+int fh_tiles_sort(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t vp_h,
+                  const uint32_t* segment_count_per_tile, void* tile_list, fh_sort_bucket* buckets) {
+  FH_CHECK_CTX(ctx);
+  int rc = check_cfg(cfg, "fh_tiles_sort");
+  if (rc) return rc;
+  FH_REQUIRE(segment_count_per_tile && tile_list && buckets, "fh_tiles_sort: null buffer");
+  const uint32_t nt = fh_tile_count(cfg, vp_w, vp_h);
+  if (nt == 0) return FH_OK;
+  const uint32_t blocks = fh::div_ceil(nt, 256);
+  sort_count_kernel<<<blocks, 256, 0, ctx->stream>>>(segment_count_per_tile, nt, cfg->sortBuckets,
+                                                       cfg->sortBucketSize, buckets);
+  if ((rc = check_launch("sort_count_kernel"))) return rc;
+  sort_scatter_kernel<<<blocks, 256, 0, ctx->stream>>>(segment_count_per_tile, nt, cfg->sortBuckets,
+                                                         cfg->sortBucketSize, buckets,
+                                                         reinterpret_cast<uint32_t*>(tile_list));
+  return check_launch("sort_scatter_kernel");
+}
+
+int fh_fine_clear(fh_ctx* ctx, uint32_t vp_w, uint32_t vp_h, void* rasterizer_result) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(rasterizer_result, "fh_fine_clear: null buffer");
+  return fill_u32(ctx, rasterizer_result, (uint64_t)vp_w * vp_h * 4, 0u);
+}
+
+int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t vp_h,
+                   const fh_render_uniforms* uniforms, const fh_hair_data* hair_data,
+                   const fh_vec4* positions, const fh_tile_result* tiles, const void* tile_segments,
+                   uint32_t segments_capacity, void* slice_data, void* rasterizer_result,
+                   fh_tex2d depth, const fh_vec4* shading, const fh_vec4* tangents,
+                   const void* tile_list, const uint32_t* slot_tile_bin, uint32_t strands_count,
+                   uint32_t points_per_strand) {
+  FH_CHECK_CTX(ctx);
+  (void)hair_data;
+  int rc = check_cfg(cfg, "fh_fine_raster");
+  if (rc) return rc;
+  FH_REQUIRE(uniforms && positions && tiles && tile_segments && slice_data && rasterizer_result &&
+                 depth.data && shading && tangents && tile_list && slot_tile_bin,
+             "fh_fine_raster: null buffer");
+  FH_REQUIRE(points_per_strand >= 2 && points_per_strand <= 256, "fh_fine_raster: points_per_strand in [2, 256]");
+  FH_REQUIRE(cfg->processorCount >= 1, "fh_fine_raster: processorCount must be >= 1");
+  FH_REQUIRE(vp_w > 0 && vp_h > 0, "fh_fine_raster: empty viewport");
+  const uint32_t T = cfg->tileSize;
+  const uint32_t nTB = fh_tile_count(cfg, vp_w, vp_h) * cfg->tileDepthBins;
+  const uint32_t limit = segments_capacity;
+  const uint32_t scanBlocks = fh::div_ceil(nTB, kScanBlock);
+  // scratch: per-bin range ends/starts, per-block sums, gathered segment ids
+  const size_t offEnds = 0;
+  const size_t offSums = fh::align_up(offEnds + (size_t)nTB * 4, 256);
+  const size_t offSeg = fh::align_up(offSums + (size_t)scanBlocks * 4, 256);
+  const size_t total = fh::align_up(offSeg + (size_t)limit * 4 + 4, 256);
+  if ((rc = ensure_scratch(ctx, total))) return rc;
+  char* base = reinterpret_cast<char*>(ctx->scratch);
+  uint32_t* ends = reinterpret_cast<uint32_t*>(base + offEnds);
+  uint32_t* sums = reinterpret_cast<uint32_t*>(base + offSums);
+  uint32_t* binSeg = reinterpret_cast<uint32_t*>(base + offSeg);
+  const fh_list_element* data =
+      reinterpret_cast<const fh_list_element*>(reinterpret_cast<const char*>(tile_segments) + 4);
+
+  bin_scan_local_kernel<<<scanBlocks, kScanBlock, 0, ctx->stream>>>(tiles, nTB, ends, sums);
+  if ((rc = check_launch("bin_scan_local_kernel"))) return rc;
+  if (scanBlocks > 1) {
+    bin_scan_add_kernel<<<scanBlocks, kScanBlock, 0, ctx->stream>>>(nTB, ends, sums);
+    if ((rc = check_launch("bin_scan_add_kernel"))) return rc;
+  }
+  const uint32_t scatterBlocks = std::max(1u, std::min(fh::div_ceil(limit, 256), (uint32_t)ctx->num_cus * 8u));
+  bin_scatter_kernel<<<scatterBlocks, 256, 0, ctx->stream>>>(
+      uniforms, reinterpret_cast<const uint32_t*>(tile_segments), data, limit, slot_tile_bin, ends, binSeg);
+  if ((rc = check_launch("bin_scatter_kernel"))) return rc;
+
+  FineArgs a;
+  a.u = uniforms;
+  a.pos = reinterpret_cast<const float4*>(positions);
+  a.tan = reinterpret_cast<const float4*>(tangents);
+  a.tiles = tiles; a.binStart = ends; a.binSeg = binSeg;
+  a.slicePool = reinterpret_cast<uint4*>(slice_data);
+  a.queue = reinterpret_cast<uint32_t*>(rasterizer_result);
+  a.fb = reinterpret_cast<float4*>(reinterpret_cast<char*>(rasterizer_result) + 16);
+  a.depth = depth.data; a.depthW = depth.width; a.depthH = depth.height;
+  a.shading = reinterpret_cast<const float4*>(shading);
+  a.tileList = reinterpret_cast<const uint32_t*>(tile_list);
+  a.S = strands_count; a.P = points_per_strand;
+  a.B = cfg->tileDepthBins; a.NS = cfg->slicesPerPixel; a.PS = cfg->shadingPoints;
+  a.perProc = fh_slice_data_per_processor(cfg);
+  a.alphaQuadratic = cfg->alphaQuadratic; a.alphaMultiplier = cfg->alphaMultiplier;
+  const uint32_t npx = T * T;
+  a.batch = std::min(npx, 256u);
+  // LDS: segment batch + slice heads + fragment pool. 80 KiB per workgroup
+  // keeps two 16x16 processors per CU (160 KiB LDS on gfx950).
+  const size_t ldsTarget = T == 8 ? 32 * 1024 : (T == 16 ? 80 * 1024 : 160 * 1024);
+  const size_t fixed = (size_t)a.batch * sizeof(SegRec) + (size_t)a.NS * npx * 4;
+  FH_REQUIRE(fixed + 16 * 64 <= ldsTarget, "fh_fine_raster: slicesPerPixel too large for the tile size");
+  a.ldsPool = (uint32_t)std::min<size_t>((ldsTarget - fixed) / 16, a.perProc);
+  const size_t lds = fixed + (size_t)a.ldsPool * 16;
+  const uint32_t grid = cfg->processorCount;
+  if (T == 8) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&fine_raster_kernel<8>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    fine_raster_kernel<8><<<grid, 64, lds, ctx->stream>>>(a);
+  } else if (T == 16) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&fine_raster_kernel<16>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    fine_raster_kernel<16><<<grid, 256, lds, ctx->stream>>>(a);
+  } else {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&fine_raster_kernel<32>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    fine_raster_kernel<32><<<grid, 1024, lds, ctx->stream>>>(a);
+  }
+  return check_launch("fine_raster_kernel");
+}
+
+// Test hook: the raster-helper kernel of swRasterizer.test.ts:165-250, which
+// the reference pins with __test__/swRasterizer.snapshot.bin.
+int fh_debug_raster_barycentric(fh_ctx* ctx, const fh_render_uniforms* uniforms,
+                                const fh_vec4* positions, const fh_vec4* tangents,
+                                uint32_t strands_count, uint32_t points_per_strand, uint32_t* out) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(uniforms && positions && tangents && out, "fh_debug_raster_barycentric: null buffer");
+  FH_REQUIRE(points_per_strand >= 2, "fh_debug_raster_barycentric: points_per_strand >= 2");
+  const uint32_t nSeg = strands_count * (points_per_strand - 1u);
+  if (nSeg == 0) return FH_OK;
+  debug_raster_kernel<<<fh::div_ceil(nSeg, 64), 64, 0, ctx->stream>>>(
+      uniforms, reinterpret_cast<const float4*>(positions), reinterpret_cast<const float4*>(tangents),
+      strands_count, points_per_strand, out);
+  return check_launch("debug_raster_kernel");
+}
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+// Raster helpers of the software hair rasterizer, shared by the tile pass,
+// the fine pass and the raster test kernel.
+// Reference: src/passes/swHair/shaderImpl/swRasterizeHair.wgsl.ts,
+// src/passes/_shaderSnippets/shaderSnippets.wgls.ts, tileUtils.wgsl.ts.
+#pragma once
+
+#include "fh_device.hpp"
+
+namespace fh {
+
+constexpr uint32_t kInvalidPtr = 0xffffffffu;
+
+struct Seg {
+  f2 v00, v01, v10, v11;
+  f4 depths;
+};
+
+// projectHairPoint — swRasterizeHair.wgsl.ts:76-105. The tangent is
+// transformed with w = 1, as in the reference.
+__device__ __forceinline__ void project_point(const fh_render_uniforms* __restrict__ u,
+                                              const float4* __restrict__ pos,
+                                              const float4* __restrict__ tan, size_t k, f2 vp,
+                                              float fiberRadius, f2& v0, f2& v1, f2& dd) {
+  const float4 pk = pos[k], tk = tan[k];
+  const f4 p0 = mul(u->modelMatrix, mk4(pk.x, pk.y, pk.z, 1.0f));
+  const f4 t0 = mul(u->modelMatrix, mk4(tk.x, tk.y, tk.z, 1.0f));
+  const f3 cam = mk3(u->cameraPosition[0], u->cameraPosition[1], u->cameraPosition[2]);
+  const f3 toCam = normalize(cam - xyz(p0));
+  const f3 right = normalize(cross(xyz(t0), toCam)) * fiberRadius;
+  const f3 a = xyz(p0) - right, b = xyz(p0) + right;
+  const f3 an = project(u->vpMatrix, mk4(a.x, a.y, a.z, 1.0f));
+  const f3 bn = project(u->vpMatrix, mk4(b.x, b.y, b.z, 1.0f));
+  // ndc2viewportPx — shaderSnippets.wgls.ts:104-107
+  v0 = mk2((an.x * 0.5f + 0.5f) * vp.x, (an.y * 0.5f + 0.5f) * vp.y);
+  v1 = mk2((bn.x * 0.5f + 0.5f) * vp.x, (bn.y * 0.5f + 0.5f) * vp.y);
+  dd = mk2(an.z, bn.z);
+}
+
+// projectHairSegment — swRasterizeHair.wgsl.ts:39-68
+__device__ __forceinline__ Seg project_segment(const fh_render_uniforms* __restrict__ u,
+                                               const float4* __restrict__ pos,
+                                               const float4* __restrict__ tan, uint32_t P,
+                                               uint32_t strand, uint32_t seg, f2 vp,
+                                               float fiberRadius) {
+  Seg r;
+  f2 a, b, dd;
+  const size_t k = (size_t)strand * P + seg;
+  project_point(u, pos, tan, k, vp, fiberRadius, a, b, dd);
+  r.v00 = a; r.v01 = b; r.depths.x = dd.x; r.depths.y = dd.y;
+  project_point(u, pos, tan, k + 1, vp, fiberRadius, a, b, dd);
+  r.v10 = a; r.v11 = b; r.depths.z = dd.x; r.depths.w = dd.y;
+  return r;
+}
+
+// edgeFunction — swRasterizeHair.wgsl.ts:127-129
+__device__ __forceinline__ float edge_fn(f2 v0, f2 v1, f2 p) {
+  return (p.x - v0.x) * (v1.y - v0.y) - (p.y - v0.y) * (v1.x - v0.x);
+}
+
+__device__ __forceinline__ bool inside_quad(const Seg& s, f2 p) {
+  const float c0 = edge_fn(s.v01, s.v00, p);
+  const float c1 = edge_fn(s.v11, s.v01, p);
+  const float c2 = edge_fn(s.v10, s.v11, p);
+  const float c3 = edge_fn(s.v00, s.v10, p);
+  return c0 >= 0.0f && c1 >= 0.0f && c2 >= 0.0f && c3 >= 0.0f;
+}
+
+// projectPointToLine — shaderSnippets.wgls.ts:95-102
+__device__ __forceinline__ f2 project_to_line(f2 l1, f2 l2, f2 p) {
+  const f2 ab = l2 - l1, ac = p - l1;
+  return l1 + (ab * dot(ab, ac)) / dot(ab, ab);
+}
+
+// interpolateHairQuad — swRasterizeHair.wgsl.ts:170-197
+__device__ __forceinline__ f2 interp_quad(const Seg& s, f2 c) {
+  const f2 sm = (s.v00 + s.v01) / 2.0f;
+  const f2 em = (s.v10 + s.v11) / 2.0f;
+  const f2 cp = project_to_line(sm, em, c);
+  const float d1 = length(cp - sm) / length(sm - em);
+  const float ws = length(s.v00 - s.v01);
+  const float we = ws * dot(normalize(s.v00 - s.v01), normalize(s.v10 - s.v11));
+  const float ew = mixf(ws, we, d1);
+  const f2 e1 = project_to_line(s.v00, s.v10, c);
+  const float d0 = length(c - e1) / ew;
+  return mk2(satf(d0), satf(d1));
+}
+
+// interpolateHairF32 — swRasterizeHair.wgsl.ts:199-203
+__device__ __forceinline__ float interp_f32(f2 w, f4 v) {
+  const float a = mixf(v.x, v.y, w.x);
+  const float b = mixf(v.z, v.w, w.x);
+  return mixf(a, b, w.y);
+}
+
+// getDepthBin — shaderSnippets.wgls.ts:148-156
+__device__ __forceinline__ uint32_t depth_bin(uint32_t n, f2 td, float d) {
+  const float span = fabsf(td.y - td.x);
+  const float t = (d - td.x) / span;
+  return clampu(f2u(t * (float)n), 0u, n - 1u);
+}
+
+// Pixel bounding box of a projected quad (getRasterizedHairBounds before the
+// scissor), as integers clamped to a safe range. A pixel can only pass all
+// four edge tests if it lies within [min - 1, max + 1] of this box.
+__device__ __forceinline__ void seg_pixel_bbox(const Seg& s, int& x0, int& y0, int& x1, int& y1) {
+  const f2 mx = vmax(vmax(s.v00, s.v01), vmax(s.v10, s.v11));
+  const f2 mn = vmin(vmin(s.v00, s.v01), vmin(s.v10, s.v11));
+  const float lo = -1048576.0f, hi = 1048576.0f;
+  x0 = (int)clampf(floorf(mn.x), lo, hi) - 1;
+  y0 = (int)clampf(floorf(mn.y), lo, hi) - 1;
+  x1 = (int)clampf(ceilf(mx.x), lo, hi) + 1;
+  y1 = (int)clampf(ceilf(mx.y), lo, hi) + 1;
+  // any NaN vertex: the quad has no inside pixel, cull it entirely
+  if (!(mx.x == mx.x && mx.y == mx.y && mn.x == mn.x && mn.y == mn.y)) { x0 = 1; x1 = 0; }
+}
+
+}  // namespace fh

@@ -1,0 +1,216 @@
+// Strand-space shading (K8) on gfx950: Kajiya-Kay diffuse with fake multiple
+// scattering, Marschner R/TT/TRT (Karis16), PCF/PCSS shadow from the shadow
+// map, AO and the fake Beer-Lambert attenuation, per (strand, shading point).
+// Reference: src/passes/hairShadingPass/hairShadingPass.wgsl.ts:61-282,
+// marschner.wgsl.ts:19-170, shadowMapPass/shared/sampleShadows.wgsl.ts:21-150,
+// aoPass/shared/textureAo.wgsl.ts:7-12, _shaderSnippets/*.
+#include "fh_device.hpp"
+#include "fh_internal.hpp"
+
+namespace {
+
+using namespace fh;
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kSqrtTwoPi = 2.5066282746310002f;
+// HAIR_F0 = (1 - eta)^2 / (1 + eta)^2 with eta = 1.55 (marschner.wgsl.ts:89-91)
+constexpr float kHairF0 = (float)((1.0 - 1.55) * (1.0 - 1.55) / ((1.0 + 1.55) * (1.0 + 1.55)));
+// linearizeDepth constants: CONFIG.camera.projection near / far (constants.ts:131-135)
+constexpr float kNear = 0.01f, kFar = 100.0f;
+
+__device__ __forceinline__ float fresnel1(float cosT, float f0) {
+  return f0 + (1.0f - f0) * powf(1.0f - cosT, 5.0f);
+}
+__device__ __forceinline__ float cos_half(float c) { return sqrtf(satf(0.5f + 0.5f * c)); }
+__device__ __forceinline__ float long_gauss(float si, float sr, float alpha, float beta) {
+  const float a = si + sr - alpha;
+  const float b = -(a * a) / (2.0f * beta * beta);
+  return expf(b) / (beta * kSqrtTwoPi);
+}
+__device__ __forceinline__ f3 powv(f3 b, float e) { return mk3(powf(b.x, e), powf(b.y, e), powf(b.z, e)); }
+
+struct Marschner {
+  f3 base;
+  float specular, weightTT, weightTRT, shift, roughness;
+};
+
+// hairSpecularMarschner — marschner.wgsl.ts:34-97
+__device__ f3 marschner(const Marschner& p, f3 L, f3 V, f3 T) {
+  const float si = dot(T, L), sr = dot(T, V);
+  const float ci = sqrtf(fmx(0.0f, 1.0f - si * si));
+  const float cr = sqrtf(fmx(0.0f, 1.0f - sr * sr));
+  const float cd = cos_half(ci * cr + si * sr);
+  const f3 lp = L - T * si, cp = V - T * sr;
+  const float cphi = dot(lp, cp) * (1.0f / sqrtf(length(lp) * length(cp) + 1e-4f));
+  const float chp = cos_half(cphi);
+  // R
+  const float beta2 = p.roughness * p.roughness;
+  const float Mr = long_gauss(si, sr, -2.0f * p.shift, beta2);
+  const float Nr = 0.25f * chp * fresnel1(cd, kHairF0);
+  // TT — azimuthalScattering_TT :412-434
+  const float Mtt = long_gauss(si, sr, p.shift, 0.5f * beta2);
+  const float eta = (1.19f / cd) + (0.36f * cd);
+  const float a = 1.0f / eta;
+  const float hTT = (1.0f + a * (0.6f - 0.8f * cphi)) * chp;
+  const f3 Ttt = powv(p.base, sqrtf(1.0f - hTT * hTT * a * a) / (2.0f * cd));
+  const float Dtt = expf(-3.65f * cphi - 3.98f);
+  const float f = fresnel1(cd * sqrtf(satf(1.0f - hTT * hTT)), kHairF0);
+  const f3 Ntt = (Ttt * ((1.0f - f) * (1.0f - f))) * Dtt;
+  // TRT — azimuthalScattering_TRT :437-454
+  const float Mtrt = long_gauss(si, sr, 4.0f * p.shift, 2.0f * beta2);
+  const f3 Ttrt = powv(p.base, 0.8f / cd);
+  const float Dtrt = expf(17.0f * cphi - 16.78f);
+  const float f2v = fresnel1(cd * 0.5f, kHairF0);
+  const f3 Ntrt = (Ttrt * ((1.0f - f2v) * (1.0f - f2v) * f2v)) * Dtrt;
+  const float r = p.specular * Mr * Nr;
+  return (mk3(r, r, r) + Ntt * (p.weightTT * Mtt)) + Ntrt * (p.weightTRT * Mtrt);
+}
+
+// hairShading — hairShadingPass.wgsl.ts:166-205
+__device__ f3 hair_light(const Marschner& p, const fh_light& light, f3 toCam, f3 T, f3 posWS,
+                         float shadow, float ao) {
+  const f3 L = normalize(mk3(light.position[0], light.position[1], light.position[2]) - posWS);
+  const f3 diffuse = p.base * mixf(0.25f, 1.0f, dot(T, L));  // KajiyaKayDiffuse :209-213
+  // fakeMultipleScattering :216-234
+  const f3 fn = normalize(toCam - T * dot(toCam, T));
+  const float ds = (satf(dot(fn, L)) + 1.0f) / (4.0f * kPi);
+  const float luma = 0.2126729f * p.base.x + 0.7151522f * p.base.y + 0.0721750f * p.base.z;
+  const f3 tint = powv(p.base / luma, 1.0f - shadow);
+  const f3 ms = (mk3(sqrtf(p.base.x), sqrtf(p.base.y), sqrtf(p.base.z)) * ds) * tint;
+  const f3 dt = (diffuse * ms) * satf(dot(T, L));
+  const f3 brdf = dt + marschner(p, L, toCam, T);
+  const f3 radiance = mk3(light.colorAndEnergy[0], light.colorAndEnergy[1], light.colorAndEnergy[2]) *
+                      light.colorAndEnergy[3];
+  return (brdf * radiance) * (1.0f - ao);
+}
+
+// textureGather on a clamp-to-edge sampler, averaged as (x + y + z + w) / 4
+// with components (i0,j1), (i1,j1), (i1,j0), (i0,j0).
+__device__ __forceinline__ float gather_avg(const float* __restrict__ t, uint32_t w, uint32_t h, f2 uv) {
+  const float x = uv.x * (float)w - 0.5f, y = uv.y * (float)h - 0.5f;
+  int i0 = (int)floorf(x), j0 = (int)floorf(y);
+  int i1 = i0 + 1, j1 = j0 + 1;
+  i0 = min(max(i0, 0), (int)w - 1); i1 = min(max(i1, 0), (int)w - 1);
+  j0 = min(max(j0, 0), (int)h - 1); j1 = min(max(j1, 0), (int)h - 1);
+  const float gx = t[(size_t)j1 * w + i0], gy = t[(size_t)j1 * w + i1];
+  const float gz = t[(size_t)j0 * w + i1], gw = t[(size_t)j0 * w + i0];
+  return (gx + gy + gz + gw) / 4.0f;
+}
+
+__device__ __forceinline__ float linearize_depth(float d) { return kNear * kFar / (kFar + (kNear - kFar) * d); }
+
+struct ShadeArgs {
+  const fh_render_uniforms* u;
+  const float4* pos;
+  const float4* tan;
+  float4* out;
+  const float* shadow;
+  uint32_t shadowW, shadowH;
+  const float* ao;
+  uint32_t aoW, aoH;
+  const float* depth;
+  uint32_t depthW, depthH;
+  uint32_t S, P, PS;
+};
+
+__global__ __launch_bounds__(256) void shading_kernel(ShadeArgs a) {
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= a.S * a.PS) return;
+  const uint32_t s = gid / a.PS, id = gid - s * a.PS;
+  const fh_render_uniforms* __restrict__ u = a.u;
+  const fh_hair_material& m = u->hairMaterial;
+  const f2 vp = mk2(u->viewport[0], u->viewport[1]);
+  const float t = (float)id / (float)a.PS;
+  // remapToIndices — shaderSnippets.wgls.ts:120-126
+  const float ra = t * ((float)a.P - 1.0f);
+  const uint32_t au = f2u(ra);
+  const uint32_t ix = clampu(au, 0u, a.P - 1u), iy = clampu(au + 1u, 0u, a.P - 1u);
+  const float tIn = fractf(ra);
+  const float4 q0 = a.pos[(size_t)s * a.P + ix], q1 = a.pos[(size_t)s * a.P + iy];
+  const float4 tq = a.tan[(size_t)s * a.P + iy];
+  const f3 s0 = mk3(q0.x, q0.y, q0.z), s1 = mk3(q1.x, q1.y, q1.z);
+  const f3 pObj = mix(s0, s1, tIn);
+  const f3 tObj = mix(normalize(s1 - s0), mk3(tq.x, tq.y, tq.z), tIn);
+  const f4 posWS = mul(u->modelMatrix, mk4(pObj.x, pObj.y, pObj.z, 1.0f));
+  const f4 tanWS = mul(u->modelMatrix, mk4(tObj.x, tObj.y, tObj.z, 1.0f));  // w = 1 as in :95
+  const f3 pproj = project(u->vpMatrix, posWS);
+  const f2 p01 = mk2(pproj.x * 0.5f + 0.5f, pproj.y * 0.5f + 0.5f);
+  const f2 texPx = mk2(vp.x * p01.x, vp.y * (1.0f - p01.y));
+  // base colour :105-114
+  f3 base = mix(mk3(m.color0[0], m.color0[1], m.color0[2]), mk3(m.color1[0], m.color1[1], m.color1[2]), t);
+  const float fs = (float)s;
+  const f3 rnd = sat(mk3(fractf(fs * 1.73f) * 1.0f, fractf(fs * 1.17f) * 1.0f, fractf(fs * 1.31f) * 1.0f));
+  base = mix(base, rnd, m.colorRng);
+  base = mix(base, base * fractf(fs / 255.0f), m.lumaRng);
+  const f3 toCam = normalize(mk3(u->cameraPosition[0], u->cameraPosition[1], u->cameraPosition[2]) - xyz(posWS));
+  const Marschner mp{base, m.specular, m.weightTT, m.weightTRT, m.shift, m.roughness};
+  // shadow — getShadow :236-254 -> calculateDirectionalShadow (sampleShadows.wgsl.ts:47-84)
+  const f4 lpos = mul(u->shadows.sourceMVP_Matrix, posWS);
+  const f3 pss = mk3(lpos.x * 0.5f + 0.5f, 1.0f - (lpos.y * 0.5f + 0.5f), lpos.z);
+  const f3 ssp = mk3(u->shadows.sourcePosition[0], u->shadows.sourcePosition[1], u->shadows.sourcePosition[2]);
+  const f3 toSrc = normalize(ssp - xyz(posWS));
+  const float bias = fmx(fabsf(u->shadows.bias) * (1.0f - dot(xyz(tanWS), toSrc)), 0.0005f);
+  int32_t radius = (int32_t)u->shadows.PCF_Radius;
+  if (u->shadows.usePCSS > 0u) {
+    const float sm = gather_avg(a.shadow, a.shadowW, a.shadowH, mk2(pss.x, pss.y));
+    const float dd = fmx(pss.z - sm, 0.0f);
+    radius = 1 + f2i(dd / sm * 10.0f);  // PCSS_PENUMBRA_BASE + depthDiff/depth * WIDTH
+  }
+  const float texel = 1.0f / (float)a.shadowW;
+  float sh = 0.0f;
+  for (int32_t x = -radius; x <= radius; x++)
+    for (int32_t y = -radius; y <= radius; y++) {
+      const f2 uv = mk2(pss.x, pss.y) + mk2((float)x * texel, (float)y * texel);
+      sh += (pss.z - bias > gather_avg(a.shadow, a.shadowW, a.shadowH, uv)) ? 1.0f : 0.0f;
+    }
+  const float pcf = (float)(radius * 2 + 1);
+  float shadowRes = sh / (pcf * pcf);
+  const bool validSample = !(pss.z > 1.0f) && !(pss.x < 0.0f || pss.x > 1.0f || pss.y < 0.0f || pss.y > 1.0f);
+  if (!validSample) shadowRes = 0.0f;
+  const float shadow = clampf(1.0f - shadowRes, 1.0f - m.shadows, 1.0f);
+  // AO — textureAo.wgsl.ts:7-12
+  const f2 at = mk2(texPx.x / vp.x, texPx.y / vp.y);
+  const float aov = tex_load(a.ao, a.aoW, a.aoH, f2i((float)a.aoW * at.x), f2i((float)a.aoH * at.y));
+  float ao = 1.0f - mixf(1.0f, aov, u->ao.strength);
+  // getAttenuation :259-282
+  const float zb = tex_load(a.depth, a.depthW, a.depthH, (int32_t)f2u(texPx.x), (int32_t)f2u(texPx.y));
+  const float att = fabsf(linearize_depth(pproj.z) - linearize_depth(zb)) * m.attenuation;
+  ao = satf(ao + att);
+  f3 rs = mk3(0.0f, 0.0f, 0.0f);
+  const f3 T = xyz(tanWS);
+  rs = rs + hair_light(mp, u->light0, toCam, T, xyz(posWS), shadow, ao);
+  rs = rs + hair_light(mp, u->light1, toCam, T, xyz(posWS), shadow, ao);
+  rs = rs + hair_light(mp, u->light2, toCam, T, xyz(posWS), shadow, ao);
+  const float alpha = id >= a.PS - 1u ? 0.0f : 1.0f;  // thin-tip alpha :160
+  a.out[(size_t)s * a.PS + id] = make_float4(rs.x, rs.y, rs.z, alpha);
+}
+
+}  // namespace
+
+extern "C" int fh_shading(fh_ctx* ctx, const fh_render_cfg* cfg, const fh_render_uniforms* uniforms,
+                          const fh_hair_data* hair_data, const fh_vec4* positions,
+                          const fh_vec4* tangents, fh_vec4* shading, fh_tex2d shadow_map, fh_tex2d ao,
+                          fh_tex2d depth, uint32_t strands_count, uint32_t points_per_strand) {
+  FH_CHECK_CTX(ctx);
+  (void)hair_data;
+  FH_REQUIRE(cfg && uniforms && positions && tangents && shading && shadow_map.data && ao.data && depth.data,
+             "fh_shading: null buffer");
+  FH_REQUIRE(cfg->shadingPoints >= 2 && cfg->shadingPoints <= 1024, "fh_shading: shadingPoints in [2, 1024]");
+  FH_REQUIRE(points_per_strand >= 2, "fh_shading: points_per_strand >= 2");
+  FH_REQUIRE(shadow_map.width > 0 && shadow_map.height > 0 && ao.width > 0 && ao.height > 0,
+             "fh_shading: empty texture");
+  const uint64_t n = (uint64_t)strands_count * cfg->shadingPoints;
+  if (n == 0) return FH_OK;
+  FH_REQUIRE(n < (1ull << 31), "fh_shading: too many shading points");
+  ShadeArgs a;
+  a.u = uniforms;
+  a.pos = reinterpret_cast<const float4*>(positions);
+  a.tan = reinterpret_cast<const float4*>(tangents);
+  a.out = reinterpret_cast<float4*>(shading);
+  a.shadow = shadow_map.data; a.shadowW = shadow_map.width; a.shadowH = shadow_map.height;
+  a.ao = ao.data; a.aoW = ao.width; a.aoH = ao.height;
+  a.depth = depth.data; a.depthW = depth.width; a.depthH = depth.height;
+  a.S = strands_count; a.P = points_per_strand; a.PS = cfg->shadingPoints;
+  shading_kernel<<<fh::div_ceil((uint32_t)n, 256), 256, 0, ctx->stream>>>(a);
+  return fh::check_launch("shading_kernel");
+}

@@ -1,0 +1,430 @@
+// Simulation step of the hair chain on gfx950: grid pre-pass (K1), strand
+// integration with constraints and collisions (K2) and the particle-to-grid
+// scatter (K3). Reference: src/passes/simulation/*.
+#include "fh_device.hpp"
+#include "fh_internal.hpp"
+
+namespace fh {
+int fill_u32(fh_ctx* ctx, void* dst, uint64_t n_words, uint32_t value);
+}
+
+namespace {
+
+using namespace fh;
+
+constexpr float kGridMul = 1000000.0f;  // GRID_FLOAT_TO_U32_MUL, grids.wgsl.ts:3
+constexpr float kEps = 1e-7f;           // FLOAT_EPSILON, shaderSnippets.wgls.ts:67
+// SDF_SAMPLE_GRAD_OFFSET = SDF_CELL_SIZE * 0.1 (collisions.wgsl.ts:23-24),
+// evaluated in double as a WGSL abstract-float constant, then rounded to f32.
+constexpr float kSdfGradOffset = (float)(0.0033918858971446753 * 0.1);
+
+__device__ __forceinline__ uint32_t grid_idx(uint32_t d, uint32_t x, uint32_t y, uint32_t z) {
+  const uint32_t m = d - 1u;
+  return min(z, m) * d * d + min(y, m) * d + min(x, m);
+}
+
+struct Cell {
+  uint32_t mn[3], mx[3];
+  f3 pin;
+};
+
+// _getGridCell — grids.wgsl.ts:61-74
+__device__ __forceinline__ Cell grid_cell(uint32_t d, f3 bmin, f3 bmax, f3 p) {
+  Cell c;
+  f3 t = sat((p - bmin) / (bmax - bmin));
+  const float dm1 = (float)(d - 1u);
+  c.pin = t * dm1;
+  c.mn[0] = f2u(floorf(t.x * dm1)); c.mn[1] = f2u(floorf(t.y * dm1)); c.mn[2] = f2u(floorf(t.z * dm1));
+  c.mx[0] = f2u(ceilf(t.x * dm1));  c.mx[1] = f2u(ceilf(t.y * dm1));  c.mx[2] = f2u(ceilf(t.z * dm1));
+  return c;
+}
+
+// _getGridCellWeights — grids.wgsl.ts:76-84
+__device__ __forceinline__ f3 cell_w(uint32_t x, uint32_t y, uint32_t z, f3 pin) {
+  return {clampf(1.0f - fabsf(pin.x - (float)x), 0.0f, 1.0f),
+          clampf(1.0f - fabsf(pin.y - (float)y), 0.0f, 1.0f),
+          clampf(1.0f - fabsf(pin.z - (float)z), 0.0f, 1.0f)};
+}
+
+__device__ __forceinline__ float grid_decode(int32_t v) { return (float)v / kGridMul; }
+
+struct SdfView {
+  const float* __restrict__ vol;
+  uint32_t dims;
+  f3 bmin, bmax;
+};
+
+// sampleSDFCollider — sdfCollider.ts:36-44
+__device__ __forceinline__ float sample_sdf(const SdfView& s, f3 p) {
+  f3 t = sat((p - s.bmin) / (s.bmax - s.bmin));
+  return sample_volume(s.vol, s.dims, t);
+}
+
+// ---------------------------------------------------------------------------
+// K1 — gridPreSimPass.wgsl.ts:38-159. One thread per grid point.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void grid_pre_sim_kernel(
+    uint32_t d, const fh_sim_uniforms* __restrict__ u,
+    const fh_density_velocity* __restrict__ dv, fh_density_grad_wind* __restrict__ out,
+    const float* __restrict__ sdf, uint32_t sdf_dims) {
+  const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= d * d * d) return;
+  const f3 gmin = mk3(u->gridBoundsMin[0], u->gridBoundsMin[1], u->gridBoundsMin[2]);
+  const f3 gmax = mk3(u->gridBoundsMax[0], u->gridBoundsMax[1], u->gridBoundsMax[2]);
+  SdfView sv{sdf, sdf_dims, mk3(u->sdfBoundsMin[0], u->sdfBoundsMin[1], u->sdfBoundsMin[2]),
+             mk3(u->sdfBoundsMax[0], u->sdfBoundsMax[1], u->sdfBoundsMax[2])};
+  const uint32_t z = id / (d * d);
+  const uint32_t xy = id - z * d * d;
+  const uint32_t y = xy / d, x = xy % d;
+  // getGridPointPositionWS — grids.wgsl.ts:23-36
+  const f3 cell = (gmax - gmin) / (float)(d - 1u);
+  const f3 pos = gmin + cell * mk3((float)x, (float)y, (float)z);
+  // getWindStrength :89-108
+  float wind;
+  const float sd = sample_sdf(sv, pos);
+  if (sd < 0.0f) {
+    wind = 0.0f;
+  } else {
+    const f3 toward = normalize(mk3(u->wind[0], u->wind[1], u->wind[2])) * -1.0f;
+    const f3 rp = pos + (toward * sd) * u->windColisionTraceOffset;
+    wind = sample_sdf(sv, rp) < 0.0f ? 0.5f : 1.0f;
+  }
+  // getDensityGradient :111-159
+  const uint32_t idx0 = grid_idx(d, x, y, z);
+  const float dens0 = grid_decode(dv[idx0].density);
+  f3 res = mk3(0.0f, 0.0f, 0.0f);
+  const int offs[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const int ox = (int)x + offs[k][0], oy = (int)y + offs[k][1], oz = (int)z + offs[k][2];
+    f3 g = mk3(0.0f, 0.0f, 0.0f);
+    if (!(ox < 0 || oy < 0 || oz < 0)) {
+      const uint32_t io = grid_idx(d, (uint32_t)ox, (uint32_t)oy, (uint32_t)oz);
+      if (io != idx0) {
+        const float m = fmx(dens0 - grid_decode(dv[io].density), 0.0f);
+        g = mk3(m * (float)offs[k][0], m * (float)offs[k][1], m * (float)offs[k][2]);
+      }
+    }
+    res = res + g;
+  }
+  *reinterpret_cast<float4*>(&out[idx0]) = make_float4(res.x, res.y, res.z, wind);
+}
+
+// ---------------------------------------------------------------------------
+// K2 — hairSimIntegrationPass.wgsl.ts:79-273. One strand per lane, one wave
+// per workgroup. The wave's 64 strands are staged into LDS with coalesced
+// loads in a lane-interleaved layout (element j of lane l at j*64 + l, so a
+// wave-wide b128 access touches 1 KiB of consecutive LDS); every sweep keeps
+// the Gauss-Seidel carry in registers.
+// ---------------------------------------------------------------------------
+template <int kSimLanes>
+__global__ __launch_bounds__(kSimLanes) void sim_integrate_kernel(
+    uint32_t d, const fh_sim_uniforms* __restrict__ u, float4* __restrict__ prev,
+    const float4* __restrict__ now, const float* __restrict__ seglen,
+    const float* __restrict__ sdf, uint32_t sdf_dims,
+    const fh_density_velocity* __restrict__ dv, const fh_density_grad_wind* __restrict__ dgw,
+    const float4* __restrict__ initial, uint32_t S, uint32_t P) {
+  extern __shared__ float4 lds[];
+  float4* A = lds;                          // positions being solved (xyz, movable)
+  float4* B = lds + (size_t)P * kSimLanes;  // (initial xyz, rest length)
+  float4* C = B + (size_t)P * kSimLanes;    // local-shape constants (h0, w0, w1, w2); PREV staging first
+  const uint32_t lane = threadIdx.x;
+  const uint32_t s0 = blockIdx.x * kSimLanes;
+  const uint32_t nStr = min((uint32_t)kSimLanes, S - s0);
+  const uint32_t nElem = nStr * P;
+  const size_t gbase = (size_t)s0 * P;
+
+  // coalesced staging: element e of this wave's block -> strand e / P, point e % P
+  for (uint32_t e = lane; e < nElem; e += kSimLanes) {
+    const uint32_t ls = e / P, pt = e - ls * P;
+    const uint32_t o = pt * kSimLanes + ls;
+    A[o] = now[gbase + e];
+    C[o] = prev[gbase + e];
+    const float4 ini = initial[gbase + e];
+    B[o] = make_float4(ini.x, ini.y, ini.z, seglen[gbase + e]);
+  }
+  __syncthreads();
+
+  const bool valid = lane < nStr;
+  const uint32_t s = s0 + lane;
+  if (valid) {
+    const f3 gmin = mk3(u->gridBoundsMin[0], u->gridBoundsMin[1], u->gridBoundsMin[2]);
+    const f3 gmax = mk3(u->gridBoundsMax[0], u->gridBoundsMax[1], u->gridBoundsMax[2]);
+    const f3 gravity = mk3(0.0f, -u->gravity, 0.0f);
+    const f3 windDir = mk3(u->wind[0], u->wind[1], u->wind[2]);
+    const float dt = u->deltaTime;
+    // wind jitter is per strand (:134-139)
+    const float timer = (float)u->frameIdx * 0.73f + u->windPhaseOffset * (float)s;
+    const float jitter = fractf(timer * u->windStrengthFrequency);
+    const float jitterDelta = mixf(-0.5f * u->windStrengthJitter, 0.5f * u->windStrengthJitter, jitter);
+    const float jitterStr = 1.0f + jitterDelta;
+
+    // Verlet integration with grid forces, points 1..P-1 (:125-162)
+    for (uint32_t i = 1; i < P; i++) {
+      const uint32_t o = i * kSimLanes + lane;
+      const float4 pn4 = A[o], pp4 = C[o];
+      const f3 pn = mk3(pn4.x, pn4.y, pn4.z), pp = mk3(pp4.x, pp4.y, pp4.z);
+      const Cell co = grid_cell(d, gmin, gmax, pn);
+      f3 grad = mk3(0.0f, 0.0f, 0.0f), vel = mk3(0.0f, 0.0f, 0.0f);
+      float wstr = 0.0f, dens = 0.0f;
+      // _getGridDensityGradAndWind (densityGradAndWindGrid.wgsl.ts:20-44)
+      for (uint32_t z = co.mn[2]; z <= co.mx[2]; z++)
+        for (uint32_t y = co.mn[1]; y <= co.mx[1]; y++)
+          for (uint32_t x = co.mn[0]; x <= co.mx[0]; x++) {
+            const f3 w = cell_w(x, y, z, co.pin);
+            const float4 g = *reinterpret_cast<const float4*>(&dgw[grid_idx(d, x, y, z)]);
+            grad = grad + w * mk3(g.x, g.y, g.z);
+            wstr = wstr + length(w) * g.w;
+          }
+      // _getGridDensityVelocity (densityVelocityGrid.wgsl.ts:30-68)
+      for (uint32_t z = co.mn[2]; z <= co.mx[2]; z++)
+        for (uint32_t y = co.mn[1]; y <= co.mx[1]; y++)
+          for (uint32_t x = co.mn[0]; x <= co.mx[0]; x++) {
+            const f3 w = cell_w(x, y, z, co.pin);
+            const int4 g = *reinterpret_cast<const int4*>(&dv[grid_idx(d, x, y, z)]);
+            vel = vel + w * mk3(grid_decode(g.x), grid_decode(g.y), grid_decode(g.z));
+            dens = dens + length(w) * grid_decode(g.w);
+          }
+      f3 force = gravity;
+      const float windCell = satf(mixf(u->windStrengthLull, 1.0f, wstr));
+      force = force + windDir * fabsf(u->wind[3] * jitterStr * windCell);
+      force = force + grad * u->volumePreservation;
+      const f3 gridDisp = dens > 0.0f ? vel / dens : mk3(0.0f, 0.0f, 0.0f);
+      // verletIntegration — integration.wgsl.ts:8-28
+      const f3 fdisp = mix(pn - pp, gridDisp, u->friction);
+      const f3 next = (pn + fdisp) + (force * dt) * dt;
+      const f3 r = mix(pp, next, pn4.w);
+      A[o] = make_float4(r.x, r.y, r.z, pn4.w);
+    }
+
+    // Local-shape constants depend only on the initial pose (constraints.wgsl.ts:65-82).
+    for (uint32_t j = 0; j + 3 < P; j++) {
+      const float4 i0 = initial[gbase + (size_t)lane * P + j];
+      const float4 i1 = initial[gbase + (size_t)lane * P + j + 1];
+      const float4 i2 = initial[gbase + (size_t)lane * P + j + 2];
+      const f3 ci = ((mk3(i0.x, i0.y, i0.z) + mk3(i1.x, i1.y, i1.z)) + mk3(i2.x, i2.y, i2.z)) / 3.0f;
+      const float h0 = length(ci - mk3(i1.x, i1.y, i1.z));
+      const float wT = i0.w + 2.0f * i1.w + i2.w;
+      C[j * kSimLanes + lane] = make_float4(h0, i0.w / wT * 2.0f, i1.w / wT * -4.0f, i2.w / wT * 2.0f);
+    }
+
+    const float iters = (float)u->constraintIterations;
+    const float kLen = u->stiffnessLengthConstr;
+    const float kGlobal = u->stiffnessGlobalConstr / iters;
+    const float kLocal = u->stiffnessLocalConstr / iters;
+    const float kColl = u->stiffnessCollisions / iters;
+    const float kSdf = u->stiffnessSDF / iters;
+    const f4 sphere = mk4(u->collisionSphere[0], u->collisionSphere[1], u->collisionSphere[2], u->collisionSphere[3]);
+    const float sdfOffset = u->sdfBoundsMax[3];
+    SdfView sv{sdf, sdf_dims, mk3(u->sdfBoundsMin[0], u->sdfBoundsMin[1], u->sdfBoundsMin[2]),
+               mk3(u->sdfBoundsMax[0], u->sdfBoundsMax[1], u->sdfBoundsMax[2])};
+    const float invPm1 = (float)(P - 1u);
+
+    for (uint32_t it = 0; it < u->constraintIterations; it++) {
+      // length constraint — constraints.wgsl.ts:5-28, hairSimIntegrationPass.wgsl.ts:176-190
+      float4 p0 = A[lane];
+      for (uint32_t j = 0; j + 1 < P; j++) {
+        float4 p1 = A[(j + 1) * kSimLanes + lane];
+        const float expected = B[j * kSimLanes + lane].w;
+        const f3 tg = mk3(p1.x - p0.x, p1.y - p0.y, p1.z - p0.z);
+        const float corr = 1.0f - expected / length(tg);
+        const float df = corr * kLen / (p0.w + p1.w + kEps);
+        const f3 delta = tg * df;
+        p0 = make_float4(p0.x + delta.x * p0.w, p0.y + delta.y * p0.w, p0.z + delta.z * p0.w, p0.w);
+        p1 = make_float4(p1.x - delta.x * p1.w, p1.y - delta.y * p1.w, p1.z - delta.z * p1.w, p1.w);
+        A[j * kSimLanes + lane] = p0;
+        p0 = p1;
+      }
+      A[(P - 1) * kSimLanes + lane] = p0;
+
+      // global shape — constraints.wgsl.ts:31-49
+      for (uint32_t j = 1; j < P; j++) {
+        const float x = (float)j / invPm1;
+        const float att = 1.0f - satf((x - u->globalConstrExtent) / u->globalConstrFade);
+        const float k = kGlobal * att;
+        const uint32_t o = j * kSimLanes + lane;
+        const float4 p = A[o];
+        const float4 ini = B[o];
+        const f3 dvv = mk3(ini.x - p.x, ini.y - p.y, ini.z - p.z);
+        A[o] = make_float4(p.x + (dvv.x * p.w) * k, p.y + (dvv.y * p.w) * k, p.z + (dvv.z * p.w) * k, p.w);
+      }
+
+      // local shape (Kelager10 triangle bending) — constraints.wgsl.ts:54-87
+      float4 a = A[lane], b = A[kSimLanes + lane];
+      uint32_t jj = 0;
+      for (; jj + 3 < P; jj++) {
+        const float4 c = A[(jj + 2) * kSimLanes + lane];
+        const float4 K = C[jj * kSimLanes + lane];
+        const f3 cc = ((mk3(a.x, a.y, a.z) + mk3(b.x, b.y, b.z)) + mk3(c.x, c.y, c.z)) / 3.0f;
+        const f3 hv = mk3(b.x, b.y, b.z) - cc;
+        const float h = length(hv);
+        const f3 delta = hv * (1.0f - K.x / h);
+        const float k0 = kLocal * K.y, k1 = kLocal * K.z, k2 = kLocal * K.w;
+        A[jj * kSimLanes + lane] = make_float4(a.x + delta.x * k0, a.y + delta.y * k0, a.z + delta.z * k0, a.w);
+        a = make_float4(b.x + delta.x * k1, b.y + delta.y * k1, b.z + delta.z * k1, b.w);
+        b = make_float4(c.x + delta.x * k2, c.y + delta.y * k2, c.z + delta.z * k2, c.w);
+      }
+      A[jj * kSimLanes + lane] = a;
+      A[(jj + 1) * kSimLanes + lane] = b;
+
+      // tip tangent match — constraints.wgsl.ts:90-120
+      float4 pa = A[(jj - 1) * kSimLanes + lane];
+      for (; jj < P; jj++) {
+        const uint32_t o = jj * kSimLanes + lane;
+        const float4 pb = A[o];
+        const float4 i0 = B[(jj - 1) * kSimLanes + lane], i1 = B[o];
+        const f3 tgi = mk3(i1.x - i0.x, i1.y - i0.y, i1.z - i0.z);
+        const f3 proj = mk3(pa.x, pa.y, pa.z) + tgi;
+        const f3 delta = proj - mk3(pb.x, pb.y, pb.z);
+        const float k = kLocal * pb.w;
+        pa = make_float4(pb.x + delta.x * k, pb.y + delta.y * k, pb.z + delta.z * k, pb.w);
+        A[o] = pa;
+      }
+
+      // sphere + SDF collisions — collisions.wgsl.ts:4-61, :237-261
+      for (uint32_t j = 1; j < P; j++) {
+        const uint32_t o = j * kSimLanes + lane;
+        const float4 p4 = A[o];
+        f3 p = mk3(p4.x, p4.y, p4.z);
+        const f3 v = p - xyz(sphere);
+        const float toShell = fmx(0.0f, -(length(v) - sphere.w));
+        p = p + (normalize(v) * toShell) * kColl;
+        if (j > 2u) {
+          const float sd = sample_sdf(sv, p);
+          const float sdo = sd + sdfOffset;
+          if (!(sdo >= 0.0f)) {
+            f3 cd;
+            cd.x = sample_sdf(sv, p + mk3(kSdfGradOffset, 0.0f, 0.0f));
+            cd.y = sample_sdf(sv, p + mk3(0.0f, kSdfGradOffset, 0.0f));
+            cd.z = sample_sdf(sv, p + mk3(0.0f, 0.0f, kSdfGradOffset));
+            cd = cd - mk3(sd, sd, sd);
+            p = p + (normalize(cd) * -sdo) * kSdf;
+          }
+        }
+        A[o] = make_float4(p.x, p.y, p.z, p4.w);
+      }
+    }
+  }
+  __syncthreads();
+  // coalesced write-back into the PREV buffer (:268-272)
+  for (uint32_t e = lane; e < nElem; e += kSimLanes) {
+    const uint32_t ls = e / P, pt = e - ls * P;
+    prev[gbase + e] = A[pt * kSimLanes + ls];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K3 — gridPostSimPass.wgsl.ts:45-72 with _addGridDensityVelocity
+// (densityVelocityGrid.wgsl.ts:93-124). One thread per point; integer adds
+// commute, so the grid is bit-identical whatever the atomic order. Adding 0
+// changes nothing, so zero contributions are not sent.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void grid_post_sim_kernel(
+    uint32_t d, const fh_sim_uniforms* __restrict__ u, const float4* __restrict__ prev,
+    const float4* __restrict__ now, fh_density_velocity* __restrict__ dv, uint32_t nPoints) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nPoints) return;
+  const f3 gmin = mk3(u->gridBoundsMin[0], u->gridBoundsMin[1], u->gridBoundsMin[2]);
+  const f3 gmax = mk3(u->gridBoundsMax[0], u->gridBoundsMax[1], u->gridBoundsMax[2]);
+  const float4 pp = prev[e], pn = now[e];
+  const f3 p = mk3(pn.x, pn.y, pn.z);
+  const f3 vel = p - mk3(pp.x, pp.y, pp.z);
+  const Cell co = grid_cell(d, gmin, gmax, p);
+  for (uint32_t z = co.mn[2]; z <= co.mx[2]; z++)
+    for (uint32_t y = co.mn[1]; y <= co.mx[1]; y++)
+      for (uint32_t x = co.mn[0]; x <= co.mx[0]; x++) {
+        const f3 w = cell_w(x, y, z, co.pin);
+        const f3 cv = vel * w;
+        int32_t* cell = reinterpret_cast<int32_t*>(&dv[grid_idx(d, x, y, z)]);
+        const int32_t vx = f2i(cv.x * kGridMul), vy = f2i(cv.y * kGridMul), vz = f2i(cv.z * kGridMul);
+        const int32_t dn = f2i(length(w) * kGridMul);
+        if (vx) atomicAdd(cell + 0, vx);
+        if (vy) atomicAdd(cell + 1, vy);
+        if (vz) atomicAdd(cell + 2, vz);
+        if (dn) atomicAdd(cell + 3, dn);
+      }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fh_grid_pre_sim(fh_ctx* ctx, uint32_t grid_dims, const fh_sim_uniforms* sim_uniforms,
+                    const fh_density_velocity* density_velocity,
+                    fh_density_grad_wind* density_grad_wind, const float* sdf_texture,
+                    uint32_t sdf_dims) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(grid_dims >= 2 && grid_dims <= 1024, "fh_grid_pre_sim: grid_dims must be in [2, 1024]");
+  FH_REQUIRE(sdf_dims >= 1, "fh_grid_pre_sim: sdf_dims must be >= 1");
+  FH_REQUIRE(sim_uniforms && density_velocity && density_grad_wind && sdf_texture,
+             "fh_grid_pre_sim: null buffer");
+  const uint32_t n = grid_dims * grid_dims * grid_dims;
+  grid_pre_sim_kernel<<<fh::div_ceil(n, 256), 256, 0, ctx->stream>>>(
+      grid_dims, sim_uniforms, density_velocity, density_grad_wind, sdf_texture, sdf_dims);
+  return fh::check_launch("grid_pre_sim_kernel");
+}
+
+int fh_sim_integrate(fh_ctx* ctx, uint32_t grid_dims, const fh_sim_uniforms* sim_uniforms,
+                     const fh_hair_data* hair_data, fh_vec4* positions_prev,
+                     const fh_vec4* positions_now, const float* segment_lengths,
+                     const float* sdf_texture, uint32_t sdf_dims,
+                     const fh_density_velocity* density_velocity,
+                     const fh_density_grad_wind* density_grad_wind,
+                     const fh_vec4* positions_initial, uint32_t strands_count,
+                     uint32_t points_per_strand) {
+  FH_CHECK_CTX(ctx);
+  (void)hair_data;
+  FH_REQUIRE(grid_dims >= 2 && grid_dims <= 1024, "fh_sim_integrate: grid_dims must be in [2, 1024]");
+  FH_REQUIRE(points_per_strand >= 4 && points_per_strand <= 64,
+             "fh_sim_integrate: points_per_strand must be in [4, 64] (tfxFileLoader.ts:164)");
+  FH_REQUIRE(sdf_dims >= 1, "fh_sim_integrate: sdf_dims must be >= 1");
+  FH_REQUIRE(sim_uniforms && positions_prev && positions_now && segment_lengths && sdf_texture &&
+                 density_velocity && density_grad_wind && positions_initial,
+             "fh_sim_integrate: null buffer");
+  if (strands_count == 0) return FH_OK;
+  // 3 float4 per point per strand in LDS; 64 strands per workgroup when that
+  // fits the 160 KiB LDS of a gfx950 CU, else 32.
+  const size_t lds64 = (size_t)3 * points_per_strand * 64 * sizeof(float4);
+  if (lds64 <= 160 * 1024) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&sim_integrate_kernel<64>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds64);
+    sim_integrate_kernel<64><<<fh::div_ceil(strands_count, 64), 64, lds64, ctx->stream>>>(
+        grid_dims, sim_uniforms, reinterpret_cast<float4*>(positions_prev),
+        reinterpret_cast<const float4*>(positions_now), segment_lengths, sdf_texture, sdf_dims,
+        density_velocity, density_grad_wind, reinterpret_cast<const float4*>(positions_initial),
+        strands_count, points_per_strand);
+  } else {
+    const size_t lds32 = lds64 / 2;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&sim_integrate_kernel<32>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds32);
+    sim_integrate_kernel<32><<<fh::div_ceil(strands_count, 32), 32, lds32, ctx->stream>>>(
+        grid_dims, sim_uniforms, reinterpret_cast<float4*>(positions_prev),
+        reinterpret_cast<const float4*>(positions_now), segment_lengths, sdf_texture, sdf_dims,
+        density_velocity, density_grad_wind, reinterpret_cast<const float4*>(positions_initial),
+        strands_count, points_per_strand);
+  }
+  return fh::check_launch("sim_integrate_kernel");
+}
+
+int fh_grid_post_sim(fh_ctx* ctx, uint32_t grid_dims, const fh_sim_uniforms* sim_uniforms,
+                     const fh_hair_data* hair_data, const fh_vec4* positions_prev,
+                     const fh_vec4* positions_now, fh_density_velocity* density_velocity,
+                     uint32_t strands_count, uint32_t points_per_strand) {
+  FH_CHECK_CTX(ctx);
+  (void)hair_data;
+  FH_REQUIRE(grid_dims >= 2 && grid_dims <= 1024, "fh_grid_post_sim: grid_dims must be in [2, 1024]");
+  FH_REQUIRE(sim_uniforms && positions_prev && positions_now && density_velocity,
+             "fh_grid_post_sim: null buffer");
+  const uint64_t cells = (uint64_t)grid_dims * grid_dims * grid_dims;
+  int rc = fh::fill_u32(ctx, density_velocity, cells * 4, 0u);  // gridPostSimPass.ts:36
+  if (rc) return rc;
+  const uint64_t n = (uint64_t)strands_count * points_per_strand;
+  FH_REQUIRE(n < (1ull << 31), "fh_grid_post_sim: too many points");
+  if (n == 0) return FH_OK;
+  grid_post_sim_kernel<<<fh::div_ceil((uint32_t)n, 256), 256, 0, ctx->stream>>>(
+      grid_dims, sim_uniforms, reinterpret_cast<const float4*>(positions_prev),
+      reinterpret_cast<const float4*>(positions_now), density_velocity, (uint32_t)n);
+  return fh::check_launch("grid_post_sim_kernel");
+}
+
+}  // extern "C"

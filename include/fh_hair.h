@@ -1,0 +1,409 @@
+/*
+ * fh_hair.h — C-ABI of the MI355X (gfx950) per-frame hair compute chain.
+ *
+ * One entry point per method of the reference's pass classes under src/passes.
+ * Every buffer argument is a plain device pointer whose byte layout is the
+ * layout of the matching WGSL binding in the reference, so a pass can replace
+ * its WGSL counterpart without touching the producers or consumers around it.
+ * Citations are file:line in Scthe/frostbitten-hair-webgpu.
+ *
+ * Conventions
+ *  - Every entry point returns FH_OK (0) or a non-zero FH_ERR_* code and never
+ *    throws; fh_last_error() gives the message of the last failure on the
+ *    calling thread. This replaces the reference's WebGPU error scopes
+ *    (src/utils/errors.ts:3-41, index.deno.ts:157-163).
+ *  - Work is enqueued on the context's HIP stream and the call returns at once,
+ *    like recording into a GPUCommandEncoder; fh_sync() waits for it.
+ *  - Values that the reference bakes into WGSL as compile-time constants
+ *    (TILE_SIZE, TILE_DEPTH_BINS_COUNT, SLICES_PER_PIXEL, GRID_DIMS,
+ *    SHADING_POINTS, ...) are run-time arguments here (fh_render_cfg,
+ *    grid_dims), so instances with different settings can share one library.
+ *  - Textures become dense f32 arrays in texture memory order (texel (x, y) at
+ *    y * width + x, row 0 is the first row written by writeTexture). Loads
+ *    outside the texture return 0 (robust image access).
+ */
+#ifndef FH_HAIR_H
+#define FH_HAIR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FH_ABI_VERSION 1
+
+enum {
+  FH_OK = 0,
+  FH_ERR_INVALID_ARG = 1,
+  FH_ERR_HIP = 2,
+  FH_ERR_UNSUPPORTED = 3,
+  FH_ERR_CAPACITY = 4,
+};
+
+typedef struct fh_ctx fh_ctx;
+
+typedef struct fh_vec4 {
+  float x, y, z, w;
+} fh_vec4;
+
+/* ------------------------------------------------------------------------ */
+/* Uniform blocks (host-packed bytes, uploaded to a device buffer)           */
+/* ------------------------------------------------------------------------ */
+
+/* struct SimulationUniforms — simulationUniformsBuffer.ts:19-51, size :56-63,
+ * packed by update() :86-152. Matrices are column-major mat4x4<f32>. */
+typedef struct fh_sim_uniforms {
+  float modelMatrix[16];          /*   0 */
+  float modelMatrixInv[16];       /*  64 */
+  float collisionSphere[4];       /* 128  object space xyz, radius */
+  float wind[4];                  /* 144  direction.xyz (object space), strength */
+  float sdfBoundsMin[4];          /* 160  sdfCollider.ts:174-177; .w debug slice */
+  float sdfBoundsMax[4];          /* 176  .w = SDF distance offset */
+  float gridBoundsMin[4];         /* 192  gridData.ts:264-268; .w debug slice */
+  float gridBoundsMax[3];         /* 208 */
+  uint32_t gridDebugDisplayValue; /* 220 */
+  float deltaTime;                /* 224 */
+  float gravity;
+  uint32_t constraintIterations;
+  uint32_t frameIdx;
+  float windStrengthLull;         /* 240 */
+  float windColisionTraceOffset;
+  float stiffnessLengthConstr;
+  float stiffnessCollisions;
+  float stiffnessSDF;             /* 256 */
+  float volumePreservation;
+  float friction;
+  float stiffnessGlobalConstr;
+  float globalConstrExtent;       /* 272 */
+  float globalConstrFade;
+  float stiffnessLocalConstr;
+  float windPhaseOffset;
+  float windStrengthFrequency;    /* 288 */
+  float windStrengthJitter;
+  float padding0;
+  float padding1;
+} fh_sim_uniforms;                /* 304 */
+
+/* struct Light — renderUniformsBuffer.ts:46-49 */
+typedef struct fh_light {
+  float position[4];
+  float colorAndEnergy[4];
+} fh_light;
+
+/* struct Shadows — renderUniformsBuffer.ts:52-61 */
+typedef struct fh_shadows {
+  float sourceModelViewMat[16];
+  float sourceProjMatrix[16];
+  float sourceMVP_Matrix[16];
+  float sourcePosition[4];        /* xyz, .w = shadow fiber radius */
+  uint32_t usePCSS;
+  uint32_t PCF_Radius;
+  float bias;
+  float strength;
+} fh_shadows;
+
+/* struct AmbientOcclusion — renderUniformsBuffer.ts:64-74 */
+typedef struct fh_ao {
+  float radius, directionOffset, falloffStart2, falloffEnd2;
+  uint32_t numDirections, numSteps;
+  float strength, padding0;
+} fh_ao;
+
+/* struct HairMaterialParams — renderUniformsBuffer.ts:76-93 */
+typedef struct fh_hair_material {
+  float color0[3];
+  float specular;
+  float color1[3];
+  float shadows;
+  float weightTT, weightTRT, shift, roughness;
+  float colorRng, lumaRng, attenuation, padding1;
+} fh_hair_material;
+
+/* struct Uniforms — renderUniformsBuffer.ts:105-137, size :163-185,
+ * packed by update() :205-288. */
+typedef struct fh_render_uniforms {
+  float vpMatrix[16];             /*    0 */
+  float vpMatrixInv[16];          /*   64 */
+  float viewMatrix[16];           /*  128 */
+  float projMatrix[16];           /*  192 */
+  float projMatrixInv[16];        /*  256 */
+  float modelMatrix[16];          /*  320 */
+  float modelViewMat[16];         /*  384 */
+  float mvpMatrix[16];            /*  448 */
+  float collisionSphereModelMatrix[16]; /* 512 */
+  float viewport[4];              /*  576 */
+  float cameraPosition[4];        /*  592 */
+  float colorMgmt[4];             /*  608 */
+  float lightAmbient[4];          /*  624 */
+  fh_light light0;                /*  640 */
+  fh_light light1;                /*  672 */
+  fh_light light2;                /*  704 */
+  fh_shadows shadows;             /*  736 */
+  fh_ao ao;                       /*  960 */
+  fh_hair_material hairMaterial;  /*  992 */
+  float sdfBoundsMin[4];          /* 1056 */
+  float sdfBoundsMax[4];          /* 1072 */
+  float gridBoundsMin[4];         /* 1088 */
+  float gridBoundsMax[3];         /* 1104 */
+  uint32_t gridDebugDisplayValue; /* 1116 */
+  float fiberRadius;              /* 1120 */
+  float dbgShadowMapPreviewSize;
+  uint32_t maxDrawnHairSegments;  /* 1128 */
+  uint32_t displayMode;
+  float backgroundColor0[3];      /* 1136 */
+  float backgroundNoiseScale;
+  float backgroundColor1[3];
+  float backgroundGradientStrength;
+  float collisionSpherePosition[3]; /* 1168 */
+  uint32_t gizmoActiveState;
+} fh_render_uniforms;             /* 1184 */
+
+/* ------------------------------------------------------------------------ */
+/* Storage-buffer element layouts                                            */
+/* ------------------------------------------------------------------------ */
+
+/* struct HairData — hairDataBuffer.ts:15-19 (buffer is 256 B) */
+typedef struct fh_hair_data {
+  float boundingSphere[4];
+  uint32_t strandsCount;
+  uint32_t pointsPerStrand;
+} fh_hair_data;
+
+/* struct DensityVelocityI32 — densityVelocityGrid.wgsl.ts:8-16; values are
+ * i32 fixed point x1e6 (grids.wgsl.ts:3, 10-12). Index z*d*d + y*d + x. */
+typedef struct fh_density_velocity {
+  int32_t velocityX, velocityY, velocityZ, density;
+} fh_density_velocity;
+
+/* struct DensityGradAndWind — densityGradAndWindGrid.wgsl.ts:8-14 */
+typedef struct fh_density_grad_wind {
+  float densityGrad[3];
+  float windStrength;
+} fh_density_grad_wind;
+
+/* struct HairTileResult — hairTilesResultBuffer.ts:77-82, T*B entries indexed
+ * by getHairTileDepthBinIdx (tileUtils.wgsl.ts:483-490). padding0 carries the
+ * number of list entries of this (tile, bin) when written by fh_tiles_draw. */
+typedef struct fh_tile_result {
+  uint32_t minDepth;      /* u32((1 - dmin) * 2^32), atomicMax */
+  uint32_t maxDepth;      /* u32(dmax * 2^32), atomicMax */
+  uint32_t tileSegmentPtr;/* head + 1, 0 = empty list */
+  uint32_t padding0;
+} fh_tile_result;
+
+/* struct LinkedListElement — hairTileSegmentsBuffer.ts:60-63. The segments
+ * buffer is struct DrawnHairSegments: u32 drawnSegmentsCount at byte 0 and
+ * the element array at byte 4 (:65-68). */
+typedef struct fh_list_element {
+  uint32_t strandAndSegmentIdxs; /* (segmentIdx << 24) | strandIdx */
+  uint32_t next;                 /* 0xffffffff = end of list */
+} fh_list_element;
+
+/* struct SortBucket — hairTileSortPass.countTiles.wgsl.ts:21-45 */
+typedef struct fh_sort_bucket {
+  uint32_t tileCount;
+  uint32_t writeOffset;
+} fh_sort_bucket;
+
+/* Tile list (tileListBuffer.ts:15-22): u32 drawnTiles at byte 0, u32 tile ids
+ * at byte 4; (4 + T) u32 allocated.
+ * Rasterizer result (hairRasterizerResultBuffer.ts:11-34): u32
+ * tileQueueAtomicIdx at byte 0, vec4f pixels at byte 16 indexed W*y + x with
+ * y up; allocated W*H*16 B so the array holds W*H - 1 pixels.
+ * Slice data (hairSlicesDataBuffer.ts:34-37): vec4u {rg f16x2, ba f16x2,
+ * next, 0}, processorCount * avgFragmentsPerSlice * slices * tile^2 entries. */
+
+/* Values the reference bakes into its WGSL from CONFIG.hairRender
+ * (constants.ts:193-265). */
+typedef struct fh_render_cfg {
+  uint32_t tileSize;                        /* 8, 16 or 32 */
+  uint32_t tileDepthBins;                   /* 1..32 */
+  uint32_t slicesPerPixel;                  /* 1..16 */
+  uint32_t avgFragmentsPerSlice;
+  uint32_t processorCount;
+  uint32_t invalidTilesPerSegmentThreshold;
+  uint32_t sortBuckets;                     /* 1..64 */
+  uint32_t sortBucketSize;
+  uint32_t shadingPoints;                   /* 2..64 */
+  uint32_t alphaQuadratic;                  /* 0 or 1 */
+  float alphaMultiplier;
+  uint32_t reserved;
+} fh_render_cfg;
+
+/* A 2D f32 texture (depth24plus read as f32, r16float AO widened to f32). */
+typedef struct fh_tex2d {
+  const float* data;
+  uint32_t width;
+  uint32_t height;
+} fh_tex2d;
+
+/* ------------------------------------------------------------------------ */
+/* Context, memory and sizes                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces createGpuDevice (src/utils/webgpu.ts:8-68): binds device `device`
+ * and creates the stream all work of this context is enqueued on. */
+int fh_create(int device, fh_ctx** out);
+int fh_destroy(fh_ctx* ctx);
+/* Use an existing hipStream_t (e.g. torch's current stream) instead. */
+int fh_set_stream(fh_ctx* ctx, void* hip_stream);
+void* fh_get_stream(fh_ctx* ctx);
+/* Replaces queue.onSubmittedWorkDone (index.deno.ts:108-120). */
+int fh_sync(fh_ctx* ctx);
+const char* fh_last_error(void);
+int fh_abi_version(void);
+
+/* Device memory for hosts without their own allocator (a Deno host). */
+int fh_alloc(fh_ctx* ctx, uint64_t bytes, void** out);
+int fh_free(fh_ctx* ctx, void* ptr);
+int fh_upload(fh_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int fh_download(fh_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* GPUCommandEncoder.clearBuffer (webgpu.ts:141-143); a fill kernel, so it is
+ * safe inside a captured graph. bytes must be a multiple of 4. */
+int fh_clear(fh_ctx* ctx, void* dst, uint64_t bytes);
+
+/* Buffer sizes the reference computes on the host. */
+/* getTileCount (shared/utils.ts:4-10) */
+uint32_t fh_tile_count(const fh_render_cfg* cfg, uint32_t vp_w, uint32_t vp_h);
+/* getLengthOfHairTileSegmentsBuffer (hairTileSegmentsBuffer.ts:81-87):
+ * 1 + ceil(tiles * avgSegmentsPerTile); also maxDrawnHairSegments. */
+uint32_t fh_segments_buffer_entries(const fh_render_cfg* cfg, uint32_t vp_w,
+                                    uint32_t vp_h, float avg_segments_per_tile);
+/* SLICE_DATA_PER_PROCESSOR_COUNT (hairSlicesDataBuffer.ts:16-20) */
+uint32_t fh_slice_data_per_processor(const fh_render_cfg* cfg);
+
+/* ------------------------------------------------------------------------ */
+/* Simulation step (src/passes/simulation)                                   */
+/* ------------------------------------------------------------------------ */
+
+/* GridPreSimPass.cmdUpdateGridsBeforeSim (gridPreSimPass.ts:34-62; kernel
+ * gridPreSimPass.wgsl.ts:38-159): wind strength and density gradient per grid
+ * point. Every grid point is written, which covers the clear at :37. */
+int fh_grid_pre_sim(fh_ctx* ctx, uint32_t grid_dims,
+                    const fh_sim_uniforms* sim_uniforms,
+                    const fh_density_velocity* density_velocity,
+                    fh_density_grad_wind* density_grad_wind,
+                    const float* sdf_texture, uint32_t sdf_dims);
+
+/* HairSimIntegrationPass.cmdSimulateHairPositions (hairSimIntegrationPass.ts
+ * :33-60; kernel hairSimIntegrationPass.wgsl.ts:79-273): Verlet with grid
+ * friction and forces, constraintIterations x [length, global shape, local
+ * shape, tip tangent, sphere + SDF collisions], result written to
+ * positions_prev (the caller then swaps the tick-tock pair, :59). */
+int fh_sim_integrate(fh_ctx* ctx, uint32_t grid_dims,
+                     const fh_sim_uniforms* sim_uniforms,
+                     const fh_hair_data* hair_data,
+                     fh_vec4* positions_prev, const fh_vec4* positions_now,
+                     const float* segment_lengths,
+                     const float* sdf_texture, uint32_t sdf_dims,
+                     const fh_density_velocity* density_velocity,
+                     const fh_density_grad_wind* density_grad_wind,
+                     const fh_vec4* positions_initial,
+                     uint32_t strands_count, uint32_t points_per_strand);
+
+/* GridPostSimPass.cmdUpdateGridsAfterSim (gridPostSimPass.ts:33-59; kernel
+ * gridPostSimPass.wgsl.ts:45-72): clears density_velocity (:36), then scatters
+ * velocity (now - prev) and density of every point into the grid with i32
+ * fixed-point atomics (densityVelocityGrid.wgsl.ts:93-124). */
+int fh_grid_post_sim(fh_ctx* ctx, uint32_t grid_dims,
+                     const fh_sim_uniforms* sim_uniforms,
+                     const fh_hair_data* hair_data,
+                     const fh_vec4* positions_prev, const fh_vec4* positions_now,
+                     fh_density_velocity* density_velocity,
+                     uint32_t strands_count, uint32_t points_per_strand);
+
+/* ------------------------------------------------------------------------ */
+/* Software rasterizer (src/passes/swHair)                                   */
+/* ------------------------------------------------------------------------ */
+
+/* HairTilesPass.cmdClearBeforeRender (hairTilesPass.ts:48-52). */
+int fh_tiles_clear(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w,
+                   uint32_t vp_h, fh_tile_result* tiles,
+                   void* tile_segments, uint32_t* segment_count_per_tile);
+
+/* HairTilesPass.cmdDrawHairToTiles (hairTilesPass.ts:78-114; kernel
+ * hairTilesPass.wgsl.ts:70-246). tile_segments is DrawnHairSegments holding
+ * segments_capacity elements. slot_tile_bin (may be NULL) receives, for each
+ * list slot, the (tile, bin) index the element was linked into; the fine pass
+ * uses it to gather each list without walking it. */
+int fh_tiles_draw(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w,
+                  uint32_t vp_h, const fh_render_uniforms* uniforms,
+                  const fh_hair_data* hair_data, const fh_vec4* positions,
+                  const fh_vec4* tangents, fh_tile_result* tiles,
+                  fh_tex2d depth, void* tile_segments,
+                  uint32_t segments_capacity,
+                  uint32_t* segment_count_per_tile, uint32_t* slot_tile_bin,
+                  uint32_t rendered_strands, uint32_t points_per_strand);
+
+/* For lists produced by another implementation of the tile pass: walks every
+ * (tile, bin) list and fills slot_tile_bin and the per-bin counts (padding0). */
+int fh_tiles_rebuild_slot_keys(fh_ctx* ctx, const fh_render_cfg* cfg,
+                               uint32_t vp_w, uint32_t vp_h,
+                               fh_tile_result* tiles, const void* tile_segments,
+                               uint32_t segments_capacity,
+                               uint32_t* slot_tile_bin);
+
+/* HairTileSortPass.cmdClearBeforeRender (hairTileSortPass.ts:63-68). */
+int fh_tiles_sort_clear(fh_ctx* ctx, const fh_render_cfg* cfg, void* tile_list,
+                        fh_sort_bucket* buckets);
+
+/* HairTileSortPass.cmdSortHairTiles (hairTileSortPass.ts:80-153; kernels
+ * hairTileSortPass.countTiles.wgsl.ts:63-97 and .sort.wgsl.ts:37-75). */
+int fh_tiles_sort(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w,
+                  uint32_t vp_h, const uint32_t* segment_count_per_tile,
+                  void* tile_list, fh_sort_bucket* buckets);
+
+/* HairFinePass.cmdClearBeforeRender (hairFinePass.ts:50-55): clears the
+ * rasterizer result, including the tile-queue counter. */
+int fh_fine_clear(fh_ctx* ctx, uint32_t vp_w, uint32_t vp_h,
+                  void* rasterizer_result);
+
+/* HairFinePass.cmdRasterizeSlicesHair (hairFinePass.ts:70-90; kernel
+ * hairFinePass.wgsl.ts:126-335): cfg->processorCount persistent workgroups
+ * pull tiles from the tile list and blend each depth bin's slices front to
+ * back into rasterizer_result. slice_data is the per-processor fragment pool
+ * (used here only for fragments that do not fit in LDS). */
+int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w,
+                   uint32_t vp_h, const fh_render_uniforms* uniforms,
+                   const fh_hair_data* hair_data, const fh_vec4* positions,
+                   const fh_tile_result* tiles, const void* tile_segments,
+                   uint32_t segments_capacity, void* slice_data,
+                   void* rasterizer_result, fh_tex2d depth,
+                   const fh_vec4* shading, const fh_vec4* tangents,
+                   const void* tile_list, const uint32_t* slot_tile_bin,
+                   uint32_t strands_count, uint32_t points_per_strand);
+
+/* ------------------------------------------------------------------------ */
+/* Strand-space shading (src/passes/hairShadingPass)                         */
+/* ------------------------------------------------------------------------ */
+
+/* HairShadingPass.cmdComputeShadingPoints (hairShadingPass.ts:41-68; kernel
+ * hairShadingPass.wgsl.ts:61-282): cfg->shadingPoints colours per strand.
+ * shadow_map is the shadow source depth (square), ao the GTAO result. */
+int fh_shading(fh_ctx* ctx, const fh_render_cfg* cfg,
+               const fh_render_uniforms* uniforms,
+               const fh_hair_data* hair_data, const fh_vec4* positions,
+               const fh_vec4* tangents, fh_vec4* shading,
+               fh_tex2d shadow_map, fh_tex2d ao, fh_tex2d depth,
+               uint32_t strands_count, uint32_t points_per_strand);
+
+/* ------------------------------------------------------------------------ */
+/* Test hook                                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* The raster-helper test kernel of src/passes/swHair/swRasterizer.test.ts
+ * :165-250: rasterises every segment and atomicMax-es debugBarycentric
+ * colours (ABGR u32) into out[(H - y) * W + x]. The reference pins it with
+ * __test__/swRasterizer.snapshot.bin. */
+int fh_debug_raster_barycentric(fh_ctx* ctx, const fh_render_uniforms* uniforms,
+                                const fh_vec4* positions, const fh_vec4* tangents,
+                                uint32_t strands_count, uint32_t points_per_strand,
+                                uint32_t* out);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* FH_HAIR_H */

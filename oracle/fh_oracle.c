@@ -1,0 +1,992 @@
+/*
+ * CPU ORACLE — TEST INFRASTRUCTURE ONLY.
+ *
+ * A sequential C restatement of the reference's per-frame hair compute chain
+ * (the WGSL compute kernels of Scthe/frostbitten-hair-webgpu). Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, and only as
+ * the checker. Each function cites the reference file:line it follows.
+ *
+ * Serialisation: where the reference runs threads concurrently and lets atomic
+ * order decide a result, this oracle runs them in ascending index order
+ * (strand-major, then segment; tiles in tile-list order). That is one legal
+ * schedule of the reference. In the fine pass it fixes the order in which
+ * fragments of one slice are blended: ascending (strand, segment).
+ *
+ * Policies the reference leaves to the WebGPU implementation (robust access):
+ * out-of-bounds texture loads return 0, out-of-bounds storage reads return 0
+ * and out-of-bounds storage writes are dropped.
+ *
+ * Parity status per function is listed in the header comment of each block:
+ * "pinned" names the check in tests/ that would fail on a plausible mistake.
+ */
+#include "fh_oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+#include "fh_oracle_math.h"
+
+#define FLOAT_EPSILON 1e-7f
+#define MAX_U32 0xffffffffu
+#define INVALID_PTR 0xffffffffu
+
+/* ======================================================================== */
+/* Shared helpers                                                            */
+/* ======================================================================== */
+
+/* sampleSDFCollider — sdfCollider.ts:36-44 with the linear clamp-to-edge
+ * sampler of sdfUtils.ts:33-42. Pinned: test_oracle_sim.py sdf texel-centre
+ * and linear-field cases. */
+static float sample_sdf(const float* sdf, uint32_t d, ov3 bmin, ov3 bmax, ov3 p) {
+  ov3 t = sat3(div3(sub3(p, bmin), sub3(bmax, bmin)));
+  float c[3] = {t.x * (float)d - 0.5f, t.y * (float)d - 0.5f, t.z * (float)d - 0.5f};
+  int i0[3], i1[3];
+  float f[3];
+  for (int k = 0; k < 3; k++) {
+    float fl = floorf(c[k]);
+    f[k] = c[k] - fl;
+    int a = (int)fl;
+    int b = a + 1;
+    i0[k] = a < 0 ? 0 : (a > (int)d - 1 ? (int)d - 1 : a);
+    i1[k] = b < 0 ? 0 : (b > (int)d - 1 ? (int)d - 1 : b);
+  }
+#define S(X, Y, Z) sdf[((size_t)(Z) * d + (size_t)(Y)) * d + (size_t)(X)]
+  float c00 = S(i0[0], i0[1], i0[2]), c10 = S(i1[0], i0[1], i0[2]);
+  float c01 = S(i0[0], i1[1], i0[2]), c11 = S(i1[0], i1[1], i0[2]);
+  float c02 = S(i0[0], i0[1], i1[2]), c12 = S(i1[0], i0[1], i1[2]);
+  float c03 = S(i0[0], i1[1], i1[2]), c13 = S(i1[0], i1[1], i1[2]);
+#undef S
+  float x0 = c00 + f[0] * (c10 - c00);
+  float x1 = c01 + f[0] * (c11 - c01);
+  float x2 = c02 + f[0] * (c12 - c02);
+  float x3 = c03 + f[0] * (c13 - c03);
+  float y0 = x0 + f[1] * (x1 - x0);
+  float y1 = x2 + f[1] * (x3 - x2);
+  return y0 + f[2] * (y1 - y0);
+}
+
+float or_sample_sdf(const float* sdf, uint32_t d, const float* bmin,
+                    const float* bmax, const float* p) {
+  return sample_sdf(sdf, d, o3(bmin[0], bmin[1], bmin[2]),
+                    o3(bmax[0], bmax[1], bmax[2]), o3(p[0], p[1], p[2]));
+}
+
+/* _getGridIdx — grids.wgsl.ts:15-21 */
+static uint32_t grid_idx(uint32_t d, uint32_t x, uint32_t y, uint32_t z) {
+  return oclampu(z, 0, d - 1) * d * d + oclampu(y, 0, d - 1) * d + oclampu(x, 0, d - 1);
+}
+
+/* _getGridCell — grids.wgsl.ts:61-74 */
+typedef struct { uint32_t mn[3], mx[3]; ov3 pin; } grid_cell;
+static grid_cell get_grid_cell(uint32_t d, ov3 bmin, ov3 bmax, ov3 p) {
+  grid_cell r;
+  ov3 t = sat3(div3(sub3(p, bmin), sub3(bmax, bmin)));
+  float dm1 = (float)(d - 1u);
+  r.pin = muls3(t, dm1);
+  r.mn[0] = ou32(floorf(t.x * dm1)); r.mn[1] = ou32(floorf(t.y * dm1)); r.mn[2] = ou32(floorf(t.z * dm1));
+  r.mx[0] = ou32(ceilf(t.x * dm1)); r.mx[1] = ou32(ceilf(t.y * dm1)); r.mx[2] = ou32(ceilf(t.z * dm1));
+  return r;
+}
+
+/* _getGridCellWeights — grids.wgsl.ts:76-84 */
+static ov3 cell_weights(uint32_t x, uint32_t y, uint32_t z, ov3 pin) {
+  return o3(oclamp(1.0f - fabsf(pin.x - (float)x), 0.0f, 1.0f),
+            oclamp(1.0f - fabsf(pin.y - (float)y), 0.0f, 1.0f),
+            oclamp(1.0f - fabsf(pin.z - (float)z), 0.0f, 1.0f));
+}
+
+static float grid_decode(int32_t v) { return (float)v / 1000000.0f; }
+
+/* ======================================================================== */
+/* K1 — gridPreSimPass.wgsl.ts:38-159                                        */
+/* Pinned: test_oracle_sim.py (uniform density -> zero gradient, single      */
+/* spike -> +-axis pattern, wind inside/half/full regions of a sphere SDF). */
+/* ======================================================================== */
+
+void or_grid_pre_sim(uint32_t d, const fh_sim_uniforms* u,
+                     const fh_density_velocity* dv, fh_density_grad_wind* out,
+                     const float* sdf, uint32_t sdf_dims) {
+  ov3 gmin = o3(u->gridBoundsMin[0], u->gridBoundsMin[1], u->gridBoundsMin[2]);
+  ov3 gmax = o3(u->gridBoundsMax[0], u->gridBoundsMax[1], u->gridBoundsMax[2]);
+  ov3 smin = o3(u->sdfBoundsMin[0], u->sdfBoundsMin[1], u->sdfBoundsMin[2]);
+  ov3 smax = o3(u->sdfBoundsMax[0], u->sdfBoundsMax[1], u->sdfBoundsMax[2]);
+  ov3 windDir = o3(u->wind[0], u->wind[1], u->wind[2]);
+  uint32_t total = d * d * d;
+  for (uint32_t id = 0; id < total; id++) {
+    /* deconstructId :77-83 */
+    uint32_t z = id / (d * d);
+    uint32_t idXY = id - z * d * d;
+    uint32_t y = idXY / d, x = idXY % d;
+    /* getGridPointPositionWS — grids.wgsl.ts:23-36 */
+    ov3 cell = divs3(sub3(gmax, gmin), (float)(d - 1u));
+    ov3 pos = add3(gmin, mul3(cell, o3((float)x, (float)y, (float)z)));
+    /* getWindStrength :89-108 */
+    float wind;
+    float sd = sample_sdf(sdf, sdf_dims, smin, smax, pos);
+    if (sd < 0.0f) {
+      wind = 0.0f;
+    } else {
+      ov3 toward = muls3(norm3(windDir), -1.0f);
+      ov3 rp = add3(pos, muls3(muls3(toward, sd), u->windColisionTraceOffset));
+      float sd2 = sample_sdf(sdf, sdf_dims, smin, smax, rp);
+      wind = sd2 < 0.0f ? 0.5f : 1.0f;
+    }
+    /* getDensityGradient :111-159 */
+    uint32_t idx0 = grid_idx(d, x, y, z);
+    float dens0 = grid_decode(dv[idx0].density);
+    ov3 res = o3(0.0f, 0.0f, 0.0f);
+    static const int offs[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    for (int k = 0; k < 6; k++) {
+      int ox = (int)x + offs[k][0], oy = (int)y + offs[k][1], oz = (int)z + offs[k][2];
+      ov3 g = o3(0.0f, 0.0f, 0.0f);
+      if (!(ox < 0 || oy < 0 || oz < 0)) {
+        uint32_t idxO = grid_idx(d, (uint32_t)ox, (uint32_t)oy, (uint32_t)oz);
+        if (idxO != idx0) {
+          float densO = grid_decode(dv[idxO].density);
+          float diff = dens0 - densO;
+          float m = omax(diff, 0.0f);
+          g = o3(m * (float)offs[k][0], m * (float)offs[k][1], m * (float)offs[k][2]);
+        }
+      }
+      res = add3(res, g);
+    }
+    out[idx0].densityGrad[0] = res.x;
+    out[idx0].densityGrad[1] = res.y;
+    out[idx0].densityGrad[2] = res.z;
+    out[idx0].windStrength = wind;
+  }
+}
+
+/* ======================================================================== */
+/* K2 — hairSimIntegrationPass.wgsl.ts:79-273 with shaderImpl/               */
+/* integration.wgsl.ts:8-28, constraints.wgsl.ts:5-120,                      */
+/* collisions.wgsl.ts:4-61.                                                  */
+/* Pinned: test_oracle_sim.py (free flight 2*now-prev, length constraint to  */
+/* rest length, global shape stiffness 1 -> initial, sphere push to shell,   */
+/* brute-force double-precision restatement of a 4-point strand).            */
+/* ======================================================================== */
+
+static ov4 ld4(const fh_vec4* a, size_t i) { return o4(a[i].x, a[i].y, a[i].z, a[i].w); }
+
+void or_sim_integrate(uint32_t d, const fh_sim_uniforms* u,
+                      fh_vec4* prev, const fh_vec4* now, const float* seglen,
+                      const float* sdf, uint32_t sdf_dims,
+                      const fh_density_velocity* dv,
+                      const fh_density_grad_wind* dgw, const fh_vec4* initial,
+                      uint32_t S, uint32_t P) {
+  ov3 gmin = o3(u->gridBoundsMin[0], u->gridBoundsMin[1], u->gridBoundsMin[2]);
+  ov3 gmax = o3(u->gridBoundsMax[0], u->gridBoundsMax[1], u->gridBoundsMax[2]);
+  ov3 smin = o3(u->sdfBoundsMin[0], u->sdfBoundsMin[1], u->sdfBoundsMin[2]);
+  ov3 smax = o3(u->sdfBoundsMax[0], u->sdfBoundsMax[1], u->sdfBoundsMax[2]);
+  float sdfOffset = u->sdfBoundsMax[3];
+  ov3 gravityForce = o3(0.0f, -u->gravity, 0.0f);
+  ov4 sphere = o4(u->collisionSphere[0], u->collisionSphere[1], u->collisionSphere[2], u->collisionSphere[3]);
+  float iters = (float)u->constraintIterations;
+  const float SDF_GRAD_OFFSET = (float)(0.0033918858971446753 * 0.1);
+  ov4* pos = (ov4*)malloc(sizeof(ov4) * P);
+
+  for (uint32_t s = 0; s < S; s++) {
+    size_t base = (size_t)s * P;
+    pos[0] = ld4(now, base);
+    /* verlet + forces :125-162 */
+    for (uint32_t i = 1; i < P; i++) {
+      ov4 pp = ld4(prev, base + i), pn = ld4(now, base + i);
+      ov3 p = xyz(pn);
+      ov3 force = gravityForce;
+      grid_cell co = get_grid_cell(d, gmin, gmax, p);
+      /* _getGridDensityGradAndWind — densityGradAndWindGrid.wgsl.ts:20-44 */
+      ov3 dgrad = o3(0, 0, 0);
+      float wstr = 0.0f;
+      for (uint32_t z = co.mn[2]; z <= co.mx[2]; z++)
+        for (uint32_t y = co.mn[1]; y <= co.mx[1]; y++)
+          for (uint32_t x = co.mn[0]; x <= co.mx[0]; x++) {
+            ov3 w = cell_weights(x, y, z, co.pin);
+            const fh_density_grad_wind* v = &dgw[grid_idx(d, x, y, z)];
+            dgrad = add3(dgrad, mul3(w, o3(v->densityGrad[0], v->densityGrad[1], v->densityGrad[2])));
+            wstr = wstr + len3(w) * v->windStrength;
+          }
+      /* _getGridDensityVelocity — densityVelocityGrid.wgsl.ts:30-68 */
+      ov3 vel = o3(0, 0, 0);
+      float dens = 0.0f;
+      for (uint32_t z = co.mn[2]; z <= co.mx[2]; z++)
+        for (uint32_t y = co.mn[1]; y <= co.mx[1]; y++)
+          for (uint32_t x = co.mn[0]; x <= co.mx[0]; x++) {
+            ov3 w = cell_weights(x, y, z, co.pin);
+            const fh_density_velocity* v = &dv[grid_idx(d, x, y, z)];
+            ov3 vv = o3(grid_decode(v->velocityX), grid_decode(v->velocityY), grid_decode(v->velocityZ));
+            vel = add3(vel, mul3(w, vv));
+            dens = dens + len3(w) * grid_decode(v->density);
+          }
+      /* wind :133-139 */
+      float timer = (float)u->frameIdx * 0.73f + u->windPhaseOffset * (float)s;
+      float windJitter = ofract(timer * u->windStrengthFrequency);
+      float jitterDelta = omix(-0.5f * u->windStrengthJitter, 0.5f * u->windStrengthJitter, windJitter);
+      float jitterStr = 1.0f + jitterDelta;
+      float windCellStr = osat(omix(u->windStrengthLull, 1.0f, wstr));
+      force = add3(force, muls3(o3(u->wind[0], u->wind[1], u->wind[2]),
+                                fabsf(u->wind[3] * jitterStr * windCellStr)));
+      /* density gradient :145 */
+      force = add3(force, muls3(dgrad, u->volumePreservation));
+      /* grid velocity :149-153 */
+      ov3 gridDisp = dens > 0.0f ? divs3(vel, dens) : o3(0, 0, 0);
+      /* verletIntegration — integration.wgsl.ts:8-28 */
+      ov3 pointDisp = sub3(xyz(pn), xyz(pp));
+      ov3 fdisp = mix3(pointDisp, gridDisp, u->friction);
+      ov3 next = add3(add3(xyz(pn), fdisp), muls3(muls3(force, u->deltaTime), u->deltaTime));
+      ov3 r = mix3(xyz(pp), next, pn.w);
+      pos[i] = o4(r.x, r.y, r.z, pn.w);
+    }
+
+    /* constraints :168-262 */
+    for (uint32_t it = 0; it < u->constraintIterations; it++) {
+      /* length — constraints.wgsl.ts:5-28 */
+      float kLen = u->stiffnessLengthConstr;
+      ov4 p0 = pos[0];
+      for (uint32_t j = 0; j + 1 < P; j++) {
+        ov4 p1 = pos[j + 1];
+        float expected = seglen[base + j];
+        float w0 = p0.w, w1 = p1.w;
+        ov3 tg = sub3(xyz(p1), xyz(p0));
+        float actual = len3(tg);
+        float corr = 1.0f - expected / actual;
+        float df = corr * kLen / (w0 + w1 + FLOAT_EPSILON);
+        ov3 delta = muls3(tg, df);
+        ov3 a = add3(xyz(p0), muls3(delta, w0));
+        ov3 b = sub3(xyz(p1), muls3(delta, w1));
+        p0 = o4(a.x, a.y, a.z, p0.w);
+        p1 = o4(b.x, b.y, b.z, p1.w);
+        pos[j] = p0;
+        pos[j + 1] = p1;
+        p0 = p1;
+      }
+      /* global shape — constraints.wgsl.ts:31-49 */
+      float kGlobal = u->stiffnessGlobalConstr / iters;
+      for (uint32_t j = 1; j < P; j++) {
+        float x = (float)j / (float)(P - 1u);
+        float att = 1.0f - osat((x - u->globalConstrExtent) / u->globalConstrFade);
+        ov4 ini = ld4(initial, base + j);
+        float k = kGlobal * att;
+        ov4 p = pos[j];
+        ov3 dvv = sub3(xyz(ini), xyz(p));
+        ov3 r = add3(xyz(p), muls3(muls3(dvv, p.w), k));
+        pos[j] = o4(r.x, r.y, r.z, p.w);
+      }
+      /* local shape — constraints.wgsl.ts:54-87 */
+      float kLocal = u->stiffnessLocalConstr / iters;
+      uint32_t jj = 0;
+      for (; jj < P - 3u; jj++) {
+        ov4 i0 = ld4(initial, base + jj), i1 = ld4(initial, base + jj + 1), i2 = ld4(initial, base + jj + 2);
+        ov3 ci = divs3(add3(add3(xyz(i0), xyz(i1)), xyz(i2)), 3.0f);
+        float h0 = len3(sub3(ci, xyz(i1)));
+        ov4 a = pos[jj], b = pos[jj + 1], c = pos[jj + 2];
+        ov3 cc = divs3(add3(add3(xyz(a), xyz(b)), xyz(c)), 3.0f);
+        ov3 hv = sub3(xyz(b), cc);
+        float h = len3(hv);
+        float wT = i0.w + 2.0f * i1.w + i2.w;
+        float w0 = i0.w / wT * 2.0f;
+        float w1 = i1.w / wT * -4.0f;
+        float w2 = i2.w / wT * 2.0f;
+        ov3 delta = muls3(hv, 1.0f - h0 / h);
+        ov3 ra = add3(xyz(a), muls3(delta, kLocal * w0));
+        ov3 rb = add3(xyz(b), muls3(delta, kLocal * w1));
+        ov3 rc = add3(xyz(c), muls3(delta, kLocal * w2));
+        pos[jj] = o4(ra.x, ra.y, ra.z, a.w);
+        pos[jj + 1] = o4(rb.x, rb.y, rb.z, b.w);
+        pos[jj + 2] = o4(rc.x, rc.y, rc.z, c.w);
+      }
+      /* tip tangent — constraints.wgsl.ts:90-120 */
+      for (; jj < P; jj++) {
+        ov4 i0 = ld4(initial, base + jj - 1), i1 = ld4(initial, base + jj);
+        ov3 tgi = sub3(xyz(i1), xyz(i0));
+        ov4 a = pos[jj - 1], b = pos[jj];
+        ov3 proj = add3(xyz(a), tgi);
+        ov3 delta = sub3(proj, xyz(b));
+        ov3 r = add3(xyz(b), muls3(delta, kLocal * b.w));
+        pos[jj] = o4(r.x, r.y, r.z, b.w);
+      }
+      /* collisions :237-261, collisions.wgsl.ts:4-61 */
+      float kColl = u->stiffnessCollisions / iters;
+      float kSdf = u->stiffnessSDF / iters;
+      for (uint32_t j = 1; j < P; j++) {
+        ov4 p = pos[j];
+        ov3 v = sub3(xyz(p), xyz(sphere));
+        float dist = len3(v) - sphere.w;
+        float toShell = omax(0.0f, -dist);
+        ov3 r = add3(xyz(p), muls3(muls3(norm3(v), toShell), kColl));
+        p = o4(r.x, r.y, r.z, p.w);
+        if (j > 2u) {
+          ov3 q = xyz(p);
+          float sd = sample_sdf(sdf, sdf_dims, smin, smax, q);
+          float sdo = sd + sdfOffset;
+          if (!(sdo >= 0.0f)) {
+            ov3 cd;
+            cd.x = sample_sdf(sdf, sdf_dims, smin, smax, add3(q, o3(SDF_GRAD_OFFSET, 0.0f, 0.0f)));
+            cd.y = sample_sdf(sdf, sdf_dims, smin, smax, add3(q, o3(0.0f, SDF_GRAD_OFFSET, 0.0f)));
+            cd.z = sample_sdf(sdf, sdf_dims, smin, smax, add3(q, o3(0.0f, 0.0f, SDF_GRAD_OFFSET)));
+            cd = sub3(cd, o3(sd, sd, sd));
+            ov3 corr = muls3(norm3(cd), -sdo);
+            ov3 rr = add3(q, muls3(corr, kSdf));
+            p = o4(rr.x, rr.y, rr.z, p.w);
+          }
+        }
+        pos[j] = p;
+      }
+    }
+    /* write back :268-272 */
+    for (uint32_t i = 0; i < P; i++) {
+      prev[base + i].x = pos[i].x;
+      prev[base + i].y = pos[i].y;
+      prev[base + i].z = pos[i].z;
+      prev[base + i].w = pos[i].w;
+    }
+  }
+  free(pos);
+}
+
+/* ======================================================================== */
+/* K3 — gridPostSimPass.wgsl.ts:45-72, _addGridDensityVelocity               */
+/* densityVelocityGrid.wgsl.ts:93-124; host clear gridPostSimPass.ts:36.     */
+/* Pinned: test_oracle_sim.py (point on a node -> one corner, weight         */
+/* length(1,1,1); cell centre -> 8 corners of length(.5,.5,.5); sum of       */
+/* density == sum over corners of i32(length(w)*1e6)).                       */
+/* ======================================================================== */
+
+void or_grid_post_sim(uint32_t d, const fh_sim_uniforms* u, const fh_vec4* prev,
+                      const fh_vec4* now, fh_density_velocity* dv, uint32_t S,
+                      uint32_t P) {
+  ov3 gmin = o3(u->gridBoundsMin[0], u->gridBoundsMin[1], u->gridBoundsMin[2]);
+  ov3 gmax = o3(u->gridBoundsMax[0], u->gridBoundsMax[1], u->gridBoundsMax[2]);
+  memset(dv, 0, sizeof(fh_density_velocity) * (size_t)d * d * d);
+  for (uint32_t s = 0; s < S; s++)
+    for (uint32_t i = 0; i < P; i++) {
+      size_t k = (size_t)s * P + i;
+      ov3 pp = o3(prev[k].x, prev[k].y, prev[k].z);
+      ov3 pn = o3(now[k].x, now[k].y, now[k].z);
+      ov3 vel = sub3(pn, pp);
+      grid_cell co = get_grid_cell(d, gmin, gmax, pn);
+      for (uint32_t z = co.mn[2]; z <= co.mx[2]; z++)
+        for (uint32_t y = co.mn[1]; y <= co.mx[1]; y++)
+          for (uint32_t x = co.mn[0]; x <= co.mx[0]; x++) {
+            uint32_t idx = grid_idx(d, x, y, z);
+            ov3 w = cell_weights(x, y, z, co.pin);
+            ov3 cv = mul3(vel, w);
+            fh_density_velocity* c = &dv[idx];
+            c->velocityX = (int32_t)((uint32_t)c->velocityX + (uint32_t)oi32(cv.x * 1000000.0f));
+            c->velocityY = (int32_t)((uint32_t)c->velocityY + (uint32_t)oi32(cv.y * 1000000.0f));
+            c->velocityZ = (int32_t)((uint32_t)c->velocityZ + (uint32_t)oi32(cv.z * 1000000.0f));
+            c->density = (int32_t)((uint32_t)c->density + (uint32_t)oi32(len3(w) * 1000000.0f));
+          }
+    }
+}
+
+/* ======================================================================== */
+/* Raster helpers — swRasterizeHair.wgsl.ts:39-203, shaderSnippets.wgls.ts   */
+/* :89-156, tileUtils.wgsl.ts:5-54.                                          */
+/* Pinned: test_oracle_raster.py against the reference's own fixture        */
+/* src/passes/swHair/__test__/swRasterizer.snapshot.bin (bit for bit).       */
+/* ======================================================================== */
+
+typedef struct {
+  ov2 v00, v01, v10, v11;
+  ov4 depths;
+} proj_seg;
+
+/* projectHairPoint — swRasterizeHair.wgsl.ts:76-105 */
+static void project_point(const fh_render_uniforms* u, const fh_vec4* positions,
+                          const fh_vec4* tangents, uint32_t P, uint32_t strand,
+                          uint32_t point, ov2 vp, float fiberRadius, ov2* v0,
+                          ov2* v1, ov2* dd) {
+  size_t k = (size_t)strand * P + point;
+  ov4 p0 = mulmv(u->modelMatrix, o4(positions[k].x, positions[k].y, positions[k].z, 1.0f));
+  ov4 t0 = mulmv(u->modelMatrix, o4(tangents[k].x, tangents[k].y, tangents[k].z, 1.0f));
+  ov3 cam = o3(u->cameraPosition[0], u->cameraPosition[1], u->cameraPosition[2]);
+  ov3 toCam = norm3(sub3(cam, xyz(p0)));
+  ov3 right = muls3(norm3(cross3(xyz(t0), toCam)), fiberRadius);
+  ov3 a = sub3(xyz(p0), right), b = add3(xyz(p0), right);
+  ov3 an = oproject(u->vpMatrix, o4(a.x, a.y, a.z, 1.0f));
+  ov3 bn = oproject(u->vpMatrix, o4(b.x, b.y, b.z, 1.0f));
+  /* ndc2viewportPx — shaderSnippets.wgls.ts:104-107 */
+  *v0 = o2((an.x * 0.5f + 0.5f) * vp.x, (an.y * 0.5f + 0.5f) * vp.y);
+  *v1 = o2((bn.x * 0.5f + 0.5f) * vp.x, (bn.y * 0.5f + 0.5f) * vp.y);
+  *dd = o2(an.z, bn.z);
+}
+
+/* projectHairSegment — swRasterizeHair.wgsl.ts:39-68 */
+static proj_seg project_segment(const fh_render_uniforms* u, const fh_vec4* positions,
+                                const fh_vec4* tangents, uint32_t P, uint32_t strand,
+                                uint32_t seg, ov2 vp, float fiberRadius) {
+  proj_seg r;
+  ov2 a, b, dd;
+  project_point(u, positions, tangents, P, strand, seg, vp, fiberRadius, &a, &b, &dd);
+  r.v00 = a; r.v01 = b; r.depths.x = dd.x; r.depths.y = dd.y;
+  project_point(u, positions, tangents, P, strand, seg + 1, vp, fiberRadius, &a, &b, &dd);
+  r.v10 = a; r.v11 = b; r.depths.z = dd.x; r.depths.w = dd.y;
+  return r;
+}
+
+/* edgeFunction — swRasterizeHair.wgsl.ts:127-129 */
+static float edge_fn(ov2 v0, ov2 v1, ov2 p) {
+  return (p.x - v0.x) * (v1.y - v0.y) - (p.y - v0.y) * (v1.x - v0.x);
+}
+
+static int inside_quad(const proj_seg* s, ov2 p) {
+  float c0 = edge_fn(s->v01, s->v00, p);
+  float c1 = edge_fn(s->v11, s->v01, p);
+  float c2 = edge_fn(s->v10, s->v11, p);
+  float c3 = edge_fn(s->v00, s->v10, p);
+  return c0 >= 0.0f && c1 >= 0.0f && c2 >= 0.0f && c3 >= 0.0f;
+}
+
+/* projectPointToLine — shaderSnippets.wgls.ts:95-102 */
+static ov2 project_to_line(ov2 l1, ov2 l2, ov2 p) {
+  ov2 ab = sub2(l2, l1), ac = sub2(p, l1);
+  ov2 ad = divs2(muls2(ab, dot2(ab, ac)), dot2(ab, ab));
+  return add2(l1, ad);
+}
+
+/* interpolateHairQuad — swRasterizeHair.wgsl.ts:170-197 */
+static ov2 interp_quad(const proj_seg* s, ov2 c) {
+  ov2 sm = divs2(add2(s->v00, s->v01), 2.0f);
+  ov2 em = divs2(add2(s->v10, s->v11), 2.0f);
+  ov2 cp = project_to_line(sm, em, c);
+  float d1 = len2(sub2(cp, sm)) / len2(sub2(sm, em));
+  float ws = len2(sub2(s->v00, s->v01));
+  float we = ws * dot2(norm2(sub2(s->v00, s->v01)), norm2(sub2(s->v10, s->v11)));
+  float ew = omix(ws, we, d1);
+  ov2 e1 = project_to_line(s->v00, s->v10, c);
+  float d0 = len2(sub2(c, e1)) / ew;
+  return o2(osat(d0), osat(d1));
+}
+
+/* interpolateHairF32 — swRasterizeHair.wgsl.ts:199-203 */
+static float interp_f32(ov2 w, ov4 v) {
+  float a = omix(v.x, v.y, w.x);
+  float b = omix(v.z, v.w, w.x);
+  return omix(a, b, w.y);
+}
+
+/* getDepthBin — shaderSnippets.wgls.ts:148-156 */
+static uint32_t depth_bin(uint32_t n, ov2 td, float d) {
+  float span = fabsf(td.y - td.x);
+  float t = (d - td.x) / span;
+  return oclampu(ou32(t * (float)n), 0u, n - 1u);
+}
+
+static float tex_load(const float* tex, uint32_t w, uint32_t h, int32_t x, int32_t y) {
+  if (x < 0 || y < 0 || (uint32_t)x >= w || (uint32_t)y >= h) return 0.0f;
+  return tex[(size_t)y * w + (uint32_t)x];
+}
+
+/* swRasterizer.test.ts:184-248 — the reference's raster-helper test kernel. */
+void or_debug_raster(const fh_render_uniforms* u, const fh_vec4* positions,
+                     const fh_vec4* tangents, uint32_t S, uint32_t P,
+                     uint32_t W, uint32_t H, uint32_t* out) {
+  ov2 vp = o2(u->viewport[0], u->viewport[1]);
+  for (uint32_t s = 0; s < S; s++)
+    for (uint32_t g = 0; g + 1 < P; g++) {
+      proj_seg ps = project_segment(u, positions, tangents, P, s, g, vp, u->fiberRadius);
+      /* getRasterizedHairBounds — swRasterizeHair.wgsl.ts:111-123 */
+      ov2 bmax = max2(max2(ps.v00, ps.v01), max2(ps.v10, ps.v11));
+      ov2 bmin = min2(min2(ps.v00, ps.v01), min2(ps.v10, ps.v11));
+      bmax = min2(o2(ceilf(bmax.x), ceilf(bmax.y)), vp);
+      bmin = max2(o2(floorf(bmin.x), floorf(bmin.y)), o2(0.0f, 0.0f));
+      for (float y = bmin.y; y < bmax.y; y += 1.0f)
+        for (float x = bmin.x; x < bmax.x; x += 1.0f) {
+          ov2 p = o2(x, y);
+          if (!inside_quad(&ps, p)) continue;
+          ov2 w = interp_quad(&ps, p);
+          uint32_t c0 = ou32(osat(w.x) * 255.0f);
+          uint32_t c1 = ou32(osat(w.y) * 255.0f);
+          uint32_t c2 = ou32(osat(0.1f) * 255.0f);
+          uint32_t value = 0xff000000u | c0 | (c1 << 8) | (c2 << 16);
+          uint32_t px = ou32(x), py = ou32(y);
+          if (px >= W || py >= H) continue;
+          uint32_t yy = H - py;
+          size_t idx = (size_t)yy * W + px;
+          if (idx >= (size_t)W * H) continue; /* dropped out-of-bounds write */
+          if (out[idx] < value) out[idx] = value;
+        }
+    }
+}
+
+/* ======================================================================== */
+/* K4 — hairTilesPass.wgsl.ts:70-246 with hairTilesResultBuffer.ts:14-42,   */
+/* hairTileSegmentsBuffer.ts:22-29, segmentCountPerTileBuffer.ts:15-20.      */
+/* Pinned: test_oracle_raster.py (raster fixture above, brute-force tiny     */
+/* scenes: every listed segment's quad covers a pixel of its tile that      */
+/* passes the depth test, every such pair is listed, bin is the closest).   */
+/* ======================================================================== */
+
+static uint32_t div_ceil(uint32_t a, uint32_t b) { return (a + b - 1u) / b; }
+
+void or_tiles_draw(const fh_render_cfg* cfg, uint32_t W, uint32_t H,
+                   const fh_render_uniforms* u, const fh_hair_data* hd,
+                   const fh_vec4* positions, const fh_vec4* tangents,
+                   fh_tile_result* tiles, const float* depth, void* segbuf,
+                   uint32_t capacity, uint32_t* countPerTile,
+                   uint32_t* slotKey, uint32_t S, uint32_t P) {
+  const uint32_t T = cfg->tileSize, B = cfg->tileDepthBins;
+  uint32_t* counter = (uint32_t*)segbuf;
+  fh_list_element* data = (fh_list_element*)((char*)segbuf + 4);
+  ov2 vp = o2(u->viewport[0], u->viewport[1]);
+  uint32_t vpW = ou32(vp.x), vpH = ou32(vp.y);
+  (void)W; (void)H;
+  uint32_t tcx = div_ceil(vpW, T);
+  uint32_t maxDrawn = u->maxDrawnHairSegments;
+  uint32_t limit = maxDrawn < capacity ? maxDrawn : capacity;
+  /* getHairDepthBoundsVS :242-246 */
+  ov4 bs = o4(hd->boundingSphere[0], hd->boundingSphere[1], hd->boundingSphere[2], hd->boundingSphere[3]);
+  ov4 bsc = mulmv(u->modelViewMat, o4(bs.x, bs.y, bs.z, 1.0f));
+  ov2 dbVS = o2(bsc.z - bs.w, bsc.z + bs.w);
+
+  for (uint32_t s = 0; s < S; s++)
+    for (uint32_t g = 0; g + 1 < P; g++) {
+      proj_seg ps = project_segment(u, positions, tangents, P, s, g, vp, u->fiberRadius);
+      ov2 bmax = max2(max2(ps.v00, ps.v01), max2(ps.v10, ps.v11));
+      ov2 bmin = min2(min2(ps.v00, ps.v01), min2(ps.v10, ps.v11));
+      bmax = min2(o2(ceilf(bmax.x), ceilf(bmax.y)), vp);
+      bmin = max2(o2(floorf(bmin.x), floorf(bmin.y)), o2(0.0f, 0.0f));
+      uint32_t tminx = ou32(bmin.x) / T, tminy = ou32(bmin.y) / T;
+      uint32_t tmaxx = ou32(bmax.x) / T, tmaxy = ou32(bmax.y) / T;
+      uint32_t tsx = (tmaxx - tminx) + 1u, tsy = (tmaxy - tminy) + 1u;
+      if (tsx * tsy > cfg->invalidTilesPerSegmentThreshold) continue;
+      for (uint32_t ty = tminy; ty <= tmaxy; ty++)
+        for (uint32_t tx = tminx; tx <= tmaxx; tx++) {
+          /* processTile :139-238; getTileBoundsPx tileUtils.wgsl.ts:44-54 */
+          uint32_t x0 = oclampu(tx * T, 0, vpW - 1), y0 = oclampu(ty * T, 0, vpH - 1);
+          uint32_t x1 = oclampu((tx + 1) * T, 0, vpW - 1), y1 = oclampu((ty + 1) * T, 0, vpH - 1);
+          float dmin = 999.0f, dmax = -999.0f;
+          uint32_t dbin = B;
+          for (uint32_t y = y0; y < y1; y++)
+            for (uint32_t x = x0; x < x1; x++) {
+              ov2 p = o2((float)x, (float)y);
+              if (!inside_quad(&ps, p)) continue;
+              ov2 w = interp_quad(&ps, p);
+              float hdp = interp_f32(w, ps.depths);
+              float db = tex_load(depth, vpW, vpH, (int32_t)x, (int32_t)(vpH - y));
+              if (hdp > db) continue;
+              ov3 vs = oproject(u->projMatrixInv, o4(p.x, p.y, hdp, 1.0f));
+              uint32_t bin = (B - 1u) - depth_bin(B, dbVS, vs.z);
+              dmin = omin(dmin, hdp);
+              dmax = omax(dmax, hdp);
+              dbin = bin < dbin ? bin : dbin;
+            }
+          if (dmin > 1.0f) continue;
+          uint32_t nextPtr = (*counter)++;
+          if (nextPtr < limit) {
+            uint32_t tileIdx = ty * tcx * B + tx * B + dbin;
+            fh_tile_result* t = &tiles[tileIdx];
+            uint32_t dMaxU = ou32(dmax * 4294967296.0f);
+            uint32_t dMinU = ou32((1.0f - dmin) * 4294967296.0f);
+            if (t->maxDepth < dMaxU) t->maxDepth = dMaxU;
+            if (t->minDepth < dMinU) t->minDepth = dMinU;
+            uint32_t last = t->tileSegmentPtr;
+            t->tileSegmentPtr = nextPtr + 1u;
+            t->padding0 += 1u;
+            uint32_t prevPtr = last == 0u ? INVALID_PTR : last - 1u;
+            data[nextPtr].strandAndSegmentIdxs = (g << 24) | s;
+            data[nextPtr].next = prevPtr;
+            if (slotKey) slotKey[nextPtr] = tileIdx;
+            countPerTile[ty * tcx + tx] += 1u;
+          }
+        }
+    }
+}
+
+/* ======================================================================== */
+/* K5 + K6 — hairTileSortPass.countTiles.wgsl.ts:63-97,                      */
+/* hairTileSortPass.sort.wgsl.ts:37-75.                                      */
+/* Pinned: test_oracle_raster.py with the inline vectors of                  */
+/* hairTileSortPass.test.ts:47-53 (histogram closed form :160-183 and        */
+/* non-increasing bucket order :122-127).                                    */
+/* ======================================================================== */
+
+void or_tiles_sort(const fh_render_cfg* cfg, uint32_t W, uint32_t H,
+                   const uint32_t* countPerTile, uint32_t* tileList,
+                   fh_sort_bucket* buckets) {
+  uint32_t n = div_ceil(W, cfg->tileSize) * div_ceil(H, cfg->tileSize);
+  uint32_t NB = cfg->sortBuckets;
+  for (uint32_t t = 0; t < n; t++) {
+    uint32_t c = countPerTile[t];
+    if (c > 0u) buckets[oclampu(c / cfg->sortBucketSize, 0, NB - 1)].tileCount += 1u;
+  }
+  uint32_t* offs = (uint32_t*)calloc(NB, sizeof(uint32_t));
+  uint32_t off = 0;
+  for (uint32_t i = 0; i < NB; i++) {
+    uint32_t idx = NB - 1u - i;
+    offs[idx] = off;
+    off += buckets[idx].tileCount;
+  }
+  for (uint32_t t = 0; t < n; t++) {
+    uint32_t c = countPerTile[t];
+    if (c > 0u) {
+      uint32_t b = oclampu(c / cfg->sortBucketSize, 0, NB - 1);
+      uint32_t o = offs[b] + buckets[b].writeOffset++;
+      tileList[1 + o] = t;
+      tileList[0] += 1u;
+    }
+  }
+  free(offs);
+}
+
+/* ======================================================================== */
+/* K7 — hairFinePass.wgsl.ts:126-335, processHairSegment.wgsl.ts:5-70,       */
+/* reduceHairSlices.wgsl.ts:9-78, hairSlicesDataBuffer.ts:42-78,             */
+/* hairSliceHeadsBuffer.ts:136-180, hairRasterizerResultBuffer.ts:14-34.     */
+/* Pinned: test_oracle_raster.py (one segment over a pixel -> shading *      */
+/* alpha; two segments in different slices -> front-to-back "over"; a >=    */
+/* 0.999 cutoff; empty bin skipped; f16 round trip).                          */
+/* canonical != 0 walks each (tile, bin) list in descending (strand, seg)   */
+/* order, i.e. the order this oracle's own tile pass produces.              */
+/* ======================================================================== */
+
+static int cmp_desc_key(const void* a, const void* b) {
+  uint32_t x = *(const uint32_t*)a, y = *(const uint32_t*)b;
+  uint32_t kx = ((x & 0xffffffu) << 8) | (x >> 24), ky = ((y & 0xffffffu) << 8) | (y >> 24);
+  return kx < ky ? 1 : (kx > ky ? -1 : 0);
+}
+
+void or_fine_raster(const fh_render_cfg* cfg, uint32_t W, uint32_t H,
+                    const fh_render_uniforms* u, const fh_vec4* positions,
+                    const fh_tile_result* tiles, const void* segbuf,
+                    uint32_t capacity, void* rasterResult, const float* depth,
+                    const fh_vec4* shading, const fh_vec4* tangents,
+                    const uint32_t* tileList, uint32_t S, uint32_t P,
+                    int canonical) {
+  const uint32_t T = cfg->tileSize, B = cfg->tileDepthBins, NS = cfg->slicesPerPixel;
+  const uint32_t PS = cfg->shadingPoints;
+  const uint32_t PER_PROC = cfg->avgFragmentsPerSlice * NS * T * T;
+  const fh_list_element* data = (const fh_list_element*)((const char*)segbuf + 4);
+  uint32_t* queue = (uint32_t*)rasterResult;
+  float* fb = (float*)((char*)rasterResult + 16);
+  const size_t fbLen = (size_t)W * H - 1u; /* W*H*16 B buffer minus header */
+  ov2 vpf = o2(u->viewport[0], u->viewport[1]);
+  uint32_t vpW = ou32(vpf.x), vpH = ou32(vpf.y);
+  (void)W; (void)H;
+  uint32_t tcx = div_ceil(vpW, T), tcy = div_ceil(vpH, T);
+  uint32_t tileCount = tcx * tcy;
+  uint32_t maxDrawn = u->maxDrawnHairSegments;
+  uint32_t maxProcessed = S * P;
+  uint32_t dispMode = u->displayMode & 0xffu;
+  uint32_t dbgMax = dispMode == 4u ? ((u->displayMode >> 8) & 0xffffu) : 0u;
+  int isDbg = dbgMax != 0u;
+  const uint32_t NPX = T * T;
+  uint32_t* heads = (uint32_t*)malloc(sizeof(uint32_t) * NPX * NS);
+  uint32_t (*pool)[4] = (uint32_t(*)[4])malloc(sizeof(uint32_t) * 4 * PER_PROC);
+  int* done = (int*)malloc(sizeof(int) * NPX);
+  uint32_t* listBuf = (uint32_t*)malloc(sizeof(uint32_t) * (maxProcessed + 1u));
+  uint32_t* listPtr = (uint32_t*)malloc(sizeof(uint32_t) * (maxProcessed + 1u));
+  for (uint32_t i = 0; i < NPX * NS; i++) heads[i] = INVALID_PTR;
+
+  uint32_t drawnTiles = tileList[0];
+  for (uint32_t qi = 0; qi < drawnTiles; qi++) {
+    *queue += 1u;
+    uint32_t tileIdx = tileList[1 + qi];
+    uint32_t row = tileIdx / tcx;
+    uint32_t tx = tileIdx - tcx * row, ty = row;
+    uint32_t bx0 = oclampu(tx * T, 0, vpW - 1), by0 = oclampu(ty * T, 0, vpH - 1);
+    for (uint32_t i = 0; i < NPX; i++) done[i] = 0;
+    for (uint32_t bin = 0; bin < B && tileIdx < tileCount; bin++) {
+      uint32_t sliceOff = 0;
+      const fh_tile_result* tr = &tiles[ty * tcx * B + tx * B + bin];
+      /* _getTileDepth — hairTilesResultBuffer.ts:47-54 */
+      ov2 tdepth = o2((float)(MAX_U32 - tr->minDepth) / 4294967296.0f,
+                      (float)tr->maxDepth / 4294967296.0f);
+      int binVisited = 0;
+      if (tdepth.y != 0.0f) {
+        binVisited = 1;
+        /* gather the list */
+        uint32_t n = 0;
+        uint32_t ptr = tr->tileSegmentPtr == 0u ? INVALID_PTR : tr->tileSegmentPtr - 1u;
+        while (n < maxProcessed && ptr < maxDrawn && ptr != INVALID_PTR && ptr < capacity) {
+          listBuf[n] = data[ptr].strandAndSegmentIdxs;
+          listPtr[n] = ptr;
+          n++;
+          ptr = data[ptr].next;
+        }
+        if (canonical) qsort(listBuf, n, sizeof(uint32_t), cmp_desc_key);
+        for (uint32_t li = 0; li < n; li++) {
+          uint32_t strand = listBuf[li] & 0x00ffffffu, seg = listBuf[li] >> 24;
+          proj_seg ps = project_segment(u, positions, tangents, P, strand, seg, vpf, u->fiberRadius);
+          for (uint32_t pix = 0; pix < NPX; pix++) {
+            if (done[pix]) continue;
+            uint32_t lx = pix % T, ly = pix / T;
+            /* processHairSegment — processHairSegment.wgsl.ts:5-54 */
+            ov2 p = o2((float)(bx0 + lx), (float)(by0 + ly));
+            if (!inside_quad(&ps, p)) continue;
+            ov2 w = interp_quad(&ps, p);
+            float hdp = interp_f32(w, ps.depths);
+            float db = tex_load(depth, vpW, vpH, oi32(p.x), oi32(vpf.y - p.y));
+            if (hdp >= db) continue;
+            uint32_t slot = sliceOff++;
+            if (!(slot < PER_PROC)) continue;
+            /* getAlphaCoverage :61-70 */
+            float alpha = 1.0f - fabsf(w.x * 2.0f - 1.0f);
+            if (cfg->alphaQuadratic) alpha = sqrtf(alpha);
+            alpha = osat(alpha * cfg->alphaMultiplier);
+            float tFull = ((float)seg + w.y) / (float)(P - 1u);
+            /* _sampleShading — hairShadingBuffer.ts:29-38 */
+            float a = tFull * ((float)PS - 1.0f);
+            uint32_t au = ou32(a);
+            uint32_t i0 = oclampu(au, 0, PS - 1), i1 = oclampu(au + 1u, 0, PS - 1);
+            float fr = ofract(a);
+            const fh_vec4* c0 = &shading[(size_t)strand * PS + i0];
+            const fh_vec4* c1 = &shading[(size_t)strand * PS + i1];
+            float cr = omix(c0->x, c1->x, fr), cg = omix(c0->y, c1->y, fr);
+            float cb = omix(c0->z, c1->z, fr), ca = omix(c0->w, c1->w, fr);
+            ca = ca * alpha;
+            uint32_t sl = depth_bin(NS, tdepth, hdp);
+            uint32_t hi = pix * NS + sl;
+            uint32_t prevPtr = heads[hi];
+            heads[hi] = slot;
+            pool[slot][0] = opack2h(cr, cg);
+            pool[slot][1] = opack2h(cb, ca);
+            pool[slot][2] = prevPtr;
+            pool[slot][3] = 0u;
+          }
+          if (!(sliceOff < PER_PROC)) break;
+        }
+        (void)listPtr;
+        /* after the segment loop the reference sets the offset to the pool
+         * size when the list ended, so the reduce below always runs for a
+         * visited bin (hairFinePass.wgsl.ts:252-275) */
+        for (uint32_t pix = 0; pix < NPX; pix++) {
+          if (done[pix]) continue;
+          /* reduceHairSlices — reduceHairSlices.wgsl.ts:9-73 */
+          uint32_t lx = pix % T, ly = pix / T;
+          uint32_t px = bx0 + lx, py = by0 + ly;
+          size_t idx = (size_t)vpW * py + px;
+          float fc[4] = {0, 0, 0, 0};
+          if (idx < fbLen) memcpy(fc, &fb[idx * 4], 16);
+          uint32_t cnt = isDbg ? ou32(fc[0] * (float)dbgMax) : 0u;
+          uint32_t s = 0;
+          for (; s < NS; s++) {
+            if (!isDbg && fc[3] >= 0.999f) break;
+            int clr = 0;
+            uint32_t sp = heads[pix * NS + s];
+            while (sp != INVALID_PTR && sp < PER_PROC) {
+              clr = 1;
+              if (!isDbg && fc[3] >= 0.999f) break;
+              uint32_t* e = pool[sp];
+              sp = e[2];
+              cnt++;
+              float sc[4] = {oh2f(e[0] & 0xffffu), oh2f(e[0] >> 16), oh2f(e[1] & 0xffffu), oh2f(e[1] >> 16)};
+              float oneMinusA = 1.0f - fc[3];
+              for (int k = 0; k < 4; k++) fc[k] = fc[k] + sc[k] * oneMinusA;
+            }
+            if (clr) heads[pix * NS + s] = INVALID_PTR;
+          }
+          for (; s < NS; s++) heads[pix * NS + s] = INVALID_PTR;
+          if (isDbg) {
+            float c = osat((float)cnt / (float)dbgMax);
+            fc[0] = c; fc[1] = 0.0f; fc[2] = 0.0f; fc[3] = 1.0f;
+          }
+          if (px < vpW && py < vpH && idx < fbLen) memcpy(&fb[idx * 4], fc, 16);
+          done[pix] = !isDbg && fc[3] >= 0.999f;
+        }
+      }
+      (void)binVisited;
+      /* checkAllPixelsInWkgrpDone :305-327 */
+      int all = 1;
+      for (uint32_t i = 0; i < NPX; i++) all &= done[i];
+      if (all) break;
+    }
+  }
+  free(heads); free(pool); free(done); free(listBuf); free(listPtr);
+}
+
+/* ======================================================================== */
+/* K8 — hairShadingPass.wgsl.ts:61-282, marschner.wgsl.ts:19-170,            */
+/* pbr.wgsl.ts:22-24, noise.wgsl.ts:45-51, sampleShadows.wgsl.ts:21-150,     */
+/* textureAo.wgsl.ts:7-12, linearDepth.wgsl.ts:9-27.                         */
+/* Pinned: test_oracle_shading.py against an independent double-precision    */
+/* evaluation of the Kajiya-Kay / Marschner (Karis16) formulas, and shadow   */
+/* strength 0 / AO strength 0 neutralising the texture inputs.               */
+/* ======================================================================== */
+
+#define O_PI 3.14159265358979323846f
+
+static float fresnel1(float cosT, float f0) {
+  return f0 + (1.0f - f0) * powf(1.0f - cosT, 5.0f);
+}
+static float cos_half(float c) { return sqrtf(osat(0.5f + 0.5f * c)); }
+static float long_gauss(float si, float sr, float alpha, float beta) {
+  const float SQRT_TWO_PI = 2.5066282746310002f;
+  float a = si + sr - alpha;
+  float b = -(a * a) / (2.0f * beta * beta);
+  return expf(b) / (beta * SQRT_TWO_PI);
+}
+
+typedef struct {
+  ov3 baseColor;
+  float specular, weightTT, weightTRT, shift, roughness;
+} marschner_params;
+
+static ov3 marschner(const marschner_params* p, ov3 L, ov3 V, ov3 T) {
+  const float HAIR_F0 = (float)((1.0 - 1.55) * (1.0 - 1.55) / ((1.0 + 1.55) * (1.0 + 1.55)));
+  float si = dot3(T, L), sr = dot3(T, V);
+  float ci = sqrtf(omax(0.0f, 1.0f - si * si));
+  float cr = sqrtf(omax(0.0f, 1.0f - sr * sr));
+  float cdf = ci * cr + si * sr;
+  float cd = cos_half(cdf);
+  ov3 lp = sub3(L, muls3(T, si)), cp = sub3(V, muls3(T, sr));
+  float cphi = dot3(lp, cp) * (1.0f / sqrtf(len3(lp) * len3(cp) + 1e-4f));
+  float chp = cos_half(cphi);
+  float alpha = -2.0f * p->shift, beta = p->roughness * p->roughness;
+  float Mr = long_gauss(si, sr, alpha, beta);
+  float Nr = 0.25f * chp * fresnel1(cd, HAIR_F0);
+  alpha = p->shift; beta = 0.5f * p->roughness * p->roughness;
+  float Mtt = long_gauss(si, sr, alpha, beta);
+  /* azimuthalScattering_TT :412-434 */
+  float eta = (1.19f / cd) + (0.36f * cd);
+  float a = 1.0f / eta;
+  float hTT = (1.0f + a * (0.6f - 0.8f * cphi)) * chp;
+  float Tnum = sqrtf(1.0f - hTT * hTT * a * a);
+  float Tden = 2.0f * cd;
+  ov3 TTT = o3(powf(p->baseColor.x, Tnum / Tden), powf(p->baseColor.y, Tnum / Tden), powf(p->baseColor.z, Tnum / Tden));
+  float DTT = expf(-3.65f * cphi - 3.98f);
+  float fa = cd * sqrtf(osat(1.0f - hTT * hTT));
+  float f = fresnel1(fa, HAIR_F0);
+  float fTT = (1.0f - f) * (1.0f - f);
+  ov3 Ntt = muls3(muls3(TTT, fTT), DTT);
+  alpha = 4.0f * p->shift; beta = 2.0f * p->roughness * p->roughness;
+  float Mtrt = long_gauss(si, sr, alpha, beta);
+  /* azimuthalScattering_TRT :437-454 */
+  float e = 0.8f / cd;
+  ov3 TTRT = o3(powf(p->baseColor.x, e), powf(p->baseColor.y, e), powf(p->baseColor.z, e));
+  float DTRT = expf(17.0f * cphi - 16.78f);
+  float f2 = fresnel1(cd * 0.5f, HAIR_F0);
+  float fTRT = (1.0f - f2) * (1.0f - f2) * f2;
+  ov3 Ntrt = muls3(muls3(TTRT, fTRT), DTRT);
+  float r = p->specular * Mr * Nr;
+  ov3 res = add3(add3(o3(r, r, r), muls3(Ntt, p->weightTT * Mtt)), muls3(Ntrt, p->weightTRT * Mtrt));
+  return res;
+}
+
+/* textureGather on a clamp-to-edge sampler: texels (i0,j1),(i1,j1),(i1,j0),(i0,j0) */
+static float gather_avg(const float* tex, uint32_t w, uint32_t h, ov2 uv) {
+  float x = uv.x * (float)w - 0.5f, y = uv.y * (float)h - 0.5f;
+  int i0 = (int)floorf(x), j0 = (int)floorf(y);
+  int i1 = i0 + 1, j1 = j0 + 1;
+#define CL(v, n) ((v) < 0 ? 0 : ((v) > (int)(n) - 1 ? (int)(n) - 1 : (v)))
+  i0 = CL(i0, w); i1 = CL(i1, w); j0 = CL(j0, h); j1 = CL(j1, h);
+#undef CL
+  float gx = tex[(size_t)j1 * w + i0], gy = tex[(size_t)j1 * w + i1];
+  float gz = tex[(size_t)j0 * w + i1], gw = tex[(size_t)j0 * w + i0];
+  return (gx + gy + gz + gw) / 4.0f;
+}
+
+static float linearize_depth(float d) {
+  const float n = 0.01f, f = 100.0f;
+  return n * f / (f + (n - f) * d);
+}
+
+static ov3 hair_shading(const marschner_params* p, const fh_light* light, ov3 toCam,
+                        ov4 tangentWS, ov4 posWS, float shadow, float ao) {
+  ov3 L = norm3(sub3(o3(light->position[0], light->position[1], light->position[2]), xyz(posWS)));
+  ov3 T = xyz(tangentWS);
+  /* KajiyaKayDiffuse :209-213 */
+  ov3 diffuse = muls3(p->baseColor, omix(0.25f, 1.0f, dot3(T, L)));
+  /* fakeMultipleScattering :216-234 */
+  ov3 fn = norm3(sub3(toCam, muls3(T, dot3(toCam, T))));
+  float NoL = osat(dot3(fn, L));
+  float ds = (NoL + 1.0f) / (4.0f * O_PI);
+  float luma = 0.2126729f * p->baseColor.x + 0.7151522f * p->baseColor.y + 0.0721750f * p->baseColor.z;
+  ov3 bl = divs3(p->baseColor, luma);
+  ov3 tint = o3(powf(bl.x, 1.0f - shadow), powf(bl.y, 1.0f - shadow), powf(bl.z, 1.0f - shadow));
+  ov3 ms = mul3(muls3(o3(sqrtf(p->baseColor.x), sqrtf(p->baseColor.y), sqrtf(p->baseColor.z)), ds), tint);
+  ov3 dt = muls3(mul3(diffuse, ms), osat(dot3(T, L)));
+  ov3 spec = marschner(p, L, toCam, T);
+  ov3 brdf = add3(dt, spec);
+  ov3 radiance = muls3(o3(light->colorAndEnergy[0], light->colorAndEnergy[1], light->colorAndEnergy[2]),
+                       light->colorAndEnergy[3]);
+  return muls3(mul3(brdf, radiance), 1.0f - ao);
+}
+
+void or_shading(const fh_render_cfg* cfg, const fh_render_uniforms* u,
+                const fh_vec4* positions, const fh_vec4* tangents,
+                fh_vec4* out, const float* shadowTex, uint32_t shadowW,
+                uint32_t shadowH, const float* aoTex, uint32_t aoW,
+                uint32_t aoH, const float* depth, uint32_t dW, uint32_t dH,
+                uint32_t S, uint32_t P) {
+  const uint32_t PS = cfg->shadingPoints;
+  ov2 vp = o2(u->viewport[0], u->viewport[1]);
+  ov3 cam = o3(u->cameraPosition[0], u->cameraPosition[1], u->cameraPosition[2]);
+  const fh_hair_material* m = &u->hairMaterial;
+  for (uint32_t s = 0; s < S; s++)
+    for (uint32_t id = 0; id < PS; id++) {
+      float t = (float)id / (float)PS;
+      /* remapToIndices — shaderSnippets.wgls.ts:120-126 */
+      float a = t * ((float)P - 1.0f);
+      uint32_t au = ou32(a);
+      uint32_t ix = oclampu(au, 0, P - 1), iy = oclampu(au + 1u, 0, P - 1);
+      float tIn = ofract(a);
+      const fh_vec4* q0 = &positions[(size_t)s * P + ix];
+      const fh_vec4* q1 = &positions[(size_t)s * P + iy];
+      ov3 s0 = o3(q0->x, q0->y, q0->z), s1 = o3(q1->x, q1->y, q1->z);
+      ov3 t0 = norm3(sub3(s1, s0));
+      const fh_vec4* tt = &tangents[(size_t)s * P + iy];
+      ov3 t1 = o3(tt->x, tt->y, tt->z);
+      ov3 pObj = mix3(s0, s1, tIn), tObj = mix3(t0, t1, tIn);
+      ov4 posWS = mulmv(u->modelMatrix, o4(pObj.x, pObj.y, pObj.z, 1.0f));
+      ov4 tanWS = mulmv(u->modelMatrix, o4(tObj.x, tObj.y, tObj.z, 1.0f));
+      ov3 pproj = oproject(u->vpMatrix, posWS);
+      ov2 p01 = o2(pproj.x * 0.5f + 0.5f, pproj.y * 0.5f + 0.5f);
+      ov2 texPx = o2(vp.x * p01.x, vp.y * (1.0f - p01.y));
+      /* base colour :105-114 */
+      ov3 c0 = o3(m->color0[0], m->color0[1], m->color0[2]);
+      ov3 c1 = o3(m->color1[0], m->color1[1], m->color1[2]);
+      ov3 base = mix3(c0, c1, t);
+      float fs = (float)s;
+      ov3 rnd = sat3(o3(ofract(fs * 1.73f) * 1.0f, ofract(fs * 1.17f) * 1.0f, ofract(fs * 1.31f) * 1.0f));
+      base = mix3(base, rnd, m->colorRng);
+      float rb = ofract(fs / 255.0f);
+      base = mix3(base, muls3(base, rb), m->lumaRng);
+      ov3 toCam = norm3(sub3(cam, xyz(posWS)));
+      marschner_params mp = {base, m->specular, m->weightTT, m->weightTRT, m->shift, m->roughness};
+      /* shadow — getShadow :236-254, calculateDirectionalShadow
+       * sampleShadows.wgsl.ts:47-84 */
+      ov4 lp = mulmv(u->shadows.sourceMVP_Matrix, posWS);
+      ov3 pss = o3(lp.x * 0.5f + 0.5f, 1.0f - (lp.y * 0.5f + 0.5f), lp.z);
+      ov3 ssp = o3(u->shadows.sourcePosition[0], u->shadows.sourcePosition[1], u->shadows.sourcePosition[2]);
+      ov3 toSrc = norm3(sub3(ssp, xyz(posWS)));
+      float bias = omax(fabsf(u->shadows.bias) * (1.0f - dot3(xyz(tanWS), toSrc)), 0.0005f);
+      int32_t radius = (int32_t)u->shadows.PCF_Radius;
+      if (u->shadows.usePCSS > 0u) {
+        float sm = gather_avg(shadowTex, shadowW, shadowH, o2(pss.x, pss.y));
+        float dd = omax(pss.z - sm, 0.0f);
+        radius = 1 + oi32(dd / sm * 10.0f);
+      }
+      float texel = 1.0f / (float)shadowW;
+      float sh = 0.0f;
+      for (int32_t x = -radius; x <= radius; x++)
+        for (int32_t y = -radius; y <= radius; y++) {
+          ov2 off = o2((float)x * texel, (float)y * texel);
+          float sm = gather_avg(shadowTex, shadowW, shadowH, add2(o2(pss.x, pss.y), off));
+          sh += (pss.z - bias > sm) ? 1.0f : 0.0f;
+        }
+      float pcf = (float)(radius * 2 + 1);
+      float shadowRes = sh / (pcf * pcf);
+      int valid = !(pss.z > 1.0f) && !(pss.x < 0.0f || pss.x > 1.0f || pss.y < 0.0f || pss.y > 1.0f);
+      if (!valid) shadowRes = 0.0f;
+      float shadow = 1.0f - shadowRes;
+      shadow = oclamp(shadow, 1.0f - m->shadows, 1.0f);
+      /* ao — textureAo.wgsl.ts:7-12 */
+      ov2 at = o2(texPx.x / vp.x, texPx.y / vp.y);
+      int32_t ax = oi32((float)aoW * at.x), ay = oi32((float)aoH * at.y);
+      float aov = tex_load(aoTex, aoW, aoH, ax, ay);
+      float ao = 1.0f - omix(1.0f, aov, u->ao.strength);
+      /* attenuation :259-282 */
+      float myD = linearize_depth(pproj.z);
+      float zb = tex_load(depth, dW, dH, (int32_t)ou32(texPx.x), (int32_t)ou32(texPx.y));
+      float att = fabsf(myD - linearize_depth(zb)) * m->attenuation;
+      ao = osat(ao + att);
+      ov3 rs = o3(0, 0, 0);
+      rs = add3(rs, hair_shading(&mp, &u->light0, toCam, tanWS, posWS, shadow, ao));
+      rs = add3(rs, hair_shading(&mp, &u->light1, toCam, tanWS, posWS, shadow, ao));
+      rs = add3(rs, hair_shading(&mp, &u->light2, toCam, tanWS, posWS, shadow, ao));
+      float alpha = id >= PS - 1u ? 0.0f : 1.0f;
+      fh_vec4* o = &out[(size_t)s * PS + id];
+      o->x = rs.x; o->y = rs.y; o->z = rs.z; o->w = alpha;
+    }
+}

@@ -1,0 +1,68 @@
+/*
+ * CPU ORACLE — TEST INFRASTRUCTURE ONLY (see fh_oracle.c).
+ * Sequential restatement of the reference's hair compute chain, used by
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the
+ * checker. Never linked into or called by the product library.
+ */
+#ifndef FH_ORACLE_H
+#define FH_ORACLE_H
+
+#include "../include/fh_hair.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+float or_sample_sdf(const float* sdf, uint32_t d, const float* bmin,
+                    const float* bmax, const float* p);
+
+void or_grid_pre_sim(uint32_t d, const fh_sim_uniforms* u,
+                     const fh_density_velocity* dv, fh_density_grad_wind* out,
+                     const float* sdf, uint32_t sdf_dims);
+
+void or_sim_integrate(uint32_t d, const fh_sim_uniforms* u, fh_vec4* prev,
+                      const fh_vec4* now, const float* seglen,
+                      const float* sdf, uint32_t sdf_dims,
+                      const fh_density_velocity* dv,
+                      const fh_density_grad_wind* dgw, const fh_vec4* initial,
+                      uint32_t S, uint32_t P);
+
+void or_grid_post_sim(uint32_t d, const fh_sim_uniforms* u, const fh_vec4* prev,
+                      const fh_vec4* now, fh_density_velocity* dv, uint32_t S,
+                      uint32_t P);
+
+void or_debug_raster(const fh_render_uniforms* u, const fh_vec4* positions,
+                     const fh_vec4* tangents, uint32_t S, uint32_t P,
+                     uint32_t W, uint32_t H, uint32_t* out);
+
+void or_tiles_draw(const fh_render_cfg* cfg, uint32_t W, uint32_t H,
+                   const fh_render_uniforms* u, const fh_hair_data* hd,
+                   const fh_vec4* positions, const fh_vec4* tangents,
+                   fh_tile_result* tiles, const float* depth, void* segbuf,
+                   uint32_t capacity, uint32_t* countPerTile,
+                   uint32_t* slotKey, uint32_t S, uint32_t P);
+
+void or_tiles_sort(const fh_render_cfg* cfg, uint32_t W, uint32_t H,
+                   const uint32_t* countPerTile, uint32_t* tileList,
+                   fh_sort_bucket* buckets);
+
+void or_fine_raster(const fh_render_cfg* cfg, uint32_t W, uint32_t H,
+                    const fh_render_uniforms* u, const fh_vec4* positions,
+                    const fh_tile_result* tiles, const void* segbuf,
+                    uint32_t capacity, void* rasterResult, const float* depth,
+                    const fh_vec4* shading, const fh_vec4* tangents,
+                    const uint32_t* tileList, uint32_t S, uint32_t P,
+                    int canonical);
+
+void or_shading(const fh_render_cfg* cfg, const fh_render_uniforms* u,
+                const fh_vec4* positions, const fh_vec4* tangents,
+                fh_vec4* out, const float* shadowTex, uint32_t shadowW,
+                uint32_t shadowH, const float* aoTex, uint32_t aoW,
+                uint32_t aoH, const float* depth, uint32_t dW, uint32_t dH,
+                uint32_t S, uint32_t P);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
