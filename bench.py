@@ -1,0 +1,203 @@
+"""Benchmark of the per-frame hair compute chain on one MI355X.
+
+A step is one frame of the chain on the 1080p character-scale workload
+(10,000 strands x 32 points, 64^3 grid, 64^3 SDF, 1920x1080, 16x16 tiles,
+8 depth bins x 4 slices, 8 shading points): the simulation step (grid
+pre-pass, integration, particle-to-grid), then the tile pass, the tile sort,
+the fine pass and the shading pass, with inputs resident in HBM.
+
+Prints one JSON line. `roofline` is for the stage that takes the most time,
+from its algorithmic bytes per frame and its average duration measured with
+HIP events on the library's stream inside the timed region. `cpu_baseline`
+is the CPU oracle (a scalar C restatement of the reference's kernels) timed
+on full frames of the same workload, rank 0 only.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "sim+raster frames/s at 1080p, batched vs plain; p50/p99 frame latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(wl, n_pairs: int, tiles: int) -> dict:
+    """Compulsory HBM bytes per frame per stage (each distinct input byte read
+    once, each output byte written once)."""
+    N = wl.strands * wl.points
+    d = wl.sim.gridDims
+    V = wl.sdf_dims ** 3
+    WH = wl.width * wl.height
+    B = wl.render.tileDepthBins
+    S, Ps = wl.strands, wl.render.shadingPoints
+    return {
+        "sim": 100 * N + 96 * d ** 3 + 8 * V,
+        "tiles": 32 * N + 4 * WH + 24 * n_pairs + 16 * tiles * B + 4 * tiles,
+        "sort": 12 * tiles + 1024,
+        "fine": 16 * tiles * B + 8 * n_pairs + 32 * N + 4 * WH + 16 * S * Ps + 32 * WH + 4 * tiles,
+        "shading": 32 * N + 4 * WH + WH // 2 + 2048 * 2048 * 4 + 16 * S * Ps,
+    }
+
+
+def cpu_baseline(wl, r, frames: int = 3) -> dict:
+    """Times the oracle on `frames` full frames of the workload on one core."""
+    import __graft_entry__ as G
+    O = G.load_oracle()
+    from frostbitten_hair_webgpu_amd import assets as A
+    from frostbitten_hair_webgpu_amd import layouts as L
+    S, P = wl.strands, wl.points
+    i = {k: np.ascontiguousarray(v.cpu().numpy()) for k, v in r.inputs.items()}
+    sl = A.segment_lengths(wl.positions, S, P)
+    tan = A.tangents(wl.positions, S, P)
+    hd = L.pack_hair_data(r.hair.sphere[0], r.hair.sphere[1], S, P)
+    cap = r.tilesPass.segmentsCapacity
+    prev = wl.positions.copy()
+    now = wl.positions.copy()
+    shading = A.initial_shading(S, wl.render.shadingPoints)
+    dv = O.grid_post_sim(wl.sim.gridDims, r.sim_uniforms(0), prev, now, S, P)
+    times = []
+    for f in range(frames):
+        t0 = time.perf_counter()
+        su = r.sim_uniforms(f)
+        dgw = O.grid_pre_sim(wl.sim.gridDims, su, dv, wl.sdf, wl.sdf_dims)
+        nxt = O.sim_integrate(wl.sim.gridDims, su, prev, now, sl, wl.sdf, wl.sdf_dims, dv, dgw, wl.positions, S, P)
+        prev, now = now, nxt
+        dv = O.grid_post_sim(wl.sim.gridDims, su, prev, now, S, P)
+        tiles, seg, counts, _ = O.tiles_draw(wl.render, wl.width, wl.height, r.render_uniforms, hd, now, tan,
+                                             i["meshDepth"], cap, S, P)
+        tl, _ = O.tiles_sort(wl.render, wl.width, wl.height, counts)
+        O.fine_raster(wl.render, wl.width, wl.height, r.render_uniforms, now, tiles, seg, cap, i["meshDepth"],
+                      shading, tan, tl, S, P, canonical=False)
+        shading = O.shading(wl.render, r.render_uniforms, now, tan, i["shadowMap"], i["ao"], i["hairDepth"],
+                            S, P).reshape(-1, 4)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    return {"value": round(1.0 / t, 4), "unit": "frames/s", "cores": 1, "kind": "port",
+            "sample": f"{frames} full frames (sim step + tile + sort + fine + shading) of the same workload, "
+                      f"median {t:.2f} s/frame, scalar C oracle on 1 host core"}
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="c2", choices=["small", "c2", "c3"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: the hair chain has no multi-GPU path here (asked for --gpus "
+              f"{args.gpus}); see DESIGN.md for how it would shard.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import __graft_entry__ as G
+    G.load_package()
+    from frostbitten_hair_webgpu_amd import assets as A
+    from frostbitten_hair_webgpu_amd import passes as PS
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    t_setup = time.perf_counter()
+    wl = A.make_workload(args.workload)
+    r = PS.HairRenderer(wl, dev)
+    total = args.warmup + args.steps
+    # per-frame simulation uniforms (frameIdx changes), uploaded once up front
+    ring = torch.from_numpy(np.concatenate([r.sim_uniforms(f).view(np.uint8).reshape(-1) for f in range(total)]))
+    ring = ring.to(dev).view(total, -1)
+    print(f"# setup {time.perf_counter() - t_setup:.1f} s, workload {wl.name}: {wl.strands}x{wl.points} "
+          f"{wl.width}x{wl.height}", file=sys.stderr, flush=True)
+
+    stages = ["sim", "tiles", "sort", "fine", "shading"]
+
+    def frame(events=None):
+        r.simulationUniforms = ring[r.frameIdx]
+        ctx = r.ctx()
+        rec = (lambda k: events[k].record()) if events else (lambda k: None)
+        rec(0)
+        r.simulateHair(ctx)
+        rec(1)
+        r.tilesPass.cmdClearBeforeRender(ctx)
+        r.finePass.cmdClearBeforeRender(ctx)
+        r.sortPass.cmdClearBeforeRender(ctx)
+        r.tilesPass.cmdDrawHairToTiles(ctx, r.hair)
+        rec(2)
+        r.sortPass.cmdSortHairTiles(ctx)
+        rec(3)
+        r.finePass.cmdRasterizeSlicesHair(ctx, r.hair, r.tilesPass)
+        rec(4)
+        r.shadingPass.cmdComputeShadingPoints(ctx, r.hair)
+        rec(5)
+        r.frameIdx += 1
+
+    r.beforeFirstFrame()
+    for _ in range(args.warmup):
+        frame()
+    torch.cuda.synchronize()
+
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        frame(evs[k])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+
+    per_frame = np.array([evs[k][0].elapsed_time(evs[k][5]) for k in range(args.steps)])  # ms
+    stage_ms = {s: float(np.mean([evs[k][i].elapsed_time(evs[k][i + 1]) for k in range(args.steps)]))
+                for i, s in enumerate(stages)}
+    n_pairs = int(r.tilesPass.hairTileSegmentsBuffer[0].item())
+    tiles = (wl.width + wl.render.tileSize - 1) // wl.render.tileSize * \
+            ((wl.height + wl.render.tileSize - 1) // wl.render.tileSize)
+    drawn_tiles = int(r.sortPass.tileListBuffer[0].item())
+    ab = algorithmic_bytes(wl, n_pairs, tiles)
+    dom = max(stage_ms, key=stage_ms.get)
+    achieved = ab[dom] / (stage_ms[dom] * 1e-3) / 1e9
+    img = r.finePass.image(wl.width, wl.height)
+    assert torch.isfinite(img).all()
+
+    ms = elapsed / args.steps * 1e3
+    out = {
+        "metric": METRIC,
+        "value": round(1e3 / ms, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded strands on a scalp cap, analytic colliders; depth/shadow/AO inputs precomputed)",
+        "config": {"workload": f"plain drop-in, {wl.strands} strands x {wl.points} points, {wl.width}x{wl.height}, "
+                               f"tile {wl.render.tileSize}, {wl.render.tileDepthBins} bins x "
+                               f"{wl.render.slicesPerPixel} slices, grid {wl.sim.gridDims}^3, SDF {wl.sdf_dims}^3, "
+                               f"{wl.sim.constraintIterations} constraint iterations",
+                   "global_batch": 1, "parallelism": "none (1 GPU)"},
+        "latency_ms": {"p50": round(float(np.percentile(per_frame, 50)), 4),
+                       "p99": round(float(np.percentile(per_frame, 99)), 4)},
+        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "throughput": {"particle_steps_per_s": round(wl.strands * wl.points * 1e3 / ms, 1),
+                       "segments_rasterized_per_s": round(wl.strands * (wl.points - 1) * 1e3 / ms, 1),
+                       "segment_tile_pairs": n_pairs, "drawn_tiles": drawn_tiles},
+        "roofline": {"bound": "hbm", "stage": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "algorithmic_bytes": int(ab[dom])},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(wl, r)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

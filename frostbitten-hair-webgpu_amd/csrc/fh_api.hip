@@ -76,6 +76,7 @@ int fh_create(int device, fh_ctx** out) {
     return fh::fail(FH_ERR_HIP, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
   }
   c->num_cus = prop.multiProcessorCount;
+  c->max_lds = prop.sharedMemPerBlock;
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete c;

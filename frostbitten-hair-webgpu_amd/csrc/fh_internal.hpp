@@ -13,6 +13,7 @@ struct fh_ctx {
   hipStream_t stream = nullptr;
   bool owns_stream = false;
   int num_cus = 0;
+  size_t max_lds = 64 * 1024;  // LDS bytes one workgroup may allocate
   // Grow-on-demand scratch owned by the context (fine-pass gather lists,
   // per-bin offsets, prefix-scan temporaries).
   void* scratch = nullptr;

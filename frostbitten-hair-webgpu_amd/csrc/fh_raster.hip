@@ -231,7 +231,11 @@ __global__ __launch_bounds__(256) void sort_scatter_kernel(const uint32_t* __res
     localBase[threadIdx.x] = atomicAdd(&buckets[threadIdx.x].writeOffset, localCount[threadIdx.x]);
   if (threadIdx.x == 0 && total) atomicAdd(&tileList[0], total);
   __syncthreads();
-  if (has) tileList[1 + bucketOffset[b] + localBase[b] + rank] = t;
+  if (has) {
+    // a caller that skipped the per-frame clear would push offsets past the list
+    const uint32_t o = bucketOffset[b] + localBase[b] + rank;
+    if (o < nTiles) tileList[1 + o] = t;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -298,13 +302,14 @@ __global__ __launch_bounds__(256) void bin_scatter_kernel(const fh_render_unifor
                                                           const uint32_t* __restrict__ segCounter,
                                                           const fh_list_element* __restrict__ data,
                                                           uint32_t capacity, const uint32_t* __restrict__ slotKey,
-                                                          uint32_t* __restrict__ ends,
+                                                          uint32_t* __restrict__ ends, uint32_t nTB,
                                                           uint32_t* __restrict__ binSeg) {
   const uint32_t n = min(*segCounter, min(capacity, u->maxDrawnHairSegments));
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
     const uint32_t key = slotKey[s];
+    if (key >= nTB) continue;  // untagged slot: not linked into any list
     const uint32_t pos = atomicSub(&ends[key], 1u) - 1u;
-    binSeg[pos] = data[s].strandAndSegmentIdxs;
+    if (pos < capacity) binSeg[pos] = data[s].strandAndSegmentIdxs;
   }
 }
 
@@ -338,6 +343,7 @@ struct FineArgs {
   uint32_t S, P;
   uint32_t B, NS, PS;
   uint32_t perProc, ldsPool, batch;
+  uint32_t capacity;  // elements of the gathered segment array
   uint32_t alphaQuadratic;
   float alphaMultiplier;
 };
@@ -405,8 +411,8 @@ __global__ __launch_bounds__(TILE* TILE) void fine_raster_kernel(FineArgs a) {
       if (tr.maxDepth != 0u) {  // _getTileDepth == 0 means an empty bin (:215)
         const f2 td = mk2((float)(0xffffffffu - tr.minDepth) / 4294967296.0f,
                           (float)tr.maxDepth / 4294967296.0f);
-        const uint32_t n = tr.padding0;
-        const uint32_t start = a.binStart[tb];
+        const uint32_t start = min(a.binStart[tb], a.capacity);
+        const uint32_t n = min(tr.padding0, a.capacity - start);
         if (tid == 0) s_sliceOff = 0;
         for (uint32_t b0 = 0; b0 < n; b0 += a.batch) {
           const uint32_t nb = min(a.batch, n - b0);
@@ -705,14 +711,14 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
   }
   const uint32_t scatterBlocks = std::max(1u, std::min(fh::div_ceil(limit, 256), (uint32_t)ctx->num_cus * 8u));
   bin_scatter_kernel<<<scatterBlocks, 256, 0, ctx->stream>>>(
-      uniforms, reinterpret_cast<const uint32_t*>(tile_segments), data, limit, slot_tile_bin, ends, binSeg);
+      uniforms, reinterpret_cast<const uint32_t*>(tile_segments), data, limit, slot_tile_bin, ends, nTB, binSeg);
   if ((rc = check_launch("bin_scatter_kernel"))) return rc;
 
   FineArgs a;
   a.u = uniforms;
   a.pos = reinterpret_cast<const float4*>(positions);
   a.tan = reinterpret_cast<const float4*>(tangents);
-  a.tiles = tiles; a.binStart = ends; a.binSeg = binSeg;
+  a.tiles = tiles; a.binStart = ends; a.binSeg = binSeg; a.capacity = limit;
   a.slicePool = reinterpret_cast<uint4*>(slice_data);
   a.queue = reinterpret_cast<uint32_t*>(rasterizer_result);
   a.fb = reinterpret_cast<float4*>(reinterpret_cast<char*>(rasterizer_result) + 16);
@@ -725,9 +731,10 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
   a.alphaQuadratic = cfg->alphaQuadratic; a.alphaMultiplier = cfg->alphaMultiplier;
   const uint32_t npx = T * T;
   a.batch = std::min(npx, 256u);
-  // LDS: segment batch + slice heads + fragment pool. 80 KiB per workgroup
-  // keeps two 16x16 processors per CU (160 KiB LDS on gfx950).
-  const size_t ldsTarget = T == 8 ? 32 * 1024 : (T == 16 ? 80 * 1024 : 160 * 1024);
+  // LDS: segment batch + slice heads + fragment pool. Under 80 KiB per
+  // workgroup keeps two 16x16 processors per CU (160 KiB LDS on gfx950).
+  const size_t ldsTarget = std::min<size_t>(T == 8 ? 32 * 1024 : (T == 16 ? 78 * 1024 : 128 * 1024),
+                                            ctx->max_lds - 256);
   const size_t fixed = (size_t)a.batch * sizeof(SegRec) + (size_t)a.NS * npx * 4;
   FH_REQUIRE(fixed + 16 * 64 <= ldsTarget, "fh_fine_raster: slicesPerPixel too large for the tile size");
   a.ldsPool = (uint32_t)std::min<size_t>((ldsTarget - fixed) / 16, a.perProc);

@@ -383,9 +383,10 @@ int fh_sim_integrate(fh_ctx* ctx, uint32_t grid_dims, const fh_sim_uniforms* sim
              "fh_sim_integrate: null buffer");
   if (strands_count == 0) return FH_OK;
   // 3 float4 per point per strand in LDS; 64 strands per workgroup when that
-  // fits the 160 KiB LDS of a gfx950 CU, else 32.
+  // fits the device's per-workgroup LDS (160 KiB on gfx950), else 32.
   const size_t lds64 = (size_t)3 * points_per_strand * 64 * sizeof(float4);
-  if (lds64 <= 160 * 1024) {
+  FH_REQUIRE(lds64 / 2 <= ctx->max_lds, "fh_sim_integrate: points_per_strand too large for this device's LDS");
+  if (lds64 <= ctx->max_lds) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&sim_integrate_kernel<64>),
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds64);
     sim_integrate_kernel<64><<<fh::div_ceil(strands_count, 64), 64, lds64, ctx->stream>>>(

@@ -1,0 +1,140 @@
+"""ctypes binding of libfhhip.so (include/fh_hair.h).
+
+Loading fails loudly when the library is missing: there is no CPU fallback
+for any pass.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libfhhip.so"
+
+vp = C.c_void_p
+u32 = C.c_uint32
+i32 = C.c_int
+
+
+class RenderCfg(C.Structure):
+    """struct fh_render_cfg"""
+    _fields_ = [
+        ("tileSize", u32), ("tileDepthBins", u32), ("slicesPerPixel", u32), ("avgFragmentsPerSlice", u32),
+        ("processorCount", u32), ("invalidTilesPerSegmentThreshold", u32), ("sortBuckets", u32),
+        ("sortBucketSize", u32), ("shadingPoints", u32), ("alphaQuadratic", u32), ("alphaMultiplier", C.c_float),
+        ("reserved", u32),
+    ]
+
+    @classmethod
+    def from_config(cls, rc) -> "RenderCfg":
+        return cls(rc.tileSize, rc.tileDepthBins, rc.slicesPerPixel, rc.avgFragmentsPerSlice, rc.processorCount,
+                   rc.invalidTilesPerSegmentThreshold, rc.sortBuckets, rc.sortBucketSize, rc.shadingPoints,
+                   1 if rc.alphaQuadratic else 0, rc.alphaMultiplier, 0)
+
+
+class Tex2D(C.Structure):
+    """struct fh_tex2d"""
+    _fields_ = [("data", vp), ("width", u32), ("height", u32)]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "fh_create": (i32, [i32, C.POINTER(vp)]),
+    "fh_destroy": (i32, [vp]),
+    "fh_set_stream": (i32, [vp, vp]),
+    "fh_get_stream": (vp, [vp]),
+    "fh_sync": (i32, [vp]),
+    "fh_last_error": (C.c_char_p, []),
+    "fh_abi_version": (i32, []),
+    "fh_alloc": (i32, [vp, C.c_uint64, C.POINTER(vp)]),
+    "fh_free": (i32, [vp, vp]),
+    "fh_upload": (i32, [vp, vp, vp, C.c_uint64]),
+    "fh_download": (i32, [vp, vp, vp, C.c_uint64]),
+    "fh_clear": (i32, [vp, vp, C.c_uint64]),
+    "fh_tile_count": (u32, [C.POINTER(RenderCfg), u32, u32]),
+    "fh_segments_buffer_entries": (u32, [C.POINTER(RenderCfg), u32, u32, C.c_float]),
+    "fh_slice_data_per_processor": (u32, [C.POINTER(RenderCfg)]),
+    "fh_grid_pre_sim": (i32, [vp, u32, vp, vp, vp, vp, u32]),
+    "fh_sim_integrate": (i32, [vp, u32, vp, vp, vp, vp, vp, vp, u32, vp, vp, vp, u32, u32]),
+    "fh_grid_post_sim": (i32, [vp, u32, vp, vp, vp, vp, vp, u32, u32]),
+    "fh_tiles_clear": (i32, [vp, C.POINTER(RenderCfg), u32, u32, vp, vp, vp]),
+    "fh_tiles_draw": (i32, [vp, C.POINTER(RenderCfg), u32, u32, vp, vp, vp, vp, vp, Tex2D, vp, u32, vp, vp,
+                            u32, u32]),
+    "fh_tiles_rebuild_slot_keys": (i32, [vp, C.POINTER(RenderCfg), u32, u32, vp, vp, u32, vp]),
+    "fh_tiles_sort_clear": (i32, [vp, C.POINTER(RenderCfg), vp, vp]),
+    "fh_tiles_sort": (i32, [vp, C.POINTER(RenderCfg), u32, u32, vp, vp, vp]),
+    "fh_fine_clear": (i32, [vp, u32, u32, vp]),
+    "fh_fine_raster": (i32, [vp, C.POINTER(RenderCfg), u32, u32, vp, vp, vp, vp, vp, u32, vp, vp, Tex2D, vp, vp,
+                             vp, vp, u32, u32]),
+    "fh_shading": (i32, [vp, C.POINTER(RenderCfg), vp, vp, vp, vp, vp, Tex2D, Tex2D, Tex2D, u32, u32]),
+    "fh_debug_raster_barycentric": (i32, [vp, vp, vp, vp, u32, u32, vp]),
+}
+
+_lib = None
+
+
+class FhError(RuntimeError):
+    pass
+
+
+def load(path: Path | None = None) -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise FhError(f"HIP library not built: {p} is missing (run __graft_entry__.build())")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = _lib.fh_last_error().decode(errors="replace") if _lib else "?"
+        raise FhError(f"{what} failed with code {rc}: {msg}")
+
+
+def ptr(t) -> vp:
+    """Device pointer of a torch tensor (or None)."""
+    return vp(t.data_ptr()) if t is not None else vp(0)
+
+
+def tex(t, width: int, height: int) -> Tex2D:
+    return Tex2D(t.data_ptr(), width, height)
+
+
+class Context:
+    """fh_ctx bound to the current torch stream of `device`."""
+
+    def __init__(self, device: int = 0, stream=None):
+        self.lib = load()
+        h = vp()
+        check(self.lib.fh_create(device, C.byref(h)), "fh_create")
+        self.handle = h
+        if stream is not None:
+            self.set_stream(stream)
+
+    def set_stream(self, stream) -> None:
+        s = stream.cuda_stream if hasattr(stream, "cuda_stream") else stream
+        check(self.lib.fh_set_stream(self.handle, vp(s)), "fh_set_stream")
+
+    def call(self, name: str, *args) -> None:
+        check(getattr(self.lib, name)(self.handle, *args), name)
+
+    def sync(self) -> None:
+        check(self.lib.fh_sync(self.handle), "fh_sync")
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.fh_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,349 @@
+"""Host mirror of the reference's pass API (src/passes/**, src/scene/hair,
+src/renderer.ts) over the HIP C-ABI. Class and method names follow the
+reference so a caller drives the chain the way Renderer does:
+
+    GridPreSimPass.cmdUpdateGridsBeforeSim(ctx, hairObject)
+    HairSimIntegrationPass.cmdSimulateHairPositions(ctx, hairObject)
+    GridPostSimPass.cmdUpdateGridsAfterSim(ctx, hairObject)
+    HairTilesPass.cmdClearBeforeRender(ctx) / cmdDrawHairToTiles(ctx, hairObject)
+    HairTileSortPass.cmdClearBeforeRender(ctx) / cmdSortHairTiles(ctx)
+    HairFinePass.cmdClearBeforeRender(ctx) / cmdRasterizeSlicesHair(ctx, hairObject)
+    HairShadingPass.cmdComputeShadingPoints(ctx, hairObject)
+
+Device memory is torch-allocated; every pass enqueues on the context's stream
+and returns, like recording into a GPUCommandEncoder.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import assets as A
+from . import layouts as L
+from . import native as N
+
+
+def _dev_bytes(arr: np.ndarray, device) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(arr).view(np.uint8).reshape(-1).copy()).to(device)
+
+
+def _dev(arr: np.ndarray, device) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(arr).copy()).to(device)
+
+
+class HairObject:
+    """src/scene/hair/hairObject.ts:29-155 — strand buffers and the tick-tock
+    position pair (swap after integration, :102-106)."""
+
+    def __init__(self, name: str, positions: np.ndarray, strands: int, points: int, device):
+        self.name = name
+        self.strandsCount = strands
+        self.pointsPerStrand = points
+        self.box = A.bounding_box(positions)
+        self.sphere = A.bounding_sphere(self.box)
+        self.initialPositions = _dev(positions.astype(np.float32), device)
+        self.positions = [self.initialPositions.clone(), self.initialPositions.clone()]
+        self.tangents = _dev(A.tangents(positions, strands, points), device)
+        self.segmentLengths = _dev(A.segment_lengths(positions, strands, points), device)
+        self.hairData = _dev(L.pack_hair_data(self.sphere[0], self.sphere[1], strands, points), device)
+        self.shading = None
+        self.current = 0
+        self.lodRenderPercent = 100.0
+
+    def allocShading(self, shading_points: int, device) -> None:
+        self.shading = _dev(A.initial_shading(self.strandsCount, shading_points), device)
+
+    def getRenderedStrandCount(self) -> int:
+        """hairObject.ts:52-57"""
+        pct = min(max(self.lodRenderPercent, 0), 100)
+        return min(max(math.ceil(self.strandsCount * pct / 100.0), 0), self.strandsCount)
+
+    @property
+    def positionsNow(self) -> torch.Tensor:
+        return self.positions[self.current]
+
+    @property
+    def positionsPrev(self) -> torch.Tensor:
+        return self.positions[1 - self.current]
+
+    def swapPositionBuffersAfterSimIntegration(self) -> None:
+        self.current = 1 - self.current
+
+    def resetSimulation(self) -> None:
+        """hairObject.ts:75-91"""
+        self.positions[0].copy_(self.initialPositions)
+        self.positions[1].copy_(self.initialPositions)
+
+
+class GridData:
+    """src/passes/simulation/grids/gridData.ts:14-107"""
+
+    def __init__(self, hair_box, dims: int, scale: float, device):
+        self.dims = dims
+        self.bounds = A.scale_box(hair_box, scale)
+        n = dims ** 3
+        self.densityVelocity = torch.zeros(n * 4, dtype=torch.int32, device=device)
+        self.densityGradAndWind = torch.zeros(n * 4, dtype=torch.float32, device=device)
+
+
+class SDFCollider:
+    """src/scene/sdfCollider/sdfCollider.ts:9-92"""
+
+    def __init__(self, dims: int, bounds, distances: np.ndarray, device):
+        self.dims = dims
+        self.bounds = bounds
+        self.texture = _dev(distances.astype(np.float32), device)
+
+
+@dataclass
+class PassCtx:
+    """src/passes/passCtx.ts:9-35 (fields used by the hair chain)."""
+    native: N.Context
+    cfg: N.RenderCfg
+    viewport: tuple
+    frameIdx: int
+    globalUniforms: torch.Tensor
+    simulationUniforms: torch.Tensor
+    physicsForcesGrid: GridData
+    sdfCollider: SDFCollider
+    depthTexture: N.Tex2D          # mesh depth (drawMeshesPass)
+    shadingDepthTexture: N.Tex2D   # mesh + hair depth (after hwHairPass)
+    aoTexture: N.Tex2D
+    shadowDepthTexture: N.Tex2D
+    hairTilesBuffer: torch.Tensor = None
+    hairTileSegmentsBuffer: torch.Tensor = None
+    hairSegmentCountPerTileBuffer: torch.Tensor = None
+    hairTileListBuffer: torch.Tensor = None
+    hairRasterizerResultsBuffer: torch.Tensor = None
+
+
+class GridPreSimPass:
+    NAME = "GridPreSimPass"
+
+    def cmdUpdateGridsBeforeSim(self, ctx: PassCtx, hair: HairObject) -> None:
+        g, s = ctx.physicsForcesGrid, ctx.sdfCollider
+        ctx.native.call("fh_grid_pre_sim", g.dims, N.ptr(ctx.simulationUniforms), N.ptr(g.densityVelocity),
+                        N.ptr(g.densityGradAndWind), N.ptr(s.texture), s.dims)
+
+
+class HairSimIntegrationPass:
+    NAME = "HairSimIntegrationPass"
+
+    def cmdSimulateHairPositions(self, ctx: PassCtx, hair: HairObject) -> None:
+        g, s = ctx.physicsForcesGrid, ctx.sdfCollider
+        ctx.native.call("fh_sim_integrate", g.dims, N.ptr(ctx.simulationUniforms), N.ptr(hair.hairData),
+                        N.ptr(hair.positionsPrev), N.ptr(hair.positionsNow), N.ptr(hair.segmentLengths),
+                        N.ptr(s.texture), s.dims, N.ptr(g.densityVelocity), N.ptr(g.densityGradAndWind),
+                        N.ptr(hair.initialPositions), hair.strandsCount, hair.pointsPerStrand)
+        hair.swapPositionBuffersAfterSimIntegration()  # hairSimIntegrationPass.ts:59
+
+
+class GridPostSimPass:
+    NAME = "GridPostSimPass"
+
+    def cmdUpdateGridsAfterSim(self, ctx: PassCtx, hair: HairObject) -> None:
+        g = ctx.physicsForcesGrid
+        ctx.native.call("fh_grid_post_sim", g.dims, N.ptr(ctx.simulationUniforms), N.ptr(hair.hairData),
+                        N.ptr(hair.positionsPrev), N.ptr(hair.positionsNow), N.ptr(g.densityVelocity),
+                        hair.strandsCount, hair.pointsPerStrand)
+
+
+class HairTilesPass:
+    NAME = "HairTilesPass"
+
+    def __init__(self):
+        self.hairTilesBuffer = None
+        self.hairTileSegmentsBuffer = None
+        self.segmentCountPerTileBuffer = None
+        self.slotTileBinBuffer = None
+        self.segmentsCapacity = 0
+
+    def onViewportResize(self, ctx_cfg: N.RenderCfg, rc: L.RenderConfig, width: int, height: int, device) -> None:
+        tiles = math.ceil(width / rc.tileSize) * math.ceil(height / rc.tileSize)
+        entries = L.segments_buffer_entries(rc, width, height)
+        self.hairTilesBuffer = torch.zeros(tiles * rc.tileDepthBins * 4, dtype=torch.int32, device=device)
+        # DrawnHairSegments: (1 + cnt) * 8 bytes; the element array at byte 4 holds cnt elements
+        self.hairTileSegmentsBuffer = torch.zeros(entries * 2, dtype=torch.int32, device=device)
+        self.segmentsCapacity = entries - 1
+        self.segmentCountPerTileBuffer = torch.zeros(tiles, dtype=torch.int32, device=device)
+        self.slotTileBinBuffer = torch.zeros(max(1, self.segmentsCapacity), dtype=torch.int32, device=device)
+
+    def cmdClearBeforeRender(self, ctx: PassCtx) -> None:
+        w, h = ctx.viewport
+        ctx.native.call("fh_tiles_clear", ctx.cfg, w, h, N.ptr(self.hairTilesBuffer),
+                        N.ptr(self.hairTileSegmentsBuffer), N.ptr(self.segmentCountPerTileBuffer))
+
+    def cmdDrawHairToTiles(self, ctx: PassCtx, hair: HairObject) -> None:
+        w, h = ctx.viewport
+        ctx.native.call("fh_tiles_draw", ctx.cfg, w, h, N.ptr(ctx.globalUniforms), N.ptr(hair.hairData),
+                        N.ptr(hair.positionsNow), N.ptr(hair.tangents), N.ptr(self.hairTilesBuffer),
+                        ctx.depthTexture, N.ptr(self.hairTileSegmentsBuffer), self.segmentsCapacity,
+                        N.ptr(self.segmentCountPerTileBuffer), N.ptr(self.slotTileBinBuffer),
+                        hair.getRenderedStrandCount(), hair.pointsPerStrand)
+
+
+class HairTileSortPass:
+    NAME = "HairTileSortPass"
+
+    def __init__(self, rc: L.RenderConfig, device):
+        self.bucketsDataBuffer = torch.zeros(rc.sortBuckets * 2, dtype=torch.int32, device=device)
+        self.tileListBuffer = None
+
+    def onViewportResize(self, rc: L.RenderConfig, width: int, height: int, device) -> None:
+        tiles = math.ceil(width / rc.tileSize) * math.ceil(height / rc.tileSize)
+        self.tileListBuffer = torch.zeros(4 + tiles, dtype=torch.int32, device=device)
+
+    def cmdClearBeforeRender(self, ctx: PassCtx) -> None:
+        ctx.native.call("fh_tiles_sort_clear", ctx.cfg, N.ptr(self.tileListBuffer), N.ptr(self.bucketsDataBuffer))
+
+    def cmdSortHairTiles(self, ctx: PassCtx) -> None:
+        w, h = ctx.viewport
+        ctx.native.call("fh_tiles_sort", ctx.cfg, w, h, N.ptr(ctx.hairSegmentCountPerTileBuffer),
+                        N.ptr(self.tileListBuffer), N.ptr(self.bucketsDataBuffer))
+
+
+class HairFinePass:
+    NAME = "HairFinePass"
+
+    def __init__(self, rc: L.RenderConfig, device):
+        per_proc = rc.avgFragmentsPerSlice * rc.slicesPerPixel * rc.tileSize * rc.tileSize
+        self.hairSlicesDataBuffer = torch.zeros(rc.processorCount * per_proc * 4, dtype=torch.int32, device=device)
+        self.hairRasterizerResultsBuffer = None
+
+    def onViewportResize(self, width: int, height: int, device) -> None:
+        self.hairRasterizerResultsBuffer = torch.zeros(width * height * 4, dtype=torch.float32, device=device)
+
+    def cmdClearBeforeRender(self, ctx: PassCtx) -> None:
+        w, h = ctx.viewport
+        ctx.native.call("fh_fine_clear", w, h, N.ptr(self.hairRasterizerResultsBuffer))
+
+    def cmdRasterizeSlicesHair(self, ctx: PassCtx, hair: HairObject, tiles: HairTilesPass) -> None:
+        w, h = ctx.viewport
+        ctx.native.call("fh_fine_raster", ctx.cfg, w, h, N.ptr(ctx.globalUniforms), N.ptr(hair.hairData),
+                        N.ptr(hair.positionsNow), N.ptr(ctx.hairTilesBuffer), N.ptr(ctx.hairTileSegmentsBuffer),
+                        tiles.segmentsCapacity, N.ptr(self.hairSlicesDataBuffer),
+                        N.ptr(self.hairRasterizerResultsBuffer), ctx.depthTexture, N.ptr(hair.shading),
+                        N.ptr(hair.tangents), N.ptr(ctx.hairTileListBuffer), N.ptr(tiles.slotTileBinBuffer),
+                        hair.strandsCount, hair.pointsPerStrand)
+
+    def image(self, width: int, height: int) -> torch.Tensor:
+        """Pixels of the rasterizer result as (H, W, 4), y up; the last pixel
+        lies past the end of the array and reads as 0 (hairRasterizerResultBuffer.ts:55-57)."""
+        flat = self.hairRasterizerResultsBuffer.view(-1, 4)[1:]
+        out = torch.zeros(width * height, 4, dtype=torch.float32, device=flat.device)
+        out[: width * height - 1] = flat[: width * height - 1]
+        return out.view(height, width, 4)
+
+
+class HairShadingPass:
+    NAME = "HairShadingPass"
+
+    def cmdComputeShadingPoints(self, ctx: PassCtx, hair: HairObject) -> None:
+        ctx.native.call("fh_shading", ctx.cfg, N.ptr(ctx.globalUniforms), N.ptr(hair.hairData),
+                        N.ptr(hair.positionsNow), N.ptr(hair.tangents), N.ptr(hair.shading),
+                        ctx.shadowDepthTexture, ctx.aoTexture, ctx.shadingDepthTexture,
+                        hair.strandsCount, hair.pointsPerStrand)
+
+
+class HairRenderer:
+    """The hair part of src/renderer.ts: beforeFirstFrame (:145-169),
+    cmdRender -> simulateHair (:212-240) -> cmdDrawScene (:242-283) ->
+    updateResourcesForNextFrame (:285-295), with the hardware-raster passes'
+    outputs (depth, AO, shadow map) provided as precomputed inputs."""
+
+    def __init__(self, wl: A.Workload, device="cuda", native: N.Context | None = None, shadow_size: int = 2048,
+                 inputs: dict | None = None):
+        self.wl = wl
+        self.device = torch.device(device)
+        self.native = native or N.Context(self.device.index or 0)
+        self.native.set_stream(torch.cuda.current_stream(self.device))
+        rc, sc = wl.render, wl.sim
+        self.cfg = N.RenderCfg.from_config(rc)
+        self.model = np.eye(4)
+        self.hair = HairObject("hair", wl.positions, wl.strands, wl.points, self.device)
+        self.hair.allocShading(rc.shadingPoints, self.device)
+        self.grid = GridData(self.hair.box, sc.gridDims, sc.gridScale, self.device)
+        self.sdf = SDFCollider(wl.sdf_dims, wl.sdf_bounds, wl.sdf, self.device)
+        self.frameIdx = 0
+        W, H = wl.width, wl.height
+        self.render_uniforms = L.pack_render_uniforms(
+            rc, sc, wl.camera, W, H, self.model, self.hair.sphere, [(np.zeros(3), 0.3)], wl.sdf_bounds,
+            self.grid.bounds)
+        self.globalUniforms = _dev_bytes(self.render_uniforms, self.device)
+        self.simulationUniforms = _dev_bytes(self.sim_uniforms(0), self.device)
+        if inputs is None:
+            inputs = make_inputs(wl, self.render_uniforms, shadow_size)
+        self.inputs = {k: _dev(v, self.device) for k, v in inputs.items()}
+        self.tilesPass = HairTilesPass()
+        self.tilesPass.onViewportResize(self.cfg, rc, W, H, self.device)
+        self.sortPass = HairTileSortPass(rc, self.device)
+        self.sortPass.onViewportResize(rc, W, H, self.device)
+        self.finePass = HairFinePass(rc, self.device)
+        self.finePass.onViewportResize(W, H, self.device)
+        self.gridPreSimPass = GridPreSimPass()
+        self.simPass = HairSimIntegrationPass()
+        self.gridPostSimPass = GridPostSimPass()
+        self.shadingPass = HairShadingPass()
+
+    def sim_uniforms(self, frame_idx: int) -> np.ndarray:
+        return L.pack_sim_uniforms(self.wl.sim, self.model, self.wl.sdf_bounds, self.grid.bounds, frame_idx)
+
+    def ctx(self) -> PassCtx:
+        W, H = self.wl.width, self.wl.height
+        i = self.inputs
+        return PassCtx(
+            native=self.native, cfg=self.cfg, viewport=(W, H), frameIdx=self.frameIdx,
+            globalUniforms=self.globalUniforms, simulationUniforms=self.simulationUniforms,
+            physicsForcesGrid=self.grid, sdfCollider=self.sdf,
+            depthTexture=N.tex(i["meshDepth"], W, H), shadingDepthTexture=N.tex(i["hairDepth"], W, H),
+            aoTexture=N.tex(i["ao"], i["ao"].shape[1], i["ao"].shape[0]),
+            shadowDepthTexture=N.tex(i["shadowMap"], i["shadowMap"].shape[1], i["shadowMap"].shape[0]),
+            hairTilesBuffer=self.tilesPass.hairTilesBuffer,
+            hairTileSegmentsBuffer=self.tilesPass.hairTileSegmentsBuffer,
+            hairSegmentCountPerTileBuffer=self.tilesPass.segmentCountPerTileBuffer,
+            hairTileListBuffer=self.sortPass.tileListBuffer,
+            hairRasterizerResultsBuffer=self.finePass.hairRasterizerResultsBuffer)
+
+    def upload_sim_uniforms(self) -> None:
+        u = torch.from_numpy(self.sim_uniforms(self.frameIdx).view(np.uint8).reshape(-1).copy())
+        self.simulationUniforms.copy_(u, non_blocking=False)
+
+    def beforeFirstFrame(self) -> None:
+        ctx = self.ctx()
+        self.gridPostSimPass.cmdUpdateGridsAfterSim(ctx, self.hair)
+        self.shadingPass.cmdComputeShadingPoints(ctx, self.hair)
+
+    def simulateHair(self, ctx: PassCtx) -> None:
+        self.gridPreSimPass.cmdUpdateGridsBeforeSim(ctx, self.hair)
+        self.simPass.cmdSimulateHairPositions(ctx, self.hair)
+        self.gridPostSimPass.cmdUpdateGridsAfterSim(ctx, self.hair)
+
+    def drawHair(self, ctx: PassCtx) -> None:
+        self.tilesPass.cmdClearBeforeRender(ctx)
+        self.finePass.cmdClearBeforeRender(ctx)
+        self.sortPass.cmdClearBeforeRender(ctx)
+        self.tilesPass.cmdDrawHairToTiles(ctx, self.hair)
+        self.sortPass.cmdSortHairTiles(ctx)
+        self.finePass.cmdRasterizeSlicesHair(ctx, self.hair, self.tilesPass)
+        self.shadingPass.cmdComputeShadingPoints(ctx, self.hair)
+
+    def frame(self, upload_uniforms: bool = True) -> None:
+        """One sim step + render of the hair chain (Renderer.cmdRender)."""
+        if upload_uniforms:
+            self.upload_sim_uniforms()
+        ctx = self.ctx()
+        self.simulateHair(ctx)
+        self.drawHair(ctx)
+        self.frameIdx += 1
+
+
+def make_inputs(wl: A.Workload, render_uniforms: np.ndarray, shadow_size: int = 2048) -> dict:
+    """Depth (mesh only, and mesh + hair), AO and shadow map for a workload."""
+    md = A.mesh_depth(wl, render_uniforms)
+    hd = A.hair_depth(wl, render_uniforms, wl.positions, md)
+    ao = np.ones((max(1, wl.height // 2), max(1, wl.width // 2)), dtype=np.float32)
+    sm = A.shadow_map(wl, render_uniforms, shadow_size)
+    return {"meshDepth": md, "hairDepth": hd, "ao": ao, "shadowMap": sm}

@@ -1,0 +1,123 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes wrapper of the CPU oracle (liboracle.so).
+
+Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the
+checker; the product library never loads it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+_lib = None
+
+
+class RenderCfg(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in (
+        "tileSize", "tileDepthBins", "slicesPerPixel", "avgFragmentsPerSlice", "processorCount",
+        "invalidTilesPerSegmentThreshold", "sortBuckets", "sortBucketSize", "shadingPoints", "alphaQuadratic")]
+    _fields_ = _fields_ + [("alphaMultiplier", C.c_float), ("reserved", C.c_uint32)]
+
+    @classmethod
+    def from_config(cls, rc):
+        return cls(rc.tileSize, rc.tileDepthBins, rc.slicesPerPixel, rc.avgFragmentsPerSlice, rc.processorCount,
+                   rc.invalidTilesPerSegmentThreshold, rc.sortBuckets, rc.sortBucketSize, rc.shadingPoints,
+                   1 if rc.alphaQuadratic else 0, rc.alphaMultiplier, 0)
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        _lib = C.CDLL(str(LIB))
+        _lib.or_sample_sdf.restype = C.c_float
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def sample_sdf(sdf, d, bmin, bmax, p) -> float:
+    f = lambda x: np.ascontiguousarray(x, dtype=np.float32)
+    return lib().or_sample_sdf(_p(f(sdf)), C.c_uint32(d), _p(f(bmin)), _p(f(bmax)), _p(f(p)))
+
+
+def grid_pre_sim(d, sim_u, dv, sdf, sdf_dims):
+    out = np.zeros(d ** 3 * 4, dtype=np.float32)
+    lib().or_grid_pre_sim(C.c_uint32(d), _p(sim_u), _p(dv), _p(out), _p(sdf), C.c_uint32(sdf_dims))
+    return out
+
+
+def sim_integrate(d, sim_u, prev, now, seglen, sdf, sdf_dims, dv, dgw, initial, S, P):
+    prev = np.ascontiguousarray(prev, dtype=np.float32).copy()
+    lib().or_sim_integrate(C.c_uint32(d), _p(sim_u), _p(prev), _p(now), _p(seglen), _p(sdf), C.c_uint32(sdf_dims),
+                           _p(dv), _p(dgw), _p(initial), C.c_uint32(S), C.c_uint32(P))
+    return prev
+
+
+def grid_post_sim(d, sim_u, prev, now, S, P):
+    dv = np.zeros(d ** 3 * 4, dtype=np.int32)
+    lib().or_grid_post_sim(C.c_uint32(d), _p(sim_u), _p(prev), _p(now), _p(dv), C.c_uint32(S), C.c_uint32(P))
+    return dv
+
+
+def debug_raster(render_u, positions, tangents, S, P, W, H):
+    out = np.zeros(W * H, dtype=np.uint32)
+    lib().or_debug_raster(_p(render_u), _p(positions), _p(tangents), C.c_uint32(S), C.c_uint32(P),
+                          C.c_uint32(W), C.c_uint32(H), _p(out))
+    return out
+
+
+def tiles_draw(rc, W, H, render_u, hair_data, positions, tangents, depth, capacity, S, P):
+    cfg = RenderCfg.from_config(rc)
+    tiles_n = -(-W // rc.tileSize) * -(-H // rc.tileSize)
+    tiles = np.zeros(tiles_n * rc.tileDepthBins * 4, dtype=np.uint32)
+    seg = np.zeros(2 * (capacity + 1), dtype=np.uint32)
+    counts = np.zeros(tiles_n, dtype=np.uint32)
+    keys = np.zeros(max(1, capacity), dtype=np.uint32)
+    lib().or_tiles_draw(C.byref(cfg), C.c_uint32(W), C.c_uint32(H), _p(render_u), _p(hair_data), _p(positions),
+                        _p(tangents), _p(tiles), _p(depth), _p(seg), C.c_uint32(capacity), _p(counts), _p(keys),
+                        C.c_uint32(S), C.c_uint32(P))
+    return tiles, seg, counts, keys
+
+
+def tiles_sort(rc, W, H, counts):
+    cfg = RenderCfg.from_config(rc)
+    tiles_n = -(-W // rc.tileSize) * -(-H // rc.tileSize)
+    tile_list = np.zeros(4 + tiles_n, dtype=np.uint32)
+    buckets = np.zeros(rc.sortBuckets * 2, dtype=np.uint32)
+    counts = np.ascontiguousarray(counts, dtype=np.uint32)
+    lib().or_tiles_sort(C.byref(cfg), C.c_uint32(W), C.c_uint32(H), _p(counts), _p(tile_list), _p(buckets))
+    return tile_list, buckets
+
+
+def fine_raster(rc, W, H, render_u, positions, tiles, seg, capacity, depth, shading, tangents, tile_list, S, P,
+                canonical=True, raster_in=None):
+    cfg = RenderCfg.from_config(rc)
+    raster = np.zeros(W * H * 4, dtype=np.float32) if raster_in is None else raster_in.copy()
+    lib().or_fine_raster(C.byref(cfg), C.c_uint32(W), C.c_uint32(H), _p(render_u), _p(positions), _p(tiles),
+                         _p(seg), C.c_uint32(capacity), _p(raster), _p(depth), _p(shading), _p(tangents),
+                         _p(tile_list), C.c_uint32(S), C.c_uint32(P), C.c_int(1 if canonical else 0))
+    return raster
+
+
+def shading(rc, render_u, positions, tangents, shadow, ao, depth, S, P):
+    out = np.zeros(S * rc.shadingPoints * 4, dtype=np.float32)
+    cfg = RenderCfg.from_config(rc)
+    lib().or_shading(C.byref(cfg), _p(render_u), _p(positions), _p(tangents), _p(out), _p(shadow),
+                     C.c_uint32(shadow.shape[1]), C.c_uint32(shadow.shape[0]), _p(ao), C.c_uint32(ao.shape[1]),
+                     C.c_uint32(ao.shape[0]), _p(depth), C.c_uint32(depth.shape[1]), C.c_uint32(depth.shape[0]),
+                     C.c_uint32(S), C.c_uint32(P))
+    return out
