@@ -40,12 +40,25 @@ __device__ __forceinline__ TileGeom tile_geom(f2 vp, uint32_t T) {
 }
 
 // ---------------------------------------------------------------------------
-// K4 — hairTilesPass.wgsl.ts:70-246. One thread per segment (strand-major,
-// segment fastest). For every tile of the segment's rectangle, only the pixels
-// within the segment's bounding box (+1 px) are tested: the others fail the
-// edge tests anyway. Appends are compacted per wave: one atomicAdd on
-// drawnSegmentsCount per wave per tile round (ballot + mbcnt), then the
-// (tile, bin) head exchange and depth maxima as in the reference.
+// K4 — hairTilesPass.wgsl.ts:70-246, as one wavefront per 64 segments.
+//
+// The reference gives each thread one segment and loops over every pixel of
+// every tile of its rectangle. With thread-private loops a wave runs as long
+// as its longest segment, and the costly per-pixel branch (interpolation,
+// depth test, view-space bin) executes whenever any lane takes it. Here the
+// wave shares the work instead:
+//   1. each lane projects its segment (swRasterizeHair.wgsl.ts:39-68) and
+//      finds its tile rectangle (:94-117);
+//   2. the wave flattens its (segment, tile) pairs and takes them 64 at a
+//      time; every pair owns the pixels of its tile (getTileBoundsPx) that
+//      can pass the edge tests (the segment's box, see seg_pixel_bbox);
+//   3. the wave enumerates those candidate pixels 64 at a time, runs the four
+//      edge tests and queues the pixels inside the quad in LDS;
+//   4. every 64 queued pixels the whole wave interpolates depth, runs the
+//      depth test and folds (min depth, max depth, min bin) into its pair
+//      with LDS atomics (min/max are exact, so the order does not matter);
+//   5. each pair with a hit is appended to the tile lists as in processTile
+//      :205-238, with one drawnSegmentsCount atomic per 64 pairs.
 // ---------------------------------------------------------------------------
 struct TilesArgs {
   const fh_render_uniforms* u;
@@ -58,20 +71,97 @@ struct TilesArgs {
   uint32_t* segCounter;          // DrawnHairSegments.drawnSegmentsCount
   fh_list_element* segData;      // DrawnHairSegments.data
   uint32_t limit;                // min(maxDrawnHairSegments, capacity)
-  uint32_t* countPerTile;
   uint32_t* slotKey;
   uint32_t T, B, threshold;
   uint32_t S, P;
 };
 
-__global__ __launch_bounds__(256) void tiles_draw_kernel(TilesArgs a) {
+// order-preserving map of f32 to u32 (for LDS min/max of non-NaN floats)
+__device__ __forceinline__ uint32_t f32_ord(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float ord_f32(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+// inclusive prefix sum over the 64 lanes
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const uint32_t lane = __lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = (uint32_t)__shfl_up((int)v, o);
+    if (lane >= (uint32_t)o) v += t;
+  }
+  return v;
+}
+
+// first index i in [0, 64) with ends[i] > x, for x < ends[63]
+__device__ __forceinline__ uint32_t search64(const uint32_t* ends, uint32_t x) {
+  uint32_t lo = 0;
+#pragma unroll
+  for (uint32_t step = 32; step > 0; step >>= 1)
+    if (ends[lo + step - 1] <= x) lo += step;
+  return lo;
+}
+
+constexpr uint32_t kTileQueue = 128;  // ring of queued inside pixels (power of two, >= 127)
+
+struct TileWaveLds {
+  // per segment (lane of the wave)
+  float v[64][8];          // v00, v01, v10, v11
+  float d[64][4];          // projected depths
+  int box[64][4];          // pixel box (seg_pixel_bbox)
+  uint32_t rect[64][3];    // first tile x, first tile y, tiles per row
+  uint32_t pairEnd[64];
+  // per pair of the current window
+  uint32_t owner[64];
+  int px0[64], py0[64];
+  uint32_t pw[64];
+  uint32_t candEnd[64];
+  uint32_t accMin[64], accMax[64], accBin[64];
+  uint32_t queue[kTileQueue];
+};
+
+// Step 4: the queued inside pixels [head, head + n), n <= 64, one per lane.
+__device__ __forceinline__ void tiles_shade_queue(const TilesArgs& a, TileWaveLds& L, uint32_t head,
+                                                  uint32_t n, const TileGeom& tg, f2 dbVS) {
+  const uint32_t lane = __lane_id();
+  if (lane < n) {
+    const uint32_t e = L.queue[(head + lane) & (kTileQueue - 1u)];
+    const uint32_t slot = e >> 16;
+    const uint32_t x = (uint32_t)L.px0[slot] + (e & 0xffu);
+    const uint32_t y = (uint32_t)L.py0[slot] + ((e >> 8) & 0xffu);
+    const uint32_t o = L.owner[slot];
+    Seg ps;
+    ps.v00 = mk2(L.v[o][0], L.v[o][1]); ps.v01 = mk2(L.v[o][2], L.v[o][3]);
+    ps.v10 = mk2(L.v[o][4], L.v[o][5]); ps.v11 = mk2(L.v[o][6], L.v[o][7]);
+    ps.depths = mk4(L.d[o][0], L.d[o][1], L.d[o][2], L.d[o][3]);
+    const f2 p = mk2((float)x, (float)y);
+    const f2 w = interp_quad(ps, p);
+    const float hdp = interp_f32(w, ps.depths);
+    const float db = tex_load(a.depth, a.depthW, a.depthH, (int32_t)x, (int32_t)(tg.vpH - y));
+    if (!(hdp > db)) {  // depth test with GL_LESS (:193-195)
+      const f3 vs = project(a.u->projMatrixInv, mk4(p.x, p.y, hdp, 1.0f));
+      const uint32_t bin = (a.B - 1u) - depth_bin(a.B, dbVS, vs.z);
+      // fminf/fmaxf in the reference skip a NaN depth; its bin still counts
+      if (hdp == hdp) {
+        atomicMin(&L.accMin[slot], f32_ord(hdp));
+        atomicMax(&L.accMax[slot], f32_ord(hdp));
+      }
+      atomicMin(&L.accBin[slot], bin);
+    }
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(64) void tiles_draw_kernel(TilesArgs a) {
+  __shared__ TileWaveLds L;
   const fh_render_uniforms* __restrict__ u = a.u;
+  const uint32_t lane = threadIdx.x;
   const uint32_t segsPerStrand = a.P - 1u;
-  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t gid = blockIdx.x * 64u + lane;
   const uint32_t nSeg = a.S * segsPerStrand;
-  const bool active = gid < nSeg;
-  const uint32_t strand = active ? gid / segsPerStrand : 0u;
-  const uint32_t seg = active ? gid - strand * segsPerStrand : 0u;
   const f2 vp = mk2(u->viewport[0], u->viewport[1]);
   const TileGeom tg = tile_geom(vp, a.T);
   const uint32_t T = a.T, B = a.B;
@@ -82,76 +172,142 @@ __global__ __launch_bounds__(256) void tiles_draw_kernel(TilesArgs a) {
                                           a.hd->boundingSphere[2], 1.0f));
   const f2 dbVS = mk2(bsC.z - a.hd->boundingSphere[3], bsC.z + a.hd->boundingSphere[3]);
 
-  Seg ps;
-  uint32_t nTiles = 0, tminx = 0, tminy = 0, tsx = 1;
-  int cx0 = 1, cy0 = 1, cx1 = 0, cy1 = 0;
-  if (active) {
-    ps = project_segment(u, a.pos, a.tan, a.P, strand, seg, vp, u->fiberRadius);
+  // 1. project
+  uint32_t nTiles = 0;
+  if (gid < nSeg) {
+    const uint32_t strand = gid / segsPerStrand;
+    const uint32_t seg = gid - strand * segsPerStrand;
+    const Seg ps = project_segment(u, a.pos, a.tan, a.P, strand, seg, vp, u->fiberRadius);
     // getRasterizedHairBounds — swRasterizeHair.wgsl.ts:111-123
     f2 bmax = vmax(vmax(ps.v00, ps.v01), vmax(ps.v10, ps.v11));
     f2 bmin = vmin(vmin(ps.v00, ps.v01), vmin(ps.v10, ps.v11));
     bmax = vmin(mk2(ceilf(bmax.x), ceilf(bmax.y)), vp);
     bmin = vmax(mk2(floorf(bmin.x), floorf(bmin.y)), mk2(0.0f, 0.0f));
-    tminx = f2u(bmin.x) / T; tminy = f2u(bmin.y) / T;
+    const uint32_t tminx = f2u(bmin.x) / T, tminy = f2u(bmin.y) / T;
     const uint32_t tmaxx = f2u(bmax.x) / T, tmaxy = f2u(bmax.y) / T;
-    tsx = (tmaxx - tminx) + 1u;
-    const uint32_t tsy = (tmaxy - tminy) + 1u;
+    const uint32_t tsx = (tmaxx - tminx) + 1u, tsy = (tmaxy - tminy) + 1u;
     // reject degenerate segments (:109-117); an inverted rectangle has no tile
     if (!(tsx * tsy > a.threshold) && tmaxx >= tminx && tmaxy >= tminy) nTiles = tsx * tsy;
+    int cx0, cy0, cx1, cy1;
     seg_pixel_bbox(ps, cx0, cy0, cx1, cy1);
+    L.v[lane][0] = ps.v00.x; L.v[lane][1] = ps.v00.y; L.v[lane][2] = ps.v01.x; L.v[lane][3] = ps.v01.y;
+    L.v[lane][4] = ps.v10.x; L.v[lane][5] = ps.v10.y; L.v[lane][6] = ps.v11.x; L.v[lane][7] = ps.v11.y;
+    L.d[lane][0] = ps.depths.x; L.d[lane][1] = ps.depths.y; L.d[lane][2] = ps.depths.z; L.d[lane][3] = ps.depths.w;
+    L.box[lane][0] = cx0; L.box[lane][1] = cy0; L.box[lane][2] = cx1; L.box[lane][3] = cy1;
+    L.rect[lane][0] = tminx; L.rect[lane][1] = tminy; L.rect[lane][2] = tsx;
   }
-  const uint32_t rounds = wave_max_u32(nTiles);
-  for (uint32_t k = 0; k < rounds; k++) {
-    bool hit = false;
-    uint32_t tx = 0, ty = 0, dbin = B;
-    float dmin = 999.0f, dmax = -999.0f;
-    if (k < nTiles) {
-      ty = tminy + k / tsx;
-      tx = tminx + k % tsx;
-      // processTile :139-238 with getTileBoundsPx (tileUtils.wgsl.ts:44-54)
+  const uint32_t pairEnd = wave_incl_scan(nTiles);
+  L.pairEnd[lane] = pairEnd;
+  const uint32_t nPairs = (uint32_t)__shfl((int)pairEnd, 63);
+  __syncthreads();
+
+  // 2. pairs, 64 at a time
+  for (uint32_t base = 0; base < nPairs; base += 64u) {
+    const uint32_t pidx = base + lane;
+    uint32_t cnt = 0, tx = 0, ty = 0;
+    if (pidx < nPairs) {
+      const uint32_t o = search64(L.pairEnd, pidx);
+      const uint32_t k = pidx - (o ? L.pairEnd[o - 1u] : 0u);
+      const uint32_t tsx = L.rect[o][2];
+      ty = L.rect[o][1] + k / tsx;
+      tx = L.rect[o][0] + k % tsx;
+      // getTileBoundsPx — tileUtils.wgsl.ts:44-54
       const uint32_t bx0 = clampu(tx * T, 0u, tg.vpW - 1u), by0 = clampu(ty * T, 0u, tg.vpH - 1u);
       const uint32_t bx1 = clampu((tx + 1u) * T, 0u, tg.vpW - 1u), by1 = clampu((ty + 1u) * T, 0u, tg.vpH - 1u);
-      const int xs = max((int)bx0, cx0), xe = min((int)bx1, cx1 + 1);
-      const int ys = max((int)by0, cy0), ye = min((int)by1, cy1 + 1);
-      for (int y = ys; y < ye; y++) {
-        for (int x = xs; x < xe; x++) {
-          const f2 p = mk2((float)x, (float)y);
-          if (!inside_quad(ps, p)) continue;
-          const f2 w = interp_quad(ps, p);
-          const float hdp = interp_f32(w, ps.depths);
-          const float db = tex_load(a.depth, a.depthW, a.depthH, x, (int32_t)(tg.vpH - (uint32_t)y));
-          if (hdp > db) continue;
-          const f3 vs = project(u->projMatrixInv, mk4(p.x, p.y, hdp, 1.0f));
-          const uint32_t bin = (B - 1u) - depth_bin(B, dbVS, vs.z);
-          dmin = fmn(dmin, hdp);
-          dmax = fmx(dmax, hdp);
-          dbin = min(dbin, bin);
-        }
+      const int xs = max((int)bx0, L.box[o][0]), xe = min((int)bx1, L.box[o][2] + 1);
+      const int ys = max((int)by0, L.box[o][1]), ye = min((int)by1, L.box[o][3] + 1);
+      if (xe > xs && ye > ys) cnt = (uint32_t)(xe - xs) * (uint32_t)(ye - ys);
+      L.owner[lane] = o;
+      L.px0[lane] = xs; L.py0[lane] = ys;
+      L.pw[lane] = cnt ? (uint32_t)(xe - xs) : 1u;
+    }
+    L.accMin[lane] = f32_ord(999.0f);
+    L.accMax[lane] = f32_ord(-999.0f);
+    L.accBin[lane] = B;
+    const uint32_t candEnd = wave_incl_scan(cnt);
+    L.candEnd[lane] = candEnd;
+    const uint32_t nCand = (uint32_t)__shfl((int)candEnd, 63);
+    __syncthreads();
+
+    // 3. candidate pixels, 64 at a time; 4. inside pixels, 64 at a time
+    uint32_t head = 0, tail = 0;
+    for (uint32_t c0 = 0; c0 < nCand; c0 += 64u) {
+      const uint32_t c = c0 + lane;
+      bool inside = false;
+      uint32_t entry = 0;
+      if (c < nCand) {
+        const uint32_t slot = search64(L.candEnd, c);
+        const uint32_t li = c - (slot ? L.candEnd[slot - 1u] : 0u);
+        const uint32_t w = L.pw[slot];
+        const uint32_t ly = li / w, lx = li - ly * w;
+        const uint32_t o = L.owner[slot];
+        Seg ps;
+        ps.v00 = mk2(L.v[o][0], L.v[o][1]); ps.v01 = mk2(L.v[o][2], L.v[o][3]);
+        ps.v10 = mk2(L.v[o][4], L.v[o][5]); ps.v11 = mk2(L.v[o][6], L.v[o][7]);
+        const f2 p = mk2((float)(L.px0[slot] + (int)lx), (float)(L.py0[slot] + (int)ly));
+        inside = inside_quad(ps, p);
+        entry = (slot << 16) | (ly << 8) | lx;
       }
+      const uint64_t m = __ballot(inside);
+      if (inside) L.queue[(tail + mask_prefix(m)) & (kTileQueue - 1u)] = entry;
+      tail += (uint32_t)__popcll(m);
+      __syncthreads();
+      if (tail - head >= 64u) {
+        tiles_shade_queue(a, L, head, 64u, tg, dbVS);
+        head += 64u;
+      }
+    }
+    if (tail > head) tiles_shade_queue(a, L, head, tail - head, tg, dbVS);
+
+    // 5. append the pairs with a hit (processTile :205-238)
+    bool hit = false;
+    float dmin = 999.0f, dmax = -999.0f;
+    uint32_t dbin = B;
+    if (pidx < nPairs) {
+      dmin = ord_f32(L.accMin[lane]);
+      dmax = ord_f32(L.accMax[lane]);
+      dbin = L.accBin[lane];
       hit = !(dmin > 1.0f);
     }
     const uint64_t mask = __ballot(hit);
-    if (mask == 0) continue;
-    uint32_t base = 0;
-    const uint32_t leader = (uint32_t)__builtin_ctzll(mask);
-    if (lane_id() == leader) base = atomicAdd(a.segCounter, (uint32_t)__popcll(mask));
-    base = (uint32_t)__shfl((int)base, (int)leader);
-    if (hit) {
-      const uint32_t slot = base + mask_prefix(mask);
-      if (slot < limit) {
-        const uint32_t tb = ty * tg.tilesX * B + tx * B + dbin;  // getHairTileDepthBinIdx
-        fh_tile_result* t = &a.tiles[tb];
-        atomicMax(&t->maxDepth, f2u(dmax * 4294967296.0f));
-        atomicMax(&t->minDepth, f2u((1.0f - dmin) * 4294967296.0f));
-        const uint32_t last = atomicExch(&t->tileSegmentPtr, slot + 1u);
-        atomicAdd(&t->padding0, 1u);
-        const uint32_t prevPtr = last == 0u ? kInvalidPtr : last - 1u;
-        *reinterpret_cast<uint2*>(&a.segData[slot]) = make_uint2((seg << 24) | strand, prevPtr);
-        if (a.slotKey) a.slotKey[slot] = tb;
-        atomicAdd(&a.countPerTile[ty * tg.tilesX + tx], 1u);
+    if (mask != 0) {
+      uint32_t sbase = 0;
+      const uint32_t leader = (uint32_t)__builtin_ctzll(mask);
+      if (lane == leader) sbase = atomicAdd(a.segCounter, (uint32_t)__popcll(mask));
+      sbase = (uint32_t)__shfl((int)sbase, (int)leader);
+      if (hit) {
+        const uint32_t slot = sbase + mask_prefix(mask);
+        if (slot < limit) {
+          const uint32_t o = L.owner[lane];
+          const uint32_t sgid = blockIdx.x * 64u + o;
+          const uint32_t strand = sgid / segsPerStrand;
+          const uint32_t seg = sgid - strand * segsPerStrand;
+          const uint32_t tb = ty * tg.tilesX * B + tx * B + dbin;  // getHairTileDepthBinIdx
+          fh_tile_result* t = &a.tiles[tb];
+          atomicMax(&t->maxDepth, f2u(dmax * 4294967296.0f));
+          atomicMax(&t->minDepth, f2u((1.0f - dmin) * 4294967296.0f));
+          const uint32_t last = atomicExch(&t->tileSegmentPtr, slot + 1u);
+          atomicAdd(&t->padding0, 1u);
+          const uint32_t prevPtr = last == 0u ? kInvalidPtr : last - 1u;
+          *reinterpret_cast<uint2*>(&a.segData[slot]) = make_uint2((seg << 24) | strand, prevPtr);
+          if (a.slotKey) a.slotKey[slot] = tb;
+        }
       }
     }
+    __syncthreads();
   }
+}
+
+// hairSegmentCountPerTile (processTile :236): the per-bin counts of each tile
+// summed, which is what the per-append increments of the reference add up to.
+__global__ __launch_bounds__(256) void tile_counts_kernel(const fh_tile_result* __restrict__ tiles,
+                                                          uint32_t nTiles, uint32_t B,
+                                                          uint32_t* __restrict__ countPerTile) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nTiles) return;
+  uint32_t n = 0;
+  for (uint32_t b = 0; b < B; b++) n += tiles[(size_t)t * B + b].padding0;
+  countPerTile[t] = n;
 }
 
 // ---------------------------------------------------------------------------
@@ -615,11 +771,15 @@ int fh_tiles_draw(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t
   a.segCounter = reinterpret_cast<uint32_t*>(tile_segments);
   a.segData = reinterpret_cast<fh_list_element*>(reinterpret_cast<char*>(tile_segments) + 4);
   a.limit = segments_capacity;  // the kernel also caps by uniforms->maxDrawnHairSegments
-  a.countPerTile = segment_count_per_tile; a.slotKey = slot_tile_bin;
+  a.slotKey = slot_tile_bin;
   a.T = cfg->tileSize; a.B = cfg->tileDepthBins; a.threshold = cfg->invalidTilesPerSegmentThreshold;
   a.S = rendered_strands; a.P = points_per_strand;
-  tiles_draw_kernel<<<fh::div_ceil((uint32_t)nSeg, 256), 256, 0, ctx->stream>>>(a);
-  return check_launch("tiles_draw_kernel");
+  tiles_draw_kernel<<<fh::div_ceil((uint32_t)nSeg, 64), 64, 0, ctx->stream>>>(a);
+  if ((rc = check_launch("tiles_draw_kernel"))) return rc;
+  const uint32_t nTiles = fh_tile_count(cfg, vp_w, vp_h);
+  tile_counts_kernel<<<fh::div_ceil(nTiles, 256), 256, 0, ctx->stream>>>(tiles, nTiles, cfg->tileDepthBins,
+                                                                       segment_count_per_tile);
+  return check_launch("tile_counts_kernel");
 }
 
 int fh_tiles_rebuild_slot_keys(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t vp_h,

@@ -102,6 +102,15 @@ __device__ __forceinline__ uint32_t depth_bin(uint32_t n, f2 td, float d) {
 // Pixel bounding box of a projected quad (getRasterizedHairBounds before the
 // scissor), as integers clamped to a safe range. A pixel can only pass all
 // four edge tests if it lies within [min - 1, max + 1] of this box.
+//
+// Why: the four edge functions E_i(p) are the orientations of the triangles
+// (p, a_i, b_i) over the closed loop v00 -> v10 -> v11 -> v01. For p outside
+// the convex hull of the vertices, all of them are seen from p within an open
+// half-turn, so the signed angles swept along the loop sum to zero; all E_i >= 0
+// then forces every E_i = 0, i.e. p and the four vertices collinear. Any quad
+// (convex, concave or self-intersecting) thus has its inside pixels in the
+// hull; the 1 px margin absorbs f32 rounding near the hull. An exactly
+// collinear quad gets the unbounded box (the caller tests every pixel).
 __device__ __forceinline__ void seg_pixel_bbox(const Seg& s, int& x0, int& y0, int& x1, int& y1) {
   const f2 mx = vmax(vmax(s.v00, s.v01), vmax(s.v10, s.v11));
   const f2 mn = vmin(vmin(s.v00, s.v01), vmin(s.v10, s.v11));
@@ -110,7 +119,16 @@ __device__ __forceinline__ void seg_pixel_bbox(const Seg& s, int& x0, int& y0, i
   y0 = (int)clampf(floorf(mn.y), lo, hi) - 1;
   x1 = (int)clampf(ceilf(mx.x), lo, hi) + 1;
   y1 = (int)clampf(ceilf(mx.y), lo, hi) + 1;
-  // any NaN vertex: the quad has no inside pixel, cull it entirely
+  const f2 e1 = s.v10 - s.v00, e2 = s.v11 - s.v00, e3 = s.v01 - s.v00;
+  const float k1 = e1.x * e2.y - e1.y * e2.x;
+  const float k2 = e1.x * e3.y - e1.y * e3.x;
+  const float k3 = e2.x * e3.y - e2.y * e3.x;
+  if (k1 == 0.0f && k2 == 0.0f && k3 == 0.0f) {
+    x0 = y0 = -(1 << 21);
+    x1 = y1 = 1 << 21;
+  }
+  // all vertices NaN: no pixel passes an edge test (a NaN vertex fails the
+  // two edges it belongs to, so partly-NaN quads are empty as well)
   if (!(mx.x == mx.x && mx.y == mx.y && mn.x == mn.x && mn.y == mn.y)) { x0 = 1; x1 = 0; }
 }
 

@@ -214,8 +214,15 @@ def test_fine_pass_image(gpu, oracle, tile, bins, slices):
     assert exp[4:].reshape(-1, 4)[:, 3].max() > 0.5
     assert got[0:1].view(np.uint32)[0] >= tl[0]  # the queue served every tile
     ge, ee = got[4:], exp[4:]
+    diff = np.abs(ge - ee).reshape(-1, 4)
+    bad = np.nonzero(diff.max(axis=1) > 1e-6)[0]
+    if bad.size:
+        k = bad[np.argmax(diff[bad].max(axis=1))]
+        print(f"pixels differing by >1e-6: {bad.size} of {diff.shape[0]}; worst pixel {k} "
+              f"(x={k % wl.width}, y={k // wl.width}) got {ge.reshape(-1, 4)[k]} exp {ee.reshape(-1, 4)[k]}")
     assert np.abs(to_u8(ge) - to_u8(ee)).max() <= 2
-    assert np.abs(ge - ee).max() <= 1e-4
+    # one f16 unit in the last place at 0.5 is 4.9e-4: a rounding flip of a packed fragment colour
+    assert diff.max() <= 5e-4
 
 
 def test_shading(gpu, oracle):
