@@ -52,8 +52,9 @@ __device__ __forceinline__ TileGeom tile_geom(f2 vp, uint32_t T) {
 //   2. the wave flattens its (segment, tile) pairs and takes them 64 at a
 //      time; every pair owns the pixels of its tile (getTileBoundsPx) that
 //      can pass the edge tests (the segment's box, see seg_pixel_bbox);
-//   3. the wave enumerates those candidate pixels 64 at a time, runs the four
-//      edge tests and queues the pixels inside the quad in LDS;
+//   3. row by row, a conservative x span (the quad's convex hull near the
+//      row) bounds the candidate pixels; the wave enumerates them 64 at a
+//      time, runs the four edge tests and queues the inside pixels in LDS;
 //   4. every 64 queued pixels the whole wave interpolates depth, runs the
 //      depth test and folds (min depth, max depth, min bin) into its pair
 //      with LDS atomics (min/max are exact, so the order does not matter);
@@ -111,17 +112,51 @@ struct TileWaveLds {
   // per segment (lane of the wave)
   float v[64][8];          // v00, v01, v10, v11
   float d[64][4];          // projected depths
+  float slope[64][6];      // dx/dy of the six vertex pairs (hull_strip_x)
   int box[64][4];          // pixel box (seg_pixel_bbox)
   uint32_t rect[64][3];    // first tile x, first tile y, tiles per row
   uint32_t pairEnd[64];
   // per pair of the current window
   uint32_t owner[64];
-  int px0[64], py0[64];
-  uint32_t pw[64];
-  uint32_t candEnd[64];
+  int px0[64], py0[64], px1[64];
+  uint32_t rowEnd[64];
   uint32_t accMin[64], accMax[64], accBin[64];
+  // per pixel row of the current row window
+  uint32_t itemSlot[64];
+  int itemX[64], itemY[64];
+  uint32_t candEnd[64];
   uint32_t queue[kTileQueue];
 };
+
+// The six vertex pairs of a quad, as indices into v00, v01, v10, v11.
+__device__ __forceinline__ void hull_pair(int k, int& i, int& j) {
+  i = k < 3 ? 0 : (k < 5 ? 1 : 2);
+  j = k < 3 ? k + 1 : (k < 5 ? k - 1 : 3);
+}
+
+// x extent of the convex hull of the quad within the horizontal strip
+// [ya, yb]: the hull vertices in the strip and the crossings of the strip's
+// edges with the six vertex pairs (the hull's edges are among them and the
+// other pairs are chords inside it). Empty when xlo > xhi.
+__device__ __forceinline__ void hull_strip_x(const float* v, const float* slope, float ya, float yb,
+                                             float& xlo, float& xhi) {
+  xlo = 3.0e38f;
+  xhi = -3.0e38f;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const float x = v[2 * i], y = v[2 * i + 1];
+    if (y >= ya && y <= yb) { xlo = fminf(xlo, x); xhi = fmaxf(xhi, x); }
+  }
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    int i, j;
+    hull_pair(k, i, j);
+    const float py = v[2 * i + 1], qy = v[2 * j + 1], px = v[2 * i];
+    const float lo = fminf(py, qy), hi = fmaxf(py, qy);
+    if (ya > lo && ya < hi) { const float x = px + (ya - py) * slope[k]; xlo = fminf(xlo, x); xhi = fmaxf(xhi, x); }
+    if (yb > lo && yb < hi) { const float x = px + (yb - py) * slope[k]; xlo = fminf(xlo, x); xhi = fmaxf(xhi, x); }
+  }
+}
 
 // Step 4: the queued inside pixels [head, head + n), n <= 64, one per lane.
 __device__ __forceinline__ void tiles_shade_queue(const TilesArgs& a, TileWaveLds& L, uint32_t head,
@@ -190,8 +225,15 @@ __global__ __launch_bounds__(64) void tiles_draw_kernel(TilesArgs a) {
     if (!(tsx * tsy > a.threshold) && tmaxx >= tminx && tmaxy >= tminy) nTiles = tsx * tsy;
     int cx0, cy0, cx1, cy1;
     seg_pixel_bbox(ps, cx0, cy0, cx1, cy1);
-    L.v[lane][0] = ps.v00.x; L.v[lane][1] = ps.v00.y; L.v[lane][2] = ps.v01.x; L.v[lane][3] = ps.v01.y;
-    L.v[lane][4] = ps.v10.x; L.v[lane][5] = ps.v10.y; L.v[lane][6] = ps.v11.x; L.v[lane][7] = ps.v11.y;
+    const float vv[8] = {ps.v00.x, ps.v00.y, ps.v01.x, ps.v01.y, ps.v10.x, ps.v10.y, ps.v11.x, ps.v11.y};
+#pragma unroll
+    for (int i = 0; i < 8; i++) L.v[lane][i] = vv[i];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      int i, j;
+      hull_pair(k, i, j);
+      L.slope[lane][k] = (vv[2 * j] - vv[2 * i]) / (vv[2 * j + 1] - vv[2 * i + 1]);
+    }
     L.d[lane][0] = ps.depths.x; L.d[lane][1] = ps.depths.y; L.d[lane][2] = ps.depths.z; L.d[lane][3] = ps.depths.w;
     L.box[lane][0] = cx0; L.box[lane][1] = cy0; L.box[lane][2] = cx1; L.box[lane][3] = cy1;
     L.rect[lane][0] = tminx; L.rect[lane][1] = tminy; L.rect[lane][2] = tsx;
@@ -204,7 +246,7 @@ __global__ __launch_bounds__(64) void tiles_draw_kernel(TilesArgs a) {
   // 2. pairs, 64 at a time
   for (uint32_t base = 0; base < nPairs; base += 64u) {
     const uint32_t pidx = base + lane;
-    uint32_t cnt = 0, tx = 0, ty = 0;
+    uint32_t rows = 0, tx = 0, ty = 0;
     if (pidx < nPairs) {
       const uint32_t o = search64(L.pairEnd, pidx);
       const uint32_t k = pidx - (o ? L.pairEnd[o - 1u] : 0u);
@@ -216,45 +258,79 @@ __global__ __launch_bounds__(64) void tiles_draw_kernel(TilesArgs a) {
       const uint32_t bx1 = clampu((tx + 1u) * T, 0u, tg.vpW - 1u), by1 = clampu((ty + 1u) * T, 0u, tg.vpH - 1u);
       const int xs = max((int)bx0, L.box[o][0]), xe = min((int)bx1, L.box[o][2] + 1);
       const int ys = max((int)by0, L.box[o][1]), ye = min((int)by1, L.box[o][3] + 1);
-      if (xe > xs && ye > ys) cnt = (uint32_t)(xe - xs) * (uint32_t)(ye - ys);
+      if (xe > xs && ye > ys) rows = (uint32_t)(ye - ys);
       L.owner[lane] = o;
-      L.px0[lane] = xs; L.py0[lane] = ys;
-      L.pw[lane] = cnt ? (uint32_t)(xe - xs) : 1u;
+      L.px0[lane] = xs; L.py0[lane] = ys; L.px1[lane] = xe;
     }
     L.accMin[lane] = f32_ord(999.0f);
     L.accMax[lane] = f32_ord(-999.0f);
     L.accBin[lane] = B;
-    const uint32_t candEnd = wave_incl_scan(cnt);
-    L.candEnd[lane] = candEnd;
-    const uint32_t nCand = (uint32_t)__shfl((int)candEnd, 63);
+    const uint32_t rowEnd = wave_incl_scan(rows);
+    L.rowEnd[lane] = rowEnd;
+    const uint32_t nRows = (uint32_t)__shfl((int)rowEnd, 63);
     __syncthreads();
 
-    // 3. candidate pixels, 64 at a time; 4. inside pixels, 64 at a time
     uint32_t head = 0, tail = 0;
-    for (uint32_t c0 = 0; c0 < nCand; c0 += 64u) {
-      const uint32_t c = c0 + lane;
-      bool inside = false;
-      uint32_t entry = 0;
-      if (c < nCand) {
-        const uint32_t slot = search64(L.candEnd, c);
-        const uint32_t li = c - (slot ? L.candEnd[slot - 1u] : 0u);
-        const uint32_t w = L.pw[slot];
-        const uint32_t ly = li / w, lx = li - ly * w;
+    // 3a. pixel rows of the pairs, 64 at a time: the conservative x span of
+    // each row (hull within 1 px of the row, widened by 1 px)
+    for (uint32_t r0 = 0; r0 < nRows; r0 += 64u) {
+      const uint32_t ridx = r0 + lane;
+      uint32_t cnt = 0;
+      if (ridx < nRows) {
+        const uint32_t slot = search64(L.rowEnd, ridx);
+        const int y = L.py0[slot] + (int)(ridx - (slot ? L.rowEnd[slot - 1u] : 0u));
         const uint32_t o = L.owner[slot];
-        Seg ps;
-        ps.v00 = mk2(L.v[o][0], L.v[o][1]); ps.v01 = mk2(L.v[o][2], L.v[o][3]);
-        ps.v10 = mk2(L.v[o][4], L.v[o][5]); ps.v11 = mk2(L.v[o][6], L.v[o][7]);
-        const f2 p = mk2((float)(L.px0[slot] + (int)lx), (float)(L.py0[slot] + (int)ly));
-        inside = inside_quad(ps, p);
-        entry = (slot << 16) | (ly << 8) | lx;
+        int xa = L.px0[slot], xb = L.px1[slot];
+        const float* v = L.v[o];
+        const float amax = fmaxf(fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3]))),
+                                 fmaxf(fmaxf(fabsf(v[4]), fabsf(v[5])), fmaxf(fabsf(v[6]), fabsf(v[7]))));
+        // collinear quads test whole tiles; far-off vertices skip the span
+        const bool span = L.box[o][0] > -(1 << 20) && amax < 65536.0f;
+        if (span) {
+          float xlo, xhi;
+          hull_strip_x(v, L.slope[o], (float)y - 1.0f, (float)y + 1.0f, xlo, xhi);
+          if (xlo <= xhi) {
+            xa = max(xa, (int)ceilf(xlo - 1.0f));
+            xb = min(xb, (int)floorf(xhi + 1.0f) + 1);
+          } else {
+            xb = xa;
+          }
+        }
+        if (xb > xa) cnt = (uint32_t)(xb - xa);
+        L.itemSlot[lane] = slot;
+        L.itemX[lane] = xa;
+        L.itemY[lane] = y;
       }
-      const uint64_t m = __ballot(inside);
-      if (inside) L.queue[(tail + mask_prefix(m)) & (kTileQueue - 1u)] = entry;
-      tail += (uint32_t)__popcll(m);
+      const uint32_t candEnd = wave_incl_scan(cnt);
+      L.candEnd[lane] = candEnd;
+      const uint32_t nCand = (uint32_t)__shfl((int)candEnd, 63);
       __syncthreads();
-      if (tail - head >= 64u) {
-        tiles_shade_queue(a, L, head, 64u, tg, dbVS);
-        head += 64u;
+
+      // 3b. candidate pixels, 64 at a time; 4. inside pixels, 64 at a time
+      for (uint32_t c0 = 0; c0 < nCand; c0 += 64u) {
+        const uint32_t c = c0 + lane;
+        bool inside = false;
+        uint32_t entry = 0;
+        if (c < nCand) {
+          const uint32_t it = search64(L.candEnd, c);
+          const int x = L.itemX[it] + (int)(c - (it ? L.candEnd[it - 1u] : 0u));
+          const int y = L.itemY[it];
+          const uint32_t slot = L.itemSlot[it];
+          const uint32_t o = L.owner[slot];
+          Seg ps;
+          ps.v00 = mk2(L.v[o][0], L.v[o][1]); ps.v01 = mk2(L.v[o][2], L.v[o][3]);
+          ps.v10 = mk2(L.v[o][4], L.v[o][5]); ps.v11 = mk2(L.v[o][6], L.v[o][7]);
+          inside = inside_quad(ps, mk2((float)x, (float)y));
+          entry = (slot << 16) | ((uint32_t)(y - L.py0[slot]) << 8) | (uint32_t)(x - L.px0[slot]);
+        }
+        const uint64_t m = __ballot(inside);
+        if (inside) L.queue[(tail + mask_prefix(m)) & (kTileQueue - 1u)] = entry;
+        tail += (uint32_t)__popcll(m);
+        __syncthreads();
+        if (tail - head >= 64u) {
+          tiles_shade_queue(a, L, head, 64u, tg, dbVS);
+          head += 64u;
+        }
       }
     }
     if (tail > head) tiles_shade_queue(a, L, head, tail - head, tg, dbVS);
