@@ -541,22 +541,82 @@ __global__ __launch_bounds__(256) void bin_scatter_kernel(const fh_render_unifor
     const uint32_t key = slotKey[s];
     if (key >= nTB) continue;  // untagged slot: not linked into any list
     const uint32_t pos = atomicSub(&ends[key], 1u) - 1u;
-    if (pos < capacity) binSeg[pos] = data[s].strandAndSegmentIdxs;
+    if (pos < capacity) {
+      const uint32_t packed = data[s].strandAndSegmentIdxs;  // segment << 24 | strand
+      binSeg[pos] = ((packed & 0x00ffffffu) << 8) | (packed >> 24);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Each gathered range sorted by blend key (strand << 8 | segment), largest
+// first. The fine pass pushes fragments at the head of per-pixel lists in
+// this order, so every list reads back in ascending key order: a fixed blend
+// order for equal-depth-slice fragments, whatever order the tile pass's
+// atomics appended the segments in. Keys are unique within a range (a
+// segment enters each tile once, in one bin).
+//   n <= kSortLds: bitonic network in LDS;
+//   larger: the same network over a zero-padded copy in global scratch at
+//   [2 * start, 2 * start + 2^ceil(log2 n)), disjoint between ranges.
+// Padding with 0 is safe: it sorts last, and a real key 0 ties with it.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kSortLds = 4096;
+
+__device__ __forceinline__ void bitonic_desc(uint32_t* k, uint32_t n2, uint32_t tid, uint32_t nthr) {
+  for (uint32_t size = 2; size <= n2; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (uint32_t t = tid; t < (n2 >> 1); t += nthr) {
+        const uint32_t i = 2u * t - (t & (stride - 1u));
+        const uint32_t j = i + stride;
+        const bool desc = (i & size) == 0u;
+        const uint32_t a = k[i], b = k[j];
+        if (desc ? (a < b) : (a > b)) { k[i] = b; k[j] = a; }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void bin_sort_kernel(const fh_tile_result* __restrict__ tiles,
+                                                       const uint32_t* __restrict__ starts, uint32_t nTB,
+                                                       uint32_t capacity, uint32_t* __restrict__ keys,
+                                                       uint32_t* __restrict__ big) {
+  __shared__ uint32_t sk[kSortLds];
+  for (uint32_t tb = blockIdx.x; tb < nTB; tb += gridDim.x) {
+    const uint32_t start = min(starts[tb], capacity);
+    const uint32_t n = min(tiles[tb].padding0, capacity - start);
+    if (n <= 1u) continue;
+    uint32_t n2 = 2;
+    while (n2 < n) n2 <<= 1;
+    uint32_t* k = sk;
+    if (n2 > kSortLds) k = big + 2ull * start;
+    for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x) k[i] = i < n ? keys[start + i] : 0u;
+    __syncthreads();
+    bitonic_desc(k, n2, threadIdx.x, blockDim.x);
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) keys[start + i] = k[i];
+    __syncthreads();
   }
 }
 
 // ---------------------------------------------------------------------------
 // K7 — hairFinePass.wgsl.ts:126-335 with processHairSegment.wgsl.ts:5-70 and
 // reduceHairSlices.wgsl.ts:9-78. Persistent workgroups ("processors") pop
-// tiles from the sorted tile list; one thread per pixel. Per depth bin:
-//  * up to one segment per thread is projected into LDS, then every pixel
-//    tests the whole batch (2 barriers per batch instead of 2 per segment);
-//  * a covered pixel's fragment (f16 colour, coverage alpha, key) goes to
-//    the processor's fragment pool, LDS first and the per-processor global
-//    region beyond it, and is linked into its slice list in ascending
-//    (strand, segment) order, which makes the blend order independent of
-//    the atomic order in which the tile pass appended the segments;
-//  * slices are then blended front to back into the pixel.
+// tiles from the sorted tile list; one pixel per thread. Per depth bin, the
+// bin's segments arrive sorted by blend key (largest first) in batches:
+//  * one thread per segment projects it into LDS (64 B record: quad, depths,
+//    pixel box relative to the tile, nearest depth);
+//  * each pixel walks the batch in order and runs the cheap tests (box,
+//    a conservative depth pre-test against the mesh depth, four edge tests);
+//    a pixel that passes takes a pool slot, links it at the head of its own
+//    list and queues (slot, segment, pixel) in its wave's LDS ring;
+//  * every 64 queued fragments the wave computes them together: segment-space
+//    coordinates, depth test, slice-memory budget (perProc per bin, as
+//    _hasMoreSliceDataSlots), coverage alpha, shading and slice index;
+//  * the pixel then blends its list once per slice, front to back; within a
+//    slice the list order is ascending blend key.
+// Pool slots live in LDS first, then in the processor's region of the
+// reference's slice buffer (perProc entries); a fragment rejected after it
+// was queued keeps its slot with a skip tag.
 // ---------------------------------------------------------------------------
 struct FineArgs {
   const fh_render_uniforms* u;
@@ -564,7 +624,7 @@ struct FineArgs {
   const float4* tan;
   const fh_tile_result* tiles;
   const uint32_t* binStart;  // first element of each (tile, bin) range
-  const uint32_t* binSeg;    // packed (segment << 24 | strand)
+  const uint32_t* binKey;    // blend keys (strand << 8 | segment), sorted largest first per range
   uint4* slicePool;          // processorCount * perProc entries
   uint32_t* queue;           // HairRasterResult.tileQueueAtomicIdx
   float4* fb;                // HairRasterResult.data
@@ -574,7 +634,7 @@ struct FineArgs {
   const uint32_t* tileList;
   uint32_t S, P;
   uint32_t B, NS, PS;
-  uint32_t perProc, ldsPool, batch;
+  uint32_t perProc, ldsPool, physCap, batch;
   uint32_t capacity;  // elements of the gathered segment array
   uint32_t alphaQuadratic;
   float alphaMultiplier;
@@ -583,24 +643,105 @@ struct FineArgs {
 struct SegRec {
   float v00x, v00y, v01x, v01y, v10x, v10y, v11x, v11y;
   float d0, d1, d2, d3;
-  uint32_t packed;
+  uint32_t key;
   int16_t bx0, by0, bx1, by1;
-  uint32_t pad;
+  float dnear;  // smallest projected depth of the quad, -inf if any is NaN
 };
 static_assert(sizeof(SegRec) == 64, "SegRec must be 64 bytes");
+
+constexpr uint32_t kFineRing = 128;      // per-wave ring of queued fragments
+constexpr uint32_t kSkipTag = 0xffffffffu;
+
+template <int TILE>
+struct FineLds {
+  static constexpr uint32_t NPX = TILE * TILE;
+  static constexpr uint32_t NW = NPX / 64;
+  SegRec* recs;
+  uint32_t* heads;   // NPX
+  float* db;         // NPX
+  uint32_t* ring;    // NW * kFineRing
+  uint4* pool;       // ldsPool
+};
+
+// The queued fragments [head, head + n) of wave `w`, n <= 64, one per lane.
+template <int TILE>
+__device__ __forceinline__ void fine_shade_queue(const FineArgs& a, const FineLds<TILE>& L, uint4* gpool,
+                                                 uint32_t w, uint32_t head, uint32_t n, uint32_t bx0,
+                                                 uint32_t by0, f2 td, float invSegs, uint32_t* sliceOff) {
+  const uint32_t lane = __lane_id();
+  bool accept = false;
+  uint32_t slot = 0;
+  f2 wq = mk2(0.0f, 0.0f);
+  float hdp = 0.0f;
+  const SegRec* r = nullptr;
+  if (lane < n) {
+    const uint32_t e = L.ring[w * kFineRing + ((head + lane) & (kFineRing - 1u))];
+    slot = e >> 14;
+    r = &L.recs[(e >> 6) & 0xffu];
+    const uint32_t pix = w * 64u + (e & 63u);
+    const f2 p = mk2((float)(bx0 + pix % TILE), (float)(by0 + pix / TILE));
+    Seg sg;
+    sg.v00 = mk2(r->v00x, r->v00y); sg.v01 = mk2(r->v01x, r->v01y);
+    sg.v10 = mk2(r->v10x, r->v10y); sg.v11 = mk2(r->v11x, r->v11y);
+    sg.depths = mk4(r->d0, r->d1, r->d2, r->d3);
+    wq = interp_quad(sg, p);
+    hdp = interp_f32(wq, sg.depths);
+    accept = !(hdp >= L.db[pix]);  // depth test with GL_LESS (:31-34)
+  }
+  // _hasMoreSliceDataSlots (hairFinePass.wgsl.ts:252-264): perProc per bin
+  const uint64_t m = __ballot(accept);
+  uint32_t base = 0;
+  if (m) {
+    const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+    if (lane == leader) base = atomicAdd(sliceOff, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, (int)leader);
+  }
+  if (lane < n) {
+    if (accept && base + mask_prefix(m) >= a.perProc) accept = false;
+    uint32_t rg = 0, ba = 0, tag = kSkipTag;
+    if (accept) {
+      // getAlphaCoverage (processHairSegment.wgsl.ts:61-70)
+      float alpha = 1.0f - fabsf(wq.x * 2.0f - 1.0f);
+      if (a.alphaQuadratic) alpha = sqrtf(alpha);
+      alpha = satf(alpha * a.alphaMultiplier);
+      const uint32_t strand = r->key >> 8, seg = r->key & 0xffu;
+      const float tFull = ((float)seg + wq.y) / invSegs;
+      // _sampleShading — hairShadingBuffer.ts:29-38
+      const float sa = tFull * ((float)a.PS - 1.0f);
+      const uint32_t au = f2u(sa);
+      const uint32_t i0 = clampu(au, 0u, a.PS - 1u), i1 = clampu(au + 1u, 0u, a.PS - 1u);
+      const float fr = fractf(sa);
+      const float4 c0 = a.shading[(size_t)strand * a.PS + i0];
+      const float4 c1 = a.shading[(size_t)strand * a.PS + i1];
+      const float cr = mixf(c0.x, c1.x, fr), cg = mixf(c0.y, c1.y, fr);
+      const float cb = mixf(c0.z, c1.z, fr), ca = mixf(c0.w, c1.w, fr) * alpha;
+      rg = pack_half2(cr, cg);
+      ba = pack_half2(cb, ca);
+      tag = depth_bin(a.NS, td, hdp);  // getSliceIdx (:72-75)
+    }
+    uint4* e = slot < a.ldsPool ? &L.pool[slot] : &gpool[slot - a.ldsPool];
+    e->x = rg; e->y = ba; e->w = tag;
+  }
+}
 
 template <int TILE>
 __global__ __launch_bounds__(TILE* TILE) void fine_raster_kernel(FineArgs a) {
   constexpr uint32_t NPX = TILE * TILE;
+  constexpr uint32_t NW = NPX / 64;
   extern __shared__ __align__(16) unsigned char smem[];
-  SegRec* recs = reinterpret_cast<SegRec*>(smem);
-  uint32_t* heads = reinterpret_cast<uint32_t*>(smem + (size_t)a.batch * sizeof(SegRec));
-  uint4* pool = reinterpret_cast<uint4*>(heads + (size_t)a.NS * NPX);
+  FineLds<TILE> L;
+  L.recs = reinterpret_cast<SegRec*>(smem);
+  L.heads = reinterpret_cast<uint32_t*>(smem + (size_t)a.batch * sizeof(SegRec));
+  L.db = reinterpret_cast<float*>(L.heads + NPX);
+  L.ring = reinterpret_cast<uint32_t*>(L.db + NPX);
+  L.pool = reinterpret_cast<uint4*>(L.ring + NW * kFineRing);
   __shared__ uint32_t s_tile;
-  __shared__ uint32_t s_sliceOff;
+  __shared__ uint32_t s_sliceOff;  // accepted fragments of the bin
+  __shared__ uint32_t s_phys;      // pool slots taken in the bin
 
   const fh_render_uniforms* __restrict__ u = a.u;
   const uint32_t tid = threadIdx.x;
+  const uint32_t lane = __lane_id(), w = tid >> 6;
   const uint32_t lx = tid % TILE, ly = tid / TILE;
   const f2 vpf = mk2(u->viewport[0], u->viewport[1]);
   const TileGeom tg = tile_geom(vpf, TILE);
@@ -613,7 +754,7 @@ __global__ __launch_bounds__(TILE* TILE) void fine_raster_kernel(FineArgs a) {
   const float invSegs = (float)(a.P - 1u);
   uint4* gpool = a.slicePool + (size_t)blockIdx.x * a.perProc;
 
-  for (uint32_t s = 0; s < a.NS; s++) heads[s * NPX + tid] = kInvalidPtr;
+  L.heads[tid] = kInvalidPtr;
 
   while (true) {
     __syncthreads();
@@ -629,10 +770,10 @@ __global__ __launch_bounds__(TILE* TILE) void fine_raster_kernel(FineArgs a) {
     const uint32_t bx0 = clampu(tx * TILE, 0u, tg.vpW - 1u), by0 = clampu(ty * TILE, 0u, tg.vpH - 1u);
     const uint32_t px = bx0 + lx, py = by0 + ly;
     const f2 p = mk2((float)px, (float)py);
-    const int ipx = (int)px, ipy = (int)py;
     const bool readable = px < tg.vpW && py < tg.vpH && (tg.vpW * py + px) < fbLen;
     const uint32_t fbIdx = tg.vpW * py + px;
-    const int32_t depthY = f2i(vpf.y - p.y);
+    const float mydb = tex_load(a.depth, a.depthW, a.depthH, (int)px, f2i(vpf.y - p.y));
+    L.db[tid] = mydb;
     bool done = false;
     f4 fc = mk4(0.0f, 0.0f, 0.0f, 0.0f);
     if (readable) { const float4 v = a.fb[fbIdx]; fc = mk4(v.x, v.y, v.z, v.w); }
@@ -640,117 +781,110 @@ __global__ __launch_bounds__(TILE* TILE) void fine_raster_kernel(FineArgs a) {
     for (uint32_t bin = 0; bin < a.B && tileIdx < tileCount; bin++) {
       const uint32_t tb = tileIdx * a.B + bin;
       const fh_tile_result tr = a.tiles[tb];
-      if (tr.maxDepth != 0u) {  // _getTileDepth == 0 means an empty bin (:215)
-        const f2 td = mk2((float)(0xffffffffu - tr.minDepth) / 4294967296.0f,
-                          (float)tr.maxDepth / 4294967296.0f);
-        const uint32_t start = min(a.binStart[tb], a.capacity);
-        const uint32_t n = min(tr.padding0, a.capacity - start);
-        if (tid == 0) s_sliceOff = 0;
-        for (uint32_t b0 = 0; b0 < n; b0 += a.batch) {
-          const uint32_t nb = min(a.batch, n - b0);
-          __syncthreads();
-          if (tid < nb) {
-            const uint32_t packed = a.binSeg[start + b0 + tid];
-            const uint32_t strand = packed & 0x00ffffffu, seg = packed >> 24;
-            const Seg sg = project_segment(u, a.pos, a.tan, a.P, strand, seg, vpf, u->fiberRadius);
-            int x0, y0, x1, y1;
-            seg_pixel_bbox(sg, x0, y0, x1, y1);
-            SegRec r;
-            r.v00x = sg.v00.x; r.v00y = sg.v00.y; r.v01x = sg.v01.x; r.v01y = sg.v01.y;
-            r.v10x = sg.v10.x; r.v10y = sg.v10.y; r.v11x = sg.v11.x; r.v11y = sg.v11.y;
-            r.d0 = sg.depths.x; r.d1 = sg.depths.y; r.d2 = sg.depths.z; r.d3 = sg.depths.w;
-            r.packed = packed;
-            // relative to the tile origin, clamped to the int16 range
-            r.bx0 = (int16_t)max(min(x0 - (int)bx0, 30000), -30000);
-            r.by0 = (int16_t)max(min(y0 - (int)by0, 30000), -30000);
-            r.bx1 = (int16_t)max(min(x1 - (int)bx0, 30000), -30000);
-            r.by1 = (int16_t)max(min(y1 - (int)by0, 30000), -30000);
-            r.pad = 0;
-            recs[tid] = r;
-          }
-          __syncthreads();
+      if (tr.maxDepth == 0u) continue;  // _getTileDepth == 0 means an empty bin (:215)
+      const f2 td = mk2((float)(0xffffffffu - tr.minDepth) / 4294967296.0f,
+                        (float)tr.maxDepth / 4294967296.0f);
+      const uint32_t start = min(a.binStart[tb], a.capacity);
+      const uint32_t n = min(tr.padding0, a.capacity - start);
+      if (tid == 0) { s_sliceOff = 0; s_phys = 0; }
+      for (uint32_t b0 = 0; b0 < n; b0 += a.batch) {
+        const uint32_t nb = min(a.batch, n - b0);
+        __syncthreads();
+        if (tid < nb) {
+          const uint32_t key = a.binKey[start + b0 + tid];
+          const Seg sg = project_segment(u, a.pos, a.tan, a.P, key >> 8, key & 0xffu, vpf, u->fiberRadius);
+          int x0, y0, x1, y1;
+          seg_pixel_bbox(sg, x0, y0, x1, y1);
+          SegRec r;
+          r.v00x = sg.v00.x; r.v00y = sg.v00.y; r.v01x = sg.v01.x; r.v01y = sg.v01.y;
+          r.v10x = sg.v10.x; r.v10y = sg.v10.y; r.v11x = sg.v11.x; r.v11y = sg.v11.y;
+          r.d0 = sg.depths.x; r.d1 = sg.depths.y; r.d2 = sg.depths.z; r.d3 = sg.depths.w;
+          r.key = key;
+          // relative to the tile origin, clamped to the int16 range
+          r.bx0 = (int16_t)max(min(x0 - (int)bx0, 30000), -30000);
+          r.by0 = (int16_t)max(min(y0 - (int)by0, 30000), -30000);
+          r.bx1 = (int16_t)max(min(x1 - (int)bx0, 30000), -30000);
+          r.by1 = (int16_t)max(min(y1 - (int)by0, 30000), -30000);
+          // The interpolated depth is a two-level lerp of the four depths with
+          // weights in [0, 1]: at least their minimum, less the rounding of
+          // two lerps (a few ulp of the largest magnitude). Non-finite depths
+          // can interpolate to NaN, which passes the depth test.
+          const bool finite = isfinite(sg.depths.x) && isfinite(sg.depths.y) && isfinite(sg.depths.z) &&
+                              isfinite(sg.depths.w);
+          const float dn = fminf(fminf(sg.depths.x, sg.depths.y), fminf(sg.depths.z, sg.depths.w));
+          const float dm = fmaxf(fmaxf(fabsf(sg.depths.x), fabsf(sg.depths.y)),
+                                 fmaxf(fabsf(sg.depths.z), fabsf(sg.depths.w)));
+          r.dnear = finite ? dn - (dm * 1e-6f + 1e-30f) : -INFINITY;
+          L.recs[tid] = r;
+        }
+        __syncthreads();
+        uint32_t qh = 0, qt = 0;
+        for (uint32_t j = 0; j < nb; j++) {
+          bool push = false;
           if (!done) {
-            for (uint32_t j = 0; j < nb; j++) {
-              const SegRec& r = recs[j];
-              if ((int)lx < r.bx0 || (int)lx > r.bx1 || (int)ly < r.by0 || (int)ly > r.by1) continue;
+            const SegRec& r = L.recs[j];
+            if ((int)lx >= r.bx0 && (int)lx <= r.bx1 && (int)ly >= r.by0 && (int)ly <= r.by1 &&
+                !(r.dnear >= mydb)) {
               Seg sg;
               sg.v00 = mk2(r.v00x, r.v00y); sg.v01 = mk2(r.v01x, r.v01y);
               sg.v10 = mk2(r.v10x, r.v10y); sg.v11 = mk2(r.v11x, r.v11y);
-              if (!inside_quad(sg, p)) continue;
-              sg.depths = mk4(r.d0, r.d1, r.d2, r.d3);
-              const f2 w = interp_quad(sg, p);
-              const float hdp = interp_f32(w, sg.depths);
-              const float db = tex_load(a.depth, a.depthW, a.depthH, ipx, depthY);
-              if (hdp >= db) continue;
-              const uint32_t slot = atomicAdd(&s_sliceOff, 1u);
-              if (slot >= a.perProc) continue;
-              // getAlphaCoverage (processHairSegment.wgsl.ts:61-70)
-              float alpha = 1.0f - fabsf(w.x * 2.0f - 1.0f);
-              if (a.alphaQuadratic) alpha = sqrtf(alpha);
-              alpha = satf(alpha * a.alphaMultiplier);
-              const uint32_t strand = r.packed & 0x00ffffffu, seg = r.packed >> 24;
-              const float tFull = ((float)seg + w.y) / invSegs;
-              // _sampleShading — hairShadingBuffer.ts:29-38
-              const float sa = tFull * ((float)a.PS - 1.0f);
-              const uint32_t au = f2u(sa);
-              const uint32_t i0 = clampu(au, 0u, a.PS - 1u), i1 = clampu(au + 1u, 0u, a.PS - 1u);
-              const float fr = fractf(sa);
-              const float4 c0 = a.shading[(size_t)strand * a.PS + i0];
-              const float4 c1 = a.shading[(size_t)strand * a.PS + i1];
-              const float cr = mixf(c0.x, c1.x, fr), cg = mixf(c0.y, c1.y, fr);
-              const float cb = mixf(c0.z, c1.z, fr), ca = mixf(c0.w, c1.w, fr) * alpha;
-              const uint32_t sl = depth_bin(a.NS, td, hdp);
-              const uint32_t key = (strand << 8) | seg;
-              // sorted insert, ascending key from the head
-              uint32_t* headp = &heads[sl * NPX + tid];
-              uint32_t prevSlot = kInvalidPtr, cur = *headp;
-              while (cur != kInvalidPtr) {
-                const uint4 e = cur < a.ldsPool ? pool[cur] : gpool[cur];
-                if (e.w > key) break;
-                prevSlot = cur;
-                cur = e.z;
-              }
-              const uint4 entry = make_uint4(pack_half2(cr, cg), pack_half2(cb, ca), cur, key);
-              if (slot < a.ldsPool) pool[slot] = entry; else gpool[slot] = entry;
-              if (prevSlot == kInvalidPtr) {
-                *headp = slot;
-              } else if (prevSlot < a.ldsPool) {
-                pool[prevSlot].z = slot;
-              } else {
-                gpool[prevSlot].z = slot;
-              }
+              push = inside_quad(sg, p);
             }
           }
-          __syncthreads();
-          if (s_sliceOff >= a.perProc) break;  // slice memory exhausted (:252-264)
-        }
-        // reduceHairSlices — reduceHairSlices.wgsl.ts:9-73
-        if (!done) {
-          if (!readable) fc = mk4(0.0f, 0.0f, 0.0f, 0.0f);
-          uint32_t cnt = isDbg ? f2u(fc.x * (float)dbgMax) : 0u;
-          for (uint32_t s = 0; s < a.NS; s++) {
-            uint32_t cur = heads[s * NPX + tid];
-            heads[s * NPX + tid] = kInvalidPtr;
-            if (!isDbg && fc.w >= 0.999f) continue;
-            while (cur != kInvalidPtr) {
-              if (!isDbg && fc.w >= 0.999f) break;
-              const uint4 e = cur < a.ldsPool ? pool[cur] : gpool[cur];
-              cur = e.z;
-              cnt++;
-              const float oneMinusA = 1.0f - fc.w;
-              fc.x = fc.x + half_lo(e.x) * oneMinusA;
-              fc.y = fc.y + half_hi(e.x) * oneMinusA;
-              fc.z = fc.z + half_lo(e.y) * oneMinusA;
-              fc.w = fc.w + half_hi(e.y) * oneMinusA;
-            }
+          const uint64_t m = __ballot(push);
+          if (m == 0) continue;
+          uint32_t base = 0;
+          const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+          if (lane == leader) base = atomicAdd(&s_phys, (uint32_t)__popcll(m));
+          base = (uint32_t)__shfl((int)base, (int)leader);
+          const uint32_t slot = base + mask_prefix(m);
+          const bool keep = push && slot < a.physCap;
+          const uint64_t mk = __ballot(keep);
+          if (keep) {
+            uint4* e = slot < a.ldsPool ? &L.pool[slot] : &gpool[slot - a.ldsPool];
+            e->z = L.heads[tid];
+            L.heads[tid] = slot;
+            L.ring[w * kFineRing + ((qt + mask_prefix(mk)) & (kFineRing - 1u))] = (slot << 14) | (j << 6) | lane;
           }
-          if (isDbg) {
-            const float c = satf((float)cnt / (float)dbgMax);
-            fc = mk4(c, 0.0f, 0.0f, 1.0f);
+          qt += (uint32_t)__popcll(mk);
+          if (qt - qh >= 64u) {
+            fine_shade_queue<TILE>(a, L, gpool, w, qh, 64u, bx0, by0, td, invSegs, &s_sliceOff);
+            qh += 64u;
           }
-          if (readable) a.fb[fbIdx] = make_float4(fc.x, fc.y, fc.z, fc.w);
-          done = !isDbg && fc.w >= 0.999f;
         }
+        if (qt > qh) fine_shade_queue<TILE>(a, L, gpool, w, qh, qt - qh, bx0, by0, td, invSegs, &s_sliceOff);
+        __syncthreads();
+        if (s_sliceOff >= a.perProc) break;  // slice memory exhausted (:252-264)
+      }
+      __syncthreads();
+      // reduceHairSlices — reduceHairSlices.wgsl.ts:9-73
+      if (!done) {
+        if (!readable) fc = mk4(0.0f, 0.0f, 0.0f, 0.0f);
+        uint32_t cnt = isDbg ? f2u(fc.x * (float)dbgMax) : 0u;
+        const uint32_t head = L.heads[tid];
+        L.heads[tid] = kInvalidPtr;
+        for (uint32_t s = 0; s < a.NS; s++) {
+          if (!isDbg && fc.w >= 0.999f) continue;
+          uint32_t cur = head;
+          while (cur != kInvalidPtr) {
+            if (!isDbg && fc.w >= 0.999f) break;
+            const uint4 e = cur < a.ldsPool ? L.pool[cur] : gpool[cur - a.ldsPool];
+            cur = e.z;
+            if (e.w != s) continue;
+            cnt++;
+            const float oneMinusA = 1.0f - fc.w;
+            fc.x = fc.x + half_lo(e.x) * oneMinusA;
+            fc.y = fc.y + half_hi(e.x) * oneMinusA;
+            fc.z = fc.z + half_lo(e.y) * oneMinusA;
+            fc.w = fc.w + half_hi(e.y) * oneMinusA;
+          }
+        }
+        if (isDbg) {
+          const float c = satf((float)cnt / (float)dbgMax);
+          fc = mk4(c, 0.0f, 0.0f, 1.0f);
+        }
+        if (readable) a.fb[fbIdx] = make_float4(fc.x, fc.y, fc.z, fc.w);
+        done = !isDbg && fc.w >= 0.999f;
       }
       // checkAllPixelsInWkgrpDone :305-327
       if (__syncthreads_and(done)) break;
@@ -930,12 +1064,14 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
   const size_t offEnds = 0;
   const size_t offSums = fh::align_up(offEnds + (size_t)nTB * 4, 256);
   const size_t offSeg = fh::align_up(offSums + (size_t)scanBlocks * 4, 256);
-  const size_t total = fh::align_up(offSeg + (size_t)limit * 4 + 4, 256);
+  const size_t offBig = fh::align_up(offSeg + (size_t)limit * 4 + 4, 256);
+  const size_t total = fh::align_up(offBig + (size_t)limit * 8 + 8, 256);
   if ((rc = ensure_scratch(ctx, total))) return rc;
   char* base = reinterpret_cast<char*>(ctx->scratch);
   uint32_t* ends = reinterpret_cast<uint32_t*>(base + offEnds);
   uint32_t* sums = reinterpret_cast<uint32_t*>(base + offSums);
   uint32_t* binSeg = reinterpret_cast<uint32_t*>(base + offSeg);
+  uint32_t* big = reinterpret_cast<uint32_t*>(base + offBig);
   const fh_list_element* data =
       reinterpret_cast<const fh_list_element*>(reinterpret_cast<const char*>(tile_segments) + 4);
 
@@ -949,12 +1085,15 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
   bin_scatter_kernel<<<scatterBlocks, 256, 0, ctx->stream>>>(
       uniforms, reinterpret_cast<const uint32_t*>(tile_segments), data, limit, slot_tile_bin, ends, nTB, binSeg);
   if ((rc = check_launch("bin_scatter_kernel"))) return rc;
+  bin_sort_kernel<<<std::min(nTB, (uint32_t)ctx->num_cus * 8u), 256, 0, ctx->stream>>>(tiles, ends, nTB, limit,
+                                                                                       binSeg, big);
+  if ((rc = check_launch("bin_sort_kernel"))) return rc;
 
   FineArgs a;
   a.u = uniforms;
   a.pos = reinterpret_cast<const float4*>(positions);
   a.tan = reinterpret_cast<const float4*>(tangents);
-  a.tiles = tiles; a.binStart = ends; a.binSeg = binSeg; a.capacity = limit;
+  a.tiles = tiles; a.binStart = ends; a.binKey = binSeg; a.capacity = limit;
   a.slicePool = reinterpret_cast<uint4*>(slice_data);
   a.queue = reinterpret_cast<uint32_t*>(rasterizer_result);
   a.fb = reinterpret_cast<float4*>(reinterpret_cast<char*>(rasterizer_result) + 16);
@@ -967,13 +1106,16 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
   a.alphaQuadratic = cfg->alphaQuadratic; a.alphaMultiplier = cfg->alphaMultiplier;
   const uint32_t npx = T * T;
   a.batch = std::min(npx, 256u);
-  // LDS: segment batch + slice heads + fragment pool. Under 80 KiB per
-  // workgroup keeps two 16x16 processors per CU (160 KiB LDS on gfx950).
-  const size_t ldsTarget = std::min<size_t>(T == 8 ? 32 * 1024 : (T == 16 ? 78 * 1024 : 128 * 1024),
+  // LDS: segment batch, list heads, mesh depth, fragment rings, then the
+  // LDS part of the fragment pool. Under 80 KiB per workgroup keeps two
+  // 16x16 processors per CU (160 KiB LDS on gfx950).
+  const size_t ldsTarget = std::min<size_t>(T == 8 ? 40 * 1024 : (T == 16 ? 78 * 1024 : 150 * 1024),
                                             ctx->max_lds - 256);
-  const size_t fixed = (size_t)a.batch * sizeof(SegRec) + (size_t)a.NS * npx * 4;
-  FH_REQUIRE(fixed + 16 * 64 <= ldsTarget, "fh_fine_raster: slicesPerPixel too large for the tile size");
-  a.ldsPool = (uint32_t)std::min<size_t>((ldsTarget - fixed) / 16, a.perProc);
+  const size_t fixed = (size_t)a.batch * sizeof(SegRec) + (size_t)npx * 8 + (size_t)(npx / 64) * kFineRing * 4;
+  FH_REQUIRE(fixed + 16 * 64 <= ldsTarget, "fh_fine_raster: LDS too small for the tile size");
+  // pool slot indices must fit the 18 bits of a ring entry
+  a.ldsPool = (uint32_t)std::min<size_t>((ldsTarget - fixed) / 16, 1u << 17);
+  a.physCap = (uint32_t)std::min<uint64_t>((uint64_t)a.ldsPool + a.perProc, 1u << 18);
   const size_t lds = fixed + (size_t)a.ldsPool * 16;
   const uint32_t grid = cfg->processorCount;
   if (T == 8) {
