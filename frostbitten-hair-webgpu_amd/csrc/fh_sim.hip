@@ -315,34 +315,73 @@ __global__ __launch_bounds__(kSimLanes) void sim_integrate_kernel(
 
 // ---------------------------------------------------------------------------
 // K3 — gridPostSimPass.wgsl.ts:45-72 with _addGridDensityVelocity
-// (densityVelocityGrid.wgsl.ts:93-124). One thread per point; integer adds
-// commute, so the grid is bit-identical whatever the atomic order. Adding 0
-// changes nothing, so zero contributions are not sent.
+// (densityVelocityGrid.wgsl.ts:93-124). Every point adds fixed-point
+// velocity and density to the 8 grid nodes around it. Hair is dense around
+// few nodes (a whole head of strands starts at the scalp), so direct global
+// atomics serialize on the hot nodes. Instead each workgroup folds the
+// contributions of kP2GPoints consecutive points into an LDS hash table
+// (node -> 4 x i32) and then adds each table entry to the grid with one
+// global atomic per non-zero component. i32 addition is associative and
+// commutative (mod 2^32), so the grid is bit-identical to one atomic per
+// contribution in any order; a node that finds no free table slot within
+// kP2GProbe probes goes straight to the global grid.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void grid_post_sim_kernel(
+constexpr uint32_t kP2GThreads = 1024;
+constexpr uint32_t kP2GPoints = 2048;
+constexpr uint32_t kP2GTable = 4096;  // power of two
+constexpr uint32_t kP2GProbe = 64;
+constexpr uint32_t kEmptyNode = 0xffffffffu;
+static_assert(kP2GTable == 1u << 12, "the hash below takes 12 bits");
+
+__global__ __launch_bounds__(kP2GThreads) void grid_post_sim_kernel(
     uint32_t d, const fh_sim_uniforms* __restrict__ u, const float4* __restrict__ prev,
     const float4* __restrict__ now, fh_density_velocity* __restrict__ dv, uint32_t nPoints) {
-  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nPoints) return;
+  __shared__ uint32_t keys[kP2GTable];
+  __shared__ int32_t vals[kP2GTable][4];
+  for (uint32_t i = threadIdx.x; i < kP2GTable; i += kP2GThreads) {
+    keys[i] = kEmptyNode;
+    vals[i][0] = vals[i][1] = vals[i][2] = vals[i][3] = 0;
+  }
+  __syncthreads();
   const f3 gmin = mk3(u->gridBoundsMin[0], u->gridBoundsMin[1], u->gridBoundsMin[2]);
   const f3 gmax = mk3(u->gridBoundsMax[0], u->gridBoundsMax[1], u->gridBoundsMax[2]);
-  const float4 pp = prev[e], pn = now[e];
-  const f3 p = mk3(pn.x, pn.y, pn.z);
-  const f3 vel = p - mk3(pp.x, pp.y, pp.z);
-  const Cell co = grid_cell(d, gmin, gmax, p);
-  for (uint32_t z = co.mn[2]; z <= co.mx[2]; z++)
-    for (uint32_t y = co.mn[1]; y <= co.mx[1]; y++)
-      for (uint32_t x = co.mn[0]; x <= co.mx[0]; x++) {
-        const f3 w = cell_w(x, y, z, co.pin);
-        const f3 cv = vel * w;
-        int32_t* cell = reinterpret_cast<int32_t*>(&dv[grid_idx(d, x, y, z)]);
-        const int32_t vx = f2i(cv.x * kGridMul), vy = f2i(cv.y * kGridMul), vz = f2i(cv.z * kGridMul);
-        const int32_t dn = f2i(length(w) * kGridMul);
-        if (vx) atomicAdd(cell + 0, vx);
-        if (vy) atomicAdd(cell + 1, vy);
-        if (vz) atomicAdd(cell + 2, vz);
-        if (dn) atomicAdd(cell + 3, dn);
-      }
+  const uint32_t first = blockIdx.x * kP2GPoints;
+  for (uint32_t e = first + threadIdx.x; e < min(first + kP2GPoints, nPoints); e += kP2GThreads) {
+    const float4 pp = prev[e], pn = now[e];
+    const f3 p = mk3(pn.x, pn.y, pn.z);
+    const f3 vel = p - mk3(pp.x, pp.y, pp.z);
+    const Cell co = grid_cell(d, gmin, gmax, p);
+    for (uint32_t z = co.mn[2]; z <= co.mx[2]; z++)
+      for (uint32_t y = co.mn[1]; y <= co.mx[1]; y++)
+        for (uint32_t x = co.mn[0]; x <= co.mx[0]; x++) {
+          const f3 w = cell_w(x, y, z, co.pin);
+          const f3 cv = vel * w;
+          const int32_t c[4] = {f2i(cv.x * kGridMul), f2i(cv.y * kGridMul), f2i(cv.z * kGridMul),
+                                f2i(length(w) * kGridMul)};
+          if ((c[0] | c[1] | c[2] | c[3]) == 0) continue;  // adding 0 changes nothing
+          const uint32_t node = grid_idx(d, x, y, z);
+          uint32_t h = (node * 2654435761u) >> (32 - 12);
+          int32_t* slot = nullptr;
+          for (uint32_t k = 0; k < kP2GProbe; k++) {
+            const uint32_t old = atomicCAS(&keys[h], kEmptyNode, node);
+            if (old == kEmptyNode || old == node) { slot = vals[h]; break; }
+            h = (h + 1u) & (kP2GTable - 1u);
+          }
+          if (!slot) slot = reinterpret_cast<int32_t*>(&dv[node]);
+#pragma unroll
+          for (int i = 0; i < 4; i++)
+            if (c[i]) atomicAdd(slot + i, c[i]);
+        }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < kP2GTable; i += kP2GThreads) {
+    const uint32_t node = keys[i];
+    if (node == kEmptyNode) continue;
+    int32_t* cell = reinterpret_cast<int32_t*>(&dv[node]);
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+      if (vals[i][c]) atomicAdd(cell + c, vals[i][c]);
+  }
 }
 
 }  // namespace
@@ -422,7 +461,7 @@ int fh_grid_post_sim(fh_ctx* ctx, uint32_t grid_dims, const fh_sim_uniforms* sim
   const uint64_t n = (uint64_t)strands_count * points_per_strand;
   FH_REQUIRE(n < (1ull << 31), "fh_grid_post_sim: too many points");
   if (n == 0) return FH_OK;
-  grid_post_sim_kernel<<<fh::div_ceil((uint32_t)n, 256), 256, 0, ctx->stream>>>(
+  grid_post_sim_kernel<<<fh::div_ceil((uint32_t)n, kP2GPoints), kP2GThreads, 0, ctx->stream>>>(
       grid_dims, sim_uniforms, reinterpret_cast<const float4*>(positions_prev),
       reinterpret_cast<const float4*>(positions_now), density_velocity, (uint32_t)n);
   return fh::check_launch("grid_post_sim_kernel");
