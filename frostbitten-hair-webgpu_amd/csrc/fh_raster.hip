@@ -641,25 +641,36 @@ struct FineArgs {
 };
 
 struct SegRec {
-  float v00x, v00y, v01x, v01y, v10x, v10y, v11x, v11y;
-  float d0, d1, d2, d3;
-  uint32_t key;
-  int16_t bx0, by0, bx1, by1;
-  float dnear;  // smallest projected depth of the quad, -inf if any is NaN
+  float v[8];        // v00, v01, v10, v11
+  float d[4];        // projected depths
+  float slope[6];    // dx/dy of the six vertex pairs (hull_strip_x)
+  uint32_t key;      // strand << 8 | segment
+  int16_t bx0, by0, bx1, by1;  // pixel box relative to the tile, inclusive, clamped to [-1, TILE]
+  float dnear;       // lower bound of the interpolated depth (-inf when unknown)
+  uint32_t span;     // 1 when per-row hull spans may narrow the box
+  uint32_t pad;
 };
-static_assert(sizeof(SegRec) == 64, "SegRec must be 64 bytes");
+static_assert(sizeof(SegRec) == 96, "SegRec must be 96 bytes");
 
 constexpr uint32_t kFineRing = 128;      // per-wave ring of queued fragments
 constexpr uint32_t kSkipTag = 0xffffffffu;
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
 
 template <int TILE>
 struct FineLds {
   static constexpr uint32_t NPX = TILE * TILE;
   static constexpr uint32_t NW = NPX / 64;
   SegRec* recs;
-  uint32_t* heads;   // NPX
-  float* db;         // NPX
+  uint32_t* heads;   // NPX: newest pool slot of each pixel
+  float* db;         // NPX: mesh depth
+  uint32_t* done;    // NPX: pixel saturated
+  uint32_t* claim;   // NPX: lowest lane pushing to the pixel (64 = none)
   uint32_t* ring;    // NW * kFineRing
+  uint32_t* wsc;     // NW * 5 * 64: per-wave enumeration scratch
   uint4* pool;       // ldsPool
 };
 
@@ -681,9 +692,9 @@ __device__ __forceinline__ void fine_shade_queue(const FineArgs& a, const FineLd
     const uint32_t pix = w * 64u + (e & 63u);
     const f2 p = mk2((float)(bx0 + pix % TILE), (float)(by0 + pix / TILE));
     Seg sg;
-    sg.v00 = mk2(r->v00x, r->v00y); sg.v01 = mk2(r->v01x, r->v01y);
-    sg.v10 = mk2(r->v10x, r->v10y); sg.v11 = mk2(r->v11x, r->v11y);
-    sg.depths = mk4(r->d0, r->d1, r->d2, r->d3);
+    sg.v00 = mk2(r->v[0], r->v[1]); sg.v01 = mk2(r->v[2], r->v[3]);
+    sg.v10 = mk2(r->v[4], r->v[5]); sg.v11 = mk2(r->v[6], r->v[7]);
+    sg.depths = mk4(r->d[0], r->d[1], r->d[2], r->d[3]);
     wq = interp_quad(sg, p);
     hdp = interp_f32(wq, sg.depths);
     accept = !(hdp >= L.db[pix]);  // depth test with GL_LESS (:31-34)
@@ -724,6 +735,122 @@ __device__ __forceinline__ void fine_shade_queue(const FineArgs& a, const FineLd
   }
 }
 
+// Cheap tests of one batch for wave `w`, which owns the pixel rows
+// [w * 64 / TILE, (w + 1) * 64 / TILE) of the tile. Candidates are enumerated
+// record by record (records are in blend order), row by row within the
+// record's box and its hull span, 64 per step; a pixel that passes is linked
+// at the head of its list. Two lanes may push to the same pixel in one step:
+// they link in lane order (one lane per pixel per round), which is record
+// order, so every list stays in blend order.
+template <int TILE>
+__device__ __forceinline__ void fine_cheap_batch(const FineArgs& a, const FineLds<TILE>& L, uint4* gpool,
+                                                 uint32_t nb, uint32_t bx0, uint32_t by0, f2 td,
+                                                 float invSegs, uint32_t* sliceOff, uint32_t* phys) {
+  constexpr int RW = 64 / TILE;
+  const uint32_t lane = __lane_id();
+  const uint32_t w = threadIdx.x >> 6;
+  const int wy0 = (int)w * RW, wy1 = wy0 + RW - 1;
+  uint32_t* rowEnd = L.wsc + (w * 5u + 0u) * 64u;
+  uint32_t* itemRec = L.wsc + (w * 5u + 1u) * 64u;
+  uint32_t* itemY = L.wsc + (w * 5u + 2u) * 64u;
+  uint32_t* itemX = L.wsc + (w * 5u + 3u) * 64u;
+  uint32_t* candEnd = L.wsc + (w * 5u + 4u) * 64u;
+  uint32_t qh = 0, qt = 0;
+  for (uint32_t rb = 0; rb < nb; rb += 64u) {
+    const uint32_t jr = rb + lane;
+    uint32_t nrows = 0;
+    if (jr < nb) {
+      const SegRec& r = L.recs[jr];
+      const int ya = max((int)r.by0, wy0), yb = min((int)r.by1, wy1);
+      const int xa = max((int)r.bx0, 0), xb = min((int)r.bx1, TILE - 1);
+      if (yb >= ya && xb >= xa) nrows = (uint32_t)(yb - ya + 1);
+    }
+    const uint32_t re = wave_incl_scan(nrows);
+    rowEnd[lane] = re;
+    const uint32_t nRows = (uint32_t)__shfl((int)re, 63);
+    wave_sync();
+    for (uint32_t r0 = 0; r0 < nRows; r0 += 64u) {
+      const uint32_t ridx = r0 + lane;
+      uint32_t cnt = 0;
+      if (ridx < nRows) {
+        const uint32_t k = search64(rowEnd, ridx);
+        const uint32_t j = rb + k;
+        const SegRec& r = L.recs[j];
+        const int y = max((int)r.by0, wy0) + (int)(ridx - (k ? rowEnd[k - 1u] : 0u));
+        int xa = max((int)r.bx0, 0), xb = min((int)r.bx1, TILE - 1) + 1;
+        if (r.span) {
+          float xlo, xhi;
+          const float yf = (float)(by0 + (uint32_t)y);
+          hull_strip_x(r.v, r.slope, yf - 1.0f, yf + 1.0f, xlo, xhi);
+          if (xlo <= xhi) {
+            xa = max(xa, (int)ceilf(xlo - 1.0f) - (int)bx0);
+            xb = min(xb, (int)floorf(xhi + 1.0f) + 1 - (int)bx0);
+          } else {
+            xb = xa;
+          }
+        }
+        if (xb > xa) cnt = (uint32_t)(xb - xa);
+        itemRec[lane] = j;
+        itemY[lane] = (uint32_t)y;
+        itemX[lane] = (uint32_t)xa;
+      }
+      const uint32_t ce = wave_incl_scan(cnt);
+      candEnd[lane] = ce;
+      const uint32_t nCand = (uint32_t)__shfl((int)ce, 63);
+      wave_sync();
+      for (uint32_t c0 = 0; c0 < nCand; c0 += 64u) {
+        const uint32_t c = c0 + lane;
+        bool push = false;
+        uint32_t pix = 0, j = 0;
+        if (c < nCand) {
+          const uint32_t it = search64(candEnd, c);
+          const uint32_t x = itemX[it] + (c - (it ? candEnd[it - 1u] : 0u));
+          const uint32_t y = itemY[it];
+          j = itemRec[it];
+          pix = y * TILE + x;
+          const SegRec& r = L.recs[j];
+          if (!L.done[pix] && !(r.dnear >= L.db[pix])) {
+            Seg sg;
+            sg.v00 = mk2(r.v[0], r.v[1]); sg.v01 = mk2(r.v[2], r.v[3]);
+            sg.v10 = mk2(r.v[4], r.v[5]); sg.v11 = mk2(r.v[6], r.v[7]);
+            push = inside_quad(sg, mk2((float)(bx0 + x), (float)(by0 + y)));
+          }
+        }
+        const uint64_t m = __ballot(push);
+        if (m == 0) continue;
+        uint32_t base = 0;
+        const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+        if (lane == leader) base = atomicAdd(phys, (uint32_t)__popcll(m));
+        base = (uint32_t)__shfl((int)base, (int)leader);
+        const uint32_t slot = base + mask_prefix(m);
+        bool pend = push && slot < a.physCap;
+        const uint64_t mk = __ballot(pend);
+        if (pend)
+          L.ring[w * kFineRing + ((qt + mask_prefix(mk)) & (kFineRing - 1u))] =
+              (slot << 14) | (j << 6) | (pix - w * 64u);
+        while (__ballot(pend)) {
+          if (pend) atomicMin(&L.claim[pix], lane);
+          wave_sync();
+          if (pend && L.claim[pix] == lane) {
+            uint4* e = slot < a.ldsPool ? &L.pool[slot] : &gpool[slot - a.ldsPool];
+            e->z = L.heads[pix];
+            L.heads[pix] = slot;
+            L.claim[pix] = 64u;
+            pend = false;
+          }
+          wave_sync();
+        }
+        qt += (uint32_t)__popcll(mk);
+        if (qt - qh >= 64u) {
+          fine_shade_queue<TILE>(a, L, gpool, w, qh, 64u, bx0, by0, td, invSegs, sliceOff);
+          qh += 64u;
+        }
+      }
+    }
+  }
+  if (qt > qh) fine_shade_queue<TILE>(a, L, gpool, w, qh, qt - qh, bx0, by0, td, invSegs, sliceOff);
+}
+
 template <int TILE>
 __global__ __launch_bounds__(TILE* TILE) void fine_raster_kernel(FineArgs a) {
   constexpr uint32_t NPX = TILE * TILE;
@@ -733,15 +860,17 @@ __global__ __launch_bounds__(TILE* TILE) void fine_raster_kernel(FineArgs a) {
   L.recs = reinterpret_cast<SegRec*>(smem);
   L.heads = reinterpret_cast<uint32_t*>(smem + (size_t)a.batch * sizeof(SegRec));
   L.db = reinterpret_cast<float*>(L.heads + NPX);
-  L.ring = reinterpret_cast<uint32_t*>(L.db + NPX);
-  L.pool = reinterpret_cast<uint4*>(L.ring + NW * kFineRing);
+  L.done = reinterpret_cast<uint32_t*>(L.db + NPX);
+  L.claim = L.done + NPX;
+  L.ring = L.claim + NPX;
+  L.wsc = L.ring + NW * kFineRing;
+  L.pool = reinterpret_cast<uint4*>(L.wsc + NW * 5u * 64u);
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_sliceOff;  // accepted fragments of the bin
   __shared__ uint32_t s_phys;      // pool slots taken in the bin
 
   const fh_render_uniforms* __restrict__ u = a.u;
   const uint32_t tid = threadIdx.x;
-  const uint32_t lane = __lane_id(), w = tid >> 6;
   const uint32_t lx = tid % TILE, ly = tid / TILE;
   const f2 vpf = mk2(u->viewport[0], u->viewport[1]);
   const TileGeom tg = tile_geom(vpf, TILE);
@@ -755,6 +884,7 @@ __global__ __launch_bounds__(TILE* TILE) void fine_raster_kernel(FineArgs a) {
   uint4* gpool = a.slicePool + (size_t)blockIdx.x * a.perProc;
 
   L.heads[tid] = kInvalidPtr;
+  L.claim[tid] = 64u;
 
   while (true) {
     __syncthreads();
@@ -772,8 +902,8 @@ __global__ __launch_bounds__(TILE* TILE) void fine_raster_kernel(FineArgs a) {
     const f2 p = mk2((float)px, (float)py);
     const bool readable = px < tg.vpW && py < tg.vpH && (tg.vpW * py + px) < fbLen;
     const uint32_t fbIdx = tg.vpW * py + px;
-    const float mydb = tex_load(a.depth, a.depthW, a.depthH, (int)px, f2i(vpf.y - p.y));
-    L.db[tid] = mydb;
+    L.db[tid] = tex_load(a.depth, a.depthW, a.depthH, (int)px, f2i(vpf.y - p.y));
+    L.done[tid] = 0u;
     bool done = false;
     f4 fc = mk4(0.0f, 0.0f, 0.0f, 0.0f);
     if (readable) { const float4 v = a.fb[fbIdx]; fc = mk4(v.x, v.y, v.z, v.w); }
@@ -796,15 +926,26 @@ __global__ __launch_bounds__(TILE* TILE) void fine_raster_kernel(FineArgs a) {
           int x0, y0, x1, y1;
           seg_pixel_bbox(sg, x0, y0, x1, y1);
           SegRec r;
-          r.v00x = sg.v00.x; r.v00y = sg.v00.y; r.v01x = sg.v01.x; r.v01y = sg.v01.y;
-          r.v10x = sg.v10.x; r.v10y = sg.v10.y; r.v11x = sg.v11.x; r.v11y = sg.v11.y;
-          r.d0 = sg.depths.x; r.d1 = sg.depths.y; r.d2 = sg.depths.z; r.d3 = sg.depths.w;
+          const float vv[8] = {sg.v00.x, sg.v00.y, sg.v01.x, sg.v01.y, sg.v10.x, sg.v10.y, sg.v11.x, sg.v11.y};
+          float amax = 0.0f;
+#pragma unroll
+          for (int i = 0; i < 8; i++) { r.v[i] = vv[i]; amax = fmaxf(amax, fabsf(vv[i])); }
+#pragma unroll
+          for (int k = 0; k < 6; k++) {
+            int i, j;
+            hull_pair(k, i, j);
+            r.slope[k] = (vv[2 * j] - vv[2 * i]) / (vv[2 * j + 1] - vv[2 * i + 1]);
+          }
+          r.d[0] = sg.depths.x; r.d[1] = sg.depths.y; r.d[2] = sg.depths.z; r.d[3] = sg.depths.w;
           r.key = key;
-          // relative to the tile origin, clamped to the int16 range
-          r.bx0 = (int16_t)max(min(x0 - (int)bx0, 30000), -30000);
-          r.by0 = (int16_t)max(min(y0 - (int)by0, 30000), -30000);
-          r.bx1 = (int16_t)max(min(x1 - (int)bx0, 30000), -30000);
-          r.by1 = (int16_t)max(min(y1 - (int)by0, 30000), -30000);
+          // relative to the tile origin, clamped to one pixel beyond the tile
+          r.bx0 = (int16_t)max(min(x0 - (int)bx0, TILE), -1);
+          r.by0 = (int16_t)max(min(y0 - (int)by0, TILE), -1);
+          r.bx1 = (int16_t)max(min(x1 - (int)bx0, TILE), -1);
+          r.by1 = (int16_t)max(min(y1 - (int)by0, TILE), -1);
+          // collinear quads test the whole box; far-off vertices skip the span
+          r.span = (x0 > -(1 << 20) && amax < 65536.0f) ? 1u : 0u;
+          r.pad = 0;
           // The interpolated depth is a two-level lerp of the four depths with
           // weights in [0, 1]: at least their minimum, less the rounding of
           // two lerps (a few ulp of the largest magnitude). Non-finite depths
@@ -818,41 +959,7 @@ __global__ __launch_bounds__(TILE* TILE) void fine_raster_kernel(FineArgs a) {
           L.recs[tid] = r;
         }
         __syncthreads();
-        uint32_t qh = 0, qt = 0;
-        for (uint32_t j = 0; j < nb; j++) {
-          bool push = false;
-          if (!done) {
-            const SegRec& r = L.recs[j];
-            if ((int)lx >= r.bx0 && (int)lx <= r.bx1 && (int)ly >= r.by0 && (int)ly <= r.by1 &&
-                !(r.dnear >= mydb)) {
-              Seg sg;
-              sg.v00 = mk2(r.v00x, r.v00y); sg.v01 = mk2(r.v01x, r.v01y);
-              sg.v10 = mk2(r.v10x, r.v10y); sg.v11 = mk2(r.v11x, r.v11y);
-              push = inside_quad(sg, p);
-            }
-          }
-          const uint64_t m = __ballot(push);
-          if (m == 0) continue;
-          uint32_t base = 0;
-          const uint32_t leader = (uint32_t)__builtin_ctzll(m);
-          if (lane == leader) base = atomicAdd(&s_phys, (uint32_t)__popcll(m));
-          base = (uint32_t)__shfl((int)base, (int)leader);
-          const uint32_t slot = base + mask_prefix(m);
-          const bool keep = push && slot < a.physCap;
-          const uint64_t mk = __ballot(keep);
-          if (keep) {
-            uint4* e = slot < a.ldsPool ? &L.pool[slot] : &gpool[slot - a.ldsPool];
-            e->z = L.heads[tid];
-            L.heads[tid] = slot;
-            L.ring[w * kFineRing + ((qt + mask_prefix(mk)) & (kFineRing - 1u))] = (slot << 14) | (j << 6) | lane;
-          }
-          qt += (uint32_t)__popcll(mk);
-          if (qt - qh >= 64u) {
-            fine_shade_queue<TILE>(a, L, gpool, w, qh, 64u, bx0, by0, td, invSegs, &s_sliceOff);
-            qh += 64u;
-          }
-        }
-        if (qt > qh) fine_shade_queue<TILE>(a, L, gpool, w, qh, qt - qh, bx0, by0, td, invSegs, &s_sliceOff);
+        fine_cheap_batch<TILE>(a, L, gpool, nb, bx0, by0, td, invSegs, &s_sliceOff, &s_phys);
         __syncthreads();
         if (s_sliceOff >= a.perProc) break;  // slice memory exhausted (:252-264)
       }
@@ -885,6 +992,7 @@ __global__ __launch_bounds__(TILE* TILE) void fine_raster_kernel(FineArgs a) {
         }
         if (readable) a.fb[fbIdx] = make_float4(fc.x, fc.y, fc.z, fc.w);
         done = !isDbg && fc.w >= 0.999f;
+        L.done[tid] = done ? 1u : 0u;
       }
       // checkAllPixelsInWkgrpDone :305-327
       if (__syncthreads_and(done)) break;
@@ -1111,7 +1219,8 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
   // 16x16 processors per CU (160 KiB LDS on gfx950).
   const size_t ldsTarget = std::min<size_t>(T == 8 ? 40 * 1024 : (T == 16 ? 78 * 1024 : 150 * 1024),
                                             ctx->max_lds - 256);
-  const size_t fixed = (size_t)a.batch * sizeof(SegRec) + (size_t)npx * 8 + (size_t)(npx / 64) * kFineRing * 4;
+  const size_t fixed = (size_t)a.batch * sizeof(SegRec) + (size_t)npx * 16 +
+                       (size_t)(npx / 64) * (kFineRing + 5 * 64) * 4;
   FH_REQUIRE(fixed + 16 * 64 <= ldsTarget, "fh_fine_raster: LDS too small for the tile size");
   // pool slot indices must fit the 18 bits of a ring entry
   a.ldsPool = (uint32_t)std::min<size_t>((ldsTarget - fixed) / 16, 1u << 17);
