@@ -111,58 +111,58 @@ __global__ __launch_bounds__(256) void grid_pre_sim_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// K2 — hairSimIntegrationPass.wgsl.ts:79-273. One strand per lane, one wave
-// per workgroup. The wave's 64 strands are staged into LDS with coalesced
-// loads in a lane-interleaved layout (element j of lane l at j*64 + l, so a
-// wave-wide b128 access touches 1 KiB of consecutive LDS); every sweep keeps
-// the Gauss-Seidel carry in registers.
+// K2 — hairSimIntegrationPass.wgsl.ts:79-273. A workgroup of kSimThreads
+// threads owns kSimStrands strands, staged in LDS lane-interleaved (point j
+// of strand l at j * kSimStrands + l). The two Gauss-Seidel sweeps along the
+// strand (length constraint; local shape with the tip-tangent tail) run one
+// strand per lane on the first wave, so each wave instruction advances 32
+// strands; the point-local phases (Verlet integration with both grid
+// samples, global shape, sphere and SDF collisions) are memory-latency bound
+// and spread over all threads, one (strand, point) per thread at a time.
 // ---------------------------------------------------------------------------
-template <int kSimLanes>
-__global__ __launch_bounds__(kSimLanes) void sim_integrate_kernel(
+constexpr uint32_t kSimStrands = 32;
+constexpr uint32_t kSimThreads = 256;
+
+__global__ __launch_bounds__(kSimThreads) void sim_integrate_kernel(
     uint32_t d, const fh_sim_uniforms* __restrict__ u, float4* __restrict__ prev,
     const float4* __restrict__ now, const float* __restrict__ seglen,
     const float* __restrict__ sdf, uint32_t sdf_dims,
     const fh_density_velocity* __restrict__ dv, const fh_density_grad_wind* __restrict__ dgw,
     const float4* __restrict__ initial, uint32_t S, uint32_t P) {
   extern __shared__ float4 lds[];
-  float4* A = lds;                          // positions being solved (xyz, movable)
-  float4* B = lds + (size_t)P * kSimLanes;  // (initial xyz, rest length)
-  float4* C = B + (size_t)P * kSimLanes;    // local-shape constants (h0, w0, w1, w2); PREV staging first
-  const uint32_t lane = threadIdx.x;
-  const uint32_t s0 = blockIdx.x * kSimLanes;
-  const uint32_t nStr = min((uint32_t)kSimLanes, S - s0);
+  constexpr uint32_t L = kSimStrands;
+  float4* A = lds;                       // positions being solved (xyz, movable)
+  float4* Bi = lds + (size_t)P * L;      // initial positions
+  float4* K = Bi + (size_t)P * L;        // local-shape constants (h0, w0, w1, w2)
+  float* Len = reinterpret_cast<float*>(K + (size_t)P * L);  // rest lengths
+  const uint32_t tid = threadIdx.x;
+  const uint32_t s0 = blockIdx.x * L;
+  const uint32_t nStr = min(L, S - s0);
   const uint32_t nElem = nStr * P;
   const size_t gbase = (size_t)s0 * P;
 
-  // coalesced staging: element e of this wave's block -> strand e / P, point e % P
-  for (uint32_t e = lane; e < nElem; e += kSimLanes) {
+  const f3 gmin = mk3(u->gridBoundsMin[0], u->gridBoundsMin[1], u->gridBoundsMin[2]);
+  const f3 gmax = mk3(u->gridBoundsMax[0], u->gridBoundsMax[1], u->gridBoundsMax[2]);
+  const f3 gravity = mk3(0.0f, -u->gravity, 0.0f);
+  const f3 windDir = mk3(u->wind[0], u->wind[1], u->wind[2]);
+  const float dt = u->deltaTime;
+
+  // staging in global order (coalesced) + Verlet integration with grid
+  // forces for points 1..P-1 (:125-162)
+  for (uint32_t e = tid; e < nElem; e += kSimThreads) {
     const uint32_t ls = e / P, pt = e - ls * P;
-    const uint32_t o = pt * kSimLanes + ls;
-    A[o] = now[gbase + e];
-    C[o] = prev[gbase + e];
-    const float4 ini = initial[gbase + e];
-    B[o] = make_float4(ini.x, ini.y, ini.z, seglen[gbase + e]);
-  }
-  __syncthreads();
-
-  const bool valid = lane < nStr;
-  const uint32_t s = s0 + lane;
-  if (valid) {
-    const f3 gmin = mk3(u->gridBoundsMin[0], u->gridBoundsMin[1], u->gridBoundsMin[2]);
-    const f3 gmax = mk3(u->gridBoundsMax[0], u->gridBoundsMax[1], u->gridBoundsMax[2]);
-    const f3 gravity = mk3(0.0f, -u->gravity, 0.0f);
-    const f3 windDir = mk3(u->wind[0], u->wind[1], u->wind[2]);
-    const float dt = u->deltaTime;
-    // wind jitter is per strand (:134-139)
-    const float timer = (float)u->frameIdx * 0.73f + u->windPhaseOffset * (float)s;
-    const float jitter = fractf(timer * u->windStrengthFrequency);
-    const float jitterDelta = mixf(-0.5f * u->windStrengthJitter, 0.5f * u->windStrengthJitter, jitter);
-    const float jitterStr = 1.0f + jitterDelta;
-
-    // Verlet integration with grid forces, points 1..P-1 (:125-162)
-    for (uint32_t i = 1; i < P; i++) {
-      const uint32_t o = i * kSimLanes + lane;
-      const float4 pn4 = A[o], pp4 = C[o];
+    const uint32_t o = pt * L + ls;
+    const float4 pn4 = now[gbase + e];
+    Bi[o] = initial[gbase + e];
+    Len[o] = seglen[gbase + e];
+    float4 r = pn4;
+    if (pt >= 1u) {
+      const float4 pp4 = prev[gbase + e];
+      // wind jitter is per strand (:134-139)
+      const float timer = (float)u->frameIdx * 0.73f + u->windPhaseOffset * (float)(s0 + ls);
+      const float jitter = fractf(timer * u->windStrengthFrequency);
+      const float jitterDelta = mixf(-0.5f * u->windStrengthJitter, 0.5f * u->windStrengthJitter, jitter);
+      const float jitterStr = 1.0f + jitterDelta;
       const f3 pn = mk3(pn4.x, pn4.y, pn4.z), pp = mk3(pp4.x, pp4.y, pp4.z);
       const Cell co = grid_cell(d, gmin, gmax, pn);
       f3 grad = mk3(0.0f, 0.0f, 0.0f), vel = mk3(0.0f, 0.0f, 0.0f);
@@ -172,18 +172,18 @@ __global__ __launch_bounds__(kSimLanes) void sim_integrate_kernel(
         for (uint32_t y = co.mn[1]; y <= co.mx[1]; y++)
           for (uint32_t x = co.mn[0]; x <= co.mx[0]; x++) {
             const f3 w = cell_w(x, y, z, co.pin);
-            const float4 g = *reinterpret_cast<const float4*>(&dgw[grid_idx(d, x, y, z)]);
-            grad = grad + w * mk3(g.x, g.y, g.z);
-            wstr = wstr + length(w) * g.w;
+            const float4 gw = *reinterpret_cast<const float4*>(&dgw[grid_idx(d, x, y, z)]);
+            grad = grad + w * mk3(gw.x, gw.y, gw.z);
+            wstr = wstr + length(w) * gw.w;
           }
       // _getGridDensityVelocity (densityVelocityGrid.wgsl.ts:30-68)
       for (uint32_t z = co.mn[2]; z <= co.mx[2]; z++)
         for (uint32_t y = co.mn[1]; y <= co.mx[1]; y++)
           for (uint32_t x = co.mn[0]; x <= co.mx[0]; x++) {
             const f3 w = cell_w(x, y, z, co.pin);
-            const int4 g = *reinterpret_cast<const int4*>(&dv[grid_idx(d, x, y, z)]);
-            vel = vel + w * mk3(grid_decode(g.x), grid_decode(g.y), grid_decode(g.z));
-            dens = dens + length(w) * grid_decode(g.w);
+            const int4 gv = *reinterpret_cast<const int4*>(&dv[grid_idx(d, x, y, z)]);
+            vel = vel + w * mk3(grid_decode(gv.x), grid_decode(gv.y), grid_decode(gv.z));
+            dens = dens + length(w) * grid_decode(gv.w);
           }
       f3 force = gravity;
       const float windCell = satf(mixf(u->windStrengthLull, 1.0f, wstr));
@@ -193,86 +193,96 @@ __global__ __launch_bounds__(kSimLanes) void sim_integrate_kernel(
       // verletIntegration — integration.wgsl.ts:8-28
       const f3 fdisp = mix(pn - pp, gridDisp, u->friction);
       const f3 next = (pn + fdisp) + (force * dt) * dt;
-      const f3 r = mix(pp, next, pn4.w);
-      A[o] = make_float4(r.x, r.y, r.z, pn4.w);
+      const f3 rr = mix(pp, next, pn4.w);
+      r = make_float4(rr.x, rr.y, rr.z, pn4.w);
     }
+    A[o] = r;
+  }
+  __syncthreads();
+  // Local-shape constants depend only on the initial pose (constraints.wgsl.ts:65-82).
+  for (uint32_t e = tid; e < nElem; e += kSimThreads) {
+    const uint32_t ls = e / P, pt = e - ls * P;
+    if (pt + 3u >= P) continue;
+    const uint32_t o = pt * L + ls;
+    const float4 i0 = Bi[o], i1 = Bi[o + L], i2 = Bi[o + 2 * L];
+    const f3 ci = ((mk3(i0.x, i0.y, i0.z) + mk3(i1.x, i1.y, i1.z)) + mk3(i2.x, i2.y, i2.z)) / 3.0f;
+    const float h0 = length(ci - mk3(i1.x, i1.y, i1.z));
+    const float wT = i0.w + 2.0f * i1.w + i2.w;
+    K[o] = make_float4(h0, i0.w / wT * 2.0f, i1.w / wT * -4.0f, i2.w / wT * 2.0f);
+  }
 
-    // Local-shape constants depend only on the initial pose (constraints.wgsl.ts:65-82).
-    for (uint32_t j = 0; j + 3 < P; j++) {
-      const float4 i0 = initial[gbase + (size_t)lane * P + j];
-      const float4 i1 = initial[gbase + (size_t)lane * P + j + 1];
-      const float4 i2 = initial[gbase + (size_t)lane * P + j + 2];
-      const f3 ci = ((mk3(i0.x, i0.y, i0.z) + mk3(i1.x, i1.y, i1.z)) + mk3(i2.x, i2.y, i2.z)) / 3.0f;
-      const float h0 = length(ci - mk3(i1.x, i1.y, i1.z));
-      const float wT = i0.w + 2.0f * i1.w + i2.w;
-      C[j * kSimLanes + lane] = make_float4(h0, i0.w / wT * 2.0f, i1.w / wT * -4.0f, i2.w / wT * 2.0f);
-    }
+  const float iters = (float)u->constraintIterations;
+  const float kLen = u->stiffnessLengthConstr;
+  const float kGlobal = u->stiffnessGlobalConstr / iters;
+  const float kLocal = u->stiffnessLocalConstr / iters;
+  const float kColl = u->stiffnessCollisions / iters;
+  const float kSdf = u->stiffnessSDF / iters;
+  const f4 sphere = mk4(u->collisionSphere[0], u->collisionSphere[1], u->collisionSphere[2], u->collisionSphere[3]);
+  const float sdfOffset = u->sdfBoundsMax[3];
+  SdfView sv{sdf, sdf_dims, mk3(u->sdfBoundsMin[0], u->sdfBoundsMin[1], u->sdfBoundsMin[2]),
+             mk3(u->sdfBoundsMax[0], u->sdfBoundsMax[1], u->sdfBoundsMax[2])};
+  const float invPm1 = (float)(P - 1u);
+  const bool chain = tid < nStr;  // the sweeps: one strand per lane of wave 0
+  const uint32_t lane = tid;
 
-    const float iters = (float)u->constraintIterations;
-    const float kLen = u->stiffnessLengthConstr;
-    const float kGlobal = u->stiffnessGlobalConstr / iters;
-    const float kLocal = u->stiffnessLocalConstr / iters;
-    const float kColl = u->stiffnessCollisions / iters;
-    const float kSdf = u->stiffnessSDF / iters;
-    const f4 sphere = mk4(u->collisionSphere[0], u->collisionSphere[1], u->collisionSphere[2], u->collisionSphere[3]);
-    const float sdfOffset = u->sdfBoundsMax[3];
-    SdfView sv{sdf, sdf_dims, mk3(u->sdfBoundsMin[0], u->sdfBoundsMin[1], u->sdfBoundsMin[2]),
-               mk3(u->sdfBoundsMax[0], u->sdfBoundsMax[1], u->sdfBoundsMax[2])};
-    const float invPm1 = (float)(P - 1u);
-
-    for (uint32_t it = 0; it < u->constraintIterations; it++) {
-      // length constraint — constraints.wgsl.ts:5-28, hairSimIntegrationPass.wgsl.ts:176-190
+  for (uint32_t it = 0; it < u->constraintIterations; it++) {
+    __syncthreads();
+    // length constraint — constraints.wgsl.ts:5-28, hairSimIntegrationPass.wgsl.ts:176-190
+    if (chain) {
       float4 p0 = A[lane];
       for (uint32_t j = 0; j + 1 < P; j++) {
-        float4 p1 = A[(j + 1) * kSimLanes + lane];
-        const float expected = B[j * kSimLanes + lane].w;
+        float4 p1 = A[(j + 1) * L + lane];
+        const float expected = Len[j * L + lane];
         const f3 tg = mk3(p1.x - p0.x, p1.y - p0.y, p1.z - p0.z);
         const float corr = 1.0f - expected / length(tg);
         const float df = corr * kLen / (p0.w + p1.w + kEps);
         const f3 delta = tg * df;
         p0 = make_float4(p0.x + delta.x * p0.w, p0.y + delta.y * p0.w, p0.z + delta.z * p0.w, p0.w);
         p1 = make_float4(p1.x - delta.x * p1.w, p1.y - delta.y * p1.w, p1.z - delta.z * p1.w, p1.w);
-        A[j * kSimLanes + lane] = p0;
+        A[j * L + lane] = p0;
         p0 = p1;
       }
-      A[(P - 1) * kSimLanes + lane] = p0;
-
-      // global shape — constraints.wgsl.ts:31-49
-      for (uint32_t j = 1; j < P; j++) {
-        const float x = (float)j / invPm1;
-        const float att = 1.0f - satf((x - u->globalConstrExtent) / u->globalConstrFade);
-        const float k = kGlobal * att;
-        const uint32_t o = j * kSimLanes + lane;
-        const float4 p = A[o];
-        const float4 ini = B[o];
-        const f3 dvv = mk3(ini.x - p.x, ini.y - p.y, ini.z - p.z);
-        A[o] = make_float4(p.x + (dvv.x * p.w) * k, p.y + (dvv.y * p.w) * k, p.z + (dvv.z * p.w) * k, p.w);
-      }
-
+      A[(P - 1) * L + lane] = p0;
+    }
+    __syncthreads();
+    // global shape — constraints.wgsl.ts:31-49
+    for (uint32_t e = tid; e < nElem; e += kSimThreads) {
+      const uint32_t ls = e / P, pt = e - ls * P;
+      if (pt == 0u) continue;
+      const uint32_t o = pt * L + ls;
+      const float x = (float)pt / invPm1;
+      const float att = 1.0f - satf((x - u->globalConstrExtent) / u->globalConstrFade);
+      const float k = kGlobal * att;
+      const float4 p = A[o];
+      const float4 ini = Bi[o];
+      const f3 dvv = mk3(ini.x - p.x, ini.y - p.y, ini.z - p.z);
+      A[o] = make_float4(p.x + (dvv.x * p.w) * k, p.y + (dvv.y * p.w) * k, p.z + (dvv.z * p.w) * k, p.w);
+    }
+    __syncthreads();
+    if (chain) {
       // local shape (Kelager10 triangle bending) — constraints.wgsl.ts:54-87
-      float4 a = A[lane], b = A[kSimLanes + lane];
+      float4 a = A[lane], b = A[L + lane];
       uint32_t jj = 0;
       for (; jj + 3 < P; jj++) {
-        const float4 c = A[(jj + 2) * kSimLanes + lane];
-        const float4 K = C[jj * kSimLanes + lane];
+        const float4 c = A[(jj + 2) * L + lane];
+        const float4 Kc = K[jj * L + lane];
         const f3 cc = ((mk3(a.x, a.y, a.z) + mk3(b.x, b.y, b.z)) + mk3(c.x, c.y, c.z)) / 3.0f;
         const f3 hv = mk3(b.x, b.y, b.z) - cc;
         const float h = length(hv);
-        const f3 delta = hv * (1.0f - K.x / h);
-        const float k0 = kLocal * K.y, k1 = kLocal * K.z, k2 = kLocal * K.w;
-        A[jj * kSimLanes + lane] = make_float4(a.x + delta.x * k0, a.y + delta.y * k0, a.z + delta.z * k0, a.w);
+        const f3 delta = hv * (1.0f - Kc.x / h);
+        const float k0 = kLocal * Kc.y, k1 = kLocal * Kc.z, k2 = kLocal * Kc.w;
+        A[jj * L + lane] = make_float4(a.x + delta.x * k0, a.y + delta.y * k0, a.z + delta.z * k0, a.w);
         a = make_float4(b.x + delta.x * k1, b.y + delta.y * k1, b.z + delta.z * k1, b.w);
         b = make_float4(c.x + delta.x * k2, c.y + delta.y * k2, c.z + delta.z * k2, c.w);
       }
-      A[jj * kSimLanes + lane] = a;
-      A[(jj + 1) * kSimLanes + lane] = b;
-
+      A[jj * L + lane] = a;
+      A[(jj + 1) * L + lane] = b;
       // tip tangent match — constraints.wgsl.ts:90-120
-      float4 pa = A[(jj - 1) * kSimLanes + lane];
+      float4 pa = A[(jj - 1) * L + lane];
       for (; jj < P; jj++) {
-        const uint32_t o = jj * kSimLanes + lane;
+        const uint32_t o = jj * L + lane;
         const float4 pb = A[o];
-        const float4 i0 = B[(jj - 1) * kSimLanes + lane], i1 = B[o];
+        const float4 i0 = Bi[o - L], i1 = Bi[o];
         const f3 tgi = mk3(i1.x - i0.x, i1.y - i0.y, i1.z - i0.z);
         const f3 proj = mk3(pa.x, pa.y, pa.z) + tgi;
         const f3 delta = proj - mk3(pb.x, pb.y, pb.z);
@@ -280,36 +290,38 @@ __global__ __launch_bounds__(kSimLanes) void sim_integrate_kernel(
         pa = make_float4(pb.x + delta.x * k, pb.y + delta.y * k, pb.z + delta.z * k, pb.w);
         A[o] = pa;
       }
-
-      // sphere + SDF collisions — collisions.wgsl.ts:4-61, :237-261
-      for (uint32_t j = 1; j < P; j++) {
-        const uint32_t o = j * kSimLanes + lane;
-        const float4 p4 = A[o];
-        f3 p = mk3(p4.x, p4.y, p4.z);
-        const f3 v = p - xyz(sphere);
-        const float toShell = fmx(0.0f, -(length(v) - sphere.w));
-        p = p + (normalize(v) * toShell) * kColl;
-        if (j > 2u) {
-          const float sd = sample_sdf(sv, p);
-          const float sdo = sd + sdfOffset;
-          if (!(sdo >= 0.0f)) {
-            f3 cd;
-            cd.x = sample_sdf(sv, p + mk3(kSdfGradOffset, 0.0f, 0.0f));
-            cd.y = sample_sdf(sv, p + mk3(0.0f, kSdfGradOffset, 0.0f));
-            cd.z = sample_sdf(sv, p + mk3(0.0f, 0.0f, kSdfGradOffset));
-            cd = cd - mk3(sd, sd, sd);
-            p = p + (normalize(cd) * -sdo) * kSdf;
-          }
+    }
+    __syncthreads();
+    // sphere + SDF collisions — collisions.wgsl.ts:4-61, :237-261
+    for (uint32_t e = tid; e < nElem; e += kSimThreads) {
+      const uint32_t ls = e / P, pt = e - ls * P;
+      if (pt == 0u) continue;
+      const uint32_t o = pt * L + ls;
+      const float4 p4 = A[o];
+      f3 p = mk3(p4.x, p4.y, p4.z);
+      const f3 v = p - xyz(sphere);
+      const float toShell = fmx(0.0f, -(length(v) - sphere.w));
+      p = p + (normalize(v) * toShell) * kColl;
+      if (pt > 2u) {
+        const float sd = sample_sdf(sv, p);
+        const float sdo = sd + sdfOffset;
+        if (!(sdo >= 0.0f)) {
+          f3 cd;
+          cd.x = sample_sdf(sv, p + mk3(kSdfGradOffset, 0.0f, 0.0f));
+          cd.y = sample_sdf(sv, p + mk3(0.0f, kSdfGradOffset, 0.0f));
+          cd.z = sample_sdf(sv, p + mk3(0.0f, 0.0f, kSdfGradOffset));
+          cd = cd - mk3(sd, sd, sd);
+          p = p + (normalize(cd) * -sdo) * kSdf;
         }
-        A[o] = make_float4(p.x, p.y, p.z, p4.w);
       }
+      A[o] = make_float4(p.x, p.y, p.z, p4.w);
     }
   }
   __syncthreads();
   // coalesced write-back into the PREV buffer (:268-272)
-  for (uint32_t e = lane; e < nElem; e += kSimLanes) {
+  for (uint32_t e = tid; e < nElem; e += kSimThreads) {
     const uint32_t ls = e / P, pt = e - ls * P;
-    prev[gbase + e] = A[pt * kSimLanes + ls];
+    prev[gbase + e] = A[pt * kSimStrands + ls];
   }
 }
 
@@ -421,28 +433,15 @@ int fh_sim_integrate(fh_ctx* ctx, uint32_t grid_dims, const fh_sim_uniforms* sim
                  density_velocity && density_grad_wind && positions_initial,
              "fh_sim_integrate: null buffer");
   if (strands_count == 0) return FH_OK;
-  // 3 float4 per point per strand in LDS; 64 strands per workgroup when that
-  // fits the device's per-workgroup LDS (160 KiB on gfx950), else 32.
-  const size_t lds64 = (size_t)3 * points_per_strand * 64 * sizeof(float4);
-  FH_REQUIRE(lds64 / 2 <= ctx->max_lds, "fh_sim_integrate: points_per_strand too large for this device's LDS");
-  if (lds64 <= ctx->max_lds) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&sim_integrate_kernel<64>),
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds64);
-    sim_integrate_kernel<64><<<fh::div_ceil(strands_count, 64), 64, lds64, ctx->stream>>>(
-        grid_dims, sim_uniforms, reinterpret_cast<float4*>(positions_prev),
-        reinterpret_cast<const float4*>(positions_now), segment_lengths, sdf_texture, sdf_dims,
-        density_velocity, density_grad_wind, reinterpret_cast<const float4*>(positions_initial),
-        strands_count, points_per_strand);
-  } else {
-    const size_t lds32 = lds64 / 2;
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&sim_integrate_kernel<32>),
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds32);
-    sim_integrate_kernel<32><<<fh::div_ceil(strands_count, 32), 32, lds32, ctx->stream>>>(
-        grid_dims, sim_uniforms, reinterpret_cast<float4*>(positions_prev),
-        reinterpret_cast<const float4*>(positions_now), segment_lengths, sdf_texture, sdf_dims,
-        density_velocity, density_grad_wind, reinterpret_cast<const float4*>(positions_initial),
-        strands_count, points_per_strand);
-  }
+  const size_t lds = (size_t)points_per_strand * kSimStrands * (3 * sizeof(float4) + sizeof(float));
+  FH_REQUIRE(lds <= ctx->max_lds, "fh_sim_integrate: points_per_strand too large for this device's LDS");
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&sim_integrate_kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  sim_integrate_kernel<<<fh::div_ceil(strands_count, kSimStrands), kSimThreads, lds, ctx->stream>>>(
+      grid_dims, sim_uniforms, reinterpret_cast<float4*>(positions_prev),
+      reinterpret_cast<const float4*>(positions_now), segment_lengths, sdf_texture, sdf_dims,
+      density_velocity, density_grad_wind, reinterpret_cast<const float4*>(positions_initial),
+      strands_count, points_per_strand);
   return fh::check_launch("sim_integrate_kernel");
 }
 
