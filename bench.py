@@ -6,9 +6,10 @@ A step is one frame of the chain on the 1080p character-scale workload
 pre-pass, integration, particle-to-grid), then the tile pass, the tile sort,
 the fine pass and the shading pass, with inputs resident in HBM.
 
-Prints one JSON line. `roofline` is for the stage that takes the most time,
-from its algorithmic bytes per frame and its average duration measured with
-HIP events on the library's stream inside the timed region. `cpu_baseline`
+Prints one JSON line. `roofline` is for the kernel that takes the most time,
+from its algorithmic bytes per launch and its average launch duration, both
+measured with HIP events that the library records around each of its kernels
+on its own stream inside the timed region. `cpu_baseline`
 is the CPU oracle (a scalar C restatement of the reference's kernels) timed
 on full frames of the same workload, rank 0 only.
 """
@@ -29,9 +30,34 @@ METRIC = "sim+raster frames/s at 1080p, batched vs plain; p50/p99 frame latency"
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(wl, n_pairs: int, tiles: int) -> dict:
-    """Compulsory HBM bytes per frame per stage (each distinct input byte read
-    once, each output byte written once)."""
+def kernel_bytes(wl, n_pairs: int, tiles: int) -> dict:
+    """Compulsory HBM bytes of one launch of each main kernel: every distinct
+    input byte read once, every output byte written once (re-reads, atomics
+    and scratch excluded)."""
+    N = wl.strands * wl.points
+    d3 = wl.sim.gridDims ** 3
+    V = wl.sdf_dims ** 3
+    WH = wl.width * wl.height
+    B = wl.render.tileDepthBins
+    S, Ps = wl.strands, wl.render.shadingPoints
+    return {
+        # prev + now + initial (16 B each), rest length, write of the new position
+        "sim_integrate": 68 * N + 32 * d3 + 4 * V,
+        "grid_pre_sim": 32 * d3 + 4 * V,
+        "grid_post_sim": 32 * N + 16 * d3,
+        # positions + tangents, mesh depth, list element + slot tag per pair, tile headers
+        "tiles_draw": 32 * N + 4 * WH + 12 * n_pairs + 16 * tiles * B,
+        "bin_scatter": 12 * n_pairs + 4 * tiles * B,
+        "bin_sort": 8 * n_pairs + 20 * tiles * B,
+        # tile headers, sorted keys, positions + tangents, mesh depth, shading
+        # points, framebuffer read + write, tile list
+        "fine_raster": 16 * tiles * B + 4 * n_pairs + 32 * N + 4 * WH + 16 * S * Ps + 32 * WH + 4 * tiles,
+        "shading": 32 * N + 4 * WH + WH // 2 + 2048 * 2048 * 4 + 16 * S * Ps,
+    }
+
+
+def stage_bytes(wl, n_pairs: int, tiles: int) -> dict:
+    """Compulsory HBM bytes per frame per stage."""
     N = wl.strands * wl.points
     d = wl.sim.gridDims
     V = wl.sdf_dims ** 3
@@ -47,7 +73,19 @@ def algorithmic_bytes(wl, n_pairs: int, tiles: int) -> dict:
     }
 
 
-def cpu_baseline(wl, r, frames: int = 3) -> dict:
+def read_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary committed under profiles/ (None when absent)."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            return int(float(line.split()[1]))
+    return None
+
+
+def cpu_baseline(wl, r, frames: int = 8) -> dict:
     """Times the oracle on `frames` full frames of the workload on one core."""
     import __graft_entry__ as G
     O = G.load_oracle()
@@ -164,12 +202,16 @@ def main() -> None:
     torch.cuda.synchronize()
 
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(args.steps)]
+    # HIP events around every kernel the library launches, on its stream
+    r.native.profiler_enable(64 * args.steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
         frame(evs[k])
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    kernels = r.native.profiler_read()
+    r.native.profiler_enable(0)
 
     per_frame = np.array([evs[k][0].elapsed_time(evs[k][5]) for k in range(args.steps)])  # ms
     stage_ms = {s: float(np.mean([evs[k][i].elapsed_time(evs[k][i + 1]) for k in range(args.steps)]))
@@ -178,9 +220,11 @@ def main() -> None:
     tiles = (wl.width + wl.render.tileSize - 1) // wl.render.tileSize * \
             ((wl.height + wl.render.tileSize - 1) // wl.render.tileSize)
     drawn_tiles = int(r.sortPass.tileListBuffer[0].item())
-    ab = algorithmic_bytes(wl, n_pairs, tiles)
-    dom = max(stage_ms, key=stage_ms.get)
-    achieved = ab[dom] / (stage_ms[dom] * 1e-3) / 1e9
+    kb = kernel_bytes(wl, n_pairs, tiles)
+    kernel_ms = {k: t / n for k, (n, t) in kernels.items()}
+    dom = max((k for k in kernel_ms if k in kb), key=lambda k: kernels[k][1])
+    achieved = kb[dom] / (kernel_ms[dom] * 1e-3) / 1e9
+    traffic = read_traffic(dom, wl.name)
     img = r.finePass.image(wl.width, wl.height)
     assert torch.isfinite(img).all()
 
@@ -206,12 +250,14 @@ def main() -> None:
         "latency_ms": {"p50": round(float(np.percentile(per_frame, 50)), 4),
                        "p99": round(float(np.percentile(per_frame, 99)), 4)},
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "stage_hbm_GBps": {k: round(v / (stage_ms[k] * 1e-3) / 1e9, 2) for k, v in stage_bytes(wl, n_pairs, tiles).items()},
         "throughput": {"particle_steps_per_s": round(wl.strands * wl.points * 1e3 / ms, 1),
                        "segments_rasterized_per_s": round(wl.strands * (wl.points - 1) * 1e3 / ms, 1),
                        "segment_tile_pairs": n_pairs, "drawn_tiles": drawn_tiles},
-        "roofline": {"bound": "hbm", "stage": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "algorithmic_bytes": int(ab[dom])},
+        "kernel_us": {k: round(v * 1e3, 2) for k, v in sorted(kernel_ms.items(), key=lambda kv: -kernels[kv[0]][1])},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "algorithmic_bytes": int(kb[dom]), "avg_launch_us": round(kernel_ms[dom] * 1e3, 2)},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, r)

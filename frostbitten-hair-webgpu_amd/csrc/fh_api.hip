@@ -44,11 +44,25 @@ int ensure_scratch(fh_ctx* ctx, size_t bytes) {
   return FH_OK;
 }
 
+int prof_begin(fh_ctx* ctx, const char* kernel) {
+  if (ctx->prof_names.size() >= ctx->prof_cap) return -1;
+  const int k = (int)ctx->prof_names.size();
+  if (hipEventRecord(ctx->prof_events[2 * k], ctx->stream) != hipSuccess) return -1;
+  ctx->prof_names.push_back(kernel);
+  return k;
+}
+
+void prof_end(fh_ctx* ctx, int k) {
+  if (k >= 0) (void)hipEventRecord(ctx->prof_events[2 * k + 1], ctx->stream);
+}
+
 int fill_u32(fh_ctx* ctx, void* dst, uint64_t n_words, uint32_t value) {
   if (n_words == 0) return FH_OK;
   uint64_t blocks = (n_words + 255) / 256;
   if (blocks > 4096) blocks = 4096;
+  const int pk0 = fh::prof_begin(ctx, "fill_u32");
   fill_u32_kernel<<<(unsigned)blocks, 256, 0, ctx->stream>>>((uint32_t*)dst, n_words, value);
+  fh::prof_end(ctx, pk0);
   return check_launch("fill_u32_kernel");
 }
 
@@ -91,6 +105,7 @@ int fh_destroy(fh_ctx* ctx) {
   if (!ctx) return FH_OK;
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  for (hipEvent_t e : ctx->prof_events) hipEventDestroy(e);
   if (ctx->scratch) hipFree(ctx->scratch);
   if (ctx->owns_stream && ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -109,6 +124,45 @@ int fh_set_stream(fh_ctx* ctx, void* stream) {
 }
 
 void* fh_get_stream(fh_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int fh_profiler_enable(fh_ctx* ctx, uint32_t max_launches) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(max_launches <= (1u << 20), "fh_profiler_enable: max_launches must be <= 2^20");
+  FH_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->prof_names.clear();
+  while (ctx->prof_events.size() < 2ull * max_launches) {
+    hipEvent_t e;
+    FH_HIP(hipEventCreate(&e));
+    ctx->prof_events.push_back(e);
+  }
+  ctx->prof_names.reserve(max_launches);
+  ctx->prof_cap = max_launches;
+  return FH_OK;
+}
+
+int fh_profiler_read(fh_ctx* ctx, fh_kernel_time* out, uint32_t max_out, uint32_t* n_out) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(n_out != nullptr && (out != nullptr || max_out == 0), "fh_profiler_read: null output");
+  FH_HIP(hipStreamSynchronize(ctx->stream));
+  uint32_t n = 0;
+  for (size_t k = 0; k < ctx->prof_names.size(); k++) {
+    float ms = 0.0f;
+    FH_HIP(hipEventElapsedTime(&ms, ctx->prof_events[2 * k], ctx->prof_events[2 * k + 1]));
+    uint32_t i = 0;
+    while (i < n && std::strncmp(out[i].name, ctx->prof_names[k], sizeof(out[i].name)) != 0) i++;
+    if (i == n) {
+      if (n == max_out) continue;
+      std::memset(&out[n], 0, sizeof(out[n]));
+      std::strncpy(out[n].name, ctx->prof_names[k], sizeof(out[n].name) - 1);
+      n++;
+    }
+    out[i].launches++;
+    out[i].total_ms += (double)ms;
+  }
+  ctx->prof_names.clear();
+  *n_out = n;
+  return FH_OK;
+}
 
 int fh_sync(fh_ctx* ctx) {
   FH_CHECK_CTX(ctx);

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/fh_hair.h"
 
@@ -18,6 +19,10 @@ struct fh_ctx {
   // per-bin offsets, prefix-scan temporaries).
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
+  // Kernel timing (fh_profiler_enable): an event pair per timed launch.
+  std::vector<hipEvent_t> prof_events;
+  std::vector<const char*> prof_names;  // kernel of each timed launch
+  uint32_t prof_cap = 0;                // 0: timing off
 };
 
 namespace fh {
@@ -27,6 +32,11 @@ int fail(int code, const std::string& msg);
 int check_launch(const char* what);
 // Ensure ctx->scratch holds at least `bytes`; frees and reallocates if not.
 int ensure_scratch(fh_ctx* ctx, size_t bytes);
+// Kernel timing: prof_begin records the start event of the next launch on
+// the context's stream and returns its index (-1 when timing is off or the
+// event pool is used up); prof_end records the matching end event.
+int prof_begin(fh_ctx* ctx, const char* kernel);
+void prof_end(fh_ctx* ctx, int k);
 
 inline uint32_t div_ceil(uint32_t a, uint32_t b) { return (a + b - 1u) / b; }
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }

@@ -1092,11 +1092,15 @@ int fh_tiles_draw(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t
   a.slotKey = slot_tile_bin;
   a.T = cfg->tileSize; a.B = cfg->tileDepthBins; a.threshold = cfg->invalidTilesPerSegmentThreshold;
   a.S = rendered_strands; a.P = points_per_strand;
+  const int pk0 = fh::prof_begin(ctx, "tiles_draw");
   tiles_draw_kernel<<<fh::div_ceil((uint32_t)nSeg, 64), 64, 0, ctx->stream>>>(a);
+  fh::prof_end(ctx, pk0);
   if ((rc = check_launch("tiles_draw_kernel"))) return rc;
   const uint32_t nTiles = fh_tile_count(cfg, vp_w, vp_h);
+  const int pk1 = fh::prof_begin(ctx, "tile_counts");
   tile_counts_kernel<<<fh::div_ceil(nTiles, 256), 256, 0, ctx->stream>>>(tiles, nTiles, cfg->tileDepthBins,
                                                                        segment_count_per_tile);
+  fh::prof_end(ctx, pk1);
   return check_launch("tile_counts_kernel");
 }
 
@@ -1108,9 +1112,11 @@ int fh_tiles_rebuild_slot_keys(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t v
   if (rc) return rc;
   FH_REQUIRE(tiles && tile_segments && slot_tile_bin, "fh_tiles_rebuild_slot_keys: null buffer");
   const uint32_t nTB = fh_tile_count(cfg, vp_w, vp_h) * cfg->tileDepthBins;
+  const int pk2 = fh::prof_begin(ctx, "rebuild_keys");
   rebuild_keys_kernel<<<fh::div_ceil(nTB, 256), 256, 0, ctx->stream>>>(
       tiles, reinterpret_cast<const fh_list_element*>(reinterpret_cast<const char*>(tile_segments) + 4),
       segments_capacity, nTB, slot_tile_bin);
+  fh::prof_end(ctx, pk2);
   return check_launch("rebuild_keys_kernel");
 }
 
@@ -1132,12 +1138,16 @@ int fh_tiles_sort(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t
   const uint32_t nt = fh_tile_count(cfg, vp_w, vp_h);
   if (nt == 0) return FH_OK;
   const uint32_t blocks = fh::div_ceil(nt, 256);
+  const int pk3 = fh::prof_begin(ctx, "sort_count");
   sort_count_kernel<<<blocks, 256, 0, ctx->stream>>>(segment_count_per_tile, nt, cfg->sortBuckets,
                                                        cfg->sortBucketSize, buckets);
+  fh::prof_end(ctx, pk3);
   if ((rc = check_launch("sort_count_kernel"))) return rc;
+  const int pk4 = fh::prof_begin(ctx, "sort_scatter");
   sort_scatter_kernel<<<blocks, 256, 0, ctx->stream>>>(segment_count_per_tile, nt, cfg->sortBuckets,
                                                          cfg->sortBucketSize, buckets,
                                                          reinterpret_cast<uint32_t*>(tile_list));
+  fh::prof_end(ctx, pk4);
   return check_launch("sort_scatter_kernel");
 }
 
@@ -1183,18 +1193,26 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
   const fh_list_element* data =
       reinterpret_cast<const fh_list_element*>(reinterpret_cast<const char*>(tile_segments) + 4);
 
+  const int pk5 = fh::prof_begin(ctx, "bin_scan_local");
   bin_scan_local_kernel<<<scanBlocks, kScanBlock, 0, ctx->stream>>>(tiles, nTB, ends, sums);
+  fh::prof_end(ctx, pk5);
   if ((rc = check_launch("bin_scan_local_kernel"))) return rc;
   if (scanBlocks > 1) {
+    const int pk6 = fh::prof_begin(ctx, "bin_scan_add");
     bin_scan_add_kernel<<<scanBlocks, kScanBlock, 0, ctx->stream>>>(nTB, ends, sums);
+    fh::prof_end(ctx, pk6);
     if ((rc = check_launch("bin_scan_add_kernel"))) return rc;
   }
   const uint32_t scatterBlocks = std::max(1u, std::min(fh::div_ceil(limit, 256), (uint32_t)ctx->num_cus * 8u));
+  const int pk7 = fh::prof_begin(ctx, "bin_scatter");
   bin_scatter_kernel<<<scatterBlocks, 256, 0, ctx->stream>>>(
       uniforms, reinterpret_cast<const uint32_t*>(tile_segments), data, limit, slot_tile_bin, ends, nTB, binSeg);
+  fh::prof_end(ctx, pk7);
   if ((rc = check_launch("bin_scatter_kernel"))) return rc;
+  const int pk8 = fh::prof_begin(ctx, "bin_sort");
   bin_sort_kernel<<<std::min(nTB, (uint32_t)ctx->num_cus * 8u), 256, 0, ctx->stream>>>(tiles, ends, nTB, limit,
                                                                                        binSeg, big);
+  fh::prof_end(ctx, pk8);
   if ((rc = check_launch("bin_sort_kernel"))) return rc;
 
   FineArgs a;
@@ -1230,15 +1248,21 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
   if (T == 8) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&fine_raster_kernel<8>),
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const int pk9 = fh::prof_begin(ctx, "fine_raster");
     fine_raster_kernel<8><<<grid, 64, lds, ctx->stream>>>(a);
+    fh::prof_end(ctx, pk9);
   } else if (T == 16) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&fine_raster_kernel<16>),
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const int pk10 = fh::prof_begin(ctx, "fine_raster");
     fine_raster_kernel<16><<<grid, 256, lds, ctx->stream>>>(a);
+    fh::prof_end(ctx, pk10);
   } else {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&fine_raster_kernel<32>),
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const int pk11 = fh::prof_begin(ctx, "fine_raster");
     fine_raster_kernel<32><<<grid, 1024, lds, ctx->stream>>>(a);
+    fh::prof_end(ctx, pk11);
   }
   return check_launch("fine_raster_kernel");
 }
@@ -1253,9 +1277,11 @@ int fh_debug_raster_barycentric(fh_ctx* ctx, const fh_render_uniforms* uniforms,
   FH_REQUIRE(points_per_strand >= 2, "fh_debug_raster_barycentric: points_per_strand >= 2");
   const uint32_t nSeg = strands_count * (points_per_strand - 1u);
   if (nSeg == 0) return FH_OK;
+  const int pk12 = fh::prof_begin(ctx, "debug_raster");
   debug_raster_kernel<<<fh::div_ceil(nSeg, 64), 64, 0, ctx->stream>>>(
       uniforms, reinterpret_cast<const float4*>(positions), reinterpret_cast<const float4*>(tangents),
       strands_count, points_per_strand, out);
+  fh::prof_end(ctx, pk12);
   return check_launch("debug_raster_kernel");
 }
 

@@ -211,6 +211,8 @@ extern "C" int fh_shading(fh_ctx* ctx, const fh_render_cfg* cfg, const fh_render
   a.ao = ao.data; a.aoW = ao.width; a.aoH = ao.height;
   a.depth = depth.data; a.depthW = depth.width; a.depthH = depth.height;
   a.S = strands_count; a.P = points_per_strand; a.PS = cfg->shadingPoints;
+  const int pk0 = fh::prof_begin(ctx, "shading");
   shading_kernel<<<fh::div_ceil((uint32_t)n, 256), 256, 0, ctx->stream>>>(a);
+  fh::prof_end(ctx, pk0);
   return fh::check_launch("shading_kernel");
 }

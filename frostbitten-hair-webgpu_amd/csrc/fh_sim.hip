@@ -410,8 +410,10 @@ int fh_grid_pre_sim(fh_ctx* ctx, uint32_t grid_dims, const fh_sim_uniforms* sim_
   FH_REQUIRE(sim_uniforms && density_velocity && density_grad_wind && sdf_texture,
              "fh_grid_pre_sim: null buffer");
   const uint32_t n = grid_dims * grid_dims * grid_dims;
+  const int pk0 = fh::prof_begin(ctx, "grid_pre_sim");
   grid_pre_sim_kernel<<<fh::div_ceil(n, 256), 256, 0, ctx->stream>>>(
       grid_dims, sim_uniforms, density_velocity, density_grad_wind, sdf_texture, sdf_dims);
+  fh::prof_end(ctx, pk0);
   return fh::check_launch("grid_pre_sim_kernel");
 }
 
@@ -437,11 +439,13 @@ int fh_sim_integrate(fh_ctx* ctx, uint32_t grid_dims, const fh_sim_uniforms* sim
   FH_REQUIRE(lds <= ctx->max_lds, "fh_sim_integrate: points_per_strand too large for this device's LDS");
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&sim_integrate_kernel),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const int pk1 = fh::prof_begin(ctx, "sim_integrate");
   sim_integrate_kernel<<<fh::div_ceil(strands_count, kSimStrands), kSimThreads, lds, ctx->stream>>>(
       grid_dims, sim_uniforms, reinterpret_cast<float4*>(positions_prev),
       reinterpret_cast<const float4*>(positions_now), segment_lengths, sdf_texture, sdf_dims,
       density_velocity, density_grad_wind, reinterpret_cast<const float4*>(positions_initial),
       strands_count, points_per_strand);
+  fh::prof_end(ctx, pk1);
   return fh::check_launch("sim_integrate_kernel");
 }
 
@@ -460,9 +464,11 @@ int fh_grid_post_sim(fh_ctx* ctx, uint32_t grid_dims, const fh_sim_uniforms* sim
   const uint64_t n = (uint64_t)strands_count * points_per_strand;
   FH_REQUIRE(n < (1ull << 31), "fh_grid_post_sim: too many points");
   if (n == 0) return FH_OK;
+  const int pk2 = fh::prof_begin(ctx, "grid_post_sim");
   grid_post_sim_kernel<<<fh::div_ceil((uint32_t)n, kP2GPoints), kP2GThreads, 0, ctx->stream>>>(
       grid_dims, sim_uniforms, reinterpret_cast<const float4*>(positions_prev),
       reinterpret_cast<const float4*>(positions_now), density_velocity, (uint32_t)n);
+  fh::prof_end(ctx, pk2);
   return fh::check_launch("grid_post_sim_kernel");
 }
 

@@ -36,6 +36,11 @@ class Tex2D(C.Structure):
     _fields_ = [("data", vp), ("width", u32), ("height", u32)]
 
 
+class KernelTime(C.Structure):
+    """struct fh_kernel_time"""
+    _fields_ = [("name", C.c_char * 48), ("launches", u32), ("reserved", u32), ("total_ms", C.c_double)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "fh_create": (i32, [i32, C.POINTER(vp)]),
@@ -67,6 +72,8 @@ SIGNATURES = {
                              vp, vp, u32, u32]),
     "fh_shading": (i32, [vp, C.POINTER(RenderCfg), vp, vp, vp, vp, vp, Tex2D, Tex2D, Tex2D, u32, u32]),
     "fh_debug_raster_barycentric": (i32, [vp, vp, vp, vp, u32, u32, vp]),
+    "fh_profiler_enable": (i32, [vp, u32]),
+    "fh_profiler_read": (i32, [vp, C.POINTER(KernelTime), u32, C.POINTER(u32)]),
 }
 
 _lib = None
@@ -127,6 +134,17 @@ class Context:
 
     def sync(self) -> None:
         check(self.lib.fh_sync(self.handle), "fh_sync")
+
+    def profiler_enable(self, max_launches: int) -> None:
+        """Times the next `max_launches` kernel launches with HIP events (0 stops)."""
+        check(self.lib.fh_profiler_enable(self.handle, max_launches), "fh_profiler_enable")
+
+    def profiler_read(self) -> dict:
+        """{kernel name: (launches, total ms)} of the launches timed since the last read."""
+        out = (KernelTime * 64)()
+        n = u32(0)
+        check(self.lib.fh_profiler_read(self.handle, out, 64, C.byref(n)), "fh_profiler_read")
+        return {out[i].name.decode(): (out[i].launches, out[i].total_ms) for i in range(n.value)}
 
     def close(self) -> None:
         if self.handle:

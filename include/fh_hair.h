@@ -390,6 +390,28 @@ int fh_shading(fh_ctx* ctx, const fh_render_cfg* cfg,
                uint32_t strands_count, uint32_t points_per_strand);
 
 /* ------------------------------------------------------------------------ */
+/* Kernel timing                                                             */
+/* ------------------------------------------------------------------------ */
+
+/* Per-kernel GPU time, the role of the reference's per-pass timestamp
+ * queries (src/gpuProfiler.ts:7-60, BEGIN/END timestamps per pass). */
+typedef struct fh_kernel_time {
+  char name[48];      /* kernel name, e.g. "fine_raster" */
+  uint32_t launches;  /* launches timed since the last read */
+  uint32_t reserved;
+  double total_ms;    /* summed duration of those launches */
+} fh_kernel_time;
+
+/* max_launches > 0 starts timing: each kernel the library launches on this
+ * context is bracketed by two HIP events on the context's stream, for the
+ * next max_launches launches (later ones are not timed). 0 stops it. Waits
+ * for the stream; the events are created here, not per launch. */
+int fh_profiler_enable(fh_ctx* ctx, uint32_t max_launches);
+/* Waits for the stream, sums the timed launches per kernel name into
+ * out[0 .. *n_out) (at most max_out names) and forgets them. */
+int fh_profiler_read(fh_ctx* ctx, fh_kernel_time* out, uint32_t max_out, uint32_t* n_out);
+
+/* ------------------------------------------------------------------------ */
 /* Test hook                                                                 */
 /* ------------------------------------------------------------------------ */
 
