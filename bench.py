@@ -48,7 +48,10 @@ def kernel_bytes(wl, n_pairs: int, tiles: int) -> dict:
         # positions + tangents, mesh depth, list element + slot tag per pair, tile headers
         "tiles_draw": 32 * N + 4 * WH + 12 * n_pairs + 16 * tiles * B,
         "bin_scatter": 12 * n_pairs + 4 * tiles * B,
-        "bin_sort": 8 * n_pairs + 20 * tiles * B,
+        # bin_sort_small reads every range header and sorts the short ranges,
+        # bin_sort_large the long ones; together each key is read and written once
+        "bin_sort_small": 8 * n_pairs + 20 * tiles * B,
+        "bin_sort_large": 8 * n_pairs,
         # tile headers, sorted keys, positions + tangents, mesh depth, shading
         # points, framebuffer read + write, tile list
         "fine_raster": 16 * tiles * B + 4 * n_pairs + 32 * N + 4 * WH + 16 * S * Ps + 32 * WH + 4 * tiles,

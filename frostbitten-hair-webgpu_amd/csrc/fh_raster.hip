@@ -109,12 +109,13 @@ __device__ __forceinline__ uint32_t search64(const uint32_t* ends, uint32_t x) {
 constexpr uint32_t kTileQueue = 128;  // ring of queued inside pixels (power of two, >= 127)
 
 struct TileWaveLds {
-  // per segment (lane of the wave)
-  float v[64][8];          // v00, v01, v10, v11
-  float d[64][4];          // projected depths
-  float slope[64][6];      // dx/dy of the six vertex pairs (hull_strip_x)
-  int box[64][4];          // pixel box (seg_pixel_bbox)
-  uint32_t rect[64][3];    // first tile x, first tile y, tiles per row
+  // per segment (lane of the wave), field-major so that lanes reading
+  // different segments hit different banks
+  float v[8][64];          // v00, v01, v10, v11
+  float d[4][64];          // projected depths
+  float slope[6][64];      // dx/dy of the six vertex pairs (hull_strip_x)
+  int box[4][64];          // pixel box (seg_pixel_bbox)
+  uint32_t rect[3][64];    // first tile x, first tile y, tiles per row
   uint32_t pairEnd[64];
   // per pair of the current window
   uint32_t owner[64];
@@ -137,24 +138,27 @@ __device__ __forceinline__ void hull_pair(int k, int& i, int& j) {
 // x extent of the convex hull of the quad within the horizontal strip
 // [ya, yb]: the hull vertices in the strip and the crossings of the strip's
 // edges with the six vertex pairs (the hull's edges are among them and the
-// other pairs are chords inside it). Empty when xlo > xhi.
+// other pairs are chords inside it). Empty when xlo > xhi. Vertex k's
+// coordinates and slope k are STRIDE floats apart.
+template <int STRIDE>
 __device__ __forceinline__ void hull_strip_x(const float* v, const float* slope, float ya, float yb,
                                              float& xlo, float& xhi) {
   xlo = 3.0e38f;
   xhi = -3.0e38f;
 #pragma unroll
   for (int i = 0; i < 4; i++) {
-    const float x = v[2 * i], y = v[2 * i + 1];
+    const float x = v[(2 * i) * STRIDE], y = v[(2 * i + 1) * STRIDE];
     if (y >= ya && y <= yb) { xlo = fminf(xlo, x); xhi = fmaxf(xhi, x); }
   }
 #pragma unroll
   for (int k = 0; k < 6; k++) {
     int i, j;
     hull_pair(k, i, j);
-    const float py = v[2 * i + 1], qy = v[2 * j + 1], px = v[2 * i];
+    const float py = v[(2 * i + 1) * STRIDE], qy = v[(2 * j + 1) * STRIDE], px = v[(2 * i) * STRIDE];
+    const float sl = slope[k * STRIDE];
     const float lo = fminf(py, qy), hi = fmaxf(py, qy);
-    if (ya > lo && ya < hi) { const float x = px + (ya - py) * slope[k]; xlo = fminf(xlo, x); xhi = fmaxf(xhi, x); }
-    if (yb > lo && yb < hi) { const float x = px + (yb - py) * slope[k]; xlo = fminf(xlo, x); xhi = fmaxf(xhi, x); }
+    if (ya > lo && ya < hi) { const float x = px + (ya - py) * sl; xlo = fminf(xlo, x); xhi = fmaxf(xhi, x); }
+    if (yb > lo && yb < hi) { const float x = px + (yb - py) * sl; xlo = fminf(xlo, x); xhi = fmaxf(xhi, x); }
   }
 }
 
@@ -169,9 +173,9 @@ __device__ __forceinline__ void tiles_shade_queue(const TilesArgs& a, TileWaveLd
     const uint32_t y = (uint32_t)L.py0[slot] + ((e >> 8) & 0xffu);
     const uint32_t o = L.owner[slot];
     Seg ps;
-    ps.v00 = mk2(L.v[o][0], L.v[o][1]); ps.v01 = mk2(L.v[o][2], L.v[o][3]);
-    ps.v10 = mk2(L.v[o][4], L.v[o][5]); ps.v11 = mk2(L.v[o][6], L.v[o][7]);
-    ps.depths = mk4(L.d[o][0], L.d[o][1], L.d[o][2], L.d[o][3]);
+    ps.v00 = mk2(L.v[0][o], L.v[1][o]); ps.v01 = mk2(L.v[2][o], L.v[3][o]);
+    ps.v10 = mk2(L.v[4][o], L.v[5][o]); ps.v11 = mk2(L.v[6][o], L.v[7][o]);
+    ps.depths = mk4(L.d[0][o], L.d[1][o], L.d[2][o], L.d[3][o]);
     const f2 p = mk2((float)x, (float)y);
     const f2 w = interp_quad(ps, p);
     const float hdp = interp_f32(w, ps.depths);
@@ -227,16 +231,16 @@ __global__ __launch_bounds__(64) void tiles_draw_kernel(TilesArgs a) {
     seg_pixel_bbox(ps, cx0, cy0, cx1, cy1);
     const float vv[8] = {ps.v00.x, ps.v00.y, ps.v01.x, ps.v01.y, ps.v10.x, ps.v10.y, ps.v11.x, ps.v11.y};
 #pragma unroll
-    for (int i = 0; i < 8; i++) L.v[lane][i] = vv[i];
+    for (int i = 0; i < 8; i++) L.v[i][lane] = vv[i];
 #pragma unroll
     for (int k = 0; k < 6; k++) {
       int i, j;
       hull_pair(k, i, j);
-      L.slope[lane][k] = (vv[2 * j] - vv[2 * i]) / (vv[2 * j + 1] - vv[2 * i + 1]);
+      L.slope[k][lane] = (vv[2 * j] - vv[2 * i]) / (vv[2 * j + 1] - vv[2 * i + 1]);
     }
-    L.d[lane][0] = ps.depths.x; L.d[lane][1] = ps.depths.y; L.d[lane][2] = ps.depths.z; L.d[lane][3] = ps.depths.w;
-    L.box[lane][0] = cx0; L.box[lane][1] = cy0; L.box[lane][2] = cx1; L.box[lane][3] = cy1;
-    L.rect[lane][0] = tminx; L.rect[lane][1] = tminy; L.rect[lane][2] = tsx;
+    L.d[0][lane] = ps.depths.x; L.d[1][lane] = ps.depths.y; L.d[2][lane] = ps.depths.z; L.d[3][lane] = ps.depths.w;
+    L.box[0][lane] = cx0; L.box[1][lane] = cy0; L.box[2][lane] = cx1; L.box[3][lane] = cy1;
+    L.rect[0][lane] = tminx; L.rect[1][lane] = tminy; L.rect[2][lane] = tsx;
   }
   const uint32_t pairEnd = wave_incl_scan(nTiles);
   L.pairEnd[lane] = pairEnd;
@@ -250,14 +254,14 @@ __global__ __launch_bounds__(64) void tiles_draw_kernel(TilesArgs a) {
     if (pidx < nPairs) {
       const uint32_t o = search64(L.pairEnd, pidx);
       const uint32_t k = pidx - (o ? L.pairEnd[o - 1u] : 0u);
-      const uint32_t tsx = L.rect[o][2];
-      ty = L.rect[o][1] + k / tsx;
-      tx = L.rect[o][0] + k % tsx;
+      const uint32_t tsx = L.rect[2][o];
+      ty = L.rect[1][o] + k / tsx;
+      tx = L.rect[0][o] + k % tsx;
       // getTileBoundsPx — tileUtils.wgsl.ts:44-54
       const uint32_t bx0 = clampu(tx * T, 0u, tg.vpW - 1u), by0 = clampu(ty * T, 0u, tg.vpH - 1u);
       const uint32_t bx1 = clampu((tx + 1u) * T, 0u, tg.vpW - 1u), by1 = clampu((ty + 1u) * T, 0u, tg.vpH - 1u);
-      const int xs = max((int)bx0, L.box[o][0]), xe = min((int)bx1, L.box[o][2] + 1);
-      const int ys = max((int)by0, L.box[o][1]), ye = min((int)by1, L.box[o][3] + 1);
+      const int xs = max((int)bx0, L.box[0][o]), xe = min((int)bx1, L.box[2][o] + 1);
+      const int ys = max((int)by0, L.box[1][o]), ye = min((int)by1, L.box[3][o] + 1);
       if (xe > xs && ye > ys) rows = (uint32_t)(ye - ys);
       L.owner[lane] = o;
       L.px0[lane] = xs; L.py0[lane] = ys; L.px1[lane] = xe;
@@ -281,14 +285,15 @@ __global__ __launch_bounds__(64) void tiles_draw_kernel(TilesArgs a) {
         const int y = L.py0[slot] + (int)(ridx - (slot ? L.rowEnd[slot - 1u] : 0u));
         const uint32_t o = L.owner[slot];
         int xa = L.px0[slot], xb = L.px1[slot];
-        const float* v = L.v[o];
-        const float amax = fmaxf(fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3]))),
-                                 fmaxf(fmaxf(fabsf(v[4]), fabsf(v[5])), fmaxf(fabsf(v[6]), fabsf(v[7]))));
+        const float* v = &L.v[0][o];
+        float amax = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; q++) amax = fmaxf(amax, fabsf(v[q * 64]));
         // collinear quads test whole tiles; far-off vertices skip the span
-        const bool span = L.box[o][0] > -(1 << 20) && amax < 65536.0f;
+        const bool span = L.box[0][o] > -(1 << 20) && amax < 65536.0f;
         if (span) {
           float xlo, xhi;
-          hull_strip_x(v, L.slope[o], (float)y - 1.0f, (float)y + 1.0f, xlo, xhi);
+          hull_strip_x<64>(v, &L.slope[0][o], (float)y - 1.0f, (float)y + 1.0f, xlo, xhi);
           if (xlo <= xhi) {
             xa = max(xa, (int)ceilf(xlo - 1.0f));
             xb = min(xb, (int)floorf(xhi + 1.0f) + 1);
@@ -318,8 +323,8 @@ __global__ __launch_bounds__(64) void tiles_draw_kernel(TilesArgs a) {
           const uint32_t slot = L.itemSlot[it];
           const uint32_t o = L.owner[slot];
           Seg ps;
-          ps.v00 = mk2(L.v[o][0], L.v[o][1]); ps.v01 = mk2(L.v[o][2], L.v[o][3]);
-          ps.v10 = mk2(L.v[o][4], L.v[o][5]); ps.v11 = mk2(L.v[o][6], L.v[o][7]);
+          ps.v00 = mk2(L.v[0][o], L.v[1][o]); ps.v01 = mk2(L.v[2][o], L.v[3][o]);
+          ps.v10 = mk2(L.v[4][o], L.v[5][o]); ps.v11 = mk2(L.v[6][o], L.v[7][o]);
           inside = inside_quad(ps, mk2((float)x, (float)y));
           entry = (slot << 16) | ((uint32_t)(y - L.py0[slot]) << 8) | (uint32_t)(x - L.px0[slot]);
         }
@@ -363,10 +368,10 @@ __global__ __launch_bounds__(64) void tiles_draw_kernel(TilesArgs a) {
           atomicMax(&t->maxDepth, f2u(dmax * 4294967296.0f));
           atomicMax(&t->minDepth, f2u((1.0f - dmin) * 4294967296.0f));
           const uint32_t last = atomicExch(&t->tileSegmentPtr, slot + 1u);
-          atomicAdd(&t->padding0, 1u);
+          const uint32_t rank = atomicAdd(&t->padding0, 1u);
           const uint32_t prevPtr = last == 0u ? kInvalidPtr : last - 1u;
           *reinterpret_cast<uint2*>(&a.segData[slot]) = make_uint2((seg << 24) | strand, prevPtr);
-          if (a.slotKey) a.slotKey[slot] = tb;
+          if (a.slotKey) *reinterpret_cast<uint2*>(&a.slotKey[2ull * slot]) = make_uint2(tb, rank);
         }
       }
     }
@@ -400,7 +405,8 @@ __global__ __launch_bounds__(256) void rebuild_keys_kernel(fh_tile_result* __res
   ptr = ptr == 0u ? kInvalidPtr : ptr - 1u;
   uint32_t n = 0;
   while (ptr != kInvalidPtr && ptr < limit && n < limit) {
-    slotKey[ptr] = tb;
+    slotKey[2ull * ptr] = tb;
+    slotKey[2ull * ptr + 1] = n;
     n++;
     ptr = data[ptr].next;
   }
@@ -473,9 +479,9 @@ __global__ __launch_bounds__(256) void sort_scatter_kernel(const uint32_t* __res
 // ---------------------------------------------------------------------------
 // Fine-pass gather: the per-(tile, bin) linked lists become one contiguous
 // range per (tile, bin) of a scratch array, so the fine pass streams each
-// list instead of chasing pointers. Step 1 is an inclusive scan of the list
-// lengths (HairTileResult.padding0); step 2 scatters every slot to
-// --end[key], leaving end[key] at the first element of the range.
+// list instead of chasing pointers. Step 1 is an exclusive scan of the list
+// lengths (HairTileResult.padding0); step 2 writes every slot to the start
+// of its range plus its rank in the list, both tagged by the tile pass.
 // ---------------------------------------------------------------------------
 constexpr int kScanBlock = 1024;
 
@@ -510,7 +516,7 @@ __global__ __launch_bounds__(kScanBlock) void bin_scan_local_kernel(const fh_til
   const uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
   const uint32_t v = i < n ? tiles[i].padding0 : 0u;
   const uint32_t s = block_inclusive_scan(v, warpSums);
-  if (i < n) ends[i] = s;
+  if (i < n) ends[i] = s - v;  // exclusive: first element of the range
   if (threadIdx.x == kScanBlock - 1) blockSums[blockIdx.x] = s;
 }
 
@@ -533,14 +539,14 @@ __global__ __launch_bounds__(kScanBlock) void bin_scan_add_kernel(uint32_t n, ui
 __global__ __launch_bounds__(256) void bin_scatter_kernel(const fh_render_uniforms* __restrict__ u,
                                                           const uint32_t* __restrict__ segCounter,
                                                           const fh_list_element* __restrict__ data,
-                                                          uint32_t capacity, const uint32_t* __restrict__ slotKey,
-                                                          uint32_t* __restrict__ ends, uint32_t nTB,
+                                                          uint32_t capacity, const uint2* __restrict__ slotTag,
+                                                          const uint32_t* __restrict__ starts, uint32_t nTB,
                                                           uint32_t* __restrict__ binSeg) {
   const uint32_t n = min(*segCounter, min(capacity, u->maxDrawnHairSegments));
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-    const uint32_t key = slotKey[s];
-    if (key >= nTB) continue;  // untagged slot: not linked into any list
-    const uint32_t pos = atomicSub(&ends[key], 1u) - 1u;
+    const uint2 tag = slotTag[s];  // (tile-bin, rank in its list)
+    if (tag.x >= nTB) continue;    // untagged slot: not linked into any list
+    const uint32_t pos = starts[tag.x] + tag.y;
     if (pos < capacity) {
       const uint32_t packed = data[s].strandAndSegmentIdxs;  // segment << 24 | strand
       binSeg[pos] = ((packed & 0x00ffffffu) << 8) | (packed >> 24);
@@ -550,12 +556,13 @@ __global__ __launch_bounds__(256) void bin_scatter_kernel(const fh_render_unifor
 
 // ---------------------------------------------------------------------------
 // Each gathered range sorted by blend key (strand << 8 | segment), largest
-// first. The fine pass pushes fragments at the head of per-pixel lists in
+// first (bin_sort_kernel). The fine pass pushes fragments at the head of per-pixel lists in
 // this order, so every list reads back in ascending key order: a fixed blend
 // order for equal-depth-slice fragments, whatever order the tile pass's
 // atomics appended the segments in. Keys are unique within a range (a
 // segment enters each tile once, in one bin).
-//   n <= kSortLds: bitonic network in LDS;
+//   n <= 64: one wave, one key per lane, shuffle network in registers;
+//   n <= kSortLds: bitonic network in LDS over the whole workgroup;
 //   larger: the same network over a zero-padded copy in global scratch at
 //   [2 * start, 2 * start + 2^ceil(log2 n)), disjoint between ranges.
 // Padding with 0 is safe: it sorts last, and a real key 0 ties with it.
@@ -577,24 +584,77 @@ __device__ __forceinline__ void bitonic_desc(uint32_t* k, uint32_t n2, uint32_t 
   }
 }
 
-__global__ __launch_bounds__(256) void bin_sort_kernel(const fh_tile_result* __restrict__ tiles,
-                                                       const uint32_t* __restrict__ starts, uint32_t nTB,
-                                                       uint32_t capacity, uint32_t* __restrict__ keys,
-                                                       uint32_t* __restrict__ big) {
+// descending bitonic sort of one key per lane across the wave
+__device__ __forceinline__ uint32_t wave_sort_desc(uint32_t v) {
+  const uint32_t lane = __lane_id();
+#pragma unroll
+  for (uint32_t size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      const uint32_t o = (uint32_t)__shfl_xor((int)v, (int)stride);
+      const bool desc = (lane & size) == 0u;
+      const bool lower = (lane & stride) == 0u;
+      v = (desc == lower) ? max(v, o) : min(v, o);
+    }
+  }
+  return v;
+}
+
+// One workgroup per 256 consecutive ranges: ranges of up to 64 keys are
+// sorted one per wave in registers; larger ones are appended to `large`
+// (count in large[0]) for bin_sort_large_kernel, so that the few heavy
+// ranges, which cluster on screen, spread over many workgroups.
+__global__ __launch_bounds__(256) void bin_sort_small_kernel(const fh_tile_result* __restrict__ tiles,
+                                                             const uint32_t* __restrict__ starts, uint32_t nTB,
+                                                             uint32_t capacity, uint32_t* __restrict__ keys,
+                                                             uint32_t* __restrict__ large) {
+  __shared__ uint32_t smallList[256];
+  __shared__ uint32_t sStart[256], sCount[256];
+  __shared__ uint32_t nSmall;
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) nSmall = 0;
+  __syncthreads();
+  const uint32_t tb = blockIdx.x * 256u + tid;
+  uint32_t start = 0, n = 0;
+  if (tb < nTB) {
+    start = min(starts[tb], capacity);
+    n = min(tiles[tb].padding0, capacity - start);
+  }
+  sStart[tid] = start;
+  sCount[tid] = n;
+  if (n >= 2u && n <= 64u) smallList[atomicAdd(&nSmall, 1u)] = tid;
+  if (n > 64u) large[1u + atomicAdd(&large[0], 1u)] = tb;
+  __syncthreads();
+  const uint32_t lane = __lane_id(), w = tid >> 6;
+  for (uint32_t i = w; i < nSmall; i += blockDim.x >> 6) {
+    const uint32_t j = smallList[i];
+    const uint32_t st = sStart[j], cnt = sCount[j];
+    uint32_t v = lane < cnt ? keys[st + lane] : 0u;
+    v = wave_sort_desc(v);
+    if (lane < cnt) keys[st + lane] = v;
+  }
+}
+
+// One workgroup per listed range (more than 64 keys).
+__global__ __launch_bounds__(256) void bin_sort_large_kernel(const fh_tile_result* __restrict__ tiles,
+                                                             const uint32_t* __restrict__ starts,
+                                                             uint32_t capacity, uint32_t* __restrict__ keys,
+                                                             const uint32_t* __restrict__ large,
+                                                             uint32_t* __restrict__ big) {
   __shared__ uint32_t sk[kSortLds];
-  for (uint32_t tb = blockIdx.x; tb < nTB; tb += gridDim.x) {
-    const uint32_t start = min(starts[tb], capacity);
-    const uint32_t n = min(tiles[tb].padding0, capacity - start);
-    if (n <= 1u) continue;
+  const uint32_t count = large[0];
+  for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
+    const uint32_t tb = large[1u + i];
+    const uint32_t st = min(starts[tb], capacity);
+    const uint32_t cnt = min(tiles[tb].padding0, capacity - st);
     uint32_t n2 = 2;
-    while (n2 < n) n2 <<= 1;
-    uint32_t* k = sk;
-    if (n2 > kSortLds) k = big + 2ull * start;
-    for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x) k[i] = i < n ? keys[start + i] : 0u;
+    while (n2 < cnt) n2 <<= 1;
+    uint32_t* k = n2 > kSortLds ? big + 2ull * st : sk;
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < n2; e += blockDim.x) k[e] = e < cnt ? keys[st + e] : 0u;
     __syncthreads();
     bitonic_desc(k, n2, threadIdx.x, blockDim.x);
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) keys[start + i] = k[i];
-    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < cnt; e += blockDim.x) keys[st + e] = k[e];
   }
 }
 
@@ -640,17 +700,31 @@ struct FineArgs {
   float alphaMultiplier;
 };
 
-struct SegRec {
-  float v[8];        // v00, v01, v10, v11
-  float d[4];        // projected depths
-  float slope[6];    // dx/dy of the six vertex pairs (hull_strip_x)
-  uint32_t key;      // strand << 8 | segment
-  int16_t bx0, by0, bx1, by1;  // pixel box relative to the tile, inclusive, clamped to [-1, TILE]
-  float dnear;       // lower bound of the interpolated depth (-inf when unknown)
-  uint32_t span;     // 1 when per-row hull spans may narrow the box
-  uint32_t pad;
+// Segment records of one batch, field-major in LDS (field f of record j at
+// f * batch + j): lanes reading different records hit different banks.
+enum RecField : uint32_t {
+  kRecV = 0,       // 8: v00, v01, v10, v11
+  kRecD = 8,       // 4: projected depths
+  kRecSlope = 12,  // 6: dx/dy of the six vertex pairs (hull_strip_x)
+  kRecKey = 18,    // strand << 8 | segment
+  kRecBox0 = 19,   // pixel box relative to the tile, inclusive, clamped to
+  kRecBox1 = 20,   //   [-1, TILE]: (x0 | y0 << 16), (x1 | y1 << 16) as int16
+  kRecNear = 21,   // lower bound of the interpolated depth (-inf when unknown)
+  kRecSpan = 22,   // 1 when per-row hull spans may narrow the box
+  kRecFields = 23
 };
-static_assert(sizeof(SegRec) == 96, "SegRec must be 96 bytes");
+
+template <uint32_t RB>
+struct RecStore {
+  float* f;
+  __device__ __forceinline__ float& at(uint32_t field, uint32_t j) const { return f[field * RB + j]; }
+  __device__ __forceinline__ uint32_t u(uint32_t field, uint32_t j) const { return __float_as_uint(f[field * RB + j]); }
+  __device__ __forceinline__ int lo16(uint32_t field, uint32_t j) const { return (int)(int16_t)(u(field, j) & 0xffffu); }
+  __device__ __forceinline__ int hi16(uint32_t field, uint32_t j) const { return (int)(int16_t)(u(field, j) >> 16); }
+};
+
+template <int TILE>
+constexpr uint32_t fine_batch() { return TILE * TILE < 256 ? TILE * TILE : 256; }
 
 constexpr uint32_t kFineRing = 128;      // per-wave ring of queued fragments
 constexpr uint32_t kSkipTag = 0xffffffffu;
@@ -664,7 +738,7 @@ template <int TILE>
 struct FineLds {
   static constexpr uint32_t NPX = TILE * TILE;
   static constexpr uint32_t NW = NPX / 64;
-  SegRec* recs;
+  RecStore<fine_batch<TILE>()> recs;
   uint32_t* heads;   // NPX: newest pool slot of each pixel
   float* db;         // NPX: mesh depth
   uint32_t* done;    // NPX: pixel saturated
@@ -684,17 +758,18 @@ __device__ __forceinline__ void fine_shade_queue(const FineArgs& a, const FineLd
   uint32_t slot = 0;
   f2 wq = mk2(0.0f, 0.0f);
   float hdp = 0.0f;
-  const SegRec* r = nullptr;
+  uint32_t j = 0;
+  const auto& R = L.recs;
   if (lane < n) {
     const uint32_t e = L.ring[w * kFineRing + ((head + lane) & (kFineRing - 1u))];
     slot = e >> 14;
-    r = &L.recs[(e >> 6) & 0xffu];
+    j = (e >> 6) & 0xffu;
     const uint32_t pix = w * 64u + (e & 63u);
     const f2 p = mk2((float)(bx0 + pix % TILE), (float)(by0 + pix / TILE));
     Seg sg;
-    sg.v00 = mk2(r->v[0], r->v[1]); sg.v01 = mk2(r->v[2], r->v[3]);
-    sg.v10 = mk2(r->v[4], r->v[5]); sg.v11 = mk2(r->v[6], r->v[7]);
-    sg.depths = mk4(r->d[0], r->d[1], r->d[2], r->d[3]);
+    sg.v00 = mk2(R.at(kRecV + 0, j), R.at(kRecV + 1, j)); sg.v01 = mk2(R.at(kRecV + 2, j), R.at(kRecV + 3, j));
+    sg.v10 = mk2(R.at(kRecV + 4, j), R.at(kRecV + 5, j)); sg.v11 = mk2(R.at(kRecV + 6, j), R.at(kRecV + 7, j));
+    sg.depths = mk4(R.at(kRecD + 0, j), R.at(kRecD + 1, j), R.at(kRecD + 2, j), R.at(kRecD + 3, j));
     wq = interp_quad(sg, p);
     hdp = interp_f32(wq, sg.depths);
     accept = !(hdp >= L.db[pix]);  // depth test with GL_LESS (:31-34)
@@ -715,7 +790,8 @@ __device__ __forceinline__ void fine_shade_queue(const FineArgs& a, const FineLd
       float alpha = 1.0f - fabsf(wq.x * 2.0f - 1.0f);
       if (a.alphaQuadratic) alpha = sqrtf(alpha);
       alpha = satf(alpha * a.alphaMultiplier);
-      const uint32_t strand = r->key >> 8, seg = r->key & 0xffu;
+      const uint32_t key = R.u(kRecKey, j);
+      const uint32_t strand = key >> 8, seg = key & 0xffu;
       const float tFull = ((float)seg + wq.y) / invSegs;
       // _sampleShading — hairShadingBuffer.ts:29-38
       const float sa = tFull * ((float)a.PS - 1.0f);
@@ -759,10 +835,10 @@ __device__ __forceinline__ void fine_cheap_batch(const FineArgs& a, const FineLd
   for (uint32_t rb = 0; rb < nb; rb += 64u) {
     const uint32_t jr = rb + lane;
     uint32_t nrows = 0;
+    const auto& R = L.recs;
     if (jr < nb) {
-      const SegRec& r = L.recs[jr];
-      const int ya = max((int)r.by0, wy0), yb = min((int)r.by1, wy1);
-      const int xa = max((int)r.bx0, 0), xb = min((int)r.bx1, TILE - 1);
+      const int ya = max(R.hi16(kRecBox0, jr), wy0), yb = min(R.hi16(kRecBox1, jr), wy1);
+      const int xa = max(R.lo16(kRecBox0, jr), 0), xb = min(R.lo16(kRecBox1, jr), TILE - 1);
       if (yb >= ya && xb >= xa) nrows = (uint32_t)(yb - ya + 1);
     }
     const uint32_t re = wave_incl_scan(nrows);
@@ -775,13 +851,12 @@ __device__ __forceinline__ void fine_cheap_batch(const FineArgs& a, const FineLd
       if (ridx < nRows) {
         const uint32_t k = search64(rowEnd, ridx);
         const uint32_t j = rb + k;
-        const SegRec& r = L.recs[j];
-        const int y = max((int)r.by0, wy0) + (int)(ridx - (k ? rowEnd[k - 1u] : 0u));
-        int xa = max((int)r.bx0, 0), xb = min((int)r.bx1, TILE - 1) + 1;
-        if (r.span) {
+        const int y = max(R.hi16(kRecBox0, j), wy0) + (int)(ridx - (k ? rowEnd[k - 1u] : 0u));
+        int xa = max(R.lo16(kRecBox0, j), 0), xb = min(R.lo16(kRecBox1, j), TILE - 1) + 1;
+        if (R.u(kRecSpan, j)) {
           float xlo, xhi;
           const float yf = (float)(by0 + (uint32_t)y);
-          hull_strip_x(r.v, r.slope, yf - 1.0f, yf + 1.0f, xlo, xhi);
+          hull_strip_x<fine_batch<TILE>()>(&R.at(kRecV, j), &R.at(kRecSlope, j), yf - 1.0f, yf + 1.0f, xlo, xhi);
           if (xlo <= xhi) {
             xa = max(xa, (int)ceilf(xlo - 1.0f) - (int)bx0);
             xb = min(xb, (int)floorf(xhi + 1.0f) + 1 - (int)bx0);
@@ -808,11 +883,10 @@ __device__ __forceinline__ void fine_cheap_batch(const FineArgs& a, const FineLd
           const uint32_t y = itemY[it];
           j = itemRec[it];
           pix = y * TILE + x;
-          const SegRec& r = L.recs[j];
-          if (!L.done[pix] && !(r.dnear >= L.db[pix])) {
+          if (!L.done[pix] && !(R.at(kRecNear, j) >= L.db[pix])) {
             Seg sg;
-            sg.v00 = mk2(r.v[0], r.v[1]); sg.v01 = mk2(r.v[2], r.v[3]);
-            sg.v10 = mk2(r.v[4], r.v[5]); sg.v11 = mk2(r.v[6], r.v[7]);
+            sg.v00 = mk2(R.at(kRecV + 0, j), R.at(kRecV + 1, j)); sg.v01 = mk2(R.at(kRecV + 2, j), R.at(kRecV + 3, j));
+            sg.v10 = mk2(R.at(kRecV + 4, j), R.at(kRecV + 5, j)); sg.v11 = mk2(R.at(kRecV + 6, j), R.at(kRecV + 7, j));
             push = inside_quad(sg, mk2((float)(bx0 + x), (float)(by0 + y)));
           }
         }
@@ -857,8 +931,9 @@ __global__ __launch_bounds__(TILE* TILE) void fine_raster_kernel(FineArgs a) {
   constexpr uint32_t NW = NPX / 64;
   extern __shared__ __align__(16) unsigned char smem[];
   FineLds<TILE> L;
-  L.recs = reinterpret_cast<SegRec*>(smem);
-  L.heads = reinterpret_cast<uint32_t*>(smem + (size_t)a.batch * sizeof(SegRec));
+  constexpr uint32_t RB = fine_batch<TILE>();
+  L.recs.f = reinterpret_cast<float*>(smem);
+  L.heads = reinterpret_cast<uint32_t*>(smem + (size_t)RB * kRecFields * 4);
   L.db = reinterpret_cast<float*>(L.heads + NPX);
   L.done = reinterpret_cast<uint32_t*>(L.db + NPX);
   L.claim = L.done + NPX;
@@ -925,27 +1000,27 @@ __global__ __launch_bounds__(TILE* TILE) void fine_raster_kernel(FineArgs a) {
           const Seg sg = project_segment(u, a.pos, a.tan, a.P, key >> 8, key & 0xffu, vpf, u->fiberRadius);
           int x0, y0, x1, y1;
           seg_pixel_bbox(sg, x0, y0, x1, y1);
-          SegRec r;
+          const auto& R = L.recs;
           const float vv[8] = {sg.v00.x, sg.v00.y, sg.v01.x, sg.v01.y, sg.v10.x, sg.v10.y, sg.v11.x, sg.v11.y};
           float amax = 0.0f;
 #pragma unroll
-          for (int i = 0; i < 8; i++) { r.v[i] = vv[i]; amax = fmaxf(amax, fabsf(vv[i])); }
+          for (int i = 0; i < 8; i++) { R.at(kRecV + i, tid) = vv[i]; amax = fmaxf(amax, fabsf(vv[i])); }
 #pragma unroll
           for (int k = 0; k < 6; k++) {
             int i, j;
             hull_pair(k, i, j);
-            r.slope[k] = (vv[2 * j] - vv[2 * i]) / (vv[2 * j + 1] - vv[2 * i + 1]);
+            R.at(kRecSlope + k, tid) = (vv[2 * j] - vv[2 * i]) / (vv[2 * j + 1] - vv[2 * i + 1]);
           }
-          r.d[0] = sg.depths.x; r.d[1] = sg.depths.y; r.d[2] = sg.depths.z; r.d[3] = sg.depths.w;
-          r.key = key;
+          R.at(kRecD + 0, tid) = sg.depths.x; R.at(kRecD + 1, tid) = sg.depths.y;
+          R.at(kRecD + 2, tid) = sg.depths.z; R.at(kRecD + 3, tid) = sg.depths.w;
+          R.at(kRecKey, tid) = __uint_as_float(key);
           // relative to the tile origin, clamped to one pixel beyond the tile
-          r.bx0 = (int16_t)max(min(x0 - (int)bx0, TILE), -1);
-          r.by0 = (int16_t)max(min(y0 - (int)by0, TILE), -1);
-          r.bx1 = (int16_t)max(min(x1 - (int)bx0, TILE), -1);
-          r.by1 = (int16_t)max(min(y1 - (int)by0, TILE), -1);
+          const int rx0 = max(min(x0 - (int)bx0, TILE), -1), ry0 = max(min(y0 - (int)by0, TILE), -1);
+          const int rx1 = max(min(x1 - (int)bx0, TILE), -1), ry1 = max(min(y1 - (int)by0, TILE), -1);
+          R.at(kRecBox0, tid) = __uint_as_float(((uint32_t)rx0 & 0xffffu) | ((uint32_t)ry0 << 16));
+          R.at(kRecBox1, tid) = __uint_as_float(((uint32_t)rx1 & 0xffffu) | ((uint32_t)ry1 << 16));
           // collinear quads test the whole box; far-off vertices skip the span
-          r.span = (x0 > -(1 << 20) && amax < 65536.0f) ? 1u : 0u;
-          r.pad = 0;
+          R.at(kRecSpan, tid) = __uint_as_float((x0 > -(1 << 20) && amax < 65536.0f) ? 1u : 0u);
           // The interpolated depth is a two-level lerp of the four depths with
           // weights in [0, 1]: at least their minimum, less the rounding of
           // two lerps (a few ulp of the largest magnitude). Non-finite depths
@@ -955,8 +1030,7 @@ __global__ __launch_bounds__(TILE* TILE) void fine_raster_kernel(FineArgs a) {
           const float dn = fminf(fminf(sg.depths.x, sg.depths.y), fminf(sg.depths.z, sg.depths.w));
           const float dm = fmaxf(fmaxf(fabsf(sg.depths.x), fabsf(sg.depths.y)),
                                  fmaxf(fabsf(sg.depths.z), fabsf(sg.depths.w)));
-          r.dnear = finite ? dn - (dm * 1e-6f + 1e-30f) : -INFINITY;
-          L.recs[tid] = r;
+          R.at(kRecNear, tid) = finite ? dn - (dm * 1e-6f + 1e-30f) : -INFINITY;
         }
         __syncthreads();
         fine_cheap_batch<TILE>(a, L, gpool, nb, bx0, by0, td, invSegs, &s_sliceOff, &s_phys);
@@ -1183,13 +1257,15 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
   const size_t offSums = fh::align_up(offEnds + (size_t)nTB * 4, 256);
   const size_t offSeg = fh::align_up(offSums + (size_t)scanBlocks * 4, 256);
   const size_t offBig = fh::align_up(offSeg + (size_t)limit * 4 + 4, 256);
-  const size_t total = fh::align_up(offBig + (size_t)limit * 8 + 8, 256);
+  const size_t offLarge = fh::align_up(offBig + (size_t)limit * 8 + 8, 256);
+  const size_t total = fh::align_up(offLarge + ((size_t)nTB + 1) * 4, 256);
   if ((rc = ensure_scratch(ctx, total))) return rc;
   char* base = reinterpret_cast<char*>(ctx->scratch);
   uint32_t* ends = reinterpret_cast<uint32_t*>(base + offEnds);
   uint32_t* sums = reinterpret_cast<uint32_t*>(base + offSums);
   uint32_t* binSeg = reinterpret_cast<uint32_t*>(base + offSeg);
   uint32_t* big = reinterpret_cast<uint32_t*>(base + offBig);
+  uint32_t* large = reinterpret_cast<uint32_t*>(base + offLarge);
   const fh_list_element* data =
       reinterpret_cast<const fh_list_element*>(reinterpret_cast<const char*>(tile_segments) + 4);
 
@@ -1206,14 +1282,20 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
   const uint32_t scatterBlocks = std::max(1u, std::min(fh::div_ceil(limit, 256), (uint32_t)ctx->num_cus * 8u));
   const int pk7 = fh::prof_begin(ctx, "bin_scatter");
   bin_scatter_kernel<<<scatterBlocks, 256, 0, ctx->stream>>>(
-      uniforms, reinterpret_cast<const uint32_t*>(tile_segments), data, limit, slot_tile_bin, ends, nTB, binSeg);
+      uniforms, reinterpret_cast<const uint32_t*>(tile_segments), data, limit,
+      reinterpret_cast<const uint2*>(slot_tile_bin), ends, nTB, binSeg);
   fh::prof_end(ctx, pk7);
   if ((rc = check_launch("bin_scatter_kernel"))) return rc;
-  const int pk8 = fh::prof_begin(ctx, "bin_sort");
-  bin_sort_kernel<<<std::min(nTB, (uint32_t)ctx->num_cus * 8u), 256, 0, ctx->stream>>>(tiles, ends, nTB, limit,
-                                                                                       binSeg, big);
+  if ((rc = fill_u32(ctx, large, 1, 0u))) return rc;
+  const int pk8 = fh::prof_begin(ctx, "bin_sort_small");
+  bin_sort_small_kernel<<<fh::div_ceil(nTB, 256), 256, 0, ctx->stream>>>(tiles, ends, nTB, limit, binSeg, large);
   fh::prof_end(ctx, pk8);
-  if ((rc = check_launch("bin_sort_kernel"))) return rc;
+  if ((rc = check_launch("bin_sort_small_kernel"))) return rc;
+  const int pk8b = fh::prof_begin(ctx, "bin_sort_large");
+  bin_sort_large_kernel<<<std::min(nTB, (uint32_t)ctx->num_cus * 8u), 256, 0, ctx->stream>>>(
+      tiles, ends, limit, binSeg, large, big);
+  fh::prof_end(ctx, pk8b);
+  if ((rc = check_launch("bin_sort_large_kernel"))) return rc;
 
   FineArgs a;
   a.u = uniforms;
@@ -1237,7 +1319,7 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
   // 16x16 processors per CU (160 KiB LDS on gfx950).
   const size_t ldsTarget = std::min<size_t>(T == 8 ? 40 * 1024 : (T == 16 ? 78 * 1024 : 150 * 1024),
                                             ctx->max_lds - 256);
-  const size_t fixed = (size_t)a.batch * sizeof(SegRec) + (size_t)npx * 16 +
+  const size_t fixed = (size_t)a.batch * kRecFields * 4 + (size_t)npx * 16 +
                        (size_t)(npx / 64) * (kFineRing + 5 * 64) * 4;
   FH_REQUIRE(fixed + 16 * 64 <= ldsTarget, "fh_fine_raster: LDS too small for the tile size");
   // pool slot indices must fit the 18 bits of a ring entry

@@ -118,7 +118,8 @@ __global__ __launch_bounds__(256) void grid_pre_sim_kernel(
 // strand per lane on the first wave, so each wave instruction advances 32
 // strands; the point-local phases (Verlet integration with both grid
 // samples, global shape, sphere and SDF collisions) are memory-latency bound
-// and spread over all threads, one (strand, point) per thread at a time.
+// and spread over all threads, one (point, strand) per thread at a time, in
+// LDS order so that neighbouring threads touch neighbouring LDS words.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kSimStrands = 32;
 constexpr uint32_t kSimThreads = 256;
@@ -147,63 +148,67 @@ __global__ __launch_bounds__(kSimThreads) void sim_integrate_kernel(
   const f3 windDir = mk3(u->wind[0], u->wind[1], u->wind[2]);
   const float dt = u->deltaTime;
 
-  // staging in global order (coalesced) + Verlet integration with grid
-  // forces for points 1..P-1 (:125-162)
+  // Staging in global order (coalesced loads); PREV goes to K until the
+  // local-shape constants replace it.
   for (uint32_t e = tid; e < nElem; e += kSimThreads) {
     const uint32_t ls = e / P, pt = e - ls * P;
     const uint32_t o = pt * L + ls;
-    const float4 pn4 = now[gbase + e];
+    A[o] = now[gbase + e];
+    K[o] = prev[gbase + e];
     Bi[o] = initial[gbase + e];
     Len[o] = seglen[gbase + e];
-    float4 r = pn4;
-    if (pt >= 1u) {
-      const float4 pp4 = prev[gbase + e];
-      // wind jitter is per strand (:134-139)
-      const float timer = (float)u->frameIdx * 0.73f + u->windPhaseOffset * (float)(s0 + ls);
-      const float jitter = fractf(timer * u->windStrengthFrequency);
-      const float jitterDelta = mixf(-0.5f * u->windStrengthJitter, 0.5f * u->windStrengthJitter, jitter);
-      const float jitterStr = 1.0f + jitterDelta;
-      const f3 pn = mk3(pn4.x, pn4.y, pn4.z), pp = mk3(pp4.x, pp4.y, pp4.z);
-      const Cell co = grid_cell(d, gmin, gmax, pn);
-      f3 grad = mk3(0.0f, 0.0f, 0.0f), vel = mk3(0.0f, 0.0f, 0.0f);
-      float wstr = 0.0f, dens = 0.0f;
-      // _getGridDensityGradAndWind (densityGradAndWindGrid.wgsl.ts:20-44)
-      for (uint32_t z = co.mn[2]; z <= co.mx[2]; z++)
-        for (uint32_t y = co.mn[1]; y <= co.mx[1]; y++)
-          for (uint32_t x = co.mn[0]; x <= co.mx[0]; x++) {
-            const f3 w = cell_w(x, y, z, co.pin);
-            const float4 gw = *reinterpret_cast<const float4*>(&dgw[grid_idx(d, x, y, z)]);
-            grad = grad + w * mk3(gw.x, gw.y, gw.z);
-            wstr = wstr + length(w) * gw.w;
-          }
-      // _getGridDensityVelocity (densityVelocityGrid.wgsl.ts:30-68)
-      for (uint32_t z = co.mn[2]; z <= co.mx[2]; z++)
-        for (uint32_t y = co.mn[1]; y <= co.mx[1]; y++)
-          for (uint32_t x = co.mn[0]; x <= co.mx[0]; x++) {
-            const f3 w = cell_w(x, y, z, co.pin);
-            const int4 gv = *reinterpret_cast<const int4*>(&dv[grid_idx(d, x, y, z)]);
-            vel = vel + w * mk3(grid_decode(gv.x), grid_decode(gv.y), grid_decode(gv.z));
-            dens = dens + length(w) * grid_decode(gv.w);
-          }
-      f3 force = gravity;
-      const float windCell = satf(mixf(u->windStrengthLull, 1.0f, wstr));
-      force = force + windDir * fabsf(u->wind[3] * jitterStr * windCell);
-      force = force + grad * u->volumePreservation;
-      const f3 gridDisp = dens > 0.0f ? vel / dens : mk3(0.0f, 0.0f, 0.0f);
-      // verletIntegration — integration.wgsl.ts:8-28
-      const f3 fdisp = mix(pn - pp, gridDisp, u->friction);
-      const f3 next = (pn + fdisp) + (force * dt) * dt;
-      const f3 rr = mix(pp, next, pn4.w);
-      r = make_float4(rr.x, rr.y, rr.z, pn4.w);
-    }
-    A[o] = r;
+  }
+  __syncthreads();
+  // The point-local phases walk LDS in its own order, o = point * L + strand,
+  // so neighbouring threads touch neighbouring words.
+  const uint32_t nO = P * L;
+  // Verlet integration with grid forces, points 1..P-1 (:125-162)
+  for (uint32_t o = tid + L; o < nO; o += kSimThreads) {
+    const uint32_t ls = o % L;
+    if (ls >= nStr) continue;
+    const float4 pn4 = A[o], pp4 = K[o];
+    // wind jitter is per strand (:134-139)
+    const float timer = (float)u->frameIdx * 0.73f + u->windPhaseOffset * (float)(s0 + ls);
+    const float jitter = fractf(timer * u->windStrengthFrequency);
+    const float jitterDelta = mixf(-0.5f * u->windStrengthJitter, 0.5f * u->windStrengthJitter, jitter);
+    const float jitterStr = 1.0f + jitterDelta;
+    const f3 pn = mk3(pn4.x, pn4.y, pn4.z), pp = mk3(pp4.x, pp4.y, pp4.z);
+    const Cell co = grid_cell(d, gmin, gmax, pn);
+    f3 grad = mk3(0.0f, 0.0f, 0.0f), vel = mk3(0.0f, 0.0f, 0.0f);
+    float wstr = 0.0f, dens = 0.0f;
+    // _getGridDensityGradAndWind (densityGradAndWindGrid.wgsl.ts:20-44)
+    for (uint32_t z = co.mn[2]; z <= co.mx[2]; z++)
+      for (uint32_t y = co.mn[1]; y <= co.mx[1]; y++)
+        for (uint32_t x = co.mn[0]; x <= co.mx[0]; x++) {
+          const f3 w = cell_w(x, y, z, co.pin);
+          const float4 gw = *reinterpret_cast<const float4*>(&dgw[grid_idx(d, x, y, z)]);
+          grad = grad + w * mk3(gw.x, gw.y, gw.z);
+          wstr = wstr + length(w) * gw.w;
+        }
+    // _getGridDensityVelocity (densityVelocityGrid.wgsl.ts:30-68)
+    for (uint32_t z = co.mn[2]; z <= co.mx[2]; z++)
+      for (uint32_t y = co.mn[1]; y <= co.mx[1]; y++)
+        for (uint32_t x = co.mn[0]; x <= co.mx[0]; x++) {
+          const f3 w = cell_w(x, y, z, co.pin);
+          const int4 gv = *reinterpret_cast<const int4*>(&dv[grid_idx(d, x, y, z)]);
+          vel = vel + w * mk3(grid_decode(gv.x), grid_decode(gv.y), grid_decode(gv.z));
+          dens = dens + length(w) * grid_decode(gv.w);
+        }
+    f3 force = gravity;
+    const float windCell = satf(mixf(u->windStrengthLull, 1.0f, wstr));
+    force = force + windDir * fabsf(u->wind[3] * jitterStr * windCell);
+    force = force + grad * u->volumePreservation;
+    const f3 gridDisp = dens > 0.0f ? vel / dens : mk3(0.0f, 0.0f, 0.0f);
+    // verletIntegration — integration.wgsl.ts:8-28
+    const f3 fdisp = mix(pn - pp, gridDisp, u->friction);
+    const f3 next = (pn + fdisp) + (force * dt) * dt;
+    const f3 rr = mix(pp, next, pn4.w);
+    A[o] = make_float4(rr.x, rr.y, rr.z, pn4.w);
   }
   __syncthreads();
   // Local-shape constants depend only on the initial pose (constraints.wgsl.ts:65-82).
-  for (uint32_t e = tid; e < nElem; e += kSimThreads) {
-    const uint32_t ls = e / P, pt = e - ls * P;
-    if (pt + 3u >= P) continue;
-    const uint32_t o = pt * L + ls;
+  for (uint32_t o = tid; o + 3u * L < nO; o += kSimThreads) {
+    if (o % L >= nStr) continue;
     const float4 i0 = Bi[o], i1 = Bi[o + L], i2 = Bi[o + 2 * L];
     const f3 ci = ((mk3(i0.x, i0.y, i0.z) + mk3(i1.x, i1.y, i1.z)) + mk3(i2.x, i2.y, i2.z)) / 3.0f;
     const float h0 = length(ci - mk3(i1.x, i1.y, i1.z));
@@ -246,10 +251,9 @@ __global__ __launch_bounds__(kSimThreads) void sim_integrate_kernel(
     }
     __syncthreads();
     // global shape — constraints.wgsl.ts:31-49
-    for (uint32_t e = tid; e < nElem; e += kSimThreads) {
-      const uint32_t ls = e / P, pt = e - ls * P;
-      if (pt == 0u) continue;
-      const uint32_t o = pt * L + ls;
+    for (uint32_t o = tid + L; o < nO; o += kSimThreads) {
+      if (o % L >= nStr) continue;
+      const uint32_t pt = o / L;
       const float x = (float)pt / invPm1;
       const float att = 1.0f - satf((x - u->globalConstrExtent) / u->globalConstrFade);
       const float k = kGlobal * att;
@@ -293,10 +297,9 @@ __global__ __launch_bounds__(kSimThreads) void sim_integrate_kernel(
     }
     __syncthreads();
     // sphere + SDF collisions — collisions.wgsl.ts:4-61, :237-261
-    for (uint32_t e = tid; e < nElem; e += kSimThreads) {
-      const uint32_t ls = e / P, pt = e - ls * P;
-      if (pt == 0u) continue;
-      const uint32_t o = pt * L + ls;
+    for (uint32_t o = tid + L; o < nO; o += kSimThreads) {
+      if (o % L >= nStr) continue;
+      const uint32_t pt = o / L;
       const float4 p4 = A[o];
       f3 p = mk3(p4.x, p4.y, p4.z);
       const f3 v = p - xyz(sphere);

@@ -169,7 +169,7 @@ class HairTilesPass:
         self.hairTileSegmentsBuffer = torch.zeros(entries * 2, dtype=torch.int32, device=device)
         self.segmentsCapacity = entries - 1
         self.segmentCountPerTileBuffer = torch.zeros(tiles, dtype=torch.int32, device=device)
-        self.slotTileBinBuffer = torch.zeros(max(1, self.segmentsCapacity), dtype=torch.int32, device=device)
+        self.slotTileBinBuffer = torch.zeros(2 * max(1, self.segmentsCapacity), dtype=torch.int32, device=device)
 
     def cmdClearBeforeRender(self, ctx: PassCtx) -> None:
         w, h = ctx.viewport

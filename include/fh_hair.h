@@ -325,9 +325,11 @@ int fh_tiles_clear(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w,
 
 /* HairTilesPass.cmdDrawHairToTiles (hairTilesPass.ts:78-114; kernel
  * hairTilesPass.wgsl.ts:70-246). tile_segments is DrawnHairSegments holding
- * segments_capacity elements. slot_tile_bin (may be NULL) receives, for each
- * list slot, the (tile, bin) index the element was linked into; the fine pass
- * uses it to gather each list without walking it. */
+ * segments_capacity elements. slot_tile_bin (may be NULL; 2 u32 per element)
+ * receives, for each list slot k, slot_tile_bin[2k] = the (tile, bin) index
+ * the element was linked into and slot_tile_bin[2k + 1] = its rank in that
+ * list, in [0, HairTileResult.padding0). The fine pass uses both to gather
+ * each list into a contiguous range without walking it or using atomics. */
 int fh_tiles_draw(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w,
                   uint32_t vp_h, const fh_render_uniforms* uniforms,
                   const fh_hair_data* hair_data, const fh_vec4* positions,
@@ -338,7 +340,8 @@ int fh_tiles_draw(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w,
                   uint32_t rendered_strands, uint32_t points_per_strand);
 
 /* For lists produced by another implementation of the tile pass: walks every
- * (tile, bin) list and fills slot_tile_bin and the per-bin counts (padding0). */
+ * (tile, bin) list and fills slot_tile_bin (tag and rank, as fh_tiles_draw
+ * writes them) and the per-bin counts (padding0). */
 int fh_tiles_rebuild_slot_keys(fh_ctx* ctx, const fh_render_cfg* cfg,
                                uint32_t vp_w, uint32_t vp_h,
                                fh_tile_result* tiles, const void* tile_segments,

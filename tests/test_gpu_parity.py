@@ -133,6 +133,17 @@ def _tile_inputs(r):
                 depth=np.ascontiguousarray(r.inputs["meshDepth"].cpu().numpy()))
 
 
+def _check_slot_tags(tags_ranks, counts, n):
+    """Every slot names a (tile, bin) and a rank; each list's ranks are 0 .. count-1."""
+    tags, ranks = tags_ranks[0:2 * n:2], tags_ranks[1:2 * n:2]
+    assert np.array_equal(np.bincount(tags, minlength=counts.shape[0]), counts)
+    order = np.lexsort((ranks, tags))
+    st, sr = tags[order], ranks[order]
+    first = np.r_[True, st[1:] != st[:-1]]
+    run_start = np.maximum.accumulate(np.where(first, np.arange(n), 0))
+    assert np.array_equal(sr, np.arange(n) - run_start)
+
+
 @pytest.mark.parametrize("tile,bins", [(16, 4), (8, 8), (32, 4)])
 def test_tiles_bit_exact(gpu, oracle, tile, bins):
     import torch
@@ -159,11 +170,11 @@ def test_tiles_bit_exact(gpu, oracle, tile, bins):
     gl, el = walk_lists(got_tiles, got_seg, tp.segmentsCapacity), walk_lists(et, es, tp.segmentsCapacity)
     assert gl == el
     n = min(int(got_seg[0]), tp.segmentsCapacity)
-    # every slot is tagged with the (tile, bin) whose list holds it
+    # every slot is tagged with the (tile, bin) whose list holds it and with a
+    # rank in that list; the ranks of one list are 0 .. count-1
     for tb, vals in gl.items():
         assert len(vals) == g4[tb, 3]
-    tagged = np.bincount(got_keys[:n], minlength=g4.shape[0])
-    assert np.array_equal(tagged, g4[:, 3])
+    _check_slot_tags(got_keys, g4[:, 3], n)
 
 
 def test_tile_sort(gpu, oracle):
@@ -223,6 +234,43 @@ def test_fine_pass_image(gpu, oracle, tile, bins, slices):
     assert np.abs(to_u8(ge) - to_u8(ee)).max() <= 2
     # one f16 unit in the last place at 0.5 is 4.9e-4: a rounding flip of a packed fragment colour
     assert diff.max() <= 5e-4
+
+
+def test_rebuilt_slot_tags(gpu):
+    """fh_tiles_rebuild_slot_keys (for lists another tile pass wrote) tags every
+    slot with its list and a rank, and the fine pass draws the same image from
+    those tags as from the tile pass's own."""
+    import torch
+    from frostbitten_hair_webgpu_amd import native as N
+    wl, r = _renderer("small", 1200, 320, 240)
+    _step_to(r, 2)
+    ctx = r.ctx()
+    tp = r.tilesPass
+    tp.cmdClearBeforeRender(ctx)
+    r.finePass.cmdClearBeforeRender(ctx)
+    r.sortPass.cmdClearBeforeRender(ctx)
+    tp.cmdDrawHairToTiles(ctx, r.hair)
+    r.sortPass.cmdSortHairTiles(ctx)
+    r.finePass.cmdRasterizeSlicesHair(ctx, r.hair, tp)
+    torch.cuda.synchronize()
+    img1 = _np(r.finePass.hairRasterizerResultsBuffer).copy()
+    tags1 = _np(tp.slotTileBinBuffer).view(np.uint32).copy()
+    counts = _np(tp.hairTilesBuffer).view(np.uint32).reshape(-1, 4)[:, 3].copy()
+    n = min(int(_np(tp.hairTileSegmentsBuffer).view(np.uint32)[0]), tp.segmentsCapacity)
+    assert n > 0
+    rebuilt = torch.full_like(tp.slotTileBinBuffer, -1)
+    ctx.native.call("fh_tiles_rebuild_slot_keys", ctx.cfg, wl.width, wl.height, N.ptr(tp.hairTilesBuffer),
+                    N.ptr(tp.hairTileSegmentsBuffer), tp.segmentsCapacity, N.ptr(rebuilt))
+    torch.cuda.synchronize()
+    tags2 = _np(rebuilt).view(np.uint32).copy()
+    assert np.array_equal(tags2[0:2 * n:2], tags1[0:2 * n:2])
+    assert np.array_equal(_np(tp.hairTilesBuffer).view(np.uint32).reshape(-1, 4)[:, 3], counts)
+    _check_slot_tags(tags2, counts, n)
+    tp.slotTileBinBuffer = rebuilt
+    r.finePass.cmdClearBeforeRender(ctx)
+    r.finePass.cmdRasterizeSlicesHair(ctx, r.hair, tp)
+    torch.cuda.synchronize()
+    assert np.array_equal(_np(r.finePass.hairRasterizerResultsBuffer), img1)
 
 
 def test_shading(gpu, oracle):
