@@ -661,22 +661,24 @@ __global__ __launch_bounds__(256) void bin_sort_large_kernel(const fh_tile_resul
 // ---------------------------------------------------------------------------
 // K7 — hairFinePass.wgsl.ts:126-335 with processHairSegment.wgsl.ts:5-70 and
 // reduceHairSlices.wgsl.ts:9-78. Persistent workgroups ("processors") pop
-// tiles from the sorted tile list; one pixel per thread. Per depth bin, the
-// bin's segments arrive sorted by blend key (largest first) in batches:
-//  * one thread per segment projects it into LDS (64 B record: quad, depths,
-//    pixel box relative to the tile, nearest depth);
-//  * each pixel walks the batch in order and runs the cheap tests (box,
-//    a conservative depth pre-test against the mesh depth, four edge tests);
-//    a pixel that passes takes a pool slot, links it at the head of its own
-//    list and queues (slot, segment, pixel) in its wave's LDS ring;
-//  * every 64 queued fragments the wave computes them together: segment-space
-//    coordinates, depth test, slice-memory budget (perProc per bin, as
-//    _hasMoreSliceDataSlots), coverage alpha, shading and slice index;
-//  * the pixel then blends its list once per slice, front to back; within a
-//    slice the list order is ascending blend key.
-// Pool slots live in LDS first, then in the processor's region of the
-// reference's slice buffer (perProc entries); a fragment rejected after it
-// was queued keeps its slot with a skip tag.
+// tiles from the sorted tile list. A workgroup has NW waves; wave w owns the
+// pixel rows [w * TILE / NW, (w + 1) * TILE / NW) of the tile. Per depth bin,
+// the bin's segments arrive sorted by blend key (largest first) in batches:
+//  * one thread per segment projects it into LDS (field-major records: quad,
+//    depths, hull slopes, pixel box relative to the tile, nearest depth);
+//  * each wave enumerates the (segment, pixel) candidates of its rows,
+//    record by record (box rows x hull spans), 64 per step, runs the edge
+//    tests and a conservative depth pre-test, and queues the covered pixels
+//    in its LDS ring in that order;
+//  * every 64 queued pixels the wave computes the fragments together:
+//    segment-space coordinates, depth test, then a slot of the slice memory
+//    (perProc per bin, _hasMoreSliceDataSlots), coverage alpha, shading and
+//    slice index; each fragment is linked at the head of its (pixel, slice)
+//    list. Two lanes linking to one pixel in one step do it in lane order
+//    (queue order), so each list reads back in ascending blend key;
+//  * each pixel then blends its slices front to back.
+// Slots live in LDS first, then in the processor's region of the reference's
+// slice buffer.
 // ---------------------------------------------------------------------------
 struct FineArgs {
   const fh_render_uniforms* u;
@@ -694,7 +696,7 @@ struct FineArgs {
   const uint32_t* tileList;
   uint32_t S, P;
   uint32_t B, NS, PS;
-  uint32_t perProc, ldsPool, physCap, batch;
+  uint32_t perProc, ldsPool, batch;
   uint32_t capacity;  // elements of the gathered segment array
   uint32_t alphaQuadratic;
   float alphaMultiplier;
@@ -725,9 +727,11 @@ struct RecStore {
 
 template <int TILE>
 constexpr uint32_t fine_batch() { return TILE * TILE < 256 ? TILE * TILE : 256; }
+// waves per workgroup: 2 pixel rows per wave for 16x16 and 32x32 tiles
+template <int TILE>
+constexpr uint32_t fine_waves() { return TILE == 8 ? 1u : (uint32_t)TILE / 2u; }
 
-constexpr uint32_t kFineRing = 128;      // per-wave ring of queued fragments
-constexpr uint32_t kSkipTag = 0xffffffffu;
+constexpr uint32_t kFineRing = 128;  // per-wave ring of queued pixels
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -737,34 +741,34 @@ __device__ __forceinline__ void wave_sync() {
 template <int TILE>
 struct FineLds {
   static constexpr uint32_t NPX = TILE * TILE;
-  static constexpr uint32_t NW = NPX / 64;
+  static constexpr uint32_t NW = fine_waves<TILE>();
+  static constexpr uint32_t PXW = NPX / NW;  // pixels per wave
   RecStore<fine_batch<TILE>()> recs;
-  uint32_t* heads;   // NPX: newest pool slot of each pixel
+  uint32_t* heads;   // NS * NPX: newest slot of each (slice, pixel) list
   float* db;         // NPX: mesh depth
   uint32_t* done;    // NPX: pixel saturated
-  uint32_t* claim;   // NPX: lowest lane pushing to the pixel (64 = none)
+  uint32_t* claim;   // NPX: lowest lane linking to the pixel (64 = none)
   uint32_t* ring;    // NW * kFineRing
   uint32_t* wsc;     // NW * 5 * 64: per-wave enumeration scratch
-  uint4* pool;       // ldsPool
+  uint2* poolColor;  // ldsPool: f16 rg, ba
+  uint32_t* poolNext;
 };
 
-// The queued fragments [head, head + n) of wave `w`, n <= 64, one per lane.
+// The queued pixels [head, head + n) of wave `w`, n <= 64, one per lane.
 template <int TILE>
 __device__ __forceinline__ void fine_shade_queue(const FineArgs& a, const FineLds<TILE>& L, uint4* gpool,
                                                  uint32_t w, uint32_t head, uint32_t n, uint32_t bx0,
                                                  uint32_t by0, f2 td, float invSegs, uint32_t* sliceOff) {
   const uint32_t lane = __lane_id();
   bool accept = false;
-  uint32_t slot = 0;
+  uint32_t j = 0, pix = 0;
   f2 wq = mk2(0.0f, 0.0f);
   float hdp = 0.0f;
-  uint32_t j = 0;
   const auto& R = L.recs;
   if (lane < n) {
     const uint32_t e = L.ring[w * kFineRing + ((head + lane) & (kFineRing - 1u))];
-    slot = e >> 14;
-    j = (e >> 6) & 0xffu;
-    const uint32_t pix = w * 64u + (e & 63u);
+    j = e >> 8;
+    pix = w * FineLds<TILE>::PXW + (e & 0xffu);
     const f2 p = mk2((float)(bx0 + pix % TILE), (float)(by0 + pix / TILE));
     Seg sg;
     sg.v00 = mk2(R.at(kRecV + 0, j), R.at(kRecV + 1, j)); sg.v01 = mk2(R.at(kRecV + 2, j), R.at(kRecV + 3, j));
@@ -774,55 +778,64 @@ __device__ __forceinline__ void fine_shade_queue(const FineArgs& a, const FineLd
     hdp = interp_f32(wq, sg.depths);
     accept = !(hdp >= L.db[pix]);  // depth test with GL_LESS (:31-34)
   }
-  // _hasMoreSliceDataSlots (hairFinePass.wgsl.ts:252-264): perProc per bin
+  // a slot of the slice memory: _hasMoreSliceDataSlots (hairFinePass.wgsl.ts:252-264)
   const uint64_t m = __ballot(accept);
+  if (m == 0) return;
   uint32_t base = 0;
-  if (m) {
-    const uint32_t leader = (uint32_t)__builtin_ctzll(m);
-    if (lane == leader) base = atomicAdd(sliceOff, (uint32_t)__popcll(m));
-    base = (uint32_t)__shfl((int)base, (int)leader);
+  const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+  if (lane == leader) base = atomicAdd(sliceOff, (uint32_t)__popcll(m));
+  base = (uint32_t)__shfl((int)base, (int)leader);
+  const uint32_t slot = base + mask_prefix(m);
+  if (accept && slot >= a.perProc) accept = false;
+  uint32_t tag = 0;
+  if (accept) {
+    // getAlphaCoverage (processHairSegment.wgsl.ts:61-70)
+    float alpha = 1.0f - fabsf(wq.x * 2.0f - 1.0f);
+    if (a.alphaQuadratic) alpha = sqrtf(alpha);
+    alpha = satf(alpha * a.alphaMultiplier);
+    const uint32_t key = R.u(kRecKey, j);
+    const uint32_t strand = key >> 8, seg = key & 0xffu;
+    const float tFull = ((float)seg + wq.y) / invSegs;
+    // _sampleShading — hairShadingBuffer.ts:29-38
+    const float sa = tFull * ((float)a.PS - 1.0f);
+    const uint32_t au = f2u(sa);
+    const uint32_t i0 = clampu(au, 0u, a.PS - 1u), i1 = clampu(au + 1u, 0u, a.PS - 1u);
+    const float fr = fractf(sa);
+    const float4 c0 = a.shading[(size_t)strand * a.PS + i0];
+    const float4 c1 = a.shading[(size_t)strand * a.PS + i1];
+    const float cr = mixf(c0.x, c1.x, fr), cg = mixf(c0.y, c1.y, fr);
+    const float cb = mixf(c0.z, c1.z, fr), ca = mixf(c0.w, c1.w, fr) * alpha;
+    const uint2 col = make_uint2(pack_half2(cr, cg), pack_half2(cb, ca));
+    tag = depth_bin(a.NS, td, hdp);  // getSliceIdx (:72-75)
+    if (slot < a.ldsPool) L.poolColor[slot] = col;
+    else *reinterpret_cast<uint2*>(&gpool[slot - a.ldsPool]) = col;
   }
-  if (lane < n) {
-    if (accept && base + mask_prefix(m) >= a.perProc) accept = false;
-    uint32_t rg = 0, ba = 0, tag = kSkipTag;
-    if (accept) {
-      // getAlphaCoverage (processHairSegment.wgsl.ts:61-70)
-      float alpha = 1.0f - fabsf(wq.x * 2.0f - 1.0f);
-      if (a.alphaQuadratic) alpha = sqrtf(alpha);
-      alpha = satf(alpha * a.alphaMultiplier);
-      const uint32_t key = R.u(kRecKey, j);
-      const uint32_t strand = key >> 8, seg = key & 0xffu;
-      const float tFull = ((float)seg + wq.y) / invSegs;
-      // _sampleShading — hairShadingBuffer.ts:29-38
-      const float sa = tFull * ((float)a.PS - 1.0f);
-      const uint32_t au = f2u(sa);
-      const uint32_t i0 = clampu(au, 0u, a.PS - 1u), i1 = clampu(au + 1u, 0u, a.PS - 1u);
-      const float fr = fractf(sa);
-      const float4 c0 = a.shading[(size_t)strand * a.PS + i0];
-      const float4 c1 = a.shading[(size_t)strand * a.PS + i1];
-      const float cr = mixf(c0.x, c1.x, fr), cg = mixf(c0.y, c1.y, fr);
-      const float cb = mixf(c0.z, c1.z, fr), ca = mixf(c0.w, c1.w, fr) * alpha;
-      rg = pack_half2(cr, cg);
-      ba = pack_half2(cb, ca);
-      tag = depth_bin(a.NS, td, hdp);  // getSliceIdx (:72-75)
+  // link at the head of the (slice, pixel) list, one lane per pixel per round
+  bool pend = accept;
+  while (__ballot(pend)) {
+    if (pend) atomicMin(&L.claim[pix], lane);
+    wave_sync();
+    if (pend && L.claim[pix] == lane) {
+      uint32_t* hp = &L.heads[tag * FineLds<TILE>::NPX + pix];
+      if (slot < a.ldsPool) L.poolNext[slot] = *hp;
+      else gpool[slot - a.ldsPool].z = *hp;
+      *hp = slot;
+      L.claim[pix] = 64u;
+      pend = false;
     }
-    uint4* e = slot < a.ldsPool ? &L.pool[slot] : &gpool[slot - a.ldsPool];
-    e->x = rg; e->y = ba; e->w = tag;
+    wave_sync();
   }
 }
 
-// Cheap tests of one batch for wave `w`, which owns the pixel rows
-// [w * 64 / TILE, (w + 1) * 64 / TILE) of the tile. Candidates are enumerated
-// record by record (records are in blend order), row by row within the
-// record's box and its hull span, 64 per step; a pixel that passes is linked
-// at the head of its list. Two lanes may push to the same pixel in one step:
-// they link in lane order (one lane per pixel per round), which is record
-// order, so every list stays in blend order.
+// Cheap tests of one batch for wave `w`. Candidates are enumerated record by
+// record (records are in blend order), row by row within the record's box and
+// its hull span, 64 per step; covered pixels are queued in that order.
 template <int TILE>
 __device__ __forceinline__ void fine_cheap_batch(const FineArgs& a, const FineLds<TILE>& L, uint4* gpool,
                                                  uint32_t nb, uint32_t bx0, uint32_t by0, f2 td,
-                                                 float invSegs, uint32_t* sliceOff, uint32_t* phys) {
-  constexpr int RW = 64 / TILE;
+                                                 float invSegs, uint32_t* sliceOff) {
+  constexpr uint32_t PXW = FineLds<TILE>::PXW;
+  constexpr int RW = (int)(PXW / TILE);
   const uint32_t lane = __lane_id();
   const uint32_t w = threadIdx.x >> 6;
   const int wy0 = (int)w * RW, wy1 = wy0 + RW - 1;
@@ -831,11 +844,11 @@ __device__ __forceinline__ void fine_cheap_batch(const FineArgs& a, const FineLd
   uint32_t* itemY = L.wsc + (w * 5u + 2u) * 64u;
   uint32_t* itemX = L.wsc + (w * 5u + 3u) * 64u;
   uint32_t* candEnd = L.wsc + (w * 5u + 4u) * 64u;
+  const auto& R = L.recs;
   uint32_t qh = 0, qt = 0;
   for (uint32_t rb = 0; rb < nb; rb += 64u) {
     const uint32_t jr = rb + lane;
     uint32_t nrows = 0;
-    const auto& R = L.recs;
     if (jr < nb) {
       const int ya = max(R.hi16(kRecBox0, jr), wy0), yb = min(R.hi16(kRecBox1, jr), wy1);
       const int xa = max(R.lo16(kRecBox0, jr), 0), xb = min(R.lo16(kRecBox1, jr), TILE - 1);
@@ -876,45 +889,26 @@ __device__ __forceinline__ void fine_cheap_batch(const FineArgs& a, const FineLd
       for (uint32_t c0 = 0; c0 < nCand; c0 += 64u) {
         const uint32_t c = c0 + lane;
         bool push = false;
-        uint32_t pix = 0, j = 0;
+        uint32_t entry = 0;
         if (c < nCand) {
           const uint32_t it = search64(candEnd, c);
           const uint32_t x = itemX[it] + (c - (it ? candEnd[it - 1u] : 0u));
           const uint32_t y = itemY[it];
-          j = itemRec[it];
-          pix = y * TILE + x;
+          const uint32_t j = itemRec[it];
+          const uint32_t pix = y * TILE + x;
           if (!L.done[pix] && !(R.at(kRecNear, j) >= L.db[pix])) {
             Seg sg;
             sg.v00 = mk2(R.at(kRecV + 0, j), R.at(kRecV + 1, j)); sg.v01 = mk2(R.at(kRecV + 2, j), R.at(kRecV + 3, j));
             sg.v10 = mk2(R.at(kRecV + 4, j), R.at(kRecV + 5, j)); sg.v11 = mk2(R.at(kRecV + 6, j), R.at(kRecV + 7, j));
             push = inside_quad(sg, mk2((float)(bx0 + x), (float)(by0 + y)));
           }
+          entry = (j << 8) | (pix - w * PXW);
         }
         const uint64_t m = __ballot(push);
         if (m == 0) continue;
-        uint32_t base = 0;
-        const uint32_t leader = (uint32_t)__builtin_ctzll(m);
-        if (lane == leader) base = atomicAdd(phys, (uint32_t)__popcll(m));
-        base = (uint32_t)__shfl((int)base, (int)leader);
-        const uint32_t slot = base + mask_prefix(m);
-        bool pend = push && slot < a.physCap;
-        const uint64_t mk = __ballot(pend);
-        if (pend)
-          L.ring[w * kFineRing + ((qt + mask_prefix(mk)) & (kFineRing - 1u))] =
-              (slot << 14) | (j << 6) | (pix - w * 64u);
-        while (__ballot(pend)) {
-          if (pend) atomicMin(&L.claim[pix], lane);
-          wave_sync();
-          if (pend && L.claim[pix] == lane) {
-            uint4* e = slot < a.ldsPool ? &L.pool[slot] : &gpool[slot - a.ldsPool];
-            e->z = L.heads[pix];
-            L.heads[pix] = slot;
-            L.claim[pix] = 64u;
-            pend = false;
-          }
-          wave_sync();
-        }
-        qt += (uint32_t)__popcll(mk);
+        if (push) L.ring[w * kFineRing + ((qt + mask_prefix(m)) & (kFineRing - 1u))] = entry;
+        qt += (uint32_t)__popcll(m);
+        wave_sync();
         if (qt - qh >= 64u) {
           fine_shade_queue<TILE>(a, L, gpool, w, qh, 64u, bx0, by0, td, invSegs, sliceOff);
           qh += 64u;
@@ -926,26 +920,27 @@ __device__ __forceinline__ void fine_cheap_batch(const FineArgs& a, const FineLd
 }
 
 template <int TILE>
-__global__ __launch_bounds__(TILE* TILE) void fine_raster_kernel(FineArgs a) {
+__global__ __launch_bounds__(fine_waves<TILE>() * 64) void fine_raster_kernel(FineArgs a) {
   constexpr uint32_t NPX = TILE * TILE;
-  constexpr uint32_t NW = NPX / 64;
+  constexpr uint32_t NW = fine_waves<TILE>();
+  constexpr uint32_t RB = fine_batch<TILE>();
   extern __shared__ __align__(16) unsigned char smem[];
   FineLds<TILE> L;
-  constexpr uint32_t RB = fine_batch<TILE>();
   L.recs.f = reinterpret_cast<float*>(smem);
   L.heads = reinterpret_cast<uint32_t*>(smem + (size_t)RB * kRecFields * 4);
-  L.db = reinterpret_cast<float*>(L.heads + NPX);
+  L.db = reinterpret_cast<float*>(L.heads + (size_t)a.NS * NPX);
   L.done = reinterpret_cast<uint32_t*>(L.db + NPX);
   L.claim = L.done + NPX;
   L.ring = L.claim + NPX;
   L.wsc = L.ring + NW * kFineRing;
-  L.pool = reinterpret_cast<uint4*>(L.wsc + NW * 5u * 64u);
+  L.poolColor = reinterpret_cast<uint2*>(L.wsc + NW * 5u * 64u);
+  L.poolNext = reinterpret_cast<uint32_t*>(L.poolColor + a.ldsPool);
   __shared__ uint32_t s_tile;
-  __shared__ uint32_t s_sliceOff;  // accepted fragments of the bin
-  __shared__ uint32_t s_phys;      // pool slots taken in the bin
+  __shared__ uint32_t s_sliceOff;  // slots of the slice memory taken in the bin
 
   const fh_render_uniforms* __restrict__ u = a.u;
   const uint32_t tid = threadIdx.x;
+  const bool pixThread = tid < NPX;
   const uint32_t lx = tid % TILE, ly = tid / TILE;
   const f2 vpf = mk2(u->viewport[0], u->viewport[1]);
   const TileGeom tg = tile_geom(vpf, TILE);
@@ -958,8 +953,8 @@ __global__ __launch_bounds__(TILE* TILE) void fine_raster_kernel(FineArgs a) {
   const float invSegs = (float)(a.P - 1u);
   uint4* gpool = a.slicePool + (size_t)blockIdx.x * a.perProc;
 
-  L.heads[tid] = kInvalidPtr;
-  L.claim[tid] = 64u;
+  for (uint32_t i = tid; i < a.NS * NPX; i += blockDim.x) L.heads[i] = kInvalidPtr;
+  if (pixThread) L.claim[tid] = 64u;
 
   while (true) {
     __syncthreads();
@@ -975,11 +970,13 @@ __global__ __launch_bounds__(TILE* TILE) void fine_raster_kernel(FineArgs a) {
     const uint32_t bx0 = clampu(tx * TILE, 0u, tg.vpW - 1u), by0 = clampu(ty * TILE, 0u, tg.vpH - 1u);
     const uint32_t px = bx0 + lx, py = by0 + ly;
     const f2 p = mk2((float)px, (float)py);
-    const bool readable = px < tg.vpW && py < tg.vpH && (tg.vpW * py + px) < fbLen;
+    const bool readable = pixThread && px < tg.vpW && py < tg.vpH && (tg.vpW * py + px) < fbLen;
     const uint32_t fbIdx = tg.vpW * py + px;
-    L.db[tid] = tex_load(a.depth, a.depthW, a.depthH, (int)px, f2i(vpf.y - p.y));
-    L.done[tid] = 0u;
-    bool done = false;
+    if (pixThread) {
+      L.db[tid] = tex_load(a.depth, a.depthW, a.depthH, (int)px, f2i(vpf.y - p.y));
+      L.done[tid] = 0u;
+    }
+    bool done = !pixThread;
     f4 fc = mk4(0.0f, 0.0f, 0.0f, 0.0f);
     if (readable) { const float4 v = a.fb[fbIdx]; fc = mk4(v.x, v.y, v.z, v.w); }
 
@@ -991,7 +988,7 @@ __global__ __launch_bounds__(TILE* TILE) void fine_raster_kernel(FineArgs a) {
                         (float)tr.maxDepth / 4294967296.0f);
       const uint32_t start = min(a.binStart[tb], a.capacity);
       const uint32_t n = min(tr.padding0, a.capacity - start);
-      if (tid == 0) { s_sliceOff = 0; s_phys = 0; }
+      if (tid == 0) s_sliceOff = 0;
       for (uint32_t b0 = 0; b0 < n; b0 += a.batch) {
         const uint32_t nb = min(a.batch, n - b0);
         __syncthreads();
@@ -1033,7 +1030,7 @@ __global__ __launch_bounds__(TILE* TILE) void fine_raster_kernel(FineArgs a) {
           R.at(kRecNear, tid) = finite ? dn - (dm * 1e-6f + 1e-30f) : -INFINITY;
         }
         __syncthreads();
-        fine_cheap_batch<TILE>(a, L, gpool, nb, bx0, by0, td, invSegs, &s_sliceOff, &s_phys);
+        fine_cheap_batch<TILE>(a, L, gpool, nb, bx0, by0, td, invSegs, &s_sliceOff);
         __syncthreads();
         if (s_sliceOff >= a.perProc) break;  // slice memory exhausted (:252-264)
       }
@@ -1042,22 +1039,27 @@ __global__ __launch_bounds__(TILE* TILE) void fine_raster_kernel(FineArgs a) {
       if (!done) {
         if (!readable) fc = mk4(0.0f, 0.0f, 0.0f, 0.0f);
         uint32_t cnt = isDbg ? f2u(fc.x * (float)dbgMax) : 0u;
-        const uint32_t head = L.heads[tid];
-        L.heads[tid] = kInvalidPtr;
         for (uint32_t s = 0; s < a.NS; s++) {
+          uint32_t cur = L.heads[s * NPX + tid];
+          L.heads[s * NPX + tid] = kInvalidPtr;
           if (!isDbg && fc.w >= 0.999f) continue;
-          uint32_t cur = head;
           while (cur != kInvalidPtr) {
             if (!isDbg && fc.w >= 0.999f) break;
-            const uint4 e = cur < a.ldsPool ? L.pool[cur] : gpool[cur - a.ldsPool];
-            cur = e.z;
-            if (e.w != s) continue;
+            uint2 col;
+            if (cur < a.ldsPool) {
+              col = L.poolColor[cur];
+              cur = L.poolNext[cur];
+            } else {
+              const uint4 e = gpool[cur - a.ldsPool];
+              col = make_uint2(e.x, e.y);
+              cur = e.z;
+            }
             cnt++;
             const float oneMinusA = 1.0f - fc.w;
-            fc.x = fc.x + half_lo(e.x) * oneMinusA;
-            fc.y = fc.y + half_hi(e.x) * oneMinusA;
-            fc.z = fc.z + half_lo(e.y) * oneMinusA;
-            fc.w = fc.w + half_hi(e.y) * oneMinusA;
+            fc.x = fc.x + half_lo(col.x) * oneMinusA;
+            fc.y = fc.y + half_hi(col.x) * oneMinusA;
+            fc.z = fc.z + half_lo(col.y) * oneMinusA;
+            fc.w = fc.w + half_hi(col.y) * oneMinusA;
           }
         }
         if (isDbg) {
@@ -1314,37 +1316,37 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
   a.alphaQuadratic = cfg->alphaQuadratic; a.alphaMultiplier = cfg->alphaMultiplier;
   const uint32_t npx = T * T;
   a.batch = std::min(npx, 256u);
-  // LDS: segment batch, list heads, mesh depth, fragment rings, then the
-  // LDS part of the fragment pool. Under 80 KiB per workgroup keeps two
-  // 16x16 processors per CU (160 KiB LDS on gfx950).
+  // LDS: segment batch, slice heads, per-pixel state, fragment rings and
+  // enumeration scratch, then the LDS part of the slice memory (12 B per
+  // fragment). Under 80 KiB per workgroup keeps two 16x16 processors per CU
+  // (160 KiB LDS on gfx950).
+  const uint32_t nw = T == 8 ? 1u : T / 2u;
   const size_t ldsTarget = std::min<size_t>(T == 8 ? 40 * 1024 : (T == 16 ? 78 * 1024 : 150 * 1024),
                                             ctx->max_lds - 256);
-  const size_t fixed = (size_t)a.batch * kRecFields * 4 + (size_t)npx * 16 +
-                       (size_t)(npx / 64) * (kFineRing + 5 * 64) * 4;
-  FH_REQUIRE(fixed + 16 * 64 <= ldsTarget, "fh_fine_raster: LDS too small for the tile size");
-  // pool slot indices must fit the 18 bits of a ring entry
-  a.ldsPool = (uint32_t)std::min<size_t>((ldsTarget - fixed) / 16, 1u << 17);
-  a.physCap = (uint32_t)std::min<uint64_t>((uint64_t)a.ldsPool + a.perProc, 1u << 18);
-  const size_t lds = fixed + (size_t)a.ldsPool * 16;
+  const size_t fixed = (size_t)a.batch * kRecFields * 4 + (size_t)npx * 4 * (a.NS + 3) +
+                       (size_t)nw * (kFineRing + 5 * 64) * 4;
+  FH_REQUIRE(fixed + 12 * 64 <= ldsTarget, "fh_fine_raster: slicesPerPixel too large for the tile size");
+  a.ldsPool = (uint32_t)std::min<size_t>((ldsTarget - fixed) / 12, a.perProc);
+  const size_t lds = fixed + (size_t)a.ldsPool * 12;
   const uint32_t grid = cfg->processorCount;
   if (T == 8) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&fine_raster_kernel<8>),
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    const int pk9 = fh::prof_begin(ctx, "fine_raster");
-    fine_raster_kernel<8><<<grid, 64, lds, ctx->stream>>>(a);
-    fh::prof_end(ctx, pk9);
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const int pk10 = fh::prof_begin(ctx, "fine_raster");
+    fine_raster_kernel<8><<<grid, 64 * fine_waves<8>(), lds, ctx->stream>>>(a);
+    fh::prof_end(ctx, pk10);
   } else if (T == 16) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&fine_raster_kernel<16>),
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    const int pk10 = fh::prof_begin(ctx, "fine_raster");
-    fine_raster_kernel<16><<<grid, 256, lds, ctx->stream>>>(a);
-    fh::prof_end(ctx, pk10);
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const int pk11 = fh::prof_begin(ctx, "fine_raster");
+    fine_raster_kernel<16><<<grid, 64 * fine_waves<16>(), lds, ctx->stream>>>(a);
+    fh::prof_end(ctx, pk11);
   } else {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&fine_raster_kernel<32>),
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    const int pk11 = fh::prof_begin(ctx, "fine_raster");
-    fine_raster_kernel<32><<<grid, 1024, lds, ctx->stream>>>(a);
-    fh::prof_end(ctx, pk11);
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const int pk12 = fh::prof_begin(ctx, "fine_raster");
+    fine_raster_kernel<32><<<grid, 64 * fine_waves<32>(), lds, ctx->stream>>>(a);
+    fh::prof_end(ctx, pk12);
   }
   return check_launch("fine_raster_kernel");
 }
