@@ -75,7 +75,15 @@ struct TilesArgs {
   uint32_t* slotKey;
   uint32_t T, B, threshold;
   uint32_t S, P;
+  uint2* pieces;           // overflow queue: (segment, first pair) of wide segments
+  uint32_t* pieceCount;
+  uint32_t pieceCap;
 };
+
+// A segment whose rectangle has more than kTilePiece tiles is not drawn by its
+// own wave: it is queued in pieces of kTilePiece tiles that a second kernel
+// takes 64 at a time, so one wide segment spreads over many waves.
+constexpr uint32_t kTilePiece = 8;
 
 // order-preserving map of f32 to u32 (for LDS min/max of non-NaN floats)
 __device__ __forceinline__ uint32_t f32_ord(float f) {
@@ -116,6 +124,8 @@ struct TileWaveLds {
   float slope[6][64];      // dx/dy of the six vertex pairs (hull_strip_x)
   int box[4][64];          // pixel box (seg_pixel_bbox)
   uint32_t rect[3][64];    // first tile x, first tile y, tiles per row
+  uint32_t gid[64];        // segment index
+  uint32_t kBase[64];      // first pair (tile of the rectangle) of this lane
   uint32_t pairEnd[64];
   // per pair of the current window
   uint32_t owner[64];
@@ -194,13 +204,15 @@ __device__ __forceinline__ void tiles_shade_queue(const TilesArgs& a, TileWaveLd
   __syncthreads();
 }
 
-__global__ __launch_bounds__(64) void tiles_draw_kernel(TilesArgs a) {
-  __shared__ TileWaveLds L;
+// Steps 1-5 for one wave. Each lane holds segment `gid` (when `have`) and
+// draws the pairs [kBase, kBase + count) of its rectangle: in the main pass
+// kBase = 0 and count = all tiles (or none, once queued as pieces); in the
+// overflow pass one piece.
+__device__ __forceinline__ void tiles_wave(const TilesArgs& a, TileWaveLds& L, uint32_t gid, bool have,
+                                           uint32_t kBase, bool mainPass) {
   const fh_render_uniforms* __restrict__ u = a.u;
   const uint32_t lane = threadIdx.x;
   const uint32_t segsPerStrand = a.P - 1u;
-  const uint32_t gid = blockIdx.x * 64u + lane;
-  const uint32_t nSeg = a.S * segsPerStrand;
   const f2 vp = mk2(u->viewport[0], u->viewport[1]);
   const TileGeom tg = tile_geom(vp, a.T);
   const uint32_t T = a.T, B = a.B;
@@ -213,7 +225,7 @@ __global__ __launch_bounds__(64) void tiles_draw_kernel(TilesArgs a) {
 
   // 1. project
   uint32_t nTiles = 0;
-  if (gid < nSeg) {
+  if (have) {
     const uint32_t strand = gid / segsPerStrand;
     const uint32_t seg = gid - strand * segsPerStrand;
     const Seg ps = project_segment(u, a.pos, a.tan, a.P, strand, seg, vp, u->fiberRadius);
@@ -241,7 +253,28 @@ __global__ __launch_bounds__(64) void tiles_draw_kernel(TilesArgs a) {
     L.d[0][lane] = ps.depths.x; L.d[1][lane] = ps.depths.y; L.d[2][lane] = ps.depths.z; L.d[3][lane] = ps.depths.w;
     L.box[0][lane] = cx0; L.box[1][lane] = cy0; L.box[2][lane] = cx1; L.box[3][lane] = cy1;
     L.rect[0][lane] = tminx; L.rect[1][lane] = tminy; L.rect[2][lane] = tsx;
+    if (!mainPass) nTiles = kBase < nTiles ? min(kTilePiece, nTiles - kBase) : 0u;
   }
+  if (mainPass && a.pieces) {
+    // queue wide segments in pieces: one atomic per wave reserves the
+    // wave's pieces; a lane whose pieces fall past the end of the queue
+    // marks the part that fits as empty and draws its segment here
+    const uint32_t np = nTiles > kTilePiece ? (nTiles + kTilePiece - 1u) / kTilePiece : 0u;
+    const uint32_t incl = wave_incl_scan(np);
+    const uint32_t total = (uint32_t)__shfl((int)incl, 63);
+    if (total) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(a.pieceCount, total);
+      base = (uint32_t)__shfl((int)base, 0);
+      const uint32_t first = base + incl - np;
+      const bool fits = first + np <= a.pieceCap;
+      for (uint32_t q = 0; q < np && first + q < a.pieceCap; q++)
+        a.pieces[first + q] = fits ? make_uint2(gid, q * kTilePiece) : make_uint2(kInvalidPtr, 0u);
+      if (np && fits) nTiles = 0;
+    }
+  }
+  L.gid[lane] = gid;
+  L.kBase[lane] = kBase;
   const uint32_t pairEnd = wave_incl_scan(nTiles);
   L.pairEnd[lane] = pairEnd;
   const uint32_t nPairs = (uint32_t)__shfl((int)pairEnd, 63);
@@ -253,7 +286,7 @@ __global__ __launch_bounds__(64) void tiles_draw_kernel(TilesArgs a) {
     uint32_t rows = 0, tx = 0, ty = 0;
     if (pidx < nPairs) {
       const uint32_t o = search64(L.pairEnd, pidx);
-      const uint32_t k = pidx - (o ? L.pairEnd[o - 1u] : 0u);
+      const uint32_t k = L.kBase[o] + pidx - (o ? L.pairEnd[o - 1u] : 0u);
       const uint32_t tsx = L.rect[2][o];
       ty = L.rect[1][o] + k / tsx;
       tx = L.rect[0][o] + k % tsx;
@@ -262,7 +295,18 @@ __global__ __launch_bounds__(64) void tiles_draw_kernel(TilesArgs a) {
       const uint32_t bx1 = clampu((tx + 1u) * T, 0u, tg.vpW - 1u), by1 = clampu((ty + 1u) * T, 0u, tg.vpH - 1u);
       const int xs = max((int)bx0, L.box[0][o]), xe = min((int)bx1, L.box[2][o] + 1);
       const int ys = max((int)by0, L.box[1][o]), ye = min((int)by1, L.box[3][o] + 1);
-      if (xe > xs && ye > ys) rows = (uint32_t)(ye - ys);
+      if (xe > xs && ye > ys) {
+        rows = (uint32_t)(ye - ys);
+        // a tile the hull does not reach (within 1 px) has no inside pixel
+        float amax = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; q++) amax = fmaxf(amax, fabsf(L.v[q][o]));
+        if (L.box[0][o] > -(1 << 20) && amax < 65536.0f) {
+          float xlo, xhi;
+          hull_strip_x<64>(&L.v[0][o], &L.slope[0][o], (float)ys - 1.0f, (float)(ye - 1) + 1.0f, xlo, xhi);
+          if (!(xlo <= xhi) || (int)floorf(xhi + 1.0f) < xs || (int)ceilf(xlo - 1.0f) >= xe) rows = 0;
+        }
+      }
       L.owner[lane] = o;
       L.px0[lane] = xs; L.py0[lane] = ys; L.px1[lane] = xe;
     }
@@ -359,8 +403,7 @@ __global__ __launch_bounds__(64) void tiles_draw_kernel(TilesArgs a) {
       if (hit) {
         const uint32_t slot = sbase + mask_prefix(mask);
         if (slot < limit) {
-          const uint32_t o = L.owner[lane];
-          const uint32_t sgid = blockIdx.x * 64u + o;
+          const uint32_t sgid = L.gid[L.owner[lane]];
           const uint32_t strand = sgid / segsPerStrand;
           const uint32_t seg = sgid - strand * segsPerStrand;
           const uint32_t tb = ty * tg.tilesX * B + tx * B + dbin;  // getHairTileDepthBinIdx
@@ -375,6 +418,24 @@ __global__ __launch_bounds__(64) void tiles_draw_kernel(TilesArgs a) {
         }
       }
     }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(64) void tiles_draw_kernel(TilesArgs a) {
+  __shared__ TileWaveLds L;
+  const uint32_t gid = blockIdx.x * 64u + threadIdx.x;
+  tiles_wave(a, L, gid, gid < a.S * (a.P - 1u), 0u, true);
+}
+
+// The queued pieces, 64 per wave; every workgroup exits after the last one.
+__global__ __launch_bounds__(64) void tiles_pieces_kernel(TilesArgs a) {
+  __shared__ TileWaveLds L;
+  const uint32_t count = min(*a.pieceCount, a.pieceCap);
+  for (uint32_t base = blockIdx.x * 64u; base < count; base += gridDim.x * 64u) {
+    const uint32_t i = base + threadIdx.x;
+    const uint2 pc = i < count ? a.pieces[i] : make_uint2(kInvalidPtr, 0u);
+    tiles_wave(a, L, pc.x, pc.x != kInvalidPtr, pc.y, false);
     __syncthreads();
   }
 }
@@ -1168,10 +1229,23 @@ int fh_tiles_draw(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t
   a.slotKey = slot_tile_bin;
   a.T = cfg->tileSize; a.B = cfg->tileDepthBins; a.threshold = cfg->invalidTilesPerSegmentThreshold;
   a.S = rendered_strands; a.P = points_per_strand;
+  // overflow queue in the context's scratch: room for every segment's pieces
+  // up to 4 Mi pieces; a segment whose pieces do not fit is drawn by its wave
+  const uint64_t maxTiles = std::min<uint64_t>(cfg->invalidTilesPerSegmentThreshold, fh_tile_count(cfg, vp_w, vp_h));
+  a.pieceCap = (uint32_t)std::min<uint64_t>(nSeg * ((maxTiles + kTilePiece - 1) / kTilePiece), 4u << 20);
+  if ((rc = ensure_scratch(ctx, 256 + (size_t)a.pieceCap * 8))) return rc;
+  a.pieceCount = reinterpret_cast<uint32_t*>(ctx->scratch);
+  a.pieces = reinterpret_cast<uint2*>(reinterpret_cast<char*>(ctx->scratch) + 256);
+  if ((rc = fill_u32(ctx, a.pieceCount, 1, 0u))) return rc;
   const int pk0 = fh::prof_begin(ctx, "tiles_draw");
   tiles_draw_kernel<<<fh::div_ceil((uint32_t)nSeg, 64), 64, 0, ctx->stream>>>(a);
   fh::prof_end(ctx, pk0);
   if ((rc = check_launch("tiles_draw_kernel"))) return rc;
+  const int pk0b = fh::prof_begin(ctx, "tiles_pieces");
+  tiles_pieces_kernel<<<std::min(fh::div_ceil((uint32_t)nSeg, 64), (uint32_t)ctx->num_cus * 8u), 64, 0,
+                        ctx->stream>>>(a);
+  fh::prof_end(ctx, pk0b);
+  if ((rc = check_launch("tiles_pieces_kernel"))) return rc;
   const uint32_t nTiles = fh_tile_count(cfg, vp_w, vp_h);
   const int pk1 = fh::prof_begin(ctx, "tile_counts");
   tile_counts_kernel<<<fh::div_ceil(nTiles, 256), 256, 0, ctx->stream>>>(tiles, nTiles, cfg->tileDepthBins,
