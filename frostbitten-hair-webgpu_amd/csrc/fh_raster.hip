@@ -94,24 +94,56 @@ __device__ __forceinline__ float ord_f32(uint32_t u) {
   return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
 }
 
-// inclusive prefix sum over the 64 lanes
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-  const uint32_t lane = __lane_id();
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t t = (uint32_t)__shfl_up((int)v, o);
-    if (lane >= (uint32_t)o) v += t;
-  }
-  return v;
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
 }
 
-// first index i in [0, 64) with ends[i] > x, for x < ends[63]
-__device__ __forceinline__ uint32_t search64(const uint32_t* ends, uint32_t x) {
-  uint32_t lo = 0;
-#pragma unroll
-  for (uint32_t step = 32; step > 0; step >>= 1)
-    if (ends[lo + step - 1] <= x) lo += step;
-  return lo;
+// Inclusive prefix sum / max over the 64 lanes with DPP: row_shr:1,2,4,8
+// within each row of 16 lanes, then row_bcast:15 (into rows 1 and 3) and
+// row_bcast:31 (into rows 2 and 3). GFX9 DPP controls: row_shr:n = 0x110 + n,
+// row_bcast:15 = 0x142, row_bcast:31 = 0x143. A lane whose source is outside
+// its row keeps `old`, the identity.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  int x = (int)v;
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);
+  return (uint32_t)x;
+}
+
+__device__ __forceinline__ int wave_incl_max(int x) {
+  constexpr int lo = -2147483647 - 1;
+  x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x111, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x112, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x114, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x118, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x142, 0xa, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x143, 0xc, 0xf, false));
+  return x;
+}
+
+__device__ __forceinline__ uint32_t wave_last(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// Load-balancing search: lane s owns the items [end_s - cnt_s, end_s) of a
+// sequence (ends from wave_incl_scan of cnt). Returns the lane owning item
+// c0 + lane, valid while c0 + lane < the last end. Every lane whose range
+// meets [c0, c0 + 64) marks its first item there (distinct positions, since
+// the ranges are disjoint), then a prefix max spreads the marks. `mark` is 64
+// ints of wave-private LDS.
+__device__ __forceinline__ uint32_t lbs_owner(int* mark, uint32_t end, uint32_t cnt, uint32_t c0) {
+  const uint32_t lane = __lane_id();
+  mark[lane] = -1;
+  wave_sync();
+  const uint32_t start = end - cnt;
+  if (cnt && end > c0 && start < c0 + 64u) mark[(start > c0 ? start : c0) - c0] = (int)lane;
+  wave_sync();
+  return (uint32_t)wave_incl_max(mark[lane]);
 }
 
 constexpr uint32_t kTileQueue = 128;  // ring of queued inside pixels (power of two, >= 127)
@@ -126,16 +158,17 @@ struct TileWaveLds {
   uint32_t rect[3][64];    // first tile x, first tile y, tiles per row
   uint32_t gid[64];        // segment index
   uint32_t kBase[64];      // first pair (tile of the rectangle) of this lane
-  uint32_t pairEnd[64];
+  uint32_t pairStart[64];
+  int mark[64];            // lbs_owner scratch
   // per pair of the current window
   uint32_t owner[64];
   int px0[64], py0[64], px1[64];
-  uint32_t rowEnd[64];
+  uint32_t rowStart[64];
   uint32_t accMin[64], accMax[64], accBin[64];
   // per pixel row of the current row window
   uint32_t itemSlot[64];
   int itemX[64], itemY[64];
-  uint32_t candEnd[64];
+  uint32_t candStart[64];
   uint32_t queue[kTileQueue];
 };
 
@@ -276,17 +309,17 @@ __device__ __forceinline__ void tiles_wave(const TilesArgs& a, TileWaveLds& L, u
   L.gid[lane] = gid;
   L.kBase[lane] = kBase;
   const uint32_t pairEnd = wave_incl_scan(nTiles);
-  L.pairEnd[lane] = pairEnd;
-  const uint32_t nPairs = (uint32_t)__shfl((int)pairEnd, 63);
+  L.pairStart[lane] = pairEnd - nTiles;
+  const uint32_t nPairs = wave_last(pairEnd);
   __syncthreads();
 
   // 2. pairs, 64 at a time
   for (uint32_t base = 0; base < nPairs; base += 64u) {
     const uint32_t pidx = base + lane;
     uint32_t rows = 0, tx = 0, ty = 0;
+    const uint32_t o = lbs_owner(L.mark, pairEnd, nTiles, base);
     if (pidx < nPairs) {
-      const uint32_t o = search64(L.pairEnd, pidx);
-      const uint32_t k = L.kBase[o] + pidx - (o ? L.pairEnd[o - 1u] : 0u);
+      const uint32_t k = L.kBase[o] + pidx - L.pairStart[o];
       const uint32_t tsx = L.rect[2][o];
       ty = L.rect[1][o] + k / tsx;
       tx = L.rect[0][o] + k % tsx;
@@ -314,8 +347,8 @@ __device__ __forceinline__ void tiles_wave(const TilesArgs& a, TileWaveLds& L, u
     L.accMax[lane] = f32_ord(-999.0f);
     L.accBin[lane] = B;
     const uint32_t rowEnd = wave_incl_scan(rows);
-    L.rowEnd[lane] = rowEnd;
-    const uint32_t nRows = (uint32_t)__shfl((int)rowEnd, 63);
+    L.rowStart[lane] = rowEnd - rows;
+    const uint32_t nRows = wave_last(rowEnd);
     __syncthreads();
 
     uint32_t head = 0, tail = 0;
@@ -324,9 +357,9 @@ __device__ __forceinline__ void tiles_wave(const TilesArgs& a, TileWaveLds& L, u
     for (uint32_t r0 = 0; r0 < nRows; r0 += 64u) {
       const uint32_t ridx = r0 + lane;
       uint32_t cnt = 0;
+      const uint32_t slot = lbs_owner(L.mark, rowEnd, rows, r0);
       if (ridx < nRows) {
-        const uint32_t slot = search64(L.rowEnd, ridx);
-        const int y = L.py0[slot] + (int)(ridx - (slot ? L.rowEnd[slot - 1u] : 0u));
+        const int y = L.py0[slot] + (int)(ridx - L.rowStart[slot]);
         const uint32_t o = L.owner[slot];
         int xa = L.px0[slot], xb = L.px1[slot];
         const float* v = &L.v[0][o];
@@ -351,8 +384,8 @@ __device__ __forceinline__ void tiles_wave(const TilesArgs& a, TileWaveLds& L, u
         L.itemY[lane] = y;
       }
       const uint32_t candEnd = wave_incl_scan(cnt);
-      L.candEnd[lane] = candEnd;
-      const uint32_t nCand = (uint32_t)__shfl((int)candEnd, 63);
+      L.candStart[lane] = candEnd - cnt;
+      const uint32_t nCand = wave_last(candEnd);
       __syncthreads();
 
       // 3b. candidate pixels, 64 at a time; 4. inside pixels, 64 at a time
@@ -360,9 +393,9 @@ __device__ __forceinline__ void tiles_wave(const TilesArgs& a, TileWaveLds& L, u
         const uint32_t c = c0 + lane;
         bool inside = false;
         uint32_t entry = 0;
+        const uint32_t it = lbs_owner(L.mark, candEnd, cnt, c0);
         if (c < nCand) {
-          const uint32_t it = search64(L.candEnd, c);
-          const int x = L.itemX[it] + (int)(c - (it ? L.candEnd[it - 1u] : 0u));
+          const int x = L.itemX[it] + (int)(c - L.candStart[it]);
           const int y = L.itemY[it];
           const uint32_t slot = L.itemSlot[it];
           const uint32_t o = L.owner[slot];
@@ -794,11 +827,6 @@ constexpr uint32_t fine_waves() { return TILE == 8 ? 1u : (uint32_t)TILE / 2u; }
 
 constexpr uint32_t kFineRing = 128;  // per-wave ring of queued pixels
 
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
-
 template <int TILE>
 struct FineLds {
   static constexpr uint32_t NPX = TILE * TILE;
@@ -810,7 +838,7 @@ struct FineLds {
   uint32_t* done;    // NPX: pixel saturated
   uint32_t* claim;   // NPX: lowest lane linking to the pixel (64 = none)
   uint32_t* ring;    // NW * kFineRing
-  uint32_t* wsc;     // NW * 5 * 64: per-wave enumeration scratch
+  uint32_t* wsc;     // NW * 6 * 64: per-wave enumeration scratch
   uint2* poolColor;  // ldsPool: f16 rg, ba
   uint32_t* poolNext;
 };
@@ -900,11 +928,12 @@ __device__ __forceinline__ void fine_cheap_batch(const FineArgs& a, const FineLd
   const uint32_t lane = __lane_id();
   const uint32_t w = threadIdx.x >> 6;
   const int wy0 = (int)w * RW, wy1 = wy0 + RW - 1;
-  uint32_t* rowEnd = L.wsc + (w * 5u + 0u) * 64u;
-  uint32_t* itemRec = L.wsc + (w * 5u + 1u) * 64u;
-  uint32_t* itemY = L.wsc + (w * 5u + 2u) * 64u;
-  uint32_t* itemX = L.wsc + (w * 5u + 3u) * 64u;
-  uint32_t* candEnd = L.wsc + (w * 5u + 4u) * 64u;
+  uint32_t* rowStart = L.wsc + (w * 6u + 0u) * 64u;
+  uint32_t* itemRec = L.wsc + (w * 6u + 1u) * 64u;
+  uint32_t* itemY = L.wsc + (w * 6u + 2u) * 64u;
+  uint32_t* itemX = L.wsc + (w * 6u + 3u) * 64u;
+  uint32_t* candStart = L.wsc + (w * 6u + 4u) * 64u;
+  int* mark = reinterpret_cast<int*>(L.wsc + (w * 6u + 5u) * 64u);
   const auto& R = L.recs;
   uint32_t qh = 0, qt = 0;
   for (uint32_t rb = 0; rb < nb; rb += 64u) {
@@ -916,16 +945,16 @@ __device__ __forceinline__ void fine_cheap_batch(const FineArgs& a, const FineLd
       if (yb >= ya && xb >= xa) nrows = (uint32_t)(yb - ya + 1);
     }
     const uint32_t re = wave_incl_scan(nrows);
-    rowEnd[lane] = re;
-    const uint32_t nRows = (uint32_t)__shfl((int)re, 63);
+    rowStart[lane] = re - nrows;
+    const uint32_t nRows = wave_last(re);
     wave_sync();
     for (uint32_t r0 = 0; r0 < nRows; r0 += 64u) {
       const uint32_t ridx = r0 + lane;
       uint32_t cnt = 0;
+      const uint32_t k = lbs_owner(mark, re, nrows, r0);
       if (ridx < nRows) {
-        const uint32_t k = search64(rowEnd, ridx);
         const uint32_t j = rb + k;
-        const int y = max(R.hi16(kRecBox0, j), wy0) + (int)(ridx - (k ? rowEnd[k - 1u] : 0u));
+        const int y = max(R.hi16(kRecBox0, j), wy0) + (int)(ridx - rowStart[k]);
         int xa = max(R.lo16(kRecBox0, j), 0), xb = min(R.lo16(kRecBox1, j), TILE - 1) + 1;
         if (R.u(kRecSpan, j)) {
           float xlo, xhi;
@@ -944,16 +973,16 @@ __device__ __forceinline__ void fine_cheap_batch(const FineArgs& a, const FineLd
         itemX[lane] = (uint32_t)xa;
       }
       const uint32_t ce = wave_incl_scan(cnt);
-      candEnd[lane] = ce;
-      const uint32_t nCand = (uint32_t)__shfl((int)ce, 63);
+      candStart[lane] = ce - cnt;
+      const uint32_t nCand = wave_last(ce);
       wave_sync();
       for (uint32_t c0 = 0; c0 < nCand; c0 += 64u) {
         const uint32_t c = c0 + lane;
         bool push = false;
         uint32_t entry = 0;
+        const uint32_t it = lbs_owner(mark, ce, cnt, c0);
         if (c < nCand) {
-          const uint32_t it = search64(candEnd, c);
-          const uint32_t x = itemX[it] + (c - (it ? candEnd[it - 1u] : 0u));
+          const uint32_t x = itemX[it] + (c - candStart[it]);
           const uint32_t y = itemY[it];
           const uint32_t j = itemRec[it];
           const uint32_t pix = y * TILE + x;
@@ -994,7 +1023,7 @@ __global__ __launch_bounds__(fine_waves<TILE>() * 64) void fine_raster_kernel(Fi
   L.claim = L.done + NPX;
   L.ring = L.claim + NPX;
   L.wsc = L.ring + NW * kFineRing;
-  L.poolColor = reinterpret_cast<uint2*>(L.wsc + NW * 5u * 64u);
+  L.poolColor = reinterpret_cast<uint2*>(L.wsc + NW * 6u * 64u);
   L.poolNext = reinterpret_cast<uint32_t*>(L.poolColor + a.ldsPool);
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_sliceOff;  // slots of the slice memory taken in the bin
@@ -1398,7 +1427,7 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
   const size_t ldsTarget = std::min<size_t>(T == 8 ? 40 * 1024 : (T == 16 ? 78 * 1024 : 150 * 1024),
                                             ctx->max_lds - 256);
   const size_t fixed = (size_t)a.batch * kRecFields * 4 + (size_t)npx * 4 * (a.NS + 3) +
-                       (size_t)nw * (kFineRing + 5 * 64) * 4;
+                       (size_t)nw * (kFineRing + 6 * 64) * 4;
   FH_REQUIRE(fixed + 12 * 64 <= ldsTarget, "fh_fine_raster: slicesPerPixel too large for the tile size");
   a.ldsPool = (uint32_t)std::min<size_t>((ldsTarget - fixed) / 12, a.perProc);
   const size_t lds = fixed + (size_t)a.ldsPool * 12;
