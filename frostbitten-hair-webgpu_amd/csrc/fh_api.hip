@@ -1,4 +1,5 @@
 // Context, memory and error handling of the C-ABI (include/fh_hair.h).
+#include <algorithm>
 #include <cstring>
 
 #include "fh_internal.hpp"
@@ -12,6 +13,20 @@ __global__ void fill_u32_kernel(uint32_t* __restrict__ dst, uint64_t n, uint32_t
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) dst[i] = value;
+}
+
+// Several clears in one launch: blockIdx.y picks the buffer.
+struct FillJobs {
+  uint32_t* dst[fh::kMaxFillJobs];
+  uint64_t n[fh::kMaxFillJobs];
+};
+
+__global__ void fill_many_kernel(FillJobs jobs) {
+  uint32_t* __restrict__ dst = jobs.dst[blockIdx.y];
+  const uint64_t n = jobs.n[blockIdx.y];
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = 0u;
 }
 }  // namespace
 
@@ -54,6 +69,26 @@ int prof_begin(fh_ctx* ctx, const char* kernel) {
 
 void prof_end(fh_ctx* ctx, int k) {
   if (k >= 0) (void)hipEventRecord(ctx->prof_events[2 * k + 1], ctx->stream);
+}
+
+int zero_many(fh_ctx* ctx, int count, void* const* dst, const uint64_t* n_words) {
+  FillJobs jobs;
+  uint64_t most = 0;
+  int k = 0;
+  for (int i = 0; i < count; i++) {
+    if (n_words[i] == 0) continue;
+    jobs.dst[k] = (uint32_t*)dst[i];
+    jobs.n[k] = n_words[i];
+    most = std::max(most, n_words[i]);
+    k++;
+  }
+  if (k == 0) return FH_OK;
+  uint64_t blocks = (most + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  const int pk = fh::prof_begin(ctx, "fill_many");
+  fill_many_kernel<<<dim3((unsigned)blocks, (unsigned)k), 256, 0, ctx->stream>>>(jobs);
+  fh::prof_end(ctx, pk);
+  return check_launch("fill_many_kernel");
 }
 
 int fill_u32(fh_ctx* ctx, void* dst, uint64_t n_words, uint32_t value) {

@@ -6,6 +6,7 @@
 
 namespace fh {
 int fill_u32(fh_ctx* ctx, void* dst, uint64_t n_words, uint32_t value);
+int zero_many(fh_ctx* ctx, int count, void* const* dst, const uint64_t* n_words);
 }
 
 namespace {
@@ -234,10 +235,12 @@ __device__ __forceinline__ void tiles_shade_queue(const TilesArgs& a, TileWaveLd
       atomicMin(&L.accBin[slot], bin);
     }
   }
-  __syncthreads();
+  wave_sync();
 }
 
-// Steps 1-5 for one wave. Each lane holds segment `gid` (when `have`) and
+// Steps 1-5 for one wave (the whole workgroup: LDS is ordered by wave_sync,
+// which unlike a workgroup barrier does not wait for global atomics in
+// flight). Each lane holds segment `gid` (when `have`) and
 // draws the pairs [kBase, kBase + count) of its rectangle: in the main pass
 // kBase = 0 and count = all tiles (or none, once queued as pieces); in the
 // overflow pass one piece.
@@ -311,7 +314,7 @@ __device__ __forceinline__ void tiles_wave(const TilesArgs& a, TileWaveLds& L, u
   const uint32_t pairEnd = wave_incl_scan(nTiles);
   L.pairStart[lane] = pairEnd - nTiles;
   const uint32_t nPairs = wave_last(pairEnd);
-  __syncthreads();
+  wave_sync();
 
   // 2. pairs, 64 at a time
   for (uint32_t base = 0; base < nPairs; base += 64u) {
@@ -349,7 +352,7 @@ __device__ __forceinline__ void tiles_wave(const TilesArgs& a, TileWaveLds& L, u
     const uint32_t rowEnd = wave_incl_scan(rows);
     L.rowStart[lane] = rowEnd - rows;
     const uint32_t nRows = wave_last(rowEnd);
-    __syncthreads();
+    wave_sync();
 
     uint32_t head = 0, tail = 0;
     // 3a. pixel rows of the pairs, 64 at a time: the conservative x span of
@@ -386,7 +389,7 @@ __device__ __forceinline__ void tiles_wave(const TilesArgs& a, TileWaveLds& L, u
       const uint32_t candEnd = wave_incl_scan(cnt);
       L.candStart[lane] = candEnd - cnt;
       const uint32_t nCand = wave_last(candEnd);
-      __syncthreads();
+      wave_sync();
 
       // 3b. candidate pixels, 64 at a time; 4. inside pixels, 64 at a time
       for (uint32_t c0 = 0; c0 < nCand; c0 += 64u) {
@@ -408,7 +411,7 @@ __device__ __forceinline__ void tiles_wave(const TilesArgs& a, TileWaveLds& L, u
         const uint64_t m = __ballot(inside);
         if (inside) L.queue[(tail + mask_prefix(m)) & (kTileQueue - 1u)] = entry;
         tail += (uint32_t)__popcll(m);
-        __syncthreads();
+        wave_sync();
         if (tail - head >= 64u) {
           tiles_shade_queue(a, L, head, 64u, tg, dbVS);
           head += 64u;
@@ -451,7 +454,7 @@ __device__ __forceinline__ void tiles_wave(const TilesArgs& a, TileWaveLds& L, u
         }
       }
     }
-    __syncthreads();
+    wave_sync();
   }
 }
 
@@ -469,7 +472,7 @@ __global__ __launch_bounds__(64) void tiles_pieces_kernel(TilesArgs a) {
     const uint32_t i = base + threadIdx.x;
     const uint2 pc = i < count ? a.pieces[i] : make_uint2(kInvalidPtr, 0u);
     tiles_wave(a, L, pc.x, pc.x != kInvalidPtr, pc.y, false);
-    __syncthreads();
+    wave_sync();
   }
 }
 
@@ -1224,9 +1227,10 @@ int fh_tiles_clear(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
   if (rc) return rc;
   FH_REQUIRE(tiles && tile_segments && segment_count_per_tile, "fh_tiles_clear: null buffer");
   const uint64_t nt = fh_tile_count(cfg, vp_w, vp_h);
-  if ((rc = fill_u32(ctx, tiles, nt * cfg->tileDepthBins * 4, 0u))) return rc;
-  if ((rc = fill_u32(ctx, segment_count_per_tile, nt, 0u))) return rc;
-  return fill_u32(ctx, tile_segments, 1, 0u);  // drawnSegmentsCount only (hairTilesPass.ts:51)
+  // drawnSegmentsCount only of the segment buffer (hairTilesPass.ts:51)
+  void* dst[3] = {tiles, segment_count_per_tile, tile_segments};
+  const uint64_t n[3] = {nt * cfg->tileDepthBins * 4, nt, 1};
+  return zero_many(ctx, 3, dst, n);
 }
 
 int fh_tiles_draw(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t vp_h,
@@ -1304,8 +1308,9 @@ int fh_tiles_sort_clear(fh_ctx* ctx, const fh_render_cfg* cfg, void* tile_list, 
   int rc = check_cfg(cfg, "fh_tiles_sort_clear");
   if (rc) return rc;
   FH_REQUIRE(tile_list && buckets, "fh_tiles_sort_clear: null buffer");
-  if ((rc = fill_u32(ctx, tile_list, 1, 0u))) return rc;
-  return fill_u32(ctx, buckets, (uint64_t)cfg->sortBuckets * 2, 0u);
+  void* dst[2] = {tile_list, buckets};
+  const uint64_t n[2] = {1, (uint64_t)cfg->sortBuckets * 2};
+  return zero_many(ctx, 2, dst, n);
 }
 
 int fh_tiles_sort(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t vp_h,

@@ -113,10 +113,25 @@ struct ShadeArgs {
   uint32_t S, P, PS;
 };
 
+// kShadeLanes lanes per shading point split the PCF taps (a count of 0/1
+// comparisons, exact in any order). The 32 points of a workgroup then hand
+// their three lights to 96 threads, one (point, light) each, and the lights
+// are added back in the reference's order.
+constexpr uint32_t kShadeLanes = 8;
+constexpr uint32_t kShadePoints = 256 / kShadeLanes;
+
+struct LightIn {
+  float base[3], toCam[3], T[3], pos[3], shadow, ao;
+};
+
 __global__ __launch_bounds__(256) void shading_kernel(ShadeArgs a) {
-  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= a.S * a.PS) return;
-  const uint32_t s = gid / a.PS, id = gid - s * a.PS;
+  __shared__ LightIn lin[kShadePoints];
+  __shared__ float lout[kShadePoints * 3][3];
+  const uint32_t pi = threadIdx.x / kShadeLanes;
+  const uint32_t gid = blockIdx.x * kShadePoints + pi;
+  const uint32_t sub = threadIdx.x % kShadeLanes;
+  const bool valid = gid < a.S * a.PS;
+  const uint32_t s = valid ? gid / a.PS : 0u, id = valid ? gid - s * a.PS : 0u;
   const fh_render_uniforms* __restrict__ u = a.u;
   const fh_hair_material& m = u->hairMaterial;
   const f2 vp = mk2(u->viewport[0], u->viewport[1]);
@@ -143,7 +158,6 @@ __global__ __launch_bounds__(256) void shading_kernel(ShadeArgs a) {
   base = mix(base, rnd, m.colorRng);
   base = mix(base, base * fractf(fs / 255.0f), m.lumaRng);
   const f3 toCam = normalize(mk3(u->cameraPosition[0], u->cameraPosition[1], u->cameraPosition[2]) - xyz(posWS));
-  const Marschner mp{base, m.specular, m.weightTT, m.weightTRT, m.shift, m.roughness};
   // shadow — getShadow :236-254 -> calculateDirectionalShadow (sampleShadows.wgsl.ts:47-84)
   const f4 lpos = mul(u->shadows.sourceMVP_Matrix, posWS);
   const f3 pss = mk3(lpos.x * 0.5f + 0.5f, 1.0f - (lpos.y * 0.5f + 0.5f), lpos.z);
@@ -157,12 +171,17 @@ __global__ __launch_bounds__(256) void shading_kernel(ShadeArgs a) {
     radius = 1 + f2i(dd / sm * 10.0f);  // PCSS_PENUMBRA_BASE + depthDiff/depth * WIDTH
   }
   const float texel = 1.0f / (float)a.shadowW;
-  float sh = 0.0f;
-  for (int32_t x = -radius; x <= radius; x++)
-    for (int32_t y = -radius; y <= radius; y++) {
-      const f2 uv = mk2(pss.x, pss.y) + mk2((float)x * texel, (float)y * texel);
-      sh += (pss.z - bias > gather_avg(a.shadow, a.shadowW, a.shadowH, uv)) ? 1.0f : 0.0f;
-    }
+  // taps x-major as in the reference loops, dealt round-robin to the lanes
+  const int32_t side = radius * 2 + 1;
+  uint32_t hits = 0;
+  for (int32_t k = (int32_t)sub; k < side * side; k += (int32_t)kShadeLanes) {
+    const int32_t x = -radius + k / side, y = -radius + k % side;
+    const f2 uv = mk2(pss.x, pss.y) + mk2((float)x * texel, (float)y * texel);
+    hits += (pss.z - bias > gather_avg(a.shadow, a.shadowW, a.shadowH, uv)) ? 1u : 0u;
+  }
+#pragma unroll
+  for (uint32_t o = 1; o < kShadeLanes; o <<= 1) hits += (uint32_t)__shfl_xor((int)hits, (int)o);
+  const float sh = (float)hits;
   const float pcf = (float)(radius * 2 + 1);
   float shadowRes = sh / (pcf * pcf);
   const bool validSample = !(pss.z > 1.0f) && !(pss.x < 0.0f || pss.x > 1.0f || pss.y < 0.0f || pss.y > 1.0f);
@@ -176,13 +195,36 @@ __global__ __launch_bounds__(256) void shading_kernel(ShadeArgs a) {
   const float zb = tex_load(a.depth, a.depthW, a.depthH, (int32_t)f2u(texPx.x), (int32_t)f2u(texPx.y));
   const float att = fabsf(linearize_depth(pproj.z) - linearize_depth(zb)) * m.attenuation;
   ao = satf(ao + att);
-  f3 rs = mk3(0.0f, 0.0f, 0.0f);
   const f3 T = xyz(tanWS);
-  rs = rs + hair_light(mp, u->light0, toCam, T, xyz(posWS), shadow, ao);
-  rs = rs + hair_light(mp, u->light1, toCam, T, xyz(posWS), shadow, ao);
-  rs = rs + hair_light(mp, u->light2, toCam, T, xyz(posWS), shadow, ao);
-  const float alpha = id >= a.PS - 1u ? 0.0f : 1.0f;  // thin-tip alpha :160
-  a.out[(size_t)s * a.PS + id] = make_float4(rs.x, rs.y, rs.z, alpha);
+  if (sub == 0u) {
+    LightIn& li = lin[pi];
+    li.base[0] = base.x; li.base[1] = base.y; li.base[2] = base.z;
+    li.toCam[0] = toCam.x; li.toCam[1] = toCam.y; li.toCam[2] = toCam.z;
+    li.T[0] = T.x; li.T[1] = T.y; li.T[2] = T.z;
+    li.pos[0] = posWS.x; li.pos[1] = posWS.y; li.pos[2] = posWS.z;
+    li.shadow = shadow; li.ao = ao;
+  }
+  __syncthreads();
+  if (threadIdx.x < kShadePoints * 3u) {
+    const uint32_t q = threadIdx.x / 3u, l = threadIdx.x - q * 3u;
+    const LightIn& li = lin[q];
+    const Marschner lp{mk3(li.base[0], li.base[1], li.base[2]), m.specular, m.weightTT, m.weightTRT, m.shift,
+                       m.roughness};
+    const fh_light& light = l == 0u ? u->light0 : (l == 1u ? u->light1 : u->light2);
+    const f3 r = hair_light(lp, light, mk3(li.toCam[0], li.toCam[1], li.toCam[2]), mk3(li.T[0], li.T[1], li.T[2]),
+                            mk3(li.pos[0], li.pos[1], li.pos[2]), li.shadow, li.ao);
+    lout[threadIdx.x][0] = r.x; lout[threadIdx.x][1] = r.y; lout[threadIdx.x][2] = r.z;
+  }
+  __syncthreads();
+  if (valid && sub == 0u) {
+    // (0 + L0) + L1 + L2, as the reference accumulates
+    f3 rs = mk3(0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (uint32_t l = 0; l < 3u; l++)
+      rs = rs + mk3(lout[pi * 3u + l][0], lout[pi * 3u + l][1], lout[pi * 3u + l][2]);
+    const float alpha = id >= a.PS - 1u ? 0.0f : 1.0f;  // thin-tip alpha :160
+    a.out[(size_t)s * a.PS + id] = make_float4(rs.x, rs.y, rs.z, alpha);
+  }
 }
 
 }  // namespace
@@ -212,7 +254,7 @@ extern "C" int fh_shading(fh_ctx* ctx, const fh_render_cfg* cfg, const fh_render
   a.depth = depth.data; a.depthW = depth.width; a.depthH = depth.height;
   a.S = strands_count; a.P = points_per_strand; a.PS = cfg->shadingPoints;
   const int pk0 = fh::prof_begin(ctx, "shading");
-  shading_kernel<<<fh::div_ceil((uint32_t)n, 256), 256, 0, ctx->stream>>>(a);
+  shading_kernel<<<fh::div_ceil((uint32_t)n, kShadePoints), 256, 0, ctx->stream>>>(a);
   fh::prof_end(ctx, pk0);
   return fh::check_launch("shading_kernel");
 }
