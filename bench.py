@@ -6,10 +6,14 @@ A step is one frame of the chain on the 1080p character-scale workload
 pre-pass, integration, particle-to-grid), then the tile pass, the tile sort,
 the fine pass and the shading pass, with inputs resident in HBM.
 
-Prints one JSON line. `roofline` is for the kernel that takes the most time,
-from its algorithmic bytes per launch and its average launch duration, both
-measured with HIP events that the library records around each of its kernels
-on its own stream inside the timed region. `cpu_baseline`
+By default the frame is replayed as a HIP graph (one graph per parity of the
+tick-tock position buffers). Prints one JSON line. `value` and the frame
+latencies come from a timed region of --steps frames with events only at
+frame boundaries. `roofline` is for the kernel that takes the most time in a
+second timed region of as many frames, from its algorithmic bytes per launch
+and its average launch duration, measured with HIP events that the library
+records around each of its kernels on its own stream (per-kernel events cost
+frame time, so they are kept out of the first region). `cpu_baseline`
 is the CPU oracle (a scalar C restatement of the reference's kernels) timed
 on full frames of the same workload, rank 0 only.
 """
@@ -133,6 +137,10 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2", choices=["small", "c2", "c3"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="launch every call eagerly instead of replaying "
+                    "the frame as a HIP graph")
+    ap.add_argument("--no-kernel-timing", action="store_true",
+                    help="no per-kernel HIP events (roofline omitted); frame time only")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -180,7 +188,7 @@ def main() -> None:
     stages = ["sim", "tiles", "sort", "fine", "shading"]
 
     def frame(events=None):
-        r.simulationUniforms = ring[r.frameIdx]
+        r.simulationUniforms.copy_(ring[r.frameIdx])
         ctx = r.ctx()
         rec = (lambda k: events[k].record()) if events else (lambda k: None)
         rec(0)
@@ -199,35 +207,77 @@ def main() -> None:
         rec(5)
         r.frameIdx += 1
 
+    def graph_frame(events=None):
+        r.simulationUniforms.copy_(ring[r.frameIdx])
+        if events:
+            events[0].record()
+        r.frame_graph(upload_uniforms=False)
+        if events:
+            events[5].record()
+
+    use_graph = not args.no_graph
     r.beforeFirstFrame()
     for _ in range(args.warmup):
         frame()
     torch.cuda.synchronize()
+    step = graph_frame if use_graph else frame
 
+    # Timed region 1: the frames alone (events only at frame boundaries) give
+    # value and the frame latencies.
+    if use_graph:
+        r.capture_frames()
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(args.steps)]
-    # HIP events around every kernel the library launches, on its stream
-    r.native.profiler_enable(64 * args.steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        frame(evs[k])
+        step(evs[k])
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    kernels = r.native.profiler_read()
-    r.native.profiler_enable(0)
 
+    # Timed region 2: as many further frames with HIP events around every
+    # kernel on the library's stream (in graph replays, event-record nodes
+    # around every kernel node): per-kernel durations and the roofline. The
+    # events cost frame time, so they stay out of region 1.
+    kernels = {}
+    evs2 = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(args.steps)]
+    if not args.no_kernel_timing:
+        r.native.profiler_enable(64 * args.steps)
+        if use_graph:
+            r.capture_frames()
+        torch.cuda.synchronize()
+        r.native.profiler_read()
+        for k in range(args.steps):
+            step(evs2[k])
+        torch.cuda.synchronize()
+        kernels = r.native.profiler_read()
+        r.native.profiler_enable(0)
     per_frame = np.array([evs[k][0].elapsed_time(evs[k][5]) for k in range(args.steps)])  # ms
-    stage_ms = {s: float(np.mean([evs[k][i].elapsed_time(evs[k][i + 1]) for k in range(args.steps)]))
-                for i, s in enumerate(stages)}
+    if use_graph:
+        # stage sums from the per-kernel events
+        groups = {"sim": ("grid_pre_sim", "sim_integrate", "grid_post_sim"),
+                  "tiles": ("fill_many", "tiles_draw", "tiles_pieces", "tile_counts"),
+                  "sort": ("sort_count", "sort_scatter"),
+                  "fine": ("bin_scan_local", "bin_scan_add", "bin_scatter", "bin_sort_small", "bin_sort_large",
+                           "fine_raster"),
+                  "shading": ("shading",)}
+        stage_ms = {g: sum(kernels[k][1] for k in ks if k in kernels) / args.steps for g, ks in groups.items()}
+        if not kernels:
+            stage_ms = {}
+    elif kernels:
+        stage_ms = {s: float(np.mean([evs2[k][i].elapsed_time(evs2[k][i + 1]) for k in range(args.steps)]))
+                    for i, s in enumerate(stages)}
+    else:
+        stage_ms = {}
     n_pairs = int(r.tilesPass.hairTileSegmentsBuffer[0].item())
     tiles = (wl.width + wl.render.tileSize - 1) // wl.render.tileSize * \
             ((wl.height + wl.render.tileSize - 1) // wl.render.tileSize)
     drawn_tiles = int(r.sortPass.tileListBuffer[0].item())
     kb = kernel_bytes(wl, n_pairs, tiles)
     kernel_ms = {k: t / n for k, (n, t) in kernels.items()}
-    dom = max((k for k in kernel_ms if k in kb), key=lambda k: kernels[k][1])
-    achieved = kb[dom] / (kernel_ms[dom] * 1e-3) / 1e9
-    traffic = read_traffic(dom, wl.name)
+    timed = [k for k in kernel_ms if k in kb]
+    dom = max(timed, key=lambda k: kernels[k][1]) if timed else None
+    achieved = kb[dom] / (kernel_ms[dom] * 1e-3) / 1e9 if dom else None
+    traffic = read_traffic(dom, wl.name) if dom else None
     img = r.finePass.image(wl.width, wl.height)
     assert torch.isfinite(img).all()
 
@@ -245,7 +295,8 @@ def main() -> None:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded strands on a scalp cap, analytic colliders; depth/shadow/AO inputs precomputed)",
-        "config": {"workload": f"plain drop-in, {wl.strands} strands x {wl.points} points, {wl.width}x{wl.height}, "
+        "config": {"workload": f"plain drop-in{' replayed as a HIP graph' if use_graph else ''}, "
+                               f"{wl.strands} strands x {wl.points} points, {wl.width}x{wl.height}, "
                                f"tile {wl.render.tileSize}, {wl.render.tileDepthBins} bins x "
                                f"{wl.render.slicesPerPixel} slices, grid {wl.sim.gridDims}^3, SDF {wl.sdf_dims}^3, "
                                f"{wl.sim.constraintIterations} constraint iterations",
@@ -253,14 +304,17 @@ def main() -> None:
         "latency_ms": {"p50": round(float(np.percentile(per_frame, 50)), 4),
                        "p99": round(float(np.percentile(per_frame, 99)), 4)},
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
-        "stage_hbm_GBps": {k: round(v / (stage_ms[k] * 1e-3) / 1e9, 2) for k, v in stage_bytes(wl, n_pairs, tiles).items()},
+        "stage_hbm_GBps": {k: round(v / (stage_ms[k] * 1e-3) / 1e9, 2)
+                           for k, v in stage_bytes(wl, n_pairs, tiles).items() if stage_ms.get(k)},
         "throughput": {"particle_steps_per_s": round(wl.strands * wl.points * 1e3 / ms, 1),
                        "segments_rasterized_per_s": round(wl.strands * (wl.points - 1) * 1e3 / ms, 1),
                        "segment_tile_pairs": n_pairs, "drawn_tiles": drawn_tiles},
         "kernel_us": {k: round(v * 1e3, 2) for k, v in sorted(kernel_ms.items(), key=lambda kv: -kernels[kv[0]][1])},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "algorithmic_bytes": int(kb[dom]), "avg_launch_us": round(kernel_ms[dom] * 1e3, 2)},
+        "kernel_timing": f"second timed region of {args.steps} frames with HIP events around every kernel",
+        "roofline": None if dom is None else {
+            "bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+            "algorithmic_bytes": int(kb[dom]), "avg_launch_us": round(kernel_ms[dom] * 1e3, 2)},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, r)

@@ -53,7 +53,9 @@ def build(verbose: bool = False) -> Path:
     with ThreadPoolExecutor(max_workers=min(4, len(cmds))) as ex:
         list(ex.map(_run, cmds))
     tmp = LIB_PATH.with_suffix(".so.tmp")
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)])
+    # --no-undefined: a missing definition fails the build, not the first dlopen
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,--no-undefined", *map(str, objs), "-o",
+          str(tmp)])
     os.replace(tmp, LIB_PATH)
     if verbose:
         print(f"built {LIB_PATH}")

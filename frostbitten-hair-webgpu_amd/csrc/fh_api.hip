@@ -47,6 +47,8 @@ int check_launch(const char* what) {
 
 int ensure_scratch(fh_ctx* ctx, size_t bytes) {
   if (ctx->scratch_bytes >= bytes) return FH_OK;
+  if (ctx->capturing)
+    return fail(FH_ERR_UNSUPPORTED, "scratch would grow while capturing: run the calls once before capture");
   if (ctx->scratch) {
     FH_HIP(hipStreamSynchronize(ctx->stream));
     FH_HIP(hipFree(ctx->scratch));
@@ -60,6 +62,13 @@ int ensure_scratch(fh_ctx* ctx, size_t bytes) {
 }
 
 int prof_begin(fh_ctx* ctx, const char* kernel) {
+  if (ctx->prof_cap == 0) return -1;
+  if (ctx->capturing) {
+    // stream capture keeps no event records: fh_capture_end puts event-record
+    // nodes around the captured kernel nodes, named in launch order
+    ctx->cap_names.push_back(kernel);
+    return -1;
+  }
   if (ctx->prof_names.size() >= ctx->prof_cap) return -1;
   const int k = (int)ctx->prof_names.size();
   if (hipEventRecord(ctx->prof_events[2 * k], ctx->stream) != hipSuccess) return -1;
@@ -71,11 +80,67 @@ void prof_end(fh_ctx* ctx, int k) {
   if (k >= 0) (void)hipEventRecord(ctx->prof_events[2 * k + 1], ctx->stream);
 }
 
+// Puts an event-record node before and after each kernel node of a captured
+// single-stream recording (a chain), so that replays can be timed. Leaves the
+// graph untouched when its kernels do not match the recorded names.
+int add_timing_nodes(fh_graph* g, const std::vector<const char*>& names) {
+  size_t nn = 0;
+  FH_HIP(hipGraphGetNodes(g->graph, nullptr, &nn));
+  std::vector<hipGraphNode_t> nodes(nn);
+  FH_HIP(hipGraphGetNodes(g->graph, nodes.data(), &nn));
+  hipGraphNode_t cur = nullptr;
+  int roots = 0;
+  for (hipGraphNode_t x : nodes) {
+    size_t nd = 0;
+    FH_HIP(hipGraphNodeGetDependencies(x, nullptr, &nd));
+    if (nd == 0) { cur = x; roots++; }
+  }
+  if (roots != 1) return FH_OK;
+  std::vector<hipGraphNode_t> kernels;
+  for (size_t steps = 0; cur && steps <= nn; steps++) {
+    hipGraphNodeType t;
+    FH_HIP(hipGraphNodeGetType(cur, &t));
+    if (t == hipGraphNodeTypeKernel) kernels.push_back(cur);
+    size_t no = 0;
+    FH_HIP(hipGraphNodeGetDependentNodes(cur, nullptr, &no));
+    if (no == 0) break;
+    if (no != 1) return FH_OK;  // not a chain
+    FH_HIP(hipGraphNodeGetDependentNodes(cur, &cur, &no));
+  }
+  if (kernels.size() != names.size()) return FH_OK;
+  for (size_t i = 0; i < kernels.size(); i++) {
+    hipGraphNode_t K = kernels[i];
+    hipEvent_t e0, e1;
+    FH_HIP(hipEventCreate(&e0));
+    g->capEvents.push_back(e0);
+    FH_HIP(hipEventCreate(&e1));
+    g->capEvents.push_back(e1);
+    size_t nd = 0, no = 0;
+    FH_HIP(hipGraphNodeGetDependencies(K, nullptr, &nd));
+    std::vector<hipGraphNode_t> deps(nd);
+    if (nd) FH_HIP(hipGraphNodeGetDependencies(K, deps.data(), &nd));
+    FH_HIP(hipGraphNodeGetDependentNodes(K, nullptr, &no));
+    std::vector<hipGraphNode_t> outs(no);
+    if (no) FH_HIP(hipGraphNodeGetDependentNodes(K, outs.data(), &no));
+    for (hipGraphNode_t d : deps) FH_HIP(hipGraphRemoveDependencies(g->graph, &d, &K, 1));
+    for (hipGraphNode_t o : outs) FH_HIP(hipGraphRemoveDependencies(g->graph, &K, &o, 1));
+    hipGraphNode_t B, E;
+    FH_HIP(hipGraphAddEventRecordNode(&B, g->graph, deps.data(), nd, e0));
+    FH_HIP(hipGraphAddDependencies(g->graph, &B, &K, 1));
+    FH_HIP(hipGraphAddEventRecordNode(&E, g->graph, &K, 1, e1));
+    for (hipGraphNode_t o : outs) FH_HIP(hipGraphAddDependencies(g->graph, &E, &o, 1));
+    g->timedBegin.push_back(B);
+    g->timedEnd.push_back(E);
+    g->timedNames.push_back(names[i]);
+  }
+  return FH_OK;
+}
+
 int zero_many(fh_ctx* ctx, int count, void* const* dst, const uint64_t* n_words) {
   FillJobs jobs;
   uint64_t most = 0;
   int k = 0;
-  for (int i = 0; i < count; i++) {
+  for (int i = 0; i < count && k < kMaxFillJobs; i++) {
     if (n_words[i] == 0) continue;
     jobs.dst[k] = (uint32_t*)dst[i];
     jobs.n[k] = n_words[i];
@@ -196,6 +261,67 @@ int fh_profiler_read(fh_ctx* ctx, fh_kernel_time* out, uint32_t max_out, uint32_
   }
   ctx->prof_names.clear();
   *n_out = n;
+  return FH_OK;
+}
+
+int fh_capture_begin(fh_ctx* ctx) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(!ctx->capturing, "fh_capture_begin: already capturing");
+  FH_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+  ctx->capturing = true;
+  ctx->cap_names.clear();
+  return FH_OK;
+}
+
+int fh_capture_end(fh_ctx* ctx, fh_graph** out) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(ctx->capturing, "fh_capture_end: not capturing");
+  FH_REQUIRE(out != nullptr, "fh_capture_end: out is null");
+  *out = nullptr;
+  ctx->capturing = false;
+  fh_graph* g = new fh_graph();
+  hipError_t e = hipStreamEndCapture(ctx->stream, &g->graph);
+  int rc = FH_OK;
+  if (e == hipSuccess && !ctx->cap_names.empty()) rc = fh::add_timing_nodes(g, ctx->cap_names);
+  ctx->cap_names.clear();
+  if (e == hipSuccess && rc == FH_OK) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess || rc != FH_OK) {
+    const std::string msg = rc != FH_OK ? fh_last_error() : hipGetErrorString(e);
+    fh_graph_destroy(g);
+    return fh::fail(FH_ERR_HIP, std::string("fh_capture_end: ") + msg);
+  }
+  *out = g;
+  return FH_OK;
+}
+
+int fh_graph_launch(fh_ctx* ctx, fh_graph* graph) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(graph && graph->exec, "fh_graph_launch: null graph");
+  // timed kernels: bind this replay's event pairs from the profiler's pool, or
+  // the capture's own events once the pool is used up (never read back)
+  for (size_t k = 0; k < graph->timedNames.size(); k++) {
+    if (!graph->timedBegin[k] || !graph->timedEnd[k]) continue;
+    hipEvent_t e0 = graph->capEvents[2 * k], e1 = graph->capEvents[2 * k + 1];
+    const bool timed = ctx->prof_names.size() < ctx->prof_cap;
+    if (timed) {
+      const size_t j = ctx->prof_names.size();
+      e0 = ctx->prof_events[2 * j];
+      e1 = ctx->prof_events[2 * j + 1];
+    }
+    FH_HIP(hipGraphExecEventRecordNodeSetEvent(graph->exec, graph->timedBegin[k], e0));
+    FH_HIP(hipGraphExecEventRecordNodeSetEvent(graph->exec, graph->timedEnd[k], e1));
+    if (timed) ctx->prof_names.push_back(graph->timedNames[k]);
+  }
+  FH_HIP(hipGraphLaunch(graph->exec, ctx->stream));
+  return FH_OK;
+}
+
+int fh_graph_destroy(fh_graph* graph) {
+  if (!graph) return FH_OK;
+  if (graph->exec) hipGraphExecDestroy(graph->exec);
+  if (graph->graph) hipGraphDestroy(graph->graph);
+  for (hipEvent_t e : graph->capEvents) hipEventDestroy(e);
+  delete graph;
   return FH_OK;
 }
 

@@ -23,6 +23,18 @@ struct fh_ctx {
   std::vector<hipEvent_t> prof_events;
   std::vector<const char*> prof_names;  // kernel of each timed launch
   uint32_t prof_cap = 0;                // 0: timing off
+  bool capturing = false;               // between fh_capture_begin and _end
+  // Kernels launched while capturing with timing on, in launch order: the
+  // graph gets event-record nodes around them, rebound on every replay.
+  std::vector<const char*> cap_names;
+};
+
+struct fh_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<hipGraphNode_t> timedBegin, timedEnd;  // event-record nodes per timed kernel
+  std::vector<const char*> timedNames;
+  std::vector<hipEvent_t> capEvents;                  // owned: the nodes' own events (never read)
 };
 
 namespace fh {

@@ -73,6 +73,10 @@ SIGNATURES = {
     "fh_shading": (i32, [vp, C.POINTER(RenderCfg), vp, vp, vp, vp, vp, Tex2D, Tex2D, Tex2D, u32, u32]),
     "fh_debug_raster_barycentric": (i32, [vp, vp, vp, vp, u32, u32, vp]),
     "fh_profiler_enable": (i32, [vp, u32]),
+    "fh_capture_begin": (i32, [vp]),
+    "fh_capture_end": (i32, [vp, C.POINTER(vp)]),
+    "fh_graph_launch": (i32, [vp, vp]),
+    "fh_graph_destroy": (i32, [vp]),
     "fh_profiler_read": (i32, [vp, C.POINTER(KernelTime), u32, C.POINTER(u32)]),
 }
 
@@ -135,6 +139,14 @@ class Context:
     def sync(self) -> None:
         check(self.lib.fh_sync(self.handle), "fh_sync")
 
+    def capture_begin(self) -> None:
+        check(self.lib.fh_capture_begin(self.handle), "fh_capture_begin")
+
+    def capture_end(self) -> "Graph":
+        g = vp()
+        check(self.lib.fh_capture_end(self.handle, C.byref(g)), "fh_capture_end")
+        return Graph(self, g)
+
     def profiler_enable(self, max_launches: int) -> None:
         """Times the next `max_launches` kernel launches with HIP events (0 stops)."""
         check(self.lib.fh_profiler_enable(self.handle, max_launches), "fh_profiler_enable")
@@ -149,6 +161,28 @@ class Context:
     def close(self) -> None:
         if self.handle:
             self.lib.fh_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """fh_graph: a recorded sequence of library calls, replayed on the context's stream."""
+
+    def __init__(self, ctx: Context, handle: vp):
+        self.ctx = ctx
+        self.handle = handle
+
+    def launch(self) -> None:
+        check(self.ctx.lib.fh_graph_launch(self.ctx.handle, self.handle), "fh_graph_launch")
+
+    def close(self) -> None:
+        if self.handle:
+            self.ctx.lib.fh_graph_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

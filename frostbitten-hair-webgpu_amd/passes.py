@@ -339,6 +339,44 @@ class HairRenderer:
         self.drawHair(ctx)
         self.frameIdx += 1
 
+    def capture_frames(self) -> list:
+        """Records the frame's calls as two HIP graphs, one per parity of the
+        tick-tock position buffers (hairObject.ts:102-106), like recording
+        the frame's GPUCommandEncoder once. Every call must have run once
+        before (so no scratch grows while recording); the graphs read the
+        simulation uniforms from `simulationUniforms`, whose contents the
+        caller updates per frame."""
+        graphs = []
+        # recording needs a stream of its own (not the legacy default stream)
+        main = torch.cuda.current_stream(self.device)
+        side = torch.cuda.Stream(self.device)
+        side.wait_stream(main)
+        self.native.set_stream(side)
+        try:
+            for _ in range(2):
+                ctx = self.ctx()
+                self.native.capture_begin()
+                try:
+                    self.simulateHair(ctx)
+                    self.drawHair(ctx)
+                finally:
+                    g = self.native.capture_end()
+                graphs.append(g)
+        finally:
+            self.native.set_stream(main)
+        self._graphs = graphs
+        self._graphParity = 0
+        return graphs
+
+    def frame_graph(self, upload_uniforms: bool = True) -> None:
+        """One frame replayed from capture_frames()."""
+        if upload_uniforms:
+            self.upload_sim_uniforms()
+        self._graphs[self._graphParity].launch()
+        self._graphParity ^= 1
+        self.hair.swapPositionBuffersAfterSimIntegration()
+        self.frameIdx += 1
+
 
 def make_inputs(wl: A.Workload, render_uniforms: np.ndarray, shadow_size: int = 2048) -> dict:
     """Depth (mesh only, and mesh + hair), AO and shadow map for a workload."""

@@ -415,6 +415,29 @@ int fh_profiler_enable(fh_ctx* ctx, uint32_t max_launches);
 int fh_profiler_read(fh_ctx* ctx, fh_kernel_time* out, uint32_t max_out, uint32_t* n_out);
 
 /* ------------------------------------------------------------------------ */
+/* Frame capture                                                             */
+/* ------------------------------------------------------------------------ */
+
+/* A recorded sequence of entry-point calls, replayed as one HIP graph (the
+ * role of the reference's one queue.submit of a recorded GPUCommandEncoder
+ * per frame, renderer.ts:108-120). */
+typedef struct fh_graph fh_graph;
+
+/* Starts recording the context's stream: the entry points called until
+ * fh_capture_end enqueue nothing and are recorded instead. Call each entry
+ * point once without capture first, so that none needs to grow the
+ * context's scratch while recording (that returns FH_ERR_UNSUPPORTED).
+ * Kernel timing is paused while recording. */
+int fh_capture_begin(fh_ctx* ctx);
+/* Ends the recording and instantiates it. */
+int fh_capture_end(fh_ctx* ctx, fh_graph** out);
+/* Enqueues one replay on the context's stream. The recorded calls read and
+ * write the same buffers every replay; per-frame inputs (uniforms) change
+ * through those buffers' contents. */
+int fh_graph_launch(fh_ctx* ctx, fh_graph* graph);
+int fh_graph_destroy(fh_graph* graph);
+
+/* ------------------------------------------------------------------------ */
 /* Test hook                                                                 */
 /* ------------------------------------------------------------------------ */
 
