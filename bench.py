@@ -178,7 +178,7 @@ def main() -> None:
     t_setup = time.perf_counter()
     wl = A.make_workload(args.workload)
     r = PS.HairRenderer(wl, dev)
-    total = args.warmup + args.steps
+    total = args.warmup + 2 * args.steps  # two timed regions
     # per-frame simulation uniforms (frameIdx changes), uploaded once up front
     ring = torch.from_numpy(np.concatenate([r.sim_uniforms(f).view(np.uint8).reshape(-1) for f in range(total)]))
     ring = ring.to(dev).view(total, -1)
