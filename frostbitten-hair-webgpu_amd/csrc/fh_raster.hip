@@ -53,8 +53,8 @@ __device__ __forceinline__ TileGeom tile_geom(f2 vp, uint32_t T) {
 //   2. the wave flattens its (segment, tile) pairs and takes them 64 at a
 //      time; every pair owns the pixels of its tile (getTileBoundsPx) that
 //      can pass the edge tests (the segment's box, see seg_pixel_bbox);
-//   3. row by row, a conservative x span (the quad's convex hull near the
-//      row) bounds the candidate pixels; the wave enumerates them 64 at a
+//   3. row by row, a conservative x span (span_x) bounds the candidate
+//      pixels; the wave enumerates them 64 at a
 //      time, runs the four edge tests and queues the inside pixels in LDS;
 //   4. every 64 queued pixels the whole wave interpolates depth, runs the
 //      depth test and folds (min depth, max depth, min bin) into its pair
@@ -149,12 +149,86 @@ __device__ __forceinline__ uint32_t lbs_owner(int* mark, uint32_t end, uint32_t 
 
 constexpr uint32_t kTileQueue = 128;  // ring of queued inside pixels (power of two, >= 127)
 
+// Conservative x ranges of a quad's inside pixels. The quad's inside pixels
+// lie in its convex hull (seg_pixel_bbox), and the hull lies in the band
+// |n . (p - c)| <= R around the axis through the two cap midpoints (n normal
+// to the axis a, c its centre) and in the slab T0 <= a . (p - c) <= T1 along
+// it, with R, T0, T1 taken over the four vertices. Both are widened by 1 px,
+// which covers f32 rounding. On a range of rows each constraint is an x
+// interval with ends linear in y.
+enum SpanField : uint32_t {
+  kSpCx = 0, kSpCy, kSpNy, kSpR, kSpINx, kSpAy, kSpT0, kSpT1, kSpIAx, kSpOn, kSpFields
+};
+
+// The span frame of a quad; on = 0 when the bound is not usable (degenerate
+// axis, collinear or far-off vertices, NaN): the caller keeps the whole box.
+__device__ __forceinline__ void span_frame(const Seg& s, bool usable, float* f, uint32_t stride) {
+  const f2 m0 = mk2((s.v00.x + s.v01.x) * 0.5f, (s.v00.y + s.v01.y) * 0.5f);
+  const f2 m1 = mk2((s.v10.x + s.v11.x) * 0.5f, (s.v10.y + s.v11.y) * 0.5f);
+  const f2 ax = m1 - m0;
+  const f2 c = mk2((m0.x + m1.x) * 0.5f, (m0.y + m1.y) * 0.5f);
+  const f2 n = mk2(-ax.y, ax.x);
+  const float la = sqrtf(ax.x * ax.x + ax.y * ax.y);
+  const f2 vs[4] = {s.v00, s.v01, s.v10, s.v11};
+  float R = 0.0f, T0 = 3.0e38f, T1 = -3.0e38f;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const f2 q = vs[k] - c;
+    R = fmaxf(R, fabsf(n.x * q.x + n.y * q.y));
+    const float t = ax.x * q.x + ax.y * q.y;
+    T0 = fminf(T0, t);
+    T1 = fmaxf(T1, t);
+  }
+  const bool on = usable && la > 0.0f && isfinite(la) && isfinite(R) && isfinite(T0) && isfinite(T1);
+  f[kSpCx * stride] = c.x;
+  f[kSpCy * stride] = c.y;
+  f[kSpNy * stride] = n.y;
+  f[kSpR * stride] = R + la;
+  f[kSpINx * stride] = fabsf(n.x) > 1e-6f * la ? 1.0f / n.x : 0.0f;
+  f[kSpAy * stride] = ax.y;
+  f[kSpT0 * stride] = T0 - la;
+  f[kSpT1 * stride] = T1 + la;
+  f[kSpIAx * stride] = fabsf(ax.x) > 1e-6f * la ? 1.0f / ax.x : 0.0f;
+  f[kSpOn * stride] = on ? 1.0f : 0.0f;
+}
+
+// x range (absolute) that may hold inside pixels on rows y in [y0, y1];
+// empty when lo > hi. A constraint whose x coefficient vanishes bounds y only.
+template <uint32_t STRIDE>
+__device__ __forceinline__ void span_x(const float* f, float y0, float y1, float& lo, float& hi) {
+  lo = -3.0e38f;
+  hi = 3.0e38f;
+  const float cx = f[kSpCx * STRIDE], cy = f[kSpCy * STRIDE];
+  const float r0 = y0 - cy, r1 = y1 - cy;
+  const float ny = f[kSpNy * STRIDE], R = f[kSpR * STRIDE], inx = f[kSpINx * STRIDE];
+  if (inx != 0.0f) {
+    const float a0 = (-R - ny * r0) * inx, b0 = (R - ny * r0) * inx;
+    const float a1 = (-R - ny * r1) * inx, b1 = (R - ny * r1) * inx;
+    lo = fmaxf(lo, fminf(fminf(a0, b0), fminf(a1, b1)));
+    hi = fminf(hi, fmaxf(fmaxf(a0, b0), fmaxf(a1, b1)));
+  } else if (fminf(ny * r0, ny * r1) > R || fmaxf(ny * r0, ny * r1) < -R) {
+    lo = 1.0f; hi = 0.0f;
+  }
+  const float ay = f[kSpAy * STRIDE], T0 = f[kSpT0 * STRIDE], T1 = f[kSpT1 * STRIDE], iax = f[kSpIAx * STRIDE];
+  if (iax != 0.0f) {
+    const float a0 = (T0 - ay * r0) * iax, b0 = (T1 - ay * r0) * iax;
+    const float a1 = (T0 - ay * r1) * iax, b1 = (T1 - ay * r1) * iax;
+    lo = fmaxf(lo, fminf(fminf(a0, b0), fminf(a1, b1)));
+    hi = fminf(hi, fmaxf(fmaxf(a0, b0), fmaxf(a1, b1)));
+  } else if (fminf(ay * r0, ay * r1) > T1 || fmaxf(ay * r0, ay * r1) < T0) {
+    lo = 1.0f; hi = 0.0f;
+  }
+  lo += cx;
+  hi += cx;
+}
+
 struct TileWaveLds {
   // per segment (lane of the wave), field-major so that lanes reading
   // different segments hit different banks
   float v[8][64];          // v00, v01, v10, v11
   float d[4][64];          // projected depths
-  float slope[6][64];      // dx/dy of the six vertex pairs (hull_strip_x)
+  float sp[kSpFields][64]; // span frame (span_frame)
+  float ip[kIpFields][64]; // per-segment interpolation terms (interp_pre)
   int box[4][64];          // pixel box (seg_pixel_bbox)
   uint32_t rect[3][64];    // first tile x, first tile y, tiles per row
   uint32_t gid[64];        // segment index
@@ -173,67 +247,51 @@ struct TileWaveLds {
   uint32_t queue[kTileQueue];
 };
 
-// The six vertex pairs of a quad, as indices into v00, v01, v10, v11.
-__device__ __forceinline__ void hull_pair(int k, int& i, int& j) {
-  i = k < 3 ? 0 : (k < 5 ? 1 : 2);
-  j = k < 3 ? k + 1 : (k < 5 ? k - 1 : 3);
-}
-
-// x extent of the convex hull of the quad within the horizontal strip
-// [ya, yb]: the hull vertices in the strip and the crossings of the strip's
-// edges with the six vertex pairs (the hull's edges are among them and the
-// other pairs are chords inside it). Empty when xlo > xhi. Vertex k's
-// coordinates and slope k are STRIDE floats apart.
-template <int STRIDE>
-__device__ __forceinline__ void hull_strip_x(const float* v, const float* slope, float ya, float yb,
-                                             float& xlo, float& xhi) {
-  xlo = 3.0e38f;
-  xhi = -3.0e38f;
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const float x = v[(2 * i) * STRIDE], y = v[(2 * i + 1) * STRIDE];
-    if (y >= ya && y <= yb) { xlo = fminf(xlo, x); xhi = fmaxf(xhi, x); }
-  }
-#pragma unroll
-  for (int k = 0; k < 6; k++) {
-    int i, j;
-    hull_pair(k, i, j);
-    const float py = v[(2 * i + 1) * STRIDE], qy = v[(2 * j + 1) * STRIDE], px = v[(2 * i) * STRIDE];
-    const float sl = slope[k * STRIDE];
-    const float lo = fminf(py, qy), hi = fmaxf(py, qy);
-    if (ya > lo && ya < hi) { const float x = px + (ya - py) * sl; xlo = fminf(xlo, x); xhi = fmaxf(xhi, x); }
-    if (yb > lo && yb < hi) { const float x = px + (yb - py) * sl; xlo = fminf(xlo, x); xhi = fmaxf(xhi, x); }
-  }
-}
-
 // Step 4: the queued inside pixels [head, head + n), n <= 64, one per lane.
+// Queued pixels of one pair are consecutive, so the lanes of a run reduce
+// their (min depth, max depth, min bin) among themselves and only the run's
+// last lane touches the pair's LDS accumulators.
 __device__ __forceinline__ void tiles_shade_queue(const TilesArgs& a, TileWaveLds& L, uint32_t head,
                                                   uint32_t n, const TileGeom& tg, f2 dbVS) {
   const uint32_t lane = __lane_id();
+  uint32_t slot = 0xffffffffu, vmin = 0xffffffffu, vmax = 0u, vbin = 0xffffffffu;
   if (lane < n) {
     const uint32_t e = L.queue[(head + lane) & (kTileQueue - 1u)];
-    const uint32_t slot = e >> 16;
+    slot = e >> 16;
     const uint32_t x = (uint32_t)L.px0[slot] + (e & 0xffu);
     const uint32_t y = (uint32_t)L.py0[slot] + ((e >> 8) & 0xffu);
     const uint32_t o = L.owner[slot];
-    Seg ps;
-    ps.v00 = mk2(L.v[0][o], L.v[1][o]); ps.v01 = mk2(L.v[2][o], L.v[3][o]);
-    ps.v10 = mk2(L.v[4][o], L.v[5][o]); ps.v11 = mk2(L.v[6][o], L.v[7][o]);
-    ps.depths = mk4(L.d[0][o], L.d[1][o], L.d[2][o], L.d[3][o]);
     const f2 p = mk2((float)x, (float)y);
-    const f2 w = interp_quad(ps, p);
-    const float hdp = interp_f32(w, ps.depths);
+    const f2 w = interp_quad_pre<64>(&L.ip[0][o], mk2(L.v[0][o], L.v[1][o]), p);
+    const float hdp = interp_f32(w, mk4(L.d[0][o], L.d[1][o], L.d[2][o], L.d[3][o]));
     const float db = tex_load(a.depth, a.depthW, a.depthH, (int32_t)x, (int32_t)(tg.vpH - y));
     if (!(hdp > db)) {  // depth test with GL_LESS (:193-195)
       const f3 vs = project(a.u->projMatrixInv, mk4(p.x, p.y, hdp, 1.0f));
-      const uint32_t bin = (a.B - 1u) - depth_bin(a.B, dbVS, vs.z);
+      vbin = (a.B - 1u) - depth_bin(a.B, dbVS, vs.z);
       // fminf/fmaxf in the reference skip a NaN depth; its bin still counts
-      if (hdp == hdp) {
-        atomicMin(&L.accMin[slot], f32_ord(hdp));
-        atomicMax(&L.accMax[slot], f32_ord(hdp));
-      }
-      atomicMin(&L.accBin[slot], bin);
+      if (hdp == hdp) vmin = vmax = f32_ord(hdp);
     }
+  }
+  // segmented inclusive min/max over runs of equal slot
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t os = (uint32_t)__shfl_up((int)slot, o);
+    const uint32_t omin = (uint32_t)__shfl_up((int)vmin, o);
+    const uint32_t omax = (uint32_t)__shfl_up((int)vmax, o);
+    const uint32_t obin = (uint32_t)__shfl_up((int)vbin, o);
+    if (lane >= (uint32_t)o && os == slot) {
+      vmin = min(vmin, omin);
+      vmax = max(vmax, omax);
+      vbin = min(vbin, obin);
+    }
+  }
+  const uint32_t nextSlot = (uint32_t)__shfl_down((int)slot, 1);
+  if (lane < n && (lane == n - 1u || nextSlot != slot)) {
+    if (vmin != 0xffffffffu) {
+      atomicMin(&L.accMin[slot], vmin);
+      atomicMax(&L.accMax[slot], vmax);
+    }
+    if (vbin != 0xffffffffu) atomicMin(&L.accBin[slot], vbin);
   }
   wave_sync();
 }
@@ -278,14 +336,12 @@ __device__ __forceinline__ void tiles_wave(const TilesArgs& a, TileWaveLds& L, u
     int cx0, cy0, cx1, cy1;
     seg_pixel_bbox(ps, cx0, cy0, cx1, cy1);
     const float vv[8] = {ps.v00.x, ps.v00.y, ps.v01.x, ps.v01.y, ps.v10.x, ps.v10.y, ps.v11.x, ps.v11.y};
+    float amax = 0.0f;
 #pragma unroll
-    for (int i = 0; i < 8; i++) L.v[i][lane] = vv[i];
-#pragma unroll
-    for (int k = 0; k < 6; k++) {
-      int i, j;
-      hull_pair(k, i, j);
-      L.slope[k][lane] = (vv[2 * j] - vv[2 * i]) / (vv[2 * j + 1] - vv[2 * i + 1]);
-    }
+    for (int i = 0; i < 8; i++) { L.v[i][lane] = vv[i]; amax = fmaxf(amax, fabsf(vv[i])); }
+    // collinear quads test whole tiles; far-off vertices skip the span
+    span_frame(ps, cx0 > -(1 << 20) && amax < 65536.0f, &L.sp[0][lane], 64);
+    interp_pre(ps, &L.ip[0][lane], 64);
     L.d[0][lane] = ps.depths.x; L.d[1][lane] = ps.depths.y; L.d[2][lane] = ps.depths.z; L.d[3][lane] = ps.depths.w;
     L.box[0][lane] = cx0; L.box[1][lane] = cy0; L.box[2][lane] = cx1; L.box[3][lane] = cy1;
     L.rect[0][lane] = tminx; L.rect[1][lane] = tminy; L.rect[2][lane] = tsx;
@@ -333,14 +389,11 @@ __device__ __forceinline__ void tiles_wave(const TilesArgs& a, TileWaveLds& L, u
       const int ys = max((int)by0, L.box[1][o]), ye = min((int)by1, L.box[3][o] + 1);
       if (xe > xs && ye > ys) {
         rows = (uint32_t)(ye - ys);
-        // a tile the hull does not reach (within 1 px) has no inside pixel
-        float amax = 0.0f;
-#pragma unroll
-        for (int q = 0; q < 8; q++) amax = fmaxf(amax, fabsf(L.v[q][o]));
-        if (L.box[0][o] > -(1 << 20) && amax < 65536.0f) {
+        // a tile the span bound does not reach has no inside pixel
+        if (L.sp[kSpOn][o] != 0.0f) {
           float xlo, xhi;
-          hull_strip_x<64>(&L.v[0][o], &L.slope[0][o], (float)ys - 1.0f, (float)(ye - 1) + 1.0f, xlo, xhi);
-          if (!(xlo <= xhi) || (int)floorf(xhi + 1.0f) < xs || (int)ceilf(xlo - 1.0f) >= xe) rows = 0;
+          span_x<64>(&L.sp[0][o], (float)ys, (float)(ye - 1), xlo, xhi);
+          if (!(xlo <= xhi) || floorf(xhi) < (float)xs || ceilf(xlo) >= (float)xe) rows = 0;
         }
       }
       L.owner[lane] = o;
@@ -355,8 +408,8 @@ __device__ __forceinline__ void tiles_wave(const TilesArgs& a, TileWaveLds& L, u
     wave_sync();
 
     uint32_t head = 0, tail = 0;
-    // 3a. pixel rows of the pairs, 64 at a time: the conservative x span of
-    // each row (hull within 1 px of the row, widened by 1 px)
+    // 3a. pixel rows of the pairs, 64 at a time, each with its conservative
+    // x span
     for (uint32_t r0 = 0; r0 < nRows; r0 += 64u) {
       const uint32_t ridx = r0 + lane;
       uint32_t cnt = 0;
@@ -365,18 +418,12 @@ __device__ __forceinline__ void tiles_wave(const TilesArgs& a, TileWaveLds& L, u
         const int y = L.py0[slot] + (int)(ridx - L.rowStart[slot]);
         const uint32_t o = L.owner[slot];
         int xa = L.px0[slot], xb = L.px1[slot];
-        const float* v = &L.v[0][o];
-        float amax = 0.0f;
-#pragma unroll
-        for (int q = 0; q < 8; q++) amax = fmaxf(amax, fabsf(v[q * 64]));
-        // collinear quads test whole tiles; far-off vertices skip the span
-        const bool span = L.box[0][o] > -(1 << 20) && amax < 65536.0f;
-        if (span) {
+        if (L.sp[kSpOn][o] != 0.0f) {
           float xlo, xhi;
-          hull_strip_x<64>(v, &L.slope[0][o], (float)y - 1.0f, (float)y + 1.0f, xlo, xhi);
+          span_x<64>(&L.sp[0][o], (float)y, (float)y, xlo, xhi);
           if (xlo <= xhi) {
-            xa = max(xa, (int)ceilf(xlo - 1.0f));
-            xb = min(xb, (int)floorf(xhi + 1.0f) + 1);
+            xa = max(xa, (int)fmaxf(ceilf(xlo), -1.0e9f));
+            xb = min(xb, (int)fminf(floorf(xhi), 1.0e9f) + 1);
           } else {
             xb = xa;
           }
@@ -762,9 +809,9 @@ __global__ __launch_bounds__(256) void bin_sort_large_kernel(const fh_tile_resul
 // pixel rows [w * TILE / NW, (w + 1) * TILE / NW) of the tile. Per depth bin,
 // the bin's segments arrive sorted by blend key (largest first) in batches:
 //  * one thread per segment projects it into LDS (field-major records: quad,
-//    depths, hull slopes, pixel box relative to the tile, nearest depth);
+//    depths, span frame, pixel box relative to the tile, nearest depth);
 //  * each wave enumerates the (segment, pixel) candidates of its rows,
-//    record by record (box rows x hull spans), 64 per step, runs the edge
+//    record by record (box rows x row spans), 64 per step, runs the edge
 //    tests and a conservative depth pre-test, and queues the covered pixels
 //    in its LDS ring in that order;
 //  * every 64 queued pixels the wave computes the fragments together:
@@ -802,15 +849,14 @@ struct FineArgs {
 // Segment records of one batch, field-major in LDS (field f of record j at
 // f * batch + j): lanes reading different records hit different banks.
 enum RecField : uint32_t {
-  kRecV = 0,       // 8: v00, v01, v10, v11
-  kRecD = 8,       // 4: projected depths
-  kRecSlope = 12,  // 6: dx/dy of the six vertex pairs (hull_strip_x)
-  kRecKey = 18,    // strand << 8 | segment
-  kRecBox0 = 19,   // pixel box relative to the tile, inclusive, clamped to
-  kRecBox1 = 20,   //   [-1, TILE]: (x0 | y0 << 16), (x1 | y1 << 16) as int16
-  kRecNear = 21,   // lower bound of the interpolated depth (-inf when unknown)
-  kRecSpan = 22,   // 1 when per-row hull spans may narrow the box
-  kRecFields = 23
+  kRecV = 0,                     // 8: v00, v01, v10, v11
+  kRecD = 8,                     // 4: projected depths
+  kRecSpan = 12,                 // kSpFields: span frame (span_frame)
+  kRecKey = kRecSpan + kSpFields,  // strand << 8 | segment
+  kRecBox0,                      // pixel box relative to the tile, inclusive, clamped to
+  kRecBox1,                      //   [-1, TILE]: (x0 | y0 << 16), (x1 | y1 << 16) as int16
+  kRecNear,                      // lower bound of the interpolated depth (-inf when unknown)
+  kRecFields
 };
 
 template <uint32_t RB>
@@ -921,7 +967,7 @@ __device__ __forceinline__ void fine_shade_queue(const FineArgs& a, const FineLd
 
 // Cheap tests of one batch for wave `w`. Candidates are enumerated record by
 // record (records are in blend order), row by row within the record's box and
-// its hull span, 64 per step; covered pixels are queued in that order.
+// its row span, 64 per step; covered pixels are queued in that order.
 template <int TILE>
 __device__ __forceinline__ void fine_cheap_batch(const FineArgs& a, const FineLds<TILE>& L, uint4* gpool,
                                                  uint32_t nb, uint32_t bx0, uint32_t by0, f2 td,
@@ -959,13 +1005,13 @@ __device__ __forceinline__ void fine_cheap_batch(const FineArgs& a, const FineLd
         const uint32_t j = rb + k;
         const int y = max(R.hi16(kRecBox0, j), wy0) + (int)(ridx - rowStart[k]);
         int xa = max(R.lo16(kRecBox0, j), 0), xb = min(R.lo16(kRecBox1, j), TILE - 1) + 1;
-        if (R.u(kRecSpan, j)) {
+        if (R.at(kRecSpan + kSpOn, j) != 0.0f) {
           float xlo, xhi;
           const float yf = (float)(by0 + (uint32_t)y);
-          hull_strip_x<fine_batch<TILE>()>(&R.at(kRecV, j), &R.at(kRecSlope, j), yf - 1.0f, yf + 1.0f, xlo, xhi);
+          span_x<fine_batch<TILE>()>(&R.at(kRecSpan, j), yf, yf, xlo, xhi);
           if (xlo <= xhi) {
-            xa = max(xa, (int)ceilf(xlo - 1.0f) - (int)bx0);
-            xb = min(xb, (int)floorf(xhi + 1.0f) + 1 - (int)bx0);
+            xa = max(xa, (int)fmaxf(ceilf(xlo), -1.0e9f) - (int)bx0);
+            xb = min(xb, (int)fminf(floorf(xhi), 1.0e9f) + 1 - (int)bx0);
           } else {
             xb = xa;
           }
@@ -1095,12 +1141,8 @@ __global__ __launch_bounds__(fine_waves<TILE>() * 64) void fine_raster_kernel(Fi
           float amax = 0.0f;
 #pragma unroll
           for (int i = 0; i < 8; i++) { R.at(kRecV + i, tid) = vv[i]; amax = fmaxf(amax, fabsf(vv[i])); }
-#pragma unroll
-          for (int k = 0; k < 6; k++) {
-            int i, j;
-            hull_pair(k, i, j);
-            R.at(kRecSlope + k, tid) = (vv[2 * j] - vv[2 * i]) / (vv[2 * j + 1] - vv[2 * i + 1]);
-          }
+          // collinear quads test the whole box; far-off vertices skip the span
+          span_frame(sg, x0 > -(1 << 20) && amax < 65536.0f, &R.at(kRecSpan, tid), RB);
           R.at(kRecD + 0, tid) = sg.depths.x; R.at(kRecD + 1, tid) = sg.depths.y;
           R.at(kRecD + 2, tid) = sg.depths.z; R.at(kRecD + 3, tid) = sg.depths.w;
           R.at(kRecKey, tid) = __uint_as_float(key);
@@ -1109,8 +1151,6 @@ __global__ __launch_bounds__(fine_waves<TILE>() * 64) void fine_raster_kernel(Fi
           const int rx1 = max(min(x1 - (int)bx0, TILE), -1), ry1 = max(min(y1 - (int)by0, TILE), -1);
           R.at(kRecBox0, tid) = __uint_as_float(((uint32_t)rx0 & 0xffffu) | ((uint32_t)ry0 << 16));
           R.at(kRecBox1, tid) = __uint_as_float(((uint32_t)rx1 & 0xffffu) | ((uint32_t)ry1 << 16));
-          // collinear quads test the whole box; far-off vertices skip the span
-          R.at(kRecSpan, tid) = __uint_as_float((x0 > -(1 << 20) && amax < 65536.0f) ? 1u : 0u);
           // The interpolated depth is a two-level lerp of the four depths with
           // weights in [0, 1]: at least their minimum, less the rounding of
           // two lerps (a few ulp of the largest magnitude). Non-finite depths

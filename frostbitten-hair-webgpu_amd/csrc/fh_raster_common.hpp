@@ -85,6 +85,46 @@ __device__ __forceinline__ f2 interp_quad(const Seg& s, f2 c) {
   return mk2(satf(d0), satf(d1));
 }
 
+// interp_quad split into its per-segment terms (computed once) and its
+// per-pixel terms: the same operations in the same order, so the results are
+// bit-identical.
+enum InterpField : uint32_t {
+  kIpSmx = 0, kIpSmy, kIpAb1x, kIpAb1y, kIpQ1, kIpL1, kIpWs, kIpWe, kIpAb2x, kIpAb2y, kIpQ2, kIpFields
+};
+
+__device__ __forceinline__ void interp_pre(const Seg& s, float* f, uint32_t stride) {
+  const f2 sm = (s.v00 + s.v01) / 2.0f;
+  const f2 em = (s.v10 + s.v11) / 2.0f;
+  const f2 ab1 = em - sm;
+  const float ws = length(s.v00 - s.v01);
+  const float we = ws * dot(normalize(s.v00 - s.v01), normalize(s.v10 - s.v11));
+  const f2 ab2 = s.v10 - s.v00;
+  f[kIpSmx * stride] = sm.x;
+  f[kIpSmy * stride] = sm.y;
+  f[kIpAb1x * stride] = ab1.x;
+  f[kIpAb1y * stride] = ab1.y;
+  f[kIpQ1 * stride] = dot(ab1, ab1);
+  f[kIpL1 * stride] = length(sm - em);
+  f[kIpWs * stride] = ws;
+  f[kIpWe * stride] = we;
+  f[kIpAb2x * stride] = ab2.x;
+  f[kIpAb2y * stride] = ab2.y;
+  f[kIpQ2 * stride] = dot(ab2, ab2);
+}
+
+template <uint32_t STRIDE>
+__device__ __forceinline__ f2 interp_quad_pre(const float* f, f2 v00, f2 c) {
+  const f2 sm = mk2(f[kIpSmx * STRIDE], f[kIpSmy * STRIDE]);
+  const f2 ab1 = mk2(f[kIpAb1x * STRIDE], f[kIpAb1y * STRIDE]);
+  const f2 cp = sm + (ab1 * dot(ab1, c - sm)) / f[kIpQ1 * STRIDE];
+  const float d1 = length(cp - sm) / f[kIpL1 * STRIDE];
+  const float ew = mixf(f[kIpWs * STRIDE], f[kIpWe * STRIDE], d1);
+  const f2 ab2 = mk2(f[kIpAb2x * STRIDE], f[kIpAb2y * STRIDE]);
+  const f2 e1 = v00 + (ab2 * dot(ab2, c - v00)) / f[kIpQ2 * STRIDE];
+  const float d0 = length(c - e1) / ew;
+  return mk2(satf(d0), satf(d1));
+}
+
 // interpolateHairF32 — swRasterizeHair.wgsl.ts:199-203
 __device__ __forceinline__ float interp_f32(f2 w, f4 v) {
   const float a = mixf(v.x, v.y, w.x);
