@@ -48,7 +48,12 @@ def kernel_bytes(wl, n_pairs: int, tiles: int) -> dict:
         # prev + now + initial (16 B each), rest length, write of the new position
         "sim_integrate": 68 * N + 32 * d3 + 4 * V,
         "grid_pre_sim": 32 * d3 + 4 * V,
-        "grid_post_sim": 32 * N + 16 * d3,
+        # P2G: binning reads each position once per pass and writes a pair index;
+        # the accumulation reads positions and pair indices, and adds the grid once
+        "p2g_clear": 16 * d3,
+        "p2g_count": 16 * N,
+        "p2g_scatter": 20 * N,
+        "grid_post_sim": 36 * N + 16 * d3,
         # positions + tangents, mesh depth, list element + slot tag per pair, tile headers
         "tiles_draw": 32 * N + 4 * WH + 12 * n_pairs + 16 * tiles * B,
         "bin_scatter": 12 * n_pairs + 4 * tiles * B,
@@ -258,7 +263,8 @@ def main() -> None:
     per_frame = np.array([evs[k][0].elapsed_time(evs[k][5]) for k in range(args.steps)])  # ms
     if use_graph:
         # stage sums from the per-kernel events
-        groups = {"sim": ("grid_pre_sim", "sim_integrate", "grid_post_sim"),
+        groups = {"sim": ("grid_pre_sim", "sim_integrate", "p2g_clear", "p2g_count", "p2g_scan", "p2g_scatter",
+                         "grid_post_sim"),
                   "tiles": ("fill_many", "tiles_draw", "tiles_pieces", "tile_counts"),
                   "sort": ("sort_count", "sort_scatter"),
                   "fine": ("bin_scan_local", "bin_scan_add", "bin_scatter", "bin_sort_small", "bin_sort_large",

@@ -136,7 +136,7 @@ int add_timing_nodes(fh_graph* g, const std::vector<const char*>& names) {
   return FH_OK;
 }
 
-int zero_many(fh_ctx* ctx, int count, void* const* dst, const uint64_t* n_words) {
+int zero_many(fh_ctx* ctx, int count, void* const* dst, const uint64_t* n_words, const char* kernel_name) {
   FillJobs jobs;
   uint64_t most = 0;
   int k = 0;
@@ -150,7 +150,7 @@ int zero_many(fh_ctx* ctx, int count, void* const* dst, const uint64_t* n_words)
   if (k == 0) return FH_OK;
   uint64_t blocks = (most + 255) / 256;
   if (blocks > 2048) blocks = 2048;
-  const int pk = fh::prof_begin(ctx, "fill_many");
+  const int pk = fh::prof_begin(ctx, kernel_name);
   fill_many_kernel<<<dim3((unsigned)blocks, (unsigned)k), 256, 0, ctx->stream>>>(jobs);
   fh::prof_end(ctx, pk);
   return check_launch("fill_many_kernel");

@@ -50,7 +50,8 @@ int ensure_scratch(fh_ctx* ctx, size_t bytes);
 int prof_begin(fh_ctx* ctx, const char* kernel);
 // Zeroes up to kMaxFillJobs buffers (n_words u32 each) in one launch.
 constexpr int kMaxFillJobs = 4;
-int zero_many(fh_ctx* ctx, int count, void* const* dst, const uint64_t* n_words);
+int zero_many(fh_ctx* ctx, int count, void* const* dst, const uint64_t* n_words,
+              const char* kernel_name = "fill_many");
 void prof_end(fh_ctx* ctx, int k);
 
 inline uint32_t div_ceil(uint32_t a, uint32_t b) { return (a + b - 1u) / b; }

@@ -6,7 +6,6 @@
 
 namespace fh {
 int fill_u32(fh_ctx* ctx, void* dst, uint64_t n_words, uint32_t value);
-int zero_many(fh_ctx* ctx, int count, void* const* dst, const uint64_t* n_words);
 }
 
 namespace {
@@ -14,12 +13,6 @@ namespace {
 using namespace fh;
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
-
-// number of set bits of `mask` below the calling lane
-__device__ __forceinline__ uint32_t mask_prefix(uint64_t mask) {
-  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 #pragma unroll
@@ -100,22 +93,8 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// Inclusive prefix sum / max over the 64 lanes with DPP: row_shr:1,2,4,8
-// within each row of 16 lanes, then row_bcast:15 (into rows 1 and 3) and
-// row_bcast:31 (into rows 2 and 3). GFX9 DPP controls: row_shr:n = 0x110 + n,
-// row_bcast:15 = 0x142, row_bcast:31 = 0x143. A lane whose source is outside
-// its row keeps `old`, the identity.
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-  int x = (int)v;
-  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
-  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
-  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
-  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
-  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);
-  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);
-  return (uint32_t)x;
-}
-
+// Inclusive prefix max over the 64 lanes with DPP, the same steps as
+// wave_incl_scan.
 __device__ __forceinline__ int wave_incl_max(int x) {
   constexpr int lo = -2147483647 - 1;
   x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x111, 0xf, 0xf, false));

@@ -331,71 +331,218 @@ __global__ __launch_bounds__(kSimThreads) void sim_integrate_kernel(
 // ---------------------------------------------------------------------------
 // K3 — gridPostSimPass.wgsl.ts:45-72 with _addGridDensityVelocity
 // (densityVelocityGrid.wgsl.ts:93-124). Every point adds fixed-point
-// velocity and density to the 8 grid nodes around it. Hair is dense around
-// few nodes (a whole head of strands starts at the scalp), so direct global
-// atomics serialize on the hot nodes. Instead each workgroup folds the
-// contributions of kP2GPoints consecutive points into an LDS hash table
-// (node -> 4 x i32) and then adds each table entry to the grid with one
-// global atomic per non-zero component. i32 addition is associative and
-// commutative (mod 2^32), so the grid is bit-identical to one atomic per
-// contribution in any order; a node that finds no free table slot within
-// kP2GProbe probes goes straight to the global grid.
+// velocity and density to the 8 grid nodes around it. A head of hair piles
+// thousands of contributions onto the same nodes, so one global atomic per
+// contribution serializes. Here the grid is cut into blocks of 8^3 nodes:
+//   1. every point counts itself into the blocks its corners fall in (one
+//      atomic per wave per block: a wave is two runs along strands);
+//   2. one workgroup scans the counts into per-block ranges and cuts each
+//      range into chunks of `chunk` (point, block) pairs;
+//   3. the points are scattered into their blocks' ranges;
+//   4. one workgroup per chunk adds the chunk's contributions to an LDS copy
+//      of its block, then adds each non-zero node component to the grid.
+// i32 addition is associative and commutative (mod 2^32), so the grid is
+// bit-identical to one atomic per contribution in any order.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kP2GThreads = 1024;
-constexpr uint32_t kP2GPoints = 2048;
-constexpr uint32_t kP2GTable = 4096;  // power of two
-constexpr uint32_t kP2GProbe = 64;
-constexpr uint32_t kEmptyNode = 0xffffffffu;
-static_assert(kP2GTable == 1u << 12, "the hash below takes 12 bits");
+constexpr uint32_t kBlk = 8;
+constexpr uint32_t kBlkNodes = kBlk * kBlk * kBlk;
 
-__global__ __launch_bounds__(kP2GThreads) void grid_post_sim_kernel(
-    uint32_t d, const fh_sim_uniforms* __restrict__ u, const float4* __restrict__ prev,
-    const float4* __restrict__ now, fh_density_velocity* __restrict__ dv, uint32_t nPoints) {
-  __shared__ uint32_t keys[kP2GTable];
-  __shared__ int32_t vals[kP2GTable][4];
-  for (uint32_t i = threadIdx.x; i < kP2GTable; i += kP2GThreads) {
-    keys[i] = kEmptyNode;
-    vals[i][0] = vals[i][1] = vals[i][2] = vals[i][3] = 0;
+struct P2GArgs {
+  uint32_t d, nb;  // grid dims, blocks per axis
+  const fh_sim_uniforms* u;
+  const float4* prev;
+  const float4* now;
+  fh_density_velocity* dv;
+  uint32_t nPoints;
+  uint32_t chunk;        // pairs per accumulation workgroup
+  uint32_t* counts;      // nb^3
+  uint32_t* cursors;     // nb^3
+  uint32_t* offsets;     // nb^3: first pair of each block
+  uint32_t* chunkStart;  // nb^3 + 1: first chunk of each block, total last
+  uint32_t* pairs;       // point index per (point, block) pair
+};
+
+// Blocks touched by the corner nodes of point e: [b0, b1] per axis.
+__device__ __forceinline__ void point_blocks(const P2GArgs& a, f3 gmin, f3 gmax, uint32_t e, uint32_t b0[3],
+                                             uint32_t b1[3]) {
+  const float4 pn = a.now[e];
+  const Cell co = grid_cell(a.d, gmin, gmax, mk3(pn.x, pn.y, pn.z));
+#pragma unroll
+  for (int i = 0; i < 3; i++) { b0[i] = co.mn[i] / kBlk; b1[i] = co.mx[i] / kBlk; }
+}
+
+__device__ __forceinline__ uint32_t blk_idx(const P2GArgs& a, uint32_t x, uint32_t y, uint32_t z) {
+  return (z * a.nb + y) * a.nb + x;
+}
+
+// Adds `add` to counter[key] for every active lane, one atomic per distinct
+// key in the wave; returns the lane's position among the lanes sharing its
+// key, on top of the counter's old value.
+__device__ __forceinline__ uint32_t wave_keyed_add(uint32_t* counter, uint32_t key, bool active) {
+  const uint32_t lane = __lane_id();
+  uint32_t result = 0;
+  bool pend = active;
+  while (true) {
+    const uint64_t pm = __ballot(pend);
+    if (pm == 0) break;
+    const uint32_t leader = (uint32_t)__builtin_ctzll(pm);
+    const uint32_t lk = (uint32_t)__shfl((int)key, (int)leader);
+    const uint64_t m = __ballot(pend && key == lk);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&counter[lk], (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, (int)leader);
+    if (pend && key == lk) {
+      result = base + mask_prefix(m);
+      pend = false;
+    }
   }
+  return result;
+}
+
+// Steps 1 and 3. A workgroup takes `span` consecutive points. Each point
+// visits the up to 8 blocks its corner nodes fall in, one block offset
+// (dx, dy, dz) at a time so that a wave counts equal blocks with one atomic.
+// With a block count that fits LDS the counts go to a workgroup-private
+// histogram first, so hot blocks see one global atomic per workgroup:
+//   count:   histogram, then counts[b] += hist[b];
+//   scatter: histogram, then hist[b] = offsets[b] + (cursors[b] += hist[b]),
+//            then a second walk takes each pair's slot from hist[b].
+// Without LDS (a grid too large) the same walk goes to the global counters.
+constexpr uint32_t kP2GBinThreads = 256;
+constexpr uint32_t kP2GHistMax = 8192;  // blocks held in LDS: grids up to 160^3
+
+template <bool SCATTER>
+__device__ __forceinline__ void p2g_walk(const P2GArgs& a, f3 gmin, f3 gmax, uint32_t first, uint32_t last,
+                                         uint32_t* ctr, bool place) {
+  for (uint32_t base = first; base < last; base += kP2GBinThreads) {
+    const uint32_t e = base + threadIdx.x;
+    const bool valid = e < last;
+    uint32_t b0[3] = {0, 0, 0}, b1[3] = {0, 0, 0};
+    if (valid) point_blocks(a, gmin, gmax, e, b0, b1);
+#pragma unroll 1
+    for (uint32_t o = 0; o < 8u; o++) {
+      const uint32_t x = b0[0] + (o & 1u), y = b0[1] + ((o >> 1) & 1u), z = b0[2] + (o >> 2);
+      const bool on = valid && x <= b1[0] && y <= b1[1] && z <= b1[2];
+      const uint32_t blk = on ? blk_idx(a, x, y, z) : 0u;
+      const uint32_t r = wave_keyed_add(ctr, blk, on);
+      if (SCATTER && place && on) a.pairs[r] = e;
+    }
+  }
+}
+
+template <bool SCATTER, bool LDS>
+__global__ __launch_bounds__(kP2GBinThreads) void p2g_bin_kernel(P2GArgs a, uint32_t nBlocks, uint32_t span) {
+  __shared__ uint32_t hist[LDS ? kP2GHistMax : 1];
+  const f3 gmin = mk3(a.u->gridBoundsMin[0], a.u->gridBoundsMin[1], a.u->gridBoundsMin[2]);
+  const f3 gmax = mk3(a.u->gridBoundsMax[0], a.u->gridBoundsMax[1], a.u->gridBoundsMax[2]);
+  const uint32_t first = blockIdx.x * span;
+  const uint32_t last = min(first + span, a.nPoints);
+  if (!LDS) {
+    // global counters; the scatter turns each cursor into a slot through
+    // offsets, so the cursors start at each block's first pair
+    p2g_walk<SCATTER>(a, gmin, gmax, first, last, SCATTER ? a.cursors : a.counts, SCATTER);
+    return;
+  }
+  for (uint32_t i = threadIdx.x; i < nBlocks; i += kP2GBinThreads) hist[i] = 0;
   __syncthreads();
-  const f3 gmin = mk3(u->gridBoundsMin[0], u->gridBoundsMin[1], u->gridBoundsMin[2]);
-  const f3 gmax = mk3(u->gridBoundsMax[0], u->gridBoundsMax[1], u->gridBoundsMax[2]);
-  const uint32_t first = blockIdx.x * kP2GPoints;
-  for (uint32_t e = first + threadIdx.x; e < min(first + kP2GPoints, nPoints); e += kP2GThreads) {
-    const float4 pp = prev[e], pn = now[e];
-    const f3 p = mk3(pn.x, pn.y, pn.z);
-    const f3 vel = p - mk3(pp.x, pp.y, pp.z);
-    const Cell co = grid_cell(d, gmin, gmax, p);
-    for (uint32_t z = co.mn[2]; z <= co.mx[2]; z++)
-      for (uint32_t y = co.mn[1]; y <= co.mx[1]; y++)
-        for (uint32_t x = co.mn[0]; x <= co.mx[0]; x++) {
-          const f3 w = cell_w(x, y, z, co.pin);
-          const f3 cv = vel * w;
-          const int32_t c[4] = {f2i(cv.x * kGridMul), f2i(cv.y * kGridMul), f2i(cv.z * kGridMul),
-                                f2i(length(w) * kGridMul)};
-          if ((c[0] | c[1] | c[2] | c[3]) == 0) continue;  // adding 0 changes nothing
-          const uint32_t node = grid_idx(d, x, y, z);
-          uint32_t h = (node * 2654435761u) >> (32 - 12);
-          int32_t* slot = nullptr;
-          for (uint32_t k = 0; k < kP2GProbe; k++) {
-            const uint32_t old = atomicCAS(&keys[h], kEmptyNode, node);
-            if (old == kEmptyNode || old == node) { slot = vals[h]; break; }
-            h = (h + 1u) & (kP2GTable - 1u);
+  p2g_walk<false>(a, gmin, gmax, first, last, hist, false);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nBlocks; i += kP2GBinThreads) {
+    const uint32_t c = hist[i];
+    if (!c) continue;
+    if (SCATTER) hist[i] = atomicAdd(&a.cursors[i], c);
+    else atomicAdd(&a.counts[i], c);
+  }
+  if (!SCATTER) return;
+  __syncthreads();
+  p2g_walk<true>(a, gmin, gmax, first, last, hist, true);
+}
+
+// Step 2, one workgroup: exclusive scans of the counts (-> offsets) and of
+// the chunks per block (-> chunkStart, with the total at the end).
+__global__ __launch_bounds__(1024) void p2g_scan_kernel(P2GArgs a, uint32_t nBlocks) {
+  __shared__ uint32_t warpSums[2][16];
+  uint32_t carryP = 0, carryC = 0;
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  for (uint32_t base = 0; base < nBlocks; base += 1024u) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t c = i < nBlocks ? a.counts[i] : 0u;
+    const uint32_t k = (c + a.chunk - 1u) / a.chunk;
+    const uint32_t ip = wave_incl_scan(c), ik = wave_incl_scan(k);
+    if (lane == 63u) { warpSums[0][wid] = ip; warpSums[1][wid] = ik; }
+    __syncthreads();
+    uint32_t pp = 0, pk = 0, tp = 0, tk = 0;
+    for (uint32_t w = 0; w < 16u; w++) {
+      if (w < wid) { pp += warpSums[0][w]; pk += warpSums[1][w]; }
+      tp += warpSums[0][w];
+      tk += warpSums[1][w];
+    }
+    if (i < nBlocks) {
+      a.offsets[i] = carryP + pp + ip - c;
+      a.cursors[i] = carryP + pp + ip - c;  // the scatter's running slot
+      a.chunkStart[i] = carryC + pk + ik - k;
+    }
+    carryP += tp;
+    carryC += tk;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) a.chunkStart[nBlocks] = carryC;
+}
+
+// Step 4: one workgroup per chunk of a block's pairs.
+__global__ __launch_bounds__(256) void p2g_accum_kernel(P2GArgs a, uint32_t nBlocks) {
+  __shared__ int32_t acc[kBlkNodes][4];
+  __shared__ uint32_t s_blk;
+  const uint32_t total = a.chunkStart[nBlocks];
+  const f3 gmin = mk3(a.u->gridBoundsMin[0], a.u->gridBoundsMin[1], a.u->gridBoundsMin[2]);
+  const f3 gmax = mk3(a.u->gridBoundsMax[0], a.u->gridBoundsMax[1], a.u->gridBoundsMax[2]);
+  for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
+    if (threadIdx.x == 0) {
+      // last block whose first chunk is <= c
+      uint32_t lo = 0, hi = nBlocks;
+      while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.chunkStart[mid] <= c) lo = mid; else hi = mid;
+      }
+      s_blk = lo;
+    }
+    for (uint32_t i = threadIdx.x; i < kBlkNodes; i += blockDim.x) acc[i][0] = acc[i][1] = acc[i][2] = acc[i][3] = 0;
+    __syncthreads();
+    const uint32_t b = s_blk;
+    const uint32_t bx = b % a.nb, by = (b / a.nb) % a.nb, bz = b / (a.nb * a.nb);
+    const uint32_t x0 = bx * kBlk, y0 = by * kBlk, z0 = bz * kBlk;
+    const uint32_t first = a.offsets[b] + (c - a.chunkStart[b]) * a.chunk;
+    const uint32_t end = min(first + a.chunk, a.offsets[b] + a.counts[b]);
+    for (uint32_t q = first + threadIdx.x; q < end; q += blockDim.x) {
+      const uint32_t e = a.pairs[q];
+      const float4 pp = a.prev[e], pn = a.now[e];
+      const f3 p = mk3(pn.x, pn.y, pn.z);
+      const f3 vel = p - mk3(pp.x, pp.y, pp.z);
+      const Cell co = grid_cell(a.d, gmin, gmax, p);
+      for (uint32_t z = co.mn[2]; z <= co.mx[2]; z++)
+        for (uint32_t y = co.mn[1]; y <= co.mx[1]; y++)
+          for (uint32_t x = co.mn[0]; x <= co.mx[0]; x++) {
+            if (x - x0 >= kBlk || y - y0 >= kBlk || z - z0 >= kBlk) continue;  // another block's node
+            const f3 w = cell_w(x, y, z, co.pin);
+            const f3 cv = vel * w;
+            const int32_t v[4] = {f2i(cv.x * kGridMul), f2i(cv.y * kGridMul), f2i(cv.z * kGridMul),
+                                  f2i(length(w) * kGridMul)};
+            const uint32_t l = ((z - z0) * kBlk + (y - y0)) * kBlk + (x - x0);
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+              if (v[k]) atomicAdd(&acc[l][k], v[k]);
           }
-          if (!slot) slot = reinterpret_cast<int32_t*>(&dv[node]);
+    }
+    __syncthreads();
+    for (uint32_t l = threadIdx.x; l < kBlkNodes; l += blockDim.x) {
+      const uint32_t x = x0 + l % kBlk, y = y0 + (l / kBlk) % kBlk, z = z0 + l / (kBlk * kBlk);
+      if (x >= a.d || y >= a.d || z >= a.d) continue;
+      int32_t* cell = reinterpret_cast<int32_t*>(&a.dv[grid_idx(a.d, x, y, z)]);
 #pragma unroll
-          for (int i = 0; i < 4; i++)
-            if (c[i]) atomicAdd(slot + i, c[i]);
-        }
-  }
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < kP2GTable; i += kP2GThreads) {
-    const uint32_t node = keys[i];
-    if (node == kEmptyNode) continue;
-    int32_t* cell = reinterpret_cast<int32_t*>(&dv[node]);
-#pragma unroll
-    for (int c = 0; c < 4; c++)
-      if (vals[i][c]) atomicAdd(cell + c, vals[i][c]);
+      for (int k = 0; k < 4; k++)
+        if (acc[l][k]) atomicAdd(cell + k, acc[l][k]);
+    }
+    __syncthreads();
   }
 }
 
@@ -462,17 +609,56 @@ int fh_grid_post_sim(fh_ctx* ctx, uint32_t grid_dims, const fh_sim_uniforms* sim
   FH_REQUIRE(sim_uniforms && positions_prev && positions_now && density_velocity,
              "fh_grid_post_sim: null buffer");
   const uint64_t cells = (uint64_t)grid_dims * grid_dims * grid_dims;
-  int rc = fh::fill_u32(ctx, density_velocity, cells * 4, 0u);  // gridPostSimPass.ts:36
-  if (rc) return rc;
   const uint64_t n = (uint64_t)strands_count * points_per_strand;
-  FH_REQUIRE(n < (1ull << 31), "fh_grid_post_sim: too many points");
+  FH_REQUIRE(n < (1ull << 28), "fh_grid_post_sim: too many points");
+  P2GArgs a;
+  a.d = grid_dims;
+  a.nb = fh::div_ceil(grid_dims, kBlk);
+  a.u = sim_uniforms;
+  a.prev = reinterpret_cast<const float4*>(positions_prev);
+  a.now = reinterpret_cast<const float4*>(positions_now);
+  a.dv = density_velocity;
+  a.nPoints = (uint32_t)n;
+  // about two chunks per CU for the whole grid, more for hot blocks
+  a.chunk = std::min(8192u, std::max(512u, fh::div_ceil(fh::div_ceil((uint32_t)n, (uint32_t)std::max(1, ctx->num_cus) * 2u), 256u) * 256u));
+  const uint32_t nBlocks = a.nb * a.nb * a.nb;
+  // scratch: counts, cursors, offsets, chunk starts (+1), up to 8 pairs per point
+  const size_t words = (size_t)nBlocks * 4 + 1 + 8 * n;
+  int rc = fh::ensure_scratch(ctx, words * 4 + 256);
+  if (rc) return rc;
+  uint32_t* base = reinterpret_cast<uint32_t*>(ctx->scratch);
+  a.counts = base;
+  a.cursors = base + nBlocks;
+  a.offsets = base + 2 * (size_t)nBlocks;
+  a.chunkStart = base + 3 * (size_t)nBlocks;
+  a.pairs = base + 4 * (size_t)nBlocks + 1;
+  void* dst[2] = {density_velocity, base};
+  const uint64_t nw[2] = {cells * 4, nBlocks};  // the grid (gridPostSimPass.ts:36), block counts
+  if ((rc = fh::zero_many(ctx, 2, dst, nw, "p2g_clear"))) return rc;
   if (n == 0) return FH_OK;
-  const int pk2 = fh::prof_begin(ctx, "grid_post_sim");
-  grid_post_sim_kernel<<<fh::div_ceil((uint32_t)n, kP2GPoints), kP2GThreads, 0, ctx->stream>>>(
-      grid_dims, sim_uniforms, reinterpret_cast<const float4*>(positions_prev),
-      reinterpret_cast<const float4*>(positions_now), density_velocity, (uint32_t)n);
+  // points per binning workgroup: about 4 workgroups per CU, at least 1024
+  const uint32_t cus = (uint32_t)std::max(1, ctx->num_cus);
+  const uint32_t span = std::min(1u << 16, std::max(1024u, fh::div_ceil(fh::div_ceil((uint32_t)n, cus * 4u), 1024u) * 1024u));
+  const uint32_t pblocks = fh::div_ceil((uint32_t)n, span);
+  const bool lds = nBlocks <= kP2GHistMax;
+  const int pk2 = fh::prof_begin(ctx, "p2g_count");
+  if (lds) p2g_bin_kernel<false, true><<<pblocks, kP2GBinThreads, 0, ctx->stream>>>(a, nBlocks, span);
+  else p2g_bin_kernel<false, false><<<pblocks, kP2GBinThreads, 0, ctx->stream>>>(a, nBlocks, span);
   fh::prof_end(ctx, pk2);
-  return fh::check_launch("grid_post_sim_kernel");
+  if ((rc = fh::check_launch("p2g_bin_kernel"))) return rc;
+  const int pk3 = fh::prof_begin(ctx, "p2g_scan");
+  p2g_scan_kernel<<<1, 1024, 0, ctx->stream>>>(a, nBlocks);
+  fh::prof_end(ctx, pk3);
+  if ((rc = fh::check_launch("p2g_scan_kernel"))) return rc;
+  const int pk4 = fh::prof_begin(ctx, "p2g_scatter");
+  if (lds) p2g_bin_kernel<true, true><<<pblocks, kP2GBinThreads, 0, ctx->stream>>>(a, nBlocks, span);
+  else p2g_bin_kernel<true, false><<<pblocks, kP2GBinThreads, 0, ctx->stream>>>(a, nBlocks, span);
+  fh::prof_end(ctx, pk4);
+  if ((rc = fh::check_launch("p2g_bin_kernel"))) return rc;
+  const int pk5 = fh::prof_begin(ctx, "grid_post_sim");
+  p2g_accum_kernel<<<cus * 8u, 256, 0, ctx->stream>>>(a, nBlocks);
+  fh::prof_end(ctx, pk5);
+  return fh::check_launch("p2g_accum_kernel");
 }
 
 }  // extern "C"

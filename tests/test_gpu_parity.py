@@ -84,6 +84,26 @@ def test_grid_post_sim_bit_exact(gpu, oracle, kind, strands):
     assert np.array_equal(got, exp)
 
 
+@pytest.mark.parametrize("dims", [8, 200])
+def test_grid_post_sim_other_dims(gpu, oracle, dims):
+    """Grids of one block (8^3 nodes) and of more blocks than the LDS
+    histogram holds (25^3 blocks of 8^3) take the other binning paths."""
+    import torch
+    from frostbitten_hair_webgpu_amd import native as N
+    wl, r = _renderer("c2", 600)
+    _step_to(r, 2)
+    prev, now = _np(r.hair.positionsPrev), _np(r.hair.positionsNow)
+    dv = torch.full((dims ** 3 * 4,), 7, dtype=torch.int32, device=gpu)
+    r.ctx().native.call("fh_grid_post_sim", dims, N.ptr(r.simulationUniforms), N.ptr(r.hair.hairData),
+                        N.ptr(r.hair.positionsPrev), N.ptr(r.hair.positionsNow), N.ptr(dv),
+                        r.hair.strandsCount, r.hair.pointsPerStrand)
+    torch.cuda.synchronize()
+    exp = oracle.grid_post_sim(dims, r.sim_uniforms(r.frameIdx), prev, now, wl.strands, wl.points)
+    got = _np(dv).reshape(exp.shape)
+    assert got.any()
+    assert np.array_equal(got, exp)
+
+
 @pytest.mark.parametrize("kind,strands", [("small", 512), ("c2", 600)])
 def test_grid_pre_sim(gpu, oracle, kind, strands):
     import torch
