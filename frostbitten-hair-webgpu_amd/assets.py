@@ -134,8 +134,10 @@ def _head_collider(kind: str) -> Collider:
 
 
 def make_workload(kind: str, seed: int | None = None, strands: int | None = None,
-                  width: int | None = None, height: int | None = None) -> Workload:
-    """kind in {small, c2, c3}; optional overrides for tests."""
+                  width: int | None = None, height: int | None = None, points: int | None = None,
+                  collider: Collider | None = None, sdf: tuple | None = None) -> Workload:
+    """kind in {small, c2, c3}; optional overrides for tests and for the
+    batched and served instances (`collider` / `sdf` reuse a baked volume)."""
     specs = {
         "small": dict(k=1, S=2000, P=16, W=640, H=480, steps=30, sdf=32, grid=32, tile=16, bins=4, slices=4,
                       ps=4, col="sphere"),
@@ -148,7 +150,7 @@ def make_workload(kind: str, seed: int | None = None, strands: int | None = None
     seed = (0xF00D0000 + sp["k"]) if seed is None else seed
     rng = np.random.default_rng(seed)
     S = strands if strands is not None else sp["S"]
-    P = sp["P"]
+    P = points if points is not None else sp["P"]
     W = width if width is not None else sp["W"]
     H = height if height is not None else sp["H"]
     # roots
@@ -180,7 +182,7 @@ def make_workload(kind: str, seed: int | None = None, strands: int | None = None
         # period 2 s at 60 steps/s: fract(timer * f) with timer += 0.73 per step
         sim.windStrengthFrequency = 1.0 / (0.73 * 120.0)
         sim.windPhaseOffset = float(rng.uniform(0.0, 2 * math.pi))
-    collider = _head_collider(sp["col"])
+    collider = collider if collider is not None else _head_collider(sp["col"])
     # camera orbit around the head, seeded
     az = rng.uniform(-0.6, 0.6) + (0.0 if kind != "c3" else 0.3)
     el = rng.uniform(0.05, 0.35)
@@ -191,9 +193,62 @@ def make_workload(kind: str, seed: int | None = None, strands: int | None = None
     eye = target + dist * np.array([math.sin(az) * math.cos(el), math.sin(el), math.cos(az) * math.cos(el)])
     camera = L.Camera(eye=tuple(eye), target=tuple(target))
     wl = Workload(kind, seed, S, P, W, H, sp["steps"], render, sim, sp["sdf"], collider, camera, positions)
-    lo, hi, d = bake_sdf(collider, sp["sdf"])
+    lo, hi, d = sdf if sdf is not None else bake_sdf(collider, sp["sdf"])
     wl.sdf_bounds = (lo, hi)
     wl.sdf = d
+    return wl
+
+
+# ---------------------------------------------------------------------------
+# Instances of the batched and served forms
+# ---------------------------------------------------------------------------
+
+BATCH_SEED = 0xF00D0004
+SERVE_SEED = 0xF00D0005
+BATCH_RESOLUTIONS = [(512, 512), (1280, 720), (1920, 1080)]
+SERVE_RESOLUTIONS = [(1280, 720), (1920, 1080), (3840, 2160)]
+
+_shared_volume = {}
+
+
+def _union_volume(dims: int) -> tuple:
+    """The sphere + capsules collider and its baked SDF, baked once per size
+    (every instance gets its own device copy)."""
+    if dims not in _shared_volume:
+        col = _head_collider("union")
+        _shared_volume[dims] = (col, bake_sdf(col, dims))
+    return _shared_volume[dims]
+
+
+def make_instance(instance: int, base_seed: int = BATCH_SEED) -> Workload:
+    """Instance `instance` of the batched form: seed = base ^ instance;
+    strands log-uniform in [1k, 50k], points from {8, 16, 24, 32},
+    resolution from {512^2, 1280x720, 1920x1080}; scalp-cap hair with its
+    own camera and wind, a 64^3 SDF of the sphere + capsules collider and a
+    64^3 grid."""
+    rng = np.random.default_rng(base_seed ^ instance)
+    S = int(round(math.exp(rng.uniform(math.log(1000.0), math.log(50000.0)))))
+    P = int(rng.choice([8, 16, 24, 32]))
+    W, H = BATCH_RESOLUTIONS[int(rng.integers(len(BATCH_RESOLUTIONS)))]
+    col, vol = _union_volume(64)
+    wl = make_workload("c2", seed=int(rng.integers(1 << 31)), strands=S, width=W, height=H, points=P,
+                       collider=col, sdf=vol)
+    wl.name = f"batch{instance}"
+    return wl
+
+
+def make_pool_asset(asset: int, base_seed: int = SERVE_SEED) -> Workload:
+    """Asset `asset` of the served form's pool: strands log-uniform in
+    [1k, 100k], points uniform in [16, 32], resolution from {720p, 1080p,
+    4K}; otherwise as the batched instances."""
+    rng = np.random.default_rng(base_seed ^ asset)
+    S = int(round(math.exp(rng.uniform(math.log(1000.0), math.log(100000.0)))))
+    P = int(rng.integers(16, 33))
+    W, H = SERVE_RESOLUTIONS[int(rng.integers(len(SERVE_RESOLUTIONS)))]
+    col, vol = _union_volume(64)
+    wl = make_workload("c2", seed=int(rng.integers(1 << 31)), strands=S, width=W, height=H, points=P,
+                       collider=col, sdf=vol)
+    wl.name = f"asset{asset}"
     return wl
 
 
@@ -252,9 +307,20 @@ def initial_shading(S: int, PS: int) -> np.ndarray:
 # ---------------------------------------------------------------------------
 
 def _sphere_trace(col: Collider, origins: np.ndarray, dirs: np.ndarray, t_max: float, iters: int = 96) -> tuple:
-    t = np.zeros(origins.shape[0])
+    """Sphere tracing of the collider union. Rays start where they enter the
+    bounding sphere of the collider and stop where they leave it, so rays
+    that miss it take no steps."""
+    lo, hi = col.bounds(0.0)
+    c, R = 0.5 * (lo + hi), 0.5 * float(np.linalg.norm(hi - lo)) + 1e-3
+    oc = origins - c
+    b = np.einsum("ij,ij->i", oc, dirs)
+    disc = b * b - (np.einsum("ij,ij->i", oc, oc) - R * R)
+    root = np.sqrt(np.maximum(disc, 0.0))
+    t_in, t_out = -b - root, -b + root
+    t = np.maximum(t_in, 0.0)
+    t_end = np.minimum(t_out, t_max)
     hit = np.zeros(origins.shape[0], dtype=bool)
-    active = np.ones(origins.shape[0], dtype=bool)
+    active = (disc > 0.0) & (t_out > 0.0) & (t < t_end)
     for _ in range(iters):
         idx = np.nonzero(active)[0]
         if idx.size == 0:
@@ -265,7 +331,7 @@ def _sphere_trace(col: Collider, origins: np.ndarray, dirs: np.ndarray, t_max: f
         hit[idx[h]] = True
         active[idx[h]] = False
         t[idx] += np.where(h, 0.0, d)
-        far = t[idx] > t_max
+        far = t[idx] > t_end[idx]
         active[idx[far]] = False
     return t, hit
 

@@ -1,33 +1,49 @@
 // Context, memory and error handling of the C-ABI (include/fh_hair.h).
 #include <algorithm>
 #include <cstring>
+#include <memory>
 
-#include "fh_internal.hpp"
+#include "fh_launch.hpp"
 
 namespace {
 thread_local std::string g_last_error;
 
 // GPUCommandEncoder.clearBuffer as a kernel: graph-capture safe, unlike a
 // memset node.
-__global__ void fill_u32_kernel(uint32_t* __restrict__ dst, uint64_t n, uint32_t value) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) dst[i] = value;
-}
-
-// Several clears in one launch: blockIdx.y picks the buffer.
-struct FillJobs {
-  uint32_t* dst[fh::kMaxFillJobs];
-  uint64_t n[fh::kMaxFillJobs];
+struct FillU32 {
+  struct Args {
+    uint32_t* dst;
+    uint64_t n;
+    uint32_t value;
+  };
+  static constexpr uint32_t kThreads = 256;
+  static constexpr const char* kName = "fill_u32";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) {
+    uint64_t i = (uint64_t)block * kThreads + threadIdx.x;
+    const uint64_t stride = (uint64_t)blocks * kThreads;
+    for (; i < a.n; i += stride) a.dst[i] = a.value;
+  }
 };
 
-__global__ void fill_many_kernel(FillJobs jobs) {
-  uint32_t* __restrict__ dst = jobs.dst[blockIdx.y];
-  const uint64_t n = jobs.n[blockIdx.y];
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) dst[i] = 0u;
-}
+// Several clears in one launch: `per` consecutive blocks per buffer.
+struct FillMany {
+  struct Args {
+    uint32_t* dst[fh::kMaxFillJobs];
+    uint64_t n[fh::kMaxFillJobs];
+    uint32_t per;
+  };
+  static constexpr uint32_t kThreads = 256;
+  static constexpr const char* kName = "fill_many";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) {
+    (void)blocks;
+    const uint32_t j = block / a.per;
+    uint32_t* __restrict__ dst = a.dst[j];
+    const uint64_t n = a.n[j];
+    uint64_t i = (uint64_t)(block - j * a.per) * kThreads + threadIdx.x;
+    const uint64_t stride = (uint64_t)a.per * kThreads;
+    for (; i < n; i += stride) dst[i] = 0u;
+  }
+};
 }  // namespace
 
 namespace fh {
@@ -49,6 +65,9 @@ int ensure_scratch(fh_ctx* ctx, size_t bytes) {
   if (ctx->scratch_bytes >= bytes) return FH_OK;
   if (ctx->capturing)
     return fail(FH_ERR_UNSUPPORTED, "scratch would grow while capturing: run the calls once before capture");
+  // launches recorded so far keep the old scratch address: fh_record_end
+  // discards such a recording
+  if (ctx->recording) ctx->rec_grew = true;
   if (ctx->scratch) {
     FH_HIP(hipStreamSynchronize(ctx->stream));
     FH_HIP(hipFree(ctx->scratch));
@@ -137,7 +156,7 @@ int add_timing_nodes(fh_graph* g, const std::vector<const char*>& names) {
 }
 
 int zero_many(fh_ctx* ctx, int count, void* const* dst, const uint64_t* n_words, const char* kernel_name) {
-  FillJobs jobs;
+  FillMany::Args jobs{};
   uint64_t most = 0;
   int k = 0;
   for (int i = 0; i < count && k < kMaxFillJobs; i++) {
@@ -148,22 +167,26 @@ int zero_many(fh_ctx* ctx, int count, void* const* dst, const uint64_t* n_words,
     k++;
   }
   if (k == 0) return FH_OK;
-  uint64_t blocks = (most + 255) / 256;
-  if (blocks > 2048) blocks = 2048;
-  const int pk = fh::prof_begin(ctx, kernel_name);
-  fill_many_kernel<<<dim3((unsigned)blocks, (unsigned)k), 256, 0, ctx->stream>>>(jobs);
-  fh::prof_end(ctx, pk);
-  return check_launch("fill_many_kernel");
+  jobs.per = (uint32_t)std::min<uint64_t>((most + 255) / 256, 2048);
+  return launch<FillMany>(ctx, jobs.per * k, 0, jobs, kernel_name);
 }
 
 int fill_u32(fh_ctx* ctx, void* dst, uint64_t n_words, uint32_t value) {
   if (n_words == 0) return FH_OK;
-  uint64_t blocks = (n_words + 255) / 256;
-  if (blocks > 4096) blocks = 4096;
-  const int pk0 = fh::prof_begin(ctx, "fill_u32");
-  fill_u32_kernel<<<(unsigned)blocks, 256, 0, ctx->stream>>>((uint32_t*)dst, n_words, value);
-  fh::prof_end(ctx, pk0);
-  return check_launch("fill_u32_kernel");
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((n_words + 255) / 256, 4096);
+  return launch<FillU32>(ctx, blocks, 0, FillU32::Args{(uint32_t*)dst, n_words, value});
+}
+
+int record_launch(fh_ctx* ctx, const char* name, BatchLauncher fn, uint32_t blocks, uint32_t shmem,
+                  const void* args, size_t bytes) {
+  RecLaunch r;
+  r.name = name;
+  r.fn = fn;
+  r.blocks = blocks;
+  r.shmem = shmem;
+  r.args.assign(static_cast<const unsigned char*>(args), static_cast<const unsigned char*>(args) + bytes);
+  ctx->rec.push_back(std::move(r));
+  return FH_OK;
 }
 
 }  // namespace fh
@@ -261,6 +284,129 @@ int fh_profiler_read(fh_ctx* ctx, fh_kernel_time* out, uint32_t max_out, uint32_
   }
   ctx->prof_names.clear();
   *n_out = n;
+  return FH_OK;
+}
+
+int fh_record_begin(fh_ctx* ctx) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(!ctx->recording && !ctx->capturing, "fh_record_begin: already recording or capturing");
+  ctx->recording = true;
+  ctx->rec_grew = false;
+  ctx->rec.clear();
+  return FH_OK;
+}
+
+int fh_record_end(fh_ctx* ctx, fh_program** out) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(ctx->recording, "fh_record_end: not recording");
+  FH_REQUIRE(out != nullptr, "fh_record_end: out is null");
+  ctx->recording = false;
+  *out = nullptr;
+  if (ctx->rec_grew) {
+    ctx->rec.clear();
+    return fh::fail(FH_ERR_CAPACITY, "fh_record_end: the context's scratch grew while recording; record again");
+  }
+  fh_program* p = new fh_program();
+  p->rec.swap(ctx->rec);
+  p->instances = 1;
+  *out = p;
+  return FH_OK;
+}
+
+int fh_program_merge(fh_ctx* ctx, const fh_program* const* programs, uint32_t count, fh_program** out) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(programs && out && count > 0, "fh_program_merge: null argument");
+  for (uint32_t i = 0; i < count; i++)
+    FH_REQUIRE(programs[i] && !programs[i]->merged, "fh_program_merge: inputs must be recorded programs");
+  // Greedy common supersequence of the launch sequences: among the kernels
+  // at the heads, take the one whose instances have the most launches left
+  // (ties: the kernel more heads share), launch it for all those instances.
+  std::vector<size_t> at(count, 0);
+  std::vector<unsigned char> blob;
+  auto align = [&](size_t a) { blob.resize(fh::align_up(blob.size(), a)); };
+  std::unique_ptr<fh_program> m(new fh_program());
+  m->merged = true;
+  m->instances = count;
+  while (true) {
+    fh::BatchLauncher best = nullptr;
+    size_t bestLeft = 0, bestShare = 0;
+    for (uint32_t i = 0; i < count; i++) {
+      if (at[i] >= programs[i]->rec.size()) continue;
+      const fh::BatchLauncher fn = programs[i]->rec[at[i]].fn;
+      size_t left = 0, share = 0;
+      for (uint32_t j = 0; j < count; j++) {
+        if (at[j] >= programs[j]->rec.size() || programs[j]->rec[at[j]].fn != fn) continue;
+        left = std::max(left, programs[j]->rec.size() - at[j]);
+        share++;
+      }
+      if (!best || left > bestLeft || (left == bestLeft && share > bestShare)) {
+        best = fn;
+        bestLeft = left;
+        bestShare = share;
+      }
+    }
+    if (!best) break;
+    fh_program::Step st{};
+    std::vector<uint32_t> starts(1, 0u);
+    std::vector<const fh::RecLaunch*> members;
+    uint64_t total = 0;
+    for (uint32_t j = 0; j < count; j++) {
+      if (at[j] >= programs[j]->rec.size() || programs[j]->rec[at[j]].fn != best) continue;
+      const fh::RecLaunch& r = programs[j]->rec[at[j]++];
+      if (!members.empty() && r.args.size() != members[0]->args.size())
+        return fh::fail(FH_ERR_INVALID_ARG, std::string("fh_program_merge: argument sizes differ for ") + r.name);
+      members.push_back(&r);
+      total += r.blocks;
+      st.shmem = std::max(st.shmem, r.shmem);
+      starts.push_back((uint32_t)total);
+    }
+    if (total >= (1ull << 31)) return fh::fail(FH_ERR_INVALID_ARG, "fh_program_merge: too many blocks in one launch");
+    st.name = members[0]->name;
+    st.fn = best;
+    st.nInst = (uint32_t)members.size();
+    st.totalBlocks = (uint32_t)total;
+    align(256);
+    st.argsOff = blob.size();
+    for (const fh::RecLaunch* r : members) blob.insert(blob.end(), r->args.begin(), r->args.end());
+    align(256);
+    st.startsOff = blob.size();
+    const unsigned char* sb = reinterpret_cast<const unsigned char*>(starts.data());
+    blob.insert(blob.end(), sb, sb + starts.size() * sizeof(uint32_t));
+    m->steps.push_back(st);
+  }
+  if (!blob.empty()) {
+    FH_HIP(hipMalloc(&m->dmem, blob.size()));
+    FH_HIP(hipMemcpy(m->dmem, blob.data(), blob.size(), hipMemcpyHostToDevice));
+  }
+  *out = m.release();
+  return FH_OK;
+}
+
+int fh_program_launches(const fh_program* program, uint32_t* n_out) {
+  FH_REQUIRE(program && n_out, "fh_program_launches: null argument");
+  *n_out = (uint32_t)(program->merged ? program->steps.size() : program->rec.size());
+  return FH_OK;
+}
+
+int fh_program_launch(fh_ctx* ctx, const fh_program* program) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(program && program->merged, "fh_program_launch: not a merged program");
+  FH_REQUIRE(!ctx->recording, "fh_program_launch: the context is recording");
+  for (const fh_program::Step& st : program->steps) {
+    const char* base = static_cast<const char*>(program->dmem);
+    const int pk = fh::prof_begin(ctx, st.name);
+    const int rc = st.fn(ctx->stream, base + st.argsOff, reinterpret_cast<const uint32_t*>(base + st.startsOff),
+                         st.nInst, st.totalBlocks, st.shmem);
+    fh::prof_end(ctx, pk);
+    if (rc) return rc;
+  }
+  return FH_OK;
+}
+
+int fh_program_destroy(fh_program* program) {
+  if (!program) return FH_OK;
+  if (program->dmem) FH_HIP(hipFree(program->dmem));
+  delete program;
   return FH_OK;
 }
 

@@ -9,6 +9,20 @@
 
 #include "../../include/fh_hair.h"
 
+namespace fh {
+// Launches a kernel for many instances at once (fh_launch.hpp): instance i
+// owns blocks [dStarts[i], dStarts[i + 1]) and reads the i-th argument struct.
+using BatchLauncher = int (*)(hipStream_t s, const void* dArgs, const uint32_t* dStarts, uint32_t nInst,
+                              uint32_t totalBlocks, uint32_t shmem);
+// One launch kept by a recording context instead of being issued.
+struct RecLaunch {
+  const char* name;
+  BatchLauncher fn;  // also names the kernel: equal for launches of one kernel
+  uint32_t blocks, shmem;
+  std::vector<unsigned char> args;
+};
+}  // namespace fh
+
 struct fh_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -27,6 +41,26 @@ struct fh_ctx {
   // Kernels launched while capturing with timing on, in launch order: the
   // graph gets event-record nodes around them, rebound on every replay.
   std::vector<const char*> cap_names;
+  // Between fh_record_begin and _end the launches are kept here, not issued.
+  bool recording = false;
+  bool rec_grew = false;  // the scratch was reallocated while recording
+  std::vector<fh::RecLaunch> rec;
+};
+
+// A recorded launch sequence of one context, or several merged into batched
+// launches whose arguments and block ranges live in device memory.
+struct fh_program {
+  std::vector<fh::RecLaunch> rec;
+  struct Step {
+    const char* name;
+    fh::BatchLauncher fn;
+    uint32_t nInst, totalBlocks, shmem;
+    size_t argsOff, startsOff;  // byte offsets into dmem
+  };
+  std::vector<Step> steps;
+  void* dmem = nullptr;
+  uint32_t instances = 0;
+  bool merged = false;
 };
 
 struct fh_graph {

@@ -1,7 +1,7 @@
 // Software hair rasterizer on gfx950: tile pass (K4), tile sort (K5/K6) and
 // the fine OIT pass (K7), plus the raster-helper test kernel of the reference.
 // Reference: src/passes/swHair/*.
-#include "fh_internal.hpp"
+#include "fh_launch.hpp"
 #include "fh_raster_common.hpp"
 
 namespace fh {
@@ -484,17 +484,25 @@ __device__ __forceinline__ void tiles_wave(const TilesArgs& a, TileWaveLds& L, u
   }
 }
 
-__global__ __launch_bounds__(64) void tiles_draw_kernel(TilesArgs a) {
+__device__ __forceinline__ void tiles_draw_body(TilesArgs a, uint32_t block, uint32_t blocks) {
   __shared__ TileWaveLds L;
-  const uint32_t gid = blockIdx.x * 64u + threadIdx.x;
+  const uint32_t gid = block * 64u + threadIdx.x;
   tiles_wave(a, L, gid, gid < a.S * (a.P - 1u), 0u, true);
 }
 
+struct TilesDraw {
+  using Args = TilesArgs;
+  static constexpr uint32_t kThreads = 64;
+  static constexpr const char* kName = "tiles_draw";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { tiles_draw_body(a, block, blocks); }
+};
+
+
 // The queued pieces, 64 per wave; every workgroup exits after the last one.
-__global__ __launch_bounds__(64) void tiles_pieces_kernel(TilesArgs a) {
+__device__ __forceinline__ void tiles_pieces_body(TilesArgs a, uint32_t block, uint32_t blocks) {
   __shared__ TileWaveLds L;
   const uint32_t count = min(*a.pieceCount, a.pieceCap);
-  for (uint32_t base = blockIdx.x * 64u; base < count; base += gridDim.x * 64u) {
+  for (uint32_t base = block * 64u; base < count; base += blocks * 64u) {
     const uint32_t i = base + threadIdx.x;
     const uint2 pc = i < count ? a.pieces[i] : make_uint2(kInvalidPtr, 0u);
     tiles_wave(a, L, pc.x, pc.x != kInvalidPtr, pc.y, false);
@@ -502,27 +510,48 @@ __global__ __launch_bounds__(64) void tiles_pieces_kernel(TilesArgs a) {
   }
 }
 
+struct TilesPieces {
+  using Args = TilesArgs;
+  static constexpr uint32_t kThreads = 64;
+  static constexpr const char* kName = "tiles_pieces";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { tiles_pieces_body(a, block, blocks); }
+};
+
+
 // hairSegmentCountPerTile (processTile :236): the per-bin counts of each tile
 // summed, which is what the per-append increments of the reference add up to.
-__global__ __launch_bounds__(256) void tile_counts_kernel(const fh_tile_result* __restrict__ tiles,
+__device__ __forceinline__ void tile_counts_body(const fh_tile_result* __restrict__ tiles,
                                                           uint32_t nTiles, uint32_t B,
-                                                          uint32_t* __restrict__ countPerTile) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+                                                          uint32_t* __restrict__ countPerTile, uint32_t block, uint32_t blocks) {
+  const uint32_t t = block * blockDim.x + threadIdx.x;
   if (t >= nTiles) return;
   uint32_t n = 0;
   for (uint32_t b = 0; b < B; b++) n += tiles[(size_t)t * B + b].padding0;
   countPerTile[t] = n;
 }
 
+struct TileCounts {
+  struct Args {
+    const fh_tile_result* tiles;
+    uint32_t nTiles;
+    uint32_t B;
+    uint32_t* countPerTile;
+  };
+  static constexpr uint32_t kThreads = 256;
+  static constexpr const char* kName = "tile_counts";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { tile_counts_body(a.tiles, a.nTiles, a.B, a.countPerTile, block, blocks); }
+};
+
+
 // ---------------------------------------------------------------------------
 // For lists written by another tile-pass implementation: one thread per
 // (tile, bin) walks its list, tags each slot and records the list length.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void rebuild_keys_kernel(fh_tile_result* __restrict__ tiles,
+__device__ __forceinline__ void rebuild_keys_body(fh_tile_result* __restrict__ tiles,
                                                            const fh_list_element* __restrict__ data,
                                                            uint32_t limit, uint32_t nTB,
-                                                           uint32_t* __restrict__ slotKey) {
-  const uint32_t tb = blockIdx.x * blockDim.x + threadIdx.x;
+                                                           uint32_t* __restrict__ slotKey, uint32_t block, uint32_t blocks) {
+  const uint32_t tb = block * blockDim.x + threadIdx.x;
   if (tb >= nTB) return;
   uint32_t ptr = tiles[tb].tileSegmentPtr;
   ptr = ptr == 0u ? kInvalidPtr : ptr - 1u;
@@ -536,18 +565,32 @@ __global__ __launch_bounds__(256) void rebuild_keys_kernel(fh_tile_result* __res
   tiles[tb].padding0 = n;
 }
 
+struct RebuildKeys {
+  struct Args {
+    fh_tile_result* tiles;
+    const fh_list_element* data;
+    uint32_t limit;
+    uint32_t nTB;
+    uint32_t* slotKey;
+  };
+  static constexpr uint32_t kThreads = 256;
+  static constexpr const char* kName = "rebuild_keys";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { rebuild_keys_body(a.tiles, a.data, a.limit, a.nTB, a.slotKey, block, blocks); }
+};
+
+
 // ---------------------------------------------------------------------------
 // K5 — hairTileSortPass.countTiles.wgsl.ts:63-97, with the bucket histogram
 // aggregated in LDS first (one global atomic per bucket per workgroup).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void sort_count_kernel(const uint32_t* __restrict__ counts,
+__device__ __forceinline__ void sort_count_body(const uint32_t* __restrict__ counts,
                                                          uint32_t nTiles, uint32_t nBuckets,
                                                          uint32_t bucketSize,
-                                                         fh_sort_bucket* __restrict__ buckets) {
+                                                         fh_sort_bucket* __restrict__ buckets, uint32_t block, uint32_t blocks) {
   __shared__ uint32_t hist[64];
   if (threadIdx.x < 64) hist[threadIdx.x] = 0;
   __syncthreads();
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t t = block * blockDim.x + threadIdx.x;
   if (t < nTiles) {
     const uint32_t c = counts[t];
     if (c > 0u) atomicAdd(&hist[min(c / bucketSize, nBuckets - 1u)], 1u);
@@ -556,13 +599,27 @@ __global__ __launch_bounds__(256) void sort_count_kernel(const uint32_t* __restr
   if (threadIdx.x < nBuckets && hist[threadIdx.x]) atomicAdd(&buckets[threadIdx.x].tileCount, hist[threadIdx.x]);
 }
 
+struct SortCount {
+  struct Args {
+    const uint32_t* counts;
+    uint32_t nTiles;
+    uint32_t nBuckets;
+    uint32_t bucketSize;
+    fh_sort_bucket* buckets;
+  };
+  static constexpr uint32_t kThreads = 256;
+  static constexpr const char* kName = "sort_count";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { sort_count_body(a.counts, a.nTiles, a.nBuckets, a.bucketSize, a.buckets, block, blocks); }
+};
+
+
 // K6 — hairTileSortPass.sort.wgsl.ts:37-75: heavier buckets first; each
 // workgroup reserves its in-bucket range with one atomic per bucket.
-__global__ __launch_bounds__(256) void sort_scatter_kernel(const uint32_t* __restrict__ counts,
+__device__ __forceinline__ void sort_scatter_body(const uint32_t* __restrict__ counts,
                                                            uint32_t nTiles, uint32_t nBuckets,
                                                            uint32_t bucketSize,
                                                            fh_sort_bucket* __restrict__ buckets,
-                                                           uint32_t* __restrict__ tileList) {
+                                                           uint32_t* __restrict__ tileList, uint32_t block, uint32_t blocks) {
   __shared__ uint32_t bucketOffset[64];
   __shared__ uint32_t localCount[64];
   __shared__ uint32_t localBase[64];
@@ -579,7 +636,7 @@ __global__ __launch_bounds__(256) void sort_scatter_kernel(const uint32_t* __res
   }
   if (threadIdx.x < 64) localCount[threadIdx.x] = 0;
   __syncthreads();
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t t = block * blockDim.x + threadIdx.x;
   uint32_t b = 0, rank = 0;
   const bool has = t < nTiles && counts[t] > 0u;
   if (has) {
@@ -598,6 +655,21 @@ __global__ __launch_bounds__(256) void sort_scatter_kernel(const uint32_t* __res
     if (o < nTiles) tileList[1 + o] = t;
   }
 }
+
+struct SortScatter {
+  struct Args {
+    const uint32_t* counts;
+    uint32_t nTiles;
+    uint32_t nBuckets;
+    uint32_t bucketSize;
+    fh_sort_bucket* buckets;
+    uint32_t* tileList;
+  };
+  static constexpr uint32_t kThreads = 256;
+  static constexpr const char* kName = "sort_scatter";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { sort_scatter_body(a.counts, a.nTiles, a.nBuckets, a.bucketSize, a.buckets, a.tileList, block, blocks); }
+};
+
 
 // ---------------------------------------------------------------------------
 // Fine-pass gather: the per-(tile, bin) linked lists become one contiguous
@@ -632,41 +704,66 @@ __device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t v, uint32_t* w
   return v;
 }
 
-__global__ __launch_bounds__(kScanBlock) void bin_scan_local_kernel(const fh_tile_result* __restrict__ tiles,
+__device__ __forceinline__ void bin_scan_local_body(const fh_tile_result* __restrict__ tiles,
                                                                     uint32_t n, uint32_t* __restrict__ ends,
-                                                                    uint32_t* __restrict__ blockSums) {
+                                                                    uint32_t* __restrict__ blockSums, uint32_t block, uint32_t blocks) {
   __shared__ uint32_t warpSums[16];
-  const uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
+  const uint32_t i = block * kScanBlock + threadIdx.x;
   const uint32_t v = i < n ? tiles[i].padding0 : 0u;
   const uint32_t s = block_inclusive_scan(v, warpSums);
   if (i < n) ends[i] = s - v;  // exclusive: first element of the range
-  if (threadIdx.x == kScanBlock - 1) blockSums[blockIdx.x] = s;
+  if (threadIdx.x == kScanBlock - 1) blockSums[block] = s;
 }
 
-__global__ __launch_bounds__(kScanBlock) void bin_scan_add_kernel(uint32_t n, uint32_t* __restrict__ ends,
-                                                                  const uint32_t* __restrict__ blockSums) {
+struct BinScanLocal {
+  struct Args {
+    const fh_tile_result* tiles;
+    uint32_t n;
+    uint32_t* ends;
+    uint32_t* blockSums;
+  };
+  static constexpr uint32_t kThreads = kScanBlock;
+  static constexpr const char* kName = "bin_scan_local";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { bin_scan_local_body(a.tiles, a.n, a.ends, a.blockSums, block, blocks); }
+};
+
+
+__device__ __forceinline__ void bin_scan_add_body(uint32_t n, uint32_t* __restrict__ ends,
+                                                                  const uint32_t* __restrict__ blockSums, uint32_t block, uint32_t blocks) {
   __shared__ uint32_t red[16];
   // prefix of this block = sum of the sums of all earlier blocks
   uint32_t acc = 0;
-  for (uint32_t j = threadIdx.x; j < blockIdx.x; j += kScanBlock) acc += blockSums[j];
+  for (uint32_t j = threadIdx.x; j < block; j += kScanBlock) acc += blockSums[j];
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) acc += (uint32_t)__shfl_xor((int)acc, o);
   if ((threadIdx.x & 63u) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
   uint32_t prefix = 0;
   for (uint32_t w = 0; w < kScanBlock / 64; w++) prefix += red[w];
-  const uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
+  const uint32_t i = block * kScanBlock + threadIdx.x;
   if (i < n && prefix) ends[i] += prefix;
 }
 
-__global__ __launch_bounds__(256) void bin_scatter_kernel(const fh_render_uniforms* __restrict__ u,
+struct BinScanAdd {
+  struct Args {
+    uint32_t n;
+    uint32_t* ends;
+    const uint32_t* blockSums;
+  };
+  static constexpr uint32_t kThreads = kScanBlock;
+  static constexpr const char* kName = "bin_scan_add";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { bin_scan_add_body(a.n, a.ends, a.blockSums, block, blocks); }
+};
+
+
+__device__ __forceinline__ void bin_scatter_body(const fh_render_uniforms* __restrict__ u,
                                                           const uint32_t* __restrict__ segCounter,
                                                           const fh_list_element* __restrict__ data,
                                                           uint32_t capacity, const uint2* __restrict__ slotTag,
                                                           const uint32_t* __restrict__ starts, uint32_t nTB,
-                                                          uint32_t* __restrict__ binSeg) {
+                                                          uint32_t* __restrict__ binSeg, uint32_t block, uint32_t blocks) {
   const uint32_t n = min(*segCounter, min(capacity, u->maxDrawnHairSegments));
-  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+  for (uint32_t s = block * blockDim.x + threadIdx.x; s < n; s += blocks * blockDim.x) {
     const uint2 tag = slotTag[s];  // (tile-bin, rank in its list)
     if (tag.x >= nTB) continue;    // untagged slot: not linked into any list
     const uint32_t pos = starts[tag.x] + tag.y;
@@ -676,6 +773,23 @@ __global__ __launch_bounds__(256) void bin_scatter_kernel(const fh_render_unifor
     }
   }
 }
+
+struct BinScatter {
+  struct Args {
+    const fh_render_uniforms* u;
+    const uint32_t* segCounter;
+    const fh_list_element* data;
+    uint32_t capacity;
+    const uint2* slotTag;
+    const uint32_t* starts;
+    uint32_t nTB;
+    uint32_t* binSeg;
+  };
+  static constexpr uint32_t kThreads = 256;
+  static constexpr const char* kName = "bin_scatter";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { bin_scatter_body(a.u, a.segCounter, a.data, a.capacity, a.slotTag, a.starts, a.nTB, a.binSeg, block, blocks); }
+};
+
 
 // ---------------------------------------------------------------------------
 // Each gathered range sorted by blend key (strand << 8 | segment), largest
@@ -727,17 +841,17 @@ __device__ __forceinline__ uint32_t wave_sort_desc(uint32_t v) {
 // sorted one per wave in registers; larger ones are appended to `large`
 // (count in large[0]) for bin_sort_large_kernel, so that the few heavy
 // ranges, which cluster on screen, spread over many workgroups.
-__global__ __launch_bounds__(256) void bin_sort_small_kernel(const fh_tile_result* __restrict__ tiles,
+__device__ __forceinline__ void bin_sort_small_body(const fh_tile_result* __restrict__ tiles,
                                                              const uint32_t* __restrict__ starts, uint32_t nTB,
                                                              uint32_t capacity, uint32_t* __restrict__ keys,
-                                                             uint32_t* __restrict__ large) {
+                                                             uint32_t* __restrict__ large, uint32_t block, uint32_t blocks) {
   __shared__ uint32_t smallList[256];
   __shared__ uint32_t sStart[256], sCount[256];
   __shared__ uint32_t nSmall;
   const uint32_t tid = threadIdx.x;
   if (tid == 0) nSmall = 0;
   __syncthreads();
-  const uint32_t tb = blockIdx.x * 256u + tid;
+  const uint32_t tb = block * 256u + tid;
   uint32_t start = 0, n = 0;
   if (tb < nTB) {
     start = min(starts[tb], capacity);
@@ -758,15 +872,30 @@ __global__ __launch_bounds__(256) void bin_sort_small_kernel(const fh_tile_resul
   }
 }
 
+struct BinSortSmall {
+  struct Args {
+    const fh_tile_result* tiles;
+    const uint32_t* starts;
+    uint32_t nTB;
+    uint32_t capacity;
+    uint32_t* keys;
+    uint32_t* large;
+  };
+  static constexpr uint32_t kThreads = 256;
+  static constexpr const char* kName = "bin_sort_small";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { bin_sort_small_body(a.tiles, a.starts, a.nTB, a.capacity, a.keys, a.large, block, blocks); }
+};
+
+
 // One workgroup per listed range (more than 64 keys).
-__global__ __launch_bounds__(256) void bin_sort_large_kernel(const fh_tile_result* __restrict__ tiles,
+__device__ __forceinline__ void bin_sort_large_body(const fh_tile_result* __restrict__ tiles,
                                                              const uint32_t* __restrict__ starts,
                                                              uint32_t capacity, uint32_t* __restrict__ keys,
                                                              const uint32_t* __restrict__ large,
-                                                             uint32_t* __restrict__ big) {
+                                                             uint32_t* __restrict__ big, uint32_t block, uint32_t blocks) {
   __shared__ uint32_t sk[kSortLds];
   const uint32_t count = large[0];
-  for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
+  for (uint32_t i = block; i < count; i += blocks) {
     const uint32_t tb = large[1u + i];
     const uint32_t st = min(starts[tb], capacity);
     const uint32_t cnt = min(tiles[tb].padding0, capacity - st);
@@ -780,6 +909,21 @@ __global__ __launch_bounds__(256) void bin_sort_large_kernel(const fh_tile_resul
     for (uint32_t e = threadIdx.x; e < cnt; e += blockDim.x) keys[st + e] = k[e];
   }
 }
+
+struct BinSortLarge {
+  struct Args {
+    const fh_tile_result* tiles;
+    const uint32_t* starts;
+    uint32_t capacity;
+    uint32_t* keys;
+    const uint32_t* large;
+    uint32_t* big;
+  };
+  static constexpr uint32_t kThreads = 256;
+  static constexpr const char* kName = "bin_sort_large";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { bin_sort_large_body(a.tiles, a.starts, a.capacity, a.keys, a.large, a.big, block, blocks); }
+};
+
 
 // ---------------------------------------------------------------------------
 // K7 — hairFinePass.wgsl.ts:126-335 with processHairSegment.wgsl.ts:5-70 and
@@ -1038,7 +1182,7 @@ __device__ __forceinline__ void fine_cheap_batch(const FineArgs& a, const FineLd
 }
 
 template <int TILE>
-__global__ __launch_bounds__(fine_waves<TILE>() * 64) void fine_raster_kernel(FineArgs a) {
+__device__ __forceinline__ void fine_raster_body(FineArgs a, uint32_t block, uint32_t blocks) {
   constexpr uint32_t NPX = TILE * TILE;
   constexpr uint32_t NW = fine_waves<TILE>();
   constexpr uint32_t RB = fine_batch<TILE>();
@@ -1069,7 +1213,7 @@ __global__ __launch_bounds__(fine_waves<TILE>() * 64) void fine_raster_kernel(Fi
   const uint32_t dbgMax = dispMode == 4u ? ((u->displayMode >> 8) & 0xffffu) : 0u;  // USED_SLICES
   const bool isDbg = dbgMax != 0u;
   const float invSegs = (float)(a.P - 1u);
-  uint4* gpool = a.slicePool + (size_t)blockIdx.x * a.perProc;
+  uint4* gpool = a.slicePool + (size_t)block * a.perProc;
 
   for (uint32_t i = tid; i < a.NS * NPX; i += blockDim.x) L.heads[i] = kInvalidPtr;
   if (pixThread) L.claim[tid] = 64u;
@@ -1188,12 +1332,20 @@ __global__ __launch_bounds__(fine_waves<TILE>() * 64) void fine_raster_kernel(Fi
   }
 }
 
+template <uint32_t TILE>
+struct FineRaster {
+  using Args = FineArgs;
+  static constexpr uint32_t kThreads = fine_waves<TILE>() * 64;
+  static constexpr const char* kName = "fine_raster";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { fine_raster_body<TILE>(a, block, blocks); }
+};
+
 // swRasterizer.test.ts:184-248 — the reference's raster-helper test kernel.
-__global__ __launch_bounds__(64) void debug_raster_kernel(const fh_render_uniforms* __restrict__ u,
+__device__ __forceinline__ void debug_raster_body(const fh_render_uniforms* __restrict__ u,
                                                           const float4* __restrict__ pos,
                                                           const float4* __restrict__ tan, uint32_t S,
-                                                          uint32_t P, uint32_t* __restrict__ out) {
-  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+                                                          uint32_t P, uint32_t* __restrict__ out, uint32_t block, uint32_t blocks) {
+  const uint32_t gid = block * blockDim.x + threadIdx.x;
   const uint32_t nSeg = S * (P - 1u);
   if (gid >= nSeg) return;
   const uint32_t strand = gid / (P - 1u), seg = gid - strand * (P - 1u);
@@ -1219,6 +1371,21 @@ __global__ __launch_bounds__(64) void debug_raster_kernel(const fh_render_unifor
       atomicMax(&out[idx], value);
     }
 }
+
+struct DebugRaster {
+  struct Args {
+    const fh_render_uniforms* u;
+    const float4* pos;
+    const float4* tan;
+    uint32_t S;
+    uint32_t P;
+    uint32_t* out;
+  };
+  static constexpr uint32_t kThreads = 64;
+  static constexpr const char* kName = "debug_raster";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { debug_raster_body(a.u, a.pos, a.tan, a.S, a.P, a.out, block, blocks); }
+};
+
 
 int check_cfg(const fh_render_cfg* cfg, const char* fn) {
   if (!cfg) return fail(FH_ERR_INVALID_ARG, std::string(fn) + ": null cfg");
@@ -1289,21 +1456,11 @@ int fh_tiles_draw(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t
   a.pieceCount = reinterpret_cast<uint32_t*>(ctx->scratch);
   a.pieces = reinterpret_cast<uint2*>(reinterpret_cast<char*>(ctx->scratch) + 256);
   if ((rc = fill_u32(ctx, a.pieceCount, 1, 0u))) return rc;
-  const int pk0 = fh::prof_begin(ctx, "tiles_draw");
-  tiles_draw_kernel<<<fh::div_ceil((uint32_t)nSeg, 64), 64, 0, ctx->stream>>>(a);
-  fh::prof_end(ctx, pk0);
-  if ((rc = check_launch("tiles_draw_kernel"))) return rc;
-  const int pk0b = fh::prof_begin(ctx, "tiles_pieces");
-  tiles_pieces_kernel<<<std::min(fh::div_ceil((uint32_t)nSeg, 64), (uint32_t)ctx->num_cus * 8u), 64, 0,
-                        ctx->stream>>>(a);
-  fh::prof_end(ctx, pk0b);
-  if ((rc = check_launch("tiles_pieces_kernel"))) return rc;
+  if ((rc = launch<TilesDraw>(ctx, fh::div_ceil((uint32_t)nSeg, 64), 0, a))) return rc;
+  if ((rc = launch<TilesPieces>(ctx, std::min(fh::div_ceil((uint32_t)nSeg, 64), (uint32_t)ctx->num_cus * 8u), 0, a)))
+    return rc;
   const uint32_t nTiles = fh_tile_count(cfg, vp_w, vp_h);
-  const int pk1 = fh::prof_begin(ctx, "tile_counts");
-  tile_counts_kernel<<<fh::div_ceil(nTiles, 256), 256, 0, ctx->stream>>>(tiles, nTiles, cfg->tileDepthBins,
-                                                                       segment_count_per_tile);
-  fh::prof_end(ctx, pk1);
-  return check_launch("tile_counts_kernel");
+  return launch<TileCounts>(ctx, fh::div_ceil(nTiles, 256), 0, {tiles, nTiles, cfg->tileDepthBins, segment_count_per_tile});
 }
 
 int fh_tiles_rebuild_slot_keys(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t vp_h,
@@ -1314,12 +1471,10 @@ int fh_tiles_rebuild_slot_keys(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t v
   if (rc) return rc;
   FH_REQUIRE(tiles && tile_segments && slot_tile_bin, "fh_tiles_rebuild_slot_keys: null buffer");
   const uint32_t nTB = fh_tile_count(cfg, vp_w, vp_h) * cfg->tileDepthBins;
-  const int pk2 = fh::prof_begin(ctx, "rebuild_keys");
-  rebuild_keys_kernel<<<fh::div_ceil(nTB, 256), 256, 0, ctx->stream>>>(
-      tiles, reinterpret_cast<const fh_list_element*>(reinterpret_cast<const char*>(tile_segments) + 4),
-      segments_capacity, nTB, slot_tile_bin);
-  fh::prof_end(ctx, pk2);
-  return check_launch("rebuild_keys_kernel");
+  return launch<RebuildKeys>(
+      ctx, fh::div_ceil(nTB, 256), 0,
+      {tiles, reinterpret_cast<const fh_list_element*>(reinterpret_cast<const char*>(tile_segments) + 4),
+       segments_capacity, nTB, slot_tile_bin});
 }
 
 int fh_tiles_sort_clear(fh_ctx* ctx, const fh_render_cfg* cfg, void* tile_list, fh_sort_bucket* buckets) {
@@ -1341,17 +1496,12 @@ int fh_tiles_sort(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t
   const uint32_t nt = fh_tile_count(cfg, vp_w, vp_h);
   if (nt == 0) return FH_OK;
   const uint32_t blocks = fh::div_ceil(nt, 256);
-  const int pk3 = fh::prof_begin(ctx, "sort_count");
-  sort_count_kernel<<<blocks, 256, 0, ctx->stream>>>(segment_count_per_tile, nt, cfg->sortBuckets,
-                                                       cfg->sortBucketSize, buckets);
-  fh::prof_end(ctx, pk3);
-  if ((rc = check_launch("sort_count_kernel"))) return rc;
-  const int pk4 = fh::prof_begin(ctx, "sort_scatter");
-  sort_scatter_kernel<<<blocks, 256, 0, ctx->stream>>>(segment_count_per_tile, nt, cfg->sortBuckets,
-                                                         cfg->sortBucketSize, buckets,
-                                                         reinterpret_cast<uint32_t*>(tile_list));
-  fh::prof_end(ctx, pk4);
-  return check_launch("sort_scatter_kernel");
+  if ((rc = launch<SortCount>(ctx, blocks, 0,
+                              {segment_count_per_tile, nt, cfg->sortBuckets, cfg->sortBucketSize, buckets})))
+    return rc;
+  return launch<SortScatter>(ctx, blocks, 0,
+                             {segment_count_per_tile, nt, cfg->sortBuckets, cfg->sortBucketSize, buckets,
+                              reinterpret_cast<uint32_t*>(tile_list)});
 }
 
 int fh_fine_clear(fh_ctx* ctx, uint32_t vp_w, uint32_t vp_h, void* rasterizer_result) {
@@ -1398,33 +1548,18 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
   const fh_list_element* data =
       reinterpret_cast<const fh_list_element*>(reinterpret_cast<const char*>(tile_segments) + 4);
 
-  const int pk5 = fh::prof_begin(ctx, "bin_scan_local");
-  bin_scan_local_kernel<<<scanBlocks, kScanBlock, 0, ctx->stream>>>(tiles, nTB, ends, sums);
-  fh::prof_end(ctx, pk5);
-  if ((rc = check_launch("bin_scan_local_kernel"))) return rc;
-  if (scanBlocks > 1) {
-    const int pk6 = fh::prof_begin(ctx, "bin_scan_add");
-    bin_scan_add_kernel<<<scanBlocks, kScanBlock, 0, ctx->stream>>>(nTB, ends, sums);
-    fh::prof_end(ctx, pk6);
-    if ((rc = check_launch("bin_scan_add_kernel"))) return rc;
-  }
+  if ((rc = launch<BinScanLocal>(ctx, scanBlocks, 0, {tiles, nTB, ends, sums}))) return rc;
+  if (scanBlocks > 1 && (rc = launch<BinScanAdd>(ctx, scanBlocks, 0, {nTB, ends, sums}))) return rc;
   const uint32_t scatterBlocks = std::max(1u, std::min(fh::div_ceil(limit, 256), (uint32_t)ctx->num_cus * 8u));
-  const int pk7 = fh::prof_begin(ctx, "bin_scatter");
-  bin_scatter_kernel<<<scatterBlocks, 256, 0, ctx->stream>>>(
-      uniforms, reinterpret_cast<const uint32_t*>(tile_segments), data, limit,
-      reinterpret_cast<const uint2*>(slot_tile_bin), ends, nTB, binSeg);
-  fh::prof_end(ctx, pk7);
-  if ((rc = check_launch("bin_scatter_kernel"))) return rc;
+  if ((rc = launch<BinScatter>(ctx, scatterBlocks, 0,
+                               {uniforms, reinterpret_cast<const uint32_t*>(tile_segments), data, limit,
+                                reinterpret_cast<const uint2*>(slot_tile_bin), ends, nTB, binSeg})))
+    return rc;
   if ((rc = fill_u32(ctx, large, 1, 0u))) return rc;
-  const int pk8 = fh::prof_begin(ctx, "bin_sort_small");
-  bin_sort_small_kernel<<<fh::div_ceil(nTB, 256), 256, 0, ctx->stream>>>(tiles, ends, nTB, limit, binSeg, large);
-  fh::prof_end(ctx, pk8);
-  if ((rc = check_launch("bin_sort_small_kernel"))) return rc;
-  const int pk8b = fh::prof_begin(ctx, "bin_sort_large");
-  bin_sort_large_kernel<<<std::min(nTB, (uint32_t)ctx->num_cus * 8u), 256, 0, ctx->stream>>>(
-      tiles, ends, limit, binSeg, large, big);
-  fh::prof_end(ctx, pk8b);
-  if ((rc = check_launch("bin_sort_large_kernel"))) return rc;
+  if ((rc = launch<BinSortSmall>(ctx, fh::div_ceil(nTB, 256), 0, {tiles, ends, nTB, limit, binSeg, large}))) return rc;
+  if ((rc = launch<BinSortLarge>(ctx, std::min(nTB, (uint32_t)ctx->num_cus * 8u), 0,
+                                 {tiles, ends, limit, binSeg, large, big})))
+    return rc;
 
   FineArgs a;
   a.u = uniforms;
@@ -1456,26 +1591,9 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
   a.ldsPool = (uint32_t)std::min<size_t>((ldsTarget - fixed) / 12, a.perProc);
   const size_t lds = fixed + (size_t)a.ldsPool * 12;
   const uint32_t grid = cfg->processorCount;
-  if (T == 8) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&fine_raster_kernel<8>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    const int pk10 = fh::prof_begin(ctx, "fine_raster");
-    fine_raster_kernel<8><<<grid, 64 * fine_waves<8>(), lds, ctx->stream>>>(a);
-    fh::prof_end(ctx, pk10);
-  } else if (T == 16) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&fine_raster_kernel<16>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    const int pk11 = fh::prof_begin(ctx, "fine_raster");
-    fine_raster_kernel<16><<<grid, 64 * fine_waves<16>(), lds, ctx->stream>>>(a);
-    fh::prof_end(ctx, pk11);
-  } else {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&fine_raster_kernel<32>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    const int pk12 = fh::prof_begin(ctx, "fine_raster");
-    fine_raster_kernel<32><<<grid, 64 * fine_waves<32>(), lds, ctx->stream>>>(a);
-    fh::prof_end(ctx, pk12);
-  }
-  return check_launch("fine_raster_kernel");
+  if (T == 8) return launch<FineRaster<8>>(ctx, grid, (uint32_t)lds, a);
+  if (T == 16) return launch<FineRaster<16>>(ctx, grid, (uint32_t)lds, a);
+  return launch<FineRaster<32>>(ctx, grid, (uint32_t)lds, a);
 }
 
 // Test hook: the raster-helper kernel of swRasterizer.test.ts:165-250, which
@@ -1488,12 +1606,9 @@ int fh_debug_raster_barycentric(fh_ctx* ctx, const fh_render_uniforms* uniforms,
   FH_REQUIRE(points_per_strand >= 2, "fh_debug_raster_barycentric: points_per_strand >= 2");
   const uint32_t nSeg = strands_count * (points_per_strand - 1u);
   if (nSeg == 0) return FH_OK;
-  const int pk12 = fh::prof_begin(ctx, "debug_raster");
-  debug_raster_kernel<<<fh::div_ceil(nSeg, 64), 64, 0, ctx->stream>>>(
-      uniforms, reinterpret_cast<const float4*>(positions), reinterpret_cast<const float4*>(tangents),
-      strands_count, points_per_strand, out);
-  fh::prof_end(ctx, pk12);
-  return check_launch("debug_raster_kernel");
+  return launch<DebugRaster>(ctx, fh::div_ceil(nSeg, 64), 0,
+                             {uniforms, reinterpret_cast<const float4*>(positions),
+                              reinterpret_cast<const float4*>(tangents), strands_count, points_per_strand, out});
 }
 
 }  // extern "C"

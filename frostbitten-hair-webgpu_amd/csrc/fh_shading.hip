@@ -5,7 +5,7 @@
 // marschner.wgsl.ts:19-170, shadowMapPass/shared/sampleShadows.wgsl.ts:21-150,
 // aoPass/shared/textureAo.wgsl.ts:7-12, _shaderSnippets/*.
 #include "fh_device.hpp"
-#include "fh_internal.hpp"
+#include "fh_launch.hpp"
 
 namespace {
 
@@ -124,11 +124,11 @@ struct LightIn {
   float base[3], toCam[3], T[3], pos[3], shadow, ao;
 };
 
-__global__ __launch_bounds__(256) void shading_kernel(ShadeArgs a) {
+__device__ __forceinline__ void shading_body(ShadeArgs a, uint32_t block, uint32_t blocks) {
   __shared__ LightIn lin[kShadePoints];
   __shared__ float lout[kShadePoints * 3][3];
   const uint32_t pi = threadIdx.x / kShadeLanes;
-  const uint32_t gid = blockIdx.x * kShadePoints + pi;
+  const uint32_t gid = block * kShadePoints + pi;
   const uint32_t sub = threadIdx.x % kShadeLanes;
   const bool valid = gid < a.S * a.PS;
   const uint32_t s = valid ? gid / a.PS : 0u, id = valid ? gid - s * a.PS : 0u;
@@ -227,6 +227,14 @@ __global__ __launch_bounds__(256) void shading_kernel(ShadeArgs a) {
   }
 }
 
+struct Shading {
+  using Args = ShadeArgs;
+  static constexpr uint32_t kThreads = 256;
+  static constexpr const char* kName = "shading";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { shading_body(a, block, blocks); }
+};
+
+
 }  // namespace
 
 extern "C" int fh_shading(fh_ctx* ctx, const fh_render_cfg* cfg, const fh_render_uniforms* uniforms,
@@ -253,8 +261,5 @@ extern "C" int fh_shading(fh_ctx* ctx, const fh_render_cfg* cfg, const fh_render
   a.ao = ao.data; a.aoW = ao.width; a.aoH = ao.height;
   a.depth = depth.data; a.depthW = depth.width; a.depthH = depth.height;
   a.S = strands_count; a.P = points_per_strand; a.PS = cfg->shadingPoints;
-  const int pk0 = fh::prof_begin(ctx, "shading");
-  shading_kernel<<<fh::div_ceil((uint32_t)n, kShadePoints), 256, 0, ctx->stream>>>(a);
-  fh::prof_end(ctx, pk0);
-  return fh::check_launch("shading_kernel");
+  return fh::launch<Shading>(ctx, fh::div_ceil((uint32_t)n, kShadePoints), 0, a);
 }

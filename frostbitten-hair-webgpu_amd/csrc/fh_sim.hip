@@ -2,7 +2,7 @@
 // integration with constraints and collisions (K2) and the particle-to-grid
 // scatter (K3). Reference: src/passes/simulation/*.
 #include "fh_device.hpp"
-#include "fh_internal.hpp"
+#include "fh_launch.hpp"
 
 namespace fh {
 int fill_u32(fh_ctx* ctx, void* dst, uint64_t n_words, uint32_t value);
@@ -63,11 +63,11 @@ __device__ __forceinline__ float sample_sdf(const SdfView& s, f3 p) {
 // ---------------------------------------------------------------------------
 // K1 — gridPreSimPass.wgsl.ts:38-159. One thread per grid point.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void grid_pre_sim_kernel(
+__device__ __forceinline__ void grid_pre_sim_body(
     uint32_t d, const fh_sim_uniforms* __restrict__ u,
     const fh_density_velocity* __restrict__ dv, fh_density_grad_wind* __restrict__ out,
-    const float* __restrict__ sdf, uint32_t sdf_dims) {
-  const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    const float* __restrict__ sdf, uint32_t sdf_dims, uint32_t block, uint32_t blocks) {
+  const uint32_t id = block * blockDim.x + threadIdx.x;
   if (id >= d * d * d) return;
   const f3 gmin = mk3(u->gridBoundsMin[0], u->gridBoundsMin[1], u->gridBoundsMin[2]);
   const f3 gmax = mk3(u->gridBoundsMax[0], u->gridBoundsMax[1], u->gridBoundsMax[2]);
@@ -110,6 +110,21 @@ __global__ __launch_bounds__(256) void grid_pre_sim_kernel(
   *reinterpret_cast<float4*>(&out[idx0]) = make_float4(res.x, res.y, res.z, wind);
 }
 
+struct GridPreSim {
+  struct Args {
+    uint32_t d;
+    const fh_sim_uniforms* u;
+    const fh_density_velocity* dv;
+    fh_density_grad_wind* out;
+    const float* sdf;
+    uint32_t sdf_dims;
+  };
+  static constexpr uint32_t kThreads = 256;
+  static constexpr const char* kName = "grid_pre_sim";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { grid_pre_sim_body(a.d, a.u, a.dv, a.out, a.sdf, a.sdf_dims, block, blocks); }
+};
+
+
 // ---------------------------------------------------------------------------
 // K2 — hairSimIntegrationPass.wgsl.ts:79-273. A workgroup of kSimThreads
 // threads owns kSimStrands strands, staged in LDS lane-interleaved (point j
@@ -124,12 +139,12 @@ __global__ __launch_bounds__(256) void grid_pre_sim_kernel(
 constexpr uint32_t kSimStrands = 32;
 constexpr uint32_t kSimThreads = 256;
 
-__global__ __launch_bounds__(kSimThreads) void sim_integrate_kernel(
+__device__ __forceinline__ void sim_integrate_body(
     uint32_t d, const fh_sim_uniforms* __restrict__ u, float4* __restrict__ prev,
     const float4* __restrict__ now, const float* __restrict__ seglen,
     const float* __restrict__ sdf, uint32_t sdf_dims,
     const fh_density_velocity* __restrict__ dv, const fh_density_grad_wind* __restrict__ dgw,
-    const float4* __restrict__ initial, uint32_t S, uint32_t P) {
+    const float4* __restrict__ initial, uint32_t S, uint32_t P, uint32_t block, uint32_t blocks) {
   extern __shared__ float4 lds[];
   constexpr uint32_t L = kSimStrands;
   float4* A = lds;                       // positions being solved (xyz, movable)
@@ -137,7 +152,7 @@ __global__ __launch_bounds__(kSimThreads) void sim_integrate_kernel(
   float4* K = Bi + (size_t)P * L;        // local-shape constants (h0, w0, w1, w2)
   float* Len = reinterpret_cast<float*>(K + (size_t)P * L);  // rest lengths
   const uint32_t tid = threadIdx.x;
-  const uint32_t s0 = blockIdx.x * L;
+  const uint32_t s0 = block * L;
   const uint32_t nStr = min(L, S - s0);
   const uint32_t nElem = nStr * P;
   const size_t gbase = (size_t)s0 * P;
@@ -328,6 +343,27 @@ __global__ __launch_bounds__(kSimThreads) void sim_integrate_kernel(
   }
 }
 
+struct SimIntegrate {
+  struct Args {
+    uint32_t d;
+    const fh_sim_uniforms* u;
+    float4* prev;
+    const float4* now;
+    const float* seglen;
+    const float* sdf;
+    uint32_t sdf_dims;
+    const fh_density_velocity* dv;
+    const fh_density_grad_wind* dgw;
+    const float4* initial;
+    uint32_t S;
+    uint32_t P;
+  };
+  static constexpr uint32_t kThreads = kSimThreads;
+  static constexpr const char* kName = "sim_integrate";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { sim_integrate_body(a.d, a.u, a.prev, a.now, a.seglen, a.sdf, a.sdf_dims, a.dv, a.dgw, a.initial, a.S, a.P, block, blocks); }
+};
+
+
 // ---------------------------------------------------------------------------
 // K3 — gridPostSimPass.wgsl.ts:45-72 with _addGridDensityVelocity
 // (densityVelocityGrid.wgsl.ts:93-124). Every point adds fixed-point
@@ -431,11 +467,11 @@ __device__ __forceinline__ void p2g_walk(const P2GArgs& a, f3 gmin, f3 gmax, uin
 }
 
 template <bool SCATTER, bool LDS>
-__global__ __launch_bounds__(kP2GBinThreads) void p2g_bin_kernel(P2GArgs a, uint32_t nBlocks, uint32_t span) {
+__device__ __forceinline__ void p2g_bin_body(P2GArgs a, uint32_t nBlocks, uint32_t span, uint32_t block) {
   __shared__ uint32_t hist[LDS ? kP2GHistMax : 1];
   const f3 gmin = mk3(a.u->gridBoundsMin[0], a.u->gridBoundsMin[1], a.u->gridBoundsMin[2]);
   const f3 gmax = mk3(a.u->gridBoundsMax[0], a.u->gridBoundsMax[1], a.u->gridBoundsMax[2]);
-  const uint32_t first = blockIdx.x * span;
+  const uint32_t first = block * span;
   const uint32_t last = min(first + span, a.nPoints);
   if (!LDS) {
     // global counters; the scatter turns each cursor into a slot through
@@ -458,9 +494,22 @@ __global__ __launch_bounds__(kP2GBinThreads) void p2g_bin_kernel(P2GArgs a, uint
   p2g_walk<true>(a, gmin, gmax, first, last, hist, true);
 }
 
+template <bool SCATTER, bool LDS>
+struct P2GBin {
+  struct Args {
+    P2GArgs a;
+    uint32_t nBlocks, span;
+  };
+  static constexpr uint32_t kThreads = kP2GBinThreads;
+  static constexpr const char* kName = SCATTER ? "p2g_scatter" : "p2g_count";
+  static __device__ void run(const Args& x, uint32_t block, uint32_t) {
+    p2g_bin_body<SCATTER, LDS>(x.a, x.nBlocks, x.span, block);
+  }
+};
+
 // Step 2, one workgroup: exclusive scans of the counts (-> offsets) and of
 // the chunks per block (-> chunkStart, with the total at the end).
-__global__ __launch_bounds__(1024) void p2g_scan_kernel(P2GArgs a, uint32_t nBlocks) {
+__device__ __forceinline__ void p2g_scan_body(P2GArgs a, uint32_t nBlocks, uint32_t block, uint32_t blocks) {
   __shared__ uint32_t warpSums[2][16];
   uint32_t carryP = 0, carryC = 0;
   const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
@@ -489,14 +538,25 @@ __global__ __launch_bounds__(1024) void p2g_scan_kernel(P2GArgs a, uint32_t nBlo
   if (threadIdx.x == 0) a.chunkStart[nBlocks] = carryC;
 }
 
+struct P2GScan {
+  struct Args {
+    P2GArgs a;
+    uint32_t nBlocks;
+  };
+  static constexpr uint32_t kThreads = 1024;
+  static constexpr const char* kName = "p2g_scan";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { p2g_scan_body(a.a, a.nBlocks, block, blocks); }
+};
+
+
 // Step 4: one workgroup per chunk of a block's pairs.
-__global__ __launch_bounds__(256) void p2g_accum_kernel(P2GArgs a, uint32_t nBlocks) {
+__device__ __forceinline__ void p2g_accum_body(P2GArgs a, uint32_t nBlocks, uint32_t block, uint32_t blocks) {
   __shared__ int32_t acc[kBlkNodes][4];
   __shared__ uint32_t s_blk;
   const uint32_t total = a.chunkStart[nBlocks];
   const f3 gmin = mk3(a.u->gridBoundsMin[0], a.u->gridBoundsMin[1], a.u->gridBoundsMin[2]);
   const f3 gmax = mk3(a.u->gridBoundsMax[0], a.u->gridBoundsMax[1], a.u->gridBoundsMax[2]);
-  for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
+  for (uint32_t c = block; c < total; c += blocks) {
     if (threadIdx.x == 0) {
       // last block whose first chunk is <= c
       uint32_t lo = 0, hi = nBlocks;
@@ -546,6 +606,17 @@ __global__ __launch_bounds__(256) void p2g_accum_kernel(P2GArgs a, uint32_t nBlo
   }
 }
 
+struct P2GAccum {
+  struct Args {
+    P2GArgs a;
+    uint32_t nBlocks;
+  };
+  static constexpr uint32_t kThreads = 256;
+  static constexpr const char* kName = "grid_post_sim";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { p2g_accum_body(a.a, a.nBlocks, block, blocks); }
+};
+
+
 }  // namespace
 
 extern "C" {
@@ -560,11 +631,8 @@ int fh_grid_pre_sim(fh_ctx* ctx, uint32_t grid_dims, const fh_sim_uniforms* sim_
   FH_REQUIRE(sim_uniforms && density_velocity && density_grad_wind && sdf_texture,
              "fh_grid_pre_sim: null buffer");
   const uint32_t n = grid_dims * grid_dims * grid_dims;
-  const int pk0 = fh::prof_begin(ctx, "grid_pre_sim");
-  grid_pre_sim_kernel<<<fh::div_ceil(n, 256), 256, 0, ctx->stream>>>(
-      grid_dims, sim_uniforms, density_velocity, density_grad_wind, sdf_texture, sdf_dims);
-  fh::prof_end(ctx, pk0);
-  return fh::check_launch("grid_pre_sim_kernel");
+  return fh::launch<GridPreSim>(ctx, fh::div_ceil(n, 256), 0,
+                                {grid_dims, sim_uniforms, density_velocity, density_grad_wind, sdf_texture, sdf_dims});
 }
 
 int fh_sim_integrate(fh_ctx* ctx, uint32_t grid_dims, const fh_sim_uniforms* sim_uniforms,
@@ -587,16 +655,11 @@ int fh_sim_integrate(fh_ctx* ctx, uint32_t grid_dims, const fh_sim_uniforms* sim
   if (strands_count == 0) return FH_OK;
   const size_t lds = (size_t)points_per_strand * kSimStrands * (3 * sizeof(float4) + sizeof(float));
   FH_REQUIRE(lds <= ctx->max_lds, "fh_sim_integrate: points_per_strand too large for this device's LDS");
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&sim_integrate_kernel),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  const int pk1 = fh::prof_begin(ctx, "sim_integrate");
-  sim_integrate_kernel<<<fh::div_ceil(strands_count, kSimStrands), kSimThreads, lds, ctx->stream>>>(
-      grid_dims, sim_uniforms, reinterpret_cast<float4*>(positions_prev),
-      reinterpret_cast<const float4*>(positions_now), segment_lengths, sdf_texture, sdf_dims,
-      density_velocity, density_grad_wind, reinterpret_cast<const float4*>(positions_initial),
-      strands_count, points_per_strand);
-  fh::prof_end(ctx, pk1);
-  return fh::check_launch("sim_integrate_kernel");
+  return fh::launch<SimIntegrate>(
+      ctx, fh::div_ceil(strands_count, kSimStrands), (uint32_t)lds,
+      {grid_dims, sim_uniforms, reinterpret_cast<float4*>(positions_prev),
+       reinterpret_cast<const float4*>(positions_now), segment_lengths, sdf_texture, sdf_dims, density_velocity,
+       density_grad_wind, reinterpret_cast<const float4*>(positions_initial), strands_count, points_per_strand});
 }
 
 int fh_grid_post_sim(fh_ctx* ctx, uint32_t grid_dims, const fh_sim_uniforms* sim_uniforms,
@@ -641,24 +704,16 @@ int fh_grid_post_sim(fh_ctx* ctx, uint32_t grid_dims, const fh_sim_uniforms* sim
   const uint32_t span = std::min(1u << 16, std::max(1024u, fh::div_ceil(fh::div_ceil((uint32_t)n, cus * 4u), 1024u) * 1024u));
   const uint32_t pblocks = fh::div_ceil((uint32_t)n, span);
   const bool lds = nBlocks <= kP2GHistMax;
-  const int pk2 = fh::prof_begin(ctx, "p2g_count");
-  if (lds) p2g_bin_kernel<false, true><<<pblocks, kP2GBinThreads, 0, ctx->stream>>>(a, nBlocks, span);
-  else p2g_bin_kernel<false, false><<<pblocks, kP2GBinThreads, 0, ctx->stream>>>(a, nBlocks, span);
-  fh::prof_end(ctx, pk2);
-  if ((rc = fh::check_launch("p2g_bin_kernel"))) return rc;
-  const int pk3 = fh::prof_begin(ctx, "p2g_scan");
-  p2g_scan_kernel<<<1, 1024, 0, ctx->stream>>>(a, nBlocks);
-  fh::prof_end(ctx, pk3);
-  if ((rc = fh::check_launch("p2g_scan_kernel"))) return rc;
-  const int pk4 = fh::prof_begin(ctx, "p2g_scatter");
-  if (lds) p2g_bin_kernel<true, true><<<pblocks, kP2GBinThreads, 0, ctx->stream>>>(a, nBlocks, span);
-  else p2g_bin_kernel<true, false><<<pblocks, kP2GBinThreads, 0, ctx->stream>>>(a, nBlocks, span);
-  fh::prof_end(ctx, pk4);
-  if ((rc = fh::check_launch("p2g_bin_kernel"))) return rc;
-  const int pk5 = fh::prof_begin(ctx, "grid_post_sim");
-  p2g_accum_kernel<<<cus * 8u, 256, 0, ctx->stream>>>(a, nBlocks);
-  fh::prof_end(ctx, pk5);
-  return fh::check_launch("p2g_accum_kernel");
+  if (lds) rc = fh::launch<P2GBin<false, true>>(ctx, pblocks, 0, {a, nBlocks, span});
+  else rc = fh::launch<P2GBin<false, false>>(ctx, pblocks, 0, {a, nBlocks, span});
+  if (rc) return rc;
+  if ((rc = fh::launch<P2GScan>(ctx, 1, 0, {a, nBlocks}))) return rc;
+  if (lds) rc = fh::launch<P2GBin<true, true>>(ctx, pblocks, 0, {a, nBlocks, span});
+  else rc = fh::launch<P2GBin<true, false>>(ctx, pblocks, 0, {a, nBlocks, span});
+  if (rc) return rc;
+  // chunks: about 2n / chunk for the pairs, plus a partial one per non-empty block
+  const uint32_t accumBlocks = std::min(cus * 8u, fh::div_ceil(2u * (uint32_t)n, a.chunk) + 64u);
+  return fh::launch<P2GAccum>(ctx, accumBlocks, 0, {a, nBlocks});
 }
 
 }  // extern "C"

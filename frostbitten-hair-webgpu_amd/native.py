@@ -78,13 +78,23 @@ SIGNATURES = {
     "fh_graph_launch": (i32, [vp, vp]),
     "fh_graph_destroy": (i32, [vp]),
     "fh_profiler_read": (i32, [vp, C.POINTER(KernelTime), u32, C.POINTER(u32)]),
+    "fh_record_begin": (i32, [vp]),
+    "fh_record_end": (i32, [vp, C.POINTER(vp)]),
+    "fh_program_merge": (i32, [vp, C.POINTER(vp), u32, C.POINTER(vp)]),
+    "fh_program_launches": (i32, [vp, C.POINTER(u32)]),
+    "fh_program_launch": (i32, [vp, vp]),
+    "fh_program_destroy": (i32, [vp]),
 }
+
+FH_ERR_CAPACITY = 4
 
 _lib = None
 
 
 class FhError(RuntimeError):
-    pass
+    def __init__(self, msg: str, code: int = 0):
+        super().__init__(msg)
+        self.code = code
 
 
 def load(path: Path | None = None) -> C.CDLL:
@@ -106,7 +116,7 @@ def load(path: Path | None = None) -> C.CDLL:
 def check(rc: int, what: str) -> None:
     if rc != 0:
         msg = _lib.fh_last_error().decode(errors="replace") if _lib else "?"
-        raise FhError(f"{what} failed with code {rc}: {msg}")
+        raise FhError(f"{what} failed with code {rc}: {msg}", rc)
 
 
 def ptr(t) -> vp:
@@ -147,6 +157,25 @@ class Context:
         check(self.lib.fh_capture_end(self.handle, C.byref(g)), "fh_capture_end")
         return Graph(self, g)
 
+    def record_begin(self) -> None:
+        check(self.lib.fh_record_begin(self.handle), "fh_record_begin")
+
+    def record_end(self) -> "Program | None":
+        """The recorded launches, or None if the recording had to be discarded
+        because the context's scratch grew (record the same calls again)."""
+        p = vp()
+        rc = self.lib.fh_record_end(self.handle, C.byref(p))
+        if rc == FH_ERR_CAPACITY:
+            return None
+        check(rc, "fh_record_end")
+        return Program(self, p)
+
+    def program_merge(self, programs: list) -> "Program":
+        arr = (vp * len(programs))(*[p.handle for p in programs])
+        out = vp()
+        check(self.lib.fh_program_merge(self.handle, arr, len(programs), C.byref(out)), "fh_program_merge")
+        return Program(self, out)
+
     def profiler_enable(self, max_launches: int) -> None:
         """Times the next `max_launches` kernel launches with HIP events (0 stops)."""
         check(self.lib.fh_profiler_enable(self.handle, max_launches), "fh_profiler_enable")
@@ -161,6 +190,35 @@ class Context:
     def close(self) -> None:
         if self.handle:
             self.lib.fh_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Program:
+    """fh_program: recorded launches of one context, or several merged into
+    batched launches (launch() enqueues a merged one on the context's stream)."""
+
+    def __init__(self, ctx: Context, handle: vp):
+        self.ctx = ctx
+        self.handle = handle
+
+    def launches(self) -> int:
+        n = u32(0)
+        check(self.ctx.lib.fh_program_launches(self.handle, C.byref(n)), "fh_program_launches")
+        return n.value
+
+    def launch(self, ctx: Context | None = None) -> None:
+        c = ctx or self.ctx
+        check(c.lib.fh_program_launch(c.handle, self.handle), "fh_program_launch")
+
+    def close(self) -> None:
+        if self.handle:
+            self.ctx.lib.fh_program_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

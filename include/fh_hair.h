@@ -438,6 +438,45 @@ int fh_graph_launch(fh_ctx* ctx, fh_graph* graph);
 int fh_graph_destroy(fh_graph* graph);
 
 /* ------------------------------------------------------------------------ */
+/* Batching: many hair instances per launch                                 */
+/* ------------------------------------------------------------------------ */
+
+/* The batched form of the chain (no counterpart in the reference, which
+ * renders one hair object per frame, renderer.ts:212-283): each instance
+ * keeps its own context, buffers and configuration; its frame is recorded
+ * once through the ordinary entry points, and the recorded frames of all
+ * instances are merged so that every kernel of the chain runs once for all
+ * instances that use it. A merged launch gives instance i its own block
+ * range, arguments and grid size, so each instance computes exactly what its
+ * own calls compute. */
+typedef struct fh_program fh_program;
+
+/* Starts recording: until fh_record_end, the entry points called on `ctx`
+ * check their arguments and keep their kernel launches (grid, LDS bytes,
+ * arguments) instead of enqueuing them. Nothing runs on the device, so the
+ * instance's state does not advance. */
+int fh_record_begin(fh_ctx* ctx);
+/* Ends recording; *out holds the recorded launches of this context. If the
+ * context's scratch had to grow during the recording, the launches recorded
+ * before the growth point at the old scratch: the recording is discarded,
+ * *out is NULL and FH_ERR_CAPACITY is returned; recording the same calls
+ * again then succeeds. */
+int fh_record_end(fh_ctx* ctx, fh_program** out);
+/* Merges `count` recorded programs into one program of batched launches.
+ * Each input's launches keep their order; a kernel that several inputs
+ * launch at a matching position becomes one launch. The arguments and block
+ * ranges are uploaded to device memory owned by the result. The inputs stay
+ * valid and unchanged. */
+int fh_program_merge(fh_ctx* ctx, const fh_program* const* programs, uint32_t count, fh_program** out);
+/* Number of launches of a merged program. */
+int fh_program_launches(const fh_program* program, uint32_t* n_out);
+/* Enqueues a merged program on the context's stream (also under
+ * fh_capture_begin, to replay the batch as one graph). The buffers of every
+ * instance must stay allocated while the program is used. */
+int fh_program_launch(fh_ctx* ctx, const fh_program* program);
+int fh_program_destroy(fh_program* program);
+
+/* ------------------------------------------------------------------------ */
 /* Test hook                                                                 */
 /* ------------------------------------------------------------------------ */
 

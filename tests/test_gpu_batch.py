@@ -1,0 +1,85 @@
+"""Batched form: every instance computes in the merged launches exactly what
+its plain frame computes (positions, tile lists, images), over several frames
+and both parities of the position buffers, with and without the graph."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _workloads():
+    """Uneven instances: strand and point counts, resolutions and all three
+    tile sizes (so the merged frame has kernels that only some instances
+    launch, e.g. the fine pass per tile size)."""
+    from frostbitten_hair_webgpu_amd import assets as A
+    specs = [dict(kind="small", strands=300, width=320, height=240, points=None, tile=16, bins=4, slices=4),
+             dict(kind="c2", strands=900, width=400, height=300, points=8, tile=8, bins=8, slices=4),
+             dict(kind="c2", strands=1500, width=512, height=512, points=24, tile=32, bins=4, slices=8),
+             dict(kind="small", strands=64, width=160, height=120, points=None, tile=16, bins=16, slices=2)]
+    out = []
+    for i, sp in enumerate(specs):
+        wl = A.make_workload(sp["kind"], seed=1000 + i, strands=sp["strands"], width=sp["width"],
+                             height=sp["height"], points=sp["points"])
+        wl.render.tileSize, wl.render.tileDepthBins, wl.render.slicesPerPixel = sp["tile"], sp["bins"], sp["slices"]
+        out.append(wl)
+    return out
+
+
+def _state(r):
+    """Everything a frame leaves behind. The list heads and element order of
+    the tile lists follow the order of the atomic appends, so the lists are
+    compared as per-(tile, bin) sets (with the depth range and count)."""
+    from gpu_util import walk_lists
+    tiles = _np(r.tilesPass.hairTilesBuffer).reshape(-1, 4)
+    seg = _np(r.tilesPass.hairTileSegmentsBuffer).view(np.uint32)
+    lists = walk_lists(tiles.view(np.uint32), seg, r.tilesPass.segmentsCapacity)
+    return dict(now=_np(r.hair.positionsNow), prev=_np(r.hair.positionsPrev), dv=_np(r.grid.densityVelocity),
+                depth=tiles[:, [0, 1, 3]], lists=sorted(lists.items()), drawn=int(seg[0]),
+                counts=_np(r.tilesPass.segmentCountPerTileBuffer),
+                img=_np(r.finePass.image(r.wl.width, r.wl.height)), shading=_np(r.hair.shading))
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_matches_plain(gpu, use_graph):
+    import torch
+    from frostbitten_hair_webgpu_amd import batch as B
+    from frostbitten_hair_webgpu_amd import passes as PS
+    wls = _workloads()
+    plain = [PS.HairRenderer(wl, "cuda:0", shadow_size=256) for wl in wls]
+    inputs = [{k: v.cpu().numpy() for k, v in r.inputs.items()} for r in plain]
+    br = B.BatchRenderer(wls, "cuda:0", inputs=inputs, use_graph=use_graph)
+    for r in plain:
+        r.beforeFirstFrame()
+    br.beforeFirstFrame()
+    br.build()
+    for frame in range(3):
+        for r in plain:
+            r.frame()
+        br.frame()
+        torch.cuda.synchronize()
+        for k, (rp, rb) in enumerate(zip(plain, br.renderers)):
+            a, b = _state(rp), _state(rb)
+            for key in a:
+                same = a[key] == b[key] if key in ("lists", "drawn") else np.array_equal(a[key], b[key])
+                assert same, f"frame {frame} instance {k}: {key} differs"
+            assert a["img"][..., 3].max() > 0.0
+
+
+def test_batch_merge_is_shorter(gpu):
+    """The merged frame has one launch per kernel position, not one per
+    instance: far fewer launches than the instances' frames together."""
+    from frostbitten_hair_webgpu_amd import batch as B
+    from frostbitten_hair_webgpu_amd import passes as PS
+    wls = _workloads()
+    rs = [PS.HairRenderer(wl, "cuda:0", shadow_size=256) for wl in wls]
+    for r in rs:
+        r.beforeFirstFrame()
+    progs = [B.record_frame(r) for r in rs]
+    per = [p.launches() for p in progs]
+    merged = rs[0].native.program_merge(progs)
+    assert max(per) <= merged.launches() < sum(per)
+    assert merged.launches() <= max(per) + 4  # tile-size variants of the fine pass and optional scans
