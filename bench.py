@@ -139,6 +139,110 @@ def cpu_baseline(wl, r, frames: int = 8, budget_s: float = 20.0) -> dict:
                       f"median {t:.2f} s/frame, scalar C oracle on 1 host core"}
 
 
+def batched_vs_plain(n: int, steps: int, warmup: int, tune: bool, dev, log) -> dict:
+    """The batched form against plain runs on the same instances: `n`
+    instances of uneven sizes (strands log-uniform in [1k, 50k], points from
+    {8, 16, 24, 32}, resolutions from {512^2, 1280x720, 1920x1080}, each with
+    its own camera, wind and 64^3 SDF), render configuration from the
+    autotuner. Plain: each instance replays its own frame graph, one instance
+    after another. Batched: one graph per frame runs every kernel of the chain
+    once for all instances. Both are timed over `steps` rounds (one frame of
+    every instance) with the per-frame simulation uniforms already on the
+    device."""
+    import torch
+    from frostbitten_hair_webgpu_amd import assets as A
+    from frostbitten_hair_webgpu_amd import autotune as AT
+    from frostbitten_hair_webgpu_amd import batch as B
+    from frostbitten_hair_webgpu_amd import passes as PS
+    t0 = time.perf_counter()
+    wls = [A.make_instance(i) for i in range(n)]
+    inputs = [PS.make_inputs(wl, PS.workload_render_uniforms(wl), 1024) for wl in wls]
+    t1 = time.perf_counter()
+    log(f"# batch: {n} instances generated in {t1 - t0:.1f} s")
+    if tune:
+        choice = AT.tune(wls, inputs, log=log)
+    else:
+        choice = [(16, 8, 4)] * n
+        for wl in wls:
+            AT.apply(wl, choice[0])
+    t2 = time.perf_counter()
+    log(f"# batch: tuned in {t2 - t1:.1f} s")
+    br = B.BatchRenderer(wls, dev, inputs=inputs)
+    br.beforeFirstFrame()
+    for r in br.renderers:  # every call once (scratch sizes), then the plain graphs
+        r.frame()
+    for r in br.renderers:
+        r.capture_frames()
+    br.frameIdx = br.renderers[0].frameIdx
+    br.build()
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    log(f"# batch: built in {t3 - t2:.1f} s, {br.launches_per_frame()} launches per batched frame")
+    total = 2 * (warmup + steps)
+    ring = torch.from_numpy(np.stack([br.sim_uniforms(br.frameIdx + f) for f in range(total)])).to(dev)
+
+    def plain_round(f):
+        for i, r in enumerate(br.renderers):
+            r.simulationUniforms.copy_(ring[f, i])
+            r.frame_graph(upload_uniforms=False)
+
+    def batch_frame(f):
+        br.simPacked.copy_(ring[f])
+        br.frame(upload_uniforms=False)
+
+    def timed(fn, first):
+        for k in range(warmup):
+            fn(first + k)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+        torch.cuda.synchronize()
+        c0 = time.perf_counter()
+        ev[0].record()
+        for k in range(steps):
+            fn(first + warmup + k)
+            ev[k + 1].record()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - c0
+        per = np.array([ev[k].elapsed_time(ev[k + 1]) for k in range(steps)])
+        return el, per
+
+    el_p, per_p = timed(plain_round, 0)
+    el_b, per_b = timed(batch_frame, warmup + steps)
+    # per-kernel times of the batched frame: its launches issued directly
+    # with HIP events around each (a third region, kept out of the timing above)
+    graphs, br.graphs = br.graphs, None
+    br.native.profiler_enable(64 * 4)
+    br.native.profiler_read()
+    for k in range(4):
+        batch_frame(total - 4 + k)
+    torch.cuda.synchronize()
+    kernels = br.native.profiler_read()
+    br.native.profiler_enable(0)
+    br.graphs = graphs
+    S = np.array([w.strands for w in wls])
+    P = np.array([w.points for w in wls])
+    plain_fps = n * steps / el_p
+    batch_fps = n * steps / el_b
+    return {
+        "instances": n, "steps": steps,
+        "value": round(batch_fps, 2), "unit": "instance-frames/s",
+        "plain_value": round(plain_fps, 2),
+        "speedup_vs_plain": round(batch_fps / plain_fps, 3),
+        "batch_frame_ms": {"mean": round(el_b / steps * 1e3, 3), "p50": round(float(np.percentile(per_b, 50)), 3),
+                           "p99": round(float(np.percentile(per_b, 99)), 3)},
+        "plain_round_ms": {"mean": round(el_p / steps * 1e3, 3), "p50": round(float(np.percentile(per_p, 50)), 3),
+                           "p99": round(float(np.percentile(per_p, 99)), 3)},
+        "launches_per_batched_frame": br.launches_per_frame(),
+        "kernel_us": {k: round(t / c * 1e3, 1) for k, (c, t) in sorted(kernels.items(), key=lambda kv: -kv[1][1])},
+        "strands_total": int(S.sum()), "points_total": int((S * P).sum()),
+        "pixels_total": int(sum(w.width * w.height for w in wls)),
+        "particle_steps_per_s": round(float((S * P).sum()) * steps / el_b, 1),
+        "segments_rasterized_per_s": round(float((S * (P - 1)).sum()) * steps / el_b, 1),
+        "autotune": AT.summary(choice) if tune else "off (tile 16, 8 bins x 4 slices)",
+        "device_bytes": br.memory_bytes(),
+        "setup_s": round(time.perf_counter() - t0, 1),
+    }
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +250,9 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2", choices=["small", "c2", "c3"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-instances", type=int, default=64,
+                    help="instances of the batched-versus-plain comparison (0: skip it)")
+    ap.add_argument("--no-tune", action="store_true", help="batched comparison without the autotuner")
     ap.add_argument("--no-graph", action="store_true", help="launch every call eagerly instead of replaying "
                     "the frame as a HIP graph")
     ap.add_argument("--no-kernel-timing", action="store_true",
@@ -328,6 +435,10 @@ def main() -> None:
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, r)
+    if args.batch_instances:
+        del r
+        out["batched"] = batched_vs_plain(args.batch_instances, min(args.steps, 20), 2, not args.no_tune, dev,
+                                          lambda m: print(m, file=sys.stderr, flush=True))
     print(json.dumps(out), flush=True)
 
 

@@ -15,6 +15,8 @@ frame's per-instance inputs (frameIdx, wind timer) arrive in one copy.
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 import torch
 
@@ -68,7 +70,7 @@ class BatchRenderer:
         self.use_graph = use_graph
         self.programs = None
         self.graphs = None
-        self.parity = 0
+        self.build_current = 0
 
     def __len__(self) -> int:
         return len(self.renderers)
@@ -84,9 +86,16 @@ class BatchRenderer:
         for r in self.renderers:
             r.beforeFirstFrame()
 
-    def build(self) -> None:
+    def build(self, device_blocks: int | None = None) -> None:
         """Records both parities of every instance, merges them and (with
-        graphs) captures the merged frames."""
+        graphs) captures the merged frames. The grid-stride kernels of each
+        instance get its share (by point count) of `device_blocks`
+        workgroups (default 8 per CU), at least 64."""
+        if device_blocks is None:
+            device_blocks = 8 * torch.cuda.get_device_properties(self.device).multi_processor_count
+        pts = np.array([r.wl.strands * r.wl.points for r in self.renderers], dtype=np.float64)
+        for r, w in zip(self.renderers, pts / pts.sum()):
+            r.native.set_persistent_blocks(max(64, int(math.ceil(device_blocks * w))))
         recorded = [[], []]
         for r in self.renderers:
             for parity in range(2):
@@ -95,7 +104,7 @@ class BatchRenderer:
         for lst in recorded:
             for p in lst:
                 p.close()
-        self.parity = 0
+        self.build_current = self.renderers[0].hair.current  # program 0 reads positions[build_current]
         if self.use_graph:
             main = torch.cuda.current_stream(self.device)
             side = torch.cuda.Stream(self.device)
@@ -121,11 +130,13 @@ class BatchRenderer:
         """One frame of every instance."""
         if upload_uniforms:
             self.upload_sim_uniforms()
+        parity = self.renderers[0].hair.current ^ self.build_current
+        assert all(r.hair.current ^ self.build_current == parity for r in self.renderers), \
+            "instances must advance together"
         if self.graphs is not None:
-            self.graphs[self.parity].launch()
+            self.graphs[parity].launch()
         else:
-            self.programs[self.parity].launch(self.native)
-        self.parity ^= 1
+            self.programs[parity].launch(self.native)
         for r in self.renderers:
             r.hair.swapPositionBuffersAfterSimIntegration()
             r.frameIdx += 1

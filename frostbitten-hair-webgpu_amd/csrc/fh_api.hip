@@ -287,6 +287,12 @@ int fh_profiler_read(fh_ctx* ctx, fh_kernel_time* out, uint32_t max_out, uint32_
   return FH_OK;
 }
 
+int fh_set_persistent_blocks(fh_ctx* ctx, uint32_t blocks) {
+  FH_CHECK_CTX(ctx);
+  ctx->persist_blocks = blocks;
+  return FH_OK;
+}
+
 int fh_record_begin(fh_ctx* ctx) {
   FH_CHECK_CTX(ctx);
   FH_REQUIRE(!ctx->recording && !ctx->capturing, "fh_record_begin: already recording or capturing");

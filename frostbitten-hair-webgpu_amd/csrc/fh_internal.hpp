@@ -28,6 +28,8 @@ struct fh_ctx {
   hipStream_t stream = nullptr;
   bool owns_stream = false;
   int num_cus = 0;
+  // Workgroups of a grid-stride ("persistent") launch; 0: 8 per CU.
+  uint32_t persist_blocks = 0;
   size_t max_lds = 64 * 1024;  // LDS bytes one workgroup may allocate
   // Grow-on-demand scratch owned by the context (fine-pass gather lists,
   // per-bin offsets, prefix-scan temporaries).
@@ -89,6 +91,9 @@ int zero_many(fh_ctx* ctx, int count, void* const* dst, const uint64_t* n_words,
 void prof_end(fh_ctx* ctx, int k);
 
 inline uint32_t div_ceil(uint32_t a, uint32_t b) { return (a + b - 1u) / b; }
+inline uint32_t persistent_blocks(const fh_ctx* ctx) {
+  return ctx->persist_blocks ? ctx->persist_blocks : 8u * (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 1);
+}
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 }  // namespace fh

@@ -1457,7 +1457,7 @@ int fh_tiles_draw(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t
   a.pieces = reinterpret_cast<uint2*>(reinterpret_cast<char*>(ctx->scratch) + 256);
   if ((rc = fill_u32(ctx, a.pieceCount, 1, 0u))) return rc;
   if ((rc = launch<TilesDraw>(ctx, fh::div_ceil((uint32_t)nSeg, 64), 0, a))) return rc;
-  if ((rc = launch<TilesPieces>(ctx, std::min(fh::div_ceil((uint32_t)nSeg, 64), (uint32_t)ctx->num_cus * 8u), 0, a)))
+  if ((rc = launch<TilesPieces>(ctx, std::min(fh::div_ceil((uint32_t)nSeg, 64), fh::persistent_blocks(ctx)), 0, a)))
     return rc;
   const uint32_t nTiles = fh_tile_count(cfg, vp_w, vp_h);
   return launch<TileCounts>(ctx, fh::div_ceil(nTiles, 256), 0, {tiles, nTiles, cfg->tileDepthBins, segment_count_per_tile});
@@ -1550,14 +1550,14 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
 
   if ((rc = launch<BinScanLocal>(ctx, scanBlocks, 0, {tiles, nTB, ends, sums}))) return rc;
   if (scanBlocks > 1 && (rc = launch<BinScanAdd>(ctx, scanBlocks, 0, {nTB, ends, sums}))) return rc;
-  const uint32_t scatterBlocks = std::max(1u, std::min(fh::div_ceil(limit, 256), (uint32_t)ctx->num_cus * 8u));
+  const uint32_t scatterBlocks = std::max(1u, std::min(fh::div_ceil(limit, 256), fh::persistent_blocks(ctx)));
   if ((rc = launch<BinScatter>(ctx, scatterBlocks, 0,
                                {uniforms, reinterpret_cast<const uint32_t*>(tile_segments), data, limit,
                                 reinterpret_cast<const uint2*>(slot_tile_bin), ends, nTB, binSeg})))
     return rc;
   if ((rc = fill_u32(ctx, large, 1, 0u))) return rc;
   if ((rc = launch<BinSortSmall>(ctx, fh::div_ceil(nTB, 256), 0, {tiles, ends, nTB, limit, binSeg, large}))) return rc;
-  if ((rc = launch<BinSortLarge>(ctx, std::min(nTB, (uint32_t)ctx->num_cus * 8u), 0,
+  if ((rc = launch<BinSortLarge>(ctx, std::min(nTB, fh::persistent_blocks(ctx)), 0,
                                  {tiles, ends, limit, binSeg, large, big})))
     return rc;
 

@@ -712,7 +712,7 @@ int fh_grid_post_sim(fh_ctx* ctx, uint32_t grid_dims, const fh_sim_uniforms* sim
   else rc = fh::launch<P2GBin<true, false>>(ctx, pblocks, 0, {a, nBlocks, span});
   if (rc) return rc;
   // chunks: about 2n / chunk for the pairs, plus a partial one per non-empty block
-  const uint32_t accumBlocks = std::min(cus * 8u, fh::div_ceil(2u * (uint32_t)n, a.chunk) + 64u);
+  const uint32_t accumBlocks = std::min(fh::persistent_blocks(ctx), fh::div_ceil(2u * (uint32_t)n, a.chunk) + 64u);
   return fh::launch<P2GAccum>(ctx, accumBlocks, 0, {a, nBlocks});
 }
 

@@ -78,6 +78,7 @@ SIGNATURES = {
     "fh_graph_launch": (i32, [vp, vp]),
     "fh_graph_destroy": (i32, [vp]),
     "fh_profiler_read": (i32, [vp, C.POINTER(KernelTime), u32, C.POINTER(u32)]),
+    "fh_set_persistent_blocks": (i32, [vp, u32]),
     "fh_record_begin": (i32, [vp]),
     "fh_record_end": (i32, [vp, C.POINTER(vp)]),
     "fh_program_merge": (i32, [vp, C.POINTER(vp), u32, C.POINTER(vp)]),
@@ -156,6 +157,9 @@ class Context:
         g = vp()
         check(self.lib.fh_capture_end(self.handle, C.byref(g)), "fh_capture_end")
         return Graph(self, g)
+
+    def set_persistent_blocks(self, blocks: int) -> None:
+        check(self.lib.fh_set_persistent_blocks(self.handle, blocks), "fh_set_persistent_blocks")
 
     def record_begin(self) -> None:
         check(self.lib.fh_record_begin(self.handle), "fh_record_begin")

@@ -269,9 +269,7 @@ class HairRenderer:
         self.sdf = SDFCollider(wl.sdf_dims, wl.sdf_bounds, wl.sdf, self.device)
         self.frameIdx = 0
         W, H = wl.width, wl.height
-        self.render_uniforms = L.pack_render_uniforms(
-            rc, sc, wl.camera, W, H, self.model, self.hair.sphere, [(np.zeros(3), 0.3)], wl.sdf_bounds,
-            self.grid.bounds)
+        self.render_uniforms = workload_render_uniforms(wl, self.model)
         self.globalUniforms = _dev_bytes(self.render_uniforms, self.device)
         self.simulationUniforms = _dev_bytes(self.sim_uniforms(0), self.device)
         if inputs is None:
@@ -365,17 +363,27 @@ class HairRenderer:
         finally:
             self.native.set_stream(main)
         self._graphs = graphs
-        self._graphParity = 0
+        self._graphBase = self.hair.current  # graphs[0] reads positions[current] as "now"
         return graphs
 
     def frame_graph(self, upload_uniforms: bool = True) -> None:
-        """One frame replayed from capture_frames()."""
+        """One frame replayed from capture_frames() (the graph of the current
+        parity of the position buffers)."""
         if upload_uniforms:
             self.upload_sim_uniforms()
-        self._graphs[self._graphParity].launch()
-        self._graphParity ^= 1
+        self._graphs[self.hair.current ^ self._graphBase].launch()
         self.hair.swapPositionBuffersAfterSimIntegration()
         self.frameIdx += 1
+
+
+def workload_render_uniforms(wl: A.Workload, model: np.ndarray | None = None) -> np.ndarray:
+    """The render uniforms of a workload (renderUniformsBuffer.ts:205-288):
+    its camera, the hair's bounding sphere, one light at the origin, the SDF
+    and grid bounds."""
+    box = A.bounding_box(wl.positions)
+    return L.pack_render_uniforms(wl.render, wl.sim, wl.camera, wl.width, wl.height,
+                                  np.eye(4) if model is None else model, A.bounding_sphere(box),
+                                  [(np.zeros(3), 0.3)], wl.sdf_bounds, A.scale_box(box, wl.sim.gridScale))
 
 
 def make_inputs(wl: A.Workload, render_uniforms: np.ndarray, shadow_size: int = 2048) -> dict:

@@ -451,6 +451,13 @@ int fh_graph_destroy(fh_graph* graph);
  * own calls compute. */
 typedef struct fh_program fh_program;
 
+/* Caps the workgroups of the grid-stride kernels launched on `ctx` (the tile
+ * pass's queued pieces, the fine pass's gather and sort, the grid
+ * accumulation); 0 restores the default of 8 per CU. An instance batched
+ * with many others needs only its share of the device: the results do not
+ * depend on this value. */
+int fh_set_persistent_blocks(fh_ctx* ctx, uint32_t blocks);
+
 /* Starts recording: until fh_record_end, the entry points called on `ctx`
  * check their arguments and keep their kernel launches (grid, LDS bytes,
  * arguments) instead of enqueuing them. Nothing runs on the device, so the
