@@ -287,6 +287,13 @@ int fh_profiler_read(fh_ctx* ctx, fh_kernel_time* out, uint32_t max_out, uint32_
   return FH_OK;
 }
 
+int fh_scratch_bytes(fh_ctx* ctx, uint64_t* bytes) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(bytes != nullptr, "fh_scratch_bytes: null argument");
+  *bytes = ctx->scratch_bytes;
+  return FH_OK;
+}
+
 int fh_set_persistent_blocks(fh_ctx* ctx, uint32_t blocks) {
   FH_CHECK_CTX(ctx);
   ctx->persist_blocks = blocks;

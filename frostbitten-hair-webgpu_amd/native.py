@@ -79,6 +79,7 @@ SIGNATURES = {
     "fh_graph_destroy": (i32, [vp]),
     "fh_profiler_read": (i32, [vp, C.POINTER(KernelTime), u32, C.POINTER(u32)]),
     "fh_set_persistent_blocks": (i32, [vp, u32]),
+    "fh_scratch_bytes": (i32, [vp, C.POINTER(C.c_uint64)]),
     "fh_record_begin": (i32, [vp]),
     "fh_record_end": (i32, [vp, C.POINTER(vp)]),
     "fh_program_merge": (i32, [vp, C.POINTER(vp), u32, C.POINTER(vp)]),
@@ -160,6 +161,11 @@ class Context:
 
     def set_persistent_blocks(self, blocks: int) -> None:
         check(self.lib.fh_set_persistent_blocks(self.handle, blocks), "fh_set_persistent_blocks")
+
+    def scratch_bytes(self) -> int:
+        n = C.c_uint64(0)
+        check(self.lib.fh_scratch_bytes(self.handle, C.byref(n)), "fh_scratch_bytes")
+        return n.value
 
     def record_begin(self) -> None:
         check(self.lib.fh_record_begin(self.handle), "fh_record_begin")

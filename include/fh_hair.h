@@ -451,6 +451,10 @@ int fh_graph_destroy(fh_graph* graph);
  * own calls compute. */
 typedef struct fh_program fh_program;
 
+/* Device bytes currently held by the context's scratch (for memory
+ * budgets: the scratch grows on first use at a new size). */
+int fh_scratch_bytes(fh_ctx* ctx, uint64_t* bytes);
+
 /* Caps the workgroups of the grid-stride kernels launched on `ctx` (the tile
  * pass's queued pieces, the fine pass's gather and sort, the grid
  * accumulation); 0 restores the default of 8 per CU. An instance batched
