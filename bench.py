@@ -20,6 +20,7 @@ on full frames of the same workload, rank 0 only.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -243,6 +244,39 @@ def batched_vs_plain(n: int, steps: int, warmup: int, tune: bool, dev, log) -> d
     }
 
 
+def served(n_req: int, rate: float, budget_gib: float, dev, log) -> dict:
+    """The served form against sequential serving of the same request stream:
+    `n_req` Poisson arrivals at `rate` per second, each for an asset of the
+    256-asset pool (strands log-uniform in [1k, 100k], 16-32 points, 720p /
+    1080p / 4K) and U{1..120} frames; instance state stays in HBM between
+    requests, within a `budget_gib` GiB budget. Continuous batching runs one
+    merged frame of every admitted request per tick; sequential serving runs
+    one request at a time through the same loop. Asset data (strands and the
+    precomputed depth / AO / shadow inputs) is generated on the host first;
+    instance creation on the device is part of serving."""
+    from frostbitten_hair_webgpu_amd import assets as A
+    from frostbitten_hair_webgpu_amd import passes as PS
+    from frostbitten_hair_webgpu_amd import serve as SV
+    t0 = time.perf_counter()
+    reqs = SV.poisson_requests(n_req, rate, list(range(256)), 120, seed=A.SERVE_SEED)
+    pool = {}
+    for a in sorted({q.asset for q in reqs}):
+        wl = A.make_pool_asset(a)
+        pool[a] = (wl, PS.make_inputs(wl, PS.workload_render_uniforms(wl), 1024))
+    log(f"# serve: {len(pool)} assets generated in {time.perf_counter() - t0:.1f} s")
+    out = {"requests": n_req, "rate_per_s": rate, "assets": len(pool), "budget_GiB": budget_gib}
+    for mode, max_active in (("continuous", 64), ("sequential", 1)):
+        srv = SV.HairServer(pool, int(budget_gib * (1 << 30)), max_active=max_active, device=dev)
+        rs = srv.run([dataclasses.replace(q) for q in reqs])
+        out[mode] = dict(SV.summarize(rs, srv.elapsed), **{k: v for k, v in srv.stats.items()},
+                         host_s={k: round(v, 3) for k, v in srv.times.items()})
+        srv.close()
+        del srv
+        log(f"# serve {mode}: {out[mode]['frames_per_s']} frames/s, p50 {out[mode]['latency_s']['p50']} s")
+    out["speedup_vs_sequential"] = round(out["continuous"]["frames_per_s"] / out["sequential"]["frames_per_s"], 3)
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,6 +287,10 @@ def main() -> None:
     ap.add_argument("--batch-instances", type=int, default=64,
                     help="instances of the batched-versus-plain comparison (0: skip it)")
     ap.add_argument("--no-tune", action="store_true", help="batched comparison without the autotuner")
+    ap.add_argument("--serve-requests", type=int, default=32,
+                    help="requests of the served-versus-sequential comparison (0: skip it)")
+    ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrival rate, requests per second")
+    ap.add_argument("--serve-budget-gib", type=float, default=8.0, help="HBM budget of the resident instances")
     ap.add_argument("--no-graph", action="store_true", help="launch every call eagerly instead of replaying "
                     "the frame as a HIP graph")
     ap.add_argument("--no-kernel-timing", action="store_true",
@@ -439,6 +477,9 @@ def main() -> None:
         del r
         out["batched"] = batched_vs_plain(args.batch_instances, min(args.steps, 20), 2, not args.no_tune, dev,
                                           lambda m: print(m, file=sys.stderr, flush=True))
+    if args.serve_requests:
+        out["served"] = served(args.serve_requests, args.serve_rate, args.serve_budget_gib, dev,
+                               lambda m: print(m, file=sys.stderr, flush=True))
     print(json.dumps(out), flush=True)
 
 

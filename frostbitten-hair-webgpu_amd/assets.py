@@ -282,6 +282,17 @@ def bounding_box(positions: np.ndarray) -> tuple:
     return p.min(axis=0).astype(np.float64), p.max(axis=0).astype(np.float64)
 
 
+def derived(wl: Workload) -> dict:
+    """Host buffers derived from a workload's initial pose (bounding box,
+    tangents, rest lengths), computed once per workload object."""
+    d = getattr(wl, "_derived", None)
+    if d is None:
+        d = dict(box=bounding_box(wl.positions), tangents=tangents(wl.positions, wl.strands, wl.points),
+                 seglen=segment_lengths(wl.positions, wl.strands, wl.points))
+        wl._derived = d
+    return d
+
+
 def bounding_sphere(box: tuple) -> tuple:
     """calcBoundingSphere — utils/bounds.ts:101-108."""
     c = (box[0] + box[1]) * 0.5

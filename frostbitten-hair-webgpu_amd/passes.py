@@ -38,16 +38,20 @@ class HairObject:
     """src/scene/hair/hairObject.ts:29-155 — strand buffers and the tick-tock
     position pair (swap after integration, :102-106)."""
 
-    def __init__(self, name: str, positions: np.ndarray, strands: int, points: int, device):
+    def __init__(self, name: str, positions: np.ndarray, strands: int, points: int, device,
+                 derived: dict | None = None):
         self.name = name
         self.strandsCount = strands
         self.pointsPerStrand = points
-        self.box = A.bounding_box(positions)
+        if derived is None:
+            derived = dict(box=A.bounding_box(positions), tangents=A.tangents(positions, strands, points),
+                           seglen=A.segment_lengths(positions, strands, points))
+        self.box = derived["box"]
         self.sphere = A.bounding_sphere(self.box)
         self.initialPositions = _dev(positions.astype(np.float32), device)
         self.positions = [self.initialPositions.clone(), self.initialPositions.clone()]
-        self.tangents = _dev(A.tangents(positions, strands, points), device)
-        self.segmentLengths = _dev(A.segment_lengths(positions, strands, points), device)
+        self.tangents = _dev(derived["tangents"], device)
+        self.segmentLengths = _dev(derived["seglen"], device)
         self.hairData = _dev(L.pack_hair_data(self.sphere[0], self.sphere[1], strands, points), device)
         self.shading = None
         self.current = 0
@@ -263,7 +267,7 @@ class HairRenderer:
         rc, sc = wl.render, wl.sim
         self.cfg = N.RenderCfg.from_config(rc)
         self.model = np.eye(4)
-        self.hair = HairObject("hair", wl.positions, wl.strands, wl.points, self.device)
+        self.hair = HairObject("hair", wl.positions, wl.strands, wl.points, self.device, derived=A.derived(wl))
         self.hair.allocShading(rc.shadingPoints, self.device)
         self.grid = GridData(self.hair.box, sc.gridDims, sc.gridScale, self.device)
         self.sdf = SDFCollider(wl.sdf_dims, wl.sdf_bounds, wl.sdf, self.device)
@@ -380,7 +384,7 @@ def workload_render_uniforms(wl: A.Workload, model: np.ndarray | None = None) ->
     """The render uniforms of a workload (renderUniformsBuffer.ts:205-288):
     its camera, the hair's bounding sphere, one light at the origin, the SDF
     and grid bounds."""
-    box = A.bounding_box(wl.positions)
+    box = A.derived(wl)["box"]
     return L.pack_render_uniforms(wl.render, wl.sim, wl.camera, wl.width, wl.height,
                                   np.eye(4) if model is None else model, A.bounding_sphere(box),
                                   [(np.zeros(3), 0.3)], wl.sdf_bounds, A.scale_box(box, wl.sim.gridScale))

@@ -113,12 +113,15 @@ class HairServer:
         self.merged: OrderedDict = OrderedDict()       # ((asset, parity), ...) -> merged Program
         self.used = 0
         self.stats = dict(ticks=0, frames=0, evictions=0, materializations=0, merges=0)
+        # host seconds spent per activity of the loop
+        self.times = dict(materialize=0.0, evict=0.0, merge=0.0, uniforms=0.0, launch=0.0, wait=0.0)
 
     # -- residency -------------------------------------------------------
     def _bytes_of(self, asset: int) -> int:
         return self.sizes.get(asset) or estimate_bytes(self.pool[asset][0])
 
     def _evict(self, asset: int) -> None:
+        t = time.perf_counter()
         inst = self.resident.pop(asset)
         r = inst.r
         torch.cuda.synchronize(self.device)
@@ -133,6 +136,7 @@ class HairServer:
         r.native.close()
         del inst, r
         self.stats["evictions"] += 1
+        self.times["evict"] += time.perf_counter() - t
 
     def _make_room(self, need: int, busy: set) -> bool:
         for asset in list(self.resident):
@@ -143,6 +147,7 @@ class HairServer:
         return self.used + need <= self.budget
 
     def _materialize(self, asset: int) -> _Instance:
+        t = time.perf_counter()
         wl, inputs = self.pool[asset]
         ctx = N.Context(self.dev)
         ctx.set_stream(torch.cuda.current_stream(self.device))
@@ -168,6 +173,7 @@ class HairServer:
         self.resident[asset] = inst
         self.used += nbytes
         self.stats["materializations"] += 1
+        self.times["materialize"] += time.perf_counter() - t
         return inst
 
     # -- one tick ----------------------------------------------------------
@@ -175,26 +181,37 @@ class HairServer:
         key = tuple(sorted((i.asset, i.parity()) for i in insts))
         prog = self.merged.get(key)
         if prog is None:
+            t = time.perf_counter()
             ordered = sorted(insts, key=lambda i: i.asset)
             prog = self.native.program_merge([i.program() for i in ordered])
             self.merged[key] = prog
             self.stats["merges"] += 1
             while len(self.merged) > 128:
                 self.merged.popitem(last=False)[1].close()
+            self.times["merge"] += time.perf_counter() - t
         else:
             self.merged.move_to_end(key)
         return prog
 
     def _tick(self, active: dict) -> None:
+        t0 = time.perf_counter()
         insts = [self.resident[a] for a in active]
         for inst in insts:
             self.host_slots[inst.slot].numpy()[:] = inst.r.sim_uniforms(inst.r.frameIdx).view(np.uint8).reshape(-1)
         self.slots.copy_(self.host_slots, non_blocking=True)
-        self._merged_for(insts).launch(self.native)
+        t1 = time.perf_counter()
+        prog = self._merged_for(insts)
+        t2 = time.perf_counter()
+        prog.launch(self.native)
         for inst in insts:
             inst.r.hair.swapPositionBuffersAfterSimIntegration()
             inst.r.frameIdx += 1
+        t3 = time.perf_counter()
         torch.cuda.synchronize(self.device)
+        t4 = time.perf_counter()
+        self.times["uniforms"] += t1 - t0
+        self.times["launch"] += t3 - t2
+        self.times["wait"] += t4 - t3
         self.stats["ticks"] += 1
         self.stats["frames"] += len(insts)
 

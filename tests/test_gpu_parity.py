@@ -20,10 +20,11 @@ def _np(t):
     return t.detach().cpu().numpy()
 
 
-def _renderer(kind="small", strands=512, width=320, height=240, tile=None, bins=None, slices=None, seed=None):
+def _renderer(kind="small", strands=512, width=320, height=240, tile=None, bins=None, slices=None, seed=None,
+              points=None):
     from frostbitten_hair_webgpu_amd import assets as A
     from frostbitten_hair_webgpu_amd import passes as PS
-    wl = A.make_workload(kind, seed=seed, strands=strands, width=width, height=height)
+    wl = A.make_workload(kind, seed=seed, strands=strands, width=width, height=height, points=points)
     if tile:
         wl.render.tileSize = tile
     if bins:
@@ -119,12 +120,16 @@ def test_grid_pre_sim(gpu, oracle, kind, strands):
     assert np.allclose(got, exp, rtol=1e-6, atol=0.0)
 
 
-@pytest.mark.parametrize("kind,strands,steps", [("small", 512, 0), ("small", 2000, 5), ("c2", 700, 3)])
-def test_sim_integrate_step(gpu, oracle, kind, strands, steps):
-    """One step from an identical state: positions within 1e-5 relative."""
+@pytest.mark.parametrize("kind,strands,steps,points", [("small", 512, 0, None), ("small", 2000, 5, None),
+                                                        ("c2", 700, 3, None), ("c2", 300, 2, 5), ("c2", 300, 2, 24),
+                                                        ("c2", 200, 2, 40), ("c2", 130, 2, 64)])
+def test_sim_integrate_step(gpu, oracle, kind, strands, steps, points):
+    """One step from an identical state: positions within 1e-5 relative.
+    Strands of up to 32 points take the register-resident sweeps, longer
+    ones (40, 64) the LDS-staged kernel."""
     import torch
     from frostbitten_hair_webgpu_amd import assets as A
-    wl, r = _renderer(kind, strands)
+    wl, r = _renderer(kind, strands, points=points)
     _step_to(r, steps)
     r.upload_sim_uniforms()
     ctx = r.ctx()
