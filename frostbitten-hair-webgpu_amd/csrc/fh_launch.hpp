@@ -51,6 +51,27 @@ __global__ __launch_bounds__(K::kThreads) void k_batch(uint64_t argsAddr, uint64
 #endif
 }
 
+// The device hands out workgroups round-robin over its 8 XCDs (each with its
+// own L2): workgroup b runs on XCD b % 8. xcd_order(block, blocks) renumbers
+// the blocks of a (plain or batched) grid so that the blocks on one XCD take
+// consecutive numbers: a kernel whose neighbouring blocks touch neighbouring
+// data (strands next to each other, hence screen tiles, grid cells and
+// shadow texels next to each other) then keeps that data in one L2. It is a
+// bijection on [0, blocks).
+constexpr uint32_t kXcds = 8;
+
+__device__ __forceinline__ uint32_t xcd_order(uint32_t block, uint32_t blocks) {
+  const uint32_t o = (blockIdx.x - block) % kXcds;  // XCD of the grid's (instance's) first block
+  const uint32_t v = (block + o) % kXcds;           // XCD of this block
+  uint32_t before = 0;
+  for (uint32_t x = 0; x < v; x++) {
+    const uint32_t f = (x + kXcds - o) % kXcds;      // first block on XCD x
+    if (f < blocks) before += (blocks - f + kXcds - 1u) / kXcds;
+  }
+  const uint32_t f = (v + kXcds - o) % kXcds;
+  return before + (block - f) / kXcds;
+}
+
 // Dynamic LDS above the default limit must be allowed per kernel.
 template <class Fn>
 inline void allow_lds(Fn* fn, uint32_t shmem, uint32_t& allowed) {

@@ -494,7 +494,7 @@ struct TilesDraw {
   using Args = TilesArgs;
   static constexpr uint32_t kThreads = 64;
   static constexpr const char* kName = "tiles_draw";
-  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { tiles_draw_body(a, block, blocks); }
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { tiles_draw_body(a, xcd_order(block, blocks), blocks); }
 };
 
 

@@ -231,7 +231,7 @@ struct Shading {
   using Args = ShadeArgs;
   static constexpr uint32_t kThreads = 256;
   static constexpr const char* kName = "shading";
-  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { shading_body(a, block, blocks); }
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { shading_body(a, xcd_order(block, blocks), blocks); }
 };
 
 

@@ -360,7 +360,7 @@ struct SimIntegrate {
   };
   static constexpr uint32_t kThreads = kSimThreads;
   static constexpr const char* kName = "sim_integrate";
-  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { sim_integrate_body(a.d, a.u, a.prev, a.now, a.seglen, a.sdf, a.sdf_dims, a.dv, a.dgw, a.initial, a.S, a.P, block, blocks); }
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { sim_integrate_body(a.d, a.u, a.prev, a.now, a.seglen, a.sdf, a.sdf_dims, a.dv, a.dgw, a.initial, a.S, a.P, xcd_order(block, blocks), blocks); }
 };
 
 
