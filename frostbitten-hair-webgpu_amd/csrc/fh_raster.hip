@@ -72,6 +72,56 @@ struct TilesArgs {
   uint2* pieces;           // overflow queue: (segment, first pair) of wide segments
   uint32_t* pieceCount;
   uint32_t pieceCap;
+  const float2* depthBounds;  // per tile: (min, max) of the mesh depth its pixels read
+  uint32_t nBounds;
+};
+
+// Per tile, the (min, max) of every depth texel that a pixel of the tile reads
+// (tex_load at (x, vpH - y), 0 outside the texture), over the whole tile
+// rectangle. A hair depth <= min passes the depth test and one > max fails it
+// at every pixel of the tile, without a load; others load their texel. A NaN
+// texel makes the bounds (-inf, +inf), so that every pixel loads.
+struct TileDepthBounds {
+  struct Args {
+    const fh_render_uniforms* u;
+    const float* depth;
+    uint32_t depthW, depthH, T, nTiles;
+    float2* bounds;
+  };
+  static constexpr uint32_t kThreads = 256;
+  static constexpr const char* kName = "tile_depth_bounds";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) {
+    __shared__ float smin[kThreads / 64], smax[kThreads / 64];
+    __shared__ uint32_t snan[kThreads / 64];
+    const TileGeom tg = tile_geom(mk2(a.u->viewport[0], a.u->viewport[1]), a.T);
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    for (uint32_t t = block; t < a.nTiles; t += blocks) {
+      const uint32_t ty = t / max(tg.tilesX, 1u), tx = t - ty * max(tg.tilesX, 1u);
+      float lo = INFINITY, hi = -INFINITY;
+      bool nan = false;
+      for (uint32_t i = threadIdx.x; i < a.T * a.T; i += kThreads) {
+        const uint32_t x = tx * a.T + i % a.T, y = ty * a.T + i / a.T;
+        const float d = tex_load(a.depth, a.depthW, a.depthH, (int32_t)x, (int32_t)(tg.vpH - y));
+        nan |= d != d;
+        lo = fminf(lo, d);
+        hi = fmaxf(hi, d);
+      }
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) {
+        lo = fminf(lo, __shfl_xor(lo, o));
+        hi = fmaxf(hi, __shfl_xor(hi, o));
+      }
+      const uint64_t anyNan = __ballot(nan);
+      if (lane == 0) { smin[w] = lo; smax[w] = hi; snan[w] = anyNan != 0; }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        bool n = false;
+        for (uint32_t k = 0; k < kThreads / 64; k++) { lo = fminf(lo, smin[k]); hi = fmaxf(hi, smax[k]); n |= snan[k] != 0; }
+        a.bounds[t] = n ? make_float2(-INFINITY, INFINITY) : make_float2(lo, hi);
+      }
+      __syncthreads();
+    }
+  }
 };
 
 // A segment whose rectangle has more than kTilePiece tiles is not drawn by its
@@ -233,6 +283,7 @@ struct TileWaveLds {
 __device__ __forceinline__ void tiles_shade_queue(const TilesArgs& a, TileWaveLds& L, uint32_t head,
                                                   uint32_t n, const TileGeom& tg, f2 dbVS) {
   const uint32_t lane = __lane_id();
+  const uint32_t tshift = (uint32_t)__builtin_ctz(a.T);
   uint32_t slot = 0xffffffffu, vmin = 0xffffffffu, vmax = 0u, vbin = 0xffffffffu;
   if (lane < n) {
     const uint32_t e = L.queue[(head + lane) & (kTileQueue - 1u)];
@@ -243,8 +294,15 @@ __device__ __forceinline__ void tiles_shade_queue(const TilesArgs& a, TileWaveLd
     const f2 p = mk2((float)x, (float)y);
     const f2 w = interp_quad_pre<64>(&L.ip[0][o], mk2(L.v[0][o], L.v[1][o]), p);
     const float hdp = interp_f32(w, mk4(L.d[0][o], L.d[1][o], L.d[2][o], L.d[3][o]));
-    const float db = tex_load(a.depth, a.depthW, a.depthH, (int32_t)x, (int32_t)(tg.vpH - y));
-    if (!(hdp > db)) {  // depth test with GL_LESS (:193-195)
+    // depth test with GL_LESS (:193-195), decided by the tile's depth bounds
+    // when they settle it, by the texel otherwise
+    const uint32_t tile = (y >> tshift) * tg.tilesX + (x >> tshift);
+    const float2 bd = tile < a.nBounds ? a.depthBounds[tile] : make_float2(-INFINITY, INFINITY);
+    bool pass;
+    if (hdp <= bd.x) pass = true;
+    else if (hdp > bd.y) pass = false;
+    else pass = !(hdp > tex_load(a.depth, a.depthW, a.depthH, (int32_t)x, (int32_t)(tg.vpH - y)));
+    if (pass) {
       const f3 vs = project(a.u->projMatrixInv, mk4(p.x, p.y, hdp, 1.0f));
       vbin = (a.B - 1u) - depth_bin(a.B, dbVS, vs.z);
       // fminf/fmaxf in the reference skip a NaN depth; its bin still counts
@@ -1452,14 +1510,21 @@ int fh_tiles_draw(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t
   // up to 4 Mi pieces; a segment whose pieces do not fit is drawn by its wave
   const uint64_t maxTiles = std::min<uint64_t>(cfg->invalidTilesPerSegmentThreshold, fh_tile_count(cfg, vp_w, vp_h));
   a.pieceCap = (uint32_t)std::min<uint64_t>(nSeg * ((maxTiles + kTilePiece - 1) / kTilePiece), 4u << 20);
-  if ((rc = ensure_scratch(ctx, 256 + (size_t)a.pieceCap * 8))) return rc;
+  const uint32_t nTiles = fh_tile_count(cfg, vp_w, vp_h);
+  const size_t boundsOff = fh::align_up(256 + (size_t)a.pieceCap * 8, 256);
+  if ((rc = ensure_scratch(ctx, boundsOff + (size_t)nTiles * sizeof(float2)))) return rc;
   a.pieceCount = reinterpret_cast<uint32_t*>(ctx->scratch);
   a.pieces = reinterpret_cast<uint2*>(reinterpret_cast<char*>(ctx->scratch) + 256);
+  a.depthBounds = reinterpret_cast<const float2*>(reinterpret_cast<char*>(ctx->scratch) + boundsOff);
+  a.nBounds = nTiles;
   if ((rc = fill_u32(ctx, a.pieceCount, 1, 0u))) return rc;
+  if ((rc = launch<TileDepthBounds>(ctx, std::min(nTiles, fh::persistent_blocks(ctx)), 0,
+                                    {uniforms, depth.data, depth.width, depth.height, cfg->tileSize, nTiles,
+                                     const_cast<float2*>(a.depthBounds)})))
+    return rc;
   if ((rc = launch<TilesDraw>(ctx, fh::div_ceil((uint32_t)nSeg, 64), 0, a))) return rc;
   if ((rc = launch<TilesPieces>(ctx, std::min(fh::div_ceil((uint32_t)nSeg, 64), fh::persistent_blocks(ctx)), 0, a)))
     return rc;
-  const uint32_t nTiles = fh_tile_count(cfg, vp_w, vp_h);
   return launch<TileCounts>(ctx, fh::div_ceil(nTiles, 256), 0, {tiles, nTiles, cfg->tileDepthBins, segment_count_per_tile});
 }
 
