@@ -1037,7 +1037,8 @@ enum RecField : uint32_t {
   kRecBox0,                      // pixel box relative to the tile, inclusive, clamped to
   kRecBox1,                      //   [-1, TILE]: (x0 | y0 << 16), (x1 | y1 << 16) as int16
   kRecNear,                      // lower bound of the interpolated depth (-inf when unknown)
-  kRecFields
+  kRecIp,                        // kIpFields: per-segment interpolation terms (interp_pre)
+  kRecFields = kRecIp + kIpFields
 };
 
 template <uint32_t RB>
@@ -1089,12 +1090,9 @@ __device__ __forceinline__ void fine_shade_queue(const FineArgs& a, const FineLd
     j = e >> 8;
     pix = w * FineLds<TILE>::PXW + (e & 0xffu);
     const f2 p = mk2((float)(bx0 + pix % TILE), (float)(by0 + pix / TILE));
-    Seg sg;
-    sg.v00 = mk2(R.at(kRecV + 0, j), R.at(kRecV + 1, j)); sg.v01 = mk2(R.at(kRecV + 2, j), R.at(kRecV + 3, j));
-    sg.v10 = mk2(R.at(kRecV + 4, j), R.at(kRecV + 5, j)); sg.v11 = mk2(R.at(kRecV + 6, j), R.at(kRecV + 7, j));
-    sg.depths = mk4(R.at(kRecD + 0, j), R.at(kRecD + 1, j), R.at(kRecD + 2, j), R.at(kRecD + 3, j));
-    wq = interp_quad(sg, p);
-    hdp = interp_f32(wq, sg.depths);
+    // interpolateHairQuad from the record's per-segment terms (bit-identical)
+    wq = interp_quad_pre<fine_batch<TILE>()>(&R.at(kRecIp, j), mk2(R.at(kRecV + 0, j), R.at(kRecV + 1, j)), p);
+    hdp = interp_f32(wq, mk4(R.at(kRecD + 0, j), R.at(kRecD + 1, j), R.at(kRecD + 2, j), R.at(kRecD + 3, j)));
     accept = !(hdp >= L.db[pix]);  // depth test with GL_LESS (:31-34)
   }
   // a slot of the slice memory: _hasMoreSliceDataSlots (hairFinePass.wgsl.ts:252-264)
@@ -1342,6 +1340,7 @@ __device__ __forceinline__ void fine_raster_body(FineArgs a, uint32_t block, uin
           const float dm = fmaxf(fmaxf(fabsf(sg.depths.x), fabsf(sg.depths.y)),
                                  fmaxf(fabsf(sg.depths.z), fabsf(sg.depths.w)));
           R.at(kRecNear, tid) = finite ? dn - (dm * 1e-6f + 1e-30f) : -INFINITY;
+          interp_pre(sg, &R.at(kRecIp, tid), RB);
         }
         __syncthreads();
         fine_cheap_batch<TILE>(a, L, gpool, nb, bx0, by0, td, invSegs, &s_sliceOff);
