@@ -1,0 +1,100 @@
+"""Edge cases of the pass entry points: no strands, list-capacity overflow,
+one-point-wide viewports."""
+import numpy as np
+import pytest
+
+from gpu_util import walk_lists
+
+pytestmark = pytest.mark.gpu
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def test_zero_strands_is_a_no_op(gpu):
+    """Every pass accepts zero strands: nothing is drawn, the frame's clears
+    still happen (an empty framebuffer, empty tile lists)."""
+    import torch
+    from frostbitten_hair_webgpu_amd import assets as A
+    from frostbitten_hair_webgpu_amd import native as N
+    from frostbitten_hair_webgpu_amd import passes as PS
+    wl = A.make_workload("small", strands=64, width=160, height=120)
+    r = PS.HairRenderer(wl, "cuda:0", shadow_size=128)
+    r.beforeFirstFrame()
+    r.finePass.hairRasterizerResultsBuffer.fill_(7.0)
+    ctx = r.ctx()
+    g, s, h = r.grid, r.sdf, r.hair
+    ctx.native.call("fh_grid_pre_sim", g.dims, N.ptr(ctx.simulationUniforms), N.ptr(g.densityVelocity),
+                    N.ptr(g.densityGradAndWind), N.ptr(s.texture), s.dims)
+    ctx.native.call("fh_sim_integrate", g.dims, N.ptr(ctx.simulationUniforms), N.ptr(h.hairData),
+                    N.ptr(h.positionsPrev), N.ptr(h.positionsNow), N.ptr(h.segmentLengths), N.ptr(s.texture),
+                    s.dims, N.ptr(g.densityVelocity), N.ptr(g.densityGradAndWind), N.ptr(h.initialPositions),
+                    0, h.pointsPerStrand)
+    ctx.native.call("fh_grid_post_sim", g.dims, N.ptr(ctx.simulationUniforms), N.ptr(h.hairData),
+                    N.ptr(h.positionsPrev), N.ptr(h.positionsNow), N.ptr(g.densityVelocity), 0, h.pointsPerStrand)
+    r.tilesPass.cmdClearBeforeRender(ctx)
+    r.finePass.cmdClearBeforeRender(ctx)
+    r.sortPass.cmdClearBeforeRender(ctx)
+    w, hh = ctx.viewport
+    tp = r.tilesPass
+    ctx.native.call("fh_tiles_draw", ctx.cfg, w, hh, N.ptr(ctx.globalUniforms), N.ptr(h.hairData),
+                    N.ptr(h.positionsNow), N.ptr(h.tangents), N.ptr(tp.hairTilesBuffer), ctx.depthTexture,
+                    N.ptr(tp.hairTileSegmentsBuffer), tp.segmentsCapacity, N.ptr(tp.segmentCountPerTileBuffer),
+                    N.ptr(tp.slotTileBinBuffer), 0, h.pointsPerStrand)
+    r.sortPass.cmdSortHairTiles(ctx)
+    fp = r.finePass
+    ctx.native.call("fh_fine_raster", ctx.cfg, w, hh, N.ptr(ctx.globalUniforms), N.ptr(h.hairData),
+                    N.ptr(h.positionsNow), N.ptr(ctx.hairTilesBuffer), N.ptr(ctx.hairTileSegmentsBuffer),
+                    tp.segmentsCapacity, N.ptr(fp.hairSlicesDataBuffer), N.ptr(fp.hairRasterizerResultsBuffer),
+                    ctx.depthTexture, N.ptr(h.shading), N.ptr(h.tangents), N.ptr(ctx.hairTileListBuffer),
+                    N.ptr(tp.slotTileBinBuffer), 0, h.pointsPerStrand)
+    ctx.native.call("fh_shading", ctx.cfg, N.ptr(ctx.globalUniforms), N.ptr(h.hairData), N.ptr(h.positionsNow),
+                    N.ptr(h.tangents), N.ptr(h.shading), ctx.shadowDepthTexture, ctx.aoTexture,
+                    ctx.shadingDepthTexture, 0, h.pointsPerStrand)
+    torch.cuda.synchronize()
+    assert not _np(tp.hairTilesBuffer).any()
+    assert _np(tp.hairTileSegmentsBuffer)[0] == 0
+    assert not _np(tp.segmentCountPerTileBuffer).any()
+    assert _np(r.sortPass.tileListBuffer)[0] == 0
+    assert not _np(fp.hairRasterizerResultsBuffer)[4:].any()
+
+
+def test_list_capacity_overflow(gpu, oracle):
+    """With room for fewer list elements than hit (tile, segment) pairs
+    (hairTileSegmentsBuffer.ts:81-104), the counter still counts every hit;
+    exactly `capacity` pairs are stored, each in the list of its (tile, bin),
+    and the per-tile counts add up to the stored pairs. Which pairs are kept
+    follows the order of the atomic appends, as in the reference."""
+    import torch
+    from frostbitten_hair_webgpu_amd import assets as A
+    from frostbitten_hair_webgpu_amd import layouts as L
+    from frostbitten_hair_webgpu_amd import passes as PS
+    wl = A.make_workload("small", strands=800, width=320, height=240)
+    wl.render.avgSegmentsPerTile = 0.5
+    r = PS.HairRenderer(wl, "cuda:0", shadow_size=256)
+    r.beforeFirstFrame()
+    r.frame()
+    ctx = r.ctx()
+    tp = r.tilesPass
+    r.tilesPass.cmdClearBeforeRender(ctx)
+    r.tilesPass.cmdDrawHairToTiles(ctx, r.hair)
+    torch.cuda.synchronize()
+    cap = tp.segmentsCapacity
+    tiles = _np(tp.hairTilesBuffer).view(np.uint32)
+    seg = _np(tp.hairTileSegmentsBuffer).view(np.uint32)
+    counts = _np(tp.segmentCountPerTileBuffer).view(np.uint32)
+    hd = L.pack_hair_data(r.hair.sphere[0], r.hair.sphere[1], wl.strands, wl.points)
+    _, es, _, _ = oracle.tiles_draw(wl.render, wl.width, wl.height, r.render_uniforms, hd,
+                                    _np(r.hair.positionsNow), _np(r.hair.tangents),
+                                    np.ascontiguousarray(r.inputs["meshDepth"].cpu().numpy()), cap, wl.strands,
+                                    wl.points)
+    assert seg[0] == es[0] and seg[0] > cap
+    lists = walk_lists(tiles, seg, cap)
+    assert sum(len(v) for v in lists.values()) == cap
+    t4 = tiles.reshape(-1, 4)
+    for tb, vals in lists.items():
+        assert len(vals) == t4[tb, 3]
+    assert counts.sum() == cap
+    B = wl.render.tileDepthBins
+    assert np.array_equal(counts, t4[:, 3].reshape(-1, B).sum(axis=1))

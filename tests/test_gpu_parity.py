@@ -169,10 +169,13 @@ def _check_slot_tags(tags_ranks, counts, n):
     assert np.array_equal(sr, np.arange(n) - run_start)
 
 
-@pytest.mark.parametrize("tile,bins", [(16, 4), (8, 8), (32, 4)])
-def test_tiles_bit_exact(gpu, oracle, tile, bins):
+@pytest.mark.parametrize("tile,bins,width,height", [(16, 4, 320, 240), (8, 8, 320, 240), (32, 4, 320, 240),
+                                                     (16, 4, 333, 171), (8, 8, 250, 97)])
+def test_tiles_bit_exact(gpu, oracle, tile, bins, width, height):
+    """Tile depth ranges, per-(tile, bin) counts and list contents equal the
+    oracle's; the last two cases have partial tiles on both edges."""
     import torch
-    wl, r = _renderer("small", 800, 320, 240, tile=tile, bins=bins)
+    wl, r = _renderer("small", 800, width, height, tile=tile, bins=bins)
     _step_to(r, 2)
     ctx = r.ctx()
     r.tilesPass.cmdClearBeforeRender(ctx)
@@ -223,10 +226,11 @@ def test_tile_sort(gpu, oracle):
     assert (np.diff(bucket.astype(np.int64)) <= 0).all()
 
 
-@pytest.mark.parametrize("tile,bins,slices", [(16, 4, 4), (8, 8, 8), (32, 4, 4)])
-def test_fine_pass_image(gpu, oracle, tile, bins, slices):
+@pytest.mark.parametrize("tile,bins,slices,width,height", [(16, 4, 4, 320, 240), (8, 8, 8, 320, 240),
+                                                            (32, 4, 4, 320, 240), (16, 4, 4, 333, 171)])
+def test_fine_pass_image(gpu, oracle, tile, bins, slices, width, height):
     import torch
-    wl, r = _renderer("small", 1200, 320, 240, tile=tile, bins=bins, slices=slices)
+    wl, r = _renderer("small", 1200, width, height, tile=tile, bins=bins, slices=slices)
     _step_to(r, 2)
     ctx = r.ctx()
     r.tilesPass.cmdClearBeforeRender(ctx)
