@@ -300,7 +300,8 @@ def main() -> None:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the hair chain has no multi-GPU path here (asked for --gpus "
-              f"{args.gpus}); see DESIGN.md for how it would shard.", file=sys.stderr)
+              f"{args.gpus}). Sharded, the simulation would split strands (one int32 all-reduce of the grid "
+              f"per step) and the render would split screen tiles.", file=sys.stderr)
         sys.exit(2)
 
     import torch
