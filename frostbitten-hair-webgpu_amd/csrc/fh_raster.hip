@@ -276,6 +276,22 @@ struct TileWaveLds {
   uint32_t queue[kTileQueue];
 };
 
+// One step of a segmented inclusive (min, max, min) scan: combine with the
+// lane the DPP control selects when it holds the same slot. Lanes without a
+// source keep the identities.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void seg_step(uint32_t slot, uint32_t& vmin, uint32_t& vmax, uint32_t& vbin) {
+  const uint32_t os = (uint32_t)__builtin_amdgcn_update_dpp((int)0xffffffff, (int)slot, CTRL, ROWMASK, 0xf, false);
+  const uint32_t omin = (uint32_t)__builtin_amdgcn_update_dpp((int)0xffffffff, (int)vmin, CTRL, ROWMASK, 0xf, false);
+  const uint32_t omax = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)vmax, CTRL, ROWMASK, 0xf, false);
+  const uint32_t obin = (uint32_t)__builtin_amdgcn_update_dpp((int)0xffffffff, (int)vbin, CTRL, ROWMASK, 0xf, false);
+  if (os == slot) {
+    vmin = min(vmin, omin);
+    vmax = max(vmax, omax);
+    vbin = min(vbin, obin);
+  }
+}
+
 // Step 4: the queued inside pixels [head, head + n), n <= 64, one per lane.
 // Queued pixels of one pair are consecutive, so the lanes of a run reduce
 // their (min depth, max depth, min bin) among themselves and only the run's
@@ -309,19 +325,15 @@ __device__ __forceinline__ void tiles_shade_queue(const TilesArgs& a, TileWaveLd
       if (hdp == hdp) vmin = vmax = f32_ord(hdp);
     }
   }
-  // segmented inclusive min/max over runs of equal slot
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t os = (uint32_t)__shfl_up((int)slot, o);
-    const uint32_t omin = (uint32_t)__shfl_up((int)vmin, o);
-    const uint32_t omax = (uint32_t)__shfl_up((int)vmax, o);
-    const uint32_t obin = (uint32_t)__shfl_up((int)vbin, o);
-    if (lane >= (uint32_t)o && os == slot) {
-      vmin = min(vmin, omin);
-      vmax = max(vmax, omax);
-      vbin = min(vbin, obin);
-    }
-  }
+  // segmented inclusive min/max over runs of equal slot, with the DPP steps
+  // of wave_incl_scan (runs are contiguous, so a lane that matches the slot
+  // of a lane before it belongs to the same run)
+  seg_step<0x111, 0xf>(slot, vmin, vmax, vbin);
+  seg_step<0x112, 0xf>(slot, vmin, vmax, vbin);
+  seg_step<0x114, 0xf>(slot, vmin, vmax, vbin);
+  seg_step<0x118, 0xf>(slot, vmin, vmax, vbin);
+  seg_step<0x142, 0xa>(slot, vmin, vmax, vbin);
+  seg_step<0x143, 0xc>(slot, vmin, vmax, vbin);
   const uint32_t nextSlot = (uint32_t)__shfl_down((int)slot, 1);
   if (lane < n && (lane == n - 1u || nextSlot != slot)) {
     if (vmin != 0xffffffffu) {
