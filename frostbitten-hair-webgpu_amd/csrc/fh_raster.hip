@@ -402,11 +402,11 @@ __device__ __forceinline__ void tiles_wave(const TilesArgs& a, TileWaveLds& L, u
     // marks the part that fits as empty and draws its segment here
     const uint32_t np = nTiles > kTilePiece ? (nTiles + kTilePiece - 1u) / kTilePiece : 0u;
     const uint32_t incl = wave_incl_scan(np);
-    const uint32_t total = (uint32_t)__shfl((int)incl, 63);
+    const uint32_t total = wave_last(incl);
     if (total) {
       uint32_t base = 0;
       if (lane == 0) base = atomicAdd(a.pieceCount, total);
-      base = (uint32_t)__shfl((int)base, 0);
+      base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0);
       const uint32_t first = base + incl - np;
       const bool fits = first + np <= a.pieceCap;
       for (uint32_t q = 0; q < np && first + q < a.pieceCap; q++)
@@ -531,7 +531,7 @@ __device__ __forceinline__ void tiles_wave(const TilesArgs& a, TileWaveLds& L, u
       uint32_t sbase = 0;
       const uint32_t leader = (uint32_t)__builtin_ctzll(mask);
       if (lane == leader) sbase = atomicAdd(a.segCounter, (uint32_t)__popcll(mask));
-      sbase = (uint32_t)__shfl((int)sbase, (int)leader);
+      sbase = (uint32_t)__builtin_amdgcn_readlane((int)sbase, (int)leader);
       if (hit) {
         const uint32_t slot = sbase + mask_prefix(mask);
         if (slot < limit) {
@@ -1113,7 +1113,7 @@ __device__ __forceinline__ void fine_shade_queue(const FineArgs& a, const FineLd
   uint32_t base = 0;
   const uint32_t leader = (uint32_t)__builtin_ctzll(m);
   if (lane == leader) base = atomicAdd(sliceOff, (uint32_t)__popcll(m));
-  base = (uint32_t)__shfl((int)base, (int)leader);
+  base = (uint32_t)__builtin_amdgcn_readlane((int)base, (int)leader);
   const uint32_t slot = base + mask_prefix(m);
   if (accept && slot >= a.perProc) accept = false;
   uint32_t tag = 0;
