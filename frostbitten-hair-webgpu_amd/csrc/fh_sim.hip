@@ -127,8 +127,9 @@ struct GridPreSim {
 
 // ---------------------------------------------------------------------------
 // K2 — hairSimIntegrationPass.wgsl.ts:79-273. A workgroup of kSimThreads
-// threads owns kSimStrands strands, staged in LDS lane-interleaved (point j
-// of strand l at j * kSimStrands + l). The two Gauss-Seidel sweeps along the
+// threads owns L strands (32, or 16 when there are too few strands to give
+// every CU a couple of workgroups), staged in LDS lane-interleaved (point j
+// of strand l at j * L + l). The two Gauss-Seidel sweeps along the
 // strand (length constraint; local shape with the tip-tangent tail) run one
 // strand per lane on the first wave, so each wave instruction advances 32
 // strands; the point-local phases (Verlet integration with both grid
@@ -136,9 +137,9 @@ struct GridPreSim {
 // and spread over all threads, one (point, strand) per thread at a time, in
 // LDS order so that neighbouring threads touch neighbouring LDS words.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kSimStrands = 32;
 constexpr uint32_t kSimThreads = 256;
 
+template <uint32_t L>
 __device__ __forceinline__ void sim_integrate_body(
     uint32_t d, const fh_sim_uniforms* __restrict__ u, float4* __restrict__ prev,
     const float4* __restrict__ now, const float* __restrict__ seglen,
@@ -146,7 +147,6 @@ __device__ __forceinline__ void sim_integrate_body(
     const fh_density_velocity* __restrict__ dv, const fh_density_grad_wind* __restrict__ dgw,
     const float4* __restrict__ initial, uint32_t S, uint32_t P, uint32_t block, uint32_t blocks) {
   extern __shared__ float4 lds[];
-  constexpr uint32_t L = kSimStrands;
   float4* A = lds;                       // positions being solved (xyz, movable)
   float4* Bi = lds + (size_t)P * L;      // initial positions
   float4* K = Bi + (size_t)P * L;        // local-shape constants (h0, w0, w1, w2)
@@ -339,28 +339,34 @@ __device__ __forceinline__ void sim_integrate_body(
   // coalesced write-back into the PREV buffer (:268-272)
   for (uint32_t e = tid; e < nElem; e += kSimThreads) {
     const uint32_t ls = e / P, pt = e - ls * P;
-    prev[gbase + e] = A[pt * kSimStrands + ls];
+    prev[gbase + e] = A[pt * L + ls];
   }
 }
 
+struct SimArgs {
+  uint32_t d;
+  const fh_sim_uniforms* u;
+  float4* prev;
+  const float4* now;
+  const float* seglen;
+  const float* sdf;
+  uint32_t sdf_dims;
+  const fh_density_velocity* dv;
+  const fh_density_grad_wind* dgw;
+  const float4* initial;
+  uint32_t S;
+  uint32_t P;
+};
+
+template <uint32_t L>
 struct SimIntegrate {
-  struct Args {
-    uint32_t d;
-    const fh_sim_uniforms* u;
-    float4* prev;
-    const float4* now;
-    const float* seglen;
-    const float* sdf;
-    uint32_t sdf_dims;
-    const fh_density_velocity* dv;
-    const fh_density_grad_wind* dgw;
-    const float4* initial;
-    uint32_t S;
-    uint32_t P;
-  };
+  using Args = SimArgs;
   static constexpr uint32_t kThreads = kSimThreads;
   static constexpr const char* kName = "sim_integrate";
-  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { sim_integrate_body(a.d, a.u, a.prev, a.now, a.seglen, a.sdf, a.sdf_dims, a.dv, a.dgw, a.initial, a.S, a.P, xcd_order(block, blocks), blocks); }
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) {
+    sim_integrate_body<L>(a.d, a.u, a.prev, a.now, a.seglen, a.sdf, a.sdf_dims, a.dv, a.dgw, a.initial, a.S, a.P,
+                          xcd_order(block, blocks), blocks);
+  }
 };
 
 
@@ -653,13 +659,17 @@ int fh_sim_integrate(fh_ctx* ctx, uint32_t grid_dims, const fh_sim_uniforms* sim
                  density_velocity && density_grad_wind && positions_initial,
              "fh_sim_integrate: null buffer");
   if (strands_count == 0) return FH_OK;
-  const size_t lds = (size_t)points_per_strand * kSimStrands * (3 * sizeof(float4) + sizeof(float));
+  // 16 strands per workgroup when 32 would leave CUs with fewer than two
+  // workgroups (the sweeps then run on twice as many waves)
+  const uint32_t L = strands_count < 64u * fh::persistent_blocks(ctx) / 8u ? 16u : 32u;
+  const size_t lds = (size_t)points_per_strand * L * (3 * sizeof(float4) + sizeof(float));
   FH_REQUIRE(lds <= ctx->max_lds, "fh_sim_integrate: points_per_strand too large for this device's LDS");
-  return fh::launch<SimIntegrate>(
-      ctx, fh::div_ceil(strands_count, kSimStrands), (uint32_t)lds,
-      {grid_dims, sim_uniforms, reinterpret_cast<float4*>(positions_prev),
-       reinterpret_cast<const float4*>(positions_now), segment_lengths, sdf_texture, sdf_dims, density_velocity,
-       density_grad_wind, reinterpret_cast<const float4*>(positions_initial), strands_count, points_per_strand});
+  const SimArgs args{grid_dims, sim_uniforms, reinterpret_cast<float4*>(positions_prev),
+                     reinterpret_cast<const float4*>(positions_now), segment_lengths, sdf_texture, sdf_dims,
+                     density_velocity, density_grad_wind, reinterpret_cast<const float4*>(positions_initial),
+                     strands_count, points_per_strand};
+  if (L == 16u) return fh::launch<SimIntegrate<16>>(ctx, fh::div_ceil(strands_count, 16u), (uint32_t)lds, args);
+  return fh::launch<SimIntegrate<32>>(ctx, fh::div_ceil(strands_count, 32u), (uint32_t)lds, args);
 }
 
 int fh_grid_post_sim(fh_ctx* ctx, uint32_t grid_dims, const fh_sim_uniforms* sim_uniforms,
