@@ -66,6 +66,12 @@ def kernel_bytes(wl, n_pairs: int, tiles: int) -> dict:
         # points, framebuffer read + write, tile list
         "fine_raster": 16 * tiles * B + 4 * n_pairs + 32 * N + 4 * WH + 16 * S * Ps + 32 * WH + 4 * tiles,
         "shading": 32 * N + 4 * WH + WH // 2 + 2048 * 2048 * 4 + 16 * S * Ps,
+        # HDR scene restored into the HDR texture (8 B read + 8 B written per texel)
+        "copy_u32": 16 * WH,
+        # rasterizer pixel read, HDR texel read and written
+        "hair_combine": 32 * WH,
+        # HDR texel read, rgba8 texel written
+        "present": 12 * WH,
     }
 
 
@@ -83,6 +89,7 @@ def stage_bytes(wl, n_pairs: int, tiles: int) -> dict:
         "sort": 12 * tiles + 1024,
         "fine": 16 * tiles * B + 8 * n_pairs + 32 * N + 4 * WH + 16 * S * Ps + 32 * WH + 4 * tiles,
         "shading": 32 * N + 4 * WH + WH // 2 + 2048 * 2048 * 4 + 16 * S * Ps,
+        "post": 16 * WH + 32 * WH + 12 * WH,
     }
 
 
@@ -340,7 +347,7 @@ def main() -> None:
     print(f"# setup {time.perf_counter() - t_setup:.1f} s, workload {wl.name}: {wl.strands}x{wl.points} "
           f"{wl.width}x{wl.height}", file=sys.stderr, flush=True)
 
-    stages = ["sim", "tiles", "sort", "fine", "shading"]
+    stages = ["sim", "tiles", "sort", "fine", "post", "shading"]
 
     def frame(events=None):
         r.simulationUniforms.copy_(ring[r.frameIdx])
@@ -349,6 +356,7 @@ def main() -> None:
         rec(0)
         r.simulateHair(ctx)
         rec(1)
+        r.drawScene(ctx)
         r.tilesPass.cmdClearBeforeRender(ctx)
         r.finePass.cmdClearBeforeRender(ctx)
         r.sortPass.cmdClearBeforeRender(ctx)
@@ -358,8 +366,11 @@ def main() -> None:
         rec(3)
         r.finePass.cmdRasterizeSlicesHair(ctx, r.hair, r.tilesPass)
         rec(4)
-        r.shadingPass.cmdComputeShadingPoints(ctx, r.hair)
+        r.hairCombinePass.cmdCombineRasterResults(ctx)
+        r.presentPass.cmdDraw(ctx, r.screenTexture)
         rec(5)
+        r.shadingPass.cmdComputeShadingPoints(ctx, r.hair)
+        rec(6)
         r.frameIdx += 1
 
     def graph_frame(events=None):
@@ -368,7 +379,7 @@ def main() -> None:
             events[0].record()
         r.frame_graph(upload_uniforms=False)
         if events:
-            events[5].record()
+            events[6].record()
 
     use_graph = not args.no_graph
     r.beforeFirstFrame()
@@ -381,7 +392,7 @@ def main() -> None:
     # value and the frame latencies.
     if use_graph:
         r.capture_frames()
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(args.steps)]
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(7)] for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -394,7 +405,7 @@ def main() -> None:
     # around every kernel node): per-kernel durations and the roofline. The
     # events cost frame time, so they stay out of region 1.
     kernels = {}
-    evs2 = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(args.steps)]
+    evs2 = [[torch.cuda.Event(enable_timing=True) for _ in range(7)] for _ in range(args.steps)]
     if not args.no_kernel_timing:
         r.native.profiler_enable(64 * args.steps)
         if use_graph:
@@ -406,15 +417,16 @@ def main() -> None:
         torch.cuda.synchronize()
         kernels = r.native.profiler_read()
         r.native.profiler_enable(0)
-    per_frame = np.array([evs[k][0].elapsed_time(evs[k][5]) for k in range(args.steps)])  # ms
+    per_frame = np.array([evs[k][0].elapsed_time(evs[k][6]) for k in range(args.steps)])  # ms
     if use_graph:
         # stage sums from the per-kernel events
         groups = {"sim": ("grid_pre_sim", "sim_integrate", "p2g_clear", "p2g_count", "p2g_scan", "p2g_scatter",
                          "grid_post_sim"),
-                  "tiles": ("fill_many", "tiles_draw", "tiles_pieces", "tile_counts"),
+                  "tiles": ("fill_many", "tile_depth_bounds", "tiles_draw", "tiles_pieces", "tile_counts"),
                   "sort": ("sort_count", "sort_scatter"),
-                  "fine": ("bin_scan_local", "bin_scan_add", "bin_scatter", "bin_sort_small", "bin_sort_large",
-                           "fine_raster"),
+                  "fine": ("fill_u32", "bin_scan_local", "bin_scan_add", "bin_scatter", "bin_sort_small",
+                           "bin_sort_large", "fine_raster"),
+                  "post": ("copy_u32", "hair_combine", "present"),
                   "shading": ("shading",)}
         stage_ms = {g: sum(kernels[k][1] for k in ks if k in kernels) / args.steps for g, ks in groups.items()}
         if not kernels:

@@ -408,3 +408,21 @@ def shadow_map(wl: Workload, uniforms: np.ndarray, size: int) -> np.ndarray:
     t, hit = _sphere_trace(wl.collider, n, dirs, float(length.max()), iters=64)
     depth = np.where(hit, t / length, 1.0).astype(np.float32)
     return depth.reshape(size, size)
+
+
+def hdr_scene(wl: Workload, mesh: np.ndarray) -> np.ndarray:
+    """The rgba16float HDR texture (H x W x 4 half bits, rows top to bottom)
+    that the background and mesh passes leave for the combine pass: the
+    background gradient of drawBackgroundGradientPass.wgsl.ts:33-48 with its
+    noise term held at its mean (c = 0.5), and the collider meshes as flat
+    grey where the mesh depth is below 1."""
+    W, H = wl.width, wl.height
+    c0, c1 = np.array(L.col(22, 162, 188)), np.array(L.col(14, 103, 120))
+    uv_y = (np.arange(H) + 0.5) / H
+    base = c0 * 0.5 + c1 * 0.5
+    col = base[None, :] * 0.5 + (1.0 - uv_y)[:, None] * 0.5  # mix(color, 1 - uv.y, gradientStrength = 0.5)
+    img = np.empty((H, W, 4), dtype=np.float32)
+    img[..., :3] = col[:, None, :]
+    img[..., 3] = 1.0
+    img[mesh < 1.0, :3] = 0.5
+    return img.astype(np.float16).view(np.uint16)

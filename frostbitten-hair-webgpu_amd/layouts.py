@@ -55,8 +55,15 @@ RENDER_UNIFORMS_DTYPE = np.dtype({
 
 HAIR_DATA_BYTES = 256  # WEBGPU_MINIMAL_BUFFER_SIZE (webgpu.ts:6), hairDataBuffer.ts:29-47
 
+# DISPLAY_MODE (constants.ts:44-58)
 DISPLAY_MODE_FINAL = 0
+DISPLAY_MODE_TILES = 1
+DISPLAY_MODE_TILES_PPLL = 2
+DISPLAY_MODE_HW_RENDER = 3
 DISPLAY_MODE_USED_SLICES = 4
+DISPLAY_MODE_DEPTH = 5
+DISPLAY_MODE_NORMALS = 6
+DISPLAY_MODE_AO = 7
 
 
 def col(*c):
@@ -81,7 +88,9 @@ class RenderConfig:
     shadingPoints: int = 64
     lodRenderPercent: float = 100.0
     displayMode: int = DISPLAY_MODE_FINAL
+    dbgTileModeMaxSegments: int = 370
     dbgSlicesModeMaxSlices: int = 50
+    dbgShowTiles: bool = False
     material: dict = field(default_factory=lambda: dict(
         color0=col(119, 43, 119), color1=col(76, 0, 255), specular=0.9, weightTT=0.0, weightTRT=1.4,
         shift=0.0, roughness=0.3, colorRng=0.1, lumaRng=0.1, attenuation=30.0, shadows=0.5))
@@ -265,14 +274,26 @@ def pack_render_uniforms(cfg: RenderConfig, sim: SimConfig, camera: Camera, widt
     u["fiberRadius"] = cfg.fiberRadius
     u["dbgShadowMapPreviewSize"] = math.floor(min(500, width / 3, height / 3))
     u["maxDrawnHairSegments"] = segments_buffer_entries(cfg, width, height)
-    extra = cfg.dbgSlicesModeMaxSlices if cfg.displayMode == DISPLAY_MODE_USED_SLICES else 0
-    u["displayMode"] = cfg.displayMode | (extra << 8)
+    u["displayMode"] = encode_debug_mode(cfg)
     u["bgColor0"][0] = col(22, 162, 188)
     u["bgNoiseScale"] = 5.0
     u["bgColor1"][0] = col(14, 103, 120)
     u["bgGradientStrength"] = 0.5
     u["collisionSpherePosition"][0] = cs[:3]
     return u
+
+
+def encode_debug_mode(cfg: RenderConfig) -> int:
+    """encodeDebugMode — renderUniformsBuffer.ts:396-414: the display mode in
+    the low 8 bits, the mode's extra value above them."""
+    extra = 0
+    if cfg.displayMode in (DISPLAY_MODE_TILES, DISPLAY_MODE_TILES_PPLL):
+        extra = cfg.dbgTileModeMaxSegments
+    elif cfg.displayMode == DISPLAY_MODE_USED_SLICES:
+        extra = cfg.dbgSlicesModeMaxSlices
+    elif cfg.displayMode == DISPLAY_MODE_FINAL:
+        extra = 1 if cfg.dbgShowTiles else 0
+    return cfg.displayMode | (extra << 8)
 
 
 def pack_hair_data(center, radius: float, strands: int, points: int) -> np.ndarray:

@@ -72,6 +72,9 @@ SIGNATURES = {
                              vp, vp, u32, u32]),
     "fh_shading": (i32, [vp, C.POINTER(RenderCfg), vp, vp, vp, vp, vp, Tex2D, Tex2D, Tex2D, u32, u32]),
     "fh_debug_raster_barycentric": (i32, [vp, vp, vp, vp, u32, u32, vp]),
+    "fh_hair_combine": (i32, [vp, C.POINTER(RenderCfg), u32, u32, vp, vp, vp, vp, vp, vp]),
+    "fh_present": (i32, [vp, u32, u32, vp, vp, Tex2D, Tex2D, vp, u32, vp, u32]),
+    "fh_copy": (i32, [vp, vp, vp, C.c_uint64]),
     "fh_profiler_enable": (i32, [vp, u32]),
     "fh_capture_begin": (i32, [vp]),
     "fh_capture_end": (i32, [vp, C.POINTER(vp)]),

@@ -9,6 +9,8 @@ reference so a caller drives the chain the way Renderer does:
     HairTileSortPass.cmdClearBeforeRender(ctx) / cmdSortHairTiles(ctx)
     HairFinePass.cmdClearBeforeRender(ctx) / cmdRasterizeSlicesHair(ctx, hairObject)
     HairShadingPass.cmdComputeShadingPoints(ctx, hairObject)
+    HairCombinePass.cmdCombineRasterResults(ctx)
+    PresentPass.cmdDraw(ctx, screenTexture)
 
 Device memory is torch-allocated; every pass enqueues on the context's stream
 and returns, like recording into a GPUCommandEncoder.
@@ -122,6 +124,8 @@ class PassCtx:
     hairSegmentCountPerTileBuffer: torch.Tensor = None
     hairTileListBuffer: torch.Tensor = None
     hairRasterizerResultsBuffer: torch.Tensor = None
+    hdrRenderTexture: torch.Tensor = None  # rgba16float, (H, W, 4) half bits
+    normalsTexture: torch.Tensor = None    # rg16float, (H, W) u32 of two halves, or None
 
 
 class GridPreSimPass:
@@ -252,6 +256,33 @@ class HairShadingPass:
                         hair.strandsCount, hair.pointsPerStrand)
 
 
+class HairCombinePass:
+    """src/passes/hairCombine/hairCombinePass.ts:178-270."""
+    NAME = "HairCombinePass"
+
+    def cmdCombineRasterResults(self, ctx: PassCtx) -> None:
+        w, h = ctx.viewport
+        ctx.native.call("fh_hair_combine", ctx.cfg, w, h, N.ptr(ctx.globalUniforms), N.ptr(ctx.hairTilesBuffer),
+                        N.ptr(ctx.hairTileSegmentsBuffer), N.ptr(ctx.hairRasterizerResultsBuffer),
+                        N.ptr(ctx.hairSegmentCountPerTileBuffer), N.ptr(ctx.hdrRenderTexture))
+
+
+class PresentPass:
+    """src/passes/presentPass/presentPass.ts:125-210. `ditherPerPixel` False
+    reads Bayer element 0 for every pixel, as the Deno build does
+    (dither.wgsl.ts:34); True indexes the matrix by pixel (browser build)."""
+    NAME = "PresentPass"
+
+    def __init__(self, ditherPerPixel: bool = False):
+        self.ditherPerPixel = ditherPerPixel
+
+    def cmdDraw(self, ctx: PassCtx, screenTexture: torch.Tensor, rowBytes: int = 0) -> None:
+        w, h = ctx.viewport
+        ctx.native.call("fh_present", w, h, N.ptr(ctx.globalUniforms), N.ptr(ctx.hdrRenderTexture),
+                        ctx.shadingDepthTexture, ctx.aoTexture, N.ptr(ctx.normalsTexture),
+                        1 if self.ditherPerPixel else 0, N.ptr(screenTexture), rowBytes)
+
+
 class HairRenderer:
     """The hair part of src/renderer.ts: beforeFirstFrame (:145-169),
     cmdRender -> simulateHair (:212-240) -> cmdDrawScene (:242-283) ->
@@ -289,6 +320,13 @@ class HairRenderer:
         self.simPass = HairSimIntegrationPass()
         self.gridPostSimPass = GridPostSimPass()
         self.shadingPass = HairShadingPass()
+        self.hairCombinePass = HairCombinePass()
+        self.presentPass = PresentPass()
+        # the HDR scene texture, restored each frame from the scene the
+        # out-of-scope background and mesh passes would draw; the screen
+        # texture receives the presented rgba8 sRGB image
+        self.hdrRenderTexture = torch.empty_like(self.inputs["hdrScene"])
+        self.screenTexture = torch.zeros((H, W, 4), dtype=torch.uint8, device=self.device)
 
     def sim_uniforms(self, frame_idx: int) -> np.ndarray:
         return L.pack_sim_uniforms(self.wl.sim, self.model, self.wl.sdf_bounds, self.grid.bounds, frame_idx)
@@ -307,7 +345,8 @@ class HairRenderer:
             hairTileSegmentsBuffer=self.tilesPass.hairTileSegmentsBuffer,
             hairSegmentCountPerTileBuffer=self.tilesPass.segmentCountPerTileBuffer,
             hairTileListBuffer=self.sortPass.tileListBuffer,
-            hairRasterizerResultsBuffer=self.finePass.hairRasterizerResultsBuffer)
+            hairRasterizerResultsBuffer=self.finePass.hairRasterizerResultsBuffer,
+            hdrRenderTexture=self.hdrRenderTexture)
 
     def upload_sim_uniforms(self) -> None:
         u = torch.from_numpy(self.sim_uniforms(self.frameIdx).view(np.uint8).reshape(-1).copy())
@@ -323,14 +362,25 @@ class HairRenderer:
         self.simPass.cmdSimulateHairPositions(ctx, self.hair)
         self.gridPostSimPass.cmdUpdateGridsAfterSim(ctx, self.hair)
 
+    def drawScene(self, ctx: PassCtx) -> None:
+        """The HDR scene of the background and mesh passes (renderer.ts:187-188,
+        :244-246), which are out of scope: their output is an input here."""
+        src = self.inputs["hdrScene"]
+        self.native.call("fh_copy", N.ptr(self.hdrRenderTexture), N.ptr(src), src.numel() * src.element_size())
+
     def drawHair(self, ctx: PassCtx) -> None:
+        """cmdDrawScene's hair part (renderer.ts:262-295) and the present pass
+        (:205)."""
+        self.drawScene(ctx)
         self.tilesPass.cmdClearBeforeRender(ctx)
         self.finePass.cmdClearBeforeRender(ctx)
         self.sortPass.cmdClearBeforeRender(ctx)
         self.tilesPass.cmdDrawHairToTiles(ctx, self.hair)
         self.sortPass.cmdSortHairTiles(ctx)
         self.finePass.cmdRasterizeSlicesHair(ctx, self.hair, self.tilesPass)
+        self.hairCombinePass.cmdCombineRasterResults(ctx)
         self.shadingPass.cmdComputeShadingPoints(ctx, self.hair)
+        self.presentPass.cmdDraw(ctx, self.screenTexture)
 
     def frame(self, upload_uniforms: bool = True) -> None:
         """One sim step + render of the hair chain (Renderer.cmdRender)."""
@@ -396,4 +446,5 @@ def make_inputs(wl: A.Workload, render_uniforms: np.ndarray, shadow_size: int = 
     hd = A.hair_depth(wl, render_uniforms, wl.positions, md)
     ao = np.ones((max(1, wl.height // 2), max(1, wl.width // 2)), dtype=np.float32)
     sm = A.shadow_map(wl, render_uniforms, shadow_size)
-    return {"meshDepth": md, "hairDepth": hd, "ao": ao, "shadowMap": sm}
+    hdr = A.hdr_scene(wl, md)
+    return {"meshDepth": md, "hairDepth": hd, "ao": ao, "shadowMap": sm, "hdrScene": hdr}

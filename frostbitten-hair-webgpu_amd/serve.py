@@ -44,7 +44,8 @@ class Request:
     admitted: float | None = None
     done: float | None = None
     frames_done: int = 0
-    image: torch.Tensor | None = None
+    image: torch.Tensor | None = None   # rasterizer result after the last frame
+    screen: torch.Tensor | None = None  # presented rgba8 sRGB image after the last frame
 
 
 def poisson_requests(n: int, rate: float, assets: list, max_steps: int, seed: int) -> list:
@@ -69,7 +70,8 @@ def estimate_bytes(wl) -> int:
                + 2 * 16 * wl.sim.gridDims ** 3 + 4 * wl.sdf_dims ** 3
                + 16 * T * rc.tileDepthBins + 8 * entries + 8 * entries + 8 * T
                + 16 * rc.processorCount * per_proc + 16 * W * H
-               + 8 * W * H + W * H + 4 * sm * sm + 16 * wl.strands * rc.shadingPoints)
+               + 8 * W * H + W * H + 4 * sm * sm + 16 * wl.strands * rc.shadingPoints
+               + 20 * W * H)                                # HDR scene + HDR texture + screen
     scratch = 36 * N_ + 16 * entries + 8 * T * rc.tileDepthBins + (32 << 20)
     return int(buffers + scratch)
 
@@ -265,6 +267,7 @@ class HairServer:
                     if self.keep_images:
                         r = self.resident[a].r
                         q.image = r.finePass.image(r.wl.width, r.wl.height).clone()
+                        q.screen = r.screenTexture.clone()
                     del active[a]
         self.elapsed = time.perf_counter() - t0
         return requests

@@ -393,6 +393,44 @@ int fh_shading(fh_ctx* ctx, const fh_render_cfg* cfg,
                uint32_t strands_count, uint32_t points_per_strand);
 
 /* ------------------------------------------------------------------------ */
+/* Combine and present (src/passes/hairCombine, src/passes/presentPass)      */
+/* ------------------------------------------------------------------------ */
+
+/* HDR scene texture: rgba16float (HDR_RENDER_TEX_FORMAT, constants.ts:35),
+ * 8 B per texel in texture memory order (texel (x, y) at y * width + x, row 0
+ * at the top). Screen texture: rgba8unorm-srgb (index.deno.ts:42), 4 B per
+ * texel, rows of screen_row_bytes bytes (0 = 4 * width), so that it can be
+ * written straight into a row-padded readback layout (index.deno.ts:131-146). */
+
+/* HairCombinePass.cmdCombineRasterResults (hairCombinePass.ts:228-249;
+ * fragment shader hairCombinePass.wgsl.ts:51-162, blend state :204-217):
+ * blends the rasterizer result over hdr_texture with src-alpha /
+ * one-minus-src-alpha colour and one / one alpha, rasterizer row H - 1 - y
+ * onto texture row y. Display modes TILES / TILES_PPLL draw per-tile segment
+ * counts instead, as the reference's debug views do. */
+int fh_hair_combine(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t vp_h,
+                    const fh_render_uniforms* uniforms, const fh_tile_result* tiles,
+                    const void* tile_segments, const void* rasterizer_result,
+                    const uint32_t* segment_count_per_tile, void* hdr_texture);
+
+/* PresentPass.cmdDraw (presentPass.ts:182-210; fragment shader
+ * presentPass.wgsl.ts:58-103): dither, exposure, ACES tone mapping and gamma
+ * (colorMgmt) of hdr_texture into the sRGB screen texture. depth is the depth
+ * texture (DEPTH view), ao the AO texture (AO view), normals_texture the
+ * rg16float octahedral normals (NORMALS view; NULL reads as 0).
+ * dither_per_pixel = 0 reads Bayer element 0 for every pixel, as the Deno
+ * build does (dither.wgsl.ts:34); 1 indexes the matrix by pixel as in the
+ * browser. */
+int fh_present(fh_ctx* ctx, uint32_t vp_w, uint32_t vp_h, const fh_render_uniforms* uniforms,
+               const void* hdr_texture, fh_tex2d depth, fh_tex2d ao, const void* normals_texture,
+               uint32_t dither_per_pixel, void* screen_texture, uint32_t screen_row_bytes);
+
+/* Device-to-device copy as a kernel (bytes a multiple of 4), so that a
+ * captured frame can restore its inputs (e.g. the HDR scene drawn by the
+ * out-of-scope mesh passes) without memcpy nodes. */
+int fh_copy(fh_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+
+/* ------------------------------------------------------------------------ */
 /* Kernel timing                                                             */
 /* ------------------------------------------------------------------------ */
 

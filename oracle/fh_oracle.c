@@ -990,3 +990,165 @@ void or_shading(const fh_render_cfg* cfg, const fh_render_uniforms* u,
       o->x = rs.x; o->y = rs.y; o->z = rs.z; o->w = alpha;
     }
 }
+
+/* ======================================================================== */
+/* Combine and present                                                       */
+/* ======================================================================== */
+
+/* IEEE binary16 conversions as exported test hooks. Pinned:
+ * test_oracle_post.py compares them with numpy's float16 over every half
+ * and over random and boundary floats. */
+uint32_t or_f2h(float f) { return of2h(f); }
+float or_h2f(uint32_t h) { return oh2f(h); }
+
+/* HairCombinePass fragment shader (hairCombinePass.wgsl.ts:51-162) with the
+ * pipeline's blend state (hairCombinePass.ts:204-217) for every texel of the
+ * rgba16float HDR texture (texture row ty <- rasterizer row H - 1 - ty).
+ * Pinned: test_oracle_post.py (y flip, alpha 0 / 1 / 0.5 closed forms, the
+ * debug tile pattern, list-walk counts against segment_count_per_tile). */
+void or_hair_combine(const fh_render_cfg* cfg, uint32_t W, uint32_t H,
+                     const fh_render_uniforms* u, const fh_tile_result* tiles,
+                     const void* segbuf, const void* rasterResult,
+                     const uint32_t* countPerTile, uint16_t* hdr) {
+  const fh_list_element* segs =
+      (const fh_list_element*)((const char*)segbuf + 4);
+  const float* pix = (const float*)((const char*)rasterResult + 16);
+  const uint32_t tile = cfg->tileSize, bins = cfg->tileDepthBins;
+  const uint32_t tilesX = (W + tile - 1) / tile;
+  const uint32_t mode = u->displayMode & 0xffu;
+  const uint32_t extra = (u->displayMode >> 8) & 0xffffu;
+  for (uint32_t ty = 0; ty < H; ty++) {
+    for (uint32_t x = 0; x < W; x++) {
+      /* fragment centre (x + .5, ty + .5); y is inverted (:60-63) */
+      const float fy = (float)H - ((float)ty + 0.5f);
+      const uint32_t py = ou32(fy);
+      const uint32_t tileX = x / tile, tileY = py / tile;
+      float c[4];
+      if (mode == 1u || mode == 2u) {
+        uint32_t segments = 0;
+        if (mode == 1u) {
+          segments = countPerTile[tileY * tilesX + tileX];
+        } else {
+          for (uint32_t b = 0; b < bins; b++) {
+            uint32_t p = tiles[(tileY * tilesX + tileX) * bins + b].tileSegmentPtr;
+            p = p == 0u ? INVALID_PTR : p - 1u; /* _translateHeadPointer */
+            while (segments < extra) {
+              if (p >= u->maxDrawnHairSegments || p == INVALID_PTR) break;
+              segments++;
+              p = segs[p].next;
+            }
+          }
+        }
+        if (segments == 0u) continue; /* discard */
+        c[0] = (float)segments / (float)extra;
+        c[1] = 1.0f - c[0];
+        c[2] = 0.0f;
+        c[3] = 1.0f;
+      } else {
+        const uint64_t i = (uint64_t)W * py + x;
+        for (int k = 0; k < 4; k++)
+          c[k] = i < (uint64_t)W * H - 1u ? pix[i * 4 + k] : 0.0f;
+        if (c[3] <= 0.0f && mode == 0u && (extra & 1u)) {
+          const float dbg[4] = {0.0f, (float)(tileX % 2u) / 2.0f,
+                                (float)(tileY % 2u) / 2.0f, 1.0f};
+          const float a = c[3];
+          for (int k = 0; k < 4; k++) c[k] = dbg[k] + c[k] * a;
+        }
+      }
+      uint16_t* t = hdr + ((size_t)ty * W + x) * 4;
+      const float a = c[3];
+      for (int k = 0; k < 3; k++)
+        t[k] = (uint16_t)of2h(c[k] * a + oh2f(t[k]) * (1.0f - a));
+      t[3] = (uint16_t)of2h(a + oh2f(t[3]));
+    }
+  }
+}
+
+/* Entry (x, y) of the 8x8 Bayer matrix by the recursive construction: bit i
+ * of x ^ y goes to bit 5 - 2i, bit i of y to bit 4 - 2i. Pinned:
+ * test_oracle_post.py against the 64 values of dither.wgsl.ts:12-21. */
+uint32_t or_bayer8(uint32_t x, uint32_t y) {
+  const uint32_t v = (x ^ y) & 7u;
+  uint32_t r = 0;
+  for (uint32_t i = 0; i < 3u; i++) {
+    r |= ((v >> i) & 1u) << (5u - 2u * i);
+    r |= ((y >> i) & 1u) << (4u - 2u * i);
+  }
+  return r;
+}
+
+/* doACES_Tonemapping (aces.wgsl.ts:5-13). Pinned: closed forms at 0, 1 and
+ * the large-x limit a / c. */
+float or_aces(float x) {
+  return osat((x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f));
+}
+
+/* Store of a linear value into an rgba8unorm-srgb texel (IEC 61966-2-1),
+ * evaluated in double and rounded to nearest. Pinned: every 8-bit code
+ * decoded by the inverse transfer function encodes back to itself. */
+uint32_t or_srgb8(float c) {
+  double v = (double)osat(c);
+  double s = v <= 0.0031308 ? 12.92 * v : 1.055 * pow(v, 1.0 / 2.4) - 0.055;
+  return (uint32_t)floor(s * 255.0 + 0.5);
+}
+
+/* PresentPass fragment shader (presentPass.wgsl.ts:58-103) for every texel:
+ * FINAL / HW_RENDER -> ditherColor (dither.wgsl.ts:23-49), exposure, ACES,
+ * saturate, gamma; DEPTH -> linearizeDepth_0_1 (linearDepth.wgsl.ts:9-36)
+ * mapped from [0.002, 0.01]; NORMALS -> |decodeOctahedronNormal|
+ * (shaderSnippets.wgls.ts:49-58); AO -> sampleAo (textureAo.wgsl.ts:7-12);
+ * other modes pass the HDR colour through. Output rgba8 sRGB, alpha 255.
+ * Pinned: test_oracle_post.py (gamma 1 / exposure / dither closed forms,
+ * octahedral decode of the six axes, depth map end points). */
+void or_present(uint32_t W, uint32_t H, const fh_render_uniforms* u,
+                const uint16_t* hdr, const float* depth, const float* ao,
+                uint32_t aoW, uint32_t aoH, const uint32_t* normals,
+                int ditherPerPixel, uint8_t* out, uint32_t pitch) {
+  const float zNear = 0.01f, zFar = 100.0f;
+  const float gamma = u->colorMgmt[0], exposure = u->colorMgmt[1];
+  const float ditherStr = u->colorMgmt[2];
+  const uint32_t mode = u->displayMode & 0xffu;
+  for (uint32_t y = 0; y < H; y++) {
+    for (uint32_t x = 0; x < W; x++) {
+      const uint16_t* t = hdr + ((size_t)y * W + x) * 4;
+      float c[3] = {oh2f(t[0]), oh2f(t[1]), oh2f(t[2])};
+      if (mode == 0u || mode == 3u) {
+        const uint32_t m = ditherPerPixel ? or_bayer8(x % 8u, y % 8u) : or_bayer8(0, 0);
+        const float dm = (float)m / 63.0f * ditherStr / 256.0f;
+        for (int k = 0; k < 3; k++) {
+          float v = c[k] + dm;
+          v = v * exposure;
+          v = osat(or_aces(v));
+          c[k] = powf(v, 1.0f / gamma);
+        }
+      } else if (mode == 5u) {
+        const float d = depth[(size_t)y * W + x];
+        const float lin = zNear * zFar / (zFar + (zNear - zFar) * d) / (zFar - zNear);
+        const float tt = osat((lin - 0.002f) / (0.01f - 0.002f));
+        c[0] = c[1] = c[2] = omix(0.0f, 1.0f, tt);
+      } else if (mode == 6u) {
+        const uint32_t nn = normals ? normals[(size_t)y * W + x] : 0u;
+        float n[3];
+        n[0] = oh2f(nn & 0xffffu) * 2.0f - 1.0f;
+        n[1] = oh2f(nn >> 16) * 2.0f - 1.0f;
+        n[2] = 1.0f - fabsf(n[0]) - fabsf(n[1]);
+        const float tt = osat(-n[2]);
+        n[0] = n[0] >= 0.0f ? n[0] - tt : n[0] + tt;
+        n[1] = n[1] >= 0.0f ? n[1] - tt : n[1] + tt;
+        const ov3 nv = norm3(o3(n[0], n[1], n[2]));
+        c[0] = fabsf(nv.x); c[1] = fabsf(nv.y); c[2] = fabsf(nv.z);
+      } else if (mode == 7u) {
+        const float tx = ((float)x + 0.5f) / u->viewport[0];
+        const float ty = ((float)y + 0.5f) / u->viewport[1];
+        const int32_t sx = oi32((float)aoW * tx), sy = oi32((float)aoH * ty);
+        float v = 0.0f;
+        if (sx >= 0 && sy >= 0 && (uint32_t)sx < aoW && (uint32_t)sy < aoH)
+          v = ao[(size_t)sy * aoW + (uint32_t)sx];
+        c[0] = c[1] = c[2] = v;
+      }
+      uint8_t* o = out + (size_t)y * pitch + 4u * x;
+      for (int k = 0; k < 3; k++) o[k] = (uint8_t)or_srgb8(c[k]);
+      o[3] = 255;
+    }
+  }
+}

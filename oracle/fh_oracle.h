@@ -61,6 +61,22 @@ void or_shading(const fh_render_cfg* cfg, const fh_render_uniforms* u,
                 uint32_t aoH, const float* depth, uint32_t dW, uint32_t dH,
                 uint32_t S, uint32_t P);
 
+uint32_t or_f2h(float f);
+float or_h2f(uint32_t h);
+uint32_t or_bayer8(uint32_t x, uint32_t y);
+float or_aces(float x);
+uint32_t or_srgb8(float c);
+
+void or_hair_combine(const fh_render_cfg* cfg, uint32_t W, uint32_t H,
+                     const fh_render_uniforms* u, const fh_tile_result* tiles,
+                     const void* segbuf, const void* rasterResult,
+                     const uint32_t* countPerTile, uint16_t* hdr);
+
+void or_present(uint32_t W, uint32_t H, const fh_render_uniforms* u,
+                const uint16_t* hdr, const float* depth, const float* ao,
+                uint32_t aoW, uint32_t aoH, const uint32_t* normals,
+                int ditherPerPixel, uint8_t* out, uint32_t pitch);
+
 #ifdef __cplusplus
 }
 #endif

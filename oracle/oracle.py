@@ -121,3 +121,61 @@ def shading(rc, render_u, positions, tangents, shadow, ao, depth, S, P):
                      C.c_uint32(ao.shape[0]), _p(depth), C.c_uint32(depth.shape[1]), C.c_uint32(depth.shape[0]),
                      C.c_uint32(S), C.c_uint32(P))
     return out
+
+
+def _setup_post():
+    L = lib()
+    if getattr(L, "_post_ready", False):
+        return L
+    L.or_f2h.restype = C.c_uint32
+    L.or_f2h.argtypes = [C.c_float]
+    L.or_h2f.restype = C.c_float
+    L.or_h2f.argtypes = [C.c_uint32]
+    L.or_bayer8.restype = C.c_uint32
+    L.or_bayer8.argtypes = [C.c_uint32, C.c_uint32]
+    L.or_aces.restype = C.c_float
+    L.or_aces.argtypes = [C.c_float]
+    L.or_srgb8.restype = C.c_uint32
+    L.or_srgb8.argtypes = [C.c_float]
+    L._post_ready = True
+    return L
+
+
+def f2h(x: float) -> int:
+    return _setup_post().or_f2h(x)
+
+
+def h2f(h: int) -> float:
+    return _setup_post().or_h2f(h)
+
+
+def bayer8(x: int, y: int) -> int:
+    return _setup_post().or_bayer8(x, y)
+
+
+def aces(x: float) -> float:
+    return _setup_post().or_aces(x)
+
+
+def srgb8(x: float) -> int:
+    return _setup_post().or_srgb8(x)
+
+
+def hair_combine(rc, W, H, render_u, tiles, seg, raster, counts, hdr):
+    """Returns a blended copy of `hdr` (uint16 rgba16float bits, H*W*4)."""
+    cfg = RenderCfg.from_config(rc)
+    out = np.ascontiguousarray(hdr, dtype=np.uint16).copy()
+    lib().or_hair_combine(C.byref(cfg), C.c_uint32(W), C.c_uint32(H), _p(render_u), _p(tiles), _p(seg),
+                          _p(raster), _p(counts), _p(out))
+    return out
+
+
+def present(W, H, render_u, hdr, depth, ao, normals=None, dither_per_pixel=False, pitch=0):
+    """rgba8 sRGB screen bytes (H rows of `pitch` bytes, default 4*W)."""
+    pitch = pitch or 4 * W
+    out = np.zeros(H * pitch, dtype=np.uint8)
+    nptr = _p(normals) if normals is not None else C.c_void_p(0)
+    lib().or_present(C.c_uint32(W), C.c_uint32(H), _p(render_u), _p(hdr), _p(depth), _p(ao),
+                     C.c_uint32(ao.shape[1]), C.c_uint32(ao.shape[0]), nptr, C.c_int(1 if dither_per_pixel else 0),
+                     _p(out), C.c_uint32(pitch))
+    return out

@@ -40,7 +40,8 @@ def _state(r):
     return dict(now=_np(r.hair.positionsNow), prev=_np(r.hair.positionsPrev), dv=_np(r.grid.densityVelocity),
                 depth=tiles[:, [0, 1, 3]], lists=sorted(lists.items()), drawn=int(seg[0]),
                 counts=_np(r.tilesPass.segmentCountPerTileBuffer),
-                img=_np(r.finePass.image(r.wl.width, r.wl.height)), shading=_np(r.hair.shading))
+                img=_np(r.finePass.image(r.wl.width, r.wl.height)), shading=_np(r.hair.shading),
+                hdr=_np(r.hdrRenderTexture), screen=_np(r.screenTexture))
 
 
 @pytest.mark.parametrize("use_graph", [False, True])

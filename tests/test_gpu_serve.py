@@ -47,6 +47,7 @@ def test_serve_alone_equals_mix(gpu):
     for a, b in zip(alone, mix):
         assert a.image is not None and b.image is not None
         assert np.array_equal(a.image.cpu().numpy(), b.image.cpu().numpy()), f"request {a.rid} differs"
+        assert np.array_equal(a.screen.cpu().numpy(), b.screen.cpu().numpy()), f"request {a.rid} screen differs"
         assert a.image[..., 3].max() > 0
 
 
