@@ -568,13 +568,19 @@ struct TilesDraw {
 };
 
 
-// The queued pieces, 64 per wave; every workgroup exits after the last one.
+// The queued pieces, `per` consecutive ones per wave: as few as spreads the
+// queue over every workgroup (a short queue, one piece per wave), at most 64
+// (one per lane). A wave's pixel work is spread over its lanes whatever the
+// number of pieces it holds, so a short queue of wide segments runs on many
+// waves at once instead of one after another on one. Every workgroup exits
+// after the last piece.
 __device__ __forceinline__ void tiles_pieces_body(TilesArgs a, uint32_t block, uint32_t blocks) {
   __shared__ TileWaveLds L;
   const uint32_t count = min(*a.pieceCount, a.pieceCap);
-  for (uint32_t base = block * 64u; base < count; base += blocks * 64u) {
+  const uint32_t per = min(64u, max(1u, (count + blocks - 1u) / blocks));
+  for (uint32_t base = block * per; base < count; base += blocks * per) {
     const uint32_t i = base + threadIdx.x;
-    const uint2 pc = i < count ? a.pieces[i] : make_uint2(kInvalidPtr, 0u);
+    const uint2 pc = threadIdx.x < per && i < count ? a.pieces[i] : make_uint2(kInvalidPtr, 0u);
     tiles_wave(a, L, pc.x, pc.x != kInvalidPtr, pc.y, false);
     wave_sync();
   }

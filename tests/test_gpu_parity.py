@@ -169,13 +169,24 @@ def _check_slot_tags(tags_ranks, counts, n):
     assert np.array_equal(sr, np.arange(n) - run_start)
 
 
-@pytest.mark.parametrize("tile,bins,width,height", [(16, 4, 320, 240), (8, 8, 320, 240), (32, 4, 320, 240),
-                                                     (16, 4, 333, 171), (8, 8, 250, 97)])
-def test_tiles_bit_exact(gpu, oracle, tile, bins, width, height):
+@pytest.mark.parametrize("tile,bins,width,height,radius", [(16, 4, 320, 240, None), (8, 8, 320, 240, None),
+                                                            (32, 4, 320, 240, None), (16, 4, 333, 171, None),
+                                                            (8, 8, 250, 97, None), (8, 4, 320, 240, 0.06)])
+def test_tiles_bit_exact(gpu, oracle, tile, bins, width, height, radius):
     """Tile depth ranges, per-(tile, bin) counts and list contents equal the
-    oracle's; the last two cases have partial tiles on both edges."""
+    oracle's; two cases have partial tiles on both edges, and the last one has
+    fibers wide enough that most segments cover more than 8 tiles (the tile
+    pass queues those in 8-tile pieces for its second kernel)."""
     import torch
-    wl, r = _renderer("small", 800, width, height, tile=tile, bins=bins)
+    from frostbitten_hair_webgpu_amd import assets as A
+    from frostbitten_hair_webgpu_amd import passes as PS
+    if radius is None:
+        wl, r = _renderer("small", 800, width, height, tile=tile, bins=bins)
+    else:
+        wl = A.make_workload("small", strands=300, width=width, height=height)
+        wl.render.tileSize, wl.render.tileDepthBins, wl.render.fiberRadius = tile, bins, radius
+        wl.render.avgSegmentsPerTile = 1024
+        r = PS.HairRenderer(wl, "cuda:0", shadow_size=256)
     _step_to(r, 2)
     ctx = r.ctx()
     r.tilesPass.cmdClearBeforeRender(ctx)
@@ -197,6 +208,14 @@ def test_tiles_bit_exact(gpu, oracle, tile, bins, width, height):
     assert np.array_equal(got_cnt, ec)
     gl, el = walk_lists(got_tiles, got_seg, tp.segmentsCapacity), walk_lists(et, es, tp.segmentsCapacity)
     assert gl == el
+    if radius is not None:
+        tiles_of = {}
+        for tb, vals in gl.items():
+            for v in vals:
+                tiles_of.setdefault(v, set()).add(tb // bins)
+        wide = sum(len(t) > 8 for t in tiles_of.values())
+        print(f"segments drawn into more than 8 tiles: {wide} of {len(tiles_of)}", flush=True)
+        assert wide > 100
     n = min(int(got_seg[0]), tp.segmentsCapacity)
     # every slot is tagged with the (tile, bin) whose list holds it and with a
     # rank in that list; the ranks of one list are 0 .. count-1
