@@ -93,6 +93,49 @@ def stage_bytes(wl, n_pairs: int, tiles: int) -> dict:
     }
 
 
+def full_frame(wl, r0, steps: int, dev) -> dict:
+    """Frames of the same workload with the passes that feed the shading pass
+    computed on the device every frame (ShadowMapPass's hair draw, HwHairPass,
+    AoPass), as the reference's renderer runs them; replayed as HIP graphs."""
+    import torch
+    from frostbitten_hair_webgpu_amd import passes as PS
+    inputs = {k: v.cpu().numpy() for k, v in r0.inputs.items()}
+    r = PS.HairRenderer(wl, dev, inputs=inputs, hw_passes=True)
+    ring = torch.from_numpy(np.concatenate([r.sim_uniforms(f).view(np.uint8).reshape(-1)
+                                            for f in range(2 * steps + 2)])).to(dev).view(2 * steps + 2, -1)
+    r.beforeFirstFrame()
+    r.frame()
+    torch.cuda.synchronize()
+    r.capture_frames()
+
+    def step():
+        r.simulationUniforms.copy_(ring[r.frameIdx])
+        r.frame_graph(upload_uniforms=False)
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    r.native.profiler_enable(64 * steps)
+    r.capture_frames()
+    torch.cuda.synchronize()
+    r.native.profiler_read()
+    for _ in range(steps - 1):
+        step()
+    torch.cuda.synchronize()
+    kernels = r.native.profiler_read()
+    r.native.profiler_enable(0)
+    extra = ("hair_raster", "hair_resolve", "gtao")
+    return {"value": round(1e3 / ms, 3), "unit": "frames/s", "ms_per_step": round(ms, 4), "steps": steps,
+            "passes": "chain + ShadowMapPass hair draw (2048^2) + HwHairPass (depth + normals) + AoPass (GTAO, "
+                      "half resolution)",
+            "kernel_us": {k: round(t / n * 1e3, 2) for k, (n, t) in sorted(kernels.items(), key=lambda kv: -kv[1][1])
+                          if k in extra or k in ("copy_u32", "fill_u32")}}
+
+
 def read_traffic(kernel: str, workload: str):
     """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary committed under profiles/ (None when absent)."""
@@ -291,6 +334,9 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2", choices=["small", "c2", "c3"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--full-frame", type=int, default=1,
+                    help="also time frames that compute the hair depth / normals, shadow map and AO every frame "
+                         "(0 to skip)")
     ap.add_argument("--batch-instances", type=int, default=64,
                     help="instances of the batched-versus-plain comparison (0: skip it)")
     ap.add_argument("--no-tune", action="store_true", help="batched comparison without the autotuner")
@@ -486,6 +532,8 @@ def main() -> None:
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, r)
+    if args.full_frame:
+        out["full_frame"] = full_frame(wl, r, min(args.steps, 30), dev)
     if args.batch_instances:
         del r
         out["batched"] = batched_vs_plain(args.batch_instances, min(args.steps, 20), 2, not args.no_tune, dev,

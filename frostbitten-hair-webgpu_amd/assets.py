@@ -350,6 +350,27 @@ def _sphere_trace(col: Collider, origins: np.ndarray, dirs: np.ndarray, t_max: f
 def mesh_depth(wl: Workload, uniforms: np.ndarray) -> np.ndarray:
     """Depth texture (H x W, texture rows top to bottom) of the collider meshes
     as drawMeshesPass would leave it; 1.0 where nothing is drawn."""
+    return mesh_depth_normals(wl, uniforms)[0]
+
+
+def octahedral_rg16(n: np.ndarray) -> np.ndarray:
+    """encodeOctahedronNormal (shaderSnippets.wgls.ts:25-46) of unit normals
+    (..., 3) as rg16float texels (uint32, r in the low half)."""
+    n = n / np.abs(n).sum(axis=-1, keepdims=True)
+    x, y = n[..., 0].copy(), n[..., 1].copy()
+    neg = n[..., 2] < 0
+    sx, sy = np.where(x >= 0, 1.0, -1.0), np.where(y >= 0, 1.0, -1.0)
+    wx, wy = (1.0 - np.abs(y)) * sx, (1.0 - np.abs(x)) * sy
+    x, y = np.where(neg, wx, x), np.where(neg, wy, y)
+    r = (x * 0.5 + 0.5).astype(np.float16).view(np.uint16).astype(np.uint32)
+    g = (y * 0.5 + 0.5).astype(np.float16).view(np.uint16).astype(np.uint32)
+    return r | (g << 16)
+
+
+def mesh_depth_normals(wl: Workload, uniforms: np.ndarray) -> tuple:
+    """Depth (H x W f32, 1.0 where nothing is drawn) and rg16float octahedral
+    world-space normals (H x W u32, 0 where nothing is drawn) of the collider
+    meshes, as drawMeshesPass leaves them."""
     W, H = wl.width, wl.height
     vp = uniforms["vpMatrix"][0].reshape(4, 4).T.astype(np.float64)
     inv = np.linalg.inv(vp)
@@ -368,7 +389,15 @@ def mesh_depth(wl: Workload, uniforms: np.ndarray) -> np.ndarray:
     clip = np.concatenate([p, np.ones((p.shape[0], 1))], axis=1) @ vp.T
     z = clip[:, 2] / clip[:, 3]
     depth = np.where(hit, z, 1.0).astype(np.float32)
-    return depth.reshape(H, W)
+    normals = np.zeros(W * H, dtype=np.uint32)
+    if hit.any():
+        q = p[hit]
+        e = 1e-4
+        g = np.stack([wl.collider.sdf(q + e * np.eye(3)[k]) - wl.collider.sdf(q - e * np.eye(3)[k])
+                      for k in range(3)], axis=1)
+        g /= np.maximum(np.linalg.norm(g, axis=1, keepdims=True), 1e-12)
+        normals[hit] = octahedral_rg16(g)
+    return depth.reshape(H, W), normals.reshape(H, W)
 
 
 def hair_depth(wl: Workload, uniforms: np.ndarray, positions: np.ndarray, base: np.ndarray) -> np.ndarray:

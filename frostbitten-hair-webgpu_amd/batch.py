@@ -152,7 +152,7 @@ class BatchRenderer:
 def instance_bytes(r: PS.HairRenderer) -> int:
     """Device memory held by one instance's buffers (the library scratch
     excluded)."""
-    ts = list(r.inputs.values()) + [r.globalUniforms, r.hdrRenderTexture, r.screenTexture]
+    ts = list(r.inputs.values()) + [v for v in vars(r).values() if isinstance(v, torch.Tensor)]
     for obj in (r.hair, r.grid, r.sdf, r.tilesPass, r.sortPass, r.finePass):
         for v in vars(obj).values():
             if isinstance(v, torch.Tensor):

@@ -1,0 +1,500 @@
+// Producers of the shading pass's inputs that the reference draws with the
+// hardware rasterizer (gfx950): the hair's depth and normals (HwHairPass),
+// the hair in the shadow map (ShadowMapPass's hair draw) and GTAO (AoPass).
+// Reference: src/passes/hwHair/hwHairPass.wgsl.ts:46-99,
+// hwHair/shaderImpl/hwRasterizeHair.wgsl.ts:15-60, hwHair/hwHairPass.ts
+// :79-141, scene/hair/hairIndexBuffer.ts:20-54,
+// shadowMapPass/shadowMapHairPass.wgsl.ts, shadowMapPass/shadowMapPass.ts
+// :100-115, aoPass/aoPass.wgsl.ts:48-79, aoPass/gtao.wgsl.ts:26-136.
+//
+// Rasterization. Each segment is the reference's two triangles (indices
+// 2i, 2i+1, 2i+2 and 2i+2, 2i+1, 2i+3 over the strand's left / right edge
+// vertices). Vertices are snapped to 1/256 px; a pixel centre is covered when
+// the three integer edge functions are positive, or zero on a top or left
+// edge, so triangles sharing an edge cover each pixel once. Depth is the
+// screen-space barycentric interpolation of NDC z; fragments outside
+// [0, 1] are clipped, triangles with a vertex at w <= 0 or beyond 2^20 px are
+// not drawn. The depth test is the pipelines' 'less' (shared.ts:22-26)
+// against the depth already in the texture (the meshes), and among the hair
+// fragments of one pixel the least depth wins, the earliest triangle on a
+// tie, as in API order: each fragment does one 64-bit atomicMin of (depth
+// bits, triangle index) into a per-pixel key, and a resolve pass writes the
+// winner's depth and its perspective-correct normal. The result does not
+// depend on the order the triangles run in.
+#include <algorithm>
+
+#include "fh_device.hpp"
+#include "fh_launch.hpp"
+
+namespace fh {
+int fill_u32(fh_ctx* ctx, void* dst, uint64_t n_words, uint32_t value);
+}
+
+namespace {
+
+using namespace fh;
+
+constexpr float kFloatEps = 1e-7f;  // FLOAT_EPSILON (shaderSnippets.wgls.ts:67)
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kHalfPi = 1.57079632679489661923f;
+constexpr float kNear = 0.01f, kFar = 100.0f;  // CONFIG.camera.projection (constants.ts:131-135)
+constexpr uint64_t kNoHair = ~0ull;
+
+__device__ __forceinline__ f3 safe_normalize3(f3 v) {
+  return dot(v, v) >= kFloatEps ? normalize(v) : mk3(0.0f, 0.0f, 0.0f);
+}
+
+// encodeOctahedronNormal (shaderSnippets.wgls.ts:25-46), packed as rg16float
+__device__ __forceinline__ uint32_t encode_oct(f3 n0) {
+  const float s = fabsf(n0.x) + fabsf(n0.y) + fabsf(n0.z);
+  f3 n = mk3(n0.x / s, n0.y / s, n0.z / s);
+  if (n.z < 0.0f) {
+    const float sx = n.x >= 0.0f ? 1.0f : -1.0f, sy = n.y >= 0.0f ? 1.0f : -1.0f;
+    const float ax = (1.0f - fabsf(n.y)) * sx, ay = (1.0f - fabsf(n.x)) * sy;
+    n.x = ax;
+    n.y = ay;
+  }
+  return pack_half2(n.x * 0.5f + 0.5f, n.y * 0.5f + 0.5f);
+}
+
+__device__ __forceinline__ f3 decode_oct(uint32_t packed) {
+  const float fx = half_lo(packed) * 2.0f - 1.0f, fy = half_hi(packed) * 2.0f - 1.0f;
+  f3 n = mk3(fx, fy, 1.0f - fabsf(fx) - fabsf(fy));
+  const float t = satf(-n.z);
+  n.x = n.x >= 0.0f ? n.x - t : n.x + t;
+  n.y = n.y >= 0.0f ? n.y - t : n.y + t;
+  return normalize(n);
+}
+
+struct RasterArgs {
+  const fh_render_uniforms* u;
+  const float4* pos;
+  const float4* tan;
+  float* depth;             // the depth texture: tested ('less'), then written by the resolve
+  uint32_t* normals;        // rg16float, may be null (shadow map: depth only)
+  unsigned long long* key;  // per pixel: (depth bits << 32) | triangle, kNoHair when empty
+  uint32_t W, H, P, nTri, shadow;
+};
+
+// Clip-space corner `side` (0: +thickness, 'right'; 1: -thickness, 'left') of
+// point `idx` — hwRasterizeHair (hwRasterizeHair.wgsl.ts:23-48)
+__device__ __forceinline__ f4 edge_vertex(const RasterArgs& a, uint32_t idx, uint32_t side) {
+  const fh_render_uniforms* __restrict__ u = a.u;
+  const float* mv = a.shadow ? u->shadows.sourceModelViewMat : u->modelViewMat;
+  const float* proj = a.shadow ? u->shadows.sourceProjMatrix : u->projMatrix;
+  const float radius = a.shadow ? u->shadows.sourcePosition[3] : u->fiberRadius;  // getShadowFiberRadius
+  const float4 p = a.pos[idx], t = a.tan[idx];
+  const f4 pv = mul(mv, mk4(p.x, p.y, p.z, 1.0f));
+  const f4 tv = mul(mv, mk4(t.x, t.y, t.z, 1.0f));
+  const f3 right = safe_normalize3(cross(xyz(tv), mk3(0.0f, 0.0f, 1.0f)));
+  const f3 th = right * radius;
+  const f3 e = side ? xyz(pv) - th : xyz(pv) + th;
+  return mul(proj, mk4(e.x, e.y, e.z, 1.0f));
+}
+
+// Point index and side of the three vertices of triangle t (hairIndexBuffer.ts:39-46)
+__device__ __forceinline__ void tri_vertices(uint32_t t, uint32_t P, uint32_t* idx, uint32_t* side) {
+  const uint32_t seg = t >> 1, strand = seg / (P - 1u), j = seg - strand * (P - 1u);
+  const uint32_t id = strand * P + j;
+  if ((t & 1u) == 0u) {  // 2id, 2id+1, 2id+2
+    idx[0] = id; side[0] = 0u; idx[1] = id; side[1] = 1u; idx[2] = id + 1u; side[2] = 0u;
+  } else {               // 2id+2, 2id+1, 2id+3
+    idx[0] = id + 1u; side[0] = 0u; idx[1] = id; side[1] = 1u; idx[2] = id + 1u; side[2] = 1u;
+  }
+}
+
+struct SetupTri {
+  f4 c[3];           // clip positions, counter-clockwise order after setup
+  long long X[3], Y[3];
+  long long area;
+  uint32_t vi[3];    // original vertex slot of each setup vertex
+  bool ok;
+};
+
+// Snap to 1/256 px and orient so that the edge functions are positive inside.
+__device__ __forceinline__ SetupTri setup_tri(const RasterArgs& a, uint32_t t) {
+  SetupTri s;
+  uint32_t idx[3], side[3];
+  tri_vertices(t, a.P, idx, side);
+  s.ok = true;
+  for (int k = 0; k < 3; k++) {
+    s.c[k] = edge_vertex(a, idx[k], side[k]);
+    s.vi[k] = (uint32_t)k;
+    const f4 c = s.c[k];
+    if (!(c.w > 0.0f)) { s.ok = false; continue; }
+    const float nx = c.x / c.w, ny = c.y / c.w;
+    const float xs = (float)a.W * (nx * 0.5f + 0.5f);
+    const float ys = (float)a.H * (1.0f - (ny * 0.5f + 0.5f));
+    if (!(fabsf(xs) < 1048576.0f && fabsf(ys) < 1048576.0f)) { s.ok = false; continue; }
+    s.X[k] = (long long)rintf(xs * 256.0f);
+    s.Y[k] = (long long)rintf(ys * 256.0f);
+  }
+  if (!s.ok) return s;
+  long long area = (s.X[1] - s.X[0]) * (s.Y[2] - s.Y[0]) - (s.Y[1] - s.Y[0]) * (s.X[2] - s.X[0]);
+  if (area < 0) {
+    const f4 c = s.c[1]; s.c[1] = s.c[2]; s.c[2] = c;
+    long long q = s.X[1]; s.X[1] = s.X[2]; s.X[2] = q;
+    q = s.Y[1]; s.Y[1] = s.Y[2]; s.Y[2] = q;
+    const uint32_t v = s.vi[1]; s.vi[1] = s.vi[2]; s.vi[2] = v;
+    area = -area;
+  }
+  s.area = area;
+  s.ok = area > 0;
+  return s;
+}
+
+__device__ __forceinline__ long long edge_fn_i(long long ax, long long ay, long long bx, long long by, long long px,
+                                               long long py) {
+  return (bx - ax) * (py - ay) - (by - ay) * (px - ax);
+}
+// top edge (horizontal, interior below) or left edge (interior to the right)
+__device__ __forceinline__ bool top_left(long long ax, long long ay, long long bx, long long by) {
+  const long long dx = bx - ax, dy = by - ay;
+  return (dy == 0 && dx > 0) || dy < 0;
+}
+__device__ __forceinline__ bool covers(long long e, bool tl) { return e > 0 || (e == 0 && tl); }
+
+// screen-space barycentrics of a covered pixel centre
+__device__ __forceinline__ void barycentrics(const SetupTri& s, long long px, long long py, float* l) {
+  const long long e0 = edge_fn_i(s.X[1], s.Y[1], s.X[2], s.Y[2], px, py);
+  const long long e1 = edge_fn_i(s.X[2], s.Y[2], s.X[0], s.Y[0], px, py);
+  const long long e2 = edge_fn_i(s.X[0], s.Y[0], s.X[1], s.Y[1], px, py);
+  const float ar = (float)s.area;
+  l[0] = (float)e0 / ar;
+  l[1] = (float)e1 / ar;
+  l[2] = (float)e2 / ar;
+}
+
+__device__ __forceinline__ float frag_depth(const SetupTri& s, const float* l) {
+  const float z0 = s.c[0].z / s.c[0].w, z1 = s.c[1].z / s.c[1].w, z2 = s.c[2].z / s.c[2].w;
+  return l[0] * z0 + l[1] * z1 + l[2] * z2;
+}
+
+__device__ __forceinline__ long long floor_div256(long long v) { return v >= 0 ? v / 256 : -((-v + 255) / 256); }
+__device__ __forceinline__ long long floor_div(long long a, long long b) {  // b > 0
+  return a >= 0 ? a / b : -((-a + b - 1) / b);
+}
+
+// Conservative pixel range [x0, x1] of row centre cy inside edge a -> b
+// (E(cx) = dx (cy - ay) - dy (cx - ax) >= 0), one pixel wider on each side
+// than the exact bound; the caller still tests each pixel exactly.
+__device__ __forceinline__ void edge_span(long long ax, long long ay, long long bx, long long by, long long cy,
+                                          long long& x0, long long& x1) {
+  const long long dx = bx - ax, dy = by - ay;
+  const long long c = dx * (cy - ay) + dy * ax;  // E(cx) = c - dy * cx
+  if (dy == 0) {
+    if (c < 0) { x0 = 1; x1 = 0; }
+    return;
+  }
+  if (dy < 0) {  // E grows with cx: cx >= -c / -dy
+    const long long cxMin = floor_div(-c, -dy);
+    x0 = max(x0, floor_div256(cxMin - 128) - 1);
+  } else {       // E falls with cx: cx <= c / dy
+    const long long cxMax = floor_div(c, dy);
+    x1 = min(x1, floor_div256(cxMax - 128) + 1);
+  }
+}
+
+// The fragments of triangle t on rows y0 + first, y0 + first + stride, ...:
+// per row a conservative x span from the three edge functions, then the
+// exact edge tests, the depth test against the texture and one atomicMin per
+// fragment.
+__device__ __forceinline__ void raster_rows(const RasterArgs& a, const SetupTri& s, uint32_t t, long long first,
+                                            long long stride) {
+  const long long minX = min(s.X[0], min(s.X[1], s.X[2])), maxX = max(s.X[0], max(s.X[1], s.X[2]));
+  const long long minY = min(s.Y[0], min(s.Y[1], s.Y[2])), maxY = max(s.Y[0], max(s.Y[1], s.Y[2]));
+  // pixel centres 256 p + 128 inside [min, max]
+  const long long x0 = max(0ll, -floor_div256(128 - minX)), x1 = min((long long)a.W - 1, floor_div256(maxX - 128));
+  const long long y0 = max(0ll, -floor_div256(128 - minY)), y1 = min((long long)a.H - 1, floor_div256(maxY - 128));
+  const bool tl0 = top_left(s.X[1], s.Y[1], s.X[2], s.Y[2]);
+  const bool tl1 = top_left(s.X[2], s.Y[2], s.X[0], s.Y[0]);
+  const bool tl2 = top_left(s.X[0], s.Y[0], s.X[1], s.Y[1]);
+  for (long long py = y0 + first; py <= y1; py += stride) {
+    const long long cy = py * 256 + 128;
+    long long rx0 = x0, rx1 = x1;
+    edge_span(s.X[1], s.Y[1], s.X[2], s.Y[2], cy, rx0, rx1);
+    edge_span(s.X[2], s.Y[2], s.X[0], s.Y[0], cy, rx0, rx1);
+    edge_span(s.X[0], s.Y[0], s.X[1], s.Y[1], cy, rx0, rx1);
+    rx0 = max(rx0, x0);
+    rx1 = min(rx1, x1);
+    if (rx0 > rx1) continue;
+    // edge values at the row's first centre, then one step of -256 dy per pixel
+    const long long cx0 = rx0 * 256 + 128;
+    long long e0 = edge_fn_i(s.X[1], s.Y[1], s.X[2], s.Y[2], cx0, cy);
+    long long e1 = edge_fn_i(s.X[2], s.Y[2], s.X[0], s.Y[0], cx0, cy);
+    long long e2 = edge_fn_i(s.X[0], s.Y[0], s.X[1], s.Y[1], cx0, cy);
+    const long long d0 = -256 * (s.Y[2] - s.Y[1]), d1 = -256 * (s.Y[0] - s.Y[2]), d2 = -256 * (s.Y[1] - s.Y[0]);
+    for (long long px = rx0; px <= rx1; px++, e0 += d0, e1 += d1, e2 += d2) {
+      const long long cx = px * 256 + 128;
+      if (!(covers(e0, tl0) && covers(e1, tl1) && covers(e2, tl2))) continue;
+      float l[3];
+      barycentrics(s, cx, cy, l);
+      float z = frag_depth(s, l);
+      if (!(z >= 0.0f && z <= 1.0f)) continue;  // depth clip
+      z = z + 0.0f;                              // -0 -> +0
+      const size_t p = (size_t)py * a.W + (size_t)px;
+      if (!(z < a.depth[p])) continue;           // 'less' against the meshes
+      const unsigned long long k = ((unsigned long long)__float_as_uint(z) << 32) | t;
+      atomicMin(&a.key[p], k);
+    }
+  }
+}
+
+// kRowSplit threads per triangle, thread k taking rows k, k + kRowSplit, ...
+// (a tall triangle does not hold one lane while the rest of its wave idles);
+// a triangle whose pixel box is larger than kWaveRaster pixels (a strand
+// passing close to the eye) is drawn by its whole wave, one row per lane.
+constexpr uint32_t kRowSplit = 8;
+constexpr long long kWaveRaster = 1024;
+
+struct HairRaster {
+  using Args = RasterArgs;
+  static constexpr uint32_t kThreads = 256;
+  static constexpr const char* kName = "hair_raster";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) {
+    const uint32_t gid = xcd_order(block, blocks) * kThreads + threadIdx.x;
+    const uint32_t t = gid / kRowSplit, k = gid % kRowSplit;
+    SetupTri s;
+    s.ok = false;
+    if (t < a.nTri) s = setup_tri(a, t);
+    bool big = false;
+    if (s.ok) {
+      const long long bw = (max(s.X[0], max(s.X[1], s.X[2])) - min(s.X[0], min(s.X[1], s.X[2]))) / 256 + 1;
+      const long long bh = (max(s.Y[0], max(s.Y[1], s.Y[2])) - min(s.Y[0], min(s.Y[1], s.Y[2]))) / 256 + 1;
+      if (bw * bh > kWaveRaster) big = k == 0u;
+      else raster_rows(a, s, t, (long long)k, (long long)kRowSplit);
+    }
+    uint64_t todo = __ballot(big);
+    const uint32_t lane = __lane_id();
+    while (todo) {
+      const uint32_t src = (uint32_t)__builtin_ctzll(todo);
+      todo &= todo - 1ull;
+      const uint32_t tb = (uint32_t)__builtin_amdgcn_readlane((int)t, (int)src);
+      const SetupTri sb = setup_tri(a, tb);
+      raster_rows(a, sb, tb, (long long)lane, 64);
+    }
+  }
+};
+
+// The winner of each pixel: its depth, and (camera pass) its normal — the
+// world-space tangent modelMatrix * vec4(tangent, 1) interpolated with
+// perspective correction, normalized and octahedron-encoded (hwHairPass.wgsl.ts:65, :95-96).
+struct HairResolve {
+  using Args = RasterArgs;
+  static constexpr uint32_t kThreads = 256;
+  static constexpr const char* kName = "hair_resolve";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) {
+    (void)blocks;
+    const uint32_t p = block * kThreads + threadIdx.x;
+    if (p >= a.W * a.H) return;
+    const unsigned long long k = a.key[p];
+    if (k == kNoHair) return;
+    const uint32_t t = (uint32_t)k;
+    a.depth[p] = __uint_as_float((uint32_t)(k >> 32));
+    if (!a.normals) return;
+    const SetupTri s = setup_tri(a, t);
+    const uint32_t px = p % a.W, py = p / a.W;
+    float l[3];
+    barycentrics(s, (long long)px * 256 + 128, (long long)py * 256 + 128, l);
+    uint32_t idx[3], side[3];
+    tri_vertices(t, a.P, idx, side);
+    f3 num = mk3(0.0f, 0.0f, 0.0f);
+    float den = 0.0f;
+    for (int q = 0; q < 3; q++) {
+      const float4 tg = a.tan[idx[s.vi[q]]];
+      const f4 tw = mul(a.u->modelMatrix, mk4(tg.x, tg.y, tg.z, 1.0f));
+      const float wq = l[q] / s.c[q].w;
+      num = num + xyz(tw) * wq;
+      den = den + wq;
+    }
+    a.normals[p] = encode_oct(normalize(num / den));
+  }
+};
+
+struct AoArgs {
+  const fh_render_uniforms* u;
+  const float* depth;
+  const uint32_t* normals;
+  float* out;
+  uint32_t W, H, aoW, aoH;
+};
+
+__device__ __forceinline__ float depth_at(const AoArgs& a, f2 px) {
+  // textureLoad(vec2u(px)): u32() saturates at 0, texels past the edge read 0
+  const uint32_t x = f2u(px.x), y = f2u(px.y);
+  return (x < a.W && y < a.H) ? a.depth[(size_t)y * a.W + x] : 0.0f;
+}
+
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fast_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// The loop-invariant parts of reprojectPositionToVS (gtao.wgsl.ts:8-25):
+// projMatrixInv applied to (2 px / viewport - 1, depth, 1), x and y divided
+// by w, z replaced by linearizeDepth(depth).
+struct Reproj {
+  float m[16];
+  float sx, sy;  // 2 / viewport
+};
+
+__device__ __forceinline__ f3 reproject_vs(const AoArgs& a, const Reproj& R, f2 px) {
+  const float d = depth_at(a, px);
+  const float x = px.x * R.sx - 1.0f, y = px.y * R.sy - 1.0f;
+  const float* m = R.m;
+  const float X = m[0] * x + m[4] * y + m[8] * d + m[12];
+  const float Y = m[1] * x + m[5] * y + m[9] * d + m[13];
+  const float Wc = m[3] * x + m[7] * y + m[11] * d + m[15];
+  const float iw = fast_rcp(Wc);
+  return mk3(X * iw, Y * iw, (kNear * kFar) * fast_rcp(kFar + (kNear - kFar) * d));
+}
+
+__device__ __forceinline__ float round_f(float v) { return v < 0.5f ? floorf(v) : ceilf(v); }  // :130-133
+
+// gtao (gtao.wgsl.ts:26-128). The sample loop uses the hardware's reciprocal,
+// square root and sin / cos (about 1 ulp; the reference's WGSL built-ins are
+// specified no tighter).
+__device__ __forceinline__ float gtao(const AoArgs& a, const Reproj& R, f2 positionPx, f3 normalWS) {
+  const fh_render_uniforms* __restrict__ u = a.u;
+  const fh_ao& prm = u->ao;
+  const f3 positionVS = reproject_vs(a, R, positionPx);
+  const f4 nv = mul(u->viewMatrix, mk4(normalWS.x, normalWS.y, normalWS.z, 1.0f));
+  const f3 normalVS = mk3(nv.x, nv.y, -nv.z);
+  const f3 toCamera = normalize(mk3(-positionVS.x, -positionVS.y, -positionVS.z));
+  if (positionVS.z >= 0.999f * kFar) return 0.0f;
+  const float radius = fmaxf((float)prm.numSteps, prm.radius * (positionVS.z / kFar));
+  const float stepSize = radius / (float)prm.numSteps;
+  const float fallScale = fast_rcp(prm.falloffEnd2 - prm.falloffStart2);
+  float ao = 0.0f;
+  const float rotJitter = kPi * fractf(52.9829189f * fractf(positionPx.x * 0.06711056f + positionPx.y * 0.00583715f));
+  const float jitterMod = (positionPx.x + positionPx.y) * 0.25f;
+  const float radiusJitter = fractf(jitterMod) * stepSize * 0.25f;
+  for (uint32_t i = 0; i < prm.numDirections; i++) {
+    float phi = (float)i * (kPi / (float)prm.numDirections) + prm.directionOffset * kPi;
+    phi += rotJitter;
+    float currStep = 1.0f + 0.25f * stepSize;
+    currStep += radiusJitter;
+    const f3 dir = mk3(__cosf(phi), __sinf(phi), 0.0f);
+    float hx = -1.0f, hy = -1.0f;
+    for (uint32_t j = 0; j < prm.numSteps; j++) {
+      const f2 off = mk2(round_f(dir.x * currStep), round_f(dir.y * currStep));
+      f3 ws = reproject_vs(a, R, positionPx + off) - positionVS;
+      float q = dot(ws, ws);
+      float dist = fast_sqrt(q);
+      float cosh = dot(ws, toCamera) * fast_rsqrt(dist);
+      hx = fmaxf(hx, cosh - 2.0f * satf((dist - prm.falloffStart2) * fallScale));
+      ws = reproject_vs(a, R, positionPx - off) - positionVS;
+      q = dot(ws, ws);
+      dist = fast_sqrt(q);
+      cosh = dot(ws, toCamera) * fast_rsqrt(dist);
+      hy = fmaxf(hy, cosh - 2.0f * satf((dist - prm.falloffStart2) * fallScale));
+      currStep += stepSize;
+    }
+    hx = acosf(hx);
+    hy = acosf(hy);
+    const f3 bitangent = normalize(cross(dir, toCamera));
+    const f3 tangent = cross(toCamera, bitangent);
+    const f3 nx = normalVS - bitangent * dot(normalVS, bitangent);
+    const float nnx = length(nx);
+    const float nnxInv = 1.0f / (nnx + 1e-6f);
+    const float cosxi = dot(nx, tangent) * nnxInv;
+    const float gamma = acosf(cosxi) - kHalfPi;
+    const float cosgamma = dot(nx, toCamera) * nnxInv;
+    const float singamma2 = -2.0f * cosxi;
+    hx = gamma + fmaxf(-hx - gamma, -kHalfPi);
+    hy = gamma + fminf(hy - gamma, kHalfPi);
+    ao += nnx * 0.25f * ((hx * singamma2 + cosgamma - __cosf(2.0f * hx - gamma)) +
+                         (hy * singamma2 + cosgamma - __cosf(2.0f * hy - gamma)));
+  }
+  return ao / (float)prm.numDirections;
+}
+
+// AoPass fragment shader (aoPass.wgsl.ts:48-79) per AO texel, stored as
+// r16float (rounded to half, widened back to f32)
+struct Gtao {
+  using Args = AoArgs;
+  static constexpr uint32_t kThreads = 256;
+  static constexpr const char* kName = "gtao";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) {
+    const uint32_t i = xcd_order(block, blocks) * kThreads + threadIdx.x;
+    if (i >= a.aoW * a.aoH) return;
+    const uint32_t x = i % a.aoW, y = i / a.aoW;
+    Reproj R;
+#pragma unroll
+    for (int k = 0; k < 16; k++) R.m[k] = a.u->projMatrixInv[k];
+    R.sx = 2.0f / a.u->viewport[0];
+    R.sy = 2.0f / a.u->viewport[1];
+    const f2 full = mk2(((float)x + 0.5f) / 0.5f, ((float)y + 0.5f) / 0.5f);  // AO_TEXTURE_SCALE = 0.5
+    const uint32_t fx = f2u(full.x), fy = f2u(full.y);
+    const uint32_t n = (fx < a.W && fy < a.H) ? a.normals[(size_t)fy * a.W + fx] : 0u;
+    const float v = gtao(a, R, full, decode_oct(n));
+    a.out[i] = (float)(_Float16)v;
+  }
+};
+
+int raster_hair(fh_ctx* ctx, RasterArgs& a) {
+  const uint64_t px = (uint64_t)a.W * a.H;
+  int rc = ensure_scratch(ctx, px * 8);
+  if (rc) return rc;
+  a.key = static_cast<unsigned long long*>(ctx->scratch);
+  if ((rc = fill_u32(ctx, a.key, px * 2, 0xffffffffu))) return rc;
+  FH_REQUIRE((uint64_t)a.nTri * kRowSplit < (1ull << 32), "hair raster: too many triangles");
+  if ((rc = launch<HairRaster>(ctx, div_ceil(a.nTri * kRowSplit, HairRaster::kThreads), 0, a))) return rc;
+  return launch<HairResolve>(ctx, div_ceil((uint32_t)px, HairResolve::kThreads), 0, a);
+}
+
+}  // namespace
+
+extern "C" {
+
+int fh_hw_hair(fh_ctx* ctx, uint32_t vp_w, uint32_t vp_h, const fh_render_uniforms* uniforms,
+               const fh_vec4* positions, const fh_vec4* tangents, float* depth_texture, void* normals_texture,
+               uint32_t rendered_strands, uint32_t points_per_strand) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(uniforms && positions && tangents && depth_texture, "fh_hw_hair: null buffer");
+  FH_REQUIRE(points_per_strand >= 2, "fh_hw_hair: points_per_strand >= 2");
+  FH_REQUIRE((uint64_t)vp_w * vp_h < (1ull << 31), "fh_hw_hair: viewport too large");
+  const uint64_t nTri = 2ull * rendered_strands * (points_per_strand - 1u);
+  FH_REQUIRE(nTri < (1ull << 31), "fh_hw_hair: too many triangles");
+  if (nTri == 0 || vp_w == 0 || vp_h == 0) return FH_OK;
+  RasterArgs a;
+  a.u = uniforms;
+  a.pos = reinterpret_cast<const float4*>(positions);
+  a.tan = reinterpret_cast<const float4*>(tangents);
+  a.depth = depth_texture;
+  a.normals = static_cast<uint32_t*>(normals_texture);
+  a.W = vp_w; a.H = vp_h; a.P = points_per_strand; a.nTri = (uint32_t)nTri; a.shadow = 0;
+  return raster_hair(ctx, a);
+}
+
+int fh_shadow_map_hair(fh_ctx* ctx, const fh_render_uniforms* uniforms, const fh_vec4* positions,
+                       const fh_vec4* tangents, float* shadow_map, uint32_t shadow_size, uint32_t rendered_strands,
+                       uint32_t points_per_strand) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(uniforms && positions && tangents && shadow_map, "fh_shadow_map_hair: null buffer");
+  FH_REQUIRE(points_per_strand >= 2, "fh_shadow_map_hair: points_per_strand >= 2");
+  FH_REQUIRE(shadow_size > 0 && shadow_size <= 16384, "fh_shadow_map_hair: shadow_size in [1, 16384]");
+  const uint64_t nTri = 2ull * rendered_strands * (points_per_strand - 1u);
+  FH_REQUIRE(nTri < (1ull << 31), "fh_shadow_map_hair: too many triangles");
+  if (nTri == 0) return FH_OK;
+  RasterArgs a;
+  a.u = uniforms;
+  a.pos = reinterpret_cast<const float4*>(positions);
+  a.tan = reinterpret_cast<const float4*>(tangents);
+  a.depth = shadow_map;
+  a.normals = nullptr;
+  a.W = shadow_size; a.H = shadow_size; a.P = points_per_strand; a.nTri = (uint32_t)nTri; a.shadow = 1;
+  return raster_hair(ctx, a);
+}
+
+int fh_calc_ao(fh_ctx* ctx, const fh_render_uniforms* uniforms, fh_tex2d depth, const void* normals_texture,
+          float* ao_texture, uint32_t ao_w, uint32_t ao_h) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(uniforms && depth.data && normals_texture && ao_texture, "fh_calc_ao: null buffer");
+  FH_REQUIRE((uint64_t)ao_w * ao_h < (1ull << 31), "fh_calc_ao: AO texture too large");
+  if (ao_w == 0 || ao_h == 0) return FH_OK;
+  AoArgs a{uniforms, depth.data, static_cast<const uint32_t*>(normals_texture), ao_texture, depth.width,
+           depth.height, ao_w, ao_h};
+  return launch<Gtao>(ctx, div_ceil(ao_w * ao_h, Gtao::kThreads), 0, a);
+}
+
+}  // extern "C"

@@ -35,6 +35,9 @@ class Tex2D(C.Structure):
     """struct fh_tex2d"""
     _fields_ = [("data", vp), ("width", u32), ("height", u32)]
 
+    def ptr(self) -> vp:
+        return vp(self.data)
+
 
 class KernelTime(C.Structure):
     """struct fh_kernel_time"""
@@ -75,6 +78,9 @@ SIGNATURES = {
     "fh_hair_combine": (i32, [vp, C.POINTER(RenderCfg), u32, u32, vp, vp, vp, vp, vp, vp]),
     "fh_present": (i32, [vp, u32, u32, vp, vp, Tex2D, Tex2D, vp, u32, vp, u32]),
     "fh_copy": (i32, [vp, vp, vp, C.c_uint64]),
+    "fh_hw_hair": (i32, [vp, u32, u32, vp, vp, vp, vp, vp, u32, u32]),
+    "fh_shadow_map_hair": (i32, [vp, vp, vp, vp, vp, u32, u32, u32]),
+    "fh_calc_ao": (i32, [vp, vp, Tex2D, vp, vp, u32, u32]),
     "fh_profiler_enable": (i32, [vp, u32]),
     "fh_capture_begin": (i32, [vp]),
     "fh_capture_end": (i32, [vp, C.POINTER(vp)]),

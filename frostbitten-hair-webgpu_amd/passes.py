@@ -283,6 +283,45 @@ class PresentPass:
                         1 if self.ditherPerPixel else 0, N.ptr(screenTexture), rowBytes)
 
 
+class ShadowMapPass:
+    """The hair draw of src/passes/shadowMapPass/shadowMapPass.ts:100-115 into
+    the shadow map; the meshes' part (drawn first, :94-99) is an input."""
+    NAME = "ShadowMapPass"
+
+    def cmdUpdateShadowMap(self, ctx: PassCtx, hair: HairObject, meshShadowDepth: torch.Tensor) -> None:
+        sm = ctx.shadowDepthTexture
+        ctx.native.call("fh_copy", N.Tex2D.ptr(sm), N.ptr(meshShadowDepth), meshShadowDepth.numel() * 4)
+        ctx.native.call("fh_shadow_map_hair", N.ptr(ctx.globalUniforms), N.ptr(hair.positionsNow),
+                        N.ptr(hair.tangents), N.Tex2D.ptr(sm), sm.width, hair.getRenderedStrandCount(),
+                        hair.pointsPerStrand)
+
+
+class HwHairPass:
+    """src/passes/hwHair/hwHairPass.ts:79-141: the strands' depth and normals
+    drawn over the meshes' (both restored from the inputs first)."""
+    NAME = "HwHairPass"
+
+    def cmdDrawHair(self, ctx: PassCtx, hair: HairObject, meshDepth: torch.Tensor,
+                    meshNormals: torch.Tensor) -> None:
+        w, h = ctx.viewport
+        d = ctx.shadingDepthTexture
+        ctx.native.call("fh_copy", N.Tex2D.ptr(d), N.ptr(meshDepth), meshDepth.numel() * 4)
+        ctx.native.call("fh_copy", N.ptr(ctx.normalsTexture), N.ptr(meshNormals), meshNormals.numel() * 4)
+        ctx.native.call("fh_hw_hair", w, h, N.ptr(ctx.globalUniforms), N.ptr(hair.positionsNow),
+                        N.ptr(hair.tangents), N.Tex2D.ptr(d), N.ptr(ctx.normalsTexture),
+                        hair.getRenderedStrandCount(), hair.pointsPerStrand)
+
+
+class AoPass:
+    """src/passes/aoPass/aoPass.ts:45-86 (GTAO at half resolution)."""
+    NAME = "AoPass"
+
+    def cmdCalcAo(self, ctx: PassCtx) -> None:
+        ao = ctx.aoTexture
+        ctx.native.call("fh_calc_ao", N.ptr(ctx.globalUniforms), ctx.shadingDepthTexture, N.ptr(ctx.normalsTexture),
+                        N.Tex2D.ptr(ao), ao.width, ao.height)
+
+
 class HairRenderer:
     """The hair part of src/renderer.ts: beforeFirstFrame (:145-169),
     cmdRender -> simulateHair (:212-240) -> cmdDrawScene (:242-283) ->
@@ -290,7 +329,7 @@ class HairRenderer:
     outputs (depth, AO, shadow map) provided as precomputed inputs."""
 
     def __init__(self, wl: A.Workload, device="cuda", native: N.Context | None = None, shadow_size: int = 2048,
-                 inputs: dict | None = None):
+                 inputs: dict | None = None, hw_passes: bool = False):
         self.wl = wl
         self.device = torch.device(device)
         self.native = native or N.Context(self.device.index or 0)
@@ -327,6 +366,18 @@ class HairRenderer:
         # texture receives the presented rgba8 sRGB image
         self.hdrRenderTexture = torch.empty_like(self.inputs["hdrScene"])
         self.screenTexture = torch.zeros((H, W, 4), dtype=torch.uint8, device=self.device)
+        # hw_passes: the hair's depth, normals, shadow-map draw and AO are
+        # computed every frame (HwHairPass, ShadowMapPass, AoPass) from the
+        # meshes' inputs instead of taken precomputed
+        self.hw_passes = hw_passes
+        if hw_passes:
+            self.shadowMapPass = ShadowMapPass()
+            self.hwHairPass = HwHairPass()
+            self.aoPass = AoPass()
+            self.depthTexture = torch.empty_like(self.inputs["meshDepth"])
+            self.normalsTexture = torch.empty_like(self.inputs["meshNormals"])
+            self.aoTexture = torch.zeros(((H + 1) // 2, (W + 1) // 2), dtype=torch.float32, device=self.device)
+            self.shadowTexture = torch.empty_like(self.inputs["shadowMap"])
 
     def sim_uniforms(self, frame_idx: int) -> np.ndarray:
         return L.pack_sim_uniforms(self.wl.sim, self.model, self.wl.sdf_bounds, self.grid.bounds, frame_idx)
@@ -334,13 +385,18 @@ class HairRenderer:
     def ctx(self) -> PassCtx:
         W, H = self.wl.width, self.wl.height
         i = self.inputs
+        if self.hw_passes:
+            depth, ao, shadow = self.depthTexture, self.aoTexture, self.shadowTexture
+        else:
+            depth, ao, shadow = i["hairDepth"], i["ao"], i["shadowMap"]
         return PassCtx(
             native=self.native, cfg=self.cfg, viewport=(W, H), frameIdx=self.frameIdx,
             globalUniforms=self.globalUniforms, simulationUniforms=self.simulationUniforms,
             physicsForcesGrid=self.grid, sdfCollider=self.sdf,
-            depthTexture=N.tex(i["meshDepth"], W, H), shadingDepthTexture=N.tex(i["hairDepth"], W, H),
-            aoTexture=N.tex(i["ao"], i["ao"].shape[1], i["ao"].shape[0]),
-            shadowDepthTexture=N.tex(i["shadowMap"], i["shadowMap"].shape[1], i["shadowMap"].shape[0]),
+            depthTexture=N.tex(i["meshDepth"], W, H), shadingDepthTexture=N.tex(depth, W, H),
+            aoTexture=N.tex(ao, ao.shape[1], ao.shape[0]),
+            shadowDepthTexture=N.tex(shadow, shadow.shape[1], shadow.shape[0]),
+            normalsTexture=self.normalsTexture if self.hw_passes else None,
             hairTilesBuffer=self.tilesPass.hairTilesBuffer,
             hairTileSegmentsBuffer=self.tilesPass.hairTileSegmentsBuffer,
             hairSegmentCountPerTileBuffer=self.tilesPass.segmentCountPerTileBuffer,
@@ -372,6 +428,8 @@ class HairRenderer:
         """cmdDrawScene's hair part (renderer.ts:262-295) and the present pass
         (:205)."""
         self.drawScene(ctx)
+        if self.hw_passes:
+            self.shadowMapPass.cmdUpdateShadowMap(ctx, self.hair, self.inputs["shadowMap"])
         self.tilesPass.cmdClearBeforeRender(ctx)
         self.finePass.cmdClearBeforeRender(ctx)
         self.sortPass.cmdClearBeforeRender(ctx)
@@ -379,6 +437,9 @@ class HairRenderer:
         self.sortPass.cmdSortHairTiles(ctx)
         self.finePass.cmdRasterizeSlicesHair(ctx, self.hair, self.tilesPass)
         self.hairCombinePass.cmdCombineRasterResults(ctx)
+        if self.hw_passes:  # updateResourcesForNextFrame (renderer.ts:285-295)
+            self.hwHairPass.cmdDrawHair(ctx, self.hair, self.inputs["meshDepth"], self.inputs["meshNormals"])
+            self.aoPass.cmdCalcAo(ctx)
         self.shadingPass.cmdComputeShadingPoints(ctx, self.hair)
         self.presentPass.cmdDraw(ctx, self.screenTexture)
 
@@ -441,10 +502,13 @@ def workload_render_uniforms(wl: A.Workload, model: np.ndarray | None = None) ->
 
 
 def make_inputs(wl: A.Workload, render_uniforms: np.ndarray, shadow_size: int = 2048) -> dict:
-    """Depth (mesh only, and mesh + hair), AO and shadow map for a workload."""
-    md = A.mesh_depth(wl, render_uniforms)
+    """The inputs the out-of-scope passes produce: the meshes' depth and
+    normals, a precomputed depth with the hair splatted in, a flat AO, the
+    meshes' shadow map and the HDR scene (background and meshes)."""
+    md, mn = A.mesh_depth_normals(wl, render_uniforms)
     hd = A.hair_depth(wl, render_uniforms, wl.positions, md)
-    ao = np.ones((max(1, wl.height // 2), max(1, wl.width // 2)), dtype=np.float32)
+    ao = np.ones(((wl.height + 1) // 2, (wl.width + 1) // 2), dtype=np.float32)
     sm = A.shadow_map(wl, render_uniforms, shadow_size)
     hdr = A.hdr_scene(wl, md)
-    return {"meshDepth": md, "hairDepth": hd, "ao": ao, "shadowMap": sm, "hdrScene": hdr}
+    return {"meshDepth": md, "meshNormals": mn.view(np.int32), "hairDepth": hd, "ao": ao, "shadowMap": sm,
+            "hdrScene": hdr}

@@ -71,7 +71,7 @@ def estimate_bytes(wl) -> int:
                + 16 * T * rc.tileDepthBins + 8 * entries + 8 * entries + 8 * T
                + 16 * rc.processorCount * per_proc + 16 * W * H
                + 8 * W * H + W * H + 4 * sm * sm + 16 * wl.strands * rc.shadingPoints
-               + 20 * W * H)                                # HDR scene + HDR texture + screen
+               + 24 * W * H)                                # HDR scene + HDR texture + screen + mesh normals
     scratch = 36 * N_ + 16 * entries + 8 * T * rc.tileDepthBins + (32 << 20)
     return int(buffers + scratch)
 

@@ -425,6 +425,47 @@ int fh_present(fh_ctx* ctx, uint32_t vp_w, uint32_t vp_h, const fh_render_unifor
                const void* hdr_texture, fh_tex2d depth, fh_tex2d ao, const void* normals_texture,
                uint32_t dither_per_pixel, void* screen_texture, uint32_t screen_row_bytes);
 
+/* ------------------------------------------------------------------------ */
+/* Hair depth and normals, shadow map, AO (the reference's hardware-raster   */
+/* passes that feed the shading pass)                                        */
+/* ------------------------------------------------------------------------ */
+
+/* The reference draws these with the hardware rasterizer; here they are a
+ * software rasterizer with fixed rules: vertices snapped to 1/256 px,
+ * top-left fill rule at pixel centres, screen-space interpolated NDC depth
+ * clipped to [0, 1], the pipelines' 'less' test (shared.ts:22-26) in
+ * primitive order, triangles with a vertex at w <= 0 not drawn. The result
+ * does not depend on the order the GPU runs the triangles in. Uses the
+ * context's scratch (8 B per target pixel). */
+
+/* HwHairPass.cmdDrawHair (hwHairPass.ts:79-141; vertex shader
+ * hwRasterizeHair.wgsl.ts:23-48, fragment shader hwHairPass.wgsl.ts:77-99):
+ * the strands' two triangles per segment (hairIndexBuffer.ts:20-54) drawn
+ * into depth_texture (in: the meshes' depth; out: with the hair) and, where
+ * the hair wins, normals_texture (rg16float octahedral world-space tangent,
+ * W x H u32; may be NULL). The colour target is not written (display mode
+ * FINAL writes alpha 0 there). */
+int fh_hw_hair(fh_ctx* ctx, uint32_t vp_w, uint32_t vp_h, const fh_render_uniforms* uniforms,
+               const fh_vec4* positions, const fh_vec4* tangents, float* depth_texture,
+               void* normals_texture, uint32_t rendered_strands, uint32_t points_per_strand);
+
+/* The hair draw of ShadowMapPass.cmdUpdateShadowMap (shadowMapPass.ts
+ * :100-115; shadowMapHairPass.wgsl.ts): the same triangles from the shadow
+ * source (shadows.sourceModelViewMat / sourceProjMatrix, fiber radius in
+ * shadows.sourcePosition.w) into shadow_map (shadow_size^2 f32; in: the
+ * meshes' shadow depth). */
+int fh_shadow_map_hair(fh_ctx* ctx, const fh_render_uniforms* uniforms, const fh_vec4* positions,
+                       const fh_vec4* tangents, float* shadow_map, uint32_t shadow_size,
+                       uint32_t rendered_strands, uint32_t points_per_strand);
+
+/* AoPass.cmdCalcAo (aoPass.ts:45-86; aoPass.wgsl.ts:48-79, gtao.wgsl.ts
+ * :26-136): GTAO from the depth texture (with the hair) and the normals
+ * texture into the AO texture (ao_w x ao_h = ceil(W / 2) x ceil(H / 2),
+ * CONFIG.ao.textureSizeMul, renderer.ts:406-416), r16float values widened
+ * to f32. */
+int fh_calc_ao(fh_ctx* ctx, const fh_render_uniforms* uniforms, fh_tex2d depth, const void* normals_texture,
+               float* ao_texture, uint32_t ao_w, uint32_t ao_h);
+
 /* Device-to-device copy as a kernel (bytes a multiple of 4), so that a
  * captured frame can restore its inputs (e.g. the HDR scene drawn by the
  * out-of-scope mesh passes) without memcpy nodes. */

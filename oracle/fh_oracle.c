@@ -1152,3 +1152,287 @@ void or_present(uint32_t W, uint32_t H, const fh_render_uniforms* u,
     }
   }
 }
+
+/* ======================================================================== */
+/* Hair depth / normals, shadow-map hair, GTAO                               */
+/* ======================================================================== */
+
+/* The reference draws the hair's depth and normals (HwHairPass) and the hair
+ * in the shadow map with the hardware rasterizer. This restatement fixes one
+ * rasterizer: vertices snapped to 1/256 px, integer edge functions at pixel
+ * centres with the top-left rule, screen-space interpolation of NDC z, depth
+ * clip to [0, 1], no triangle with a vertex at w <= 0 or beyond 2^20 px, and
+ * the pipelines' 'less' test in primitive order (triangles run in index
+ * order; a later fragment must be strictly nearer). Pinned:
+ * test_oracle_hwraster.py (watertight coverage of shared edges, exact pixel
+ * sets of pixel-aligned quads, constant and planar depth, order of equal
+ * depths, occlusion by the meshes, octahedral normals of the axes). */
+static ov4 hw_edge_vertex(const fh_render_uniforms* u, int shadow, const fh_vec4* pos,
+                          const fh_vec4* tan, uint32_t idx, uint32_t side) {
+  const float* mv = shadow ? u->shadows.sourceModelViewMat : u->modelViewMat;
+  const float* proj = shadow ? u->shadows.sourceProjMatrix : u->projMatrix;
+  const float radius = shadow ? u->shadows.sourcePosition[3] : u->fiberRadius;
+  const ov4 pv = mulmv(mv, o4(pos[idx].x, pos[idx].y, pos[idx].z, 1.0f));
+  const ov4 tv = mulmv(mv, o4(tan[idx].x, tan[idx].y, tan[idx].z, 1.0f));
+  ov3 right = cross3(xyz(tv), o3(0.0f, 0.0f, 1.0f));
+  right = dot3(right, right) >= FLOAT_EPSILON ? norm3(right) : o3(0.0f, 0.0f, 0.0f);
+  const ov3 th = muls3(right, radius);
+  const ov3 e = side ? sub3(xyz(pv), th) : add3(xyz(pv), th);
+  return mulmv(proj, o4(e.x, e.y, e.z, 1.0f));
+}
+
+typedef struct {
+  ov4 c[3];
+  long long X[3], Y[3], area;
+  uint32_t vi[3];
+  int ok;
+} hw_tri;
+
+static void hw_tri_vertices(uint32_t t, uint32_t P, uint32_t* idx, uint32_t* side) {
+  const uint32_t seg = t >> 1, strand = seg / (P - 1u), j = seg - strand * (P - 1u);
+  const uint32_t id = strand * P + j;
+  if ((t & 1u) == 0u) {
+    idx[0] = id; side[0] = 0; idx[1] = id; side[1] = 1; idx[2] = id + 1; side[2] = 0;
+  } else {
+    idx[0] = id + 1; side[0] = 0; idx[1] = id; side[1] = 1; idx[2] = id + 1; side[2] = 1;
+  }
+}
+
+static hw_tri hw_setup(const fh_render_uniforms* u, int shadow, const fh_vec4* pos,
+                       const fh_vec4* tan, uint32_t P, uint32_t W, uint32_t H, uint32_t t) {
+  hw_tri s;
+  uint32_t idx[3], side[3];
+  hw_tri_vertices(t, P, idx, side);
+  s.ok = 1;
+  for (int k = 0; k < 3; k++) {
+    s.c[k] = hw_edge_vertex(u, shadow, pos, tan, idx[k], side[k]);
+    s.vi[k] = (uint32_t)k;
+    s.X[k] = s.Y[k] = 0;
+    if (!(s.c[k].w > 0.0f)) { s.ok = 0; continue; }
+    const float nx = s.c[k].x / s.c[k].w, ny = s.c[k].y / s.c[k].w;
+    const float xs = (float)W * (nx * 0.5f + 0.5f);
+    const float ys = (float)H * (1.0f - (ny * 0.5f + 0.5f));
+    if (!(fabsf(xs) < 1048576.0f && fabsf(ys) < 1048576.0f)) { s.ok = 0; continue; }
+    s.X[k] = (long long)rintf(xs * 256.0f);
+    s.Y[k] = (long long)rintf(ys * 256.0f);
+  }
+  s.area = 0;
+  if (!s.ok) return s;
+  long long area = (s.X[1] - s.X[0]) * (s.Y[2] - s.Y[0]) - (s.Y[1] - s.Y[0]) * (s.X[2] - s.X[0]);
+  if (area < 0) {
+    ov4 c = s.c[1]; s.c[1] = s.c[2]; s.c[2] = c;
+    long long q = s.X[1]; s.X[1] = s.X[2]; s.X[2] = q;
+    q = s.Y[1]; s.Y[1] = s.Y[2]; s.Y[2] = q;
+    uint32_t v = s.vi[1]; s.vi[1] = s.vi[2]; s.vi[2] = v;
+    area = -area;
+  }
+  s.area = area;
+  s.ok = area > 0;
+  return s;
+}
+
+static long long hw_edge(long long ax, long long ay, long long bx, long long by, long long px, long long py) {
+  return (bx - ax) * (py - ay) - (by - ay) * (px - ax);
+}
+static long long hw_floor_div256(long long v) { return v >= 0 ? v / 256 : -((-v + 255) / 256); }
+static long long hw_ceil_div256(long long v) { return -hw_floor_div256(-v); }
+static int hw_top_left(long long ax, long long ay, long long bx, long long by) {
+  const long long dx = bx - ax, dy = by - ay;
+  return (dy == 0 && dx > 0) || dy < 0;
+}
+
+/* encodeOctahedronNormal (shaderSnippets.wgls.ts:25-46) as rg16float bits */
+uint32_t or_encode_oct(float x, float y, float z) {
+  const float s = fabsf(x) + fabsf(y) + fabsf(z);
+  float nx = x / s, ny = y / s, nz = z / s;
+  if (nz < 0.0f) {
+    const float sx = nx >= 0.0f ? 1.0f : -1.0f, sy = ny >= 0.0f ? 1.0f : -1.0f;
+    const float ax = (1.0f - fabsf(ny)) * sx, ay = (1.0f - fabsf(nx)) * sy;
+    nx = ax;
+    ny = ay;
+  }
+  return opack2h(nx * 0.5f + 0.5f, ny * 0.5f + 0.5f);
+}
+
+static ov3 hw_decode_oct(uint32_t packed) {
+  const float fx = oh2f(packed & 0xffffu) * 2.0f - 1.0f, fy = oh2f(packed >> 16) * 2.0f - 1.0f;
+  ov3 n = o3(fx, fy, 1.0f - fabsf(fx) - fabsf(fy));
+  const float t = osat(-n.z);
+  n.x = n.x >= 0.0f ? n.x - t : n.x + t;
+  n.y = n.y >= 0.0f ? n.y - t : n.y + t;
+  return norm3(n);
+}
+
+/* HwHairPass.cmdDrawHair (shadow = 0: depth + normals of the camera view) or
+ * the hair draw of ShadowMapPass (shadow = 1: depth only, shadow source
+ * matrices and fiber radius). depth is read and written in place; normals
+ * (rg16float bits) may be NULL. */
+void or_hw_hair(int shadow, uint32_t W, uint32_t H, const fh_render_uniforms* u,
+                const fh_vec4* pos, const fh_vec4* tan, float* depth, uint32_t* normals,
+                uint32_t S, uint32_t P) {
+  const uint64_t nTri = 2ull * S * (P - 1u);
+  for (uint64_t t = 0; t < nTri; t++) {
+    const hw_tri s = hw_setup(u, shadow, pos, tan, P, W, H, (uint32_t)t);
+    if (!s.ok) continue;
+    long long minX = s.X[0], maxX = s.X[0], minY = s.Y[0], maxY = s.Y[0];
+    for (int k = 1; k < 3; k++) {
+      if (s.X[k] < minX) minX = s.X[k];
+      if (s.X[k] > maxX) maxX = s.X[k];
+      if (s.Y[k] < minY) minY = s.Y[k];
+      if (s.Y[k] > maxY) maxY = s.Y[k];
+    }
+    const int tl0 = hw_top_left(s.X[1], s.Y[1], s.X[2], s.Y[2]);
+    const int tl1 = hw_top_left(s.X[2], s.Y[2], s.X[0], s.Y[0]);
+    const int tl2 = hw_top_left(s.X[0], s.Y[0], s.X[1], s.Y[1]);
+    /* pixel centres 256 p + 128 within [min, max], clipped to the target */
+    long long x0 = hw_ceil_div256(minX - 128), x1 = hw_floor_div256(maxX - 128);
+    long long y0 = hw_ceil_div256(minY - 128), y1 = hw_floor_div256(maxY - 128);
+    if (x0 < 0) x0 = 0;
+    if (y0 < 0) y0 = 0;
+    if (x1 > (long long)W - 1) x1 = (long long)W - 1;
+    if (y1 > (long long)H - 1) y1 = (long long)H - 1;
+    for (long long py = y0; py <= y1; py++) {
+      const long long cy = py * 256 + 128;
+      for (long long px = x0; px <= x1; px++) {
+        const long long cx = px * 256 + 128;
+        const long long e0 = hw_edge(s.X[1], s.Y[1], s.X[2], s.Y[2], cx, cy);
+        const long long e1 = hw_edge(s.X[2], s.Y[2], s.X[0], s.Y[0], cx, cy);
+        const long long e2 = hw_edge(s.X[0], s.Y[0], s.X[1], s.Y[1], cx, cy);
+        if (!((e0 > 0 || (e0 == 0 && tl0)) && (e1 > 0 || (e1 == 0 && tl1)) &&
+              (e2 > 0 || (e2 == 0 && tl2))))
+          continue;
+        const float ar = (float)s.area;
+        const float l0 = (float)e0 / ar, l1 = (float)e1 / ar, l2 = (float)e2 / ar;
+        const float z0 = s.c[0].z / s.c[0].w, z1 = s.c[1].z / s.c[1].w, z2 = s.c[2].z / s.c[2].w;
+        float z = l0 * z0 + l1 * z1 + l2 * z2;
+        if (!(z >= 0.0f && z <= 1.0f)) continue;
+        z = z + 0.0f;
+        const size_t p = (size_t)py * W + (size_t)px;
+        if (!(z < depth[p])) continue;
+        depth[p] = z;
+        if (normals) {
+          uint32_t idx[3], side[3];
+          hw_tri_vertices((uint32_t)t, P, idx, side);
+          const float l[3] = {l0, l1, l2};
+          ov3 num = o3(0.0f, 0.0f, 0.0f);
+          float den = 0.0f;
+          for (int q = 0; q < 3; q++) {
+            const fh_vec4 tg = tan[idx[s.vi[q]]];
+            const ov4 tw = mulmv(u->modelMatrix, o4(tg.x, tg.y, tg.z, 1.0f));
+            const float wq = l[q] / s.c[q].w;
+            num = add3(num, muls3(xyz(tw), wq));
+            den = den + wq;
+          }
+          const ov3 n = norm3(divs3(num, den));
+          normals[p] = or_encode_oct(n.x, n.y, n.z);
+        }
+      }
+    }
+  }
+}
+
+/* Test hook: the number of fragments of the hair triangles at each pixel of
+ * a W x H target (no depth test, no clipping of z): checks that triangles
+ * sharing an edge cover each pixel centre once. */
+void or_hw_coverage(int shadow, uint32_t W, uint32_t H, const fh_render_uniforms* u,
+                    const fh_vec4* pos, const fh_vec4* tan, uint32_t* counts, uint32_t S, uint32_t P) {
+  const uint64_t nTri = 2ull * S * (P - 1u);
+  for (uint64_t t = 0; t < nTri; t++) {
+    const hw_tri s = hw_setup(u, shadow, pos, tan, P, W, H, (uint32_t)t);
+    if (!s.ok) continue;
+    const int tl0 = hw_top_left(s.X[1], s.Y[1], s.X[2], s.Y[2]);
+    const int tl1 = hw_top_left(s.X[2], s.Y[2], s.X[0], s.Y[0]);
+    const int tl2 = hw_top_left(s.X[0], s.Y[0], s.X[1], s.Y[1]);
+    for (long long py = 0; py < (long long)H; py++)
+      for (long long px = 0; px < (long long)W; px++) {
+        const long long cx = px * 256 + 128, cy = py * 256 + 128;
+        const long long e0 = hw_edge(s.X[1], s.Y[1], s.X[2], s.Y[2], cx, cy);
+        const long long e1 = hw_edge(s.X[2], s.Y[2], s.X[0], s.Y[0], cx, cy);
+        const long long e2 = hw_edge(s.X[0], s.Y[0], s.X[1], s.Y[1], cx, cy);
+        if ((e0 > 0 || (e0 == 0 && tl0)) && (e1 > 0 || (e1 == 0 && tl1)) && (e2 > 0 || (e2 == 0 && tl2)))
+          counts[(size_t)py * W + (size_t)px]++;
+      }
+  }
+}
+
+/* AoPass (aoPass.wgsl.ts:48-79) with gtao (gtao.wgsl.ts:26-136) per texel of
+ * the AO texture (aoW x aoH; AO_TEXTURE_SCALE 0.5), stored as r16float
+ * (rounded to half, widened). Pinned: test_oracle_hwraster.py (the
+ * background returns 0; a flat wall facing the camera is unoccluded and
+ * gives the closed form of the integral; a pit is darker than the wall). */
+void or_ao(const fh_render_uniforms* u, const float* depth, uint32_t W, uint32_t H,
+           const uint32_t* normals, float* out, uint32_t aoW, uint32_t aoH) {
+  const float PI_F = 3.14159265358979323846f, HALF_PI_F = 1.57079632679489661923f;
+  const float zNear = 0.01f, zFar = 100.0f;
+  const float vpx = u->viewport[0], vpy = u->viewport[1];
+  const fh_ao* prm = &u->ao;
+#define AO_DEPTH(px_, py_) \
+  ((ou32(px_) < W && ou32(py_) < H) ? depth[(size_t)ou32(py_) * W + ou32(px_)] : 0.0f)
+  for (uint32_t y = 0; y < aoH; y++) {
+    for (uint32_t x = 0; x < aoW; x++) {
+      const float fxp = ((float)x + 0.5f) / 0.5f, fyp = ((float)y + 0.5f) / 0.5f;
+      const uint32_t ix = ou32(fxp), iy = ou32(fyp);
+      const uint32_t nn = (ix < W && iy < H) ? normals[(size_t)iy * W + ix] : 0u;
+      const ov3 normalWS = hw_decode_oct(nn);
+      /* reprojectPositionToVS */
+      float d = AO_DEPTH(fxp, fyp);
+      ov3 posVS = oproject(u->projMatrixInv, o4(fxp / vpx * 2.0f - 1.0f, fyp / vpy * 2.0f - 1.0f, d, 1.0f));
+      posVS.z = zNear * zFar / (zFar + (zNear - zFar) * d);
+      const ov4 nv = mulmv(u->viewMatrix, o4(normalWS.x, normalWS.y, normalWS.z, 1.0f));
+      const ov3 normalVS = o3(nv.x, nv.y, -nv.z);
+      const ov3 toCamera = norm3(o3(-posVS.x, -posVS.y, -posVS.z));
+      float ao = 0.0f;
+      if (posVS.z >= 0.999f * zFar) {
+        out[(size_t)y * aoW + x] = 0.0f;
+        continue;
+      }
+      const float radius = omax((float)prm->numSteps, prm->radius * (posVS.z / zFar));
+      const float stepSize = radius / (float)prm->numSteps;
+      const float rotJitter = PI_F * ofract(52.9829189f * ofract(fxp * 0.06711056f + fyp * 0.00583715f));
+      const float jitterMod = (fxp + fyp) * 0.25f;
+      const float radiusJitter = ofract(jitterMod) * stepSize * 0.25f;
+      for (uint32_t i = 0; i < prm->numDirections; i++) {
+        float phi = (float)i * (PI_F / (float)prm->numDirections) + prm->directionOffset * PI_F;
+        phi += rotJitter;
+        float currStep = 1.0f + 0.25f * stepSize;
+        currStep += radiusJitter;
+        const ov3 dir = o3(cosf(phi), sinf(phi), 0.0f);
+        float hx = -1.0f, hy = -1.0f;
+        for (uint32_t j = 0; j < prm->numSteps; j++) {
+          const float vx = dir.x * currStep, vy = dir.y * currStep;
+          const float ofx = vx < 0.5f ? floorf(vx) : ceilf(vx), ofy = vy < 0.5f ? floorf(vy) : ceilf(vy);
+          for (int sgn = 0; sgn < 2; sgn++) {
+            const float sx = sgn ? fxp - ofx : fxp + ofx, sy = sgn ? fyp - ofy : fyp + ofy;
+            const float sd = AO_DEPTH(sx, sy);
+            ov3 sv = oproject(u->projMatrixInv, o4(sx / vpx * 2.0f - 1.0f, sy / vpy * 2.0f - 1.0f, sd, 1.0f));
+            sv.z = zNear * zFar / (zFar + (zNear - zFar) * sd);
+            const ov3 ws = sub3(sv, posVS);
+            const float dist2 = len3(ws);
+            const float cosh_ = dot3(ws, toCamera) * (1.0f / sqrtf(dist2));
+            const float fall = 2.0f * osat((dist2 - prm->falloffStart2) / (prm->falloffEnd2 - prm->falloffStart2));
+            if (sgn) hy = omax(hy, cosh_ - fall);
+            else hx = omax(hx, cosh_ - fall);
+          }
+          currStep += stepSize;
+        }
+        hx = acosf(hx);
+        hy = acosf(hy);
+        const ov3 bit = norm3(cross3(dir, toCamera));
+        const ov3 tng = cross3(toCamera, bit);
+        const ov3 nx = sub3(normalVS, muls3(bit, dot3(normalVS, bit)));
+        const float nnx = len3(nx);
+        const float nnxInv = 1.0f / (nnx + 1e-6f);
+        const float cosxi = dot3(nx, tng) * nnxInv;
+        const float gamma = acosf(cosxi) - HALF_PI_F;
+        const float cosgamma = dot3(nx, toCamera) * nnxInv;
+        const float singamma2 = -2.0f * cosxi;
+        hx = gamma + omax(-hx - gamma, -HALF_PI_F);
+        hy = gamma + omin(hy - gamma, HALF_PI_F);
+        ao += nnx * 0.25f * ((hx * singamma2 + cosgamma - cosf(2.0f * hx - gamma)) +
+                             (hy * singamma2 + cosgamma - cosf(2.0f * hy - gamma)));
+      }
+      out[(size_t)y * aoW + x] = oh2f(of2h(ao / (float)prm->numDirections));
+    }
+  }
+#undef AO_DEPTH
+}

@@ -77,6 +77,18 @@ void or_present(uint32_t W, uint32_t H, const fh_render_uniforms* u,
                 uint32_t aoW, uint32_t aoH, const uint32_t* normals,
                 int ditherPerPixel, uint8_t* out, uint32_t pitch);
 
+uint32_t or_encode_oct(float x, float y, float z);
+
+void or_hw_hair(int shadow, uint32_t W, uint32_t H, const fh_render_uniforms* u,
+                const fh_vec4* pos, const fh_vec4* tan, float* depth, uint32_t* normals,
+                uint32_t S, uint32_t P);
+
+void or_hw_coverage(int shadow, uint32_t W, uint32_t H, const fh_render_uniforms* u,
+                    const fh_vec4* pos, const fh_vec4* tan, uint32_t* counts, uint32_t S, uint32_t P);
+
+void or_ao(const fh_render_uniforms* u, const float* depth, uint32_t W, uint32_t H,
+           const uint32_t* normals, float* out, uint32_t aoW, uint32_t aoH);
+
 #ifdef __cplusplus
 }
 #endif

@@ -179,3 +179,35 @@ def present(W, H, render_u, hdr, depth, ao, normals=None, dither_per_pixel=False
                      C.c_uint32(ao.shape[1]), C.c_uint32(ao.shape[0]), nptr, C.c_int(1 if dither_per_pixel else 0),
                      _p(out), C.c_uint32(pitch))
     return out
+
+
+def encode_oct(x: float, y: float, z: float) -> int:
+    L = lib()
+    L.or_encode_oct.restype = C.c_uint32
+    L.or_encode_oct.argtypes = [C.c_float, C.c_float, C.c_float]
+    return L.or_encode_oct(x, y, z)
+
+
+def hw_hair(shadow, W, H, render_u, positions, tangents, depth, S, P, normals=None):
+    """Returns (depth, normals) after drawing the hair into copies of the inputs
+    (normals: H x W uint32 rg16float bits, or None for depth only)."""
+    d = np.ascontiguousarray(depth, dtype=np.float32).copy()
+    n = None if normals is None else np.ascontiguousarray(normals, dtype=np.uint32).copy()
+    lib().or_hw_hair(C.c_int(1 if shadow else 0), C.c_uint32(W), C.c_uint32(H), _p(render_u), _p(positions),
+                     _p(tangents), _p(d), _p(n) if n is not None else C.c_void_p(0), C.c_uint32(S), C.c_uint32(P))
+    return d, n
+
+
+def calc_ao(render_u, depth, normals, ao_w, ao_h):
+    out = np.zeros((ao_h, ao_w), dtype=np.float32)
+    H, W = depth.shape
+    lib().or_ao(_p(render_u), _p(np.ascontiguousarray(depth, dtype=np.float32)), C.c_uint32(W), C.c_uint32(H),
+                _p(np.ascontiguousarray(normals, dtype=np.uint32)), _p(out), C.c_uint32(ao_w), C.c_uint32(ao_h))
+    return out
+
+
+def hw_coverage(shadow, W, H, render_u, positions, tangents, S, P):
+    counts = np.zeros((H, W), dtype=np.uint32)
+    lib().or_hw_coverage(C.c_int(1 if shadow else 0), C.c_uint32(W), C.c_uint32(H), _p(render_u), _p(positions),
+                         _p(tangents), _p(counts), C.c_uint32(S), C.c_uint32(P))
+    return counts
