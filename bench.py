@@ -128,7 +128,7 @@ def full_frame(wl, r0, steps: int, dev) -> dict:
     torch.cuda.synchronize()
     kernels = r.native.profiler_read()
     r.native.profiler_enable(0)
-    extra = ("hair_raster", "hair_resolve", "gtao")
+    extra = ("hair_raster", "hair_raster_big", "hair_resolve", "gtao")
     return {"value": round(1e3 / ms, 3), "unit": "frames/s", "ms_per_step": round(ms, 4), "steps": steps,
             "passes": "chain + ShadowMapPass hair draw (2048^2) + HwHairPass (depth + normals) + AoPass (GTAO, "
                       "half resolution)",

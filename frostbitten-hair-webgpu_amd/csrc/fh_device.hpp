@@ -172,4 +172,42 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   return (uint32_t)x;
 }
 
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Inclusive prefix max over the 64 lanes with DPP, the same steps as
+// wave_incl_scan.
+__device__ __forceinline__ int wave_incl_max(int x) {
+  constexpr int lo = -2147483647 - 1;
+  x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x111, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x112, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x114, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x118, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x142, 0xa, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x143, 0xc, 0xf, false));
+  return x;
+}
+
+__device__ __forceinline__ uint32_t wave_last(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// Load-balancing search: lane s owns the items [end_s - cnt_s, end_s) of a
+// sequence (ends from wave_incl_scan of cnt). Returns the lane owning item
+// c0 + lane, valid while c0 + lane < the last end. Every lane whose range
+// meets [c0, c0 + 64) marks its first item there (distinct positions, since
+// the ranges are disjoint), then a prefix max spreads the marks. `mark` is 64
+// ints of wave-private LDS.
+__device__ __forceinline__ uint32_t lbs_owner(int* mark, uint32_t end, uint32_t cnt, uint32_t c0) {
+  const uint32_t lane = __lane_id();
+  mark[lane] = -1;
+  wave_sync();
+  const uint32_t start = end - cnt;
+  if (cnt && end > c0 && start < c0 + 64u) mark[(start > c0 ? start : c0) - c0] = (int)lane;
+  wave_sync();
+  return (uint32_t)wave_incl_max(mark[lane]);
+}
+
 }  // namespace fh

@@ -73,6 +73,8 @@ struct RasterArgs {
   float* depth;             // the depth texture: tested ('less'), then written by the resolve
   uint32_t* normals;        // rg16float, may be null (shadow map: depth only)
   unsigned long long* key;  // per pixel: (depth bits << 32) | triangle, kNoHair when empty
+  uint32_t* bigCount;       // queued bands of large triangles
+  uint2* big;               // (triangle, band)
   uint32_t W, H, P, nTri, shadow;
 };
 
@@ -112,15 +114,14 @@ struct SetupTri {
 };
 
 // Snap to 1/256 px and orient so that the edge functions are positive inside.
-__device__ __forceinline__ SetupTri setup_tri(const RasterArgs& a, uint32_t t) {
+__device__ __forceinline__ SetupTri setup_from(const RasterArgs& a, f4 c0, f4 c1, f4 c2) {
   SetupTri s;
-  uint32_t idx[3], side[3];
-  tri_vertices(t, a.P, idx, side);
+  s.c[0] = c0; s.c[1] = c1; s.c[2] = c2;
   s.ok = true;
   for (int k = 0; k < 3; k++) {
-    s.c[k] = edge_vertex(a, idx[k], side[k]);
     s.vi[k] = (uint32_t)k;
     const f4 c = s.c[k];
+    s.X[k] = s.Y[k] = 0;
     if (!(c.w > 0.0f)) { s.ok = false; continue; }
     const float nx = c.x / c.w, ny = c.y / c.w;
     const float xs = (float)a.W * (nx * 0.5f + 0.5f);
@@ -129,6 +130,7 @@ __device__ __forceinline__ SetupTri setup_tri(const RasterArgs& a, uint32_t t) {
     s.X[k] = (long long)rintf(xs * 256.0f);
     s.Y[k] = (long long)rintf(ys * 256.0f);
   }
+  s.area = 0;
   if (!s.ok) return s;
   long long area = (s.X[1] - s.X[0]) * (s.Y[2] - s.Y[0]) - (s.Y[1] - s.Y[0]) * (s.X[2] - s.X[0]);
   if (area < 0) {
@@ -141,6 +143,30 @@ __device__ __forceinline__ SetupTri setup_tri(const RasterArgs& a, uint32_t t) {
   s.area = area;
   s.ok = area > 0;
   return s;
+}
+
+__device__ __forceinline__ SetupTri setup_tri(const RasterArgs& a, uint32_t t) {
+  uint32_t idx[3], side[3];
+  tri_vertices(t, a.P, idx, side);
+  return setup_from(a, edge_vertex(a, idx[0], side[0]), edge_vertex(a, idx[1], side[1]),
+                    edge_vertex(a, idx[2], side[2]));
+}
+
+// Both corners of point idx (0: +thickness, 1: -thickness), sharing the
+// point's transform: the same values edge_vertex gives for each side.
+__device__ __forceinline__ void point_corners(const RasterArgs& a, uint32_t idx, f4& plus, f4& minus) {
+  const fh_render_uniforms* __restrict__ u = a.u;
+  const float* mv = a.shadow ? u->shadows.sourceModelViewMat : u->modelViewMat;
+  const float* proj = a.shadow ? u->shadows.sourceProjMatrix : u->projMatrix;
+  const float radius = a.shadow ? u->shadows.sourcePosition[3] : u->fiberRadius;
+  const float4 p = a.pos[idx], t = a.tan[idx];
+  const f4 pv = mul(mv, mk4(p.x, p.y, p.z, 1.0f));
+  const f4 tv = mul(mv, mk4(t.x, t.y, t.z, 1.0f));
+  const f3 right = safe_normalize3(cross(xyz(tv), mk3(0.0f, 0.0f, 1.0f)));
+  const f3 th = right * radius;
+  const f3 ep = xyz(pv) + th, em = xyz(pv) - th;
+  plus = mul(proj, mk4(ep.x, ep.y, ep.z, 1.0f));
+  minus = mul(proj, mk4(em.x, em.y, em.z, 1.0f));
 }
 
 __device__ __forceinline__ long long edge_fn_i(long long ax, long long ay, long long bx, long long by, long long px,
@@ -171,13 +197,11 @@ __device__ __forceinline__ float frag_depth(const SetupTri& s, const float* l) {
 }
 
 __device__ __forceinline__ long long floor_div256(long long v) { return v >= 0 ? v / 256 : -((-v + 255) / 256); }
-__device__ __forceinline__ long long floor_div(long long a, long long b) {  // b > 0
-  return a >= 0 ? a / b : -((-a + b - 1) / b);
-}
 
 // Conservative pixel range [x0, x1] of row centre cy inside edge a -> b
-// (E(cx) = dx (cy - ay) - dy (cx - ax) >= 0), one pixel wider on each side
-// than the exact bound; the caller still tests each pixel exactly.
+// (E(cx) = dx (cy - ay) - dy (cx - ax) >= 0), two pixels wider on each side
+// than the bound evaluated in f32 (whose error is far below a pixel); the
+// caller still tests each pixel exactly.
 __device__ __forceinline__ void edge_span(long long ax, long long ay, long long bx, long long by, long long cy,
                                           long long& x0, long long& x1) {
   const long long dx = bx - ax, dy = by - ay;
@@ -186,66 +210,77 @@ __device__ __forceinline__ void edge_span(long long ax, long long ay, long long 
     if (c < 0) { x0 = 1; x1 = 0; }
     return;
   }
-  if (dy < 0) {  // E grows with cx: cx >= -c / -dy
-    const long long cxMin = floor_div(-c, -dy);
-    x0 = max(x0, floor_div256(cxMin - 128) - 1);
-  } else {       // E falls with cx: cx <= c / dy
-    const long long cxMax = floor_div(c, dy);
-    x1 = min(x1, floor_div256(cxMax - 128) + 1);
-  }
+  const float cxb = (float)c * __builtin_amdgcn_rcpf((float)dy);  // the cx where E = 0
+  const float pxb = fminf(fmaxf((cxb - 128.0f) * (1.0f / 256.0f), -1.0e9f), 1.0e9f);
+  if (dy < 0) x0 = max(x0, (long long)floorf(pxb) - 2);  // E grows with cx
+  else x1 = min(x1, (long long)floorf(pxb) + 2);          // E falls with cx
 }
 
-// The fragments of triangle t on rows y0 + first, y0 + first + stride, ...:
-// per row a conservative x span from the three edge functions, then the
-// exact edge tests, the depth test against the texture and one atomicMin per
-// fragment.
-__device__ __forceinline__ void raster_rows(const RasterArgs& a, const SetupTri& s, uint32_t t, long long first,
-                                            long long stride) {
+// Pixel box of a set-up triangle: centres 256 p + 128 inside its snapped
+// bounds, clipped to the target (empty when x0 > x1 or y0 > y1).
+struct PixBox {
+  long long x0, x1, y0, y1;
+};
+__device__ __forceinline__ PixBox pix_box(const RasterArgs& a, const SetupTri& s) {
   const long long minX = min(s.X[0], min(s.X[1], s.X[2])), maxX = max(s.X[0], max(s.X[1], s.X[2]));
   const long long minY = min(s.Y[0], min(s.Y[1], s.Y[2])), maxY = max(s.Y[0], max(s.Y[1], s.Y[2]));
-  // pixel centres 256 p + 128 inside [min, max]
-  const long long x0 = max(0ll, -floor_div256(128 - minX)), x1 = min((long long)a.W - 1, floor_div256(maxX - 128));
-  const long long y0 = max(0ll, -floor_div256(128 - minY)), y1 = min((long long)a.H - 1, floor_div256(maxY - 128));
+  PixBox b;
+  b.x0 = max(0ll, -floor_div256(128 - minX));
+  b.x1 = min((long long)a.W - 1, floor_div256(maxX - 128));
+  b.y0 = max(0ll, -floor_div256(128 - minY));
+  b.y1 = min((long long)a.H - 1, floor_div256(maxY - 128));
+  return b;
+}
+
+// The fragments of triangle t on row py between pixels xa and xb: a
+// conservative span from the three edge functions, then the exact edge
+// tests stepped along the row, the depth clip, the 'less' test against the
+// texture and one atomicMin per fragment.
+__device__ __forceinline__ void raster_row(const RasterArgs& a, const SetupTri& s, uint32_t t, long long py,
+                                           long long xa, long long xb) {
+  const long long cy = py * 256 + 128;
+  edge_span(s.X[1], s.Y[1], s.X[2], s.Y[2], cy, xa, xb);
+  edge_span(s.X[2], s.Y[2], s.X[0], s.Y[0], cy, xa, xb);
+  edge_span(s.X[0], s.Y[0], s.X[1], s.Y[1], cy, xa, xb);
+  if (xa > xb) return;
   const bool tl0 = top_left(s.X[1], s.Y[1], s.X[2], s.Y[2]);
   const bool tl1 = top_left(s.X[2], s.Y[2], s.X[0], s.Y[0]);
   const bool tl2 = top_left(s.X[0], s.Y[0], s.X[1], s.Y[1]);
-  for (long long py = y0 + first; py <= y1; py += stride) {
-    const long long cy = py * 256 + 128;
-    long long rx0 = x0, rx1 = x1;
-    edge_span(s.X[1], s.Y[1], s.X[2], s.Y[2], cy, rx0, rx1);
-    edge_span(s.X[2], s.Y[2], s.X[0], s.Y[0], cy, rx0, rx1);
-    edge_span(s.X[0], s.Y[0], s.X[1], s.Y[1], cy, rx0, rx1);
-    rx0 = max(rx0, x0);
-    rx1 = min(rx1, x1);
-    if (rx0 > rx1) continue;
-    // edge values at the row's first centre, then one step of -256 dy per pixel
-    const long long cx0 = rx0 * 256 + 128;
-    long long e0 = edge_fn_i(s.X[1], s.Y[1], s.X[2], s.Y[2], cx0, cy);
-    long long e1 = edge_fn_i(s.X[2], s.Y[2], s.X[0], s.Y[0], cx0, cy);
-    long long e2 = edge_fn_i(s.X[0], s.Y[0], s.X[1], s.Y[1], cx0, cy);
-    const long long d0 = -256 * (s.Y[2] - s.Y[1]), d1 = -256 * (s.Y[0] - s.Y[2]), d2 = -256 * (s.Y[1] - s.Y[0]);
-    for (long long px = rx0; px <= rx1; px++, e0 += d0, e1 += d1, e2 += d2) {
-      const long long cx = px * 256 + 128;
-      if (!(covers(e0, tl0) && covers(e1, tl1) && covers(e2, tl2))) continue;
-      float l[3];
-      barycentrics(s, cx, cy, l);
-      float z = frag_depth(s, l);
-      if (!(z >= 0.0f && z <= 1.0f)) continue;  // depth clip
-      z = z + 0.0f;                              // -0 -> +0
-      const size_t p = (size_t)py * a.W + (size_t)px;
-      if (!(z < a.depth[p])) continue;           // 'less' against the meshes
-      const unsigned long long k = ((unsigned long long)__float_as_uint(z) << 32) | t;
-      atomicMin(&a.key[p], k);
-    }
+  // edge values at the first centre, then one step of -256 dy per pixel
+  const long long cx0 = xa * 256 + 128;
+  long long e0 = edge_fn_i(s.X[1], s.Y[1], s.X[2], s.Y[2], cx0, cy);
+  long long e1 = edge_fn_i(s.X[2], s.Y[2], s.X[0], s.Y[0], cx0, cy);
+  long long e2 = edge_fn_i(s.X[0], s.Y[0], s.X[1], s.Y[1], cx0, cy);
+  const long long d0 = -256 * (s.Y[2] - s.Y[1]), d1 = -256 * (s.Y[0] - s.Y[2]), d2 = -256 * (s.Y[1] - s.Y[0]);
+  for (long long px = xa; px <= xb; px++, e0 += d0, e1 += d1, e2 += d2) {
+    if (!(covers(e0, tl0) && covers(e1, tl1) && covers(e2, tl2))) continue;
+    float l[3];
+    barycentrics(s, px * 256 + 128, cy, l);
+    float z = frag_depth(s, l);
+    if (!(z >= 0.0f && z <= 1.0f)) continue;  // depth clip
+    z = z + 0.0f;                              // -0 -> +0
+    const size_t p = (size_t)py * a.W + (size_t)px;
+    if (!(z < a.depth[p])) continue;           // 'less' against the meshes
+    atomicMin(&a.key[p], ((unsigned long long)__float_as_uint(z) << 32) | t);
   }
 }
 
+__device__ __forceinline__ void raster_rows(const RasterArgs& a, const SetupTri& s, uint32_t t, long long first,
+                                            long long stride) {
+  const PixBox b = pix_box(a, s);
+  for (long long py = b.y0 + first; py <= b.y1; py += stride) raster_row(a, s, t, py, b.x0, b.x1);
+}
+
 // kRowSplit threads per triangle, thread k taking rows k, k + kRowSplit, ...
-// (a tall triangle does not hold one lane while the rest of its wave idles);
-// a triangle whose pixel box is larger than kWaveRaster pixels (a strand
-// passing close to the eye) is drawn by its whole wave, one row per lane.
+// (a tall triangle does not hold one lane while the rest of its wave idles).
+// A triangle whose pixel box is larger than kWaveRaster pixels (a strand
+// passing close to the eye) is queued in bands of kBandRows rows, each drawn
+// by a whole workgroup of HairRasterBig; when the queue is full, by the
+// triangle's own wave, one row per lane.
 constexpr uint32_t kRowSplit = 8;
 constexpr long long kWaveRaster = 1024;
+constexpr uint32_t kBandRows = 64;
+constexpr uint32_t kBigCap = 1u << 20;  // queued bands
 
 struct HairRaster {
   using Args = RasterArgs;
@@ -253,25 +288,71 @@ struct HairRaster {
   static constexpr const char* kName = "hair_raster";
   static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) {
     const uint32_t gid = xcd_order(block, blocks) * kThreads + threadIdx.x;
-    const uint32_t t = gid / kRowSplit, k = gid % kRowSplit;
-    SetupTri s;
-    s.ok = false;
-    if (t < a.nTri) s = setup_tri(a, t);
-    bool big = false;
-    if (s.ok) {
-      const long long bw = (max(s.X[0], max(s.X[1], s.X[2])) - min(s.X[0], min(s.X[1], s.X[2]))) / 256 + 1;
-      const long long bh = (max(s.Y[0], max(s.Y[1], s.Y[2])) - min(s.Y[0], min(s.Y[1], s.Y[2]))) / 256 + 1;
-      if (bw * bh > kWaveRaster) big = k == 0u;
-      else raster_rows(a, s, t, (long long)k, (long long)kRowSplit);
+    const uint32_t seg = gid / kRowSplit, k = gid % kRowSplit;
+    // the segment's two triangles (hairIndexBuffer.ts:39-46): (A+, A-, B+), (B+, A-, B-)
+    uint32_t bigMask = 0;
+    if (2u * seg < a.nTri) {
+      const uint32_t strand = seg / (a.P - 1u), j = seg - strand * (a.P - 1u);
+      const uint32_t ia = strand * a.P + j;
+      f4 ap, am, bp, bm;
+      point_corners(a, ia, ap, am);
+      point_corners(a, ia + 1u, bp, bm);
+      for (uint32_t h = 0; h < 2u; h++) {
+        const uint32_t t = 2u * seg + h;
+        const SetupTri s = h == 0u ? setup_from(a, ap, am, bp) : setup_from(a, bp, am, bm);
+        if (!s.ok) continue;
+        const PixBox b = pix_box(a, s);
+        const long long bw = b.x1 - b.x0 + 1, bh = b.y1 - b.y0 + 1;
+        if (bw <= 0 || bh <= 0) continue;
+        if (bw * bh <= kWaveRaster) {
+          raster_rows(a, s, t, (long long)k, (long long)kRowSplit);
+        } else if (k == 0u) {
+          const uint32_t bands = (uint32_t)((bh + kBandRows - 1) / kBandRows);
+          const uint32_t first = atomicAdd(a.bigCount, bands);
+          if (first + bands <= kBigCap) {
+            for (uint32_t q = 0; q < bands; q++) a.big[first + q] = make_uint2(t, q);
+          } else {
+            bigMask |= 1u << h;
+          }
+        }
+      }
     }
-    uint64_t todo = __ballot(big);
+    // queue full: the wave draws the triangle, one row per lane
     const uint32_t lane = __lane_id();
-    while (todo) {
-      const uint32_t src = (uint32_t)__builtin_ctzll(todo);
-      todo &= todo - 1ull;
-      const uint32_t tb = (uint32_t)__builtin_amdgcn_readlane((int)t, (int)src);
-      const SetupTri sb = setup_tri(a, tb);
-      raster_rows(a, sb, tb, (long long)lane, 64);
+    for (uint32_t h = 0; h < 2u; h++) {
+      uint64_t todo = __ballot((bigMask >> h) & 1u);
+      while (todo) {
+        const uint32_t src = (uint32_t)__builtin_ctzll(todo);
+        todo &= todo - 1ull;
+        const uint32_t tb = 2u * (uint32_t)__builtin_amdgcn_readlane((int)seg, (int)src) + h;
+        const SetupTri sb = setup_tri(a, tb);
+        raster_rows(a, sb, tb, (long long)lane, 64);
+      }
+    }
+  }
+};
+
+// The queued bands: a workgroup per band, its threads taking the band's
+// (row, 64-pixel chunk) items in turn; every workgroup exits after the last
+// band.
+struct HairRasterBig {
+  using Args = RasterArgs;
+  static constexpr uint32_t kThreads = 256;
+  static constexpr const char* kName = "hair_raster_big";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) {
+    const uint32_t count = min(*a.bigCount, kBigCap);
+    for (uint32_t i = block; i < count; i += blocks) {
+      const uint2 e = a.big[i];
+      const SetupTri s = setup_tri(a, e.x);
+      const PixBox b = pix_box(a, s);
+      const long long y0 = b.y0 + (long long)e.y * kBandRows, y1 = min(b.y1, y0 + kBandRows - 1);
+      const long long cpr = (b.x1 - b.x0) / 64 + 1;
+      const long long items = (y1 - y0 + 1) * cpr;
+      for (long long it = threadIdx.x; it < items; it += kThreads) {
+        const long long row = it / cpr, chunk = it - row * cpr;
+        const long long xa = b.x0 + chunk * 64;
+        raster_row(a, s, e.x, y0 + row, xa, min(b.x1, xa + 63));
+      }
     }
   }
 };
@@ -433,12 +514,17 @@ struct Gtao {
 
 int raster_hair(fh_ctx* ctx, RasterArgs& a) {
   const uint64_t px = (uint64_t)a.W * a.H;
-  int rc = ensure_scratch(ctx, px * 8);
+  const size_t bigOff = align_up(px * 8, 256);
+  int rc = ensure_scratch(ctx, bigOff + 256 + (size_t)kBigCap * 8);
   if (rc) return rc;
   a.key = static_cast<unsigned long long*>(ctx->scratch);
+  a.bigCount = reinterpret_cast<uint32_t*>(static_cast<char*>(ctx->scratch) + bigOff);
+  a.big = reinterpret_cast<uint2*>(static_cast<char*>(ctx->scratch) + bigOff + 256);
   if ((rc = fill_u32(ctx, a.key, px * 2, 0xffffffffu))) return rc;
+  if ((rc = fill_u32(ctx, a.bigCount, 1, 0u))) return rc;
   FH_REQUIRE((uint64_t)a.nTri * kRowSplit < (1ull << 32), "hair raster: too many triangles");
-  if ((rc = launch<HairRaster>(ctx, div_ceil(a.nTri * kRowSplit, HairRaster::kThreads), 0, a))) return rc;
+  if ((rc = launch<HairRaster>(ctx, div_ceil(a.nTri / 2u * kRowSplit, HairRaster::kThreads), 0, a))) return rc;
+  if ((rc = launch<HairRasterBig>(ctx, persistent_blocks(ctx), 0, a))) return rc;
   return launch<HairResolve>(ctx, div_ceil((uint32_t)px, HairResolve::kThreads), 0, a);
 }
 
