@@ -305,3 +305,71 @@ def pack_hair_data(center, radius: float, strands: int, points: int) -> np.ndarr
     b[4] = strands
     b[5] = points
     return b
+
+
+def render_params(cfg: RenderConfig, sim: SimConfig, camera: Camera, width: int, height: int, model: np.ndarray,
+                  hair_sphere, mesh_spheres: list, sdf_bounds, grid_bounds):
+    """The fh_render_params of the same inputs as pack_render_uniforms (for
+    the library's own packer). Keeps the mesh-sphere array alive on the
+    returned struct."""
+    from . import native as N
+    p = N.RenderParams()
+    p.width, p.height = width, height
+    p.cameraEye[:] = list(camera.eye)
+    p.cameraTarget[:] = list(camera.target)
+    p.cameraUp[:] = list(camera.up)
+    p.fovDeg, p.near, p.far = cfg.camera_fov_deg, cfg.camera_near, cfg.camera_far
+    p.modelMatrix[:] = list(np.asarray(model, dtype=np.float64).T.reshape(16))
+    p.hairSphere[:] = [*hair_sphere[0], hair_sphere[1]]
+    spheres = (N.d * (4 * max(1, len(mesh_spheres))))(*[x for c, r in mesh_spheres for x in (*c, r)])
+    p.meshSpheres = spheres
+    p.meshSphereCount = len(mesh_spheres)
+    p._keep = spheres
+    p.tileSize, p.avgSegmentsPerTile = cfg.tileSize, cfg.avgSegmentsPerTile
+    p.sdfBoundsMin[:], p.sdfBoundsMax[:] = list(sdf_bounds[0]), list(sdf_bounds[1])
+    p.sdfDistanceOffset = sim.sdfDistanceOffset
+    p.gridBoundsMin[:], p.gridBoundsMax[:] = list(grid_bounds[0]), list(grid_bounds[1])
+    p.collisionSphere[:] = list(sim.collisionSphere)
+    p.fiberRadius = cfg.fiberRadius
+    p.displayMode, p.dbgTileModeMaxSegments = cfg.displayMode, cfg.dbgTileModeMaxSegments
+    p.dbgSlicesModeMaxSlices, p.dbgShowTiles = cfg.dbgSlicesModeMaxSlices, 1 if cfg.dbgShowTiles else 0
+    p.colors[:] = list(cfg.colors)
+    p.lightAmbient[:] = list(cfg.lightAmbient)
+    for i, l in enumerate(cfg.lights):
+        p.lights[i].posPhi, p.lights[i].posTheta = l["posPhi"], l["posTheta"]
+        p.lights[i].color[:] = list(l["color"])
+        p.lights[i].energy = l["energy"]
+    s = cfg.shadows
+    p.shadowUsePCSS, p.shadowPCFRadius = 1 if s["usePCSS"] else 0, s["PCF_Radius"]
+    p.shadowBias, p.shadowStrength = s["bias"], s["strength"]
+    p.shadowPosPhi, p.shadowPosTheta, p.shadowDistance = s["posPhi"], s["posTheta"], s["distance"]
+    p.shadowTarget[:] = list(s["target"])
+    a = cfg.ao
+    p.aoRadius, p.aoDirectionOffset = a["radius"], a["directionOffset"]
+    p.aoFalloffStart2, p.aoFalloffEnd2 = a["falloffStart2"], a["falloffEnd2"]
+    p.aoNumDirections, p.aoNumSteps, p.aoStrength = a["numDirections"], a["numSteps"], a["strength"]
+    m = cfg.material
+    p.hairColor0[:], p.hairColor1[:] = list(m["color0"]), list(m["color1"])
+    for k in ("specular", "shadows", "weightTT", "weightTRT", "shift", "roughness", "colorRng", "lumaRng",
+              "attenuation"):
+        setattr(p, k, m[k])
+    return p
+
+
+def sim_params(cfg: SimConfig, model: np.ndarray, sdf_bounds, grid_bounds, frame_idx: int):
+    """The fh_sim_params of the same inputs as pack_sim_uniforms."""
+    from . import native as N
+    p = N.SimParams()
+    p.modelMatrix[:] = list(np.asarray(model, dtype=np.float64).T.reshape(16))
+    p.collisionSphere[:] = list(cfg.collisionSphere)
+    p.windDirPhi, p.windDirTheta, p.windStrength = cfg.windDirPhi, cfg.windDirTheta, cfg.windStrength
+    p.sdfBoundsMin[:], p.sdfBoundsMax[:] = list(sdf_bounds[0]), list(sdf_bounds[1])
+    p.sdfDistanceOffset = cfg.sdfDistanceOffset
+    p.gridBoundsMin[:], p.gridBoundsMax[:] = list(grid_bounds[0]), list(grid_bounds[1])
+    p.frameIdx, p.constraintIterations = frame_idx, cfg.constraintIterations
+    for k in ("deltaTime", "gravity", "windStrengthLull", "windColisionTraceOffset", "stiffnessLengthConstr",
+              "stiffnessCollisions", "stiffnessSDF", "volumePreservation", "friction", "stiffnessGlobalConstr",
+              "stiffnessLocalConstr", "windPhaseOffset", "windStrengthFrequency", "windStrengthJitter"):
+        setattr(p, k, getattr(cfg, k))
+    p.globalConstrExtent, p.globalConstrFade = cfg.globalExtent, cfg.globalFade
+    return p

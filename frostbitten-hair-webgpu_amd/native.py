@@ -39,6 +39,47 @@ class Tex2D(C.Structure):
         return vp(self.data)
 
 
+d = C.c_double
+
+
+class LightParams(C.Structure):
+    """struct fh_light_params"""
+    _fields_ = [("posPhi", d), ("posTheta", d), ("color", d * 3), ("energy", d)]
+
+
+class RenderParams(C.Structure):
+    """struct fh_render_params"""
+    _fields_ = [
+        ("width", u32), ("height", u32), ("cameraEye", d * 3), ("cameraTarget", d * 3), ("cameraUp", d * 3),
+        ("fovDeg", d), ("near", d), ("far", d), ("modelMatrix", d * 16), ("hairSphere", d * 4),
+        ("meshSpheres", C.POINTER(d)), ("meshSphereCount", u32), ("tileSize", u32), ("avgSegmentsPerTile", d),
+        ("sdfBoundsMin", d * 3), ("sdfBoundsMax", d * 3), ("sdfDistanceOffset", d), ("gridBoundsMin", d * 3),
+        ("gridBoundsMax", d * 3), ("collisionSphere", d * 4), ("fiberRadius", d), ("displayMode", u32),
+        ("dbgTileModeMaxSegments", u32), ("dbgSlicesModeMaxSlices", u32), ("dbgShowTiles", u32), ("colors", d * 3),
+        ("lightAmbient", d * 4), ("lights", LightParams * 3), ("shadowUsePCSS", u32), ("shadowPCFRadius", u32),
+        ("shadowBias", d), ("shadowStrength", d), ("shadowPosPhi", d), ("shadowPosTheta", d),
+        ("shadowDistance", d), ("shadowTarget", d * 3), ("aoRadius", d), ("aoDirectionOffset", d),
+        ("aoFalloffStart2", d), ("aoFalloffEnd2", d), ("aoNumDirections", u32), ("aoNumSteps", u32),
+        ("aoStrength", d), ("hairColor0", d * 3), ("hairColor1", d * 3), ("specular", d), ("shadows", d),
+        ("weightTT", d), ("weightTRT", d), ("shift", d), ("roughness", d), ("colorRng", d), ("lumaRng", d),
+        ("attenuation", d),
+    ]
+
+
+class SimParams(C.Structure):
+    """struct fh_sim_params"""
+    _fields_ = [
+        ("modelMatrix", d * 16), ("collisionSphere", d * 4), ("windDirPhi", d), ("windDirTheta", d),
+        ("windStrength", d), ("sdfBoundsMin", d * 3), ("sdfBoundsMax", d * 3), ("sdfDistanceOffset", d),
+        ("gridBoundsMin", d * 3), ("gridBoundsMax", d * 3), ("frameIdx", u32), ("constraintIterations", u32),
+        ("deltaTime", d), ("gravity", d), ("windStrengthLull", d), ("windColisionTraceOffset", d),
+        ("stiffnessLengthConstr", d), ("stiffnessCollisions", d), ("stiffnessSDF", d), ("volumePreservation", d),
+        ("friction", d), ("stiffnessGlobalConstr", d), ("globalConstrExtent", d), ("globalConstrFade", d),
+        ("stiffnessLocalConstr", d), ("windPhaseOffset", d), ("windStrengthFrequency", d),
+        ("windStrengthJitter", d),
+    ]
+
+
 class KernelTime(C.Structure):
     """struct fh_kernel_time"""
     _fields_ = [("name", C.c_char * 48), ("launches", u32), ("reserved", u32), ("total_ms", C.c_double)]
@@ -81,6 +122,8 @@ SIGNATURES = {
     "fh_hw_hair": (i32, [vp, u32, u32, vp, vp, vp, vp, vp, u32, u32]),
     "fh_shadow_map_hair": (i32, [vp, vp, vp, vp, vp, u32, u32, u32]),
     "fh_calc_ao": (i32, [vp, vp, Tex2D, vp, vp, u32, u32]),
+    "fh_pack_render_uniforms": (i32, [C.POINTER(RenderParams), vp]),
+    "fh_pack_sim_uniforms": (i32, [C.POINTER(SimParams), vp]),
     "fh_profiler_enable": (i32, [vp, u32]),
     "fh_capture_begin": (i32, [vp]),
     "fh_capture_end": (i32, [vp, C.POINTER(vp)]),
@@ -128,6 +171,22 @@ def check(rc: int, what: str) -> None:
     if rc != 0:
         msg = _lib.fh_last_error().decode(errors="replace") if _lib else "?"
         raise FhError(f"{what} failed with code {rc}: {msg}", rc)
+
+
+def pack_render_uniforms(params: RenderParams) -> bytes:
+    """fh_pack_render_uniforms: the 1184 bytes of struct Uniforms (host only)."""
+    lib = load()
+    out = C.create_string_buffer(1184)
+    check(lib.fh_pack_render_uniforms(C.byref(params), C.cast(out, vp)), "fh_pack_render_uniforms")
+    return out.raw
+
+
+def pack_sim_uniforms(params: SimParams) -> bytes:
+    """fh_pack_sim_uniforms: the 304 bytes of struct SimulationUniforms (host only)."""
+    lib = load()
+    out = C.create_string_buffer(304)
+    check(lib.fh_pack_sim_uniforms(C.byref(params), C.cast(out, vp)), "fh_pack_sim_uniforms")
+    return out.raw
 
 
 def ptr(t) -> vp:

@@ -472,6 +472,73 @@ int fh_calc_ao(fh_ctx* ctx, const fh_render_uniforms* uniforms, fh_tex2d depth, 
 int fh_copy(fh_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 
 /* ------------------------------------------------------------------------ */
+/* Uniform packing (host side)                                               */
+/* ------------------------------------------------------------------------ */
+
+/* The inputs of RenderUniformsBuffer.update (renderUniformsBuffer.ts
+ * :205-288): the camera (camera.ts, CONFIG.camera), the model matrix, the
+ * bounding spheres the shadow source frames (getMVP_ShadowSourceMatrix.ts
+ * :14-79) and the CONFIG values it copies. Reals are double: the matrices
+ * are built in double and stored as f32, as wgpu-matrix stores into
+ * Float32Array. */
+typedef struct fh_light_params {
+  double posPhi, posTheta;     /* degrees; position = 2 * sphericalToCartesian */
+  double color[3];
+  double energy;
+} fh_light_params;
+
+typedef struct fh_render_params {
+  uint32_t width, height;
+  double cameraEye[3], cameraTarget[3], cameraUp[3];
+  double fovDeg, near, far;
+  double modelMatrix[16];      /* column-major */
+  double hairSphere[4];        /* centre xyz, radius */
+  const double* meshSpheres;   /* meshSphereCount x (centre xyz, radius) */
+  uint32_t meshSphereCount;
+  uint32_t tileSize;
+  double avgSegmentsPerTile;
+  double sdfBoundsMin[3], sdfBoundsMax[3], sdfDistanceOffset;
+  double gridBoundsMin[3], gridBoundsMax[3];
+  double collisionSphere[4];   /* CONFIG.hairSimulation.collisionSphere: xyz, radius */
+  double fiberRadius;
+  uint32_t displayMode, dbgTileModeMaxSegments, dbgSlicesModeMaxSlices, dbgShowTiles;
+  double colors[3];            /* gamma, exposure, ditherStrength */
+  double lightAmbient[4];
+  fh_light_params lights[3];
+  uint32_t shadowUsePCSS, shadowPCFRadius;
+  double shadowBias, shadowStrength;
+  double shadowPosPhi, shadowPosTheta, shadowDistance, shadowTarget[3];
+  double aoRadius, aoDirectionOffset, aoFalloffStart2, aoFalloffEnd2;
+  uint32_t aoNumDirections, aoNumSteps;
+  double aoStrength;
+  double hairColor0[3], hairColor1[3];
+  double specular, shadows, weightTT, weightTRT, shift, roughness, colorRng, lumaRng, attenuation;
+} fh_render_params;
+
+/* RenderUniformsBuffer.update (renderUniformsBuffer.ts:205-288) with
+ * encodeDebugMode (:396-414): host-only, writes *out (host memory). */
+int fh_pack_render_uniforms(const fh_render_params* params, fh_render_uniforms* out);
+
+/* The inputs of SimulationUniformsBuffer.update
+ * (simulationUniformsBuffer.ts:86-152). */
+typedef struct fh_sim_params {
+  double modelMatrix[16];      /* column-major */
+  double collisionSphere[4];   /* world xyz, radius */
+  double windDirPhi, windDirTheta, windStrength;
+  double sdfBoundsMin[3], sdfBoundsMax[3], sdfDistanceOffset;
+  double gridBoundsMin[3], gridBoundsMax[3];
+  uint32_t frameIdx, constraintIterations;
+  double deltaTime, gravity, windStrengthLull, windColisionTraceOffset;
+  double stiffnessLengthConstr, stiffnessCollisions, stiffnessSDF, volumePreservation, friction;
+  double stiffnessGlobalConstr, globalConstrExtent, globalConstrFade, stiffnessLocalConstr;
+  double windPhaseOffset, windStrengthFrequency, windStrengthJitter;
+} fh_sim_params;
+
+/* SimulationUniformsBuffer.update (simulationUniformsBuffer.ts:86-152):
+ * host-only, writes *out (host memory). */
+int fh_pack_sim_uniforms(const fh_sim_params* params, fh_sim_uniforms* out);
+
+/* ------------------------------------------------------------------------ */
 /* Kernel timing                                                             */
 /* ------------------------------------------------------------------------ */
 
