@@ -220,6 +220,15 @@ int fh_create(int device, fh_ctx** out) {
     return fh::fail(FH_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
   c->owns_stream = true;
+  e = hipMalloc(reinterpret_cast<void**>(&c->srgb_thresholds), 256 * sizeof(float));
+  if (e == hipSuccess)
+    e = hipMemcpy(c->srgb_thresholds, fh::srgb_thresholds(), 256 * sizeof(float), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (c->srgb_thresholds) hipFree(c->srgb_thresholds);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return fh::fail(FH_ERR_HIP, std::string("fh_create: sRGB table: ") + hipGetErrorString(e));
+  }
   *out = c;
   return FH_OK;
 }
@@ -230,6 +239,7 @@ int fh_destroy(fh_ctx* ctx) {
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   for (hipEvent_t e : ctx->prof_events) hipEventDestroy(e);
   if (ctx->scratch) hipFree(ctx->scratch);
+  if (ctx->srgb_thresholds) hipFree(ctx->srgb_thresholds);
   if (ctx->owns_stream && ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
   return FH_OK;

@@ -35,6 +35,9 @@ struct fh_ctx {
   // per-bin offsets, prefix-scan temporaries).
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
+  // The 256 thresholds of the rgba8unorm-srgb store (fh_present), uploaded
+  // at creation so that no call allocates or copies them later.
+  float* srgb_thresholds = nullptr;
   // Kernel timing (fh_profiler_enable): an event pair per timed launch.
   std::vector<hipEvent_t> prof_events;
   std::vector<const char*> prof_names;  // kernel of each timed launch
@@ -84,6 +87,8 @@ int ensure_scratch(fh_ctx* ctx, size_t bytes);
 // the context's stream and returns its index (-1 when timing is off or the
 // event pool is used up); prof_end records the matching end event.
 int prof_begin(fh_ctx* ctx, const char* kernel);
+// The host copy of the sRGB store thresholds (fh_post.hip).
+const float* srgb_thresholds();
 // Zeroes up to kMaxFillJobs buffers (n_words u32 each) in one launch.
 constexpr int kMaxFillJobs = 4;
 int zero_many(fh_ctx* ctx, int count, void* const* dst, const uint64_t* n_words,

@@ -100,6 +100,9 @@ int record_launch(fh_ctx* ctx, const char* name, BatchLauncher fn, uint32_t bloc
 template <class K>
 int launch(fh_ctx* ctx, uint32_t blocks, uint32_t shmem, const typename K::Args& a, const char* name = K::kName) {
   static_assert(std::is_trivially_copyable<typename K::Args>::value, "kernel arguments are copied as bytes");
+  // a batched launch copies each instance's Args into registers: large
+  // arrays belong in device memory, with a pointer here
+  static_assert(sizeof(typename K::Args) <= 256, "kernel argument struct too large for a batched launch");
   if (blocks == 0) return FH_OK;
   if (ctx->recording) return record_launch(ctx, name, &batch_launch<K>, blocks, shmem, &a, sizeof(a));
   static uint32_t allowed = 0;

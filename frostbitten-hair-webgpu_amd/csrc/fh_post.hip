@@ -132,8 +132,10 @@ struct PresentArgs {
   uint8_t* out;                // rgba8unorm-srgb rows of outPitch bytes
   uint32_t W, H, outPitch, ditherPerPixel;
   // srgbT[k] (k = 1..255): the least f32 linear value that an
-  // rgba8unorm-srgb store rounds to code k or above (srgbT[0] = 0)
-  float srgbT[256];
+  // rgba8unorm-srgb store rounds to code k or above (srgbT[0] = 0); device
+  // memory of the context (an array in the arguments would be copied per
+  // thread by a batched launch)
+  const float* srgbT;
 };
 
 // doACES_Tonemapping (aces.wgsl.ts:5-13)
@@ -238,8 +240,12 @@ struct CopyWords {
   }
 };
 
+}  // namespace
+
+namespace fh {
+
 // Code of a linear value under the exact encoding (evaluated in double).
-uint32_t srgb_code_exact(float c) {
+static uint32_t srgb_code_exact(float c) {
   double v = c > 0.0f ? (c < 1.0f ? (double)c : 1.0) : 0.0;
   double s = v <= 0.0031308 ? 12.92 * v : 1.055 * std::pow(v, 1.0 / 2.4) - 0.055;
   return (uint32_t)std::floor(s * 255.0 + 0.5);
@@ -263,7 +269,7 @@ const float* srgb_thresholds() {
   return table.data();
 }
 
-}  // namespace
+}  // namespace fh
 
 extern "C" {
 
@@ -324,7 +330,7 @@ int fh_present(fh_ctx* ctx, uint32_t vp_w, uint32_t vp_h, const fh_render_unifor
   a.normals = static_cast<const uint32_t*>(normals_texture);
   a.out = static_cast<uint8_t*>(screen_texture);
   a.W = vp_w; a.H = vp_h; a.outPitch = pitch; a.ditherPerPixel = dither_per_pixel ? 1u : 0u;
-  std::memcpy(a.srgbT, srgb_thresholds(), sizeof(a.srgbT));
+  a.srgbT = ctx->srgb_thresholds;
   return fh::launch<Present>(ctx, fh::div_ceil(vp_w * vp_h, Present::kThreads), 0, a);
 }
 

@@ -231,7 +231,6 @@ struct TileWaveLds {
   int px0[64], py0[64], px1[64];
   uint32_t rowStart[64];
   uint32_t accMin[64], accMax[64], accBin[64];
-  float bndMin[64], bndMax[64];  // mesh-depth bounds of the pair's tile
   // per pixel row of the current row window
   uint32_t itemSlot[64];
   int itemX[64], itemY[64];
@@ -262,6 +261,7 @@ __device__ __forceinline__ void seg_step(uint32_t slot, uint32_t& vmin, uint32_t
 __device__ __forceinline__ void tiles_shade_queue(const TilesArgs& a, TileWaveLds& L, uint32_t head,
                                                   uint32_t n, const TileGeom& tg, f2 dbVS) {
   const uint32_t lane = __lane_id();
+  const uint32_t tshift = (uint32_t)__builtin_ctz(a.T);
   uint32_t slot = 0xffffffffu, vmin = 0xffffffffu, vmax = 0u, vbin = 0xffffffffu;
   if (lane < n) {
     const uint32_t e = L.queue[(head + lane) & (kTileQueue - 1u)];
@@ -272,11 +272,13 @@ __device__ __forceinline__ void tiles_shade_queue(const TilesArgs& a, TileWaveLd
     const f2 p = mk2((float)x, (float)y);
     const f2 w = interp_quad_pre<64>(&L.ip[0][o], mk2(L.v[0][o], L.v[1][o]), p);
     const float hdp = interp_f32(w, mk4(L.d[0][o], L.d[1][o], L.d[2][o], L.d[3][o]));
-    // depth test with GL_LESS (:193-195), decided by the pair's tile's depth
-    // bounds when they settle it, by the texel otherwise
+    // depth test with GL_LESS (:193-195), decided by the tile's depth bounds
+    // when they settle it, by the texel otherwise
+    const uint32_t tile = (y >> tshift) * tg.tilesX + (x >> tshift);
+    const float2 bd = tile < a.nBounds ? a.depthBounds[tile] : make_float2(-INFINITY, INFINITY);
     bool pass;
-    if (hdp <= L.bndMin[slot]) pass = true;
-    else if (hdp > L.bndMax[slot]) pass = false;
+    if (hdp <= bd.x) pass = true;
+    else if (hdp > bd.y) pass = false;
     else pass = !(hdp > tex_load(a.depth, a.depthW, a.depthH, (int32_t)x, (int32_t)(tg.vpH - y)));
     if (pass) {
       const f3 vs = project(a.u->projMatrixInv, mk4(p.x, p.y, hdp, 1.0f));
@@ -407,10 +409,6 @@ __device__ __forceinline__ void tiles_wave(const TilesArgs& a, TileWaveLds& L, u
       }
       L.owner[lane] = o;
       L.px0[lane] = xs; L.py0[lane] = ys; L.px1[lane] = xe;
-      const uint32_t tile = ty * tg.tilesX + tx;
-      const float2 bd = tile < a.nBounds ? a.depthBounds[tile] : make_float2(-INFINITY, INFINITY);
-      L.bndMin[lane] = bd.x;
-      L.bndMax[lane] = bd.y;
     }
     L.accMin[lane] = f32_ord(999.0f);
     L.accMax[lane] = f32_ord(-999.0f);
