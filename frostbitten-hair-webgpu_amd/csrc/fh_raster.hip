@@ -970,7 +970,9 @@ struct BinSortLarge {
 // pixel rows [w * TILE / NW, (w + 1) * TILE / NW) of the tile. Per depth bin,
 // the bin's segments arrive sorted by blend key (largest first) in batches:
 //  * one thread per segment projects it into LDS (field-major records: quad,
-//    depths, span frame, pixel box relative to the tile, nearest depth);
+//    depths, span frame, pixel box relative to the tile, nearest depth); a
+//    tile with at most one batch of segments in all its bins is projected
+//    once, before its first bin;
 //  * each wave enumerates the (segment, pixel) candidates of its rows,
 //    record by record (box rows x row spans), 64 per step, runs the edge
 //    tests and a conservative depth pre-test, and queues the covered pixels
@@ -1124,13 +1126,14 @@ __device__ __forceinline__ void fine_shade_queue(const FineArgs& a, const FineLd
   }
 }
 
-// Cheap tests of one batch for wave `w`. Candidates are enumerated record by
-// record (records are in blend order), row by row within the record's box and
-// its row span, 64 per step; covered pixels are queued in that order.
+// Cheap tests of the records [rec0, rec0 + nb) for wave `w`. Candidates are
+// enumerated record by record (records are in blend order), row by row within
+// the record's box and its row span, 64 per step; covered pixels are queued in
+// that order.
 template <int TILE>
 __device__ __forceinline__ void fine_cheap_batch(const FineArgs& a, const FineLds<TILE>& L, uint4* gpool,
-                                                 uint32_t nb, uint32_t bx0, uint32_t by0, f2 td,
-                                                 float invSegs, uint32_t* sliceOff) {
+                                                 uint32_t rec0, uint32_t nb, uint32_t bx0, uint32_t by0,
+                                                 f2 td, float invSegs, uint32_t* sliceOff) {
   constexpr uint32_t PXW = FineLds<TILE>::PXW;
   constexpr int RW = (int)(PXW / TILE);
   const uint32_t lane = __lane_id();
@@ -1145,9 +1148,9 @@ __device__ __forceinline__ void fine_cheap_batch(const FineArgs& a, const FineLd
   const auto& R = L.recs;
   uint32_t qh = 0, qt = 0;
   for (uint32_t rb = 0; rb < nb; rb += 64u) {
-    const uint32_t jr = rb + lane;
+    const uint32_t jr = rec0 + rb + lane;
     uint32_t nrows = 0;
-    if (jr < nb) {
+    if (rb + lane < nb) {
       const int ya = max(R.hi16(kRecBox0, jr), wy0), yb = min(R.hi16(kRecBox1, jr), wy1);
       const int xa = max(R.lo16(kRecBox0, jr), 0), xb = min(R.lo16(kRecBox1, jr), TILE - 1);
       if (yb >= ya && xb >= xa) nrows = (uint32_t)(yb - ya + 1);
@@ -1161,7 +1164,7 @@ __device__ __forceinline__ void fine_cheap_batch(const FineArgs& a, const FineLd
       uint32_t cnt = 0;
       const uint32_t k = lbs_owner(mark, re, nrows, r0);
       if (ridx < nRows) {
-        const uint32_t j = rb + k;
+        const uint32_t j = rec0 + rb + k;
         const int y = max(R.hi16(kRecBox0, j), wy0) + (int)(ridx - rowStart[k]);
         int xa = max(R.lo16(kRecBox0, j), 0), xb = min(R.lo16(kRecBox1, j), TILE - 1) + 1;
         if (R.at(kRecSpan + kSpOn, j) != 0.0f) {
@@ -1217,6 +1220,43 @@ __device__ __forceinline__ void fine_cheap_batch(const FineArgs& a, const FineLd
   if (qt > qh) fine_shade_queue<TILE>(a, L, gpool, w, qh, qt - qh, bx0, by0, td, invSegs, sliceOff);
 }
 
+// Projects segment `key` into record `j`: quad, depths, span frame, pixel box
+// relative to the tile, nearest depth and the per-segment interpolation terms.
+template <int TILE>
+__device__ __forceinline__ void fine_project(const FineArgs& a, const FineLds<TILE>& L, uint32_t j, uint32_t key,
+                                             f2 vpf, uint32_t bx0, uint32_t by0) {
+  constexpr uint32_t RB = fine_batch<TILE>();
+  const fh_render_uniforms* __restrict__ u = a.u;
+  const Seg sg = project_segment(u, a.pos, a.tan, a.P, key >> 8, key & 0xffu, vpf, u->fiberRadius);
+  int x0, y0, x1, y1;
+  seg_pixel_bbox(sg, x0, y0, x1, y1);
+  const auto& R = L.recs;
+  const float vv[8] = {sg.v00.x, sg.v00.y, sg.v01.x, sg.v01.y, sg.v10.x, sg.v10.y, sg.v11.x, sg.v11.y};
+  float amax = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 8; i++) { R.at(kRecV + i, j) = vv[i]; amax = fmaxf(amax, fabsf(vv[i])); }
+  // collinear quads test the whole box; far-off vertices skip the span
+  span_frame(sg, x0 > -(1 << 20) && amax < 65536.0f, &R.at(kRecSpan, j), RB);
+  R.at(kRecD + 0, j) = sg.depths.x; R.at(kRecD + 1, j) = sg.depths.y;
+  R.at(kRecD + 2, j) = sg.depths.z; R.at(kRecD + 3, j) = sg.depths.w;
+  R.at(kRecKey, j) = __uint_as_float(key);
+  // relative to the tile origin, clamped to one pixel beyond the tile
+  const int rx0 = max(min(x0 - (int)bx0, TILE), -1), ry0 = max(min(y0 - (int)by0, TILE), -1);
+  const int rx1 = max(min(x1 - (int)bx0, TILE), -1), ry1 = max(min(y1 - (int)by0, TILE), -1);
+  R.at(kRecBox0, j) = __uint_as_float(((uint32_t)rx0 & 0xffffu) | ((uint32_t)ry0 << 16));
+  R.at(kRecBox1, j) = __uint_as_float(((uint32_t)rx1 & 0xffffu) | ((uint32_t)ry1 << 16));
+  // The interpolated depth is a two-level lerp of the four depths with
+  // weights in [0, 1]: at least their minimum, less the rounding of two lerps
+  // (a few ulp of the largest magnitude). Non-finite depths can interpolate
+  // to NaN, which passes the depth test.
+  const bool finite = isfinite(sg.depths.x) && isfinite(sg.depths.y) && isfinite(sg.depths.z) &&
+                      isfinite(sg.depths.w);
+  const float dn = fminf(fminf(sg.depths.x, sg.depths.y), fminf(sg.depths.z, sg.depths.w));
+  const float dm = fmaxf(fmaxf(fabsf(sg.depths.x), fabsf(sg.depths.y)), fmaxf(fabsf(sg.depths.z), fabsf(sg.depths.w)));
+  R.at(kRecNear, j) = finite ? dn - (dm * 1e-6f + 1e-30f) : -INFINITY;
+  interp_pre(sg, &R.at(kRecIp, j), RB);
+}
+
 template <int TILE>
 __device__ __forceinline__ void fine_raster_body(FineArgs a, uint32_t block, uint32_t blocks) {
   constexpr uint32_t NPX = TILE * TILE;
@@ -1235,6 +1275,10 @@ __device__ __forceinline__ void fine_raster_body(FineArgs a, uint32_t block, uin
   L.poolNext = reinterpret_cast<uint32_t*>(L.poolColor + a.ldsPool);
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_sliceOff;  // slots of the slice memory taken in the bin
+  // the tile's bins, loaded together: minDepth, maxDepth, segments, first
+  // element; and the first record of each bin when the tile is projected whole
+  __shared__ uint4 s_bin[64];
+  __shared__ uint32_t s_binRec[65];
 
   const fh_render_uniforms* __restrict__ u = a.u;
   const uint32_t tid = threadIdx.x;
@@ -1263,6 +1307,22 @@ __device__ __forceinline__ void fine_raster_body(FineArgs a, uint32_t block, uin
     __syncthreads();
     const uint32_t tileIdx = s_tile;
     if (tileIdx == kInvalidPtr) break;
+    if (tid < 64u) {
+      // the tile's bins: segment ranges, and where each bin's records start
+      // when the whole tile is projected at once
+      uint4 tr = make_uint4(0u, 0u, 0u, 0u);
+      uint32_t n = 0;
+      if (tid < a.B && tileIdx < tileCount) {
+        const uint32_t tb = tileIdx * a.B + tid;
+        const fh_tile_result r = a.tiles[tb];
+        tr = make_uint4(r.minDepth, r.maxDepth, r.padding0, min(a.binStart[tb], a.capacity));
+        if (tr.y != 0u) n = min(tr.z, a.capacity - tr.w);
+      }
+      const uint32_t e = wave_incl_scan(n);
+      s_bin[tid] = make_uint4(tr.x, tr.y, n, tr.w);
+      s_binRec[tid] = e - n;
+      if (tid == 63u) s_binRec[64] = e;
+    }
     const uint32_t row = tileIdx / tg.tilesX;
     const uint32_t tx = tileIdx - tg.tilesX * row, ty = row;
     const uint32_t bx0 = clampu(tx * TILE, 0u, tg.vpW - 1u), by0 = clampu(ty * TILE, 0u, tg.vpH - 1u);
@@ -1275,55 +1335,43 @@ __device__ __forceinline__ void fine_raster_body(FineArgs a, uint32_t block, uin
       L.done[tid] = 0u;
     }
     bool done = !pixThread;
+    bool blended = false;  // a bin's slices were blended into fc
     f4 fc = mk4(0.0f, 0.0f, 0.0f, 0.0f);
     if (readable) { const float4 v = a.fb[fbIdx]; fc = mk4(v.x, v.y, v.z, v.w); }
 
+    __syncthreads();  // s_bin, s_binRec
+    // A tile whose segments fit in one batch is projected once, all bins
+    // together, so a bin costs no memory round trips before its fragments.
+    const uint32_t tileSegs = s_binRec[64];
+    const bool whole = tileSegs <= a.batch;
+    if (whole) {
+      if (tid < tileSegs) {
+        uint32_t lo = 0, hi = a.B;  // the last bin whose first record is <= tid
+        while (hi - lo > 1u) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (s_binRec[mid] <= tid) lo = mid;
+          else hi = mid;
+        }
+        fine_project<TILE>(a, L, tid, a.binKey[s_bin[lo].w + (tid - s_binRec[lo])], vpf, bx0, by0);
+      }
+      __syncthreads();
+    }
     for (uint32_t bin = 0; bin < a.B && tileIdx < tileCount; bin++) {
-      const uint32_t tb = tileIdx * a.B + bin;
-      const fh_tile_result tr = a.tiles[tb];
-      if (tr.maxDepth == 0u) continue;  // _getTileDepth == 0 means an empty bin (:215)
-      const f2 td = mk2((float)(0xffffffffu - tr.minDepth) / 4294967296.0f,
-                        (float)tr.maxDepth / 4294967296.0f);
-      const uint32_t start = min(a.binStart[tb], a.capacity);
-      const uint32_t n = min(tr.padding0, a.capacity - start);
+      const uint4 tr = s_bin[bin];
+      if (tr.y == 0u) continue;  // _getTileDepth == 0 means an empty bin (:215)
+      const f2 td = mk2((float)(0xffffffffu - tr.x) / 4294967296.0f, (float)tr.y / 4294967296.0f);
+      const uint32_t start = tr.w, n = tr.z;
       if (tid == 0) s_sliceOff = 0;
-      for (uint32_t b0 = 0; b0 < n; b0 += a.batch) {
+      if (whole) {
+        __syncthreads();
+        fine_cheap_batch<TILE>(a, L, gpool, s_binRec[bin], n, bx0, by0, td, invSegs, &s_sliceOff);
+      }
+      for (uint32_t b0 = 0; !whole && b0 < n; b0 += a.batch) {
         const uint32_t nb = min(a.batch, n - b0);
         __syncthreads();
-        if (tid < nb) {
-          const uint32_t key = a.binKey[start + b0 + tid];
-          const Seg sg = project_segment(u, a.pos, a.tan, a.P, key >> 8, key & 0xffu, vpf, u->fiberRadius);
-          int x0, y0, x1, y1;
-          seg_pixel_bbox(sg, x0, y0, x1, y1);
-          const auto& R = L.recs;
-          const float vv[8] = {sg.v00.x, sg.v00.y, sg.v01.x, sg.v01.y, sg.v10.x, sg.v10.y, sg.v11.x, sg.v11.y};
-          float amax = 0.0f;
-#pragma unroll
-          for (int i = 0; i < 8; i++) { R.at(kRecV + i, tid) = vv[i]; amax = fmaxf(amax, fabsf(vv[i])); }
-          // collinear quads test the whole box; far-off vertices skip the span
-          span_frame(sg, x0 > -(1 << 20) && amax < 65536.0f, &R.at(kRecSpan, tid), RB);
-          R.at(kRecD + 0, tid) = sg.depths.x; R.at(kRecD + 1, tid) = sg.depths.y;
-          R.at(kRecD + 2, tid) = sg.depths.z; R.at(kRecD + 3, tid) = sg.depths.w;
-          R.at(kRecKey, tid) = __uint_as_float(key);
-          // relative to the tile origin, clamped to one pixel beyond the tile
-          const int rx0 = max(min(x0 - (int)bx0, TILE), -1), ry0 = max(min(y0 - (int)by0, TILE), -1);
-          const int rx1 = max(min(x1 - (int)bx0, TILE), -1), ry1 = max(min(y1 - (int)by0, TILE), -1);
-          R.at(kRecBox0, tid) = __uint_as_float(((uint32_t)rx0 & 0xffffu) | ((uint32_t)ry0 << 16));
-          R.at(kRecBox1, tid) = __uint_as_float(((uint32_t)rx1 & 0xffffu) | ((uint32_t)ry1 << 16));
-          // The interpolated depth is a two-level lerp of the four depths with
-          // weights in [0, 1]: at least their minimum, less the rounding of
-          // two lerps (a few ulp of the largest magnitude). Non-finite depths
-          // can interpolate to NaN, which passes the depth test.
-          const bool finite = isfinite(sg.depths.x) && isfinite(sg.depths.y) && isfinite(sg.depths.z) &&
-                              isfinite(sg.depths.w);
-          const float dn = fminf(fminf(sg.depths.x, sg.depths.y), fminf(sg.depths.z, sg.depths.w));
-          const float dm = fmaxf(fmaxf(fabsf(sg.depths.x), fabsf(sg.depths.y)),
-                                 fmaxf(fabsf(sg.depths.z), fabsf(sg.depths.w)));
-          R.at(kRecNear, tid) = finite ? dn - (dm * 1e-6f + 1e-30f) : -INFINITY;
-          interp_pre(sg, &R.at(kRecIp, tid), RB);
-        }
+        if (tid < nb) fine_project<TILE>(a, L, tid, a.binKey[start + b0 + tid], vpf, bx0, by0);
         __syncthreads();
-        fine_cheap_batch<TILE>(a, L, gpool, nb, bx0, by0, td, invSegs, &s_sliceOff);
+        fine_cheap_batch<TILE>(a, L, gpool, 0u, nb, bx0, by0, td, invSegs, &s_sliceOff);
         __syncthreads();
         if (s_sliceOff >= a.perProc) break;  // slice memory exhausted (:252-264)
       }
@@ -1359,13 +1407,17 @@ __device__ __forceinline__ void fine_raster_body(FineArgs a, uint32_t block, uin
           const float c = satf((float)cnt / (float)dbgMax);
           fc = mk4(c, 0.0f, 0.0f, 1.0f);
         }
-        if (readable) a.fb[fbIdx] = make_float4(fc.x, fc.y, fc.z, fc.w);
+        blended = true;
         done = !isDbg && fc.w >= 0.999f;
         L.done[tid] = done ? 1u : 0u;
       }
       // checkAllPixelsInWkgrpDone :305-327
       if (__syncthreads_and(done)) break;
     }
+    // The pixel's value after its last blended bin. Stored once per tile: a
+    // barrier waits for the workgroup's outstanding stores, so a store per
+    // bin would put a memory round trip between bins.
+    if (readable && blended) a.fb[fbIdx] = make_float4(fc.x, fc.y, fc.z, fc.w);
   }
 }
 
