@@ -6,10 +6,14 @@ A step is one frame of the chain on the 1080p character-scale workload
 pre-pass, integration, particle-to-grid), then the tile pass, the tile sort,
 the fine pass and the shading pass, with inputs resident in HBM.
 
-By default the frame is replayed as a HIP graph (one graph per parity of the
-tick-tock position buffers). Prints one JSON line. `value` and the frame
-latencies come from a timed region of --steps frames with events only at
-frame boundaries. `roofline` is for the kernel that takes the most time in a
+By default frames are pipelined and replayed as HIP graphs (one graph per
+parity of the tick-tock position buffers): each frame runs the simulation
+step of frame f on a second stream beside the render passes of frame f-1, so
+every frame still does one simulation step and one render of the chain.
+Prints one JSON line. `value` and the frame latencies come from a timed
+region of --steps frames with events only at frame boundaries; the latency
+from an image's simulation step to its present, and the rate of one graph per
+frame without pipelining (`unpipelined`), are reported beside them. `roofline` is for the kernel that takes the most time in a
 second timed region of as many frames, from its algorithmic bytes per launch
 and its average launch duration, measured with HIP events that the library
 records around each of its kernels on its own stream (per-kernel events cost
@@ -93,14 +97,14 @@ def stage_bytes(wl, n_pairs: int, tiles: int) -> dict:
     }
 
 
-def full_frame(wl, r0, steps: int, dev) -> dict:
+def full_frame(wl, r0, steps: int, dev, branches: bool = True) -> dict:
     """Frames of the same workload with the passes that feed the shading pass
     computed on the device every frame (ShadowMapPass's hair draw, HwHairPass,
     AoPass), as the reference's renderer runs them; replayed as HIP graphs."""
     import torch
     from frostbitten_hair_webgpu_amd import passes as PS
     inputs = {k: v.cpu().numpy() for k, v in r0.inputs.items()}
-    r = PS.HairRenderer(wl, dev, inputs=inputs, hw_passes=True)
+    r = PS.HairRenderer(wl, dev, inputs=inputs, hw_passes=True, branches=branches)
     ring = torch.from_numpy(np.concatenate([r.sim_uniforms(f).view(np.uint8).reshape(-1)
                                             for f in range(2 * steps + 2)])).to(dev).view(2 * steps + 2, -1)
     r.beforeFirstFrame()
@@ -346,6 +350,12 @@ def main() -> None:
     ap.add_argument("--serve-budget-gib", type=float, default=8.0, help="HBM budget of the resident instances")
     ap.add_argument("--no-graph", action="store_true", help="launch every call eagerly instead of replaying "
                     "the frame as a HIP graph")
+    ap.add_argument("--no-branches", action="store_true",
+                    help="run every pass of the frame on one stream (no overlap of the grid accumulation, "
+                         "hardware-raster and shading passes with the render chain)")
+    ap.add_argument("--no-pipeline", action="store_true",
+                    help="one graph per frame (simulation step then render passes) instead of pipelined frames "
+                         "(the simulation step of frame f beside the render passes of frame f-1)")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="no per-kernel HIP events (roofline omitted); frame time only")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -385,8 +395,8 @@ def main() -> None:
         return
     t_setup = time.perf_counter()
     wl = A.make_workload(args.workload)
-    r = PS.HairRenderer(wl, dev)
-    total = args.warmup + 2 * args.steps  # two timed regions
+    r = PS.HairRenderer(wl, dev, branches=not args.no_branches)
+    total = args.warmup + 3 * args.steps + 4  # three timed regions, pipeline prologue and sizing frame
     # per-frame simulation uniforms (frameIdx changes), uploaded once up front
     ring = torch.from_numpy(np.concatenate([r.sim_uniforms(f).view(np.uint8).reshape(-1) for f in range(total)]))
     ring = ring.to(dev).view(total, -1)
@@ -416,6 +426,7 @@ def main() -> None:
         r.presentPass.cmdDraw(ctx, r.screenTexture)
         rec(5)
         r.shadingPass.cmdComputeShadingPoints(ctx, r.hair)
+        r.join_branch()  # the grid accumulation ran beside the render passes
         rec(6)
         r.frameIdx += 1
 
@@ -427,7 +438,25 @@ def main() -> None:
         if events:
             events[6].record()
 
+    def pipelined_frame(events=None):
+        r.simulationUniforms.copy_(ring[r.frameIdx])
+        if events:
+            events[0].record()
+        r.frame_pipelined_graph(upload_uniforms=False)
+        if events:
+            events[6].record()
+
+    def region(step):
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(7)] for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            step(evs[k])
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, evs
+
     use_graph = not args.no_graph
+    pipelined = use_graph and not args.no_branches and not args.no_pipeline
     r.beforeFirstFrame()
     for _ in range(args.warmup):
         frame()
@@ -435,16 +464,31 @@ def main() -> None:
     step = graph_frame if use_graph else frame
 
     # Timed region 1: the frames alone (events only at frame boundaries) give
-    # value and the frame latencies.
-    if use_graph:
+    # value and the frame latencies. Pipelined, each frame runs the simulation
+    # step of frame f beside the render passes of frame f-1.
+    unpipelined = None
+    if pipelined:
+        r.simulationUniforms.copy_(ring[r.frameIdx])
+        r.pipeline_begin(upload_uniforms=False)
+        r.simulationUniforms.copy_(ring[r.frameIdx])
+        r.frame_pipelined(upload_uniforms=False)  # every call once before capture
+        r.capture_pipelined()
+        elapsed, evs = region(pipelined_frame)
+        r.pipeline_end()
+        # sim-to-present: from the start of the frame that ran an image's
+        # simulation step to the end of the frame that presented it
+        s2p = np.array([evs[k - 1][0].elapsed_time(evs[k][6]) for k in range(1, args.steps)])
         r.capture_frames()
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(7)] for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        step(evs[k])
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+        el_u, evs_u = region(graph_frame)
+        per_u = np.array([evs_u[k][0].elapsed_time(evs_u[k][6]) for k in range(args.steps)])
+        unpipelined = {"value": round(args.steps / el_u, 3), "ms_per_step": round(el_u / args.steps * 1e3, 4),
+                       "latency_ms": {"p50": round(float(np.percentile(per_u, 50)), 4),
+                                      "p99": round(float(np.percentile(per_u, 99)), 4)},
+                       "frame": "one graph per frame: simulation step, then the render passes"}
+    else:
+        if use_graph:
+            r.capture_frames()
+        elapsed, evs = region(step)
 
     # Timed region 2: as many further frames with HIP events around every
     # kernel on the library's stream (in graph replays, event-record nodes
@@ -514,9 +558,14 @@ def main() -> None:
                                f"tile {wl.render.tileSize}, {wl.render.tileDepthBins} bins x "
                                f"{wl.render.slicesPerPixel} slices, grid {wl.sim.gridDims}^3, SDF {wl.sdf_dims}^3, "
                                f"{wl.sim.constraintIterations} constraint iterations",
-                   "global_batch": 1, "parallelism": "none (1 GPU)"},
+                   "global_batch": 1, "parallelism": "none (1 GPU)",
+                   "overlap": "none (one stream)" if args.no_branches else
+                              ("pipelined: simulation step of frame f beside the render passes of frame f-1; "
+                               "shading beside combine/present" if pipelined else
+                               "grid accumulation beside tiles/sort/fine, shading beside combine/present")},
         "latency_ms": {"p50": round(float(np.percentile(per_frame, 50)), 4),
                        "p99": round(float(np.percentile(per_frame, 99)), 4)},
+        "pipelined": pipelined,
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "stage_hbm_GBps": {k: round(v / (stage_ms[k] * 1e-3) / 1e9, 2)
                            for k, v in stage_bytes(wl, n_pairs, tiles).items() if stage_ms.get(k)},
@@ -524,16 +573,22 @@ def main() -> None:
                        "segments_rasterized_per_s": round(wl.strands * (wl.points - 1) * 1e3 / ms, 1),
                        "segment_tile_pairs": n_pairs, "drawn_tiles": drawn_tiles},
         "kernel_us": {k: round(v * 1e3, 2) for k, v in sorted(kernel_ms.items(), key=lambda kv: -kernels[kv[0]][1])},
-        "kernel_timing": f"second timed region of {args.steps} frames with HIP events around every kernel",
+        "kernel_timing": f"a further timed region of {args.steps} frames with HIP events around every kernel "
+                         f"(one graph per frame, kernels one after another)",
         "roofline": None if dom is None else {
             "bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
             "algorithmic_bytes": int(kb[dom]), "avg_launch_us": round(kernel_ms[dom] * 1e3, 2)},
     }
+    if pipelined:
+        out["latency_ms"]["frame"] = "per pipelined frame (frame period)"
+        out["latency_ms"]["sim_to_present_p50"] = round(float(np.percentile(s2p, 50)), 4)
+        out["latency_ms"]["sim_to_present_p99"] = round(float(np.percentile(s2p, 99)), 4)
+        out["unpipelined"] = unpipelined
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, r)
     if args.full_frame:
-        out["full_frame"] = full_frame(wl, r, min(args.steps, 30), dev)
+        out["full_frame"] = full_frame(wl, r, min(args.steps, 30), dev, branches=not args.no_branches)
     if args.batch_instances:
         del r
         out["batched"] = batched_vs_plain(args.batch_instances, min(args.steps, 20), 2, not args.no_tune, dev,

@@ -70,6 +70,7 @@ int ensure_scratch(fh_ctx* ctx, size_t bytes) {
   if (ctx->recording) ctx->rec_grew = true;
   if (ctx->scratch) {
     FH_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->branch_stream) FH_HIP(hipStreamSynchronize(ctx->branch_stream));
     FH_HIP(hipFree(ctx->scratch));
     ctx->scratch = nullptr;
     ctx->scratch_bytes = 0;
@@ -236,9 +237,17 @@ int fh_create(int device, fh_ctx** out) {
 int fh_destroy(fh_ctx* ctx) {
   if (!ctx) return FH_OK;
   hipSetDevice(ctx->device);
+  if (ctx->branch_open) fh_branch_end(ctx);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  if (ctx->branch_stream) {
+    hipStreamSynchronize(ctx->branch_stream);
+    hipStreamDestroy(ctx->branch_stream);
+  }
+  if (ctx->branch_fork) hipEventDestroy(ctx->branch_fork);
+  if (ctx->branch_done) hipEventDestroy(ctx->branch_done);
   for (hipEvent_t e : ctx->prof_events) hipEventDestroy(e);
   if (ctx->scratch) hipFree(ctx->scratch);
+  if (ctx->branch_scratch) hipFree(ctx->branch_scratch);
   if (ctx->srgb_thresholds) hipFree(ctx->srgb_thresholds);
   if (ctx->owns_stream && ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -247,6 +256,7 @@ int fh_destroy(fh_ctx* ctx) {
 
 int fh_set_stream(fh_ctx* ctx, void* stream) {
   FH_CHECK_CTX(ctx);
+  FH_REQUIRE(!ctx->branch_open && !ctx->branch_pending, "fh_set_stream: a branch is open or not joined");
   if (ctx->owns_stream && ctx->stream) {
     FH_HIP(hipStreamSynchronize(ctx->stream));
     FH_HIP(hipStreamDestroy(ctx->stream));
@@ -261,6 +271,7 @@ void* fh_get_stream(fh_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 int fh_profiler_enable(fh_ctx* ctx, uint32_t max_launches) {
   FH_CHECK_CTX(ctx);
   FH_REQUIRE(max_launches <= (1u << 20), "fh_profiler_enable: max_launches must be <= 2^20");
+  FH_REQUIRE(!ctx->branch_open && !ctx->branch_pending, "fh_profiler_enable: a branch is open or not joined");
   FH_HIP(hipStreamSynchronize(ctx->stream));
   ctx->prof_names.clear();
   while (ctx->prof_events.size() < 2ull * max_launches) {
@@ -300,7 +311,54 @@ int fh_profiler_read(fh_ctx* ctx, fh_kernel_time* out, uint32_t max_out, uint32_
 int fh_scratch_bytes(fh_ctx* ctx, uint64_t* bytes) {
   FH_CHECK_CTX(ctx);
   FH_REQUIRE(bytes != nullptr, "fh_scratch_bytes: null argument");
-  *bytes = ctx->scratch_bytes;
+  *bytes = ctx->scratch_bytes + ctx->branch_scratch_bytes;
+  return FH_OK;
+}
+
+int fh_branch_begin(fh_ctx* ctx) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(!ctx->branch_open, "fh_branch_begin: a branch is already open");
+  const bool live = !ctx->recording && ctx->prof_cap == 0;
+  if (live) {
+    if (!ctx->branch_stream) {
+      FH_HIP(hipSetDevice(ctx->device));
+      FH_HIP(hipStreamCreateWithFlags(&ctx->branch_stream, hipStreamNonBlocking));
+      FH_HIP(hipEventCreateWithFlags(&ctx->branch_fork, hipEventDisableTiming));
+      FH_HIP(hipEventCreateWithFlags(&ctx->branch_done, hipEventDisableTiming));
+    }
+    FH_HIP(hipEventRecord(ctx->branch_fork, ctx->stream));
+    FH_HIP(hipStreamWaitEvent(ctx->branch_stream, ctx->branch_fork, 0));
+    ctx->main_stream = ctx->stream;
+    ctx->stream = ctx->branch_stream;
+  }
+  std::swap(ctx->scratch, ctx->branch_scratch);
+  std::swap(ctx->scratch_bytes, ctx->branch_scratch_bytes);
+  ctx->branch_open = true;
+  ctx->branch_live = live;
+  return FH_OK;
+}
+
+int fh_branch_end(fh_ctx* ctx) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(ctx->branch_open, "fh_branch_end: no branch is open");
+  std::swap(ctx->scratch, ctx->branch_scratch);
+  std::swap(ctx->scratch_bytes, ctx->branch_scratch_bytes);
+  ctx->branch_open = false;
+  if (!ctx->branch_live) return FH_OK;
+  ctx->branch_live = false;
+  ctx->stream = ctx->main_stream;
+  ctx->main_stream = nullptr;
+  ctx->branch_pending = true;
+  FH_HIP(hipEventRecord(ctx->branch_done, ctx->branch_stream));
+  return FH_OK;
+}
+
+int fh_branch_join(fh_ctx* ctx) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(!ctx->branch_open, "fh_branch_join: the branch is still open");
+  if (!ctx->branch_pending) return FH_OK;
+  ctx->branch_pending = false;
+  FH_HIP(hipStreamWaitEvent(ctx->stream, ctx->branch_done, 0));
   return FH_OK;
 }
 
@@ -313,6 +371,7 @@ int fh_set_persistent_blocks(fh_ctx* ctx, uint32_t blocks) {
 int fh_record_begin(fh_ctx* ctx) {
   FH_CHECK_CTX(ctx);
   FH_REQUIRE(!ctx->recording && !ctx->capturing, "fh_record_begin: already recording or capturing");
+  FH_REQUIRE(!ctx->branch_open, "fh_record_begin: a branch is open");
   ctx->recording = true;
   ctx->rec_grew = false;
   ctx->rec.clear();
@@ -436,6 +495,7 @@ int fh_program_destroy(fh_program* program) {
 int fh_capture_begin(fh_ctx* ctx) {
   FH_CHECK_CTX(ctx);
   FH_REQUIRE(!ctx->capturing, "fh_capture_begin: already capturing");
+  FH_REQUIRE(!ctx->branch_open && !ctx->branch_pending, "fh_capture_begin: a branch is open or not joined");
   FH_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
   ctx->capturing = true;
   ctx->cap_names.clear();
@@ -447,10 +507,14 @@ int fh_capture_end(fh_ctx* ctx, fh_graph** out) {
   FH_REQUIRE(ctx->capturing, "fh_capture_end: not capturing");
   FH_REQUIRE(out != nullptr, "fh_capture_end: out is null");
   *out = nullptr;
+  // a capture must end on its origin stream with every branch joined into it
+  const bool was_open = ctx->branch_open;
+  if (was_open) fh_branch_end(ctx);
+  fh_branch_join(ctx);
   ctx->capturing = false;
   fh_graph* g = new fh_graph();
   hipError_t e = hipStreamEndCapture(ctx->stream, &g->graph);
-  int rc = FH_OK;
+  int rc = was_open ? fh::fail(FH_ERR_INVALID_ARG, "a branch was still open") : FH_OK;
   if (e == hipSuccess && !ctx->cap_names.empty()) rc = fh::add_timing_nodes(g, ctx->cap_names);
   ctx->cap_names.clear();
   if (e == hipSuccess && rc == FH_OK) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
@@ -497,6 +561,7 @@ int fh_graph_destroy(fh_graph* graph) {
 int fh_sync(fh_ctx* ctx) {
   FH_CHECK_CTX(ctx);
   FH_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->branch_stream) FH_HIP(hipStreamSynchronize(ctx->branch_stream));
   return FH_OK;
 }
 

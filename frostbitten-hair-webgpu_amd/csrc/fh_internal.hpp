@@ -35,6 +35,20 @@ struct fh_ctx {
   // per-bin offsets, prefix-scan temporaries).
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
+  // Branch (fh_branch_begin .. fh_branch_end): calls that may overlap the
+  // main stream's later work run on a second stream, ordered after the main
+  // stream's work at fh_branch_begin and joined back by fh_branch_join. A
+  // branch has its own scratch, swapped into `scratch` while it is open, so
+  // the two streams never share temporaries. While recording or timing
+  // kernels the calls stay on the main stream (the scratch is still swapped).
+  hipStream_t branch_stream = nullptr;
+  hipStream_t main_stream = nullptr;  // the context's stream while a branch runs live
+  hipEvent_t branch_fork = nullptr, branch_done = nullptr;
+  void* branch_scratch = nullptr;
+  size_t branch_scratch_bytes = 0;
+  bool branch_open = false;     // between fh_branch_begin and fh_branch_end
+  bool branch_live = false;     // the open branch runs on branch_stream
+  bool branch_pending = false;  // branch work not yet joined into the main stream
   // The 256 thresholds of the rgba8unorm-srgb store (fh_present), uploaded
   // at creation so that no call allocates or copies them later.
   float* srgb_thresholds = nullptr;

@@ -126,6 +126,9 @@ SIGNATURES = {
     "fh_pack_sim_uniforms": (i32, [C.POINTER(SimParams), vp]),
     "fh_profiler_enable": (i32, [vp, u32]),
     "fh_capture_begin": (i32, [vp]),
+    "fh_branch_begin": (i32, [vp]),
+    "fh_branch_end": (i32, [vp]),
+    "fh_branch_join": (i32, [vp]),
     "fh_capture_end": (i32, [vp, C.POINTER(vp)]),
     "fh_graph_launch": (i32, [vp, vp]),
     "fh_graph_destroy": (i32, [vp]),
@@ -218,6 +221,15 @@ class Context:
 
     def sync(self) -> None:
         check(self.lib.fh_sync(self.handle), "fh_sync")
+
+    def branch_begin(self) -> None:
+        check(self.lib.fh_branch_begin(self.handle), "fh_branch_begin")
+
+    def branch_end(self) -> None:
+        check(self.lib.fh_branch_end(self.handle), "fh_branch_end")
+
+    def branch_join(self) -> None:
+        check(self.lib.fh_branch_join(self.handle), "fh_branch_join")
 
     def capture_begin(self) -> None:
         check(self.lib.fh_capture_begin(self.handle), "fh_capture_begin")

@@ -329,8 +329,12 @@ class HairRenderer:
     outputs (depth, AO, shadow map) provided as precomputed inputs."""
 
     def __init__(self, wl: A.Workload, device="cuda", native: N.Context | None = None, shadow_size: int = 2048,
-                 inputs: dict | None = None, hw_passes: bool = False):
+                 inputs: dict | None = None, hw_passes: bool = False, branches: bool = True):
         self.wl = wl
+        # branches: passes that neither read what the tile, sort and fine
+        # passes write nor write what they read run beside them on the
+        # context's second stream (fh_branch_*)
+        self.branches = branches
         self.device = torch.device(device)
         self.native = native or N.Context(self.device.index or 0)
         self.native.set_stream(torch.cuda.current_stream(self.device))
@@ -374,10 +378,12 @@ class HairRenderer:
             self.shadowMapPass = ShadowMapPass()
             self.hwHairPass = HwHairPass()
             self.aoPass = AoPass()
-            self.depthTexture = torch.empty_like(self.inputs["meshDepth"])
-            self.normalsTexture = torch.empty_like(self.inputs["meshNormals"])
+            # zero-initialised like every WebGPU texture: the shading pass of
+            # beforeFirstFrame reads them before any frame has drawn them
+            self.depthTexture = torch.zeros_like(self.inputs["meshDepth"])
+            self.normalsTexture = torch.zeros_like(self.inputs["meshNormals"])
             self.aoTexture = torch.zeros(((H + 1) // 2, (W + 1) // 2), dtype=torch.float32, device=self.device)
-            self.shadowTexture = torch.empty_like(self.inputs["shadowMap"])
+            self.shadowTexture = torch.zeros_like(self.inputs["shadowMap"])
 
     def sim_uniforms(self, frame_idx: int) -> np.ndarray:
         return L.pack_sim_uniforms(self.wl.sim, self.model, self.wl.sdf_bounds, self.grid.bounds, frame_idx)
@@ -413,10 +419,25 @@ class HairRenderer:
         self.gridPostSimPass.cmdUpdateGridsAfterSim(ctx, self.hair)
         self.shadingPass.cmdComputeShadingPoints(ctx, self.hair)
 
+    def _branch_begin(self) -> None:
+        if self.branches:
+            self.native.branch_begin()
+
+    def _branch_end(self) -> None:
+        if self.branches:
+            self.native.branch_end()
+
+    def join_branch(self) -> None:
+        if self.branches:
+            self.native.branch_join()
+
     def simulateHair(self, ctx: PassCtx) -> None:
         self.gridPreSimPass.cmdUpdateGridsBeforeSim(ctx, self.hair)
         self.simPass.cmdSimulateHairPositions(ctx, self.hair)
+        # the grid accumulation feeds only the next frame's grid passes
+        self._branch_begin()
         self.gridPostSimPass.cmdUpdateGridsAfterSim(ctx, self.hair)
+        self._branch_end()
 
     def drawScene(self, ctx: PassCtx) -> None:
         """The HDR scene of the background and mesh passes (renderer.ts:187-188,
@@ -426,22 +447,37 @@ class HairRenderer:
 
     def drawHair(self, ctx: PassCtx) -> None:
         """cmdDrawScene's hair part (renderer.ts:262-295) and the present pass
-        (:205)."""
+        (:205).
+
+        The passes run in the reference's order except for where they overlap:
+        the shadow map, the hair depth / normals and AO (updateResourcesForNextFrame,
+        renderer.ts:285-295) read the positions and the meshes' inputs and write
+        textures that only the shading pass and the present pass's debug views
+        read, so they run on the branch beside the tile, sort and fine passes
+        (which read the meshes' depth, not the hair's). The shading pass writes
+        the shading points the fine pass reads, so it starts after the fine
+        pass, beside the combine and present passes."""
         self.drawScene(ctx)
         if self.hw_passes:
+            self._branch_begin()
             self.shadowMapPass.cmdUpdateShadowMap(ctx, self.hair, self.inputs["shadowMap"])
+            self.hwHairPass.cmdDrawHair(ctx, self.hair, self.inputs["meshDepth"], self.inputs["meshNormals"])
+            self.aoPass.cmdCalcAo(ctx)
+            self._branch_end()
         self.tilesPass.cmdClearBeforeRender(ctx)
         self.finePass.cmdClearBeforeRender(ctx)
         self.sortPass.cmdClearBeforeRender(ctx)
         self.tilesPass.cmdDrawHairToTiles(ctx, self.hair)
         self.sortPass.cmdSortHairTiles(ctx)
         self.finePass.cmdRasterizeSlicesHair(ctx, self.hair, self.tilesPass)
-        self.hairCombinePass.cmdCombineRasterResults(ctx)
-        if self.hw_passes:  # updateResourcesForNextFrame (renderer.ts:285-295)
-            self.hwHairPass.cmdDrawHair(ctx, self.hair, self.inputs["meshDepth"], self.inputs["meshNormals"])
-            self.aoPass.cmdCalcAo(ctx)
+        self._branch_begin()
         self.shadingPass.cmdComputeShadingPoints(ctx, self.hair)
+        self._branch_end()
+        self.hairCombinePass.cmdCombineRasterResults(ctx)
+        if self.hw_passes:  # the present pass's debug views read the hair depth, normals and AO
+            self.join_branch()
         self.presentPass.cmdDraw(ctx, self.screenTexture)
+        self.join_branch()
 
     def frame(self, upload_uniforms: bool = True) -> None:
         """One sim step + render of the hair chain (Renderer.cmdRender)."""
@@ -452,6 +488,69 @@ class HairRenderer:
         self.drawHair(ctx)
         self.frameIdx += 1
 
+    # Pipelined frames. The simulation step of frame f writes only the older
+    # buffer of the tick-tock position pair (the new "now"), and the render
+    # passes of frame f-1 read only the newer one (frame f-1's "now"); the
+    # grids and the shading points are read by one chain and written by the
+    # other in an order that a frame already fixes. So one pipelined frame
+    # runs the simulation step of frame f on the branch beside the render
+    # passes of frame f-1, and each image is presented one frame after its
+    # simulation step. Every pass computes exactly what it computes in
+    # frame(): images and positions are the same, frame for frame.
+
+    def pipeline_begin(self, upload_uniforms: bool = True) -> None:
+        """The prologue: the simulation step of the first frame alone."""
+        if upload_uniforms:
+            self.upload_sim_uniforms()
+        ctx = self.ctx()
+        self.simulateHair(ctx)
+        self.join_branch()
+        self._pipelined = True
+        self.frameIdx += 1
+
+    def _pipelined_calls(self) -> None:
+        ctx = self.ctx()
+        drawn = self.hair.current  # frame f-1's positions
+        self._branch_begin()
+        self.gridPreSimPass.cmdUpdateGridsBeforeSim(ctx, self.hair)
+        self.simPass.cmdSimulateHairPositions(ctx, self.hair)  # swaps the pair
+        self.gridPostSimPass.cmdUpdateGridsAfterSim(ctx, self.hair)
+        self._branch_end()
+        stepped = self.hair.current
+        self.hair.current = drawn
+        try:
+            self.drawHair(ctx)
+        finally:
+            self.hair.current = stepped
+
+    def frame_pipelined(self, upload_uniforms: bool = True) -> None:
+        """The simulation step of frame frameIdx beside the render passes of
+        frame frameIdx - 1 (after pipeline_begin)."""
+        assert getattr(self, "_pipelined", False), "pipeline_begin() first"
+        if upload_uniforms:
+            self.upload_sim_uniforms()
+        self._pipelined_calls()
+        self.frameIdx += 1
+
+    def pipeline_end(self) -> None:
+        """The epilogue: the render passes of the last simulated frame."""
+        assert getattr(self, "_pipelined", False), "pipeline_begin() first"
+        self.drawHair(self.ctx())
+        self._pipelined = False
+
+    def capture_pipelined(self) -> list:
+        """capture_frames() for pipelined frames (two graphs, one per parity
+        of the position pair); replayed by frame_pipelined_graph()."""
+        return self._capture(self._pipelined_calls, "_pgraphs", "_pgraphBase")
+
+    def frame_pipelined_graph(self, upload_uniforms: bool = True) -> None:
+        assert getattr(self, "_pipelined", False), "pipeline_begin() first"
+        if upload_uniforms:
+            self.upload_sim_uniforms()
+        self._pgraphs[self.hair.current ^ self._pgraphBase].launch()
+        self.hair.swapPositionBuffersAfterSimIntegration()
+        self.frameIdx += 1
+
     def capture_frames(self) -> list:
         """Records the frame's calls as two HIP graphs, one per parity of the
         tick-tock position buffers (hairObject.ts:102-106), like recording
@@ -459,6 +558,13 @@ class HairRenderer:
         before (so no scratch grows while recording); the graphs read the
         simulation uniforms from `simulationUniforms`, whose contents the
         caller updates per frame."""
+        def calls():
+            ctx = self.ctx()
+            self.simulateHair(ctx)
+            self.drawHair(ctx)
+        return self._capture(calls, "_graphs", "_graphBase")
+
+    def _capture(self, calls, attr: str, base_attr: str) -> list:
         graphs = []
         # recording needs a stream of its own (not the legacy default stream)
         main = torch.cuda.current_stream(self.device)
@@ -467,18 +573,16 @@ class HairRenderer:
         self.native.set_stream(side)
         try:
             for _ in range(2):
-                ctx = self.ctx()
                 self.native.capture_begin()
                 try:
-                    self.simulateHair(ctx)
-                    self.drawHair(ctx)
+                    calls()
                 finally:
                     g = self.native.capture_end()
                 graphs.append(g)
         finally:
             self.native.set_stream(main)
-        self._graphs = graphs
-        self._graphBase = self.hair.current  # graphs[0] reads positions[current] as "now"
+        setattr(self, attr, graphs)
+        setattr(self, base_attr, self.hair.current)  # graphs[0] reads positions[current] as "now"
         return graphs
 
     def frame_graph(self, upload_uniforms: bool = True) -> None:

@@ -583,6 +583,25 @@ int fh_capture_end(fh_ctx* ctx, fh_graph** out);
 int fh_graph_launch(fh_ctx* ctx, fh_graph* graph);
 int fh_graph_destroy(fh_graph* graph);
 
+/* Branches: the entry points called between fh_branch_begin and
+ * fh_branch_end run on a second stream of the context, after everything
+ * enqueued on the context's stream before fh_branch_begin, and may overlap
+ * what the context's stream runs next; fh_branch_join orders the context's
+ * later work after them. The reference records one command encoder per
+ * frame (renderer.ts:108-120) and leaves overlap to the WebGPU
+ * implementation; a caller uses a branch for passes that neither read what
+ * the overlapped passes write nor write what they read (the grid
+ * accumulation beside the tile, sort and fine passes). A branch has a
+ * scratch of its own. Consecutive branches run in order on the second
+ * stream. While the context records (fh_record_begin) or times kernels
+ * (fh_profiler_enable) the calls stay on the context's stream. Inside a
+ * capture the fork and join become edges of the graph; fh_capture_end
+ * joins an unjoined branch and fails if one is still open. fh_sync waits
+ * for both streams. */
+int fh_branch_begin(fh_ctx* ctx);
+int fh_branch_end(fh_ctx* ctx);
+int fh_branch_join(fh_ctx* ctx);
+
 /* ------------------------------------------------------------------------ */
 /* Batching: many hair instances per launch                                 */
 /* ------------------------------------------------------------------------ */

@@ -1,0 +1,119 @@
+"""Branches (fh_branch_*): frames whose grid accumulation, hardware-raster and
+shading passes run on the context's second stream beside the tile, sort and
+fine passes leave exactly what frames on one stream leave: positions, grids,
+tile lists, images, shading, depth / normals / AO and the presented screen,
+eagerly and replayed as HIP graphs, over both parities of the position
+buffers. Pipelined frames (the simulation step of frame f beside the render
+passes of frame f-1) leave, frame for frame, the simulation state of frame f
+and the render state of frame f-1 that plain frames leave."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+SIM_KEYS = ("now", "prev", "dv", "dgw")
+
+
+def _state(r):
+    from gpu_util import walk_lists
+    tiles = _np(r.tilesPass.hairTilesBuffer).reshape(-1, 4)
+    seg = _np(r.tilesPass.hairTileSegmentsBuffer).view(np.uint32)
+    lists = walk_lists(tiles.view(np.uint32), seg, r.tilesPass.segmentsCapacity)
+    out = dict(now=_np(r.hair.positionsNow), prev=_np(r.hair.positionsPrev), dv=_np(r.grid.densityVelocity),
+               dgw=_np(r.grid.densityGradAndWind), depth=tiles[:, [0, 1, 3]], lists=sorted(lists.items()),
+               counts=_np(r.tilesPass.segmentCountPerTileBuffer),
+               img=_np(r.finePass.image(r.wl.width, r.wl.height)), shading=_np(r.hair.shading),
+               hdr=_np(r.hdrRenderTexture), screen=_np(r.screenTexture))
+    if r.hw_passes:
+        out.update(hair_depth=_np(r.depthTexture), normals=_np(r.normalsTexture), ao=_np(r.aoTexture),
+                   shadow=_np(r.shadowTexture))
+    return out
+
+
+def _same(a, b, where):
+    assert a.keys() == b.keys()
+    for k in a:
+        if isinstance(a[k], np.ndarray):
+            x, y = np.ascontiguousarray(a[k]), np.ascontiguousarray(b[k])
+            assert x.dtype == y.dtype and np.array_equal(x.view(np.uint8), y.view(np.uint8)), f"{where}: {k} differs"
+        else:
+            assert a[k] == b[k], f"{where}: {k} differs"
+
+
+@pytest.mark.parametrize("hw_passes", [False, True])
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_branched_frames_match_one_stream(gpu, hw_passes, use_graph):
+    import torch
+    from frostbitten_hair_webgpu_amd import assets as A
+    from frostbitten_hair_webgpu_amd import passes as PS
+    wl = A.make_workload("c2", seed=77, strands=1200, width=480, height=320, points=16)
+    print(f"branches: hw_passes={hw_passes} graph={use_graph} {wl.strands}x{wl.points} {wl.width}x{wl.height}",
+          flush=True)
+    serial = PS.HairRenderer(wl, "cuda:0", shadow_size=512, hw_passes=hw_passes, branches=False)
+    inputs = {k: v.cpu().numpy() for k, v in serial.inputs.items()}
+    branched = PS.HairRenderer(wl, "cuda:0", shadow_size=512, inputs=inputs, hw_passes=hw_passes, branches=True)
+    for r in (serial, branched):
+        r.beforeFirstFrame()
+        r.frame()  # every call once before capture
+    torch.cuda.synchronize()
+    _same(_state(serial), _state(branched), "first frame")
+    if use_graph:
+        for r in (serial, branched):
+            r.capture_frames()
+    for f in range(3):
+        for r in (serial, branched):
+            if use_graph:
+                r.frame_graph()
+            else:
+                r.frame()
+        torch.cuda.synchronize()
+        _same(_state(serial), _state(branched), f"frame {f + 2}")
+    assert branched.native.scratch_bytes() > 0
+
+
+def _split(st):
+    sim = {k: v for k, v in st.items() if k in SIM_KEYS}
+    render = {k: v for k, v in st.items() if k not in SIM_KEYS}
+    return sim, render
+
+
+@pytest.mark.parametrize("hw_passes", [False, True])
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_pipelined_frames_match_plain(gpu, hw_passes, use_graph):
+    import torch
+    from frostbitten_hair_webgpu_amd import assets as A
+    from frostbitten_hair_webgpu_amd import passes as PS
+    wl = A.make_workload("c2", seed=78, strands=1000, width=400, height=288, points=24)
+    print(f"pipelined: hw_passes={hw_passes} graph={use_graph} {wl.strands}x{wl.points} {wl.width}x{wl.height}",
+          flush=True)
+    plain = PS.HairRenderer(wl, "cuda:0", shadow_size=512, hw_passes=hw_passes, branches=False)
+    inputs = {k: v.cpu().numpy() for k, v in plain.inputs.items()}
+    pipe = PS.HairRenderer(wl, "cuda:0", shadow_size=512, inputs=inputs, hw_passes=hw_passes, branches=True)
+    frames = 6
+    plain.beforeFirstFrame()
+    ref = []
+    for _ in range(frames):
+        plain.frame()
+        torch.cuda.synchronize()
+        ref.append(_split(_state(plain)))
+    pipe.beforeFirstFrame()
+    pipe.pipeline_begin()  # simulation step of frame 0
+    for f in range(1, frames):
+        if use_graph and f >= 2:
+            if f == 2:
+                pipe.capture_pipelined()
+            pipe.frame_pipelined_graph()
+        else:
+            pipe.frame_pipelined()  # frame 1 runs every call once before the capture
+        torch.cuda.synchronize()
+        sim, render = _split(_state(pipe))
+        _same(ref[f][0], sim, f"simulation state of frame {f}")
+        _same(ref[f - 1][1], render, f"render state of frame {f - 1}")
+    pipe.pipeline_end()
+    torch.cuda.synchronize()
+    _same(ref[frames - 1][1], _split(_state(pipe))[1], f"render state of frame {frames - 1}")
