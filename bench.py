@@ -106,15 +106,26 @@ def full_frame(wl, r0, steps: int, dev, branches: bool = True) -> dict:
     inputs = {k: v.cpu().numpy() for k, v in r0.inputs.items()}
     r = PS.HairRenderer(wl, dev, inputs=inputs, hw_passes=True, branches=branches)
     ring = torch.from_numpy(np.concatenate([r.sim_uniforms(f).view(np.uint8).reshape(-1)
-                                            for f in range(2 * steps + 2)])).to(dev).view(2 * steps + 2, -1)
+                                            for f in range(2 * steps + 4)])).to(dev).view(2 * steps + 4, -1)
     r.beforeFirstFrame()
     r.frame()
     torch.cuda.synchronize()
-    r.capture_frames()
+    pipelined = branches
+    if branches:  # pipelined frames, as the main measurement
+        r.simulationUniforms.copy_(ring[r.frameIdx])
+        r.pipeline_begin(upload_uniforms=False)
+        r.simulationUniforms.copy_(ring[r.frameIdx])
+        r.frame_pipelined(upload_uniforms=False)
+        r.capture_pipelined()
+    else:
+        r.capture_frames()
 
     def step():
         r.simulationUniforms.copy_(ring[r.frameIdx])
-        r.frame_graph(upload_uniforms=False)
+        if branches:
+            r.frame_pipelined_graph(upload_uniforms=False)
+        else:
+            r.frame_graph(upload_uniforms=False)
 
     step()
     torch.cuda.synchronize()
@@ -123,6 +134,9 @@ def full_frame(wl, r0, steps: int, dev, branches: bool = True) -> dict:
         step()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) / steps * 1e3
+    if branches:
+        r.pipeline_end()
+        branches = False  # the kernel timing below replays one graph per frame
     r.native.profiler_enable(64 * steps)
     r.capture_frames()
     torch.cuda.synchronize()
@@ -135,7 +149,7 @@ def full_frame(wl, r0, steps: int, dev, branches: bool = True) -> dict:
     extra = ("hair_raster", "hair_raster_big", "hair_resolve", "gtao")
     return {"value": round(1e3 / ms, 3), "unit": "frames/s", "ms_per_step": round(ms, 4), "steps": steps,
             "passes": "chain + ShadowMapPass hair draw (2048^2) + HwHairPass (depth + normals) + AoPass (GTAO, "
-                      "half resolution)",
+                      "half resolution)", "pipelined": pipelined,
             "kernel_us": {k: round(t / n * 1e3, 2) for k, (n, t) in sorted(kernels.items(), key=lambda kv: -kv[1][1])
                           if k in extra or k in ("copy_u32", "fill_u32")}}
 

@@ -925,26 +925,110 @@ struct BinSortSmall {
 };
 
 
-// One workgroup per listed range (more than 64 keys).
+// The same descending network over n2 = 256 * R keys (n2 <= kSortLds) held
+// in registers by the 256 threads of a workgroup: thread t holds elements
+// t * R .. t * R + R - 1. A compare-exchange at distance s runs
+//   s < R:        between two registers of one thread;
+//   s < 64 * R:   between lanes t and t ^ (s / R) of one wave (shuffle);
+//   otherwise:    through LDS (only distances n2 / 4 and n2 / 2: 3 stages),
+//                 rows padded to R + 1 words so that lanes hit distinct banks.
+// Each stage compares the same pairs in the same direction as bitonic_desc.
+template <uint32_t R>
+__device__ __forceinline__ void bitonic_regs_desc(uint32_t (&k)[R], uint32_t* lds) {
+  constexpr uint32_t n2 = 256u * R;
+  constexpr uint32_t P = R == 1 ? 1u : R + 1u;  // LDS row pitch
+  const uint32_t t = threadIdx.x;
+  for (uint32_t size = 2; size <= n2; size <<= 1) {
+    for (uint32_t s = size >> 1; s > 0; s >>= 1) {
+      if (s < R) {
+#pragma unroll
+        for (uint32_t sb = 1; sb < R; sb <<= 1) {
+          if (s != sb) continue;
+#pragma unroll
+          for (uint32_t r = 0; r < R; r++) {
+            if (r & sb) continue;
+            const bool desc = ((t * R + r) & size) == 0u;
+            const uint32_t hi = max(k[r], k[r | sb]), lo = min(k[r], k[r | sb]);
+            k[r] = desc ? hi : lo;
+            k[r | sb] = desc ? lo : hi;
+          }
+        }
+      } else {
+        const uint32_t q = s / R;  // distance in threads
+        const bool lower = (t & q) == 0u;
+        if (q < 64u) {
+#pragma unroll
+          for (uint32_t r = 0; r < R; r++) {
+            const uint32_t o = (uint32_t)__shfl_xor((int)k[r], (int)q);
+            const bool desc = ((t * R + r) & size) == 0u;
+            k[r] = (desc == lower) ? max(k[r], o) : min(k[r], o);
+          }
+        } else {
+          __syncthreads();  // earlier reads of lds are done
+#pragma unroll
+          for (uint32_t r = 0; r < R; r++) lds[t * P + r] = k[r];
+          __syncthreads();
+          const uint32_t pt = t ^ q;
+#pragma unroll
+          for (uint32_t r = 0; r < R; r++) {
+            const uint32_t o = lds[pt * P + r];
+            const bool desc = ((t * R + r) & size) == 0u;
+            k[r] = (desc == lower) ? max(k[r], o) : min(k[r], o);
+          }
+        }
+      }
+    }
+  }
+}
+
+template <uint32_t R>
+__device__ __forceinline__ void sort_range_regs(uint32_t* __restrict__ keys, uint32_t st, uint32_t cnt,
+                                                uint32_t* lds) {
+  uint32_t k[R];
+  const uint32_t t = threadIdx.x;
+#pragma unroll
+  for (uint32_t r = 0; r < R; r++) {
+    const uint32_t e = t * R + r;
+    k[r] = e < cnt ? keys[st + e] : 0u;
+  }
+  bitonic_regs_desc<R>(k, lds);
+#pragma unroll
+  for (uint32_t r = 0; r < R; r++) {
+    const uint32_t e = t * R + r;
+    if (e < cnt) keys[st + e] = k[r];
+  }
+}
+
+// One workgroup of 256 threads per listed range (more than 64 keys).
 __device__ __forceinline__ void bin_sort_large_body(const fh_tile_result* __restrict__ tiles,
                                                              const uint32_t* __restrict__ starts,
                                                              uint32_t capacity, uint32_t* __restrict__ keys,
                                                              const uint32_t* __restrict__ large,
                                                              uint32_t* __restrict__ big, uint32_t block, uint32_t blocks) {
-  __shared__ uint32_t sk[kSortLds];
+  __shared__ uint32_t sk[256u * 17u];
   const uint32_t count = large[0];
   for (uint32_t i = block; i < count; i += blocks) {
     const uint32_t tb = large[1u + i];
     const uint32_t st = min(starts[tb], capacity);
     const uint32_t cnt = min(tiles[tb].padding0, capacity - st);
-    uint32_t n2 = 2;
+    uint32_t n2 = 256;
     while (n2 < cnt) n2 <<= 1;
-    uint32_t* k = n2 > kSortLds ? big + 2ull * st : sk;
-    __syncthreads();
-    for (uint32_t e = threadIdx.x; e < n2; e += blockDim.x) k[e] = e < cnt ? keys[st + e] : 0u;
-    __syncthreads();
-    bitonic_desc(k, n2, threadIdx.x, blockDim.x);
-    for (uint32_t e = threadIdx.x; e < cnt; e += blockDim.x) keys[st + e] = k[e];
+    // every branch below is uniform over the workgroup (n2 is)
+    __syncthreads();  // the previous range's reads of sk are done
+    switch (n2) {
+      case 256u: sort_range_regs<1>(keys, st, cnt, sk); break;
+      case 512u: sort_range_regs<2>(keys, st, cnt, sk); break;
+      case 1024u: sort_range_regs<4>(keys, st, cnt, sk); break;
+      case 2048u: sort_range_regs<8>(keys, st, cnt, sk); break;
+      case 4096u: sort_range_regs<16>(keys, st, cnt, sk); break;
+      default: {
+        uint32_t* k = big + 2ull * st;
+        for (uint32_t e = threadIdx.x; e < n2; e += blockDim.x) k[e] = e < cnt ? keys[st + e] : 0u;
+        __syncthreads();
+        bitonic_desc(k, n2, threadIdx.x, blockDim.x);
+        for (uint32_t e = threadIdx.x; e < cnt; e += blockDim.x) keys[st + e] = k[e];
+      }
+    }
   }
 }
 

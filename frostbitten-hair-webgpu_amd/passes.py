@@ -61,6 +61,12 @@ class HairObject:
 
     def allocShading(self, shading_points: int, device) -> None:
         self.shading = _dev(A.initial_shading(self.strandsCount, shading_points), device)
+        # the buffer a pipelined frame's shading pass writes while the fine
+        # pass still reads `shading` (same layout)
+        self.shadingBack = self.shading.clone()
+
+    def swapShadingBuffers(self) -> None:
+        self.shading, self.shadingBack = self.shadingBack, self.shading
 
     def getRenderedStrandCount(self) -> int:
         """hairObject.ts:52-57"""
@@ -445,7 +451,7 @@ class HairRenderer:
         src = self.inputs["hdrScene"]
         self.native.call("fh_copy", N.ptr(self.hdrRenderTexture), N.ptr(src), src.numel() * src.element_size())
 
-    def drawHair(self, ctx: PassCtx) -> None:
+    def drawHair(self, ctx: PassCtx, shading_first: bool = False) -> None:
         """cmdDrawScene's hair part (renderer.ts:262-295) and the present pass
         (:205).
 
@@ -456,13 +462,26 @@ class HairRenderer:
         read, so they run on the branch beside the tile, sort and fine passes
         (which read the meshes' depth, not the hair's). The shading pass writes
         the shading points the fine pass reads, so it starts after the fine
-        pass, beside the combine and present passes."""
+        pass, beside the combine and present passes.
+
+        shading_first: the shading pass writes the hair's back shading buffer
+        on the branch before the tile pass starts, and the buffers swap when
+        the calls are issued (the pipelined frame); the fine pass still reads
+        the previous frame's shading points."""
         self.drawScene(ctx)
-        if self.hw_passes:
+        if self.hw_passes or shading_first:
             self._branch_begin()
-            self.shadowMapPass.cmdUpdateShadowMap(ctx, self.hair, self.inputs["shadowMap"])
-            self.hwHairPass.cmdDrawHair(ctx, self.hair, self.inputs["meshDepth"], self.inputs["meshNormals"])
-            self.aoPass.cmdCalcAo(ctx)
+            if self.hw_passes:
+                self.shadowMapPass.cmdUpdateShadowMap(ctx, self.hair, self.inputs["shadowMap"])
+                self.hwHairPass.cmdDrawHair(ctx, self.hair, self.inputs["meshDepth"], self.inputs["meshNormals"])
+                self.aoPass.cmdCalcAo(ctx)
+            if shading_first:
+                front = self.hair.shading
+                self.hair.shading = self.hair.shadingBack
+                try:
+                    self.shadingPass.cmdComputeShadingPoints(ctx, self.hair)
+                finally:
+                    self.hair.shading = front
             self._branch_end()
         self.tilesPass.cmdClearBeforeRender(ctx)
         self.finePass.cmdClearBeforeRender(ctx)
@@ -470,9 +489,12 @@ class HairRenderer:
         self.tilesPass.cmdDrawHairToTiles(ctx, self.hair)
         self.sortPass.cmdSortHairTiles(ctx)
         self.finePass.cmdRasterizeSlicesHair(ctx, self.hair, self.tilesPass)
-        self._branch_begin()
-        self.shadingPass.cmdComputeShadingPoints(ctx, self.hair)
-        self._branch_end()
+        if shading_first:
+            self.hair.swapShadingBuffers()
+        else:
+            self._branch_begin()
+            self.shadingPass.cmdComputeShadingPoints(ctx, self.hair)
+            self._branch_end()
         self.hairCombinePass.cmdCombineRasterResults(ctx)
         if self.hw_passes:  # the present pass's debug views read the hair depth, normals and AO
             self.join_branch()
@@ -519,7 +541,7 @@ class HairRenderer:
         stepped = self.hair.current
         self.hair.current = drawn
         try:
-            self.drawHair(ctx)
+            self.drawHair(ctx, shading_first=self.branches)
         finally:
             self.hair.current = stepped
 
@@ -541,13 +563,18 @@ class HairRenderer:
     def capture_pipelined(self) -> list:
         """capture_frames() for pipelined frames (two graphs, one per parity
         of the position pair); replayed by frame_pipelined_graph()."""
+        self._pshadingBase = self.hair.shading  # both pairs swap once per frame
         return self._capture(self._pipelined_calls, "_pgraphs", "_pgraphBase")
 
     def frame_pipelined_graph(self, upload_uniforms: bool = True) -> None:
         assert getattr(self, "_pipelined", False), "pipeline_begin() first"
         if upload_uniforms:
             self.upload_sim_uniforms()
-        self._pgraphs[self.hair.current ^ self._pgraphBase].launch()
+        k = self.hair.current ^ self._pgraphBase
+        if self.branches:
+            assert (self.hair.shading is self._pshadingBase) == (k == 0), "shading buffers out of step"
+            self.hair.swapShadingBuffers()
+        self._pgraphs[k].launch()
         self.hair.swapPositionBuffersAfterSimIntegration()
         self.frameIdx += 1
 

@@ -284,6 +284,51 @@ def test_fine_pass_image(gpu, oracle, tile, bins, slices, width, height):
     assert diff.max() <= 5e-4
 
 
+def test_fine_pass_dense_bins(gpu, oracle):
+    """6,000 strands on 160x120 with 32x32 tiles: (tile, bin) lists of up to
+    ~4,200 segments, so the bin sort takes every path (one wave up to 64
+    keys, registers up to 256, 512, 1024, 2048 and 4096 keys, global memory
+    beyond), and the image still matches the oracle's. The slice pool is
+    sized so that no bin overflows it."""
+    import torch
+    from frostbitten_hair_webgpu_amd import assets as A
+    from frostbitten_hair_webgpu_amd import passes as PS
+    wl = A.make_workload("small", strands=6000, width=160, height=120)
+    rc = wl.render
+    rc.tileSize, rc.tileDepthBins, rc.slicesPerPixel = 32, 4, 4
+    rc.avgSegmentsPerTile, rc.avgFragmentsPerSlice, rc.processorCount = 4096, 64, 32
+    r = PS.HairRenderer(wl, "cuda:0", shadow_size=256)
+    r.beforeFirstFrame()
+    ctx = r.ctx()
+    r.tilesPass.cmdClearBeforeRender(ctx)
+    r.finePass.cmdClearBeforeRender(ctx)
+    r.sortPass.cmdClearBeforeRender(ctx)
+    r.tilesPass.cmdDrawHairToTiles(ctx, r.hair)
+    r.sortPass.cmdSortHairTiles(ctx)
+    torch.cuda.synchronize()
+    tp = r.tilesPass
+    tiles = _np(tp.hairTilesBuffer).view(np.uint32).copy()
+    seg = _np(tp.hairTileSegmentsBuffer).view(np.uint32).copy()
+    tl = _np(r.sortPass.tileListBuffer).view(np.uint32).copy()
+    counts = tiles.reshape(-1, 4)[:, 3]
+    print(f"dense bins: largest list {counts.max()}, lists over 256: {(counts > 256).sum()}, "
+          f"over 1024: {(counts > 1024).sum()}, over 4096: {(counts > 4096).sum()}", flush=True)
+    assert counts.max() > 4096 and ((counts > 64) & (counts <= 256)).any() and ((counts > 1024) & (counts <= 4096)).any()
+    assert seg[0] <= tp.segmentsCapacity
+    shading = _np(r.hair.shading)
+    r.finePass.cmdRasterizeSlicesHair(ctx, r.hair, tp)
+    torch.cuda.synchronize()
+    got = _np(r.finePass.hairRasterizerResultsBuffer)
+    i = _tile_inputs(r)
+    exp = oracle.fine_raster(rc, wl.width, wl.height, i["render_u"], i["positions"], tiles, seg,
+                             tp.segmentsCapacity, i["depth"], shading, i["tangents"], tl, wl.strands,
+                             wl.points, canonical=True)
+    assert exp[4:].reshape(-1, 4)[:, 3].max() > 0.5
+    ge, ee = got[4:], exp[4:]
+    assert np.abs(to_u8(ge) - to_u8(ee)).max() <= 2
+    assert np.abs(ge - ee).max() <= 5e-4
+
+
 def test_rebuilt_slot_tags(gpu):
     """fh_tiles_rebuild_slot_keys (for lists another tile pass wrote) tags every
     slot with its list and a rank, and the fine pass draws the same image from
