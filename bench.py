@@ -247,7 +247,7 @@ def batched_vs_plain(n: int, steps: int, warmup: int, tune: bool, dev, log) -> d
     torch.cuda.synchronize()
     t3 = time.perf_counter()
     log(f"# batch: built in {t3 - t2:.1f} s, {br.launches_per_frame()} launches per batched frame")
-    total = 2 * (warmup + steps)
+    total = 4 * (warmup + steps) + 8
     ring = torch.from_numpy(np.stack([br.sim_uniforms(br.frameIdx + f) for f in range(total)])).to(dev)
 
     def plain_round(f):
@@ -276,36 +276,79 @@ def batched_vs_plain(n: int, steps: int, warmup: int, tune: bool, dev, log) -> d
 
     el_p, per_p = timed(plain_round, 0)
     el_b, per_b = timed(batch_frame, warmup + steps)
+    k_end = 2 * (warmup + steps)  # ring rows used so far, before the kernel timing region
     # per-kernel times of the batched frame: its launches issued directly
     # with HIP events around each (a third region, kept out of the timing above)
     graphs, br.graphs = br.graphs, None
     br.native.profiler_enable(64 * 4)
     br.native.profiler_read()
     for k in range(4):
-        batch_frame(total - 4 + k)
+        batch_frame(k_end + k)
     torch.cuda.synchronize()
     kernels = br.native.profiler_read()
     br.native.profiler_enable(0)
     br.graphs = graphs
+    f0 = k_end + 4
+
+    # Pipelined: each instance's simulation step of frame f beside its render
+    # passes of frame f-1. Plain: every instance replays its own pipelined
+    # graphs, one after another; batched: one pipelined graph per frame.
+    def plain_pipelined_round(f):
+        for i, r in enumerate(br.renderers):
+            r.simulationUniforms.copy_(ring[f, i])
+            r.frame_pipelined_graph(upload_uniforms=False)
+
+    def batch_pipelined_frame(f):
+        br.simPacked.copy_(ring[f])
+        br.frame_pipelined(upload_uniforms=False)
+
+    for i, r in enumerate(br.renderers):
+        r.simulationUniforms.copy_(ring[f0, i])
+        r.pipeline_begin(upload_uniforms=False)
+        r.simulationUniforms.copy_(ring[f0 + 1, i])
+        r.frame_pipelined(upload_uniforms=False)  # every call once before the capture
+        r.capture_pipelined()
+    el_pp, per_pp = timed(plain_pipelined_round, f0 + 2)
+    for r in br.renderers:
+        r.pipeline_end()
+    f1 = f0 + 2 + warmup + steps
+    br.build_pipelined()
+    br.simPacked.copy_(ring[f1])
+    br.pipeline_begin(upload_uniforms=False)
+    el_bp, per_bp = timed(batch_pipelined_frame, f1 + 1)
+    br.pipeline_end()
+    torch.cuda.synchronize()
     S = np.array([w.strands for w in wls])
     P = np.array([w.points for w in wls])
     plain_fps = n * steps / el_p
     batch_fps = n * steps / el_b
+    plain_pp_fps = n * steps / el_pp
+    batch_pp_fps = n * steps / el_bp
+
+    def ms(el, per):
+        return {"mean": round(el / steps * 1e3, 3), "p50": round(float(np.percentile(per, 50)), 3),
+                "p99": round(float(np.percentile(per, 99)), 3)}
+
+    best_plain = max(plain_fps, plain_pp_fps)
     return {
         "instances": n, "steps": steps,
-        "value": round(batch_fps, 2), "unit": "instance-frames/s",
-        "plain_value": round(plain_fps, 2),
-        "speedup_vs_plain": round(batch_fps / plain_fps, 3),
-        "batch_frame_ms": {"mean": round(el_b / steps * 1e3, 3), "p50": round(float(np.percentile(per_b, 50)), 3),
-                           "p99": round(float(np.percentile(per_b, 99)), 3)},
-        "plain_round_ms": {"mean": round(el_p / steps * 1e3, 3), "p50": round(float(np.percentile(per_p, 50)), 3),
-                           "p99": round(float(np.percentile(per_p, 99)), 3)},
+        "value": round(batch_pp_fps, 2), "unit": "instance-frames/s",
+        "form": "batched, pipelined (merged simulation step of frame f beside the merged render of frame f-1)",
+        "plain_value": round(best_plain, 2),
+        "speedup_vs_plain": round(batch_pp_fps / best_plain, 3),
+        "speedup_vs": "the faster plain form (each instance replaying its own graphs, one after another)",
+        "forms_instance_frames_per_s": {"plain": round(plain_fps, 2), "plain_pipelined": round(plain_pp_fps, 2),
+                                        "batched": round(batch_fps, 2), "batched_pipelined": round(batch_pp_fps, 2)},
+        "batch_frame_ms": ms(el_bp, per_bp),
+        "batch_unpipelined_frame_ms": ms(el_b, per_b),
+        "plain_round_ms": ms(el_p, per_p),
+        "plain_pipelined_round_ms": ms(el_pp, per_pp),
         "launches_per_batched_frame": br.launches_per_frame(),
         "kernel_us": {k: round(t / c * 1e3, 1) for k, (c, t) in sorted(kernels.items(), key=lambda kv: -kv[1][1])},
         "strands_total": int(S.sum()), "points_total": int((S * P).sum()),
         "pixels_total": int(sum(w.width * w.height for w in wls)),
-        "particle_steps_per_s": round(float((S * P).sum()) * steps / el_b, 1),
-        "segments_rasterized_per_s": round(float((S * (P - 1)).sum()) * steps / el_b, 1),
+        "particle_steps_per_s": round(float((S * P).sum()) * steps / el_bp, 1),
+        "segments_rasterized_per_s": round(float((S * (P - 1)).sum()) * steps / el_bp, 1),
         "autotune": AT.summary(choice) if tune else "off (tile 16, 8 bins x 4 slices)",
         "device_bytes": br.memory_bytes(),
         "setup_s": round(time.perf_counter() - t0, 1),

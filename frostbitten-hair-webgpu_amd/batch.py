@@ -9,6 +9,9 @@ one sequence of batched launches (fh_program_merge): every kernel of the chain
 runs once for all instances that launch it, each instance on its own block
 range with its own arguments, so each instance computes exactly what its plain
 frame computes. The merged frame is then captured as one HIP graph per parity.
+Pipelined, each instance's frame is recorded as a simulation part and a render
+part; the merged simulation step of frame f then runs on a second stream
+beside the merged render passes of frame f-1.
 
 The simulation uniforms of all instances live in one packed device array, so a
 frame's per-instance inputs (frameIdx, wind timer) arrive in one copy.
@@ -40,6 +43,40 @@ def record_frame(r: PS.HairRenderer) -> N.Program:
             return prog
         # the scratch grew while recording: undo the swap and record again
         r.hair.swapPositionBuffersAfterSimIntegration()
+    raise N.FhError("recording failed twice")
+
+
+def record_parts(r: PS.HairRenderer) -> tuple:
+    """The simulation step and the render passes of one frame of `r`,
+    recorded separately at the same parity of the position buffers: the
+    render part reads the positions that the simulation part reads as "now"
+    (frame f-1's) and the simulation part writes the other buffer, so the two
+    can run side by side (the pipelined frame of HairRenderer). The grid
+    accumulation records on the instance's branch scratch, so the two parts
+    share no temporaries."""
+    assert not r.hw_passes, "the hardware-raster passes share the branch scratch with the grid accumulation"
+    c = r.hair.current
+    for _ in range(2):
+        ctx = r.ctx()
+        r.hair.current = c
+        r.native.record_begin()
+        try:
+            r.simulateHair(ctx)
+        finally:
+            sim = r.native.record_end()
+        r.hair.current = c
+        r.native.record_begin()
+        try:
+            r.drawHair(ctx)
+        finally:
+            ren = r.native.record_end()
+        if sim is not None and ren is not None:
+            r.hair.current = 1 - c
+            return sim, ren
+        for prog in (sim, ren):  # a scratch grew while recording: record again
+            if prog is not None:
+                prog.close()
+    r.hair.current = c
     raise N.FhError("recording failed twice")
 
 
@@ -91,11 +128,7 @@ class BatchRenderer:
         graphs) captures the merged frames. The grid-stride kernels of each
         instance get its share (by point count) of `device_blocks`
         workgroups (default 8 per CU), at least 64."""
-        if device_blocks is None:
-            device_blocks = 8 * torch.cuda.get_device_properties(self.device).multi_processor_count
-        pts = np.array([r.wl.strands * r.wl.points for r in self.renderers], dtype=np.float64)
-        for r, w in zip(self.renderers, pts / pts.sum()):
-            r.native.set_persistent_blocks(max(64, int(math.ceil(device_blocks * w))))
+        self._share_blocks(device_blocks)
         recorded = [[], []]
         for r in self.renderers:
             for parity in range(2):
@@ -106,22 +139,105 @@ class BatchRenderer:
                 p.close()
         self.build_current = self.renderers[0].hair.current  # program 0 reads positions[build_current]
         if self.use_graph:
-            main = torch.cuda.current_stream(self.device)
-            side = torch.cuda.Stream(self.device)
-            side.wait_stream(main)
-            torch.cuda.synchronize(self.device)
-            self.native.set_stream(side)
-            try:
-                self.graphs = []
-                for p in range(2):
-                    self.native.capture_begin()
-                    try:
-                        self.programs[p].launch(self.native)
-                    finally:
-                        g = self.native.capture_end()
-                    self.graphs.append(g)
-            finally:
-                self.native.set_stream(main)
+            self.graphs = self._capture(lambda p: self.programs[p].launch(self.native))
+
+    def _share_blocks(self, device_blocks: int | None) -> None:
+        if device_blocks is None:
+            device_blocks = 8 * torch.cuda.get_device_properties(self.device).multi_processor_count
+        pts = np.array([r.wl.strands * r.wl.points for r in self.renderers], dtype=np.float64)
+        for r, w in zip(self.renderers, pts / pts.sum()):
+            r.native.set_persistent_blocks(max(64, int(math.ceil(device_blocks * w))))
+
+    def _capture(self, launch) -> list:
+        main = torch.cuda.current_stream(self.device)
+        side = torch.cuda.Stream(self.device)
+        side.wait_stream(main)
+        torch.cuda.synchronize(self.device)
+        self.native.set_stream(side)
+        graphs = []
+        try:
+            for p in range(2):
+                self.native.capture_begin()
+                try:
+                    launch(p)
+                finally:
+                    g = self.native.capture_end()
+                graphs.append(g)
+        finally:
+            self.native.set_stream(main)
+        return graphs
+
+    # Pipelined batched frames: the merged simulation step of frame f on the
+    # batch context's branch beside the merged render passes of frame f-1
+    # (HairRenderer's pipelined frame, for every instance at once).
+
+    def build_pipelined(self, device_blocks: int | None = None) -> None:
+        """Records both parities of every instance as a simulation part and a
+        render part, merges each part over the instances and (with graphs)
+        captures the pipelined frame per parity."""
+        self._share_blocks(device_blocks)
+        sims, rens = [[], []], [[], []]
+        for r in self.renderers:
+            for parity in range(2):
+                a, b = record_parts(r)
+                sims[parity].append(a)
+                rens[parity].append(b)
+        self.psim = [self.native.program_merge(sims[p]) for p in range(2)]
+        self.prender = [self.native.program_merge(rens[p]) for p in range(2)]
+        for lst in sims + rens:
+            for prog in lst:
+                prog.close()
+        self.pbuild_current = self.renderers[0].hair.current
+        self.pgraphs = None
+        if self.use_graph:
+            self.pgraphs = self._capture(self._pipelined_launch)
+        self.pipelined = False
+
+    def _pipelined_launch(self, p: int) -> None:
+        self.native.branch_begin()
+        self.psim[p].launch(self.native)
+        self.native.branch_end()
+        self.prender[p].launch(self.native)
+        self.native.branch_join()
+
+    def _pparity(self) -> int:
+        parity = self.renderers[0].hair.current ^ self.pbuild_current
+        assert all(r.hair.current ^ self.pbuild_current == parity for r in self.renderers), \
+            "instances must advance together"
+        return parity
+
+    def _advance(self) -> None:
+        for r in self.renderers:
+            r.hair.swapPositionBuffersAfterSimIntegration()
+            r.frameIdx += 1
+        self.frameIdx += 1
+
+    def pipeline_begin(self, upload_uniforms: bool = True) -> None:
+        """The prologue: the simulation step of every instance alone."""
+        if upload_uniforms:
+            self.upload_sim_uniforms()
+        self.psim[self._pparity()].launch(self.native)
+        self._advance()
+        self.pipelined = True
+
+    def frame_pipelined(self, upload_uniforms: bool = True) -> None:
+        """The simulation step of frame frameIdx beside the render passes of
+        frame frameIdx - 1, for every instance."""
+        assert self.pipelined, "pipeline_begin() first"
+        if upload_uniforms:
+            self.upload_sim_uniforms()
+        p = self._pparity()
+        if self.pgraphs is not None:
+            self.pgraphs[p].launch()
+        else:
+            self._pipelined_launch(p)
+        self._advance()
+
+    def pipeline_end(self) -> None:
+        """The epilogue: the render passes of the last simulated frame."""
+        assert self.pipelined, "pipeline_begin() first"
+        self.prender[self._pparity()].launch(self.native)
+        self.pipelined = False
 
     def launches_per_frame(self) -> int:
         return self.programs[0].launches()

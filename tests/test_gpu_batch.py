@@ -70,6 +70,58 @@ def test_batch_matches_plain(gpu, use_graph):
             assert a["img"][..., 3].max() > 0.0
 
 
+SIM_KEYS = ("now", "prev", "dv")
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_pipelined_matches_plain(gpu, use_graph):
+    """Pipelined batched frames (the merged simulation step of frame f beside
+    the merged render passes of frame f-1) leave, for every instance, the
+    simulation state of its plain frame f and the render state of its plain
+    frame f-1; the epilogue leaves the render state of the last frame."""
+    import torch
+    from frostbitten_hair_webgpu_amd import batch as B
+    from frostbitten_hair_webgpu_amd import passes as PS
+    wls = _workloads()
+    print(f"batch pipelined: graph={use_graph}, {len(wls)} instances", flush=True)
+    plain = [PS.HairRenderer(wl, "cuda:0", shadow_size=256) for wl in wls]
+    inputs = [{k: v.cpu().numpy() for k, v in r.inputs.items()} for r in plain]
+    br = B.BatchRenderer(wls, "cuda:0", inputs=inputs, use_graph=use_graph)
+    for r in plain + br.renderers:
+        r.beforeFirstFrame()
+    for r in br.renderers:  # every call once (scratch sizes), as a plain frame
+        r.frame()
+    frames = 4
+    ref = [[] for _ in plain]
+    for _ in range(frames + 1):
+        for k, r in enumerate(plain):
+            r.frame()
+            torch.cuda.synchronize()
+            ref[k].append(_state(r))
+    br.frameIdx = br.renderers[0].frameIdx
+    br.build_pipelined()
+    br.pipeline_begin()  # simulation step of plain frame 1 (plain frame 0 ran eagerly above)
+
+    def check(k, st, want, what):
+        for key in want:
+            same = want[key] == st[key] if key in ("lists", "drawn") else np.array_equal(want[key], st[key])
+            assert same, f"{what}, instance {k}: {key} differs"
+
+    for f in range(2, frames + 1):
+        br.frame_pipelined()
+        torch.cuda.synchronize()
+        for k, rb in enumerate(br.renderers):
+            st = _state(rb)
+            check(k, st, {x: ref[k][f][x] for x in SIM_KEYS}, f"simulation state of frame {f}")
+            check(k, st, {x: v for x, v in ref[k][f - 1].items() if x not in SIM_KEYS},
+                  f"render state of frame {f - 1}")
+    br.pipeline_end()
+    torch.cuda.synchronize()
+    for k, rb in enumerate(br.renderers):
+        st = _state(rb)
+        check(k, st, {x: v for x, v in ref[k][frames].items() if x not in SIM_KEYS}, f"render state of frame {frames}")
+
+
 def test_batch_merge_is_shorter(gpu):
     """The merged frame has one launch per kernel position, not one per
     instance: far fewer launches than the instances' frames together."""
