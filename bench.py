@@ -10,8 +10,9 @@ By default frames are pipelined and replayed as HIP graphs (one graph per
 parity of the tick-tock position buffers): each frame runs the simulation
 step of frame f on a second stream beside the render passes of frame f-1, so
 every frame still does one simulation step and one render of the chain.
-Prints one JSON line. `value` and the frame latencies come from a timed
-region of --steps frames with events only at frame boundaries; the latency
+Prints one JSON line. `value` comes from a timed region of --steps frames
+with no events between them, the frame latencies from a second region of as
+many frames with events at the frame boundaries; the latency
 from an image's simulation step to its present, and the rate of one graph per
 frame without pipelining (`unpipelined`), are reported beside them. `roofline` is for the kernel that takes the most time in a
 second timed region of as many frames, from its algorithmic bytes per launch
@@ -453,7 +454,7 @@ def main() -> None:
     t_setup = time.perf_counter()
     wl = A.make_workload(args.workload)
     r = PS.HairRenderer(wl, dev, branches=not args.no_branches)
-    total = args.warmup + 3 * args.steps + 4  # three timed regions, pipeline prologue and sizing frame
+    total = args.warmup + 5 * args.steps + 4  # five timed regions, pipeline prologue and sizing frame
     # per-frame simulation uniforms (frameIdx changes), uploaded once up front
     ring = torch.from_numpy(np.concatenate([r.sim_uniforms(f).view(np.uint8).reshape(-1) for f in range(total)]))
     ring = ring.to(dev).view(total, -1)
@@ -503,14 +504,21 @@ def main() -> None:
         if events:
             events[6].record()
 
-    def region(step):
+    def region(step, events=True):
         evs = [[torch.cuda.Event(enable_timing=True) for _ in range(7)] for _ in range(args.steps)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for k in range(args.steps):
-            step(evs[k])
+            step(evs[k] if events else None)
         torch.cuda.synchronize()
         return time.perf_counter() - t0, evs
+
+    def timed_pair(step):
+        """Wall time of `steps` frames with no events between them (value),
+        then as many frames with an event at every frame boundary (latency)."""
+        el, _ = region(step, events=False)
+        _, evs = region(step, events=True)
+        return el, evs
 
     use_graph = not args.no_graph
     pipelined = use_graph and not args.no_branches and not args.no_pipeline
@@ -530,13 +538,13 @@ def main() -> None:
         r.simulationUniforms.copy_(ring[r.frameIdx])
         r.frame_pipelined(upload_uniforms=False)  # every call once before capture
         r.capture_pipelined()
-        elapsed, evs = region(pipelined_frame)
+        elapsed, evs = timed_pair(pipelined_frame)
         r.pipeline_end()
         # sim-to-present: from the start of the frame that ran an image's
         # simulation step to the end of the frame that presented it
         s2p = np.array([evs[k - 1][0].elapsed_time(evs[k][6]) for k in range(1, args.steps)])
         r.capture_frames()
-        el_u, evs_u = region(graph_frame)
+        el_u, evs_u = timed_pair(graph_frame)
         per_u = np.array([evs_u[k][0].elapsed_time(evs_u[k][6]) for k in range(args.steps)])
         unpipelined = {"value": round(args.steps / el_u, 3), "ms_per_step": round(el_u / args.steps * 1e3, 4),
                        "latency_ms": {"p50": round(float(np.percentile(per_u, 50)), 4),
@@ -545,7 +553,9 @@ def main() -> None:
     else:
         if use_graph:
             r.capture_frames()
-        elapsed, evs = region(step)
+            elapsed, evs = timed_pair(step)
+        else:  # eager frames: the events also time the stages
+            elapsed, evs = region(step)
 
     # Timed region 2: as many further frames with HIP events around every
     # kernel on the library's stream (in graph replays, event-record nodes

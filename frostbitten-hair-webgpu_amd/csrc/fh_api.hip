@@ -44,9 +44,51 @@ struct FillMany {
     for (; i < n; i += stride) dst[i] = 0u;
   }
 };
+// The jobs of a clear batch in one launch: `per` consecutive blocks per job;
+// a job whose buffers are 16-byte aligned and whose length is a multiple of
+// four words moves 16 bytes per access.
+struct FillJobs {
+  static constexpr int kMax = 6;
+  struct Args {
+    uint32_t* dst[kMax];
+    const uint32_t* src[kMax];  // null: fill with value
+    uint64_t n[kMax];           // u32 words
+    uint32_t value[kMax];
+    uint32_t vec;               // bit j: job j moves uint4
+    uint32_t per;
+  };
+  static constexpr uint32_t kThreads = 256;
+  static constexpr const char* kName = "clear_batch";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) {
+    (void)blocks;
+    const uint32_t j = block / a.per;
+    const uint64_t stride = (uint64_t)a.per * kThreads;
+    uint64_t i = (uint64_t)(block - j * a.per) * kThreads + threadIdx.x;
+    const uint32_t v = a.value[j];
+    if ((a.vec >> j) & 1u) {
+      uint4* __restrict__ d = reinterpret_cast<uint4*>(a.dst[j]);
+      const uint4* __restrict__ s = reinterpret_cast<const uint4*>(a.src[j]);
+      const uint64_t n = a.n[j] / 4u;
+      if (s) for (; i < n; i += stride) d[i] = s[i];
+      else for (; i < n; i += stride) d[i] = make_uint4(v, v, v, v);
+    } else {
+      uint32_t* __restrict__ d = a.dst[j];
+      const uint32_t* __restrict__ s = a.src[j];
+      const uint64_t n = a.n[j];
+      if (s) for (; i < n; i += stride) d[i] = s[i];
+      else for (; i < n; i += stride) d[i] = v;
+    }
+  }
+};
 }  // namespace
 
 namespace fh {
+
+bool batch_fill(fh_ctx* ctx, void* dst, const void* src, uint64_t n_words, uint32_t value) {
+  if (!ctx->fill_batch) return false;
+  if (n_words) ctx->fill_jobs.push_back(FillJob{dst, src, n_words, value});
+  return true;
+}
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 
@@ -157,6 +199,10 @@ int add_timing_nodes(fh_graph* g, const std::vector<const char*>& names) {
 }
 
 int zero_many(fh_ctx* ctx, int count, void* const* dst, const uint64_t* n_words, const char* kernel_name) {
+  if (ctx->fill_batch) {
+    for (int i = 0; i < count; i++) batch_fill(ctx, dst[i], nullptr, n_words[i], 0u);
+    return FH_OK;
+  }
   FillMany::Args jobs{};
   uint64_t most = 0;
   int k = 0;
@@ -173,7 +219,7 @@ int zero_many(fh_ctx* ctx, int count, void* const* dst, const uint64_t* n_words,
 }
 
 int fill_u32(fh_ctx* ctx, void* dst, uint64_t n_words, uint32_t value) {
-  if (n_words == 0) return FH_OK;
+  if (n_words == 0 || batch_fill(ctx, dst, nullptr, n_words, value)) return FH_OK;
   const uint32_t blocks = (uint32_t)std::min<uint64_t>((n_words + 255) / 256, 4096);
   return launch<FillU32>(ctx, blocks, 0, FillU32::Args{(uint32_t*)dst, n_words, value});
 }
@@ -315,9 +361,46 @@ int fh_scratch_bytes(fh_ctx* ctx, uint64_t* bytes) {
   return FH_OK;
 }
 
+int fh_clear_batch_begin(fh_ctx* ctx) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(!ctx->fill_batch, "fh_clear_batch_begin: a clear batch is already open");
+  ctx->fill_batch = true;
+  ctx->fill_jobs.clear();
+  return FH_OK;
+}
+
+int fh_clear_batch_end(fh_ctx* ctx) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(ctx->fill_batch, "fh_clear_batch_end: no clear batch is open");
+  ctx->fill_batch = false;
+  std::vector<fh::FillJob> jobs;
+  jobs.swap(ctx->fill_jobs);
+  for (size_t k = 0; k < jobs.size(); k += FillJobs::kMax) {
+    FillJobs::Args a{};
+    const int m = (int)std::min<size_t>(FillJobs::kMax, jobs.size() - k);
+    uint64_t most = 0;
+    for (int j = 0; j < m; j++) {
+      const fh::FillJob& f = jobs[k + j];
+      a.dst[j] = static_cast<uint32_t*>(f.dst);
+      a.src[j] = static_cast<const uint32_t*>(f.src);
+      a.n[j] = f.n;
+      a.value[j] = f.value;
+      const bool vec = f.n % 4 == 0 && reinterpret_cast<uintptr_t>(f.dst) % 16 == 0 &&
+                       reinterpret_cast<uintptr_t>(f.src) % 16 == 0;
+      if (vec) a.vec |= 1u << j;
+      most = std::max(most, vec ? f.n / 4 : f.n);
+    }
+    a.per = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((most + 255) / 256, 2048));
+    int rc = fh::launch<FillJobs>(ctx, a.per * (uint32_t)m, 0, a);
+    if (rc) return rc;
+  }
+  return FH_OK;
+}
+
 int fh_branch_begin(fh_ctx* ctx) {
   FH_CHECK_CTX(ctx);
   FH_REQUIRE(!ctx->branch_open, "fh_branch_begin: a branch is already open");
+  FH_REQUIRE(!ctx->fill_batch, "fh_branch_begin: a clear batch is open");
   const bool live = !ctx->recording && ctx->prof_cap == 0;
   if (live) {
     if (!ctx->branch_stream) {
@@ -341,6 +424,7 @@ int fh_branch_begin(fh_ctx* ctx) {
 int fh_branch_end(fh_ctx* ctx) {
   FH_CHECK_CTX(ctx);
   FH_REQUIRE(ctx->branch_open, "fh_branch_end: no branch is open");
+  FH_REQUIRE(!ctx->fill_batch, "fh_branch_end: a clear batch is open");
   std::swap(ctx->scratch, ctx->branch_scratch);
   std::swap(ctx->scratch_bytes, ctx->branch_scratch_bytes);
   ctx->branch_open = false;
@@ -356,6 +440,7 @@ int fh_branch_end(fh_ctx* ctx) {
 int fh_branch_join(fh_ctx* ctx) {
   FH_CHECK_CTX(ctx);
   FH_REQUIRE(!ctx->branch_open, "fh_branch_join: the branch is still open");
+  FH_REQUIRE(!ctx->fill_batch, "fh_branch_join: a clear batch is open");
   if (!ctx->branch_pending) return FH_OK;
   ctx->branch_pending = false;
   FH_HIP(hipStreamWaitEvent(ctx->stream, ctx->branch_done, 0));
@@ -381,6 +466,7 @@ int fh_record_begin(fh_ctx* ctx) {
 int fh_record_end(fh_ctx* ctx, fh_program** out) {
   FH_CHECK_CTX(ctx);
   FH_REQUIRE(ctx->recording, "fh_record_end: not recording");
+  FH_REQUIRE(!ctx->fill_batch, "fh_record_end: a clear batch is open");
   FH_REQUIRE(out != nullptr, "fh_record_end: out is null");
   ctx->recording = false;
   *out = nullptr;
@@ -508,6 +594,7 @@ int fh_capture_end(fh_ctx* ctx, fh_graph** out) {
   FH_REQUIRE(out != nullptr, "fh_capture_end: out is null");
   *out = nullptr;
   // a capture must end on its origin stream with every branch joined into it
+  if (ctx->fill_batch) fh_clear_batch_end(ctx);
   const bool was_open = ctx->branch_open;
   if (was_open) fh_branch_end(ctx);
   fh_branch_join(ctx);
@@ -560,6 +647,7 @@ int fh_graph_destroy(fh_graph* graph) {
 
 int fh_sync(fh_ctx* ctx) {
   FH_CHECK_CTX(ctx);
+  FH_REQUIRE(!ctx->fill_batch, "fh_sync: a clear batch is open");
   FH_HIP(hipStreamSynchronize(ctx->stream));
   if (ctx->branch_stream) FH_HIP(hipStreamSynchronize(ctx->branch_stream));
   return FH_OK;

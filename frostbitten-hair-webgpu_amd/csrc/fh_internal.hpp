@@ -14,6 +14,14 @@ namespace fh {
 // owns blocks [dStarts[i], dStarts[i + 1]) and reads the i-th argument struct.
 using BatchLauncher = int (*)(hipStream_t s, const void* dArgs, const uint32_t* dStarts, uint32_t nInst,
                               uint32_t totalBlocks, uint32_t shmem);
+// One clear or copy collected by an open clear batch (fh_clear_batch_begin):
+// n u32 words of dst set to `value`, or copied from src when src is set.
+struct FillJob {
+  void* dst;
+  const void* src;
+  uint64_t n;
+  uint32_t value;
+};
 // One launch kept by a recording context instead of being issued.
 struct RecLaunch {
   const char* name;
@@ -49,6 +57,10 @@ struct fh_ctx {
   bool branch_open = false;     // between fh_branch_begin and fh_branch_end
   bool branch_live = false;     // the open branch runs on branch_stream
   bool branch_pending = false;  // branch work not yet joined into the main stream
+  // Between fh_clear_batch_begin and _end the clears and copies of the entry
+  // points are collected here and issued as one launch by _end.
+  bool fill_batch = false;
+  std::vector<fh::FillJob> fill_jobs;
   // The 256 thresholds of the rgba8unorm-srgb store (fh_present), uploaded
   // at creation so that no call allocates or copies them later.
   float* srgb_thresholds = nullptr;
@@ -107,6 +119,8 @@ const float* srgb_thresholds();
 constexpr int kMaxFillJobs = 4;
 int zero_many(fh_ctx* ctx, int count, void* const* dst, const uint64_t* n_words,
               const char* kernel_name = "fill_many");
+// While a clear batch is open: keeps the job and returns true.
+bool batch_fill(fh_ctx* ctx, void* dst, const void* src, uint64_t n_words, uint32_t value);
 void prof_end(fh_ctx* ctx, int k);
 
 inline uint32_t div_ceil(uint32_t a, uint32_t b) { return (a + b - 1u) / b; }

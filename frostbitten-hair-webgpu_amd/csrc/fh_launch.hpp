@@ -103,6 +103,9 @@ int launch(fh_ctx* ctx, uint32_t blocks, uint32_t shmem, const typename K::Args&
   // a batched launch copies each instance's Args into registers: large
   // arrays belong in device memory, with a pointer here
   static_assert(sizeof(typename K::Args) <= 256, "kernel argument struct too large for a batched launch");
+  // a clear batch holds clears and copies only: a kernel of another entry
+  // point would run before the fills it depends on
+  if (ctx->fill_batch) return fail(FH_ERR_INVALID_ARG, std::string(name) + ": only clears and copies inside a clear batch");
   if (blocks == 0) return FH_OK;
   if (ctx->recording) return record_launch(ctx, name, &batch_launch<K>, blocks, shmem, &a, sizeof(a));
   static uint32_t allowed = 0;

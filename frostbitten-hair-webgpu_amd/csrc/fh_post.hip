@@ -278,6 +278,7 @@ int fh_copy(fh_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
   FH_REQUIRE(bytes % 4 == 0, "fh_copy: bytes must be a multiple of 4");
   if (bytes == 0) return FH_OK;
   FH_REQUIRE(dst && src, "fh_copy: null buffer");
+  if (fh::batch_fill(ctx, dst, src, bytes / 4, 0u)) return FH_OK;
   if (bytes % 16 == 0 && reinterpret_cast<uintptr_t>(dst) % 16 == 0 && reinterpret_cast<uintptr_t>(src) % 16 == 0) {
     using K = CopyWords<uint4>;
     const uint64_t n = bytes / 16;

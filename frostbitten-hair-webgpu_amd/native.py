@@ -129,6 +129,8 @@ SIGNATURES = {
     "fh_branch_begin": (i32, [vp]),
     "fh_branch_end": (i32, [vp]),
     "fh_branch_join": (i32, [vp]),
+    "fh_clear_batch_begin": (i32, [vp]),
+    "fh_clear_batch_end": (i32, [vp]),
     "fh_capture_end": (i32, [vp, C.POINTER(vp)]),
     "fh_graph_launch": (i32, [vp, vp]),
     "fh_graph_destroy": (i32, [vp]),
@@ -230,6 +232,12 @@ class Context:
 
     def branch_join(self) -> None:
         check(self.lib.fh_branch_join(self.handle), "fh_branch_join")
+
+    def clear_batch_begin(self) -> None:
+        check(self.lib.fh_clear_batch_begin(self.handle), "fh_clear_batch_begin")
+
+    def clear_batch_end(self) -> None:
+        check(self.lib.fh_clear_batch_end(self.handle), "fh_clear_batch_end")
 
     def capture_begin(self) -> None:
         check(self.lib.fh_capture_begin(self.handle), "fh_capture_begin")

@@ -468,7 +468,16 @@ class HairRenderer:
         on the branch before the tile pass starts, and the buffers swap when
         the calls are issued (the pipelined frame); the fine pass still reads
         the previous frame's shading points."""
-        self.drawScene(ctx)
+        # the scene copy and the three passes' clears touch disjoint buffers:
+        # one launch for all of them
+        self.native.clear_batch_begin()
+        try:
+            self.drawScene(ctx)
+            self.tilesPass.cmdClearBeforeRender(ctx)
+            self.finePass.cmdClearBeforeRender(ctx)
+            self.sortPass.cmdClearBeforeRender(ctx)
+        finally:
+            self.native.clear_batch_end()
         if self.hw_passes or shading_first:
             self._branch_begin()
             if self.hw_passes:
@@ -483,9 +492,6 @@ class HairRenderer:
                 finally:
                     self.hair.shading = front
             self._branch_end()
-        self.tilesPass.cmdClearBeforeRender(ctx)
-        self.finePass.cmdClearBeforeRender(ctx)
-        self.sortPass.cmdClearBeforeRender(ctx)
         self.tilesPass.cmdDrawHairToTiles(ctx, self.hair)
         self.sortPass.cmdSortHairTiles(ctx)
         self.finePass.cmdRasterizeSlicesHair(ctx, self.hair, self.tilesPass)

@@ -602,6 +602,18 @@ int fh_branch_begin(fh_ctx* ctx);
 int fh_branch_end(fh_ctx* ctx);
 int fh_branch_join(fh_ctx* ctx);
 
+/* Clear batches: the clears and copies of the entry points called between
+ * fh_clear_batch_begin and fh_clear_batch_end (fh_tiles_clear,
+ * fh_tiles_sort_clear, fh_fine_clear, fh_clear, fh_copy) are issued together
+ * as one launch by fh_clear_batch_end, in place of one launch each; the
+ * per-pass clears of a frame (hairTilesPass.ts:48-52, hairFinePass.ts:50-55,
+ * hairTileSortPass.ts:63-68) touch disjoint buffers, so their order does not
+ * matter. Any other entry point called inside a batch fails
+ * (FH_ERR_INVALID_ARG), and so do fh_sync, fh_record_end and the branch
+ * calls; fh_capture_end ends an open batch. */
+int fh_clear_batch_begin(fh_ctx* ctx);
+int fh_clear_batch_end(fh_ctx* ctx);
+
 /* ------------------------------------------------------------------------ */
 /* Batching: many hair instances per launch                                 */
 /* ------------------------------------------------------------------------ */

@@ -98,3 +98,39 @@ def test_list_capacity_overflow(gpu, oracle):
     assert counts.sum() == cap
     B = wl.render.tileDepthBins
     assert np.array_equal(counts, t4[:, 3].reshape(-1, B).sum(axis=1))
+
+
+def test_clear_batch(gpu):
+    """A clear batch issues its clears and copies as one launch, with 16-byte
+    accesses where buffers and lengths allow and word accesses otherwise;
+    any other kernel inside a batch fails loudly and leaves the batch open."""
+    import torch
+    from frostbitten_hair_webgpu_amd import native as N
+    ctx = N.Context(0)
+    ctx.set_stream(torch.cuda.current_stream())
+    g = torch.Generator(device="cpu").manual_seed(5)
+    src = torch.randint(0, 2**31 - 1, (8192,), generator=g, dtype=torch.int32).cuda()
+    dst = torch.full((8192,), -1, dtype=torch.int32, device="cuda")
+    odd = torch.full((1001,), 7, dtype=torch.int32, device="cuda")
+    keep = torch.full((8,), 3, dtype=torch.int32, device="cuda")
+    ctx.clear_batch_begin()
+    ctx.call("fh_copy", N.ptr(dst), N.ptr(src), 4096 * 4)                # aligned: uint4
+    ctx.call("fh_copy", N.ptr(dst[4097:]), N.ptr(src[4097:]), 3001 * 4)  # misaligned: words
+    ctx.call("fh_clear", N.ptr(odd[1:]), 999 * 4)                        # misaligned, odd length
+    ctx.clear_batch_end()
+    torch.cuda.synchronize()
+    d, s_ = dst.cpu().numpy(), src.cpu().numpy()
+    assert np.array_equal(d[:4096], s_[:4096]) and d[4096] == -1
+    assert np.array_equal(d[4097:7098], s_[4097:7098]) and (d[7098:] == -1).all()
+    o = odd.cpu().numpy()
+    assert o[0] == 7 and (o[1:1000] == 0).all() and o[1000] == 7
+    from frostbitten_hair_webgpu_amd import assets as A
+    cfg = N.RenderCfg.from_config(A.make_workload("small", strands=64, width=64, height=64).render)
+    ctx.clear_batch_begin()  # the sort below is refused before it launches anything
+    with pytest.raises(N.FhError, match="only clears and copies"):
+        ctx.call("fh_tiles_sort", cfg, 64, 64, N.ptr(keep), N.ptr(keep), N.ptr(keep))
+    with pytest.raises(N.FhError, match="clear batch is open"):
+        ctx.sync()
+    ctx.clear_batch_end()
+    torch.cuda.synchronize()
+    assert (keep.cpu().numpy() == 3).all()
