@@ -64,17 +64,30 @@ struct FillJobs {
     const uint32_t j = block / a.per;
     const uint64_t stride = (uint64_t)a.per * kThreads;
     uint64_t i = (uint64_t)(block - j * a.per) * kThreads + threadIdx.x;
-    const uint32_t v = a.value[j];
+    // the job's fields by constant index: a run-time index into the argument
+    // struct would copy it to per-lane scratch in the batched launch
+    uint32_t* dst = nullptr;
+    const uint32_t* src = nullptr;
+    uint64_t n = 0;
+    uint32_t v = 0;
+#pragma unroll
+    for (int q = 0; q < kMax; q++) {
+      if ((uint32_t)q == j) {
+        dst = a.dst[q];
+        src = a.src[q];
+        n = a.n[q];
+        v = a.value[q];
+      }
+    }
     if ((a.vec >> j) & 1u) {
-      uint4* __restrict__ d = reinterpret_cast<uint4*>(a.dst[j]);
-      const uint4* __restrict__ s = reinterpret_cast<const uint4*>(a.src[j]);
-      const uint64_t n = a.n[j] / 4u;
-      if (s) for (; i < n; i += stride) d[i] = s[i];
-      else for (; i < n; i += stride) d[i] = make_uint4(v, v, v, v);
+      uint4* __restrict__ d = reinterpret_cast<uint4*>(dst);
+      const uint4* __restrict__ s = reinterpret_cast<const uint4*>(src);
+      const uint64_t n4 = n / 4u;
+      if (s) for (; i < n4; i += stride) d[i] = s[i];
+      else for (; i < n4; i += stride) d[i] = make_uint4(v, v, v, v);
     } else {
-      uint32_t* __restrict__ d = a.dst[j];
-      const uint32_t* __restrict__ s = a.src[j];
-      const uint64_t n = a.n[j];
+      uint32_t* __restrict__ d = dst;
+      const uint32_t* __restrict__ s = src;
       if (s) for (; i < n; i += stride) d[i] = s[i];
       else for (; i < n; i += stride) d[i] = v;
     }
@@ -602,7 +615,7 @@ int fh_capture_end(fh_ctx* ctx, fh_graph** out) {
   fh_graph* g = new fh_graph();
   hipError_t e = hipStreamEndCapture(ctx->stream, &g->graph);
   int rc = was_open ? fh::fail(FH_ERR_INVALID_ARG, "a branch was still open") : FH_OK;
-  if (e == hipSuccess && !ctx->cap_names.empty()) rc = fh::add_timing_nodes(g, ctx->cap_names);
+  if (rc == FH_OK && e == hipSuccess && !ctx->cap_names.empty()) rc = fh::add_timing_nodes(g, ctx->cap_names);
   ctx->cap_names.clear();
   if (e == hipSuccess && rc == FH_OK) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
   if (e != hipSuccess || rc != FH_OK) {

@@ -117,3 +117,44 @@ def test_pipelined_frames_match_plain(gpu, hw_passes, use_graph):
     pipe.pipeline_end()
     torch.cuda.synchronize()
     _same(ref[frames - 1][1], _split(_state(pipe))[1], f"render state of frame {frames - 1}")
+
+
+def test_branch_errors(gpu):
+    """Misuse of the branch calls fails with an error and leaves the context
+    usable: end without begin, nested begin, join or set_stream or profiling
+    while a branch is open or not joined, a capture ended with a branch open."""
+    import torch
+    from frostbitten_hair_webgpu_amd import native as N
+    ctx = N.Context(0)
+    ctx.set_stream(torch.cuda.current_stream())
+    with pytest.raises(N.FhError, match="no branch is open"):
+        ctx.branch_end()
+    ctx.branch_begin()
+    with pytest.raises(N.FhError, match="already open"):
+        ctx.branch_begin()
+    with pytest.raises(N.FhError, match="still open"):
+        ctx.branch_join()
+    with pytest.raises(N.FhError, match="branch is open"):
+        ctx.set_stream(torch.cuda.current_stream())
+    buf = torch.full((1024,), 9, dtype=torch.int32, device="cuda")
+    ctx.call("fh_clear", N.ptr(buf), 1024 * 4)  # runs on the branch
+    ctx.branch_end()
+    with pytest.raises(N.FhError, match="not joined"):
+        ctx.profiler_enable(8)
+    ctx.branch_join()
+    ctx.sync()
+    assert (buf.cpu().numpy() == 0).all()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    ctx.set_stream(side)
+    ctx.capture_begin()
+    ctx.branch_begin()
+    ctx.call("fh_clear", N.ptr(buf), 1024 * 4)
+    with pytest.raises(N.FhError, match="branch was still open"):
+        ctx.capture_end()
+    ctx.set_stream(torch.cuda.current_stream())
+    buf.fill_(5)
+    ctx.call("fh_clear", N.ptr(buf), 512 * 4)  # the context still works
+    ctx.sync()
+    b = buf.cpu().numpy()
+    assert (b[:512] == 0).all() and (b[512:] == 5).all()
