@@ -362,7 +362,9 @@ def served(n_req: int, rate: float, budget_gib: float, dev, log) -> dict:
     256-asset pool (strands log-uniform in [1k, 100k], 16-32 points, 720p /
     1080p / 4K) and U{1..120} frames; instance state stays in HBM between
     requests, within a `budget_gib` GiB budget. Continuous batching runs one
-    merged frame of every admitted request per tick; sequential serving runs
+    merged frame of every admitted request per tick (pipelined: the merged
+    simulation steps beside the merged renders of the previous steps);
+    sequential serving runs
     one request at a time through the same loop. Asset data (strands and the
     precomputed depth / AO / shadow inputs) is generated on the host first;
     instance creation on the device is part of serving."""
@@ -377,15 +379,20 @@ def served(n_req: int, rate: float, budget_gib: float, dev, log) -> dict:
         pool[a] = (wl, PS.make_inputs(wl, PS.workload_render_uniforms(wl), 1024))
     log(f"# serve: {len(pool)} assets generated in {time.perf_counter() - t0:.1f} s")
     out = {"requests": n_req, "rate_per_s": rate, "assets": len(pool), "budget_GiB": budget_gib}
-    for mode, max_active in (("continuous", 64), ("sequential", 1)):
-        srv = SV.HairServer(pool, int(budget_gib * (1 << 30)), max_active=max_active, device=dev)
+    for mode, max_active, pipelined in (("continuous_pipelined", 64, True), ("continuous", 64, False),
+                                        ("sequential", 1, False)):
+        srv = SV.HairServer(pool, int(budget_gib * (1 << 30)), max_active=max_active, device=dev,
+                            pipelined=pipelined)
         rs = srv.run([dataclasses.replace(q) for q in reqs])
         out[mode] = dict(SV.summarize(rs, srv.elapsed), **{k: v for k, v in srv.stats.items()},
                          host_s={k: round(v, 3) for k, v in srv.times.items()})
         srv.close()
         del srv
         log(f"# serve {mode}: {out[mode]['frames_per_s']} frames/s, p50 {out[mode]['latency_s']['p50']} s")
-    out["speedup_vs_sequential"] = round(out["continuous"]["frames_per_s"] / out["sequential"]["frames_per_s"], 3)
+    best = max(out["continuous"]["frames_per_s"], out["continuous_pipelined"]["frames_per_s"])
+    out["speedup_vs_sequential"] = round(best / out["sequential"]["frames_per_s"], 3)
+    out["speedup_vs"] = ("the faster continuous form (pipelined ticks: the merged simulation step beside the "
+                         "merged render of the previous step) over one request at a time")
     return out
 
 

@@ -18,6 +18,13 @@ bit for bit when the asset is requested again).
 
 With max_active = 1 the same loop serves one request at a time: the plain,
 sequential way to serve the same stream.
+
+Pipelined (`pipelined=True`), each instance's frame is recorded as a
+simulation part and a render part (batch.record_parts), and a tick runs the
+merged simulation parts of the requests with steps left on a second stream
+beside the merged render parts of the requests whose last simulated frame is
+not yet rendered: a request's first tick only simulates, its last only
+renders, and every image is the one its plain frames give.
 """
 from __future__ import annotations
 
@@ -43,7 +50,8 @@ class Request:
     arrival: float               # seconds after the start of the run
     admitted: float | None = None
     done: float | None = None
-    frames_done: int = 0
+    frames_done: int = 0         # frames rendered
+    sims_done: int = 0           # simulation steps run (ahead by one while pipelined)
     image: torch.Tensor | None = None   # rasterizer result after the last frame
     screen: torch.Tensor | None = None  # presented rgba8 sRGB image after the last frame
 
@@ -80,14 +88,21 @@ class _Instance:
     """A resident asset: its renderer, uniform slot and recorded frames."""
 
     def __init__(self, asset: int, r: PS.HairRenderer, slot: int, progs: list, base: int, nbytes: int):
+        # progs: per parity, the whole frame, or (simulation part, render part)
         self.asset, self.r, self.slot, self.progs, self.base, self.nbytes = asset, r, slot, progs, base, nbytes
         self.last_used = 0.0
 
     def parity(self) -> int:
         return self.r.hair.current ^ self.base
 
-    def program(self) -> N.Program:
-        return self.progs[self.parity()]
+    def program(self, part: str = "frame") -> N.Program:
+        p = self.progs[self.parity()]
+        return p if part == "frame" else p[0 if part == "sim" else 1]
+
+    def close(self) -> None:
+        for p in self.progs:
+            for q in (p if isinstance(p, tuple) else (p,)):
+                q.close()
 
 
 class HairServer:
@@ -96,8 +111,9 @@ class HairServer:
     instances and at most `max_active` requests per tick."""
 
     def __init__(self, pool: dict, budget_bytes: int, max_active: int = 64, device="cuda", keep_images=False,
-                 max_resident: int = 256):
+                 max_resident: int = 256, pipelined: bool = False):
         self.pool = pool
+        self.pipelined = pipelined
         self.budget = int(budget_bytes)
         self.max_active = max_active
         self.device = torch.device(device)
@@ -112,7 +128,7 @@ class HairServer:
         self.resident: OrderedDict = OrderedDict()   # asset -> _Instance, least recently used first
         self.saved = {}                                # asset -> host copy of an evicted instance's state
         self.sizes = {}                                # asset -> measured device bytes
-        self.merged: OrderedDict = OrderedDict()       # ((asset, parity), ...) -> merged Program
+        self.merged: OrderedDict = OrderedDict()       # (part, ((asset, parity), ...)) -> merged Program
         self.used = 0
         self.stats = dict(ticks=0, frames=0, evictions=0, materializations=0, merges=0)
         # host seconds spent per activity of the loop
@@ -129,9 +145,8 @@ class HairServer:
         torch.cuda.synchronize(self.device)
         self.saved[asset] = dict(positions=[p.cpu() for p in r.hair.positions], current=r.hair.current,
                                  frameIdx=r.frameIdx, dv=r.grid.densityVelocity.cpu(), shading=r.hair.shading.cpu())
-        for p in inst.progs:
-            p.close()
-        for key in [k for k in self.merged if any(a == asset for a, _ in k)]:
+        inst.close()
+        for key in [k for k in self.merged if any(a == asset for a, _ in k[1])]:
             self.merged.pop(key).close()
         self.free_slots.append(inst.slot)
         self.used -= inst.nbytes
@@ -168,7 +183,9 @@ class HairServer:
             r.grid.densityVelocity.copy_(st["dv"])
             r.hair.shading.copy_(st["shading"])
         base = r.hair.current
-        progs = [B.record_frame(r), B.record_frame(r)]  # parity of `base`, then the other one
+        # parity of `base`, then the other one
+        rec = B.record_parts if self.pipelined else B.record_frame
+        progs = [rec(r), rec(r)]
         nbytes = B.instance_bytes(r) + ctx.scratch_bytes()
         self.sizes[asset] = nbytes
         inst = _Instance(asset, r, slot, progs, base, nbytes)
@@ -179,13 +196,13 @@ class HairServer:
         return inst
 
     # -- one tick ----------------------------------------------------------
-    def _merged_for(self, insts: list) -> N.Program:
-        key = tuple(sorted((i.asset, i.parity()) for i in insts))
+    def _merged_for(self, insts: list, part: str = "frame") -> N.Program:
+        key = (part, tuple(sorted((i.asset, i.parity()) for i in insts)))
         prog = self.merged.get(key)
         if prog is None:
             t = time.perf_counter()
             ordered = sorted(insts, key=lambda i: i.asset)
-            prog = self.native.program_merge([i.program() for i in ordered])
+            prog = self.native.program_merge([i.program(part) for i in ordered])
             self.merged[key] = prog
             self.stats["merges"] += 1
             while len(self.merged) > 128:
@@ -196,16 +213,36 @@ class HairServer:
         return prog
 
     def _tick(self, active: dict) -> None:
+        """One frame of every active request; pipelined, the simulation step
+        of the requests with steps left beside the render of those with a
+        simulated frame not yet rendered."""
         t0 = time.perf_counter()
-        insts = [self.resident[a] for a in active]
-        for inst in insts:
+        if self.pipelined:
+            sim = [self.resident[a] for a, q in active.items() if q.sims_done < q.steps]
+            ren = [self.resident[a] for a, q in active.items() if q.sims_done > q.frames_done]
+        else:
+            sim = ren = [self.resident[a] for a in active]
+        for inst in sim:
             self.host_slots[inst.slot].numpy()[:] = inst.r.sim_uniforms(inst.r.frameIdx).view(np.uint8).reshape(-1)
         self.slots.copy_(self.host_slots, non_blocking=True)
         t1 = time.perf_counter()
-        prog = self._merged_for(insts)
+        if self.pipelined:
+            sp = self._merged_for(sim, "sim") if sim else None
+            rp = self._merged_for(ren, "render") if ren else None
+        else:
+            prog = self._merged_for(sim)
         t2 = time.perf_counter()
-        prog.launch(self.native)
-        for inst in insts:
+        if self.pipelined:
+            if sp is not None:
+                self.native.branch_begin()
+                sp.launch(self.native)
+                self.native.branch_end()
+            if rp is not None:
+                rp.launch(self.native)
+            self.native.branch_join()
+        else:
+            prog.launch(self.native)
+        for inst in sim:
             inst.r.hair.swapPositionBuffersAfterSimIntegration()
             inst.r.frameIdx += 1
         t3 = time.perf_counter()
@@ -215,7 +252,7 @@ class HairServer:
         self.times["launch"] += t3 - t2
         self.times["wait"] += t4 - t3
         self.stats["ticks"] += 1
-        self.stats["frames"] += len(insts)
+        self.stats["frames"] += len(ren)
 
     # -- the request loop --------------------------------------------------
     def run(self, requests: list, realtime: bool = True) -> list:
@@ -257,11 +294,15 @@ class HairServer:
                     time.sleep(max(0.0, min(pending[k].arrival - clock(), 0.002)))
                     continue
                 raise RuntimeError("requests that cannot be admitted within the memory budget")
+            rendering = {a for a, q in active.items() if not self.pipelined or q.sims_done > q.frames_done}
             self._tick(active)
             t = time.perf_counter() - t0
             for a in list(active):
                 q = active[a]
-                q.frames_done += 1
+                if q.sims_done < q.steps:
+                    q.sims_done += 1
+                if a in rendering:
+                    q.frames_done += 1
                 if q.frames_done == q.steps:
                     q.done = t
                     if self.keep_images:
@@ -278,8 +319,7 @@ class HairServer:
         self.merged.clear()
         for a in list(self.resident):
             inst = self.resident.pop(a)
-            for p in inst.progs:
-                p.close()
+            inst.close()
             inst.r.native.close()
 
 

@@ -29,19 +29,25 @@ def _requests():
     return [SV.Request(i, a, s, 0.0) for i, (a, s) in enumerate(order)]
 
 
-def _serve(pool, max_active, budget):
+def _serve(pool, max_active, budget, pipelined=False):
     from frostbitten_hair_webgpu_amd import serve as SV
-    srv = SV.HairServer(pool, budget, max_active=max_active, device="cuda:0", keep_images=True)
+    print(f"serve: max_active={max_active} budget={budget} pipelined={pipelined}", flush=True)
+    srv = SV.HairServer(pool, budget, max_active=max_active, device="cuda:0", keep_images=True,
+                        pipelined=pipelined)
     reqs = srv.run(_requests(), realtime=False)
     stats = dict(srv.stats)
     srv.close()
     return reqs, stats
 
 
-def test_serve_alone_equals_mix(gpu):
+@pytest.mark.parametrize("pipelined", [False, True])
+def test_serve_alone_equals_mix(gpu, pipelined):
+    """Each request's image is the same served alone (one request at a time,
+    plain ticks) and in the mix (continuous batching, plain or pipelined
+    ticks)."""
     pool = _pool()
     alone, st_alone = _serve(pool, 1, 1 << 40)
-    mix, st_mix = _serve(pool, 8, 1 << 40)
+    mix, st_mix = _serve(pool, 8, 1 << 40, pipelined)
     assert st_mix["ticks"] < st_alone["ticks"]  # requests overlapped
     assert st_alone["frames"] == st_mix["frames"] == sum(q.steps for q in _requests())
     for a, b in zip(alone, mix):
@@ -51,13 +57,14 @@ def test_serve_alone_equals_mix(gpu):
         assert a.image[..., 3].max() > 0
 
 
-def test_serve_eviction_restores_state(gpu):
+@pytest.mark.parametrize("pipelined", [False, True])
+def test_serve_eviction_restores_state(gpu, pipelined):
     from frostbitten_hair_webgpu_amd import serve as SV
     pool = _pool()
     alone, _ = _serve(pool, 1, 1 << 40)
     # room for about two instances at a time: later requests evict idle ones
     sizes = sorted(SV.estimate_bytes(wl) for wl, _ in pool.values())
-    tight, st = _serve(pool, 8, sizes[-1] + sizes[-2])
+    tight, st = _serve(pool, 8, sizes[-1] + sizes[-2], pipelined)
     assert st["evictions"] > 0
     for a, b in zip(alone, tight):
         assert np.array_equal(a.image.cpu().numpy(), b.image.cpu().numpy()), f"request {a.rid} differs"
