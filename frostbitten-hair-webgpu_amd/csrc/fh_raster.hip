@@ -875,28 +875,34 @@ __device__ __forceinline__ uint32_t wave_sort_desc(uint32_t v) {
   return v;
 }
 
-// One workgroup per 256 consecutive ranges: ranges of up to 64 keys are
-// sorted one per wave in registers; larger ones are appended to `large`
+// One workgroup per kSmallRanges consecutive ranges: ranges of up to 64 keys
+// are sorted one per wave in registers (each of the 4 waves takes every 4th
+// short range, so a workgroup's ranges sort 4 at a time and the grid has
+// enough waves to keep every SIMD busy); larger ones are appended to `large`
 // (count in large[0]) for bin_sort_large_kernel, so that the few heavy
 // ranges, which cluster on screen, spread over many workgroups.
+constexpr uint32_t kSmallRanges = 64;
+
 __device__ __forceinline__ void bin_sort_small_body(const fh_tile_result* __restrict__ tiles,
                                                              const uint32_t* __restrict__ starts, uint32_t nTB,
                                                              uint32_t capacity, uint32_t* __restrict__ keys,
                                                              uint32_t* __restrict__ large, uint32_t block, uint32_t blocks) {
-  __shared__ uint32_t smallList[256];
-  __shared__ uint32_t sStart[256], sCount[256];
+  __shared__ uint32_t smallList[kSmallRanges];
+  __shared__ uint32_t sStart[kSmallRanges], sCount[kSmallRanges];
   __shared__ uint32_t nSmall;
   const uint32_t tid = threadIdx.x;
   if (tid == 0) nSmall = 0;
   __syncthreads();
-  const uint32_t tb = block * 256u + tid;
+  const uint32_t tb = block * kSmallRanges + tid;
   uint32_t start = 0, n = 0;
-  if (tb < nTB) {
+  if (tid < kSmallRanges && tb < nTB) {
     start = min(starts[tb], capacity);
     n = min(tiles[tb].padding0, capacity - start);
   }
-  sStart[tid] = start;
-  sCount[tid] = n;
+  if (tid < kSmallRanges) {
+    sStart[tid] = start;
+    sCount[tid] = n;
+  }
   if (n >= 2u && n <= 64u) smallList[atomicAdd(&nSmall, 1u)] = tid;
   if (n > 64u) large[1u + atomicAdd(&large[0], 1u)] = tb;
   __syncthreads();
@@ -1736,7 +1742,8 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
                                 reinterpret_cast<const uint2*>(slot_tile_bin), ends, nTB, binSeg})))
     return rc;
   if ((rc = fill_u32(ctx, large, 1, 0u))) return rc;
-  if ((rc = launch<BinSortSmall>(ctx, fh::div_ceil(nTB, 256), 0, {tiles, ends, nTB, limit, binSeg, large}))) return rc;
+  if ((rc = launch<BinSortSmall>(ctx, fh::div_ceil(nTB, kSmallRanges), 0, {tiles, ends, nTB, limit, binSeg, large})))
+    return rc;
   if ((rc = launch<BinSortLarge>(ctx, std::min(nTB, fh::persistent_blocks(ctx)), 0,
                                  {tiles, ends, limit, binSeg, large, big})))
     return rc;
