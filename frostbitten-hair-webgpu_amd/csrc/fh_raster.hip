@@ -87,10 +87,12 @@ struct TileDepthBounds {
     const float* depth;
     uint32_t depthW, depthH, T, nTiles;
     float2* bounds;
+    uint32_t* zero;  // the tile pass's piece counter (set to 0 here)
   };
   static constexpr uint32_t kThreads = 256;
   static constexpr const char* kName = "tile_depth_bounds";
   static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) {
+    if (block == 0 && threadIdx.x == 0 && a.zero) *a.zero = 0u;
     __shared__ float smin[kThreads / 64], smax[kThreads / 64];
     __shared__ uint32_t snan[kThreads / 64];
     const TileGeom tg = tile_geom(mk2(a.u->viewport[0], a.u->viewport[1]), a.T);
@@ -759,10 +761,14 @@ struct BinScanLocal {
     uint32_t n;
     uint32_t* ends;
     uint32_t* blockSums;
+    uint32_t* zero;  // a counter the later sort kernels count into (set to 0 here)
   };
   static constexpr uint32_t kThreads = kScanBlock;
   static constexpr const char* kName = "bin_scan_local";
-  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { bin_scan_local_body(a.tiles, a.n, a.ends, a.blockSums, block, blocks); }
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) {
+    if (block == 0 && threadIdx.x == 0 && a.zero) *a.zero = 0u;
+    bin_scan_local_body(a.tiles, a.n, a.ends, a.blockSums, block, blocks);
+  }
 };
 
 
@@ -1638,10 +1644,10 @@ int fh_tiles_draw(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t
   a.pieces = reinterpret_cast<uint2*>(reinterpret_cast<char*>(ctx->scratch) + 256);
   a.depthBounds = reinterpret_cast<const float2*>(reinterpret_cast<char*>(ctx->scratch) + boundsOff);
   a.nBounds = nTiles;
-  if ((rc = fill_u32(ctx, a.pieceCount, 1, 0u))) return rc;
+  // the depth-bounds pass also zeroes the piece counter (nTiles >= 1: it always runs)
   if ((rc = launch<TileDepthBounds>(ctx, std::min(nTiles, fh::persistent_blocks(ctx)), 0,
                                     {uniforms, depth.data, depth.width, depth.height, cfg->tileSize, nTiles,
-                                     const_cast<float2*>(a.depthBounds)})))
+                                     const_cast<float2*>(a.depthBounds), a.pieceCount})))
     return rc;
   if ((rc = launch<TilesDraw>(ctx, fh::div_ceil((uint32_t)nSeg, 64), 0, a))) return rc;
   if ((rc = launch<TilesPieces>(ctx, std::min(fh::div_ceil((uint32_t)nSeg, 64), fh::persistent_blocks(ctx)), 0, a)))
@@ -1734,14 +1740,14 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
   const fh_list_element* data =
       reinterpret_cast<const fh_list_element*>(reinterpret_cast<const char*>(tile_segments) + 4);
 
-  if ((rc = launch<BinScanLocal>(ctx, scanBlocks, 0, {tiles, nTB, ends, sums}))) return rc;
+  // the scan also zeroes the long-range counter of the bin sort (large[0])
+  if ((rc = launch<BinScanLocal>(ctx, scanBlocks, 0, {tiles, nTB, ends, sums, large}))) return rc;
   if (scanBlocks > 1 && (rc = launch<BinScanAdd>(ctx, scanBlocks, 0, {nTB, ends, sums}))) return rc;
   const uint32_t scatterBlocks = std::max(1u, std::min(fh::div_ceil(limit, 256), fh::persistent_blocks(ctx)));
   if ((rc = launch<BinScatter>(ctx, scatterBlocks, 0,
                                {uniforms, reinterpret_cast<const uint32_t*>(tile_segments), data, limit,
                                 reinterpret_cast<const uint2*>(slot_tile_bin), ends, nTB, binSeg})))
     return rc;
-  if ((rc = fill_u32(ctx, large, 1, 0u))) return rc;
   if ((rc = launch<BinSortSmall>(ctx, fh::div_ceil(nTB, kSmallRanges), 0, {tiles, ends, nTB, limit, binSeg, large})))
     return rc;
   if ((rc = launch<BinSortLarge>(ctx, std::min(nTB, fh::persistent_blocks(ctx)), 0,
