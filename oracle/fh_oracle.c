@@ -35,8 +35,10 @@
 /* ======================================================================== */
 
 /* sampleSDFCollider — sdfCollider.ts:36-44 with the linear clamp-to-edge
- * sampler of sdfUtils.ts:33-42. Pinned: test_oracle_sim.py sdf texel-centre
- * and linear-field cases. */
+ * sampler of sdfUtils.ts:33-42. Pinned: test_oracle_golden.py
+ * test_sdf_roundtrip_and_texel_centres (texel centres, a midpoint, clamp to
+ * edge) and test_oracle_sim.py test_sdf_pushes_points_along_the_plane_normal
+ * (a linear field reproduced through forward differences). */
 static float sample_sdf(const float* sdf, uint32_t d, ov3 bmin, ov3 bmax, ov3 p) {
   ov3 t = sat3(div3(sub3(p, bmin), sub3(bmax, bmin)));
   float c[3] = {t.x * (float)d - 0.5f, t.y * (float)d - 0.5f, t.z * (float)d - 0.5f};
@@ -99,8 +101,9 @@ static float grid_decode(int32_t v) { return (float)v / 1000000.0f; }
 
 /* ======================================================================== */
 /* K1 — gridPreSimPass.wgsl.ts:38-159                                        */
-/* Pinned: test_oracle_sim.py (uniform density -> zero gradient, single      */
-/* spike -> +-axis pattern, wind inside/half/full regions of a sphere SDF). */
+/* Pinned: test_oracle_golden.py test_grid_pre_sim_known_answers (uniform   */
+/* density -> zero gradient, symmetric spike -> zero, ramp -> -step, lone    */
+/* corner -> (1,1,1), wind 0 / 0.5 / 1 regions around a sphere SDF).        */
 /* ======================================================================== */
 
 void or_grid_pre_sim(uint32_t d, const fh_sim_uniforms* u,
@@ -161,9 +164,14 @@ void or_grid_pre_sim(uint32_t d, const fh_sim_uniforms* u,
 /* K2 — hairSimIntegrationPass.wgsl.ts:79-273 with shaderImpl/               */
 /* integration.wgsl.ts:8-28, constraints.wgsl.ts:5-120,                      */
 /* collisions.wgsl.ts:4-61.                                                  */
-/* Pinned: test_oracle_sim.py (free flight 2*now-prev, length constraint to  */
-/* rest length, global shape stiffness 1 -> initial, sphere push to shell,   */
-/* brute-force double-precision restatement of a 4-point strand).            */
+/* Pinned: test_oracle_golden.py (test_integration_free_flight,            */
+/* test_length_constraint_restores_rest_length,                              */
+/* test_global_shape_stiffness_one_returns_initial,                          */
+/* test_sphere_collision_pushes_to_shell, and                                */
+/* test_integration_matches_double_precision_restatement: a float64 step of  */
+/* a 6-point strand) and test_oracle_sim.py (grid velocity / friction,       */
+/* density gradient, wind with strand jitter, forces together, SDF push-out  */
+/* along an analytic plane's normal, j > 2 only).                            */
 /* ======================================================================== */
 
 static ov4 ld4(const fh_vec4* a, size_t i) { return o4(a[i].x, a[i].y, a[i].z, a[i].w); }
@@ -346,9 +354,9 @@ void or_sim_integrate(uint32_t d, const fh_sim_uniforms* u,
 /* ======================================================================== */
 /* K3 — gridPostSimPass.wgsl.ts:45-72, _addGridDensityVelocity               */
 /* densityVelocityGrid.wgsl.ts:93-124; host clear gridPostSimPass.ts:36.     */
-/* Pinned: test_oracle_sim.py (point on a node -> one corner, weight         */
-/* length(1,1,1); cell centre -> 8 corners of length(.5,.5,.5); sum of       */
-/* density == sum over corners of i32(length(w)*1e6)).                       */
+/* Pinned: test_oracle_golden.py test_p2g_closed_forms (point on a node ->  */
+/* one corner of weight |(1,1,1)|, cell centre -> 8 corners of              */
+/* |(.5,.5,.5)|, density sum, velocity truncated toward zero).               */
 /* ======================================================================== */
 
 void or_grid_post_sim(uint32_t d, const fh_sim_uniforms* u, const fh_vec4* prev,
@@ -382,7 +390,8 @@ void or_grid_post_sim(uint32_t d, const fh_sim_uniforms* u, const fh_vec4* prev,
 /* ======================================================================== */
 /* Raster helpers — swRasterizeHair.wgsl.ts:39-203, shaderSnippets.wgls.ts   */
 /* :89-156, tileUtils.wgsl.ts:5-54.                                          */
-/* Pinned: test_oracle_raster.py against the reference's own fixture        */
+/* Pinned: test_oracle_golden.py test_raster_helpers_match_reference_      */
+/* snapshot, against the reference's own fixture                             */
 /* src/passes/swHair/__test__/swRasterizer.snapshot.bin (bit for bit).       */
 /* ======================================================================== */
 
@@ -512,9 +521,11 @@ void or_debug_raster(const fh_render_uniforms* u, const fh_vec4* positions,
 /* ======================================================================== */
 /* K4 — hairTilesPass.wgsl.ts:70-246 with hairTilesResultBuffer.ts:14-42,   */
 /* hairTileSegmentsBuffer.ts:22-29, segmentCountPerTileBuffer.ts:15-20.      */
-/* Pinned: test_oracle_raster.py (raster fixture above, brute-force tiny     */
-/* scenes: every listed segment's quad covers a pixel of its tile that      */
-/* passes the depth test, every such pair is listed, bin is the closest).   */
+/* Pinned: test_oracle_raster.py (depth bins with B = 8 at chosen view      */
+/* depths, clamping on both sides, the least bin of a slanted strand, two    */
+/* strands of one tile in their own bins, depth words of a flat strand, the  */
+/* depth test at texture row H - y) and test_oracle_golden.py                */
+/* test_fine_pass_closed_forms (list membership and depth words, one bin).   */
 /* ======================================================================== */
 
 static uint32_t div_ceil(uint32_t a, uint32_t b) { return (a + b - 1u) / b; }
@@ -596,9 +607,9 @@ void or_tiles_draw(const fh_render_cfg* cfg, uint32_t W, uint32_t H,
 /* ======================================================================== */
 /* K5 + K6 — hairTileSortPass.countTiles.wgsl.ts:63-97,                      */
 /* hairTileSortPass.sort.wgsl.ts:37-75.                                      */
-/* Pinned: test_oracle_raster.py with the inline vectors of                  */
-/* hairTileSortPass.test.ts:47-53 (histogram closed form :160-183 and        */
-/* non-increasing bucket order :122-127).                                    */
+/* Pinned: test_oracle_golden.py test_tile_sort_reference_vectors, the     */
+/* inline vectors of hairTileSortPass.test.ts:47-53 (histogram closed form   */
+/* :160-183 and non-increasing bucket order :122-127).                       */
 /* ======================================================================== */
 
 void or_tiles_sort(const fh_render_cfg* cfg, uint32_t W, uint32_t H,
@@ -633,9 +644,12 @@ void or_tiles_sort(const fh_render_cfg* cfg, uint32_t W, uint32_t H,
 /* K7 — hairFinePass.wgsl.ts:126-335, processHairSegment.wgsl.ts:5-70,       */
 /* reduceHairSlices.wgsl.ts:9-78, hairSlicesDataBuffer.ts:42-78,             */
 /* hairSliceHeadsBuffer.ts:136-180, hairRasterizerResultBuffer.ts:14-34.     */
-/* Pinned: test_oracle_raster.py (one segment over a pixel -> shading *      */
-/* alpha; two segments in different slices -> front-to-back "over"; a >=    */
-/* 0.999 cutoff; empty bin skipped; f16 round trip).                          */
+/* Pinned: test_oracle_golden.py test_fine_pass_closed_forms (two strands  */
+/* in different slices -> front-to-back "over" at coverage 0.5 and 0.25,     */
+/* the a >= 0.999 cutoff) and test_oracle_raster.py (slice-pool exhaustion:  */
+/* the strands that fit the pool are blended and the rest of the bin is      */
+/* dropped; the depth test at texture row H - y; canonical list order). The  */
+/* f16 pack is pinned with the combine's converters in test_oracle_post.py.  */
 /* canonical != 0 walks each (tile, bin) list in descending (strand, seg)   */
 /* order, i.e. the order this oracle's own tile pass produces.              */
 /* ======================================================================== */
@@ -799,9 +813,12 @@ void or_fine_raster(const fh_render_cfg* cfg, uint32_t W, uint32_t H,
 /* K8 — hairShadingPass.wgsl.ts:61-282, marschner.wgsl.ts:19-170,            */
 /* pbr.wgsl.ts:22-24, noise.wgsl.ts:45-51, sampleShadows.wgsl.ts:21-150,     */
 /* textureAo.wgsl.ts:7-12, linearDepth.wgsl.ts:9-27.                         */
-/* Pinned: test_oracle_shading.py against an independent double-precision    */
-/* evaluation of the Kajiya-Kay / Marschner (Karis16) formulas, and shadow   */
-/* strength 0 / AO strength 0 neutralising the texture inputs.               */
+/* Pinned: test_oracle_shading.py (Kajiya-Kay diffuse and the fake         */
+/* multiple scattering at chosen angles; each Marschner lobe against the     */
+/* Karis16 formulas in float64, with the R lobe's closed form at normal      */
+/* incidence; PCF / PCSS hit counts over a step shadow map and the bias;     */
+/* the AO texel and strength; attenuation from linearizeDepth; randomRGB,    */
+/* the root-to-tip colour mix, the luma jitter and the tip alpha).           */
 /* ======================================================================== */
 
 #define O_PI 3.14159265358979323846f
