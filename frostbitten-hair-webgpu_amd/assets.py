@@ -133,6 +133,21 @@ def _head_collider(kind: str) -> Collider:
     return Collider(spheres=[head], capsules=[face, neck])
 
 
+def segment_list_budget(S: int, P: int, W: int, H: int, tile: int) -> float:
+    """avgSegmentsPerTile (constants.ts:230; the segment list holds
+    1 + tiles * avgSegmentsPerTile entries, hairTileSegmentsBuffer.ts:81-87)
+    for a workload of these generators: the reference's 128 unless the
+    expected (segment, tile) pairs need more. The scalp-cap generators draw
+    about 1.5 pairs per segment with 16 px tiles at 1080p, proportionally
+    more with the resolution (1080p: 0.41 M pairs for 10k x 32; 4K: 9.3 M for
+    100k x 32) and at most twice as many with 8 px tiles; the budget keeps
+    twice that, so that no pair is dropped. The oracle and the HIP path read
+    the same value."""
+    tiles = -(-W // tile) * -(-H // tile)
+    expected = 1.5 * S * (P - 1) * max(W / 1920.0, H / 1080.0) * max(1.0, 16.0 / tile)
+    return float(max(128, math.ceil(2.0 * expected / tiles)))
+
+
 def make_workload(kind: str, seed: int | None = None, strands: int | None = None,
                   width: int | None = None, height: int | None = None, points: int | None = None,
                   collider: Collider | None = None, sdf: tuple | None = None) -> Workload:
@@ -172,6 +187,7 @@ def make_workload(kind: str, seed: int | None = None, strands: int | None = None
 
     render = L.RenderConfig(tileSize=sp["tile"], tileDepthBins=sp["bins"], slicesPerPixel=sp["slices"],
                             shadingPoints=sp["ps"])
+    render.avgSegmentsPerTile = segment_list_budget(S, P, W, H, sp["tile"])
     sim = L.SimConfig(deltaTime=1.0 / 60.0, gridDims=sp["grid"], collisionSphere=(0.0, 0.0, 0.0, HEAD_RADIUS))
     if kind == "small":
         sim.constraintIterations = 2

@@ -22,6 +22,7 @@ import math
 import numpy as np
 import torch
 
+from . import assets as A
 from . import batch as B
 from . import passes as PS
 
@@ -32,7 +33,8 @@ def apply(wl, cand) -> None:
     """Sets the candidate (tile, bins, slices) on a workload's render config,
     with one fine-pass processor per tile up to the configured count."""
     t, b, s = cand
-    rc = dataclasses.replace(wl.render, tileSize=t, tileDepthBins=b, slicesPerPixel=s)
+    rc = dataclasses.replace(wl.render, tileSize=t, tileDepthBins=b, slicesPerPixel=s,
+                             avgSegmentsPerTile=A.segment_list_budget(wl.strands, wl.points, wl.width, wl.height, t))
     rc.processorCount = B.default_processor_count(dataclasses.replace(rc, processorCount=512), wl.width,
                                                   wl.height)
     wl.render = rc

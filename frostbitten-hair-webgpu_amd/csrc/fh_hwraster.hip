@@ -281,6 +281,7 @@ constexpr uint32_t kRowSplit = 8;
 constexpr long long kWaveRaster = 1024;
 constexpr uint32_t kBandRows = 64;
 constexpr uint32_t kBigCap = 1u << 20;  // queued bands
+constexpr uint32_t kBigSentinel = 0xffffffffu;  // an unused band slot
 
 struct HairRaster {
   using Args = RasterArgs;
@@ -312,6 +313,10 @@ struct HairRaster {
           if (first + bands <= kBigCap) {
             for (uint32_t q = 0; q < bands; q++) a.big[first + q] = make_uint2(t, q);
           } else {
+            // a reservation that crosses the cap leaves its slots below the
+            // cap to a sentinel triangle id (skipped by HairRasterBig, which
+            // reads every slot below min(count, cap)); this wave draws it
+            for (uint32_t q = first; q < kBigCap; q++) a.big[q] = make_uint2(kBigSentinel, 0u);
             bigMask |= 1u << h;
           }
         }
@@ -343,6 +348,7 @@ struct HairRasterBig {
     const uint32_t count = min(*a.bigCount, kBigCap);
     for (uint32_t i = block; i < count; i += blocks) {
       const uint2 e = a.big[i];
+      if (e.x >= a.nTri) continue;  // kBigSentinel
       const SetupTri s = setup_tri(a, e.x);
       const PixBox b = pix_box(a, s);
       const long long y0 = b.y0 + (long long)e.y * kBandRows, y1 = min(b.y1, y0 + kBandRows - 1);

@@ -27,6 +27,15 @@
 #include "fh_oracle_math.h"
 
 #define FLOAT_EPSILON 1e-7f
+
+/* Host threads for the loops whose iterations are independent (strands,
+ * grid nodes, tiles). Results do not depend on it: every order-dependent
+ * step (list appends, i32 grid sums) is done in the sequential order or
+ * with associative integer additions. */
+static int g_threads = 1;
+
+void or_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+int or_get_threads(void) { return g_threads; }
 #define MAX_U32 0xffffffffu
 #define INVALID_PTR 0xffffffffu
 
@@ -115,6 +124,7 @@ void or_grid_pre_sim(uint32_t d, const fh_sim_uniforms* u,
   ov3 smax = o3(u->sdfBoundsMax[0], u->sdfBoundsMax[1], u->sdfBoundsMax[2]);
   ov3 windDir = o3(u->wind[0], u->wind[1], u->wind[2]);
   uint32_t total = d * d * d;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
   for (uint32_t id = 0; id < total; id++) {
     /* deconstructId :77-83 */
     uint32_t z = id / (d * d);
@@ -191,8 +201,11 @@ void or_sim_integrate(uint32_t d, const fh_sim_uniforms* u,
   ov4 sphere = o4(u->collisionSphere[0], u->collisionSphere[1], u->collisionSphere[2], u->collisionSphere[3]);
   float iters = (float)u->constraintIterations;
   const float SDF_GRAD_OFFSET = (float)(0.0033918858971446753 * 0.1);
-  ov4* pos = (ov4*)malloc(sizeof(ov4) * P);
 
+#pragma omp parallel num_threads(g_threads)
+  {
+  ov4* pos = (ov4*)malloc(sizeof(ov4) * P);
+#pragma omp for schedule(dynamic, 64)
   for (uint32_t s = 0; s < S; s++) {
     size_t base = (size_t)s * P;
     pos[0] = ld4(now, base);
@@ -349,6 +362,7 @@ void or_sim_integrate(uint32_t d, const fh_sim_uniforms* u,
     }
   }
   free(pos);
+  }
 }
 
 /* ======================================================================== */
@@ -359,13 +373,11 @@ void or_sim_integrate(uint32_t d, const fh_sim_uniforms* u,
 /* |(.5,.5,.5)|, density sum, velocity truncated toward zero).               */
 /* ======================================================================== */
 
-void or_grid_post_sim(uint32_t d, const fh_sim_uniforms* u, const fh_vec4* prev,
-                      const fh_vec4* now, fh_density_velocity* dv, uint32_t S,
-                      uint32_t P) {
-  ov3 gmin = o3(u->gridBoundsMin[0], u->gridBoundsMin[1], u->gridBoundsMin[2]);
-  ov3 gmax = o3(u->gridBoundsMax[0], u->gridBoundsMax[1], u->gridBoundsMax[2]);
-  memset(dv, 0, sizeof(fh_density_velocity) * (size_t)d * d * d);
-  for (uint32_t s = 0; s < S; s++)
+/* The contributions of strands [s0, s1) added into `g` (u32 wrap-around
+ * adds: the i32 sum of the reference's atomics, in any order). */
+static void p2g_range(uint32_t d, ov3 gmin, ov3 gmax, const fh_vec4* prev, const fh_vec4* now,
+                      fh_density_velocity* g, uint32_t s0, uint32_t s1, uint32_t P) {
+  for (uint32_t s = s0; s < s1; s++)
     for (uint32_t i = 0; i < P; i++) {
       size_t k = (size_t)s * P + i;
       ov3 pp = o3(prev[k].x, prev[k].y, prev[k].z);
@@ -378,13 +390,47 @@ void or_grid_post_sim(uint32_t d, const fh_sim_uniforms* u, const fh_vec4* prev,
             uint32_t idx = grid_idx(d, x, y, z);
             ov3 w = cell_weights(x, y, z, co.pin);
             ov3 cv = mul3(vel, w);
-            fh_density_velocity* c = &dv[idx];
+            fh_density_velocity* c = &g[idx];
             c->velocityX = (int32_t)((uint32_t)c->velocityX + (uint32_t)oi32(cv.x * 1000000.0f));
             c->velocityY = (int32_t)((uint32_t)c->velocityY + (uint32_t)oi32(cv.y * 1000000.0f));
             c->velocityZ = (int32_t)((uint32_t)c->velocityZ + (uint32_t)oi32(cv.z * 1000000.0f));
             c->density = (int32_t)((uint32_t)c->density + (uint32_t)oi32(len3(w) * 1000000.0f));
           }
     }
+}
+
+void or_grid_post_sim(uint32_t d, const fh_sim_uniforms* u, const fh_vec4* prev,
+                      const fh_vec4* now, fh_density_velocity* dv, uint32_t S,
+                      uint32_t P) {
+  ov3 gmin = o3(u->gridBoundsMin[0], u->gridBoundsMin[1], u->gridBoundsMin[2]);
+  ov3 gmax = o3(u->gridBoundsMax[0], u->gridBoundsMax[1], u->gridBoundsMax[2]);
+  const size_t nodes = (size_t)d * d * d;
+  memset(dv, 0, sizeof(fh_density_velocity) * nodes);
+  int T = g_threads;
+  if (T <= 1 || S < 2u * (uint32_t)T) {
+    p2g_range(d, gmin, gmax, prev, now, dv, 0, S, P);
+    return;
+  }
+  /* one partial grid per thread over a contiguous strand range, then the
+   * node-wise sum of the partials */
+  fh_density_velocity* part = (fh_density_velocity*)calloc((size_t)T * nodes, sizeof(fh_density_velocity));
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+  for (int t = 0; t < T; t++) {
+    uint32_t s0 = (uint32_t)((uint64_t)S * t / T), s1 = (uint32_t)((uint64_t)S * (t + 1) / T);
+    p2g_range(d, gmin, gmax, prev, now, part + (size_t)t * nodes, s0, s1, P);
+  }
+#pragma omp parallel for num_threads(T) schedule(static)
+  for (size_t n = 0; n < nodes; n++) {
+    uint32_t a = 0, b = 0, c = 0, e = 0;
+    for (int t = 0; t < T; t++) {
+      const fh_density_velocity* q = &part[(size_t)t * nodes + n];
+      a += (uint32_t)q->velocityX; b += (uint32_t)q->velocityY;
+      c += (uint32_t)q->velocityZ; e += (uint32_t)q->density;
+    }
+    dv[n].velocityX = (int32_t)a; dv[n].velocityY = (int32_t)b;
+    dv[n].velocityZ = (int32_t)c; dv[n].density = (int32_t)e;
+  }
+  free(part);
 }
 
 /* ======================================================================== */
@@ -530,6 +576,25 @@ void or_debug_raster(const fh_render_uniforms* u, const fh_vec4* positions,
 
 static uint32_t div_ceil(uint32_t a, uint32_t b) { return (a + b - 1u) / b; }
 
+/* One (segment, tile) pair that passed the raster: what the append needs. */
+typedef struct {
+  uint32_t s, g, tx, ty, dbin;
+  float dmin, dmax;
+} tile_hit;
+
+typedef struct {
+  tile_hit* v;
+  size_t n, cap;
+} hit_vec;
+
+static void hit_push(hit_vec* h, tile_hit x) {
+  if (h->n == h->cap) {
+    h->cap = h->cap ? 2 * h->cap : 256;
+    h->v = (tile_hit*)realloc(h->v, h->cap * sizeof(tile_hit));
+  }
+  h->v[h->n++] = x;
+}
+
 void or_tiles_draw(const fh_render_cfg* cfg, uint32_t W, uint32_t H,
                    const fh_render_uniforms* u, const fh_hair_data* hd,
                    const fh_vec4* positions, const fh_vec4* tangents,
@@ -550,58 +615,79 @@ void or_tiles_draw(const fh_render_cfg* cfg, uint32_t W, uint32_t H,
   ov4 bsc = mulmv(u->modelViewMat, o4(bs.x, bs.y, bs.z, 1.0f));
   ov2 dbVS = o2(bsc.z - bs.w, bsc.z + bs.w);
 
-  for (uint32_t s = 0; s < S; s++)
-    for (uint32_t g = 0; g + 1 < P; g++) {
-      proj_seg ps = project_segment(u, positions, tangents, P, s, g, vp, u->fiberRadius);
-      ov2 bmax = max2(max2(ps.v00, ps.v01), max2(ps.v10, ps.v11));
-      ov2 bmin = min2(min2(ps.v00, ps.v01), min2(ps.v10, ps.v11));
-      bmax = min2(o2(ceilf(bmax.x), ceilf(bmax.y)), vp);
-      bmin = max2(o2(floorf(bmin.x), floorf(bmin.y)), o2(0.0f, 0.0f));
-      uint32_t tminx = ou32(bmin.x) / T, tminy = ou32(bmin.y) / T;
-      uint32_t tmaxx = ou32(bmax.x) / T, tmaxy = ou32(bmax.y) / T;
-      uint32_t tsx = (tmaxx - tminx) + 1u, tsy = (tmaxy - tminy) + 1u;
-      if (tsx * tsy > cfg->invalidTilesPerSegmentThreshold) continue;
-      for (uint32_t ty = tminy; ty <= tmaxy; ty++)
-        for (uint32_t tx = tminx; tx <= tmaxx; tx++) {
-          /* processTile :139-238; getTileBoundsPx tileUtils.wgsl.ts:44-54 */
-          uint32_t x0 = oclampu(tx * T, 0, vpW - 1), y0 = oclampu(ty * T, 0, vpH - 1);
-          uint32_t x1 = oclampu((tx + 1) * T, 0, vpW - 1), y1 = oclampu((ty + 1) * T, 0, vpH - 1);
-          float dmin = 999.0f, dmax = -999.0f;
-          uint32_t dbin = B;
-          for (uint32_t y = y0; y < y1; y++)
-            for (uint32_t x = x0; x < x1; x++) {
-              ov2 p = o2((float)x, (float)y);
-              if (!inside_quad(&ps, p)) continue;
-              ov2 w = interp_quad(&ps, p);
-              float hdp = interp_f32(w, ps.depths);
-              float db = tex_load(depth, vpW, vpH, (int32_t)x, (int32_t)(vpH - y));
-              if (hdp > db) continue;
-              ov3 vs = oproject(u->projMatrixInv, o4(p.x, p.y, hdp, 1.0f));
-              uint32_t bin = (B - 1u) - depth_bin(B, dbVS, vs.z);
-              dmin = omin(dmin, hdp);
-              dmax = omax(dmax, hdp);
-              dbin = bin < dbin ? bin : dbin;
-            }
-          if (dmin > 1.0f) continue;
-          uint32_t nextPtr = (*counter)++;
-          if (nextPtr < limit) {
-            uint32_t tileIdx = ty * tcx * B + tx * B + dbin;
-            fh_tile_result* t = &tiles[tileIdx];
-            uint32_t dMaxU = ou32(dmax * 4294967296.0f);
-            uint32_t dMinU = ou32((1.0f - dmin) * 4294967296.0f);
-            if (t->maxDepth < dMaxU) t->maxDepth = dMaxU;
-            if (t->minDepth < dMinU) t->minDepth = dMinU;
-            uint32_t last = t->tileSegmentPtr;
-            t->tileSegmentPtr = nextPtr + 1u;
-            t->padding0 += 1u;
-            uint32_t prevPtr = last == 0u ? INVALID_PTR : last - 1u;
-            data[nextPtr].strandAndSegmentIdxs = (g << 24) | s;
-            data[nextPtr].next = prevPtr;
-            if (slotKey) slotKey[nextPtr] = tileIdx;
-            countPerTile[ty * tcx + tx] += 1u;
+  /* The raster work of each (segment, tile) pair is independent: it runs
+   * over chunks of strands (in parallel when threads > 1), collecting the
+   * pairs that pass in (strand, segment, tile row, tile column) order. The
+   * appends then run in that order, one after another, which is the
+   * schedule a sequential run of the reference's threads produces. */
+  const uint32_t CHUNK = 64;
+  const uint32_t nChunks = div_ceil(S, CHUNK);
+  hit_vec* hits = (hit_vec*)calloc(nChunks ? nChunks : 1, sizeof(hit_vec));
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 1)
+  for (uint32_t c = 0; c < nChunks; c++) {
+    uint32_t s1 = (c + 1) * CHUNK < S ? (c + 1) * CHUNK : S;
+    for (uint32_t s = c * CHUNK; s < s1; s++)
+      for (uint32_t g = 0; g + 1 < P; g++) {
+        proj_seg ps = project_segment(u, positions, tangents, P, s, g, vp, u->fiberRadius);
+        ov2 bmax = max2(max2(ps.v00, ps.v01), max2(ps.v10, ps.v11));
+        ov2 bmin = min2(min2(ps.v00, ps.v01), min2(ps.v10, ps.v11));
+        bmax = min2(o2(ceilf(bmax.x), ceilf(bmax.y)), vp);
+        bmin = max2(o2(floorf(bmin.x), floorf(bmin.y)), o2(0.0f, 0.0f));
+        uint32_t tminx = ou32(bmin.x) / T, tminy = ou32(bmin.y) / T;
+        uint32_t tmaxx = ou32(bmax.x) / T, tmaxy = ou32(bmax.y) / T;
+        uint32_t tsx = (tmaxx - tminx) + 1u, tsy = (tmaxy - tminy) + 1u;
+        if (tsx * tsy > cfg->invalidTilesPerSegmentThreshold) continue;
+        for (uint32_t ty = tminy; ty <= tmaxy; ty++)
+          for (uint32_t tx = tminx; tx <= tmaxx; tx++) {
+            /* processTile :139-238; getTileBoundsPx tileUtils.wgsl.ts:44-54 */
+            uint32_t x0 = oclampu(tx * T, 0, vpW - 1), y0 = oclampu(ty * T, 0, vpH - 1);
+            uint32_t x1 = oclampu((tx + 1) * T, 0, vpW - 1), y1 = oclampu((ty + 1) * T, 0, vpH - 1);
+            float dmin = 999.0f, dmax = -999.0f;
+            uint32_t dbin = B;
+            for (uint32_t y = y0; y < y1; y++)
+              for (uint32_t x = x0; x < x1; x++) {
+                ov2 p = o2((float)x, (float)y);
+                if (!inside_quad(&ps, p)) continue;
+                ov2 w = interp_quad(&ps, p);
+                float hdp = interp_f32(w, ps.depths);
+                float db = tex_load(depth, vpW, vpH, (int32_t)x, (int32_t)(vpH - y));
+                if (hdp > db) continue;
+                ov3 vs = oproject(u->projMatrixInv, o4(p.x, p.y, hdp, 1.0f));
+                uint32_t bin = (B - 1u) - depth_bin(B, dbVS, vs.z);
+                dmin = omin(dmin, hdp);
+                dmax = omax(dmax, hdp);
+                dbin = bin < dbin ? bin : dbin;
+              }
+            if (dmin > 1.0f) continue;
+            tile_hit h = {s, g, tx, ty, dbin, dmin, dmax};
+            hit_push(&hits[c], h);
           }
-        }
+      }
+  }
+  for (uint32_t c = 0; c < nChunks; c++) {
+    for (size_t k = 0; k < hits[c].n; k++) {
+      const tile_hit* h = &hits[c].v[k];
+      uint32_t nextPtr = (*counter)++;
+      if (nextPtr < limit) {
+        uint32_t tileIdx = h->ty * tcx * B + h->tx * B + h->dbin;
+        fh_tile_result* t = &tiles[tileIdx];
+        uint32_t dMaxU = ou32(h->dmax * 4294967296.0f);
+        uint32_t dMinU = ou32((1.0f - h->dmin) * 4294967296.0f);
+        if (t->maxDepth < dMaxU) t->maxDepth = dMaxU;
+        if (t->minDepth < dMinU) t->minDepth = dMinU;
+        uint32_t last = t->tileSegmentPtr;
+        t->tileSegmentPtr = nextPtr + 1u;
+        t->padding0 += 1u;
+        uint32_t prevPtr = last == 0u ? INVALID_PTR : last - 1u;
+        data[nextPtr].strandAndSegmentIdxs = (h->g << 24) | h->s;
+        data[nextPtr].next = prevPtr;
+        if (slotKey) slotKey[nextPtr] = tileIdx;
+        countPerTile[h->ty * tcx + h->tx] += 1u;
+      }
     }
+    free(hits[c].v);
+  }
+  free(hits);
 }
 
 /* ======================================================================== */
@@ -685,16 +771,21 @@ void or_fine_raster(const fh_render_cfg* cfg, uint32_t W, uint32_t H,
   uint32_t dbgMax = dispMode == 4u ? ((u->displayMode >> 8) & 0xffffu) : 0u;
   int isDbg = dbgMax != 0u;
   const uint32_t NPX = T * T;
+  uint32_t drawnTiles = tileList[0];
+  /* Tiles are independent (each writes only its own pixels, and its
+   * processor state is reset per bin and per tile): they run in parallel
+   * when threads > 1, each thread with its own processor state. */
+#pragma omp parallel num_threads(g_threads)
+  {
   uint32_t* heads = (uint32_t*)malloc(sizeof(uint32_t) * NPX * NS);
   uint32_t (*pool)[4] = (uint32_t(*)[4])malloc(sizeof(uint32_t) * 4 * PER_PROC);
   int* done = (int*)malloc(sizeof(int) * NPX);
-  uint32_t* listBuf = (uint32_t*)malloc(sizeof(uint32_t) * (maxProcessed + 1u));
-  uint32_t* listPtr = (uint32_t*)malloc(sizeof(uint32_t) * (maxProcessed + 1u));
+  uint32_t listCap = 1024;
+  uint32_t* listBuf = (uint32_t*)malloc(sizeof(uint32_t) * listCap);
   for (uint32_t i = 0; i < NPX * NS; i++) heads[i] = INVALID_PTR;
 
-  uint32_t drawnTiles = tileList[0];
+#pragma omp for schedule(dynamic, 1)
   for (uint32_t qi = 0; qi < drawnTiles; qi++) {
-    *queue += 1u;
     uint32_t tileIdx = tileList[1 + qi];
     uint32_t row = tileIdx / tcx;
     uint32_t tx = tileIdx - tcx * row, ty = row;
@@ -713,8 +804,11 @@ void or_fine_raster(const fh_render_cfg* cfg, uint32_t W, uint32_t H,
         uint32_t n = 0;
         uint32_t ptr = tr->tileSegmentPtr == 0u ? INVALID_PTR : tr->tileSegmentPtr - 1u;
         while (n < maxProcessed && ptr < maxDrawn && ptr != INVALID_PTR && ptr < capacity) {
+          if (n == listCap) {
+            listCap *= 2;
+            listBuf = (uint32_t*)realloc(listBuf, sizeof(uint32_t) * listCap);
+          }
           listBuf[n] = data[ptr].strandAndSegmentIdxs;
-          listPtr[n] = ptr;
           n++;
           ptr = data[ptr].next;
         }
@@ -760,7 +854,6 @@ void or_fine_raster(const fh_render_cfg* cfg, uint32_t W, uint32_t H,
           }
           if (!(sliceOff < PER_PROC)) break;
         }
-        (void)listPtr;
         /* after the segment loop the reference sets the offset to the pool
          * size when the list ended, so the reduce below always runs for a
          * visited bin (hairFinePass.wgsl.ts:252-275) */
@@ -806,7 +899,10 @@ void or_fine_raster(const fh_render_cfg* cfg, uint32_t W, uint32_t H,
       if (all) break;
     }
   }
-  free(heads); free(pool); free(done); free(listBuf); free(listPtr);
+  free(heads); free(pool); free(done); free(listBuf);
+  }
+  /* getNextTileIdx: every processor took one queue index per tile */
+  *queue += drawnTiles;
 }
 
 /* ======================================================================== */
@@ -930,6 +1026,7 @@ void or_shading(const fh_render_cfg* cfg, const fh_render_uniforms* u,
   ov2 vp = o2(u->viewport[0], u->viewport[1]);
   ov3 cam = o3(u->cameraPosition[0], u->cameraPosition[1], u->cameraPosition[2]);
   const fh_hair_material* m = &u->hairMaterial;
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 64)
   for (uint32_t s = 0; s < S; s++)
     for (uint32_t id = 0; id < PS; id++) {
       float t = (float)id / (float)PS;

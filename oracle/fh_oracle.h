@@ -13,6 +13,10 @@
 extern "C" {
 #endif
 
+/* Host threads for the independent loops (default 1); results do not depend on it. */
+void or_set_threads(int n);
+int or_get_threads(void);
+
 float or_sample_sdf(const float* sdf, uint32_t d, const float* bmin,
                     const float* bmax, const float* p);
 

@@ -49,6 +49,16 @@ def _p(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def set_threads(n: int) -> None:
+    """Host threads of the oracle's independent loops (strands, grid nodes,
+    tiles); every result is the same for any count."""
+    lib().or_set_threads(C.c_int(int(n)))
+
+
+def get_threads() -> int:
+    return int(lib().or_get_threads())
+
+
 def sample_sdf(sdf, d, bmin, bmax, p) -> float:
     f = lambda x: np.ascontiguousarray(x, dtype=np.float32)
     return lib().or_sample_sdf(_p(f(sdf)), C.c_uint32(d), _p(f(bmin)), _p(f(bmax)), _p(f(p)))
