@@ -116,6 +116,25 @@ int check_launch(const char* what) {
   return FH_OK;
 }
 
+void add_deps(std::vector<ScratchDep>& out, const std::vector<ScratchDep>& deps) {
+  for (const ScratchDep& d : deps) {
+    bool seen = false;
+    for (const ScratchDep& o : out) seen = seen || o.first == d.first;
+    if (!seen) out.push_back(d);
+  }
+}
+
+int check_deps(const std::vector<ScratchDep>& deps, const char* what) {
+  for (const ScratchDep& d : deps)
+    if (d.first->gen != d.second)
+      return fail(FH_ERR_CAPACITY, std::string(what) +
+                                       (d.first->gen == kGenGone
+                                            ? ": a context whose scratch it addresses was destroyed"
+                                            : ": a context's scratch was reallocated after the recording; record "
+                                              "or capture again"));
+  return FH_OK;
+}
+
 int ensure_scratch(fh_ctx* ctx, size_t bytes) {
   if (ctx->scratch_bytes >= bytes) return FH_OK;
   if (ctx->capturing)
@@ -131,6 +150,7 @@ int ensure_scratch(fh_ctx* ctx, size_t bytes) {
     ctx->scratch_bytes = 0;
   }
   size_t sz = align_up(bytes + bytes / 4, 1 << 20);
+  ctx->sgen->gen++;  // programs and graphs recorded so far address the old scratch
   FH_HIP(hipMalloc(&ctx->scratch, sz));
   ctx->scratch_bytes = sz;
   return FH_OK;
@@ -305,6 +325,7 @@ int fh_destroy(fh_ctx* ctx) {
   if (ctx->branch_fork) hipEventDestroy(ctx->branch_fork);
   if (ctx->branch_done) hipEventDestroy(ctx->branch_done);
   for (hipEvent_t e : ctx->prof_events) hipEventDestroy(e);
+  ctx->sgen->gen = fh::kGenGone;
   if (ctx->scratch) hipFree(ctx->scratch);
   if (ctx->branch_scratch) hipFree(ctx->branch_scratch);
   if (ctx->srgb_thresholds) hipFree(ctx->srgb_thresholds);
@@ -460,6 +481,12 @@ int fh_branch_join(fh_ctx* ctx) {
   return FH_OK;
 }
 
+int fh_scratch_reserve(fh_ctx* ctx, uint64_t bytes) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(!ctx->recording && !ctx->capturing, "fh_scratch_reserve: the context is recording or capturing");
+  return fh::ensure_scratch(ctx, (size_t)bytes);
+}
+
 int fh_set_persistent_blocks(fh_ctx* ctx, uint32_t blocks) {
   FH_CHECK_CTX(ctx);
   ctx->persist_blocks = blocks;
@@ -490,6 +517,7 @@ int fh_record_end(fh_ctx* ctx, fh_program** out) {
   fh_program* p = new fh_program();
   p->rec.swap(ctx->rec);
   p->instances = 1;
+  p->deps.emplace_back(ctx->sgen, ctx->sgen->gen);
   *out = p;
   return FH_OK;
 }
@@ -537,6 +565,7 @@ int fh_program_merge(fh_ctx* ctx, const fh_program* const* programs, uint32_t co
       if (!members.empty() && r.args.size() != members[0]->args.size())
         return fh::fail(FH_ERR_INVALID_ARG, std::string("fh_program_merge: argument sizes differ for ") + r.name);
       members.push_back(&r);
+      fh::add_deps(m->deps, programs[j]->deps);
       total += r.blocks;
       st.shmem = std::max(st.shmem, r.shmem);
       starts.push_back((uint32_t)total);
@@ -573,6 +602,8 @@ int fh_program_launch(fh_ctx* ctx, const fh_program* program) {
   FH_CHECK_CTX(ctx);
   FH_REQUIRE(program && program->merged, "fh_program_launch: not a merged program");
   FH_REQUIRE(!ctx->recording, "fh_program_launch: the context is recording");
+  if (int rc = fh::check_deps(program->deps, "fh_program_launch")) return rc;
+  if (ctx->capturing) fh::add_deps(ctx->cap_deps, program->deps);
   for (const fh_program::Step& st : program->steps) {
     const char* base = static_cast<const char*>(program->dmem);
     const int pk = fh::prof_begin(ctx, st.name);
@@ -598,6 +629,8 @@ int fh_capture_begin(fh_ctx* ctx) {
   FH_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
   ctx->capturing = true;
   ctx->cap_names.clear();
+  ctx->cap_deps.clear();
+  ctx->cap_deps.emplace_back(ctx->sgen, ctx->sgen->gen);
   return FH_OK;
 }
 
@@ -617,6 +650,7 @@ int fh_capture_end(fh_ctx* ctx, fh_graph** out) {
   int rc = was_open ? fh::fail(FH_ERR_INVALID_ARG, "a branch was still open") : FH_OK;
   if (rc == FH_OK && e == hipSuccess && !ctx->cap_names.empty()) rc = fh::add_timing_nodes(g, ctx->cap_names);
   ctx->cap_names.clear();
+  g->deps.swap(ctx->cap_deps);
   if (e == hipSuccess && rc == FH_OK) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
   if (e != hipSuccess || rc != FH_OK) {
     const std::string msg = rc != FH_OK ? fh_last_error() : hipGetErrorString(e);
@@ -630,6 +664,7 @@ int fh_capture_end(fh_ctx* ctx, fh_graph** out) {
 int fh_graph_launch(fh_ctx* ctx, fh_graph* graph) {
   FH_CHECK_CTX(ctx);
   FH_REQUIRE(graph && graph->exec, "fh_graph_launch: null graph");
+  if (int rc = fh::check_deps(graph->deps, "fh_graph_launch")) return rc;
   // timed kernels: bind this replay's event pairs from the profiler's pool, or
   // the capture's own events once the pool is used up (never read back)
   for (size_t k = 0; k < graph->timedNames.size(); k++) {

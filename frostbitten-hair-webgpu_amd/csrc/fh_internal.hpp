@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/fh_hair.h"
@@ -22,6 +24,20 @@ struct FillJob {
   uint64_t n;
   uint32_t value;
 };
+// Generation of a context's scratch: bumped whenever the scratch (or the
+// branch scratch) is reallocated, and set to kGenGone when the context is
+// destroyed. Recorded programs and captured graphs keep the generation of
+// every context whose scratch their launches address, and refuse to run
+// once it has changed. The cell outlives the context (shared ownership).
+struct ScratchGen {
+  uint64_t gen = 0;
+};
+constexpr uint64_t kGenGone = ~0ull;
+using ScratchDep = std::pair<std::shared_ptr<ScratchGen>, uint64_t>;
+// Appends `deps` to `out`, once per cell.
+void add_deps(std::vector<ScratchDep>& out, const std::vector<ScratchDep>& deps);
+// FH_OK, or FH_ERR_CAPACITY naming `what` when a dependency's scratch changed.
+int check_deps(const std::vector<ScratchDep>& deps, const char* what);
 // One launch kept by a recording context instead of being issued.
 struct RecLaunch {
   const char* name;
@@ -43,6 +59,7 @@ struct fh_ctx {
   // per-bin offsets, prefix-scan temporaries).
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
+  std::shared_ptr<fh::ScratchGen> sgen = std::make_shared<fh::ScratchGen>();
   // Branch (fh_branch_begin .. fh_branch_end): calls that may overlap the
   // main stream's later work run on a second stream, ordered after the main
   // stream's work at fh_branch_begin and joined back by fh_branch_join. A
@@ -72,6 +89,9 @@ struct fh_ctx {
   // Kernels launched while capturing with timing on, in launch order: the
   // graph gets event-record nodes around them, rebound on every replay.
   std::vector<const char*> cap_names;
+  // Scratch generations that the calls captured so far depend on (this
+  // context's, and those of the merged programs launched in the capture).
+  std::vector<fh::ScratchDep> cap_deps;
   // Between fh_record_begin and _end the launches are kept here, not issued.
   bool recording = false;
   bool rec_grew = false;  // the scratch was reallocated while recording
@@ -92,6 +112,7 @@ struct fh_program {
   void* dmem = nullptr;
   uint32_t instances = 0;
   bool merged = false;
+  std::vector<fh::ScratchDep> deps;  // scratch generations of the recording contexts
 };
 
 struct fh_graph {
@@ -100,6 +121,7 @@ struct fh_graph {
   std::vector<hipGraphNode_t> timedBegin, timedEnd;  // event-record nodes per timed kernel
   std::vector<const char*> timedNames;
   std::vector<hipEvent_t> capEvents;                  // owned: the nodes' own events (never read)
+  std::vector<fh::ScratchDep> deps;                   // scratch generations the replay addresses
 };
 
 namespace fh {

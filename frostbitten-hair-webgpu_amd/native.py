@@ -137,6 +137,7 @@ SIGNATURES = {
     "fh_profiler_read": (i32, [vp, C.POINTER(KernelTime), u32, C.POINTER(u32)]),
     "fh_set_persistent_blocks": (i32, [vp, u32]),
     "fh_scratch_bytes": (i32, [vp, C.POINTER(C.c_uint64)]),
+    "fh_scratch_reserve": (i32, [vp, C.c_uint64]),
     "fh_record_begin": (i32, [vp]),
     "fh_record_end": (i32, [vp, C.POINTER(vp)]),
     "fh_program_merge": (i32, [vp, C.POINTER(vp), u32, C.POINTER(vp)]),
@@ -254,6 +255,9 @@ class Context:
         n = C.c_uint64(0)
         check(self.lib.fh_scratch_bytes(self.handle, C.byref(n)), "fh_scratch_bytes")
         return n.value
+
+    def scratch_reserve(self, nbytes: int) -> None:
+        check(self.lib.fh_scratch_reserve(self.handle, nbytes), "fh_scratch_reserve")
 
     def record_begin(self) -> None:
         check(self.lib.fh_record_begin(self.handle), "fh_record_begin")

@@ -631,6 +631,14 @@ typedef struct fh_program fh_program;
 /* Device bytes currently held by the context's scratch (for memory
  * budgets: the scratch grows on first use at a new size). */
 int fh_scratch_bytes(fh_ctx* ctx, uint64_t* bytes);
+/* Grows the context's scratch (the one in use: the branch's inside an open
+ * branch) to at least `bytes` now, as a first call at a larger size would,
+ * so that a later capture or recording finds it large enough. Programs
+ * recorded and graphs captured before a reallocation address the freed
+ * scratch: fh_program_launch and fh_graph_launch then return
+ * FH_ERR_CAPACITY before enqueuing anything, and so they do once a context
+ * they address was destroyed. Fails while capturing or recording. */
+int fh_scratch_reserve(fh_ctx* ctx, uint64_t bytes);
 
 /* Caps the workgroups of the grid-stride kernels launched on `ctx` (the tile
  * pass's queued pieces, the fine pass's gather and sort, the grid

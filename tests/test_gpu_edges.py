@@ -134,3 +134,55 @@ def test_clear_batch(gpu):
     ctx.clear_batch_end()
     torch.cuda.synchronize()
     assert (keep.cpu().numpy() == 3).all()
+
+
+def test_replay_refused_after_scratch_reallocation(gpu):
+    """A captured graph and a merged program address the context's scratch as
+    it was when they were recorded. Once a larger eager need (here
+    fh_scratch_reserve) reallocates it, replaying either returns
+    FH_ERR_CAPACITY before anything is enqueued, the device state is left
+    as it was, and a fresh capture or recording replays again; a program
+    whose recording context was destroyed is refused the same way."""
+    import torch
+    from frostbitten_hair_webgpu_amd import assets as A
+    from frostbitten_hair_webgpu_amd import batch as B
+    from frostbitten_hair_webgpu_amd import native as N
+    from frostbitten_hair_webgpu_amd import passes as PS
+    wl = A.make_workload("small", strands=256, width=160, height=120)
+    r = PS.HairRenderer(wl, "cuda:0", shadow_size=128)
+    r.beforeFirstFrame()
+    r.frame()  # every call once: the scratch has its size
+    torch.cuda.synchronize()
+    r.capture_frames()
+    r.frame_graph()
+    torch.cuda.synchronize()
+    prog = B.record_frame(r)
+    merged = r.native.program_merge([prog])
+    merged.launch()
+    torch.cuda.synchronize()
+    before = r.hair.positionsNow.clone()
+    r.native.scratch_reserve(r.native.scratch_bytes() * 2 + (64 << 20))
+    with pytest.raises(N.FhError) as e:
+        r.frame_graph()
+    assert e.value.code == N.FH_ERR_CAPACITY
+    with pytest.raises(N.FhError) as e:
+        merged.launch()
+    assert e.value.code == N.FH_ERR_CAPACITY
+    torch.cuda.synchronize()
+    assert torch.equal(r.hair.positionsNow, before)  # nothing ran
+    r.capture_frames()
+    r.frame_graph()
+    merged2 = r.native.program_merge([B.record_frame(r)])
+    merged2.launch()
+    torch.cuda.synchronize()
+    assert np.isfinite(_np(r.hair.positionsNow)).all()
+    # a program whose recording context is gone
+    r2 = PS.HairRenderer(wl, "cuda:0", shadow_size=128)
+    r2.beforeFirstFrame()
+    r2.frame()
+    torch.cuda.synchronize()
+    orphan = r.native.program_merge([B.record_frame(r2)])
+    r2.native.close()
+    with pytest.raises(N.FhError) as e:
+        orphan.launch()
+    assert e.value.code == N.FH_ERR_CAPACITY
