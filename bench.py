@@ -167,17 +167,33 @@ def read_traffic(kernel: str, workload: str):
     return None
 
 
-def cpu_baseline(wl, r, frames: int = 8, budget_s: float = 20.0) -> dict:
-    """Times the oracle on full frames of the workload on one core: up to
-    `frames` frames, stopping early once `budget_s` seconds are spent."""
+REFERENCE_CPU = ("unavailable: the reference's CPU path is its WGSL under Deno on a software Vulkan adapter "
+                 "(lavapipe); the GPU box has no deno binary, no Vulkan ICD (/usr/share/vulkan/icd.d, "
+                 "/etc/vulkan/icd.d) and no Deno module cache, and nothing can be fetched offline")
+
+
+def host_threads() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(wl, r, frames: int = 8, budget_s: float = 15.0) -> dict:
+    """Times the oracle on full frames of the workload on every host core:
+    up to `frames` frames, stopping once `budget_s` seconds are spent. The
+    oracle's independent loops (strands, grid nodes, tiles) run on one
+    thread per core; its results do not depend on the thread count."""
     import __graft_entry__ as G
     O = G.load_oracle()
     from frostbitten_hair_webgpu_amd import assets as A
     from frostbitten_hair_webgpu_amd import layouts as L
+    threads = host_threads()
+    O.set_threads(threads)
     S, P = wl.strands, wl.points
     i = {k: np.ascontiguousarray(v.cpu().numpy()) for k, v in r.inputs.items()}
-    sl = A.segment_lengths(wl.positions, S, P)
-    tan = A.tangents(wl.positions, S, P)
+    d = A.derived(wl)
+    sl, tan = d["seglen"], d["tangents"]
     hd = L.pack_hair_data(r.hair.sphere[0], r.hair.sphere[1], S, P)
     cap = r.tilesPass.segmentsCapacity
     prev = wl.positions.copy()
@@ -185,28 +201,34 @@ def cpu_baseline(wl, r, frames: int = 8, budget_s: float = 20.0) -> dict:
     shading = A.initial_shading(S, wl.render.shadingPoints)
     dv = O.grid_post_sim(wl.sim.gridDims, r.sim_uniforms(0), prev, now, S, P)
     times = []
-    for f in range(frames):
-        t0 = time.perf_counter()
-        su = r.sim_uniforms(f)
-        dgw = O.grid_pre_sim(wl.sim.gridDims, su, dv, wl.sdf, wl.sdf_dims)
-        nxt = O.sim_integrate(wl.sim.gridDims, su, prev, now, sl, wl.sdf, wl.sdf_dims, dv, dgw, wl.positions, S, P)
-        prev, now = now, nxt
-        dv = O.grid_post_sim(wl.sim.gridDims, su, prev, now, S, P)
-        tiles, seg, counts, _ = O.tiles_draw(wl.render, wl.width, wl.height, r.render_uniforms, hd, now, tan,
-                                             i["meshDepth"], cap, S, P)
-        tl, _ = O.tiles_sort(wl.render, wl.width, wl.height, counts)
-        O.fine_raster(wl.render, wl.width, wl.height, r.render_uniforms, now, tiles, seg, cap, i["meshDepth"],
-                      shading, tan, tl, S, P, canonical=False)
-        shading = O.shading(wl.render, r.render_uniforms, now, tan, i["shadowMap"], i["ao"], i["hairDepth"],
-                            S, P).reshape(-1, 4)
-        times.append(time.perf_counter() - t0)
-        if sum(times) > budget_s:
-            break
+    try:
+        for f in range(frames):
+            t0 = time.perf_counter()
+            su = r.sim_uniforms(f)
+            dgw = O.grid_pre_sim(wl.sim.gridDims, su, dv, wl.sdf, wl.sdf_dims)
+            nxt = O.sim_integrate(wl.sim.gridDims, su, prev, now, sl, wl.sdf, wl.sdf_dims, dv, dgw, wl.positions,
+                                  S, P)
+            prev, now = now, nxt
+            dv = O.grid_post_sim(wl.sim.gridDims, su, prev, now, S, P)
+            tiles, seg, counts, _ = O.tiles_draw(wl.render, wl.width, wl.height, r.render_uniforms, hd, now, tan,
+                                                 i["meshDepth"], cap, S, P)
+            tl, _ = O.tiles_sort(wl.render, wl.width, wl.height, counts)
+            O.fine_raster(wl.render, wl.width, wl.height, r.render_uniforms, now, tiles, seg, cap, i["meshDepth"],
+                          shading, tan, tl, S, P, canonical=False)
+            shading = O.shading(wl.render, r.render_uniforms, now, tan, i["shadowMap"], i["ao"], i["hairDepth"],
+                                S, P).reshape(-1, 4)
+            times.append(time.perf_counter() - t0)
+            if sum(times) > budget_s:
+                break
+    finally:
+        O.set_threads(1)
     frames = len(times)
     t = float(np.median(times))
-    return {"value": round(1.0 / t, 4), "unit": "frames/s", "cores": 1, "kind": "port",
+    return {"value": round(1.0 / t, 4), "unit": "frames/s", "cores": threads, "kind": "port",
             "sample": f"{frames} full frames (sim step + tile + sort + fine + shading) of the same workload, "
-                      f"median {t:.2f} s/frame, scalar C oracle on 1 host core"}
+                      f"median {t:.3f} s/frame, the C oracle (a restatement of the reference's kernels) with its "
+                      f"strand, grid-node and tile loops on {threads} host threads",
+            "reference": REFERENCE_CPU}
 
 
 def batched_vs_plain(n: int, steps: int, warmup: int, tune: bool, dev, log) -> dict:
@@ -391,9 +413,255 @@ def served(n_req: int, rate: float, budget_gib: float, dev, log) -> dict:
         log(f"# serve {mode}: {out[mode]['frames_per_s']} frames/s, p50 {out[mode]['latency_s']['p50']} s")
     best = max(out["continuous"]["frames_per_s"], out["continuous_pipelined"]["frames_per_s"])
     out["speedup_vs_sequential"] = round(best / out["sequential"]["frames_per_s"], 3)
+    # latency at loads the device sustains: the same request mix (assets and
+    # frame counts) arriving at 25%, 50% and 90% of the measured capacity in
+    # requests per second (the continuous rate / mean frames per request)
+    mean_frames = out["continuous"]["frames"] / n_req
+    capacity = best / mean_frames
+    out["capacity_requests_per_s"] = round(capacity, 3)
+    sweep = []
+    for load in (0.25, 0.5, 0.9):
+        lam = load * capacity
+        qs = SV.poisson_requests(n_req, lam, list(range(256)), 120, seed=A.SERVE_SEED)
+        srv = SV.HairServer(pool, int(budget_gib * (1 << 30)), max_active=64, device=dev, pipelined=True)
+        rs = srv.run(qs)
+        row = dict(SV.summarize(rs, srv.elapsed), rate_per_s=round(lam, 3), offered_load=load,
+                   ticks=srv.stats.get("ticks"))
+        srv.close()
+        del srv
+        sweep.append(row)
+        log(f"# serve at {load:.0%} of capacity ({lam:.2f}/s): p50 {row['latency_s']['p50']} s, "
+            f"p99 {row['latency_s']['p99']} s")
+    out["latency_vs_load"] = sweep
     out["speedup_vs"] = ("the faster continuous form (pipelined ticks: the merged simulation step beside the "
                          "merged render of the previous step) over one request at a time")
     return out
+
+
+def plain_run(kind: str, steps: int, warmup: int, args, dev) -> tuple:
+    """Frames of the plain drop-in on workload `kind`: the timed regions, the
+    per-kernel durations and the roofline of the dominant kernel. Returns the
+    result dict, the renderer and the workload."""
+    import torch
+    from frostbitten_hair_webgpu_amd import assets as A
+    from frostbitten_hair_webgpu_amd import passes as PS
+    t_setup = time.perf_counter()
+    wl = A.make_workload(kind)
+    r = PS.HairRenderer(wl, dev, branches=not args.no_branches)
+    total = warmup + 5 * steps + 4  # five timed regions, pipeline prologue and sizing frame
+    # per-frame simulation uniforms (frameIdx changes), uploaded once up front
+    ring = torch.from_numpy(np.concatenate([r.sim_uniforms(f).view(np.uint8).reshape(-1) for f in range(total)]))
+    ring = ring.to(dev).view(total, -1)
+    print(f"# setup {time.perf_counter() - t_setup:.1f} s, workload {wl.name}: {wl.strands}x{wl.points} "
+          f"{wl.width}x{wl.height}", file=sys.stderr, flush=True)
+
+    stages = ["sim", "tiles", "sort", "fine", "post", "shading"]
+
+    def frame(events=None):
+        r.simulationUniforms.copy_(ring[r.frameIdx])
+        ctx = r.ctx()
+        rec = (lambda k: events[k].record()) if events else (lambda k: None)
+        rec(0)
+        r.simulateHair(ctx)
+        rec(1)
+        r.drawScene(ctx)
+        r.tilesPass.cmdClearBeforeRender(ctx)
+        r.finePass.cmdClearBeforeRender(ctx)
+        r.sortPass.cmdClearBeforeRender(ctx)
+        r.tilesPass.cmdDrawHairToTiles(ctx, r.hair)
+        rec(2)
+        r.sortPass.cmdSortHairTiles(ctx)
+        rec(3)
+        r.finePass.cmdRasterizeSlicesHair(ctx, r.hair, r.tilesPass)
+        rec(4)
+        r.hairCombinePass.cmdCombineRasterResults(ctx)
+        r.presentPass.cmdDraw(ctx, r.screenTexture)
+        rec(5)
+        r.shadingPass.cmdComputeShadingPoints(ctx, r.hair)
+        r.join_branch()  # the grid accumulation ran beside the render passes
+        rec(6)
+        r.frameIdx += 1
+
+    def graph_frame(events=None):
+        r.simulationUniforms.copy_(ring[r.frameIdx])
+        if events:
+            events[0].record()
+        r.frame_graph(upload_uniforms=False)
+        if events:
+            events[6].record()
+
+    def pipelined_frame(events=None):
+        r.simulationUniforms.copy_(ring[r.frameIdx])
+        if events:
+            events[0].record()
+        r.frame_pipelined_graph(upload_uniforms=False)
+        if events:
+            events[6].record()
+
+    def region(step, events=True):
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(7)] for _ in range(steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            step(evs[k] if events else None)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, evs
+
+    def timed_pair(step):
+        """Wall time of `steps` frames with no events between them (value),
+        then as many frames with an event at every frame boundary (latency)."""
+        el, _ = region(step, events=False)
+        _, evs = region(step, events=True)
+        return el, evs
+
+    use_graph = not args.no_graph
+    pipelined = use_graph and not args.no_branches and not args.no_pipeline
+    r.beforeFirstFrame()
+    for _ in range(warmup):
+        frame()
+    torch.cuda.synchronize()
+    step = graph_frame if use_graph else frame
+
+    # Timed region 1: the frames alone (events only at frame boundaries) give
+    # value and the frame latencies. Pipelined, each frame runs the simulation
+    # step of frame f beside the render passes of frame f-1.
+    unpipelined = None
+    if pipelined:
+        r.simulationUniforms.copy_(ring[r.frameIdx])
+        r.pipeline_begin(upload_uniforms=False)
+        r.simulationUniforms.copy_(ring[r.frameIdx])
+        r.frame_pipelined(upload_uniforms=False)  # every call once before capture
+        r.capture_pipelined()
+        elapsed, evs = timed_pair(pipelined_frame)
+        r.pipeline_end()
+        # sim-to-present: from the start of the frame that ran an image's
+        # simulation step to the end of the frame that presented it
+        s2p = np.array([evs[k - 1][0].elapsed_time(evs[k][6]) for k in range(1, steps)])
+        r.capture_frames()
+        el_u, evs_u = timed_pair(graph_frame)
+        per_u = np.array([evs_u[k][0].elapsed_time(evs_u[k][6]) for k in range(steps)])
+        unpipelined = {"value": round(steps / el_u, 3), "ms_per_step": round(el_u / steps * 1e3, 4),
+                       "latency_ms": {"p50": round(float(np.percentile(per_u, 50)), 4),
+                                      "p99": round(float(np.percentile(per_u, 99)), 4)},
+                       "frame": "one graph per frame: simulation step, then the render passes"}
+    else:
+        if use_graph:
+            r.capture_frames()
+            elapsed, evs = timed_pair(step)
+        else:  # eager frames: the events also time the stages
+            elapsed, evs = region(step)
+
+    # Timed region 2: as many further frames with HIP events around every
+    # kernel on the library's stream (in graph replays, event-record nodes
+    # around every kernel node): per-kernel durations and the roofline. The
+    # events cost frame time, so they stay out of region 1.
+    kernels = {}
+    evs2 = [[torch.cuda.Event(enable_timing=True) for _ in range(7)] for _ in range(steps)]
+    if not args.no_kernel_timing:
+        r.native.profiler_enable(64 * steps)
+        if use_graph:
+            r.capture_frames()
+        torch.cuda.synchronize()
+        r.native.profiler_read()
+        for k in range(steps):
+            step(evs2[k])
+        torch.cuda.synchronize()
+        kernels = r.native.profiler_read()
+        r.native.profiler_enable(0)
+    per_frame = np.array([evs[k][0].elapsed_time(evs[k][6]) for k in range(steps)])  # ms
+    if use_graph:
+        # stage sums from the per-kernel events
+        groups = {"sim": ("grid_pre_sim", "sim_integrate", "p2g_clear", "p2g_count", "p2g_scan", "p2g_scatter",
+                         "grid_post_sim"),
+                  "tiles": ("fill_many", "tile_depth_bounds", "tiles_draw", "tiles_pieces", "tile_counts"),
+                  "sort": ("sort_count", "sort_scatter"),
+                  "fine": ("fill_u32", "bin_scan_local", "bin_scan_add", "bin_scatter", "bin_sort_small",
+                           "bin_sort_large", "fine_raster"),
+                  "post": ("copy_u32", "hair_combine", "present"),
+                  "shading": ("shading",)}
+        stage_ms = {g: sum(kernels[k][1] for k in ks if k in kernels) / steps for g, ks in groups.items()}
+        if not kernels:
+            stage_ms = {}
+    elif kernels:
+        stage_ms = {s: float(np.mean([evs2[k][i].elapsed_time(evs2[k][i + 1]) for k in range(steps)]))
+                    for i, s in enumerate(stages)}
+    else:
+        stage_ms = {}
+    # drawnSegmentsCount counts every pair; the list keeps the first
+    # `capacity` of them (hairTilesPass.wgsl.ts:221-237) and the bytes below
+    # count only the kept ones
+    drawn_pairs = int(r.tilesPass.hairTileSegmentsBuffer[0].item())
+    capacity = r.tilesPass.segmentsCapacity
+    n_pairs = min(drawn_pairs, capacity)
+    tiles = (wl.width + wl.render.tileSize - 1) // wl.render.tileSize * \
+            ((wl.height + wl.render.tileSize - 1) // wl.render.tileSize)
+    drawn_tiles = int(r.sortPass.tileListBuffer[0].item())
+    kb = kernel_bytes(wl, n_pairs, tiles)
+    kernel_ms = {k: t / n for k, (n, t) in kernels.items()}
+    timed = [k for k in kernel_ms if k in kb]
+    dom = max(timed, key=lambda k: kernels[k][1]) if timed else None
+    achieved = kb[dom] / (kernel_ms[dom] * 1e-3) / 1e9 if dom else None
+    traffic = read_traffic(dom, wl.name) if dom else None
+    rooflines = {}
+    for k in ("fine_raster", "tiles_draw"):
+        if k in kernel_ms:
+            a = kb[k] / (kernel_ms[k] * 1e-3) / 1e9
+            rooflines[k] = {"achieved": round(a, 2), "frac": round(a / HBM_PEAK_GBS, 5),
+                            "avg_launch_us": round(kernel_ms[k] * 1e3, 2), "algorithmic_bytes": int(kb[k]),
+                            "traffic": read_traffic(k, wl.name)}
+    img = r.finePass.image(wl.width, wl.height)
+    assert torch.isfinite(img).all()
+
+    ms = elapsed / steps * 1e3
+    out = {
+        "metric": METRIC,
+        "value": round(1e3 / ms, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded strands on a scalp cap, analytic colliders; depth/shadow/AO inputs precomputed)",
+        "config": {"workload": f"plain drop-in{' replayed as a HIP graph' if use_graph else ''}, "
+                               f"{wl.strands} strands x {wl.points} points, {wl.width}x{wl.height}, "
+                               f"tile {wl.render.tileSize}, {wl.render.tileDepthBins} bins x "
+                               f"{wl.render.slicesPerPixel} slices, grid {wl.sim.gridDims}^3, SDF {wl.sdf_dims}^3, "
+                               f"{wl.sim.constraintIterations} constraint iterations",
+                   "global_batch": 1, "parallelism": "none (1 GPU)",
+                   "overlap": "none (one stream)" if args.no_branches else
+                              ("pipelined: simulation step of frame f beside the render passes of frame f-1; "
+                               "shading beside combine/present" if pipelined else
+                               "grid accumulation beside tiles/sort/fine, shading beside combine/present")},
+        "latency_ms": {"p50": round(float(np.percentile(per_frame, 50)), 4),
+                       "p99": round(float(np.percentile(per_frame, 99)), 4)},
+        "pipelined": pipelined,
+        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "stage_hbm_GBps": {k: round(v / (stage_ms[k] * 1e-3) / 1e9, 2)
+                           for k, v in stage_bytes(wl, n_pairs, tiles).items() if stage_ms.get(k)},
+        "throughput": {"particle_steps_per_s": round(wl.strands * wl.points * 1e3 / ms, 1),
+                       "segments_rasterized_per_s": round(wl.strands * (wl.points - 1) * 1e3 / ms, 1),
+                       "segment_tile_pairs": drawn_pairs, "segment_list_capacity": capacity,
+                       "dropped_pairs": max(0, drawn_pairs - capacity), "drawn_tiles": drawn_tiles,
+                       "tile_coverage": round(drawn_tiles / tiles, 4)},
+        "kernel_us": {k: round(v * 1e3, 2) for k, v in sorted(kernel_ms.items(), key=lambda kv: -kernels[kv[0]][1])},
+        "kernel_timing": f"a further timed region of {steps} frames with HIP events around every kernel "
+                         f"(one graph per frame, kernels one after another)",
+        "roofline": None if dom is None else {
+            "bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+            "algorithmic_bytes": int(kb[dom]), "avg_launch_us": round(kernel_ms[dom] * 1e3, 2)},
+        "rooflines": rooflines,
+    }
+    if pipelined:
+        out["latency_ms"]["frame"] = "per pipelined frame (frame period)"
+        out["latency_ms"]["sim_to_present_p50"] = round(float(np.percentile(s2p, 50)), 4)
+        out["latency_ms"]["sim_to_present_p99"] = round(float(np.percentile(s2p, 99)), 4)
+        out["unpipelined"] = unpipelined
+    return out, r, wl
 
 
 def main() -> None:
@@ -403,6 +671,8 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2", choices=["small", "c2", "c3"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--c3-steps", type=int, default=20,
+                    help="frames of the 4K, 100,000-strand workload timed beside the main line (0: skip it)")
     ap.add_argument("--full-frame", type=int, default=1,
                     help="also time frames that compute the hair depth / normals, shadow map and AO every frame "
                          "(0 to skip)")
@@ -458,213 +728,28 @@ def main() -> None:
         print(json.dumps({"copy_probe_mib": args.copy_probe, "ms": ms,
                           "GB_per_s": 2 * n * 4 / (ms * 1e-3) / 1e9}), flush=True)
         return
-    t_setup = time.perf_counter()
-    wl = A.make_workload(args.workload)
-    r = PS.HairRenderer(wl, dev, branches=not args.no_branches)
-    total = args.warmup + 5 * args.steps + 4  # five timed regions, pipeline prologue and sizing frame
-    # per-frame simulation uniforms (frameIdx changes), uploaded once up front
-    ring = torch.from_numpy(np.concatenate([r.sim_uniforms(f).view(np.uint8).reshape(-1) for f in range(total)]))
-    ring = ring.to(dev).view(total, -1)
-    print(f"# setup {time.perf_counter() - t_setup:.1f} s, workload {wl.name}: {wl.strands}x{wl.points} "
-          f"{wl.width}x{wl.height}", file=sys.stderr, flush=True)
-
-    stages = ["sim", "tiles", "sort", "fine", "post", "shading"]
-
-    def frame(events=None):
-        r.simulationUniforms.copy_(ring[r.frameIdx])
-        ctx = r.ctx()
-        rec = (lambda k: events[k].record()) if events else (lambda k: None)
-        rec(0)
-        r.simulateHair(ctx)
-        rec(1)
-        r.drawScene(ctx)
-        r.tilesPass.cmdClearBeforeRender(ctx)
-        r.finePass.cmdClearBeforeRender(ctx)
-        r.sortPass.cmdClearBeforeRender(ctx)
-        r.tilesPass.cmdDrawHairToTiles(ctx, r.hair)
-        rec(2)
-        r.sortPass.cmdSortHairTiles(ctx)
-        rec(3)
-        r.finePass.cmdRasterizeSlicesHair(ctx, r.hair, r.tilesPass)
-        rec(4)
-        r.hairCombinePass.cmdCombineRasterResults(ctx)
-        r.presentPass.cmdDraw(ctx, r.screenTexture)
-        rec(5)
-        r.shadingPass.cmdComputeShadingPoints(ctx, r.hair)
-        r.join_branch()  # the grid accumulation ran beside the render passes
-        rec(6)
-        r.frameIdx += 1
-
-    def graph_frame(events=None):
-        r.simulationUniforms.copy_(ring[r.frameIdx])
-        if events:
-            events[0].record()
-        r.frame_graph(upload_uniforms=False)
-        if events:
-            events[6].record()
-
-    def pipelined_frame(events=None):
-        r.simulationUniforms.copy_(ring[r.frameIdx])
-        if events:
-            events[0].record()
-        r.frame_pipelined_graph(upload_uniforms=False)
-        if events:
-            events[6].record()
-
-    def region(step, events=True):
-        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(7)] for _ in range(args.steps)]
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for k in range(args.steps):
-            step(evs[k] if events else None)
-        torch.cuda.synchronize()
-        return time.perf_counter() - t0, evs
-
-    def timed_pair(step):
-        """Wall time of `steps` frames with no events between them (value),
-        then as many frames with an event at every frame boundary (latency)."""
-        el, _ = region(step, events=False)
-        _, evs = region(step, events=True)
-        return el, evs
-
-    use_graph = not args.no_graph
-    pipelined = use_graph and not args.no_branches and not args.no_pipeline
-    r.beforeFirstFrame()
-    for _ in range(args.warmup):
-        frame()
-    torch.cuda.synchronize()
-    step = graph_frame if use_graph else frame
-
-    # Timed region 1: the frames alone (events only at frame boundaries) give
-    # value and the frame latencies. Pipelined, each frame runs the simulation
-    # step of frame f beside the render passes of frame f-1.
-    unpipelined = None
-    if pipelined:
-        r.simulationUniforms.copy_(ring[r.frameIdx])
-        r.pipeline_begin(upload_uniforms=False)
-        r.simulationUniforms.copy_(ring[r.frameIdx])
-        r.frame_pipelined(upload_uniforms=False)  # every call once before capture
-        r.capture_pipelined()
-        elapsed, evs = timed_pair(pipelined_frame)
-        r.pipeline_end()
-        # sim-to-present: from the start of the frame that ran an image's
-        # simulation step to the end of the frame that presented it
-        s2p = np.array([evs[k - 1][0].elapsed_time(evs[k][6]) for k in range(1, args.steps)])
-        r.capture_frames()
-        el_u, evs_u = timed_pair(graph_frame)
-        per_u = np.array([evs_u[k][0].elapsed_time(evs_u[k][6]) for k in range(args.steps)])
-        unpipelined = {"value": round(args.steps / el_u, 3), "ms_per_step": round(el_u / args.steps * 1e3, 4),
-                       "latency_ms": {"p50": round(float(np.percentile(per_u, 50)), 4),
-                                      "p99": round(float(np.percentile(per_u, 99)), 4)},
-                       "frame": "one graph per frame: simulation step, then the render passes"}
-    else:
-        if use_graph:
-            r.capture_frames()
-            elapsed, evs = timed_pair(step)
-        else:  # eager frames: the events also time the stages
-            elapsed, evs = region(step)
-
-    # Timed region 2: as many further frames with HIP events around every
-    # kernel on the library's stream (in graph replays, event-record nodes
-    # around every kernel node): per-kernel durations and the roofline. The
-    # events cost frame time, so they stay out of region 1.
-    kernels = {}
-    evs2 = [[torch.cuda.Event(enable_timing=True) for _ in range(7)] for _ in range(args.steps)]
-    if not args.no_kernel_timing:
-        r.native.profiler_enable(64 * args.steps)
-        if use_graph:
-            r.capture_frames()
-        torch.cuda.synchronize()
-        r.native.profiler_read()
-        for k in range(args.steps):
-            step(evs2[k])
-        torch.cuda.synchronize()
-        kernels = r.native.profiler_read()
-        r.native.profiler_enable(0)
-    per_frame = np.array([evs[k][0].elapsed_time(evs[k][6]) for k in range(args.steps)])  # ms
-    if use_graph:
-        # stage sums from the per-kernel events
-        groups = {"sim": ("grid_pre_sim", "sim_integrate", "p2g_clear", "p2g_count", "p2g_scan", "p2g_scatter",
-                         "grid_post_sim"),
-                  "tiles": ("fill_many", "tile_depth_bounds", "tiles_draw", "tiles_pieces", "tile_counts"),
-                  "sort": ("sort_count", "sort_scatter"),
-                  "fine": ("fill_u32", "bin_scan_local", "bin_scan_add", "bin_scatter", "bin_sort_small",
-                           "bin_sort_large", "fine_raster"),
-                  "post": ("copy_u32", "hair_combine", "present"),
-                  "shading": ("shading",)}
-        stage_ms = {g: sum(kernels[k][1] for k in ks if k in kernels) / args.steps for g, ks in groups.items()}
-        if not kernels:
-            stage_ms = {}
-    elif kernels:
-        stage_ms = {s: float(np.mean([evs2[k][i].elapsed_time(evs2[k][i + 1]) for k in range(args.steps)]))
-                    for i, s in enumerate(stages)}
-    else:
-        stage_ms = {}
-    n_pairs = int(r.tilesPass.hairTileSegmentsBuffer[0].item())
-    tiles = (wl.width + wl.render.tileSize - 1) // wl.render.tileSize * \
-            ((wl.height + wl.render.tileSize - 1) // wl.render.tileSize)
-    drawn_tiles = int(r.sortPass.tileListBuffer[0].item())
-    kb = kernel_bytes(wl, n_pairs, tiles)
-    kernel_ms = {k: t / n for k, (n, t) in kernels.items()}
-    timed = [k for k in kernel_ms if k in kb]
-    dom = max(timed, key=lambda k: kernels[k][1]) if timed else None
-    achieved = kb[dom] / (kernel_ms[dom] * 1e-3) / 1e9 if dom else None
-    traffic = read_traffic(dom, wl.name) if dom else None
-    img = r.finePass.image(wl.width, wl.height)
-    assert torch.isfinite(img).all()
-
-    ms = elapsed / args.steps * 1e3
-    out = {
-        "metric": METRIC,
-        "value": round(1e3 / ms, 3),
-        "unit": "frames/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded strands on a scalp cap, analytic colliders; depth/shadow/AO inputs precomputed)",
-        "config": {"workload": f"plain drop-in{' replayed as a HIP graph' if use_graph else ''}, "
-                               f"{wl.strands} strands x {wl.points} points, {wl.width}x{wl.height}, "
-                               f"tile {wl.render.tileSize}, {wl.render.tileDepthBins} bins x "
-                               f"{wl.render.slicesPerPixel} slices, grid {wl.sim.gridDims}^3, SDF {wl.sdf_dims}^3, "
-                               f"{wl.sim.constraintIterations} constraint iterations",
-                   "global_batch": 1, "parallelism": "none (1 GPU)",
-                   "overlap": "none (one stream)" if args.no_branches else
-                              ("pipelined: simulation step of frame f beside the render passes of frame f-1; "
-                               "shading beside combine/present" if pipelined else
-                               "grid accumulation beside tiles/sort/fine, shading beside combine/present")},
-        "latency_ms": {"p50": round(float(np.percentile(per_frame, 50)), 4),
-                       "p99": round(float(np.percentile(per_frame, 99)), 4)},
-        "pipelined": pipelined,
-        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
-        "stage_hbm_GBps": {k: round(v / (stage_ms[k] * 1e-3) / 1e9, 2)
-                           for k, v in stage_bytes(wl, n_pairs, tiles).items() if stage_ms.get(k)},
-        "throughput": {"particle_steps_per_s": round(wl.strands * wl.points * 1e3 / ms, 1),
-                       "segments_rasterized_per_s": round(wl.strands * (wl.points - 1) * 1e3 / ms, 1),
-                       "segment_tile_pairs": n_pairs, "drawn_tiles": drawn_tiles},
-        "kernel_us": {k: round(v * 1e3, 2) for k, v in sorted(kernel_ms.items(), key=lambda kv: -kernels[kv[0]][1])},
-        "kernel_timing": f"a further timed region of {args.steps} frames with HIP events around every kernel "
-                         f"(one graph per frame, kernels one after another)",
-        "roofline": None if dom is None else {
-            "bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-            "algorithmic_bytes": int(kb[dom]), "avg_launch_us": round(kernel_ms[dom] * 1e3, 2)},
-    }
-    if pipelined:
-        out["latency_ms"]["frame"] = "per pipelined frame (frame period)"
-        out["latency_ms"]["sim_to_present_p50"] = round(float(np.percentile(s2p, 50)), 4)
-        out["latency_ms"]["sim_to_present_p99"] = round(float(np.percentile(s2p, 99)), 4)
-        out["unpipelined"] = unpipelined
+    out, r, wl = plain_run(args.workload, args.steps, args.warmup, args, dev)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, r)
+        out["speedup_vs_cpu_baseline"] = round(out["value"] / out["cpu_baseline"]["value"], 1)
     if args.full_frame:
         out["full_frame"] = full_frame(wl, r, min(args.steps, 30), dev, branches=not args.no_branches)
+    del r
+    if args.c3_steps and args.workload != "c3":
+        # the 4K, 100,000-strand workload (the dense fine-pass case) beside the 1080p line
+        torch.cuda.empty_cache()
+        full, r3, wl3 = plain_run("c3", args.c3_steps, 3, args, dev)
+        c3 = {k: full[k] for k in ("value", "unit", "ms_per_step", "latency_ms", "unpipelined", "throughput",
+                                   "roofline", "rooflines", "stage_ms", "kernel_us") if k in full}
+        c3["config"] = full["config"]["workload"]
+        c3["steps"] = args.c3_steps
+        if not args.no_cpu_baseline:
+            c3["cpu_baseline"] = cpu_baseline(wl3, r3, frames=3, budget_s=12.0)
+            c3["speedup_vs_cpu_baseline"] = round(c3["value"] / c3["cpu_baseline"]["value"], 1)
+        out["c3"] = c3
+        del r3
+        torch.cuda.empty_cache()
     if args.batch_instances:
-        del r
         out["batched"] = batched_vs_plain(args.batch_instances, min(args.steps, 20), 2, not args.no_tune, dev,
                                           lambda m: print(m, file=sys.stderr, flush=True))
     if args.serve_requests:
