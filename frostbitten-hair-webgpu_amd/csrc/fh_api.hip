@@ -294,6 +294,10 @@ int fh_create(int device, fh_ctx** out) {
   }
   c->num_cus = prop.multiProcessorCount;
   c->max_lds = prop.sharedMemPerBlock;
+  if (int rc = fh::setup_noalloc_scratch(device)) {
+    delete c;
+    return rc;
+  }
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete c;

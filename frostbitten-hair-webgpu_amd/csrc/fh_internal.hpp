@@ -38,6 +38,9 @@ using ScratchDep = std::pair<std::shared_ptr<ScratchGen>, uint64_t>;
 void add_deps(std::vector<ScratchDep>& out, const std::vector<ScratchDep>& deps);
 // FH_OK, or FH_ERR_CAPACITY naming `what` when a dependency's scratch changed.
 int check_deps(const std::vector<ScratchDep>& deps, const char* what);
+// Looks up, on `device` (current), the device address of the fixed scratch of
+// fh_grid_post_sim_noalloc; called once per device by fh_create.
+int setup_noalloc_scratch(int device);
 // One launch kept by a recording context instead of being issued.
 struct RecLaunch {
   const char* name;

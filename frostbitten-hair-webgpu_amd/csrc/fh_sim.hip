@@ -4,6 +4,9 @@
 #include "fh_device.hpp"
 #include "fh_launch.hpp"
 
+#include <mutex>
+#include <string>
+
 namespace fh {
 int fill_u32(fh_ctx* ctx, void* dst, uint64_t n_words, uint32_t value);
 }
@@ -623,7 +626,115 @@ struct P2GAccum {
 };
 
 
+// Scratch of fh_grid_post_sim_noalloc: a fixed device array, so that the
+// call allocates nothing (callers that may not allocate: code under HIP
+// graph capture, serving loops). Every call on one GPU shares it: calls
+// that overlap on one GPU race, and each call clears the block counts it
+// needs (the kernels write everything else before reading it).
+constexpr size_t kNoAllocScratchBytes = (size_t)16 << 20;
 }  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noallocMu;
+uint32_t* g_noallocPtr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (the current device): its
+// host-side name is not a device address, so every kernel argument and fill
+// uses the address looked up here, once per device.
+int noalloc_scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return fh::fail(FH_ERR_UNSUPPORTED, "device index above 63");
+  std::lock_guard<std::mutex> lock(g_noallocMu);
+  if (g_noallocPtr[device]) return FH_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fh::fail(FH_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_noallocPtr[device] = static_cast<uint32_t*>(p);
+  return FH_OK;
+}
+
+int noalloc_scratch(int device, uint32_t** out) {
+  std::lock_guard<std::mutex> lock(g_noallocMu);
+  *out = device >= 0 && device < kMaxDevices ? g_noallocPtr[device] : nullptr;
+  if (!*out) return fh::fail(FH_ERR_UNSUPPORTED, "fh_grid_post_sim_noalloc: no scratch address for this device");
+  return FH_OK;
+}
+
+// fh_grid_post_sim and fh_grid_post_sim_noalloc: the same kernels, with
+// their scratch in the context's grow-on-demand scratch or in `scratch`.
+int grid_post_sim(fh_ctx* ctx, uint32_t grid_dims, const fh_sim_uniforms* sim_uniforms,
+                  const fh_hair_data* hair_data, const fh_vec4* positions_prev,
+                  const fh_vec4* positions_now, fh_density_velocity* density_velocity,
+                  uint32_t strands_count, uint32_t points_per_strand, bool noalloc) {
+  FH_CHECK_CTX(ctx);
+  (void)hair_data;
+  FH_REQUIRE(grid_dims >= 2 && grid_dims <= 1024, "fh_grid_post_sim: grid_dims must be in [2, 1024]");
+  FH_REQUIRE(sim_uniforms && positions_prev && positions_now && density_velocity,
+             "fh_grid_post_sim: null buffer");
+  const uint64_t cells = (uint64_t)grid_dims * grid_dims * grid_dims;
+  const uint64_t n = (uint64_t)strands_count * points_per_strand;
+  FH_REQUIRE(n < (1ull << 28), "fh_grid_post_sim: too many points");
+  P2GArgs a;
+  a.d = grid_dims;
+  a.nb = fh::div_ceil(grid_dims, kBlk);
+  a.u = sim_uniforms;
+  a.prev = reinterpret_cast<const float4*>(positions_prev);
+  a.now = reinterpret_cast<const float4*>(positions_now);
+  a.dv = density_velocity;
+  a.nPoints = (uint32_t)n;
+  // about two chunks per CU for the whole grid, more for hot blocks
+  a.chunk = std::min(8192u, std::max(512u, fh::div_ceil(fh::div_ceil((uint32_t)n, (uint32_t)std::max(1, ctx->num_cus) * 2u), 256u) * 256u));
+  const uint32_t nBlocks = a.nb * a.nb * a.nb;
+  // scratch: counts, cursors, offsets, chunk starts (+1), up to 8 pairs per point
+  const size_t words = (size_t)nBlocks * 4 + 1 + 8 * n;
+  uint32_t* base = nullptr;
+  int rc = FH_OK;
+  if (noalloc) {
+    if (words * 4 + 256 > kNoAllocScratchBytes)
+      return fh::fail(FH_ERR_CAPACITY, "fh_grid_post_sim_noalloc: " + std::to_string(words * 4 + 256) +
+                                           " bytes of scratch needed, more than its fixed " +
+                                           std::to_string(kNoAllocScratchBytes) +
+                                           " (about 500,000 points); use fh_grid_post_sim");
+    if ((rc = noalloc_scratch(ctx->device, &base))) return rc;
+  } else {
+    if ((rc = fh::ensure_scratch(ctx, words * 4 + 256))) return rc;
+    base = reinterpret_cast<uint32_t*>(ctx->scratch);
+  }
+  a.counts = base;
+  a.cursors = base + nBlocks;
+  a.offsets = base + 2 * (size_t)nBlocks;
+  a.chunkStart = base + 3 * (size_t)nBlocks;
+  a.pairs = base + 4 * (size_t)nBlocks + 1;
+  void* dst[2] = {density_velocity, base};
+  const uint64_t nw[2] = {cells * 4, nBlocks};  // the grid (gridPostSimPass.ts:36), block counts
+  if ((rc = fh::zero_many(ctx, 2, dst, nw, "p2g_clear"))) return rc;
+  if (n == 0) return FH_OK;
+  // points per binning workgroup: about 4 workgroups per CU, at least 1024
+  const uint32_t cus = (uint32_t)std::max(1, ctx->num_cus);
+  const uint32_t span = std::min(1u << 16, std::max(1024u, fh::div_ceil(fh::div_ceil((uint32_t)n, cus * 4u), 1024u) * 1024u));
+  const uint32_t pblocks = fh::div_ceil((uint32_t)n, span);
+  const bool lds = nBlocks <= kP2GHistMax;
+  if (lds) rc = fh::launch<P2GBin<false, true>>(ctx, pblocks, 0, {a, nBlocks, span});
+  else rc = fh::launch<P2GBin<false, false>>(ctx, pblocks, 0, {a, nBlocks, span});
+  if (rc) return rc;
+  if ((rc = fh::launch<P2GScan>(ctx, 1, 0, {a, nBlocks}))) return rc;
+  if (lds) rc = fh::launch<P2GBin<true, true>>(ctx, pblocks, 0, {a, nBlocks, span});
+  else rc = fh::launch<P2GBin<true, false>>(ctx, pblocks, 0, {a, nBlocks, span});
+  if (rc) return rc;
+  // chunks: about 2n / chunk for the pairs, plus a partial one per non-empty block
+  const uint32_t accumBlocks = std::min(fh::persistent_blocks(ctx), fh::div_ceil(2u * (uint32_t)n, a.chunk) + 64u);
+  return fh::launch<P2GAccum>(ctx, accumBlocks, 0, {a, nBlocks});
+}
+
+}  // namespace
+
+namespace fh {
+int setup_noalloc_scratch(int device) { return noalloc_scratch_setup(device); }
+}  // namespace fh
 
 extern "C" {
 
@@ -676,54 +787,16 @@ int fh_grid_post_sim(fh_ctx* ctx, uint32_t grid_dims, const fh_sim_uniforms* sim
                      const fh_hair_data* hair_data, const fh_vec4* positions_prev,
                      const fh_vec4* positions_now, fh_density_velocity* density_velocity,
                      uint32_t strands_count, uint32_t points_per_strand) {
-  FH_CHECK_CTX(ctx);
-  (void)hair_data;
-  FH_REQUIRE(grid_dims >= 2 && grid_dims <= 1024, "fh_grid_post_sim: grid_dims must be in [2, 1024]");
-  FH_REQUIRE(sim_uniforms && positions_prev && positions_now && density_velocity,
-             "fh_grid_post_sim: null buffer");
-  const uint64_t cells = (uint64_t)grid_dims * grid_dims * grid_dims;
-  const uint64_t n = (uint64_t)strands_count * points_per_strand;
-  FH_REQUIRE(n < (1ull << 28), "fh_grid_post_sim: too many points");
-  P2GArgs a;
-  a.d = grid_dims;
-  a.nb = fh::div_ceil(grid_dims, kBlk);
-  a.u = sim_uniforms;
-  a.prev = reinterpret_cast<const float4*>(positions_prev);
-  a.now = reinterpret_cast<const float4*>(positions_now);
-  a.dv = density_velocity;
-  a.nPoints = (uint32_t)n;
-  // about two chunks per CU for the whole grid, more for hot blocks
-  a.chunk = std::min(8192u, std::max(512u, fh::div_ceil(fh::div_ceil((uint32_t)n, (uint32_t)std::max(1, ctx->num_cus) * 2u), 256u) * 256u));
-  const uint32_t nBlocks = a.nb * a.nb * a.nb;
-  // scratch: counts, cursors, offsets, chunk starts (+1), up to 8 pairs per point
-  const size_t words = (size_t)nBlocks * 4 + 1 + 8 * n;
-  int rc = fh::ensure_scratch(ctx, words * 4 + 256);
-  if (rc) return rc;
-  uint32_t* base = reinterpret_cast<uint32_t*>(ctx->scratch);
-  a.counts = base;
-  a.cursors = base + nBlocks;
-  a.offsets = base + 2 * (size_t)nBlocks;
-  a.chunkStart = base + 3 * (size_t)nBlocks;
-  a.pairs = base + 4 * (size_t)nBlocks + 1;
-  void* dst[2] = {density_velocity, base};
-  const uint64_t nw[2] = {cells * 4, nBlocks};  // the grid (gridPostSimPass.ts:36), block counts
-  if ((rc = fh::zero_many(ctx, 2, dst, nw, "p2g_clear"))) return rc;
-  if (n == 0) return FH_OK;
-  // points per binning workgroup: about 4 workgroups per CU, at least 1024
-  const uint32_t cus = (uint32_t)std::max(1, ctx->num_cus);
-  const uint32_t span = std::min(1u << 16, std::max(1024u, fh::div_ceil(fh::div_ceil((uint32_t)n, cus * 4u), 1024u) * 1024u));
-  const uint32_t pblocks = fh::div_ceil((uint32_t)n, span);
-  const bool lds = nBlocks <= kP2GHistMax;
-  if (lds) rc = fh::launch<P2GBin<false, true>>(ctx, pblocks, 0, {a, nBlocks, span});
-  else rc = fh::launch<P2GBin<false, false>>(ctx, pblocks, 0, {a, nBlocks, span});
-  if (rc) return rc;
-  if ((rc = fh::launch<P2GScan>(ctx, 1, 0, {a, nBlocks}))) return rc;
-  if (lds) rc = fh::launch<P2GBin<true, true>>(ctx, pblocks, 0, {a, nBlocks, span});
-  else rc = fh::launch<P2GBin<true, false>>(ctx, pblocks, 0, {a, nBlocks, span});
-  if (rc) return rc;
-  // chunks: about 2n / chunk for the pairs, plus a partial one per non-empty block
-  const uint32_t accumBlocks = std::min(fh::persistent_blocks(ctx), fh::div_ceil(2u * (uint32_t)n, a.chunk) + 64u);
-  return fh::launch<P2GAccum>(ctx, accumBlocks, 0, {a, nBlocks});
+  return grid_post_sim(ctx, grid_dims, sim_uniforms, hair_data, positions_prev, positions_now, density_velocity,
+                       strands_count, points_per_strand, false);
+}
+
+int fh_grid_post_sim_noalloc(fh_ctx* ctx, uint32_t grid_dims, const fh_sim_uniforms* sim_uniforms,
+                             const fh_hair_data* hair_data, const fh_vec4* positions_prev,
+                             const fh_vec4* positions_now, fh_density_velocity* density_velocity,
+                             uint32_t strands_count, uint32_t points_per_strand) {
+  return grid_post_sim(ctx, grid_dims, sim_uniforms, hair_data, positions_prev, positions_now, density_velocity,
+                       strands_count, points_per_strand, true);
 }
 
 }  // extern "C"

@@ -105,6 +105,7 @@ SIGNATURES = {
     "fh_grid_pre_sim": (i32, [vp, u32, vp, vp, vp, vp, u32]),
     "fh_sim_integrate": (i32, [vp, u32, vp, vp, vp, vp, vp, vp, u32, vp, vp, vp, u32, u32]),
     "fh_grid_post_sim": (i32, [vp, u32, vp, vp, vp, vp, vp, u32, u32]),
+    "fh_grid_post_sim_noalloc": (i32, [vp, u32, vp, vp, vp, vp, vp, u32, u32]),
     "fh_tiles_clear": (i32, [vp, C.POINTER(RenderCfg), u32, u32, vp, vp, vp]),
     "fh_tiles_draw": (i32, [vp, C.POINTER(RenderCfg), u32, u32, vp, vp, vp, vp, vp, Tex2D, vp, u32, vp, vp,
                             u32, u32]),

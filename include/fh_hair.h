@@ -314,6 +314,23 @@ int fh_grid_post_sim(fh_ctx* ctx, uint32_t grid_dims,
                      fh_density_velocity* density_velocity,
                      uint32_t strands_count, uint32_t points_per_strand);
 
+/* No counterpart in the reference: fh_grid_post_sim for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). Same arguments and results; its
+ * temporaries live in a fixed 16 MiB device array of the library, one per
+ * GPU and held until the process ends, instead of the context's
+ * grow-on-demand scratch, and it calls no allocating routine. Inputs needing
+ * more (about 500,000 points; 4 * (4 * ceil(d / 8)^3 + 1 + 8 * points) + 256
+ * bytes) are refused with FH_ERR_CAPACITY before anything is enqueued. Every
+ * call on one GPU shares the array: calls that overlap on one GPU (on two
+ * streams or two contexts) race. */
+int fh_grid_post_sim_noalloc(fh_ctx* ctx, uint32_t grid_dims,
+                             const fh_sim_uniforms* sim_uniforms,
+                             const fh_hair_data* hair_data,
+                             const fh_vec4* positions_prev, const fh_vec4* positions_now,
+                             fh_density_velocity* density_velocity,
+                             uint32_t strands_count, uint32_t points_per_strand);
+
 /* ------------------------------------------------------------------------ */
 /* Software rasterizer (src/passes/swHair)                                   */
 /* ------------------------------------------------------------------------ */
