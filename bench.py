@@ -611,6 +611,7 @@ def plain_run(kind: str, steps: int, warmup: int, args, dev) -> tuple:
                             "traffic": read_traffic(k, wl.name)}
     img = r.finePass.image(wl.width, wl.height)
     assert torch.isfinite(img).all()
+    exhausted_bins = r.native.fine_exhausted_bins()  # of the last frame's fine pass
 
     ms = elapsed / steps * 1e3
     out = {
@@ -646,7 +647,8 @@ def plain_run(kind: str, steps: int, warmup: int, args, dev) -> tuple:
                        "segments_rasterized_per_s": round(wl.strands * (wl.points - 1) * 1e3 / ms, 1),
                        "segment_tile_pairs": drawn_pairs, "segment_list_capacity": capacity,
                        "dropped_pairs": max(0, drawn_pairs - capacity), "drawn_tiles": drawn_tiles,
-                       "tile_coverage": round(drawn_tiles / tiles, 4)},
+                       "tile_coverage": round(drawn_tiles / tiles, 4),
+                       "exhausted_bins": exhausted_bins},
         "kernel_us": {k: round(v * 1e3, 2) for k, v in sorted(kernel_ms.items(), key=lambda kv: -kernels[kv[0]][1])},
         "kernel_timing": f"a further timed region of {steps} frames with HIP events around every kernel "
                          f"(one graph per frame, kernels one after another)",

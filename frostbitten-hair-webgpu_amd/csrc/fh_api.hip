@@ -304,9 +304,14 @@ int fh_create(int device, fh_ctx** out) {
     return fh::fail(FH_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
   c->owns_stream = true;
-  e = hipMalloc(reinterpret_cast<void**>(&c->srgb_thresholds), 256 * sizeof(float));
+  // the sRGB table, then the pass statistics words (fh_fine_exhausted_bins)
+  e = hipMalloc(reinterpret_cast<void**>(&c->srgb_thresholds), 256 * sizeof(float) + 64 * sizeof(uint32_t));
   if (e == hipSuccess)
     e = hipMemcpy(c->srgb_thresholds, fh::srgb_thresholds(), 256 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    c->stats = reinterpret_cast<uint32_t*>(c->srgb_thresholds + 256);
+    e = hipMemset(c->stats, 0, 64 * sizeof(uint32_t));
+  }
   if (e != hipSuccess) {
     if (c->srgb_thresholds) hipFree(c->srgb_thresholds);
     hipStreamDestroy(c->stream);

@@ -84,6 +84,9 @@ struct fh_ctx {
   // The 256 thresholds of the rgba8unorm-srgb store (fh_present), uploaded
   // at creation so that no call allocates or copies them later.
   float* srgb_thresholds = nullptr;
+  // Device words the passes count into (cleared by the pass that counts):
+  // [0] bins whose slice memory the last fine pass used up.
+  uint32_t* stats = nullptr;
   // Kernel timing (fh_profiler_enable): an event pair per timed launch.
   std::vector<hipEvent_t> prof_events;
   std::vector<const char*> prof_names;  // kernel of each timed launch

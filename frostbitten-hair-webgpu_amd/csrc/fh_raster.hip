@@ -761,12 +761,14 @@ struct BinScanLocal {
     uint32_t n;
     uint32_t* ends;
     uint32_t* blockSums;
-    uint32_t* zero;  // a counter the later sort kernels count into (set to 0 here)
+    uint32_t* zero;   // a counter the later sort kernels count into (set to 0 here)
+    uint32_t* zero2;  // the fine pass's count of exhausted bins (set to 0 here)
   };
   static constexpr uint32_t kThreads = kScanBlock;
   static constexpr const char* kName = "bin_scan_local";
   static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) {
     if (block == 0 && threadIdx.x == 0 && a.zero) *a.zero = 0u;
+    if (block == 0 && threadIdx.x == 0 && a.zero2) *a.zero2 = 0u;
     bin_scan_local_body(a.tiles, a.n, a.ends, a.blockSums, block, blocks);
   }
 };
@@ -1103,6 +1105,7 @@ struct FineArgs {
   uint32_t capacity;  // elements of the gathered segment array
   uint32_t alphaQuadratic;
   float alphaMultiplier;
+  uint32_t* exhausted;  // bins whose slice memory ran out (statistics)
 };
 
 // Segment records of one batch, field-major in LDS (field f of record j at
@@ -1472,6 +1475,7 @@ __device__ __forceinline__ void fine_raster_body(FineArgs a, uint32_t block, uin
         if (s_sliceOff >= a.perProc) break;  // slice memory exhausted (:252-264)
       }
       __syncthreads();
+      if (tid == 0 && a.exhausted && s_sliceOff >= a.perProc) atomicAdd(a.exhausted, 1u);
       // reduceHairSlices — reduceHairSlices.wgsl.ts:9-73
       if (!done) {
         if (!readable) fc = mk4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -1741,7 +1745,7 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
       reinterpret_cast<const fh_list_element*>(reinterpret_cast<const char*>(tile_segments) + 4);
 
   // the scan also zeroes the long-range counter of the bin sort (large[0])
-  if ((rc = launch<BinScanLocal>(ctx, scanBlocks, 0, {tiles, nTB, ends, sums, large}))) return rc;
+  if ((rc = launch<BinScanLocal>(ctx, scanBlocks, 0, {tiles, nTB, ends, sums, large, ctx->stats}))) return rc;
   if (scanBlocks > 1 && (rc = launch<BinScanAdd>(ctx, scanBlocks, 0, {nTB, ends, sums}))) return rc;
   const uint32_t scatterBlocks = std::max(1u, std::min(fh::div_ceil(limit, 256), fh::persistent_blocks(ctx)));
   if ((rc = launch<BinScatter>(ctx, scatterBlocks, 0,
@@ -1769,6 +1773,7 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
   a.B = cfg->tileDepthBins; a.NS = cfg->slicesPerPixel; a.PS = cfg->shadingPoints;
   a.perProc = fh_slice_data_per_processor(cfg);
   a.alphaQuadratic = cfg->alphaQuadratic; a.alphaMultiplier = cfg->alphaMultiplier;
+  a.exhausted = ctx->stats;
   const uint32_t npx = T * T;
   a.batch = std::min(npx, 256u);
   // LDS: segment batch, slice heads, per-pixel state, fragment rings and
@@ -1802,6 +1807,14 @@ int fh_debug_raster_barycentric(fh_ctx* ctx, const fh_render_uniforms* uniforms,
   return launch<DebugRaster>(ctx, fh::div_ceil(nSeg, 64), 0,
                              {uniforms, reinterpret_cast<const float4*>(positions),
                               reinterpret_cast<const float4*>(tangents), strands_count, points_per_strand, out});
+}
+
+int fh_fine_exhausted_bins(fh_ctx* ctx, uint32_t* out) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(out != nullptr, "fh_fine_exhausted_bins: out is null");
+  FH_HIP(hipMemcpyAsync(out, ctx->stats, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  FH_HIP(hipStreamSynchronize(ctx->stream));
+  return FH_OK;
 }
 
 }  // extern "C"

@@ -139,6 +139,7 @@ SIGNATURES = {
     "fh_set_persistent_blocks": (i32, [vp, u32]),
     "fh_scratch_bytes": (i32, [vp, C.POINTER(C.c_uint64)]),
     "fh_scratch_reserve": (i32, [vp, C.c_uint64]),
+    "fh_fine_exhausted_bins": (i32, [vp, C.POINTER(u32)]),
     "fh_record_begin": (i32, [vp]),
     "fh_record_end": (i32, [vp, C.POINTER(vp)]),
     "fh_program_merge": (i32, [vp, C.POINTER(vp), u32, C.POINTER(vp)]),
@@ -255,6 +256,12 @@ class Context:
     def scratch_bytes(self) -> int:
         n = C.c_uint64(0)
         check(self.lib.fh_scratch_bytes(self.handle, C.byref(n)), "fh_scratch_bytes")
+        return n.value
+
+    def fine_exhausted_bins(self) -> int:
+        """Bins whose slice memory the last fine pass used up (waits for the stream)."""
+        n = u32(0)
+        check(self.lib.fh_fine_exhausted_bins(self.handle, C.byref(n)), "fh_fine_exhausted_bins")
         return n.value
 
     def scratch_reserve(self, nbytes: int) -> None:

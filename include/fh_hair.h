@@ -689,6 +689,13 @@ int fh_program_launches(const fh_program* program, uint32_t* n_out);
 int fh_program_launch(fh_ctx* ctx, const fh_program* program);
 int fh_program_destroy(fh_program* program);
 
+/* Statistics (no counterpart in the reference): the number of (tile, bin)
+ * pairs whose slice memory (avgFragmentsPerSlice * slices * tile^2
+ * fragments, hairSlicesDataBuffer.ts:27-40) the context's last
+ * fh_fine_raster used up, so that later fragments of the bin were dropped
+ * (hairFinePass.wgsl.ts:252-264). Waits for the context's stream. */
+int fh_fine_exhausted_bins(fh_ctx* ctx, uint32_t* out);
+
 /* ------------------------------------------------------------------------ */
 /* Test hook                                                                 */
 /* ------------------------------------------------------------------------ */

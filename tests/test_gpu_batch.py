@@ -136,3 +136,38 @@ def test_batch_merge_is_shorter(gpu):
     merged = rs[0].native.program_merge(progs)
     assert max(per) <= merged.launches() < sum(per)
     assert merged.launches() <= max(per) + 4  # tile-size variants of the fine pass and optional scans
+
+
+def test_batch_generator_instances(gpu):
+    """Instances of the batching workload's own generator (make_instance:
+    seeded strands, points, resolution, camera, wind and SDF per instance;
+    the five smallest of its 64), with render configurations from the
+    autotuner's candidates (tile 8 / 16 / 32): the batched frames equal each
+    instance's plain frames, replayed as graphs."""
+    import torch
+    from frostbitten_hair_webgpu_amd import assets as A
+    from frostbitten_hair_webgpu_amd import autotune as AT
+    from frostbitten_hair_webgpu_amd import batch as B
+    from frostbitten_hair_webgpu_amd import passes as PS
+    wls = [A.make_instance(i) for i in (50, 63, 33, 37, 30)]
+    for wl, cand in zip(wls, AT.CANDIDATES[::2]):
+        AT.apply(wl, cand)
+    print("instances:", [(wl.strands, wl.points, wl.width, wl.height, wl.render.tileSize) for wl in wls], flush=True)
+    plain = [PS.HairRenderer(wl, "cuda:0", shadow_size=256) for wl in wls]
+    inputs = [{k: v.cpu().numpy() for k, v in r.inputs.items()} for r in plain]
+    br = B.BatchRenderer(wls, "cuda:0", inputs=inputs, use_graph=True)
+    for r in plain:
+        r.beforeFirstFrame()
+    br.beforeFirstFrame()
+    br.build()
+    for frame in range(2):
+        for r in plain:
+            r.frame()
+        br.frame()
+        torch.cuda.synchronize()
+        for k, (rp, rb) in enumerate(zip(plain, br.renderers)):
+            a, b = _state(rp), _state(rb)
+            for key in a:
+                same = a[key] == b[key] if key in ("lists", "drawn") else np.array_equal(a[key], b[key])
+                assert same, f"frame {frame} instance {k}: {key} differs"
+            assert a["img"][..., 3].max() > 0.0

@@ -194,6 +194,7 @@ def test_c3_full_size(gpu, oracle_mt):
     assert np.isfinite(st["img"]).all()
     count, cap = _check_render(r, wl, oracle_mt, st)
     assert count <= cap, f"{count - cap} pairs dropped"
+    _say(f"bins whose slice memory ran out: {r.native.fine_exhausted_bins()}")
     T = -(-wl.width // wl.render.tileSize) * -(-wl.height // wl.render.tileSize)
     cover = int(st["tl"][0]) / T
     _say(f"tiles covered: {cover:.3f}")
@@ -210,6 +211,7 @@ def test_slice_pool_exhaustion(gpu, oracle_mt, tile):
     wl, r = _renderer("c2", 4000, tileSize=tile, avgFragmentsPerSlice=1)
     _frames(r, 2)
     st = _render(r)
+    exhausted = r.native.fine_exhausted_bins()
     tp = r.tilesPass
     i = r.inputs
     pos, tan = _np(r.hair.positionsNow), _np(r.hair.tangents)
@@ -224,8 +226,8 @@ def test_slice_pool_exhaustion(gpu, oracle_mt, tile):
                                  canonical=True)
     a_exp, a_full = O_fine[4:].reshape(-1, 4)[:, 3], full[4:].reshape(-1, 4)[:, 3]
     lost = (a_full - a_exp > 1e-3).sum()
-    _say(f"tile {tile}: pixels that lose coverage to the pool limit: {lost}")
-    assert lost > 1000  # the limit bites
+    _say(f"tile {tile}: pixels that lose coverage to the pool limit: {lost}; exhausted bins: {exhausted}")
+    assert lost > 1000 and exhausted > 0  # the limit bites
     ge = st["img"][4:]
     assert np.isfinite(ge).all()
     if tile == 8:

@@ -29,12 +29,12 @@ def _requests():
     return [SV.Request(i, a, s, 0.0) for i, (a, s) in enumerate(order)]
 
 
-def _serve(pool, max_active, budget, pipelined=False):
+def _serve(pool, max_active, budget, pipelined=False, requests=None):
     from frostbitten_hair_webgpu_amd import serve as SV
     print(f"serve: max_active={max_active} budget={budget} pipelined={pipelined}", flush=True)
     srv = SV.HairServer(pool, budget, max_active=max_active, device="cuda:0", keep_images=True,
                         pipelined=pipelined)
-    reqs = srv.run(_requests(), realtime=False)
+    reqs = srv.run(requests() if requests else _requests(), realtime=False)
     stats = dict(srv.stats)
     srv.close()
     return reqs, stats
@@ -68,3 +68,28 @@ def test_serve_eviction_restores_state(gpu, pipelined):
     assert st["evictions"] > 0
     for a, b in zip(alone, tight):
         assert np.array_equal(a.image.cpu().numpy(), b.image.cpu().numpy()), f"request {a.rid} differs"
+
+
+def test_serve_generator_assets(gpu):
+    """Assets of the served workload's own pool generator (make_pool_asset:
+    seeded strands, 16-32 points, 720p here; the four smallest of its 256)
+    and a Poisson request stream of its own generator (poisson_requests,
+    frame counts U{1..12}): each request's image is the same alone and in
+    the mix, with pipelined ticks."""
+    from frostbitten_hair_webgpu_amd import assets as A
+    from frostbitten_hair_webgpu_amd import passes as PS
+    from frostbitten_hair_webgpu_amd import serve as SV
+    pool = {}
+    for a in (51, 111, 74, 116):
+        wl = A.make_pool_asset(a)
+        pool[a] = (wl, PS.make_inputs(wl, PS.workload_render_uniforms(wl), 256))
+
+    def requests():
+        return SV.poisson_requests(10, 50.0, sorted(pool), 12, seed=A.SERVE_SEED)
+
+    alone, st_alone = _serve(pool, 1, 1 << 40, requests=requests)
+    mix, st_mix = _serve(pool, 8, 1 << 40, pipelined=True, requests=requests)
+    assert st_mix["ticks"] < st_alone["ticks"]
+    for a, b in zip(alone, mix):
+        assert np.array_equal(a.image.cpu().numpy(), b.image.cpu().numpy()), f"request {a.rid} differs"
+        assert a.image[..., 3].max() > 0
