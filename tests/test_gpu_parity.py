@@ -125,8 +125,9 @@ def test_grid_pre_sim(gpu, oracle, kind, strands):
                                                         ("c2", 200, 2, 40), ("c2", 130, 2, 64)])
 def test_sim_integrate_step(gpu, oracle, kind, strands, steps, points):
     """One step from an identical state: positions within 1e-5 relative.
-    Strands of up to 32 points take the register-resident sweeps, longer
-    ones (40, 64) the LDS-staged kernel."""
+    Strands of 5 to 64 points, staged in LDS with 16 or 32 strands per
+    workgroup (the small strand counts take 16, so that every CU gets two
+    workgroups)."""
     import torch
     from frostbitten_hair_webgpu_amd import assets as A
     wl, r = _renderer(kind, strands, points=points)
