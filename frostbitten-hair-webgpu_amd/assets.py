@@ -363,6 +363,94 @@ def _sphere_trace(col: Collider, origins: np.ndarray, dirs: np.ndarray, t_max: f
     return t, hit
 
 
+def _oriented_triangles(pos: np.ndarray, outward: np.ndarray, quads: list) -> np.ndarray:
+    """Triangles (T, 3) of the quads (a, b, c, d) of a surface grid, each
+    wound counter-clockwise seen from outside (its geometric normal along the
+    outward normal at its centroid); zero-area triangles at the poles are
+    dropped."""
+    tris = []
+    for a, b, c, d in quads:
+        for t in ((a, b, c), (a, c, d)):
+            p0, p1, p2 = pos[t[0]], pos[t[1]], pos[t[2]]
+            n = np.cross(p1 - p0, p2 - p0)
+            if np.dot(n, n) < 1e-24:
+                continue
+            if np.dot(n, outward[list(t)].sum(axis=0)) < 0:
+                t = (t[0], t[2], t[1])
+            tris.append(t)
+    return np.asarray(tris, dtype=np.uint32)
+
+
+def _surface_mesh(center_of, axis: np.ndarray, radius: float, rings: int, segments: int) -> dict:
+    """A closed surface of revolution about `axis`: vertex (i, j) sits at
+    center_of(i) + radius * n(theta_i, phi_j) with n the unit sphere's normal,
+    theta_i from 0 (along +axis) to pi in `rings` steps."""
+    axis = axis / np.linalg.norm(axis)
+    v = np.cross(axis, [1.0, 0.0, 0.0] if abs(axis[0]) < 0.9 else [0.0, 1.0, 0.0])
+    v /= np.linalg.norm(v)
+    w = np.cross(axis, v)
+    pos, nrm = [], []
+    thetas = center_of("thetas", rings)
+    for i, th in enumerate(thetas):
+        for j in range(segments + 1):
+            ph = 2.0 * math.pi * j / segments
+            n = math.cos(th) * axis + math.sin(th) * (math.cos(ph) * v + math.sin(ph) * w)
+            pos.append(center_of(i, rings) + radius * n)
+            nrm.append(n)
+    pos, nrm = np.asarray(pos), np.asarray(nrm)
+    quads = []
+    for i in range(len(thetas) - 1):
+        for j in range(segments):
+            a = i * (segments + 1) + j
+            quads.append((a, a + segments + 1, a + segments + 2, a + 1))
+    idx = _oriented_triangles(pos, nrm, quads)
+    return {"positions": pos.astype(np.float32), "normals": nrm.astype(np.float32), "indices": idx}
+
+
+def sphere_mesh(center, radius: float, rings: int = 24, segments: int = 48) -> dict:
+    """A UV sphere (positions, normals float32 (V, 3), indices uint32 (T, 3),
+    counter-clockwise seen from outside)."""
+    c = np.asarray(center, dtype=np.float64)
+
+    def center_of(i, n):
+        if i == "thetas":
+            return [math.pi * k / n for k in range(n + 1)]
+        return c
+
+    return _surface_mesh(center_of, np.array([0.0, 1.0, 0.0]), radius, rings, segments)
+
+
+def capsule_mesh(a, b, radius: float, rings: int = 24, segments: int = 48) -> dict:
+    """A capsule (the cylinder a-b with hemispherical caps): the equator ring
+    is repeated, once on each cap's centre, which makes the cylinder."""
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    half = rings // 2
+    thetas = [math.pi * k / (2 * half) for k in range(half + 1)] + \
+             [math.pi * k / (2 * half) for k in range(half, 2 * half + 1)]
+
+    def center_of(i, n):
+        if i == "thetas":
+            return thetas
+        return b if i <= half else a
+
+    return _surface_mesh(center_of, b - a, radius, rings, segments)
+
+
+def scene_meshes(col: Collider, collision_sphere=None, rings: int = 24, segments: int = 48) -> list:
+    """The scene's objects (Scene.objects, loadScene.ts:25-60) for a
+    collider: one mesh per sphere and capsule, in that order, then (when
+    `collision_sphere` (x, y, z, r) is given) the collider preview, a unit
+    sphere drawn with collisionSphereModelMatrix (loadScene.ts:36,
+    CONFIG.drawColliders)."""
+    out = [sphere_mesh(c, r, rings, segments) for c, r in col.spheres]
+    out += [capsule_mesh(a, b, r, rings, segments) for a, b, r in col.capsules]
+    if collision_sphere is not None:
+        m = sphere_mesh((0.0, 0.0, 0.0), 1.0, rings, segments)
+        m["preview"] = True
+        out.append(m)
+    return out
+
+
 def mesh_depth(wl: Workload, uniforms: np.ndarray) -> np.ndarray:
     """Depth texture (H x W, texture rows top to bottom) of the collider meshes
     as drawMeshesPass would leave it; 1.0 where nothing is drawn."""

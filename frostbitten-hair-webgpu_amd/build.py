@@ -17,7 +17,8 @@ CSRC = PKG_DIR / "csrc"
 LIB_DIR = PKG_DIR / "lib"
 BUILD_DIR = PKG_DIR / "_build"
 LIB_PATH = LIB_DIR / "libfhhip.so"
-SOURCES = ["fh_api.hip", "fh_sim.hip", "fh_raster.hip", "fh_shading.hip", "fh_post.hip", "fh_hwraster.hip", "fh_uniforms.hip"]
+SOURCES = ["fh_api.hip", "fh_sim.hip", "fh_raster.hip", "fh_shading.hip", "fh_post.hip", "fh_hwraster.hip", "fh_meshes.hip",
+           "fh_uniforms.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 COMMON_FLAGS = [
@@ -50,7 +51,7 @@ def build(verbose: bool = False) -> Path:
         obj = BUILD_DIR / (Path(src).stem + ".o")
         objs.append(obj)
         cmds.append([HIPCC, *COMMON_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
-    with ThreadPoolExecutor(max_workers=min(7, len(cmds))) as ex:
+    with ThreadPoolExecutor(max_workers=min(8, len(cmds))) as ex:
         list(ex.map(_run, cmds))
     tmp = LIB_PATH.with_suffix(".so.tmp")
     # --no-undefined: a missing definition fails the build, not the first dlopen

@@ -138,6 +138,19 @@ __device__ __forceinline__ float tex_load(const float* __restrict__ t, uint32_t 
   return t[(size_t)y * w + (uint32_t)x];
 }
 
+// textureGather on a clamp-to-edge sampler, averaged as (x + y + z + w) / 4
+// with components (i0,j1), (i1,j1), (i1,j0), (i0,j0).
+__device__ __forceinline__ float gather_avg(const float* __restrict__ t, uint32_t w, uint32_t h, f2 uv) {
+  const float x = uv.x * (float)w - 0.5f, y = uv.y * (float)h - 0.5f;
+  int i0 = (int)floorf(x), j0 = (int)floorf(y);
+  int i1 = i0 + 1, j1 = j0 + 1;
+  i0 = min(max(i0, 0), (int)w - 1); i1 = min(max(i1, 0), (int)w - 1);
+  j0 = min(max(j0, 0), (int)h - 1); j1 = min(max(j1, 0), (int)h - 1);
+  const float gx = t[(size_t)j1 * w + i0], gy = t[(size_t)j1 * w + i1];
+  const float gz = t[(size_t)j0 * w + i1], gw = t[(size_t)j0 * w + i0];
+  return (gx + gy + gz + gw) / 4.0f;
+}
+
 // IEEE binary16 pack / unpack, round to nearest even (pack2x16float).
 __device__ __forceinline__ uint32_t pack_half2(float a, float b) {
   _Float16 ha = (_Float16)a, hb = (_Float16)b;

@@ -7,24 +7,18 @@
 // shadowMapPass/shadowMapHairPass.wgsl.ts, shadowMapPass/shadowMapPass.ts
 // :100-115, aoPass/aoPass.wgsl.ts:48-79, aoPass/gtao.wgsl.ts:26-136.
 //
-// Rasterization. Each segment is the reference's two triangles (indices
-// 2i, 2i+1, 2i+2 and 2i+2, 2i+1, 2i+3 over the strand's left / right edge
-// vertices). Vertices are snapped to 1/256 px; a pixel centre is covered when
-// the three integer edge functions are positive, or zero on a top or left
-// edge, so triangles sharing an edge cover each pixel once. Depth is the
-// screen-space barycentric interpolation of NDC z; fragments outside
-// [0, 1] are clipped, triangles with a vertex at w <= 0 or beyond 2^20 px are
-// not drawn. The depth test is the pipelines' 'less' (shared.ts:22-26)
-// against the depth already in the texture (the meshes), and among the hair
-// fragments of one pixel the least depth wins, the earliest triangle on a
-// tie, as in API order: each fragment does one 64-bit atomicMin of (depth
-// bits, triangle index) into a per-pixel key, and a resolve pass writes the
-// winner's depth and its perspective-correct normal. The result does not
-// depend on the order the triangles run in.
+// Rasterization (the rules are in fh_tri_raster.hpp). Each segment is the
+// reference's two triangles (indices 2i, 2i+1, 2i+2 and 2i+2, 2i+1, 2i+3 over
+// the strand's left / right edge vertices). The depth test is the pipelines'
+// 'less' (shared.ts:22-26) against the depth already in the texture (the
+// meshes), and among the hair fragments of one pixel the least depth wins,
+// the earliest triangle on a tie, as in API order; a resolve pass writes the
+// winner's depth and its perspective-correct normal.
 #include <algorithm>
 
 #include "fh_device.hpp"
 #include "fh_launch.hpp"
+#include "fh_tri_raster.hpp"
 
 namespace fh {
 int fill_u32(fh_ctx* ctx, void* dst, uint64_t n_words, uint32_t value);
@@ -33,37 +27,15 @@ int fill_u32(fh_ctx* ctx, void* dst, uint64_t n_words, uint32_t value);
 namespace {
 
 using namespace fh;
+using namespace fh::tri;
 
 constexpr float kFloatEps = 1e-7f;  // FLOAT_EPSILON (shaderSnippets.wgls.ts:67)
 constexpr float kPi = 3.14159265358979323846f;
 constexpr float kHalfPi = 1.57079632679489661923f;
 constexpr float kNear = 0.01f, kFar = 100.0f;  // CONFIG.camera.projection (constants.ts:131-135)
-constexpr uint64_t kNoHair = ~0ull;
 
 __device__ __forceinline__ f3 safe_normalize3(f3 v) {
   return dot(v, v) >= kFloatEps ? normalize(v) : mk3(0.0f, 0.0f, 0.0f);
-}
-
-// encodeOctahedronNormal (shaderSnippets.wgls.ts:25-46), packed as rg16float
-__device__ __forceinline__ uint32_t encode_oct(f3 n0) {
-  const float s = fabsf(n0.x) + fabsf(n0.y) + fabsf(n0.z);
-  f3 n = mk3(n0.x / s, n0.y / s, n0.z / s);
-  if (n.z < 0.0f) {
-    const float sx = n.x >= 0.0f ? 1.0f : -1.0f, sy = n.y >= 0.0f ? 1.0f : -1.0f;
-    const float ax = (1.0f - fabsf(n.y)) * sx, ay = (1.0f - fabsf(n.x)) * sy;
-    n.x = ax;
-    n.y = ay;
-  }
-  return pack_half2(n.x * 0.5f + 0.5f, n.y * 0.5f + 0.5f);
-}
-
-__device__ __forceinline__ f3 decode_oct(uint32_t packed) {
-  const float fx = half_lo(packed) * 2.0f - 1.0f, fy = half_hi(packed) * 2.0f - 1.0f;
-  f3 n = mk3(fx, fy, 1.0f - fabsf(fx) - fabsf(fy));
-  const float t = satf(-n.z);
-  n.x = n.x >= 0.0f ? n.x - t : n.x + t;
-  n.y = n.y >= 0.0f ? n.y - t : n.y + t;
-  return normalize(n);
 }
 
 struct RasterArgs {
@@ -72,7 +44,7 @@ struct RasterArgs {
   const float4* tan;
   float* depth;             // the depth texture: tested ('less'), then written by the resolve
   uint32_t* normals;        // rg16float, may be null (shadow map: depth only)
-  unsigned long long* key;  // per pixel: (depth bits << 32) | triangle, kNoHair when empty
+  unsigned long long* key;  // per pixel: (depth bits << 32) | triangle, kNoFragment when empty
   uint32_t* bigCount;       // queued bands of large triangles
   uint2* big;               // (triangle, band)
   uint32_t W, H, P, nTri, shadow;
@@ -105,50 +77,10 @@ __device__ __forceinline__ void tri_vertices(uint32_t t, uint32_t P, uint32_t* i
   }
 }
 
-struct SetupTri {
-  f4 c[3];           // clip positions, counter-clockwise order after setup
-  long long X[3], Y[3];
-  long long area;
-  uint32_t vi[3];    // original vertex slot of each setup vertex
-  bool ok;
-};
-
-// Snap to 1/256 px and orient so that the edge functions are positive inside.
-__device__ __forceinline__ SetupTri setup_from(const RasterArgs& a, f4 c0, f4 c1, f4 c2) {
-  SetupTri s;
-  s.c[0] = c0; s.c[1] = c1; s.c[2] = c2;
-  s.ok = true;
-  for (int k = 0; k < 3; k++) {
-    s.vi[k] = (uint32_t)k;
-    const f4 c = s.c[k];
-    s.X[k] = s.Y[k] = 0;
-    if (!(c.w > 0.0f)) { s.ok = false; continue; }
-    const float nx = c.x / c.w, ny = c.y / c.w;
-    const float xs = (float)a.W * (nx * 0.5f + 0.5f);
-    const float ys = (float)a.H * (1.0f - (ny * 0.5f + 0.5f));
-    if (!(fabsf(xs) < 1048576.0f && fabsf(ys) < 1048576.0f)) { s.ok = false; continue; }
-    s.X[k] = (long long)rintf(xs * 256.0f);
-    s.Y[k] = (long long)rintf(ys * 256.0f);
-  }
-  s.area = 0;
-  if (!s.ok) return s;
-  long long area = (s.X[1] - s.X[0]) * (s.Y[2] - s.Y[0]) - (s.Y[1] - s.Y[0]) * (s.X[2] - s.X[0]);
-  if (area < 0) {
-    const f4 c = s.c[1]; s.c[1] = s.c[2]; s.c[2] = c;
-    long long q = s.X[1]; s.X[1] = s.X[2]; s.X[2] = q;
-    q = s.Y[1]; s.Y[1] = s.Y[2]; s.Y[2] = q;
-    const uint32_t v = s.vi[1]; s.vi[1] = s.vi[2]; s.vi[2] = v;
-    area = -area;
-  }
-  s.area = area;
-  s.ok = area > 0;
-  return s;
-}
-
 __device__ __forceinline__ SetupTri setup_tri(const RasterArgs& a, uint32_t t) {
   uint32_t idx[3], side[3];
   tri_vertices(t, a.P, idx, side);
-  return setup_from(a, edge_vertex(a, idx[0], side[0]), edge_vertex(a, idx[1], side[1]),
+  return setup_from(a.W, a.H, edge_vertex(a, idx[0], side[0]), edge_vertex(a, idx[1], side[1]),
                     edge_vertex(a, idx[2], side[2]));
 }
 
@@ -169,119 +101,11 @@ __device__ __forceinline__ void point_corners(const RasterArgs& a, uint32_t idx,
   minus = mul(proj, mk4(em.x, em.y, em.z, 1.0f));
 }
 
-__device__ __forceinline__ long long edge_fn_i(long long ax, long long ay, long long bx, long long by, long long px,
-                                               long long py) {
-  return (bx - ax) * (py - ay) - (by - ay) * (px - ax);
+// The hair triangles' rows: the texture's depth is the 'less' reference.
+__device__ __forceinline__ void hair_rows(const RasterArgs& a, const SetupTri& s, uint32_t t, long long first,
+                                          long long stride) {
+  raster_rows(s, t, first, stride, a.W, a.H, a.depth, a.key);
 }
-// top edge (horizontal, interior below) or left edge (interior to the right)
-__device__ __forceinline__ bool top_left(long long ax, long long ay, long long bx, long long by) {
-  const long long dx = bx - ax, dy = by - ay;
-  return (dy == 0 && dx > 0) || dy < 0;
-}
-__device__ __forceinline__ bool covers(long long e, bool tl) { return e > 0 || (e == 0 && tl); }
-
-// screen-space barycentrics of a covered pixel centre
-__device__ __forceinline__ void barycentrics(const SetupTri& s, long long px, long long py, float* l) {
-  const long long e0 = edge_fn_i(s.X[1], s.Y[1], s.X[2], s.Y[2], px, py);
-  const long long e1 = edge_fn_i(s.X[2], s.Y[2], s.X[0], s.Y[0], px, py);
-  const long long e2 = edge_fn_i(s.X[0], s.Y[0], s.X[1], s.Y[1], px, py);
-  const float ar = (float)s.area;
-  l[0] = (float)e0 / ar;
-  l[1] = (float)e1 / ar;
-  l[2] = (float)e2 / ar;
-}
-
-__device__ __forceinline__ float frag_depth(const SetupTri& s, const float* l) {
-  const float z0 = s.c[0].z / s.c[0].w, z1 = s.c[1].z / s.c[1].w, z2 = s.c[2].z / s.c[2].w;
-  return l[0] * z0 + l[1] * z1 + l[2] * z2;
-}
-
-__device__ __forceinline__ long long floor_div256(long long v) { return v >= 0 ? v / 256 : -((-v + 255) / 256); }
-
-// Conservative pixel range [x0, x1] of row centre cy inside edge a -> b
-// (E(cx) = dx (cy - ay) - dy (cx - ax) >= 0), two pixels wider on each side
-// than the bound evaluated in f32 (whose error is far below a pixel); the
-// caller still tests each pixel exactly.
-__device__ __forceinline__ void edge_span(long long ax, long long ay, long long bx, long long by, long long cy,
-                                          long long& x0, long long& x1) {
-  const long long dx = bx - ax, dy = by - ay;
-  const long long c = dx * (cy - ay) + dy * ax;  // E(cx) = c - dy * cx
-  if (dy == 0) {
-    if (c < 0) { x0 = 1; x1 = 0; }
-    return;
-  }
-  const float cxb = (float)c * __builtin_amdgcn_rcpf((float)dy);  // the cx where E = 0
-  const float pxb = fminf(fmaxf((cxb - 128.0f) * (1.0f / 256.0f), -1.0e9f), 1.0e9f);
-  if (dy < 0) x0 = max(x0, (long long)floorf(pxb) - 2);  // E grows with cx
-  else x1 = min(x1, (long long)floorf(pxb) + 2);          // E falls with cx
-}
-
-// Pixel box of a set-up triangle: centres 256 p + 128 inside its snapped
-// bounds, clipped to the target (empty when x0 > x1 or y0 > y1).
-struct PixBox {
-  long long x0, x1, y0, y1;
-};
-__device__ __forceinline__ PixBox pix_box(const RasterArgs& a, const SetupTri& s) {
-  const long long minX = min(s.X[0], min(s.X[1], s.X[2])), maxX = max(s.X[0], max(s.X[1], s.X[2]));
-  const long long minY = min(s.Y[0], min(s.Y[1], s.Y[2])), maxY = max(s.Y[0], max(s.Y[1], s.Y[2]));
-  PixBox b;
-  b.x0 = max(0ll, -floor_div256(128 - minX));
-  b.x1 = min((long long)a.W - 1, floor_div256(maxX - 128));
-  b.y0 = max(0ll, -floor_div256(128 - minY));
-  b.y1 = min((long long)a.H - 1, floor_div256(maxY - 128));
-  return b;
-}
-
-// The fragments of triangle t on row py between pixels xa and xb: a
-// conservative span from the three edge functions, then the exact edge
-// tests stepped along the row, the depth clip, the 'less' test against the
-// texture and one atomicMin per fragment.
-__device__ __forceinline__ void raster_row(const RasterArgs& a, const SetupTri& s, uint32_t t, long long py,
-                                           long long xa, long long xb) {
-  const long long cy = py * 256 + 128;
-  edge_span(s.X[1], s.Y[1], s.X[2], s.Y[2], cy, xa, xb);
-  edge_span(s.X[2], s.Y[2], s.X[0], s.Y[0], cy, xa, xb);
-  edge_span(s.X[0], s.Y[0], s.X[1], s.Y[1], cy, xa, xb);
-  if (xa > xb) return;
-  const bool tl0 = top_left(s.X[1], s.Y[1], s.X[2], s.Y[2]);
-  const bool tl1 = top_left(s.X[2], s.Y[2], s.X[0], s.Y[0]);
-  const bool tl2 = top_left(s.X[0], s.Y[0], s.X[1], s.Y[1]);
-  // edge values at the first centre, then one step of -256 dy per pixel
-  const long long cx0 = xa * 256 + 128;
-  long long e0 = edge_fn_i(s.X[1], s.Y[1], s.X[2], s.Y[2], cx0, cy);
-  long long e1 = edge_fn_i(s.X[2], s.Y[2], s.X[0], s.Y[0], cx0, cy);
-  long long e2 = edge_fn_i(s.X[0], s.Y[0], s.X[1], s.Y[1], cx0, cy);
-  const long long d0 = -256 * (s.Y[2] - s.Y[1]), d1 = -256 * (s.Y[0] - s.Y[2]), d2 = -256 * (s.Y[1] - s.Y[0]);
-  for (long long px = xa; px <= xb; px++, e0 += d0, e1 += d1, e2 += d2) {
-    if (!(covers(e0, tl0) && covers(e1, tl1) && covers(e2, tl2))) continue;
-    float l[3];
-    barycentrics(s, px * 256 + 128, cy, l);
-    float z = frag_depth(s, l);
-    if (!(z >= 0.0f && z <= 1.0f)) continue;  // depth clip
-    z = z + 0.0f;                              // -0 -> +0
-    const size_t p = (size_t)py * a.W + (size_t)px;
-    if (!(z < a.depth[p])) continue;           // 'less' against the meshes
-    atomicMin(&a.key[p], ((unsigned long long)__float_as_uint(z) << 32) | t);
-  }
-}
-
-__device__ __forceinline__ void raster_rows(const RasterArgs& a, const SetupTri& s, uint32_t t, long long first,
-                                            long long stride) {
-  const PixBox b = pix_box(a, s);
-  for (long long py = b.y0 + first; py <= b.y1; py += stride) raster_row(a, s, t, py, b.x0, b.x1);
-}
-
-// kRowSplit threads per triangle, thread k taking rows k, k + kRowSplit, ...
-// (a tall triangle does not hold one lane while the rest of its wave idles).
-// A triangle whose pixel box is larger than kWaveRaster pixels (a strand
-// passing close to the eye) is queued in bands of kBandRows rows, each drawn
-// by a whole workgroup of HairRasterBig; when the queue is full, by the
-// triangle's own wave, one row per lane.
-constexpr uint32_t kRowSplit = 8;
-constexpr long long kWaveRaster = 1024;
-constexpr uint32_t kBandRows = 64;
-constexpr uint32_t kBigCap = 1u << 20;  // queued bands
-constexpr uint32_t kBigSentinel = 0xffffffffu;  // an unused band slot
 
 struct HairRaster {
   using Args = RasterArgs;
@@ -300,13 +124,13 @@ struct HairRaster {
       point_corners(a, ia + 1u, bp, bm);
       for (uint32_t h = 0; h < 2u; h++) {
         const uint32_t t = 2u * seg + h;
-        const SetupTri s = h == 0u ? setup_from(a, ap, am, bp) : setup_from(a, bp, am, bm);
+        const SetupTri s = h == 0u ? setup_from(a.W, a.H, ap, am, bp) : setup_from(a.W, a.H, bp, am, bm);
         if (!s.ok) continue;
-        const PixBox b = pix_box(a, s);
+        const PixBox b = pix_box(a.W, a.H, s);
         const long long bw = b.x1 - b.x0 + 1, bh = b.y1 - b.y0 + 1;
         if (bw <= 0 || bh <= 0) continue;
         if (bw * bh <= kWaveRaster) {
-          raster_rows(a, s, t, (long long)k, (long long)kRowSplit);
+          hair_rows(a, s, t, (long long)k, (long long)kRowSplit);
         } else if (k == 0u) {
           const uint32_t bands = (uint32_t)((bh + kBandRows - 1) / kBandRows);
           const uint32_t first = atomicAdd(a.bigCount, bands);
@@ -331,7 +155,7 @@ struct HairRaster {
         todo &= todo - 1ull;
         const uint32_t tb = 2u * (uint32_t)__builtin_amdgcn_readlane((int)seg, (int)src) + h;
         const SetupTri sb = setup_tri(a, tb);
-        raster_rows(a, sb, tb, (long long)lane, 64);
+        hair_rows(a, sb, tb, (long long)lane, 64);
       }
     }
   }
@@ -350,14 +174,14 @@ struct HairRasterBig {
       const uint2 e = a.big[i];
       if (e.x >= a.nTri) continue;  // kBigSentinel
       const SetupTri s = setup_tri(a, e.x);
-      const PixBox b = pix_box(a, s);
+      const PixBox b = pix_box(a.W, a.H, s);
       const long long y0 = b.y0 + (long long)e.y * kBandRows, y1 = min(b.y1, y0 + kBandRows - 1);
       const long long cpr = (b.x1 - b.x0) / 64 + 1;
       const long long items = (y1 - y0 + 1) * cpr;
       for (long long it = threadIdx.x; it < items; it += kThreads) {
         const long long row = it / cpr, chunk = it - row * cpr;
         const long long xa = b.x0 + chunk * 64;
-        raster_row(a, s, e.x, y0 + row, xa, min(b.x1, xa + 63));
+        raster_row(s, e.x, y0 + row, xa, min(b.x1, xa + 63), a.W, a.depth, a.key);
       }
     }
   }
@@ -375,7 +199,7 @@ struct HairResolve {
     const uint32_t p = block * kThreads + threadIdx.x;
     if (p >= a.W * a.H) return;
     const unsigned long long k = a.key[p];
-    if (k == kNoHair) return;
+    if (k == kNoFragment) return;
     const uint32_t t = (uint32_t)k;
     a.depth[p] = __uint_as_float((uint32_t)(k >> 32));
     if (!a.normals) return;

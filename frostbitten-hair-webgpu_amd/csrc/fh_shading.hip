@@ -84,19 +84,6 @@ __device__ f3 hair_light(const Marschner& p, const fh_light& light, f3 toCam, f3
   return (brdf * radiance) * (1.0f - ao);
 }
 
-// textureGather on a clamp-to-edge sampler, averaged as (x + y + z + w) / 4
-// with components (i0,j1), (i1,j1), (i1,j0), (i0,j0).
-__device__ __forceinline__ float gather_avg(const float* __restrict__ t, uint32_t w, uint32_t h, f2 uv) {
-  const float x = uv.x * (float)w - 0.5f, y = uv.y * (float)h - 0.5f;
-  int i0 = (int)floorf(x), j0 = (int)floorf(y);
-  int i1 = i0 + 1, j1 = j0 + 1;
-  i0 = min(max(i0, 0), (int)w - 1); i1 = min(max(i1, 0), (int)w - 1);
-  j0 = min(max(j0, 0), (int)h - 1); j1 = min(max(j1, 0), (int)h - 1);
-  const float gx = t[(size_t)j1 * w + i0], gy = t[(size_t)j1 * w + i1];
-  const float gz = t[(size_t)j0 * w + i1], gw = t[(size_t)j0 * w + i0];
-  return (gx + gy + gz + gw) / 4.0f;
-}
-
 __device__ __forceinline__ float linearize_depth(float d) { return kNear * kFar / (kFar + (kNear - kFar) * d); }
 
 struct ShadeArgs {

@@ -80,6 +80,12 @@ class SimParams(C.Structure):
     ]
 
 
+class Mesh(C.Structure):
+    """struct fh_mesh (device pointers)"""
+    _fields_ = [("positions", vp), ("normals", vp), ("indices", vp), ("vertex_count", u32),
+                ("triangle_count", u32), ("is_collider_preview", u32), ("reserved", u32)]
+
+
 class KernelTime(C.Structure):
     """struct fh_kernel_time"""
     _fields_ = [("name", C.c_char * 48), ("launches", u32), ("reserved", u32), ("total_ms", C.c_double)]
@@ -123,6 +129,12 @@ SIGNATURES = {
     "fh_hw_hair": (i32, [vp, u32, u32, vp, vp, vp, vp, vp, u32, u32]),
     "fh_shadow_map_hair": (i32, [vp, vp, vp, vp, vp, u32, u32, u32]),
     "fh_calc_ao": (i32, [vp, vp, Tex2D, vp, vp, u32, u32]),
+    "fh_meshes_create": (i32, [vp, C.POINTER(Mesh), u32, C.POINTER(vp)]),
+    "fh_meshes_destroy": (i32, [vp]),
+    "fh_meshes_triangles": (i32, [vp, C.POINTER(u32)]),
+    "fh_draw_background": (i32, [vp, u32, u32, vp, vp]),
+    "fh_draw_meshes": (i32, [vp, u32, u32, vp, vp, Tex2D, Tex2D, vp, vp, vp]),
+    "fh_shadow_map_meshes": (i32, [vp, vp, vp, vp, u32]),
     "fh_pack_render_uniforms": (i32, [C.POINTER(RenderParams), vp]),
     "fh_pack_sim_uniforms": (i32, [C.POINTER(SimParams), vp]),
     "fh_profiler_enable": (i32, [vp, u32]),
@@ -351,6 +363,44 @@ class Graph:
     def close(self) -> None:
         if self.handle:
             self.ctx.lib.fh_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Meshes:
+    """An fh_meshes object list over device buffers: `meshes` is a list of
+    dicts with positions and normals (V, 3) float32 and indices (T, 3) int32
+    or uint32 tensors on the context's device, and `preview` (bool). The
+    tensors are kept alive with the object."""
+
+    def __init__(self, ctx: Context, meshes: list):
+        self.ctx = ctx
+        self._keep = []
+        rows = []
+        for m in meshes:
+            pos, nrm, idx = m["positions"].contiguous(), m["normals"].contiguous(), m["indices"].contiguous()
+            assert pos.shape == nrm.shape and pos.shape[-1] == 3 and idx.shape[-1] == 3
+            self._keep += [pos, nrm, idx]
+            rows.append(Mesh(pos.data_ptr(), nrm.data_ptr(), idx.data_ptr(), pos.shape[0], idx.shape[0],
+                             1 if m.get("preview") else 0, 0))
+        arr = (Mesh * max(1, len(rows)))(*rows)
+        h = vp()
+        check(load().fh_meshes_create(ctx.handle, arr, len(rows), C.byref(h)), "fh_meshes_create")
+        self.handle = h
+
+    def triangles(self) -> int:
+        n = u32()
+        check(load().fh_meshes_triangles(self.handle, C.byref(n)), "fh_meshes_triangles")
+        return n.value
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            check(load().fh_meshes_destroy(self.handle), "fh_meshes_destroy")
             self.handle = None
 
     def __del__(self):

@@ -290,16 +290,54 @@ class PresentPass:
 
 
 class ShadowMapPass:
-    """The hair draw of src/passes/shadowMapPass/shadowMapPass.ts:100-115 into
-    the shadow map; the meshes' part (drawn first, :94-99) is an input."""
+    """src/passes/shadowMapPass/shadowMapPass.ts:83-133: the meshes (drawn
+    first, :94-99), then the hair (:100-115) into the shadow map. With
+    `meshes` None the meshes' depth is the precomputed `meshShadowDepth`."""
     NAME = "ShadowMapPass"
 
-    def cmdUpdateShadowMap(self, ctx: PassCtx, hair: HairObject, meshShadowDepth: torch.Tensor) -> None:
+    def cmdUpdateShadowMap(self, ctx: PassCtx, hair: HairObject, meshShadowDepth: torch.Tensor | None = None,
+                           meshes: "N.Meshes | None" = None) -> None:
         sm = ctx.shadowDepthTexture
-        ctx.native.call("fh_copy", N.Tex2D.ptr(sm), N.ptr(meshShadowDepth), meshShadowDepth.numel() * 4)
+        if meshes is not None:
+            ctx.native.call("fh_shadow_map_meshes", N.ptr(ctx.globalUniforms), meshes.handle, N.Tex2D.ptr(sm),
+                            sm.width)
+        else:
+            ctx.native.call("fh_copy", N.Tex2D.ptr(sm), N.ptr(meshShadowDepth), meshShadowDepth.numel() * 4)
         ctx.native.call("fh_shadow_map_hair", N.ptr(ctx.globalUniforms), N.ptr(hair.positionsNow),
                         N.ptr(hair.tangents), N.Tex2D.ptr(sm), sm.width, hair.getRenderedStrandCount(),
                         hair.pointsPerStrand)
+
+
+class DrawBackgroundGradientPass:
+    """src/passes/drawBackgroundGradient/drawBackgroundGradientPass.ts (the
+    gradient with its noise into the HDR texture)."""
+    NAME = "DrawBackgroundGradientPass"
+
+    def cmdDraw(self, ctx: PassCtx) -> None:
+        w, h = ctx.viewport
+        ctx.native.call("fh_draw_background", w, h, N.ptr(ctx.globalUniforms), N.ptr(ctx.hdrRenderTexture))
+
+
+class DrawMeshesPass:
+    """src/passes/drawMeshes/drawMeshesPass.ts:58-115: the scene's meshes
+    into the depth, normals and HDR textures (PBR colour with the shadow map
+    and the AO texture)."""
+    NAME = "DrawMeshesPass"
+
+    def cmdDrawMeshes(self, ctx: PassCtx, meshes: "N.Meshes", depth: torch.Tensor, normals: torch.Tensor) -> None:
+        w, h = ctx.viewport
+        ctx.native.call("fh_draw_meshes", w, h, N.ptr(ctx.globalUniforms), meshes.handle, ctx.shadowDepthTexture,
+                        ctx.aoTexture, N.ptr(depth), N.ptr(normals), N.ptr(ctx.hdrRenderTexture))
+
+
+def upload_meshes(native: N.Context, mesh_list: list, device) -> "N.Meshes":
+    """An fh_meshes object list from host meshes (assets.scene_meshes)."""
+    dev = []
+    for m in mesh_list:
+        dev.append({"positions": _dev(m["positions"].astype(np.float32), device),
+                    "normals": _dev(m["normals"].astype(np.float32), device),
+                    "indices": _dev(m["indices"].astype(np.int32), device), "preview": m.get("preview", False)})
+    return N.Meshes(native, dev)
 
 
 class HwHairPass:
@@ -335,7 +373,8 @@ class HairRenderer:
     outputs (depth, AO, shadow map) provided as precomputed inputs."""
 
     def __init__(self, wl: A.Workload, device="cuda", native: N.Context | None = None, shadow_size: int = 2048,
-                 inputs: dict | None = None, hw_passes: bool = False, branches: bool = True):
+                 inputs: dict | None = None, hw_passes: bool = False, branches: bool = True,
+                 mesh_passes: bool = False):
         self.wl = wl
         # branches: passes that neither read what the tile, sort and fine
         # passes write nor write what they read run beside them on the
@@ -379,7 +418,21 @@ class HairRenderer:
         # hw_passes: the hair's depth, normals, shadow-map draw and AO are
         # computed every frame (HwHairPass, ShadowMapPass, AoPass) from the
         # meshes' inputs instead of taken precomputed
+        # mesh_passes (implies hw_passes): the background, the meshes' depth,
+        # normals and colour and the meshes in the shadow map are drawn every
+        # frame too (DrawBackgroundGradientPass, DrawMeshesPass, ShadowMapPass)
+        # from the scene's meshes (assets.scene_meshes), so no input is
+        # precomputed but the hair and the colliders themselves
+        self.mesh_passes = mesh_passes
+        hw_passes = hw_passes or mesh_passes
         self.hw_passes = hw_passes
+        if mesh_passes:
+            self.meshList = A.scene_meshes(wl.collider, wl.sim.collisionSphere)
+            self.meshes = upload_meshes(self.native, self.meshList, self.device)
+            self.backgroundPass = DrawBackgroundGradientPass()
+            self.drawMeshesPass = DrawMeshesPass()
+            self.meshDepthTexture = torch.ones_like(self.inputs["meshDepth"])
+            self.meshNormalsTexture = torch.zeros_like(self.inputs["meshNormals"])
         if hw_passes:
             self.shadowMapPass = ShadowMapPass()
             self.hwHairPass = HwHairPass()
@@ -405,7 +458,8 @@ class HairRenderer:
             native=self.native, cfg=self.cfg, viewport=(W, H), frameIdx=self.frameIdx,
             globalUniforms=self.globalUniforms, simulationUniforms=self.simulationUniforms,
             physicsForcesGrid=self.grid, sdfCollider=self.sdf,
-            depthTexture=N.tex(i["meshDepth"], W, H), shadingDepthTexture=N.tex(depth, W, H),
+            depthTexture=N.tex(self.meshDepthTexture if self.mesh_passes else i["meshDepth"], W, H),
+            shadingDepthTexture=N.tex(depth, W, H),
             aoTexture=N.tex(ao, ao.shape[1], ao.shape[0]),
             shadowDepthTexture=N.tex(shadow, shadow.shape[1], shadow.shape[0]),
             normalsTexture=self.normalsTexture if self.hw_passes else None,
@@ -447,9 +501,20 @@ class HairRenderer:
 
     def drawScene(self, ctx: PassCtx) -> None:
         """The HDR scene of the background and mesh passes (renderer.ts:187-188,
-        :244-246), which are out of scope: their output is an input here."""
+        :244-246). Without mesh_passes their output is an input, restored
+        here; with them, drawMeshes() draws it after the clears."""
+        if self.mesh_passes:
+            return
         src = self.inputs["hdrScene"]
         self.native.call("fh_copy", N.ptr(self.hdrRenderTexture), N.ptr(src), src.numel() * src.element_size())
+
+    def drawMeshes(self, ctx: PassCtx) -> None:
+        """renderer.ts:189 and :244-246: the background, the shadow map (meshes,
+        then hair) and the meshes' depth, normals and colour, which reads the
+        shadow map."""
+        self.backgroundPass.cmdDraw(ctx)
+        self.shadowMapPass.cmdUpdateShadowMap(ctx, self.hair, meshes=self.meshes)
+        self.drawMeshesPass.cmdDrawMeshes(ctx, self.meshes, self.meshDepthTexture, self.meshNormalsTexture)
 
     def drawHair(self, ctx: PassCtx, shading_first: bool = False) -> None:
         """cmdDrawScene's hair part (renderer.ts:262-295) and the present pass
@@ -478,9 +543,15 @@ class HairRenderer:
             self.sortPass.cmdClearBeforeRender(ctx)
         finally:
             self.native.clear_batch_end()
+        if self.mesh_passes:
+            # the tile pass reads the meshes' depth: these stay on the stream
+            self.drawMeshes(ctx)
         if self.hw_passes or shading_first:
             self._branch_begin()
-            if self.hw_passes:
+            if self.mesh_passes:
+                self.hwHairPass.cmdDrawHair(ctx, self.hair, self.meshDepthTexture, self.meshNormalsTexture)
+                self.aoPass.cmdCalcAo(ctx)
+            elif self.hw_passes:
                 self.shadowMapPass.cmdUpdateShadowMap(ctx, self.hair, self.inputs["shadowMap"])
                 self.hwHairPass.cmdDrawHair(ctx, self.hair, self.inputs["meshDepth"], self.inputs["meshNormals"])
                 self.aoPass.cmdCalcAo(ctx)

@@ -483,6 +483,66 @@ int fh_shadow_map_hair(fh_ctx* ctx, const fh_render_uniforms* uniforms, const fh
 int fh_calc_ao(fh_ctx* ctx, const fh_render_uniforms* uniforms, fh_tex2d depth, const void* normals_texture,
                float* ao_texture, uint32_t ao_w, uint32_t ao_h);
 
+/* ------------------------------------------------------------------------ */
+/* Background and meshes (src/passes/drawBackgroundGradient,                 */
+/* src/passes/drawMeshes, the meshes' draw of src/passes/shadowMapPass)      */
+/* ------------------------------------------------------------------------ */
+
+/* One object of the scene (GPUMesh, scene/gpuMesh.ts:5-16, as loadObjFile
+ * leaves it, scene/objLoader.ts:21-66): device buffers of vertex positions
+ * and normals (float32x3, 12 B per vertex, VERTEX_ATTRIBUTES gpuMesh.ts
+ * :18-55) and u32 triangle indices (3 per triangle). is_collider_preview
+ * marks the collider preview sphere (loadScene.ts:36), drawn with the
+ * collision sphere's model matrix and not into the shadow map. */
+typedef struct fh_mesh {
+  const float* positions;
+  const float* normals;
+  const uint32_t* indices;
+  uint32_t vertex_count;
+  uint32_t triangle_count;
+  uint32_t is_collider_preview;
+  uint32_t reserved;
+} fh_mesh;
+
+/* The scene's object list (Scene.objects, scene/scene.ts), in draw order.
+ * Created once (the reference creates its GPUMesh objects at load,
+ * loadScene.ts:25-60): checks that every index addresses a vertex of its
+ * mesh (FH_ERR_INVALID_ARG otherwise) and keeps a device copy of the
+ * descriptors; the buffers themselves stay the caller's and must outlive the
+ * object. Not while the context records or captures. At most 256 meshes and
+ * 2^31 - 1 triangles. */
+typedef struct fh_meshes fh_meshes;
+int fh_meshes_create(fh_ctx* ctx, const fh_mesh* meshes, uint32_t count, fh_meshes** out);
+int fh_meshes_destroy(fh_meshes* meshes);
+int fh_meshes_triangles(const fh_meshes* meshes, uint32_t* n_out);
+
+/* DrawBackgroundGradientPass.cmdDraw (renderer.ts:189; fragment shader
+ * drawBackgroundGradientPass.wgsl.ts:33-48, fractalNoise noise.wgsl.ts
+ * :14-42): the background gradient with its noise into hdr_texture
+ * (rgba16float, the layout of fh_hair_combine). */
+int fh_draw_background(fh_ctx* ctx, uint32_t vp_w, uint32_t vp_h, const fh_render_uniforms* uniforms,
+                       void* hdr_texture);
+
+/* DrawMeshesPass.cmdDrawMeshes (drawMeshesPass.ts:58-115; shaders
+ * drawMeshesPass.wgsl.ts:46-168, pbr.wgsl.ts:101-160, sampleShadows.wgsl.ts
+ * :47-150, textureAo.wgsl.ts:7-12): clears depth_texture (to 1.0) and
+ * normals_texture (rg16float octahedral world normals, to 0), draws every
+ * object with back faces culled and the 'less' test, and writes each drawn
+ * pixel's PBR colour (three lights, shadow from shadow_map, AO from ao) into
+ * hdr_texture over what is there (loadOp 'load': the background). Rules and
+ * scratch as fh_hw_hair. */
+int fh_draw_meshes(fh_ctx* ctx, uint32_t vp_w, uint32_t vp_h, const fh_render_uniforms* uniforms,
+                   const fh_meshes* meshes, fh_tex2d shadow_map, fh_tex2d ao, float* depth_texture,
+                   void* normals_texture, void* hdr_texture);
+
+/* The meshes' draw of ShadowMapPass.cmdUpdateShadowMap (shadowMapPass.ts
+ * :83-99, :117-133; shader shadowMapPass.wgsl.ts): clears shadow_map
+ * (shadow_size^2 f32) to 1.0 and draws every object but the collider preview
+ * with the shadow source's MVP (shadows.sourceMVP_Matrix), no culling.
+ * fh_shadow_map_hair then draws the hair into the same map. */
+int fh_shadow_map_meshes(fh_ctx* ctx, const fh_render_uniforms* uniforms, const fh_meshes* meshes,
+                         float* shadow_map, uint32_t shadow_size);
+
 /* Device-to-device copy as a kernel (bytes a multiple of 4), so that a
  * captured frame can restore its inputs (e.g. the HDR scene drawn by the
  * out-of-scope mesh passes) without memcpy nodes. */

@@ -1469,6 +1469,350 @@ void or_hw_coverage(int shadow, uint32_t W, uint32_t H, const fh_render_uniforms
   }
 }
 
+/* ======================================================================== */
+/* Background and meshes                                                     */
+/* ======================================================================== */
+
+/* drawBackgroundGradientPass.wgsl.ts:33-48 with hash / noise / fractalNoise
+ * (noise.wgsl.ts:14-42), per pixel centre, stored as rgba16float bits.
+ * Pinned: test_oracle_meshes.py (hash at lattice points against float64,
+ * the noise's interpolation at cell corners and centres, the gradient's
+ * closed form with the noise scaled away). */
+static float bg_hash(ov2 p0) {
+  const ov2 q = o2(p0.x * 0.3183099f + 0.71f, p0.y * 0.3183099f + 0.113f);
+  const ov2 p = o2(50.0f * ofract(q.x), 50.0f * ofract(q.y));
+  return -1.0f + 2.0f * ofract(p.x * p.y * (p.x + p.y));
+}
+
+static float bg_noise(ov2 p) {
+  const ov2 i = o2(floorf(p.x), floorf(p.y));
+  const ov2 f = o2(ofract(p.x), ofract(p.y));
+  const ov2 u = o2(f.x * f.x * f.x * (f.x * (f.x * 6.0f - 15.0f) + 10.0f),
+                   f.y * f.y * f.y * (f.y * (f.y * 6.0f - 15.0f) + 10.0f));
+  const float a = omix(bg_hash(add2(i, o2(0.0f, 0.0f))), bg_hash(add2(i, o2(1.0f, 0.0f))), u.x);
+  const float b = omix(bg_hash(add2(i, o2(0.0f, 1.0f))), bg_hash(add2(i, o2(1.0f, 1.0f))), u.x);
+  return omix(a, b, u.y);
+}
+
+float or_bg_hash(float x, float y) { return bg_hash(o2(x, y)); }
+float or_bg_noise(float x, float y) { return bg_noise(o2(x, y)); }
+
+static float fractal_noise(ov2 p, float scale) {
+  ov2 uv = muls2(p, scale);
+  /* uv = mat2x2f(1.6, 1.2, -1.2, 1.6) * uv */
+#define ROT(v) o2(1.6f * (v).x + -1.2f * (v).y, 1.2f * (v).x + 1.6f * (v).y)
+  float f = 0.5000f * bg_noise(uv); uv = ROT(uv);
+  f = f + 0.2500f * bg_noise(uv); uv = ROT(uv);
+  f = f + 0.1250f * bg_noise(uv); uv = ROT(uv);
+  f = f + 0.0625f * bg_noise(uv);
+#undef ROT
+  return f;
+}
+
+float or_bg_fractal(float x, float y, float scale) { return fractal_noise(o2(x, y), scale); }
+
+static void store_rgba16f(uint16_t* hdr, size_t p, ov3 c, float a) {
+  hdr[4 * p + 0] = (uint16_t)of2h(c.x);
+  hdr[4 * p + 1] = (uint16_t)of2h(c.y);
+  hdr[4 * p + 2] = (uint16_t)of2h(c.z);
+  hdr[4 * p + 3] = (uint16_t)of2h(a);
+}
+
+void or_background(uint32_t W, uint32_t H, const fh_render_uniforms* u, uint16_t* hdr) {
+  for (uint32_t py = 0; py < H; py++)
+    for (uint32_t px = 0; px < W; px++) {
+      const ov2 uv = o2(((float)px + 0.5f) / u->viewport[0], ((float)py + 0.5f) / u->viewport[1]);
+      const float c = fractal_noise(uv, u->backgroundNoiseScale) * 0.5f + 0.5f;
+      ov3 color = mix3(o3(u->backgroundColor0[0], u->backgroundColor0[1], u->backgroundColor0[2]),
+                       o3(u->backgroundColor1[0], u->backgroundColor1[1], u->backgroundColor1[2]), c);
+      const float g = 1.0f - uv.y;
+      color = mix3(color, o3(g, g, g), u->backgroundGradientStrength);
+      store_rgba16f(hdr, (size_t)py * W + px, color, 1.0f);
+    }
+}
+
+/* DrawMeshesPass (shadow = 0; drawMeshesPass.ts:58-115, drawMeshesPass.wgsl.ts
+ * :46-168) and the meshes' draw of ShadowMapPass (shadow = 1;
+ * shadowMapPass.ts:83-133, shadowMapPass.wgsl.ts), with the rasterizer of
+ * or_hw_hair. Triangles run in scene order (mesh, then index order), back
+ * faces culled in the camera pass (counter-clockwise in NDC is front), none
+ * in the shadow pass, collider previews only in the camera pass. Each
+ * fragment that passes the 'less' test writes its depth, normal and colour
+ * at once (forward shading in primitive order). Pinned: test_oracle_meshes.py
+ * (a screen-aligned quad's exact pixel set and planar depth, back faces
+ * culled only in the camera pass, a nearer later mesh occluding an earlier
+ * one and ties kept by the earlier, the preview's flat colour, PBR against a
+ * float64 restatement of the Karis13 formulas, a step shadow map whose PCF
+ * hit count is known, AO strength). */
+static void mat_mul4(const float* a, const float* b, float* out) {
+  for (int c = 0; c < 4; c++) {
+    const ov4 r = mulmv(a, o4(b[4 * c], b[4 * c + 1], b[4 * c + 2], b[4 * c + 3]));
+    out[4 * c] = r.x; out[4 * c + 1] = r.y; out[4 * c + 2] = r.z; out[4 * c + 3] = r.w;
+  }
+}
+
+static ov3 vtx3(const float* v, uint32_t i) { return o3(v[3 * (size_t)i], v[3 * (size_t)i + 1], v[3 * (size_t)i + 2]); }
+
+static float dot_max0(ov3 a, ov3 b) { return omax(0.0f, dot3(a, b)); }
+
+typedef struct {
+  ov3 positionWS, normal, toEye, albedo;
+  float roughness, isMetallic, ao, shadow;
+} pbr_material;
+
+/* disneyPBR — pbr.wgsl.ts:140-160, pbr_CookTorrance :112-131,
+ * FresnelSchlick :18-20, DistributionGGX :35-44, GeometrySchlickGGX :54-59,
+ * GeometrySmith :71-77, pbr_mixDiffuseAndSpecular :133-138 */
+static ov3 disney_pbr(const pbr_material* m, const fh_light* light) {
+  const ov3 N = m->normal, V = m->toEye;
+  const ov3 L = norm3(sub3(o3(light->position[0], light->position[1], light->position[2]), m->positionWS));
+  const ov3 H = norm3(add3(V, L));
+  const ov3 F0 = mix3(o3(0.04f, 0.04f, 0.04f), m->albedo, m->isMetallic);
+  const float fk = powf(1.0f - dot_max0(H, V), 5.0f);
+  const ov3 F = o3(F0.x + (1.0f - F0.x) * fk, F0.y + (1.0f - F0.y) * fk, F0.z + (1.0f - F0.z) * fk);
+  const float NdotV = dot_max0(N, V), NdotL = dot_max0(N, L);
+  const float r = m->roughness + 1.0f, k = (r * r) / 8.0f;
+  const float g1 = NdotL / (NdotL * (1.0f - k) + k), g2 = NdotV / (NdotV * (1.0f - k) + k);
+  const float G = g1 * g2;
+  const float a = m->roughness * m->roughness, a2 = a * a;
+  const float NdotH = dot_max0(N, H), NdotH2 = NdotH * NdotH;
+  float denom = NdotH2 * (a2 - 1.0f) + 1.0f;
+  denom = O_PI * denom * denom;
+  const float NDF = a2 / denom;
+  const ov3 numerator = muls3(F, NDF * G);
+  const float denominator = 4.0f * dot_max0(N, V) * dot_max0(N, L);
+  const ov3 specular = divs3(numerator, omax(denominator, 0.001f));
+  const ov3 kD = mix3(sub3(o3(1.0f, 1.0f, 1.0f), F), o3(0.0f, 0.0f, 0.0f), m->isMetallic);
+  const ov3 brdf = add3(mul3(kD, m->albedo), specular);
+  const ov3 radiance = muls3(o3(light->colorAndEnergy[0], light->colorAndEnergy[1], light->colorAndEnergy[2]),
+                             light->colorAndEnergy[3]);
+  return muls3(mul3(brdf, radiance), NdotL);
+}
+
+/* calculateDirectionalShadow — sampleShadows.wgsl.ts:47-84, _sampleShadowMap
+ * :95-131, _isValidShadowSample :136-150 */
+static float directional_shadow(const fh_render_uniforms* u, const float* sm, uint32_t smSize, ov3 posWS,
+                                ov3 normalWS, ov3 pss) {
+  const fh_shadows* sh = &u->shadows;
+  const ov3 toSrc = norm3(sub3(o3(sh->sourcePosition[0], sh->sourcePosition[1], sh->sourcePosition[2]), posWS));
+  const float bias = omax(fabsf(sh->bias) * (1.0f - dot3(normalWS, toSrc)), 0.0005f);
+  int32_t radius = (int32_t)sh->PCF_Radius;
+  if (sh->usePCSS > 0u) {
+    const float d = gather_avg(sm, smSize, smSize, o2(pss.x, pss.y));
+    const float dd = omax(pss.z - d, 0.0f);
+    radius = 1 + oi32(dd / d * 10.0f);
+  }
+  const float texel = 1.0f / (float)smSize;
+  float shadow = 0.0f;
+  for (int32_t x = -radius; x <= radius; x++)
+    for (int32_t y = -radius; y <= radius; y++) {
+      const ov2 uv = add2(o2(pss.x, pss.y), o2((float)x * texel, (float)y * texel));
+      shadow += (pss.z - bias > gather_avg(sm, smSize, smSize, uv)) ? 1.0f : 0.0f;
+    }
+  const float pcf = (float)(radius * 2 + 1);
+  const float res = shadow / (pcf * pcf);
+  if (pss.z > 1.0f) return 0.0f;
+  if (pss.x < 0.0f || pss.x > 1.0f || pss.y < 0.0f || pss.y > 1.0f) return 0.0f;
+  return res;
+}
+
+/* hw_setup's snapping and orientation for given clip positions; *swapped
+ * reports that the snapped vertices ran clockwise on screen (y down), i.e.
+ * counter-clockwise in NDC (y up). */
+static hw_tri hw_setup_clip(const ov4* c, uint32_t W, uint32_t H, int* swapped) {
+  hw_tri s;
+  s.ok = 1;
+  *swapped = 0;
+  for (int k = 0; k < 3; k++) {
+    s.c[k] = c[k];
+    s.vi[k] = (uint32_t)k;
+    s.X[k] = s.Y[k] = 0;
+    if (!(s.c[k].w > 0.0f)) { s.ok = 0; continue; }
+    const float nx = s.c[k].x / s.c[k].w, ny = s.c[k].y / s.c[k].w;
+    const float xs = (float)W * (nx * 0.5f + 0.5f);
+    const float ys = (float)H * (1.0f - (ny * 0.5f + 0.5f));
+    if (!(fabsf(xs) < 1048576.0f && fabsf(ys) < 1048576.0f)) { s.ok = 0; continue; }
+    s.X[k] = (long long)rintf(xs * 256.0f);
+    s.Y[k] = (long long)rintf(ys * 256.0f);
+  }
+  s.area = 0;
+  if (!s.ok) return s;
+  long long area = (s.X[1] - s.X[0]) * (s.Y[2] - s.Y[0]) - (s.Y[1] - s.Y[0]) * (s.X[2] - s.X[0]);
+  if (area < 0) {
+    ov4 t = s.c[1]; s.c[1] = s.c[2]; s.c[2] = t;
+    long long q = s.X[1]; s.X[1] = s.X[2]; s.X[2] = q;
+    q = s.Y[1]; s.Y[1] = s.Y[2]; s.Y[2] = q;
+    uint32_t v = s.vi[1]; s.vi[1] = s.vi[2]; s.vi[2] = v;
+    area = -area;
+    *swapped = 1;
+  }
+  s.area = area;
+  s.ok = area > 0;
+  return s;
+}
+
+/* main_fs (drawMeshesPass.wgsl.ts:88-130) and doShading (:133-146) of one
+ * fragment at pixel (px, py) of triangle s (vertex indices vi, in the
+ * original order), returning the colour and the normal. */
+static ov3 mesh_fragment(const fh_render_uniforms* u, const fh_mesh* m, const hw_tri* s, const uint32_t* vi,
+                         long long px, long long py, const float* sm, uint32_t smSize, const float* ao,
+                         uint32_t aoW, uint32_t aoH, ov3* normalOut) {
+  const long long cx = px * 256 + 128, cy = py * 256 + 128;
+  const float ar = (float)s->area;
+  const float l[3] = {(float)hw_edge(s->X[1], s->Y[1], s->X[2], s->Y[2], cx, cy) / ar,
+                      (float)hw_edge(s->X[2], s->Y[2], s->X[0], s->Y[0], cx, cy) / ar,
+                      (float)hw_edge(s->X[0], s->Y[0], s->X[1], s->Y[1], cx, cy) / ar};
+  const float* model = m->is_collider_preview ? u->collisionSphereModelMatrix : u->modelMatrix;
+  ov3 pos = o3(0.0f, 0.0f, 0.0f), nrm = o3(0.0f, 0.0f, 0.0f), pss = o3(0.0f, 0.0f, 0.0f);
+  float den = 0.0f;
+  float wq[3];
+  for (int q = 0; q < 3; q++) {
+    wq[q] = l[q] / s->c[q].w;
+    den = den + wq[q];
+  }
+  for (int q = 0; q < 3; q++) {
+    const uint32_t v = vi[s->vi[q]];
+    const ov3 pv = vtx3(m->positions, v), nv = vtx3(m->normals, v);
+    /* transformNormalToWorldSpace (shaderSnippets.wgls.ts:15-19) */
+    const ov3 nws = norm3(xyz(mulmv(model, o4(nv.x, nv.y, nv.z, 0.0f))));
+    /* projectToShadowSpace (sampleShadows.wgsl.ts:21-34); the preview's
+     * matrix is mat4x4<f32>() (zeros) */
+    ov4 lp = o4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (!m->is_collider_preview) lp = mulmv(u->shadows.sourceMVP_Matrix, o4(pv.x, pv.y, pv.z, 1.0f));
+    const ov3 sp = o3(lp.x * 0.5f + 0.5f, 1.0f - (lp.y * 0.5f + 0.5f), lp.z);
+    pos = add3(pos, muls3(pv, wq[q]));
+    nrm = add3(nrm, muls3(nws, wq[q]));
+    pss = add3(pss, muls3(sp, wq[q]));
+  }
+  pos = divs3(pos, den);
+  nrm = divs3(nrm, den);
+  pss = divs3(pss, den);
+  pbr_material mt;
+  mt.positionWS = pos;
+  mt.normal = norm3(nrm);
+  mt.toEye = norm3(sub3(o3(u->cameraPosition[0], u->cameraPosition[1], u->cameraPosition[2]), pos));
+  mt.albedo = o3(0.9f, 0.9f, 0.9f);
+  mt.roughness = 0.8f;
+  mt.isMetallic = 0.0f;
+  mt.shadow = 1.0f - directional_shadow(u, sm, smSize, pos, mt.normal, pss);
+  /* sampleAo (textureAo.wgsl.ts:7-12) at the fragment position (pixel centre) */
+  const ov2 at = o2(((float)px + 0.5f) / u->viewport[0], ((float)py + 0.5f) / u->viewport[1]);
+  const float aov = tex_load(ao, aoW, aoH, oi32((float)aoW * at.x), oi32((float)aoH * at.y));
+  mt.ao = omix(1.0f, aov, u->ao.strength);
+  if (m->is_collider_preview) {
+    mt.albedo = o3(1.0f, 1.0f, 1.0f);
+    mt.shadow = 1.0f;
+    mt.ao = 1.0f;
+    mt.roughness = 1.0f;
+    mt.isMetallic = 0.4f;
+  }
+  const ov3 ambient = muls3(o3(u->lightAmbient[0], u->lightAmbient[1], u->lightAmbient[2]), u->lightAmbient[3]);
+  ov3 rs = o3(0.0f, 0.0f, 0.0f);
+  rs = add3(rs, disney_pbr(&mt, &u->light0));
+  rs = add3(rs, disney_pbr(&mt, &u->light1));
+  rs = add3(rs, disney_pbr(&mt, &u->light2));
+  rs = muls3(rs, osat(mt.ao));
+  rs = muls3(rs, oclamp(mt.shadow, 1.0f - u->shadows.strength, 1.0f));
+  ov3 color = add3(ambient, rs);
+  if (m->is_collider_preview) color = mix3(color, o3(1.0f, 1.0f, 0.0f), 0.5f);
+  *normalOut = mt.normal;
+  return color;
+}
+
+void or_meshes(int shadow, uint32_t W, uint32_t H, const fh_render_uniforms* u, const fh_mesh* meshes,
+               uint32_t count, const float* shadowMap, uint32_t shadowSize, const float* ao, uint32_t aoW,
+               uint32_t aoH, float* depth, uint32_t* normals, uint16_t* hdr) {
+  for (size_t p = 0; p < (size_t)W * H; p++) {
+    depth[p] = 1.0f;  /* the passes' depth clear */
+    if (!shadow) normals[p] = 0u;
+  }
+  float previewMvp[16], vm[16];
+  mat_mul4(u->viewMatrix, u->collisionSphereModelMatrix, vm);  /* getMVP_Mat :5-8 */
+  mat_mul4(u->projMatrix, vm, previewMvp);
+  for (uint32_t mi = 0; mi < count; mi++) {
+    const fh_mesh* m = &meshes[mi];
+    if (shadow && m->is_collider_preview) continue;
+    const float* mvp = shadow ? u->shadows.sourceMVP_Matrix : (m->is_collider_preview ? previewMvp : u->mvpMatrix);
+    for (uint32_t t = 0; t < m->triangle_count; t++) {
+      uint32_t vi[3];
+      ov4 c[3];
+      for (int q = 0; q < 3; q++) {
+        vi[q] = m->indices[3 * (size_t)t + q];
+        const ov3 p = vtx3(m->positions, vi[q]);
+        c[q] = mulmv(mvp, o4(p.x, p.y, p.z, 1.0f));
+      }
+      int swapped;
+      const hw_tri s = hw_setup_clip(c, W, H, &swapped);
+      if (!s.ok) continue;
+      if (!shadow && !swapped) continue;  /* back face */
+      long long minX = s.X[0], maxX = s.X[0], minY = s.Y[0], maxY = s.Y[0];
+      for (int k = 1; k < 3; k++) {
+        if (s.X[k] < minX) minX = s.X[k];
+        if (s.X[k] > maxX) maxX = s.X[k];
+        if (s.Y[k] < minY) minY = s.Y[k];
+        if (s.Y[k] > maxY) maxY = s.Y[k];
+      }
+      const int tl0 = hw_top_left(s.X[1], s.Y[1], s.X[2], s.Y[2]);
+      const int tl1 = hw_top_left(s.X[2], s.Y[2], s.X[0], s.Y[0]);
+      const int tl2 = hw_top_left(s.X[0], s.Y[0], s.X[1], s.Y[1]);
+      long long x0 = hw_ceil_div256(minX - 128), x1 = hw_floor_div256(maxX - 128);
+      long long y0 = hw_ceil_div256(minY - 128), y1 = hw_floor_div256(maxY - 128);
+      if (x0 < 0) x0 = 0;
+      if (y0 < 0) y0 = 0;
+      if (x1 > (long long)W - 1) x1 = (long long)W - 1;
+      if (y1 > (long long)H - 1) y1 = (long long)H - 1;
+      for (long long py = y0; py <= y1; py++) {
+        const long long cy = py * 256 + 128;
+        for (long long px = x0; px <= x1; px++) {
+          const long long cx = px * 256 + 128;
+          const long long e0 = hw_edge(s.X[1], s.Y[1], s.X[2], s.Y[2], cx, cy);
+          const long long e1 = hw_edge(s.X[2], s.Y[2], s.X[0], s.Y[0], cx, cy);
+          const long long e2 = hw_edge(s.X[0], s.Y[0], s.X[1], s.Y[1], cx, cy);
+          if (!((e0 > 0 || (e0 == 0 && tl0)) && (e1 > 0 || (e1 == 0 && tl1)) &&
+                (e2 > 0 || (e2 == 0 && tl2))))
+            continue;
+          const float ar = (float)s.area;
+          const float l0 = (float)e0 / ar, l1 = (float)e1 / ar, l2 = (float)e2 / ar;
+          const float z0 = s.c[0].z / s.c[0].w, z1 = s.c[1].z / s.c[1].w, z2 = s.c[2].z / s.c[2].w;
+          float z = l0 * z0 + l1 * z1 + l2 * z2;
+          if (!(z >= 0.0f && z <= 1.0f)) continue;
+          z = z + 0.0f;
+          const size_t p = (size_t)py * W + (size_t)px;
+          if (!(z < depth[p])) continue;  /* 'less' in primitive order */
+          depth[p] = z;
+          if (shadow) continue;
+          ov3 n;
+          const ov3 col = mesh_fragment(u, m, &s, vi, px, py, shadowMap, shadowSize, ao, aoW, aoH, &n);
+          normals[p] = or_encode_oct(n.x, n.y, n.z);
+          store_rgba16f(hdr, p, col, 1.0f);
+        }
+      }
+    }
+  }
+}
+
+/* Test hooks: disneyPBR of one light and calculateDirectionalShadow. */
+void or_disney_pbr(const float* positionWS, const float* normal, const float* toEye, const float* albedo,
+                   float roughness, float isMetallic, const fh_light* light, float* out) {
+  pbr_material m;
+  m.positionWS = o3(positionWS[0], positionWS[1], positionWS[2]);
+  m.normal = o3(normal[0], normal[1], normal[2]);
+  m.toEye = o3(toEye[0], toEye[1], toEye[2]);
+  m.albedo = o3(albedo[0], albedo[1], albedo[2]);
+  m.roughness = roughness;
+  m.isMetallic = isMetallic;
+  m.ao = 1.0f;
+  m.shadow = 1.0f;
+  const ov3 r = disney_pbr(&m, light);
+  out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+float or_directional_shadow(const fh_render_uniforms* u, const float* sm, uint32_t smSize, const float* posWS,
+                            const float* normalWS, const float* pss) {
+  return directional_shadow(u, sm, smSize, o3(posWS[0], posWS[1], posWS[2]), o3(normalWS[0], normalWS[1], normalWS[2]),
+                            o3(pss[0], pss[1], pss[2]));
+}
+
 /* AoPass (aoPass.wgsl.ts:48-79) with gtao (gtao.wgsl.ts:26-136) per texel of
  * the AO texture (aoW x aoH; AO_TEXTURE_SCALE 0.5), stored as r16float
  * (rounded to half, widened). Pinned: test_oracle_hwraster.py (the

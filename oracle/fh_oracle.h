@@ -93,6 +93,18 @@ void or_hw_coverage(int shadow, uint32_t W, uint32_t H, const fh_render_uniforms
 void or_ao(const fh_render_uniforms* u, const float* depth, uint32_t W, uint32_t H,
            const uint32_t* normals, float* out, uint32_t aoW, uint32_t aoH);
 
+float or_bg_hash(float x, float y);
+float or_bg_noise(float x, float y);
+float or_bg_fractal(float x, float y, float scale);
+void or_background(uint32_t W, uint32_t H, const fh_render_uniforms* u, uint16_t* hdr);
+void or_meshes(int shadow, uint32_t W, uint32_t H, const fh_render_uniforms* u, const fh_mesh* meshes,
+               uint32_t count, const float* shadowMap, uint32_t shadowSize, const float* ao, uint32_t aoW,
+               uint32_t aoH, float* depth, uint32_t* normals, uint16_t* hdr);
+void or_disney_pbr(const float* positionWS, const float* normal, const float* toEye, const float* albedo,
+                   float roughness, float isMetallic, const fh_light* light, float* out);
+float or_directional_shadow(const fh_render_uniforms* u, const float* sm, uint32_t smSize, const float* posWS,
+                            const float* normalWS, const float* pss);
+
 #ifdef __cplusplus
 }
 #endif

@@ -221,3 +221,97 @@ def hw_coverage(shadow, W, H, render_u, positions, tangents, S, P):
     lib().or_hw_coverage(C.c_int(1 if shadow else 0), C.c_uint32(W), C.c_uint32(H), _p(render_u), _p(positions),
                          _p(tangents), _p(counts), C.c_uint32(S), C.c_uint32(P))
     return counts
+
+
+class Mesh(C.Structure):
+    """fh_mesh with host pointers."""
+    _fields_ = [("positions", C.c_void_p), ("normals", C.c_void_p), ("indices", C.c_void_p),
+                ("vertex_count", C.c_uint32), ("triangle_count", C.c_uint32),
+                ("is_collider_preview", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+def _mesh_table(meshes):
+    """ctypes fh_mesh array for a list of (positions (V,3) f32, normals (V,3)
+    f32, indices (T,3) u32, preview) and the arrays it points into."""
+    keep, rows = [], []
+    for m in meshes:
+        pos = np.ascontiguousarray(m["positions"], dtype=np.float32)
+        nrm = np.ascontiguousarray(m["normals"], dtype=np.float32)
+        idx = np.ascontiguousarray(m["indices"], dtype=np.uint32)
+        keep += [pos, nrm, idx]
+        rows.append(Mesh(pos.ctypes.data, nrm.ctypes.data, idx.ctypes.data, pos.shape[0], idx.shape[0],
+                         1 if m.get("preview") else 0, 0))
+    arr = (Mesh * max(1, len(rows)))(*rows)
+    return arr, keep
+
+
+def background(W, H, render_u):
+    """rgba16float bits (H x W x 4 uint16) of the background gradient pass."""
+    out = np.zeros((H, W, 4), dtype=np.uint16)
+    lib().or_background(C.c_uint32(W), C.c_uint32(H), _p(render_u), _p(out))
+    return out
+
+
+def meshes(shadow, W, H, render_u, mesh_list, shadow_map=None, ao=None, hdr=None):
+    """The mesh passes: shadow=True returns the meshes' shadow depth (H x W,
+    W = H = the map size); shadow=False returns (depth, normals, hdr) of the
+    camera pass drawn over a copy of hdr."""
+    arr, keep = _mesh_table(mesh_list)
+    depth = np.zeros((H, W), dtype=np.float32)
+    if shadow:
+        lib().or_meshes(C.c_int(1), C.c_uint32(W), C.c_uint32(H), _p(render_u), arr, C.c_uint32(len(mesh_list)),
+                        C.c_void_p(0), C.c_uint32(0), C.c_void_p(0), C.c_uint32(0), C.c_uint32(0), _p(depth),
+                        C.c_void_p(0), C.c_void_p(0))
+        return depth
+    sm = np.ascontiguousarray(shadow_map, dtype=np.float32)
+    aot = np.ascontiguousarray(ao, dtype=np.float32)
+    normals = np.zeros((H, W), dtype=np.uint32)
+    out = np.ascontiguousarray(hdr, dtype=np.uint16).copy()
+    lib().or_meshes(C.c_int(0), C.c_uint32(W), C.c_uint32(H), _p(render_u), arr, C.c_uint32(len(mesh_list)),
+                    _p(sm), C.c_uint32(sm.shape[0]), _p(aot), C.c_uint32(aot.shape[1]), C.c_uint32(aot.shape[0]),
+                    _p(depth), _p(normals), _p(out))
+    return depth, normals, out
+
+
+def bg_hash(x: float, y: float) -> float:
+    L = lib()
+    L.or_bg_hash.restype = C.c_float
+    L.or_bg_hash.argtypes = [C.c_float, C.c_float]
+    return L.or_bg_hash(x, y)
+
+
+def bg_noise(x: float, y: float) -> float:
+    L = lib()
+    L.or_bg_noise.restype = C.c_float
+    L.or_bg_noise.argtypes = [C.c_float, C.c_float]
+    return L.or_bg_noise(x, y)
+
+
+def bg_fractal(x: float, y: float, scale: float) -> float:
+    L = lib()
+    L.or_bg_fractal.restype = C.c_float
+    L.or_bg_fractal.argtypes = [C.c_float, C.c_float, C.c_float]
+    return L.or_bg_fractal(x, y, scale)
+
+
+def disney_pbr(positionWS, normal, toEye, albedo, roughness, isMetallic, light_pos, light_color_energy):
+    """disneyPBR of one light (pbr.wgsl.ts:140-160)."""
+
+    class Light(C.Structure):
+        _fields_ = [("position", C.c_float * 4), ("colorAndEnergy", C.c_float * 4)]
+
+    f3 = lambda v: (C.c_float * 3)(*[float(x) for x in v])  # noqa: E731
+    light = Light((C.c_float * 4)(*[float(x) for x in light_pos]),
+                  (C.c_float * 4)(*[float(x) for x in light_color_energy]))
+    out = (C.c_float * 3)()
+    lib().or_disney_pbr(f3(positionWS), f3(normal), f3(toEye), f3(albedo), C.c_float(roughness),
+                        C.c_float(isMetallic), C.byref(light), out)
+    return np.array(list(out), dtype=np.float32)
+
+
+def directional_shadow(render_u, shadow_map, posWS, normalWS, pss) -> float:
+    L = lib()
+    L.or_directional_shadow.restype = C.c_float
+    sm = np.ascontiguousarray(shadow_map, dtype=np.float32)
+    f3 = lambda v: (C.c_float * 3)(*[float(x) for x in v])  # noqa: E731
+    return L.or_directional_shadow(_p(render_u), _p(sm), C.c_uint32(sm.shape[0]), f3(posWS), f3(normalWS), f3(pss))

@@ -3,8 +3,9 @@ shading passes run on the context's second stream beside the tile, sort and
 fine passes leave exactly what frames on one stream leave: positions, grids,
 tile lists, images, shading, depth / normals / AO and the presented screen,
 eagerly and replayed as HIP graphs, over both parities of the position
-buffers. Pipelined frames (the simulation step of frame f beside the render
-passes of frame f-1) leave, frame for frame, the simulation state of frame f
+buffers, with the precomputed scene inputs, with the hardware-raster passes
+and with the background and mesh passes drawn every frame. Pipelined frames
+(the simulation step of frame f beside the render passes of frame f-1) leave, frame for frame, the simulation state of frame f
 and the render state of frame f-1 that plain frames leave."""
 import numpy as np
 import pytest
@@ -32,7 +33,12 @@ def _state(r):
     if r.hw_passes:
         out.update(hair_depth=_np(r.depthTexture), normals=_np(r.normalsTexture), ao=_np(r.aoTexture),
                    shadow=_np(r.shadowTexture))
+    if r.mesh_passes:
+        out.update(mesh_depth=_np(r.meshDepthTexture), mesh_normals=_np(r.meshNormalsTexture))
     return out
+
+
+PASSES = {"none": dict(), "hw": dict(hw_passes=True), "mesh": dict(mesh_passes=True)}
 
 
 def _same(a, b, where):
@@ -45,18 +51,18 @@ def _same(a, b, where):
             assert a[k] == b[k], f"{where}: {k} differs"
 
 
-@pytest.mark.parametrize("hw_passes", [False, True])
+@pytest.mark.parametrize("passes", list(PASSES))
 @pytest.mark.parametrize("use_graph", [False, True])
-def test_branched_frames_match_one_stream(gpu, hw_passes, use_graph):
+def test_branched_frames_match_one_stream(gpu, passes, use_graph):
     import torch
     from frostbitten_hair_webgpu_amd import assets as A
     from frostbitten_hair_webgpu_amd import passes as PS
     wl = A.make_workload("c2", seed=77, strands=1200, width=480, height=320, points=16)
-    print(f"branches: hw_passes={hw_passes} graph={use_graph} {wl.strands}x{wl.points} {wl.width}x{wl.height}",
+    print(f"branches: passes={passes} graph={use_graph} {wl.strands}x{wl.points} {wl.width}x{wl.height}",
           flush=True)
-    serial = PS.HairRenderer(wl, "cuda:0", shadow_size=512, hw_passes=hw_passes, branches=False)
+    serial = PS.HairRenderer(wl, "cuda:0", shadow_size=512, branches=False, **PASSES[passes])
     inputs = {k: v.cpu().numpy() for k, v in serial.inputs.items()}
-    branched = PS.HairRenderer(wl, "cuda:0", shadow_size=512, inputs=inputs, hw_passes=hw_passes, branches=True)
+    branched = PS.HairRenderer(wl, "cuda:0", shadow_size=512, inputs=inputs, branches=True, **PASSES[passes])
     for r in (serial, branched):
         r.beforeFirstFrame()
         r.frame()  # every call once before capture
@@ -82,18 +88,18 @@ def _split(st):
     return sim, render
 
 
-@pytest.mark.parametrize("hw_passes", [False, True])
+@pytest.mark.parametrize("passes", list(PASSES))
 @pytest.mark.parametrize("use_graph", [False, True])
-def test_pipelined_frames_match_plain(gpu, hw_passes, use_graph):
+def test_pipelined_frames_match_plain(gpu, passes, use_graph):
     import torch
     from frostbitten_hair_webgpu_amd import assets as A
     from frostbitten_hair_webgpu_amd import passes as PS
     wl = A.make_workload("c2", seed=78, strands=1000, width=400, height=288, points=24)
-    print(f"pipelined: hw_passes={hw_passes} graph={use_graph} {wl.strands}x{wl.points} {wl.width}x{wl.height}",
+    print(f"pipelined: passes={passes} graph={use_graph} {wl.strands}x{wl.points} {wl.width}x{wl.height}",
           flush=True)
-    plain = PS.HairRenderer(wl, "cuda:0", shadow_size=512, hw_passes=hw_passes, branches=False)
+    plain = PS.HairRenderer(wl, "cuda:0", shadow_size=512, branches=False, **PASSES[passes])
     inputs = {k: v.cpu().numpy() for k, v in plain.inputs.items()}
-    pipe = PS.HairRenderer(wl, "cuda:0", shadow_size=512, inputs=inputs, hw_passes=hw_passes, branches=True)
+    pipe = PS.HairRenderer(wl, "cuda:0", shadow_size=512, inputs=inputs, branches=True, **PASSES[passes])
     frames = 6
     plain.beforeFirstFrame()
     ref = []

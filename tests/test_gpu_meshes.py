@@ -1,0 +1,124 @@
+"""HIP background, mesh and mesh-shadow passes vs the CPU oracle, through the
+C-ABI, and frames that draw them every frame.
+
+Bars: the background and the rasterized depth, normals and shadow depth are
+bit-exact (the same f32 operations in one fixed order on both sides, the
+winner of each pixel independent of the order the GPU runs the triangles
+in). The meshes' colour takes pow from the device's math library and the
+oracle from libm: the rgba16float texels agree to within 2 half-precision
+ulps, and almost all of them exactly.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _renderer(kind="small", strands=1500, width=320, height=240, shadow_size=512, **kw):
+    from frostbitten_hair_webgpu_amd import assets as A
+    from frostbitten_hair_webgpu_amd import passes as PS
+    wl = A.make_workload(kind, strands=strands, width=width, height=height)
+    r = PS.HairRenderer(wl, "cuda:0", shadow_size=shadow_size, mesh_passes=True, **kw)
+    return wl, r
+
+
+def _h2f(bits):
+    return np.asarray(bits).view(np.uint16).view(np.float16).astype(np.float32)
+
+
+def _half_ulps(a_bits, b_bits):
+    """Distance in half-precision ulps of two rgba16float arrays of finite,
+    non-negative values (bit patterns order like the values)."""
+    a = np.asarray(a_bits).view(np.uint16).astype(np.int64)
+    b = np.asarray(b_bits).view(np.uint16).astype(np.int64)
+    return np.abs(a - b)
+
+
+@pytest.mark.parametrize("kind,width,height", [("small", 320, 240), ("c2", 1920, 1080)])
+def test_background_bit_exact(gpu, oracle, kind, width, height):
+    import torch
+    wl, r = _renderer(kind, strands=500, width=width, height=height)
+    print(f"background {kind} {width}x{height}", flush=True)
+    r.backgroundPass.cmdDraw(r.ctx())
+    torch.cuda.synchronize()
+    got = _np(r.hdrRenderTexture).view(np.uint16).reshape(height, width, 4)
+    exp = oracle.background(width, height, r.render_uniforms)
+    assert np.unique(exp[..., 0]).size > 50
+    assert np.array_equal(got, exp), f"{(got != exp).any(axis=-1).sum()} texels differ"
+
+
+@pytest.mark.parametrize("kind,size", [("small", 512), ("c2", 2048)])
+def test_shadow_map_meshes_bit_exact(gpu, oracle, kind, size):
+    import torch
+    wl, r = _renderer(kind, strands=500, shadow_size=size)
+    print(f"shadow meshes {kind} {size}^2, {r.meshes.triangles()} triangles", flush=True)
+    sm = r.ctx().shadowDepthTexture
+    r.native.call("fh_shadow_map_meshes", r.globalUniforms.data_ptr(), r.meshes.handle, sm.data, size)
+    torch.cuda.synchronize()
+    got = _np(r.shadowTexture)
+    exp = oracle.meshes(True, size, size, r.render_uniforms, r.meshList)
+    assert (exp < 1).sum() > size * size // 400
+    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), f"{(got != exp).sum()} texels differ"
+
+
+@pytest.mark.parametrize("kind,width,height", [("small", 320, 240), ("small", 333, 171), ("c2", 1920, 1080)])
+def test_draw_meshes_vs_oracle(gpu, oracle, kind, width, height):
+    """The camera pass over the background, with the GPU's shadow map of the
+    meshes and the hair and a seeded AO texture: depth and normals bit-exact,
+    colour within 2 half ulps."""
+    import torch
+    wl, r = _renderer(kind, strands=1500 if kind == "small" else 4000, width=width, height=height)
+    print(f"draw meshes {kind} {width}x{height}, {r.meshes.triangles()} triangles", flush=True)
+    rng = np.random.default_rng(5)
+    r.aoTexture.copy_(torch.from_numpy(rng.uniform(0.2, 1.0, r.aoTexture.shape).astype(np.float16)
+                                       .astype(np.float32)))
+    ctx = r.ctx()
+    r.backgroundPass.cmdDraw(ctx)
+    r.shadowMapPass.cmdUpdateShadowMap(ctx, r.hair, meshes=r.meshes)
+    torch.cuda.synchronize()
+    bg = _np(r.hdrRenderTexture).view(np.uint16).reshape(height, width, 4).copy()
+    r.drawMeshesPass.cmdDrawMeshes(ctx, r.meshes, r.meshDepthTexture, r.meshNormalsTexture)
+    torch.cuda.synchronize()
+    sm = _np(r.shadowTexture)
+    d, n, c = oracle.meshes(False, width, height, r.render_uniforms, r.meshList, shadow_map=sm,
+                            ao=_np(r.aoTexture), hdr=bg)
+    got_d, got_n = _np(r.meshDepthTexture), _np(r.meshNormalsTexture).view(np.uint32)
+    got_c = _np(r.hdrRenderTexture).view(np.uint16).reshape(height, width, 4)
+    drawn = d < 1
+    assert drawn.mean() > 0.01
+    assert np.array_equal(got_d.view(np.uint32), d.view(np.uint32)), f"{(got_d != d).sum()} depths differ"
+    assert np.array_equal(got_n, n), f"{(got_n != n).sum()} normals differ"
+    ulps = _half_ulps(got_c, c)
+    print(f"colour: max {ulps.max()} half ulps, {(ulps > 0).mean():.5f} of channels differ", flush=True)
+    assert ulps.max() <= 2 and (ulps > 0).mean() < 0.01
+    assert np.array_equal(got_c[~drawn], bg[~drawn])
+    # the shadow map darkens part of the meshes: not every drawn pixel lit alike
+    assert np.unique(_h2f(c[drawn][:, 0])).size > 100
+
+
+def test_frames_read_this_frames_meshes(gpu, oracle):
+    """Frames with the mesh passes (graph replays): the meshes' depth and
+    normals that the tile pass and the hair draw read are this frame's,
+    equal to the oracle's camera pass, and the hair draw adds the hair over
+    them."""
+    import torch
+    wl, r = _renderer()
+    r.beforeFirstFrame()
+    r.frame()
+    torch.cuda.synchronize()
+    r.capture_frames()
+    for _ in range(2):
+        r.frame_graph()
+    torch.cuda.synchronize()
+    d, n, _ = oracle.meshes(False, wl.width, wl.height, r.render_uniforms, r.meshList,
+                            shadow_map=_np(r.shadowTexture), ao=np.ones_like(_np(r.aoTexture)),
+                            hdr=np.zeros((wl.height, wl.width, 4), np.uint16))
+    assert (d < 1).mean() > 0.02
+    assert np.array_equal(_np(r.meshDepthTexture).view(np.uint32), d.view(np.uint32))
+    assert np.array_equal(_np(r.meshNormalsTexture).view(np.uint32), n)
+    hair = _np(r.depthTexture) < d
+    assert hair.sum() > 100 and np.array_equal(_np(r.depthTexture)[~hair], d[~hair])
