@@ -98,14 +98,17 @@ def stage_bytes(wl, n_pairs: int, tiles: int) -> dict:
     }
 
 
-def full_frame(wl, r0, steps: int, dev, branches: bool = True) -> dict:
+def full_frame(wl, r0, steps: int, dev, branches: bool = True, mesh_passes: bool = True) -> dict:
     """Frames of the same workload with the passes that feed the shading pass
     computed on the device every frame (ShadowMapPass's hair draw, HwHairPass,
-    AoPass), as the reference's renderer runs them; replayed as HIP graphs."""
+    AoPass) and, with mesh_passes, the background, the meshes' shadow map and
+    the meshes' depth, normals and colour (DrawBackgroundGradientPass,
+    ShadowMapPass's mesh draw, DrawMeshesPass): every pass of the reference's
+    renderer; replayed as HIP graphs."""
     import torch
     from frostbitten_hair_webgpu_amd import passes as PS
     inputs = {k: v.cpu().numpy() for k, v in r0.inputs.items()}
-    r = PS.HairRenderer(wl, dev, inputs=inputs, hw_passes=True, branches=branches)
+    r = PS.HairRenderer(wl, dev, inputs=inputs, hw_passes=True, branches=branches, mesh_passes=mesh_passes)
     ring = torch.from_numpy(np.concatenate([r.sim_uniforms(f).view(np.uint8).reshape(-1)
                                             for f in range(2 * steps + 4)])).to(dev).view(2 * steps + 4, -1)
     r.beforeFirstFrame()
@@ -147,10 +150,15 @@ def full_frame(wl, r0, steps: int, dev, branches: bool = True) -> dict:
     torch.cuda.synchronize()
     kernels = r.native.profiler_read()
     r.native.profiler_enable(0)
-    extra = ("hair_raster", "hair_raster_big", "hair_resolve", "gtao")
+    extra = ("hair_raster", "hair_raster_big", "hair_resolve", "gtao", "background", "mesh_raster",
+             "mesh_raster_big", "mesh_resolve")
+    passes = ("chain + ShadowMapPass hair draw (2048^2) + HwHairPass (depth + normals) + AoPass (GTAO, "
+              "half resolution)")
+    if mesh_passes:
+        passes = ("chain + DrawBackgroundGradientPass + ShadowMapPass (meshes and hair, 2048^2) + DrawMeshesPass "
+                  f"(depth, normals, PBR colour; {r.meshes.triangles()} triangles) + HwHairPass + AoPass")
     return {"value": round(1e3 / ms, 3), "unit": "frames/s", "ms_per_step": round(ms, 4), "steps": steps,
-            "passes": "chain + ShadowMapPass hair draw (2048^2) + HwHairPass (depth + normals) + AoPass (GTAO, "
-                      "half resolution)", "pipelined": pipelined,
+            "passes": passes, "pipelined": pipelined,
             "kernel_us": {k: round(t / n * 1e3, 2) for k, (n, t) in sorted(kernels.items(), key=lambda kv: -kv[1][1])
                           if k in extra or k in ("copy_u32", "fill_u32")}}
 
@@ -676,8 +684,8 @@ def main() -> None:
     ap.add_argument("--c3-steps", type=int, default=20,
                     help="frames of the 4K, 100,000-strand workload timed beside the main line (0: skip it)")
     ap.add_argument("--full-frame", type=int, default=1,
-                    help="also time frames that compute the hair depth / normals, shadow map and AO every frame "
-                         "(0 to skip)")
+                    help="also time frames that draw the background, the meshes, the shadow map, the hair depth / "
+                         "normals and AO every frame, and frames with the meshes precomputed (0 to skip)")
     ap.add_argument("--batch-instances", type=int, default=64,
                     help="instances of the batched-versus-plain comparison (0: skip it)")
     ap.add_argument("--no-tune", action="store_true", help="batched comparison without the autotuner")
@@ -736,6 +744,8 @@ def main() -> None:
         out["speedup_vs_cpu_baseline"] = round(out["value"] / out["cpu_baseline"]["value"], 1)
     if args.full_frame:
         out["full_frame"] = full_frame(wl, r, min(args.steps, 30), dev, branches=not args.no_branches)
+        out["full_frame_precomputed_meshes"] = full_frame(wl, r, min(args.steps, 30), dev,
+                                                          branches=not args.no_branches, mesh_passes=False)
     del r
     if args.c3_steps and args.workload != "c3":
         # the 4K, 100,000-strand workload (the dense fine-pass case) beside the 1080p line

@@ -55,6 +55,7 @@ using namespace fh::tri;
 
 constexpr float kPi = 3.14159265358979323846f;
 constexpr uint32_t kMaxMeshes = 256;
+constexpr uint32_t kNoTriangle = 0xffffffffu;
 
 // --------------------------------------------------------------------------
 // Background (drawBackgroundGradientPass.wgsl.ts:33-48; noise.wgsl.ts:14-42)
@@ -130,9 +131,11 @@ struct MeshArgs {
   unsigned long long* key;  // per pixel: (depth bits << 32) | triangle, kNoFragment when empty
   uint32_t* bigCount;
   uint2* big;
-  // resolve outputs (camera pass: depth, normals, hdr; shadow pass: depth)
+  // resolve outputs (camera pass: depth, normals, visibility and hdr, each
+  // but depth optional; shadow pass: depth)
   float* depth;
   uint32_t* normals;
+  uint32_t* visibility;  // scene triangle of each pixel's fragment, kNoTriangle where none
   void* hdr;
   const float* shadow;
   const float* ao;
@@ -340,12 +343,23 @@ __device__ __forceinline__ float directional_shadow(const MeshArgs& a, f3 posWS,
     radius = 1 + f2i(dd / sm * 10.0f);  // PCSS_PENUMBRA_BASE + depthDiff / depth * PCSS_PENUMBRA_WIDTH
   }
   const float texel = 1.0f / (float)a.shadowW;
+  // the taps x-major as in the reference's loops, eight gathers issued
+  // before their comparisons (a count of 0 / 1 results, exact in any order)
+  const int32_t side = radius * 2 + 1, taps = side * side;
   float shadow = 0.0f;
-  for (int32_t x = -radius; x <= radius; x++)
-    for (int32_t y = -radius; y <= radius; y++) {
+  for (int32_t k0 = 0; k0 < taps; k0 += 8) {
+    float g[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int32_t k = min(k0 + j, taps - 1);
+      const int32_t x = -radius + k / side, y = -radius + k % side;
       const f2 uv = mk2(pss.x, pss.y) + mk2((float)x * texel, (float)y * texel);
-      shadow += (pss.z - bias > gather_avg(a.shadow, a.shadowW, a.shadowW, uv)) ? 1.0f : 0.0f;
+      g[j] = gather_avg(a.shadow, a.shadowW, a.shadowW, uv);
     }
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      if (k0 + j < taps) shadow += (pss.z - bias > g[j]) ? 1.0f : 0.0f;
+  }
   const float pcf = (float)(radius * 2 + 1);
   const float res = shadow / (pcf * pcf);
   if (pss.z > 1.0f) return 0.0f;
@@ -353,9 +367,79 @@ __device__ __forceinline__ float directional_shadow(const MeshArgs& a, f3 posWS,
   return res;
 }
 
+// main_fs (drawMeshesPass.wgsl.ts:88-130, doShading :133-146) for the
+// fragment of scene triangle t at pixel (px, py): the encoded normal and,
+// when `colour`, the colour. False when the triangle is not drawn (not
+// reached for a pixel's winner).
+__device__ __forceinline__ bool mesh_fragment(const MeshArgs& a, uint32_t t, uint32_t px, uint32_t py, bool colour,
+                                              uint32_t& normalOut, f3& colourOut) {
+  SetupTri s;
+  uint32_t m, vi[3];
+  if (!mesh_setup(a, t, a.W, a.H, s, m, vi)) return false;
+  const MeshDesc& d = a.meshes[m];
+  const fh_render_uniforms* __restrict__ u = a.u;
+  float l[3], wq[3];
+  barycentrics(s, (long long)px * 256 + 128, (long long)py * 256 + 128, l);
+  const float den = persp_weights(s, l, wq);
+  const float* model = d.preview ? u->collisionSphereModelMatrix : u->modelMatrix;
+  const float* smvp = u->shadows.sourceMVP_Matrix;
+  f3 pos = mk3(0.0f, 0.0f, 0.0f), nrm = mk3(0.0f, 0.0f, 0.0f), pss = mk3(0.0f, 0.0f, 0.0f);
+  for (int q = 0; q < 3; q++) {
+    const uint32_t v = vi[s.vi[q]];
+    const f3 pv = vtx3(d.pos, v), nv = vtx3(d.nrm, v);
+    // transformNormalToWorldSpace (shaderSnippets.wgls.ts:15-19)
+    const f3 nws = normalize(xyz(mul(model, mk4(nv.x, nv.y, nv.z, 0.0f))));
+    // projectToShadowSpace (sampleShadows.wgsl.ts:21-34); the preview's
+    // shadow matrix is mat4x4<f32>(), all zeros
+    f4 lp = mk4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (!d.preview) lp = mul(smvp, mk4(pv.x, pv.y, pv.z, 1.0f));
+    const f3 sp = mk3(lp.x * 0.5f + 0.5f, 1.0f - (lp.y * 0.5f + 0.5f), lp.z);
+    pos = pos + pv * wq[q];
+    nrm = nrm + nws * wq[q];
+    pss = pss + sp * wq[q];
+  }
+  pos = pos / den;
+  nrm = nrm / den;
+  pss = pss / den;
+  // createDefaultMaterial (:148-168)
+  Material mt;
+  mt.normal = normalize(nrm);
+  normalOut = encode_oct(mt.normal);
+  if (!colour) return true;
+  mt.positionWS = pos;
+  mt.toEye = normalize(mk3(u->cameraPosition[0], u->cameraPosition[1], u->cameraPosition[2]) - pos);
+  mt.albedo = mk3(0.9f, 0.9f, 0.9f);
+  mt.roughness = 0.8f;
+  mt.isMetallic = 0.0f;
+  mt.shadow = 1.0f - directional_shadow(a, pos, mt.normal, pss);
+  // sampleAo (textureAo.wgsl.ts:7-12) at the fragment's position
+  const f2 at = mk2(((float)px + 0.5f) / u->viewport[0], ((float)py + 0.5f) / u->viewport[1]);
+  const float aov = tex_load(a.ao, a.aoW, a.aoH, f2i((float)a.aoW * at.x), f2i((float)a.aoH * at.y));
+  mt.ao = mixf(1.0f, aov, u->ao.strength);
+  if (d.preview) {
+    mt.albedo = mk3(1.0f, 1.0f, 1.0f);
+    mt.shadow = 1.0f;
+    mt.ao = 1.0f;
+    mt.roughness = 1.0f;
+    mt.isMetallic = 0.4f;
+  }
+  // doShading (:133-146): (0 + L0) + L1 + L2
+  const f3 ambient = mk3(u->lightAmbient[0], u->lightAmbient[1], u->lightAmbient[2]) * u->lightAmbient[3];
+  f3 rs = mk3(0.0f, 0.0f, 0.0f);
+  rs = rs + disney_pbr(mt, u->light0);
+  rs = rs + disney_pbr(mt, u->light1);
+  rs = rs + disney_pbr(mt, u->light2);
+  rs = rs * satf(mt.ao);
+  rs = rs * clampf(mt.shadow, 1.0f - u->shadows.strength, 1.0f);
+  f3 color = ambient + rs;
+  if (d.preview) color = mix(color, mk3(1.0f, 1.0f, 0.0f), 0.5f);
+  colourOut = color;
+  return true;
+}
+
 // The winner of each pixel. Camera pass: its depth, its normal
-// (encodeOctahedronNormal of the normalized interpolated world normal) and
-// main_fs's colour (drawMeshesPass.wgsl.ts:88-130, doShading :133-146);
+// (encodeOctahedronNormal of the normalized interpolated world normal), its
+// triangle (visibility, when asked for) and its colour (when hdr is set);
 // pixels with no fragment get the clear values (depth 1, normals 0) and keep
 // the colour already in the target ('load'). Shadow pass: the depth.
 struct MeshResolve {
@@ -368,76 +452,38 @@ struct MeshResolve {
     const unsigned long long key = a.key[p];
     if (key == kNoFragment) {
       a.depth[p] = 1.0f;
-      if (!a.shadowPass) a.normals[p] = 0u;
+      if (a.shadowPass) return;
+      a.normals[p] = 0u;
+      if (a.visibility) a.visibility[p] = kNoTriangle;
       return;
     }
     a.depth[p] = __uint_as_float((uint32_t)(key >> 32));
     if (a.shadowPass) return;
     const uint32_t t = (uint32_t)key;
-    SetupTri s;
-    uint32_t m, vi[3];
-    if (!mesh_setup(a, t, a.W, a.H, s, m, vi)) {  // not reached: the winner was drawn
-      a.normals[p] = 0u;
-      return;
-    }
-    const MeshDesc& d = a.meshes[m];
-    const fh_render_uniforms* __restrict__ u = a.u;
-    const uint32_t px = p % a.W, py = p / a.W;
-    float l[3], wq[3];
-    barycentrics(s, (long long)px * 256 + 128, (long long)py * 256 + 128, l);
-    const float den = persp_weights(s, l, wq);
-    const float* model = d.preview ? u->collisionSphereModelMatrix : u->modelMatrix;
-    const float* smvp = u->shadows.sourceMVP_Matrix;
-    f3 pos = mk3(0.0f, 0.0f, 0.0f), nrm = mk3(0.0f, 0.0f, 0.0f), pss = mk3(0.0f, 0.0f, 0.0f);
-    for (int q = 0; q < 3; q++) {
-      const uint32_t v = vi[s.vi[q]];
-      const f3 pv = vtx3(d.pos, v), nv = vtx3(d.nrm, v);
-      // transformNormalToWorldSpace (shaderSnippets.wgls.ts:15-19)
-      const f3 nws = normalize(xyz(mul(model, mk4(nv.x, nv.y, nv.z, 0.0f))));
-      // projectToShadowSpace (sampleShadows.wgsl.ts:21-34); the preview's
-      // shadow matrix is mat4x4<f32>(), all zeros
-      f4 lp = mk4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (!d.preview) lp = mul(smvp, mk4(pv.x, pv.y, pv.z, 1.0f));
-      const f3 sp = mk3(lp.x * 0.5f + 0.5f, 1.0f - (lp.y * 0.5f + 0.5f), lp.z);
-      pos = pos + pv * wq[q];
-      nrm = nrm + nws * wq[q];
-      pss = pss + sp * wq[q];
-    }
-    pos = pos / den;
-    nrm = nrm / den;
-    pss = pss / den;
-    // createDefaultMaterial (:148-168)
-    Material mt;
-    mt.positionWS = pos;
-    mt.normal = normalize(nrm);
-    mt.toEye = normalize(mk3(u->cameraPosition[0], u->cameraPosition[1], u->cameraPosition[2]) - pos);
-    mt.albedo = mk3(0.9f, 0.9f, 0.9f);
-    mt.roughness = 0.8f;
-    mt.isMetallic = 0.0f;
-    mt.shadow = 1.0f - directional_shadow(a, pos, mt.normal, pss);
-    // sampleAo (textureAo.wgsl.ts:7-12) at the fragment's position
-    const f2 at = mk2(((float)px + 0.5f) / u->viewport[0], ((float)py + 0.5f) / u->viewport[1]);
-    const float aov = tex_load(a.ao, a.aoW, a.aoH, f2i((float)a.aoW * at.x), f2i((float)a.aoH * at.y));
-    mt.ao = mixf(1.0f, aov, u->ao.strength);
-    if (d.preview) {
-      mt.albedo = mk3(1.0f, 1.0f, 1.0f);
-      mt.shadow = 1.0f;
-      mt.ao = 1.0f;
-      mt.roughness = 1.0f;
-      mt.isMetallic = 0.4f;
-    }
-    // doShading (:133-146): (0 + L0) + L1 + L2
-    const f3 ambient = mk3(u->lightAmbient[0], u->lightAmbient[1], u->lightAmbient[2]) * u->lightAmbient[3];
-    f3 rs = mk3(0.0f, 0.0f, 0.0f);
-    rs = rs + disney_pbr(mt, u->light0);
-    rs = rs + disney_pbr(mt, u->light1);
-    rs = rs + disney_pbr(mt, u->light2);
-    rs = rs * satf(mt.ao);
-    rs = rs * clampf(mt.shadow, 1.0f - u->shadows.strength, 1.0f);
-    f3 color = ambient + rs;
-    if (d.preview) color = mix(color, mk3(1.0f, 1.0f, 0.0f), 0.5f);
-    a.normals[p] = encode_oct(mt.normal);
-    store_rgba16f(a.hdr, p, color, 1.0f);
+    if (a.visibility) a.visibility[p] = t;
+    uint32_t n = 0u;
+    f3 c;
+    const bool drawn = mesh_fragment(a, t, p % a.W, p / a.W, a.hdr != nullptr, n, c);
+    a.normals[p] = drawn ? n : 0u;
+    if (drawn && a.hdr) store_rgba16f(a.hdr, p, c, 1.0f);
+  }
+};
+
+// The colour of each pixel's fragment from a visibility texture
+// (fh_shade_meshes): what MeshResolve writes with hdr set. Pixels without a
+// triangle (or with an index past the scene's) keep the target's colour.
+struct MeshShade {
+  using Args = MeshArgs;
+  static constexpr uint32_t kThreads = 256;
+  static constexpr const char* kName = "mesh_shade";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) {
+    const uint32_t p = xcd_order(block, blocks) * kThreads + threadIdx.x;
+    if (p >= a.W * a.H) return;
+    const uint32_t t = a.visibility[p];
+    if (t >= a.nTri) return;
+    uint32_t n;
+    f3 c;
+    if (mesh_fragment(a, t, p % a.W, p / a.W, true, n, c)) store_rgba16f(a.hdr, p, c, 1.0f);
   }
 };
 
@@ -580,6 +626,56 @@ int fh_draw_meshes(fh_ctx* ctx, uint32_t vp_w, uint32_t vp_h, const fh_render_un
   a.H = vp_h;
   a.shadowPass = 0;
   return raster_meshes(ctx, a);
+}
+
+int fh_draw_meshes_visibility(fh_ctx* ctx, uint32_t vp_w, uint32_t vp_h, const fh_render_uniforms* uniforms,
+                              const fh_meshes* meshes, float* depth_texture, void* normals_texture,
+                              uint32_t* visibility) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(uniforms && meshes && depth_texture && normals_texture && visibility,
+             "fh_draw_meshes_visibility: null buffer");
+  FH_REQUIRE(meshes->device == ctx->device, "fh_draw_meshes_visibility: meshes of another device");
+  FH_REQUIRE((uint64_t)vp_w * vp_h < (1ull << 31), "fh_draw_meshes_visibility: viewport too large");
+  if (vp_w == 0 || vp_h == 0) return FH_OK;
+  MeshArgs a{};
+  a.u = uniforms;
+  a.meshes = meshes->dtable;
+  a.nMeshes = (uint32_t)meshes->table.size();
+  a.nTri = meshes->nTri;
+  a.depth = depth_texture;
+  a.normals = static_cast<uint32_t*>(normals_texture);
+  a.visibility = visibility;
+  a.W = vp_w;
+  a.H = vp_h;
+  return raster_meshes(ctx, a);
+}
+
+int fh_shade_meshes(fh_ctx* ctx, uint32_t vp_w, uint32_t vp_h, const fh_render_uniforms* uniforms,
+                    const fh_meshes* meshes, const uint32_t* visibility, fh_tex2d shadow_map, fh_tex2d ao,
+                    void* hdr_texture) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(uniforms && meshes && visibility && hdr_texture, "fh_shade_meshes: null buffer");
+  FH_REQUIRE(meshes->device == ctx->device, "fh_shade_meshes: meshes of another device");
+  FH_REQUIRE(shadow_map.data && shadow_map.width > 0 && shadow_map.width == shadow_map.height,
+             "fh_shade_meshes: the shadow map must be square");
+  FH_REQUIRE(ao.data && ao.width > 0 && ao.height > 0, "fh_shade_meshes: null AO texture");
+  FH_REQUIRE((uint64_t)vp_w * vp_h < (1ull << 31), "fh_shade_meshes: viewport too large");
+  if (vp_w == 0 || vp_h == 0 || meshes->nTri == 0) return FH_OK;
+  MeshArgs a{};
+  a.u = uniforms;
+  a.meshes = meshes->dtable;
+  a.nMeshes = (uint32_t)meshes->table.size();
+  a.nTri = meshes->nTri;
+  a.visibility = const_cast<uint32_t*>(visibility);
+  a.hdr = hdr_texture;
+  a.shadow = shadow_map.data;
+  a.shadowW = shadow_map.width;
+  a.ao = ao.data;
+  a.aoW = ao.width;
+  a.aoH = ao.height;
+  a.W = vp_w;
+  a.H = vp_h;
+  return launch<MeshShade>(ctx, div_ceil(vp_w * vp_h, MeshShade::kThreads), 0, a);
 }
 
 int fh_shadow_map_meshes(fh_ctx* ctx, const fh_render_uniforms* uniforms, const fh_meshes* meshes, float* shadow_map,

@@ -134,6 +134,8 @@ SIGNATURES = {
     "fh_meshes_triangles": (i32, [vp, C.POINTER(u32)]),
     "fh_draw_background": (i32, [vp, u32, u32, vp, vp]),
     "fh_draw_meshes": (i32, [vp, u32, u32, vp, vp, Tex2D, Tex2D, vp, vp, vp]),
+    "fh_draw_meshes_visibility": (i32, [vp, u32, u32, vp, vp, vp, vp, vp]),
+    "fh_shade_meshes": (i32, [vp, u32, u32, vp, vp, vp, Tex2D, Tex2D, vp]),
     "fh_shadow_map_meshes": (i32, [vp, vp, vp, vp, u32]),
     "fh_pack_render_uniforms": (i32, [C.POINTER(RenderParams), vp]),
     "fh_pack_sim_uniforms": (i32, [C.POINTER(SimParams), vp]),

@@ -324,10 +324,22 @@ class DrawMeshesPass:
     and the AO texture)."""
     NAME = "DrawMeshesPass"
 
-    def cmdDrawMeshes(self, ctx: PassCtx, meshes: "N.Meshes", depth: torch.Tensor, normals: torch.Tensor) -> None:
+    def cmdDrawMeshes(self, ctx: PassCtx, meshes: "N.Meshes", depth: torch.Tensor, normals: torch.Tensor,
+                      visibility: torch.Tensor | None = None) -> None:
+        """The whole pass; with `visibility`, only its depth, normals and
+        per-pixel triangle (cmdShadeMeshes adds the colour)."""
         w, h = ctx.viewport
+        if visibility is not None:
+            ctx.native.call("fh_draw_meshes_visibility", w, h, N.ptr(ctx.globalUniforms), meshes.handle,
+                            N.ptr(depth), N.ptr(normals), N.ptr(visibility))
+            return
         ctx.native.call("fh_draw_meshes", w, h, N.ptr(ctx.globalUniforms), meshes.handle, ctx.shadowDepthTexture,
                         ctx.aoTexture, N.ptr(depth), N.ptr(normals), N.ptr(ctx.hdrRenderTexture))
+
+    def cmdShadeMeshes(self, ctx: PassCtx, meshes: "N.Meshes", visibility: torch.Tensor) -> None:
+        w, h = ctx.viewport
+        ctx.native.call("fh_shade_meshes", w, h, N.ptr(ctx.globalUniforms), meshes.handle, N.ptr(visibility),
+                        ctx.shadowDepthTexture, ctx.aoTexture, N.ptr(ctx.hdrRenderTexture))
 
 
 def upload_meshes(native: N.Context, mesh_list: list, device) -> "N.Meshes":
@@ -433,6 +445,8 @@ class HairRenderer:
             self.drawMeshesPass = DrawMeshesPass()
             self.meshDepthTexture = torch.ones_like(self.inputs["meshDepth"])
             self.meshNormalsTexture = torch.zeros_like(self.inputs["meshNormals"])
+            # for the two-call form of the mesh pass (cmdDrawMeshes with visibility)
+            self.meshVisibility = torch.zeros_like(self.inputs["meshNormals"])
         if hw_passes:
             self.shadowMapPass = ShadowMapPass()
             self.hwHairPass = HwHairPass()
@@ -509,9 +523,9 @@ class HairRenderer:
         self.native.call("fh_copy", N.ptr(self.hdrRenderTexture), N.ptr(src), src.numel() * src.element_size())
 
     def drawMeshes(self, ctx: PassCtx) -> None:
-        """renderer.ts:189 and :244-246: the background, the shadow map (meshes,
-        then hair) and the meshes' depth, normals and colour, which reads the
-        shadow map."""
+        """renderer.ts:189 and :244-246 in the reference's order: the
+        background, the shadow map (meshes, then hair) and the meshes' depth,
+        normals and colour, which reads the shadow map."""
         self.backgroundPass.cmdDraw(ctx)
         self.shadowMapPass.cmdUpdateShadowMap(ctx, self.hair, meshes=self.meshes)
         self.drawMeshesPass.cmdDrawMeshes(ctx, self.meshes, self.meshDepthTexture, self.meshNormalsTexture)
@@ -544,7 +558,10 @@ class HairRenderer:
         finally:
             self.native.clear_batch_end()
         if self.mesh_passes:
-            # the tile pass reads the meshes' depth: these stay on the stream
+            # on the stream: the tile pass reads the meshes' depth and the
+            # combine pass their colour. Splitting off the colour (which waits
+            # for the shadow map) onto the branch measured slower: the branch
+            # already holds the simulation step, the hair draw and AO.
             self.drawMeshes(ctx)
         if self.hw_passes or shading_first:
             self._branch_begin()

@@ -535,6 +535,22 @@ int fh_draw_meshes(fh_ctx* ctx, uint32_t vp_w, uint32_t vp_h, const fh_render_un
                    const fh_meshes* meshes, fh_tex2d shadow_map, fh_tex2d ao, float* depth_texture,
                    void* normals_texture, void* hdr_texture);
 
+/* fh_draw_meshes in two calls, for callers that run the colour beside other
+ * work (the colour needs the shadow map; the depth, which the tile pass
+ * reads, does not). fh_draw_meshes_visibility clears and draws depth_texture
+ * and normals_texture as fh_draw_meshes does and writes, per pixel, the scene
+ * triangle index of the drawn fragment (0xffffffff where none; triangles
+ * numbered over the meshes in draw order) into visibility (W x H u32).
+ * fh_shade_meshes then writes those fragments' colour into hdr_texture
+ * (indices past the scene's triangles are skipped). Together they leave
+ * exactly what fh_draw_meshes leaves. */
+int fh_draw_meshes_visibility(fh_ctx* ctx, uint32_t vp_w, uint32_t vp_h, const fh_render_uniforms* uniforms,
+                              const fh_meshes* meshes, float* depth_texture, void* normals_texture,
+                              uint32_t* visibility);
+int fh_shade_meshes(fh_ctx* ctx, uint32_t vp_w, uint32_t vp_h, const fh_render_uniforms* uniforms,
+                    const fh_meshes* meshes, const uint32_t* visibility, fh_tex2d shadow_map, fh_tex2d ao,
+                    void* hdr_texture);
+
 /* The meshes' draw of ShadowMapPass.cmdUpdateShadowMap (shadowMapPass.ts
  * :83-99, :117-133; shader shadowMapPass.wgsl.ts): clears shadow_map
  * (shadow_size^2 f32) to 1.0 and draws every object but the collider preview

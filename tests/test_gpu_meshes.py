@@ -122,3 +122,37 @@ def test_frames_read_this_frames_meshes(gpu, oracle):
     assert np.array_equal(_np(r.meshNormalsTexture).view(np.uint32), n)
     hair = _np(r.depthTexture) < d
     assert hair.sum() > 100 and np.array_equal(_np(r.depthTexture)[~hair], d[~hair])
+
+
+def test_split_pass_equals_fused(gpu):
+    """fh_draw_meshes_visibility + fh_shade_meshes leave exactly what
+    fh_draw_meshes leaves; the visibility texture holds each drawn pixel's
+    scene triangle and 0xffffffff elsewhere, and indices past the scene's
+    triangles are skipped by the shading call."""
+    import torch
+    wl, r = _renderer(width=333, height=171)
+    ctx = r.ctx()
+    r.backgroundPass.cmdDraw(ctx)
+    r.shadowMapPass.cmdUpdateShadowMap(ctx, r.hair, meshes=r.meshes)
+    bg = r.hdrRenderTexture.clone()
+    r.drawMeshesPass.cmdDrawMeshes(ctx, r.meshes, r.meshDepthTexture, r.meshNormalsTexture)
+    torch.cuda.synchronize()
+    fused = [t.clone() for t in (r.meshDepthTexture, r.meshNormalsTexture, r.hdrRenderTexture)]
+    r.hdrRenderTexture.copy_(bg)
+    r.meshDepthTexture.zero_()
+    r.meshNormalsTexture.fill_(7)
+    r.drawMeshesPass.cmdDrawMeshes(ctx, r.meshes, r.meshDepthTexture, r.meshNormalsTexture,
+                                   visibility=r.meshVisibility)
+    r.drawMeshesPass.cmdShadeMeshes(ctx, r.meshes, r.meshVisibility)
+    torch.cuda.synchronize()
+    for a, b, name in zip(fused, (r.meshDepthTexture, r.meshNormalsTexture, r.hdrRenderTexture),
+                          ("depth", "normals", "hdr")):
+        assert torch.equal(a, b), name
+    vis = _np(r.meshVisibility).view(np.uint32)
+    drawn = _np(r.meshDepthTexture) < 1
+    assert (vis[~drawn] == 0xffffffff).all() and (vis[drawn] < r.meshes.triangles()).all()
+    r.meshVisibility.fill_(-2)  # 0xfffffffe: past the scene's triangles
+    before = r.hdrRenderTexture.clone()
+    r.drawMeshesPass.cmdShadeMeshes(ctx, r.meshes, r.meshVisibility)
+    torch.cuda.synchronize()
+    assert torch.equal(before, r.hdrRenderTexture)
