@@ -239,7 +239,8 @@ def cpu_baseline(wl, r, frames: int = 8, budget_s: float = 15.0) -> dict:
             "reference": REFERENCE_CPU}
 
 
-def batched_vs_plain(n: int, steps: int, warmup: int, tune: bool, dev, log) -> dict:
+def batched_vs_plain(n: int, steps: int, warmup: int, tune: bool, dev, log, gap: bool = False,
+                     instances: tuple | None = None) -> dict:
     """The batched form against plain runs on the same instances: `n`
     instances of uneven sizes (strands log-uniform in [1k, 50k], points from
     {8, 16, 24, 32}, resolutions from {512^2, 1280x720, 1920x1080}, each with
@@ -248,18 +249,32 @@ def batched_vs_plain(n: int, steps: int, warmup: int, tune: bool, dev, log) -> d
     after another. Batched: one graph per frame runs every kernel of the chain
     once for all instances. Both are timed over `steps` rounds (one frame of
     every instance) with the per-frame simulation uniforms already on the
-    device."""
+    device.
+
+    gap: also time each instance's plain frame alone and, for the instances
+    that take the larger half of the plain round, the batched frame of that
+    half alone: with the other instances free, batching could not beat
+    (whole plain round) / (batched frame of that half). `instances` =
+    (workloads, inputs) skips generation and tuning (the workloads keep the
+    configuration applied to them)."""
     import torch
     from frostbitten_hair_webgpu_amd import assets as A
     from frostbitten_hair_webgpu_amd import autotune as AT
     from frostbitten_hair_webgpu_amd import batch as B
     from frostbitten_hair_webgpu_amd import passes as PS
     t0 = time.perf_counter()
-    wls = [A.make_instance(i) for i in range(n)]
-    inputs = [PS.make_inputs(wl, PS.workload_render_uniforms(wl), 1024) for wl in wls]
+    if instances is not None:
+        wls, inputs = instances
+        n = len(wls)
+        tune = False
+    else:
+        wls = [A.make_instance(i) for i in range(n)]
+        inputs = [PS.make_inputs(wl, PS.workload_render_uniforms(wl), 1024) for wl in wls]
     t1 = time.perf_counter()
     log(f"# batch: {n} instances generated in {t1 - t0:.1f} s")
-    if tune:
+    if instances is not None:
+        choice = None
+    elif tune:
         choice = AT.tune(wls, inputs, log=log)
     else:
         choice = [(16, 8, 4)] * n
@@ -278,7 +293,7 @@ def batched_vs_plain(n: int, steps: int, warmup: int, tune: bool, dev, log) -> d
     torch.cuda.synchronize()
     t3 = time.perf_counter()
     log(f"# batch: built in {t3 - t2:.1f} s, {br.launches_per_frame()} launches per batched frame")
-    total = 4 * (warmup + steps) + 8
+    total = 6 * (warmup + steps) + 16
     ring = torch.from_numpy(np.stack([br.sim_uniforms(br.frameIdx + f) for f in range(total)])).to(dev)
 
     def plain_round(f):
@@ -361,6 +376,42 @@ def batched_vs_plain(n: int, steps: int, warmup: int, tune: bool, dev, log) -> d
                 "p99": round(float(np.percentile(per, 99)), 3)}
 
     best_plain = max(plain_fps, plain_pp_fps)
+    gap_out = None
+    if gap:
+        # each instance's plain frame alone (events between the instances of
+        # a round; one stream, so the durations add up to the round)
+        f2 = f1 + 2 + warmup + steps
+        rounds = 5
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(n + 1)] for _ in range(rounds)]
+        for q in range(rounds):
+            ev[q][0].record()
+            for i, r in enumerate(br.renderers):
+                r.simulationUniforms.copy_(ring[f2 + q, i])
+                r.frame_graph(upload_uniforms=False)
+                ev[q][i + 1].record()
+        torch.cuda.synchronize()
+        t_inst = np.median([[ev[q][i].elapsed_time(ev[q][i + 1]) for i in range(n)] for q in range(rounds)], axis=0)
+        order = np.argsort(-t_inst)
+        cum = np.cumsum(t_inst[order])
+        k = int(np.searchsorted(cum, 0.5 * cum[-1])) + 1  # the largest instances, half of the plain round
+        big = sorted(order[:k].tolist())
+        sub = batched_vs_plain(k, steps, warmup, False, dev, log,
+                               instances=([wls[i] for i in big], [inputs[i] for i in big]))
+        round_ms = float(t_inst.sum())
+        big_batch_ms = min(sub["batch_frame_ms"]["mean"], sub["batch_unpipelined_frame_ms"]["mean"])
+        gap_out = {
+            "plain_instance_ms": {"sum": round(round_ms, 3), "median": round(float(np.median(t_inst)), 3),
+                                  "max": round(float(t_inst.max()), 3), "min": round(float(t_inst.min()), 3)},
+            "largest_half": {"instances": k, "plain_ms": round(float(t_inst[big].sum()), 3),
+                             "batched_frame_ms": big_batch_ms,
+                             "speedup_vs_plain": sub["speedup_vs_plain"],
+                             "strands": int(sum(wls[i].strands for i in big)),
+                             "pixels": int(sum(wls[i].width * wls[i].height for i in big))},
+            "rest": {"instances": n - k, "plain_ms": round(float(round_ms - t_inst[big].sum()), 3)},
+            "speedup_bound": round(round_ms / big_batch_ms, 3),
+            "bound": "plain round (sum of the instances' plain frames) / batched frame of the largest half "
+                     "alone: the speedup if batching made every other instance free",
+        }
     return {
         "instances": n, "steps": steps,
         "value": round(batch_pp_fps, 2), "unit": "instance-frames/s",
@@ -380,9 +431,11 @@ def batched_vs_plain(n: int, steps: int, warmup: int, tune: bool, dev, log) -> d
         "pixels_total": int(sum(w.width * w.height for w in wls)),
         "particle_steps_per_s": round(float((S * P).sum()) * steps / el_bp, 1),
         "segments_rasterized_per_s": round(float((S * (P - 1)).sum()) * steps / el_bp, 1),
-        "autotune": AT.summary(choice) if tune else "off (tile 16, 8 bins x 4 slices)",
+        "autotune": (AT.summary(choice) if tune else "off (tile 16, 8 bins x 4 slices)") if instances is None
+        else "as configured by the caller",
         "device_bytes": br.memory_bytes(),
         "setup_s": round(time.perf_counter() - t0, 1),
+        **({"gap": gap_out} if gap_out is not None else {}),
     }
 
 
@@ -686,6 +739,8 @@ def main() -> None:
     ap.add_argument("--full-frame", type=int, default=1,
                     help="also time frames that draw the background, the meshes, the shadow map, the hair depth / "
                          "normals and AO every frame, and frames with the meshes precomputed (0 to skip)")
+    ap.add_argument("--batch-gap", action="store_true",
+                    help="also time each instance's plain frame and the batched frame of the largest half alone")
     ap.add_argument("--batch-instances", type=int, default=64,
                     help="instances of the batched-versus-plain comparison (0: skip it)")
     ap.add_argument("--no-tune", action="store_true", help="batched comparison without the autotuner")
@@ -763,7 +818,7 @@ def main() -> None:
         torch.cuda.empty_cache()
     if args.batch_instances:
         out["batched"] = batched_vs_plain(args.batch_instances, min(args.steps, 20), 2, not args.no_tune, dev,
-                                          lambda m: print(m, file=sys.stderr, flush=True))
+                                          lambda m: print(m, file=sys.stderr, flush=True), gap=args.batch_gap)
     if args.serve_requests:
         out["served"] = served(args.serve_requests, args.serve_rate, args.serve_budget_gib, dev,
                                lambda m: print(m, file=sys.stderr, flush=True))
