@@ -1148,7 +1148,6 @@ struct FineLds {
   uint32_t* heads;   // NS * NPX: newest slot of each (slice, pixel) list
   float* db;         // NPX: mesh depth
   uint32_t* done;    // NPX: pixel saturated
-  uint32_t* claim;   // NPX: lowest lane linking to the pixel (64 = none)
   uint32_t* ring;    // NW * kFineRing
   uint32_t* wsc;     // NW * 6 * 64: per-wave enumeration scratch
   uint2* poolColor;  // ldsPool: f16 rg, ba
@@ -1160,20 +1159,34 @@ template <int TILE>
 __device__ __forceinline__ void fine_shade_queue(const FineArgs& a, const FineLds<TILE>& L, uint4* gpool,
                                                  uint32_t w, uint32_t head, uint32_t n, uint32_t bx0,
                                                  uint32_t by0, f2 td, float invSegs, uint32_t* sliceOff) {
+  constexpr uint32_t PXW = FineLds<TILE>::PXW;
   const uint32_t lane = __lane_id();
   bool accept = false;
   uint32_t j = 0, pix = 0;
   f2 wq = mk2(0.0f, 0.0f);
-  float hdp = 0.0f;
+  float hdp = 0.0f, fr = 0.0f;
+  float4 c0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c1 = c0;
   const auto& R = L.recs;
   if (lane < n) {
     const uint32_t e = L.ring[w * kFineRing + ((head + lane) & (kFineRing - 1u))];
     j = e >> 8;
-    pix = w * FineLds<TILE>::PXW + (e & 0xffu);
+    pix = w * PXW + (e & 0xffu);
     const f2 p = mk2((float)(bx0 + pix % TILE), (float)(by0 + pix / TILE));
     // interpolateHairQuad from the record's per-segment terms (bit-identical)
     wq = interp_quad_pre<fine_batch<TILE>()>(&R.at(kRecIp, j), mk2(R.at(kRecV + 0, j), R.at(kRecV + 1, j)), p);
     hdp = interp_f32(wq, mk4(R.at(kRecD + 0, j), R.at(kRecD + 1, j), R.at(kRecD + 2, j), R.at(kRecD + 3, j)));
+    // _sampleShading (hairShadingBuffer.ts:29-38): the two shading points are
+    // loaded now, before the depth test and the slot, so that their latency
+    // overlaps that work (a rejected fragment's loads are simply unused)
+    const uint32_t key = R.u(kRecKey, j);
+    const uint32_t strand = key >> 8, seg = key & 0xffu;
+    const float tFull = ((float)seg + wq.y) / invSegs;
+    const float sa = tFull * ((float)a.PS - 1.0f);
+    const uint32_t au = f2u(sa);
+    const uint32_t i0 = clampu(au, 0u, a.PS - 1u), i1 = clampu(au + 1u, 0u, a.PS - 1u);
+    fr = fractf(sa);
+    c0 = a.shading[(size_t)strand * a.PS + i0];
+    c1 = a.shading[(size_t)strand * a.PS + i1];
     accept = !(hdp >= L.db[pix]);  // depth test with GL_LESS (:31-34)
   }
   // a slot of the slice memory: _hasMoreSliceDataSlots (hairFinePass.wgsl.ts:252-264)
@@ -1191,16 +1204,6 @@ __device__ __forceinline__ void fine_shade_queue(const FineArgs& a, const FineLd
     float alpha = 1.0f - fabsf(wq.x * 2.0f - 1.0f);
     if (a.alphaQuadratic) alpha = sqrtf(alpha);
     alpha = satf(alpha * a.alphaMultiplier);
-    const uint32_t key = R.u(kRecKey, j);
-    const uint32_t strand = key >> 8, seg = key & 0xffu;
-    const float tFull = ((float)seg + wq.y) / invSegs;
-    // _sampleShading — hairShadingBuffer.ts:29-38
-    const float sa = tFull * ((float)a.PS - 1.0f);
-    const uint32_t au = f2u(sa);
-    const uint32_t i0 = clampu(au, 0u, a.PS - 1u), i1 = clampu(au + 1u, 0u, a.PS - 1u);
-    const float fr = fractf(sa);
-    const float4 c0 = a.shading[(size_t)strand * a.PS + i0];
-    const float4 c1 = a.shading[(size_t)strand * a.PS + i1];
     const float cr = mixf(c0.x, c1.x, fr), cg = mixf(c0.y, c1.y, fr);
     const float cb = mixf(c0.z, c1.z, fr), ca = mixf(c0.w, c1.w, fr) * alpha;
     const uint2 col = make_uint2(pack_half2(cr, cg), pack_half2(cb, ca));
@@ -1208,21 +1211,29 @@ __device__ __forceinline__ void fine_shade_queue(const FineArgs& a, const FineLd
     if (slot < a.ldsPool) L.poolColor[slot] = col;
     else *reinterpret_cast<uint2*>(&gpool[slot - a.ldsPool]) = col;
   }
-  // link at the head of the (slice, pixel) list, one lane per pixel per round
-  bool pend = accept;
-  while (__ballot(pend)) {
-    if (pend) atomicMin(&L.claim[pix], lane);
-    wave_sync();
-    if (pend && L.claim[pix] == lane) {
-      uint32_t* hp = &L.heads[tag * FineLds<TILE>::NPX + pix];
-      if (slot < a.ldsPool) L.poolNext[slot] = *hp;
-      else gpool[slot - a.ldsPool].z = *hp;
-      *hp = slot;
-      L.claim[pix] = 64u;
-      pend = false;
-    }
-    wave_sync();
+  // Link at the head of the (slice, pixel) lists in lane order (queue order):
+  // the lanes of one list find each other with one ballot per bit of the
+  // list's index within the wave's pixels; each links to the next lower such
+  // lane (the lowest to the old head) and the highest becomes the head.
+  uint64_t grp = __ballot(accept);
+  const uint32_t lk = tag * PXW + (pix - w * PXW);
+  const uint32_t nbits = 32u - (uint32_t)__builtin_clz(a.NS * PXW - 1u | 1u);
+  for (uint32_t b = 0; b < nbits; b++) {
+    const uint64_t bal = __ballot((lk >> b) & 1u);
+    grp &= ((lk >> b) & 1u) ? bal : ~bal;
   }
+  const uint64_t below = grp & ((1ull << lane) - 1ull);
+  const uint64_t above = lane == 63u ? 0ull : (grp >> (lane + 1u));
+  const uint32_t prev = below ? 63u - (uint32_t)__builtin_clzll(below) : lane;
+  const uint32_t prevSlot = (uint32_t)__shfl((int)slot, (int)prev);
+  if (accept) {
+    uint32_t* hp = &L.heads[tag * FineLds<TILE>::NPX + pix];
+    const uint32_t next = below ? prevSlot : *hp;
+    if (slot < a.ldsPool) L.poolNext[slot] = next;
+    else gpool[slot - a.ldsPool].z = next;
+    if (above == 0ull) *hp = slot;
+  }
+  wave_sync();
 }
 
 // Cheap tests of the records [rec0, rec0 + nb) for wave `w`. Candidates are
@@ -1322,11 +1333,9 @@ __device__ __forceinline__ void fine_cheap_batch(const FineArgs& a, const FineLd
 // Projects segment `key` into record `j`: quad, depths, span frame, pixel box
 // relative to the tile, nearest depth and the per-segment interpolation terms.
 template <int TILE>
-__device__ __forceinline__ void fine_project(const FineArgs& a, const FineLds<TILE>& L, uint32_t j, uint32_t key,
-                                             f2 vpf, uint32_t bx0, uint32_t by0) {
+__device__ __forceinline__ void fine_record(const FineArgs& a, const FineLds<TILE>& L, uint32_t j, uint32_t key,
+                                            const Seg& sg, uint32_t bx0, uint32_t by0) {
   constexpr uint32_t RB = fine_batch<TILE>();
-  const fh_render_uniforms* __restrict__ u = a.u;
-  const Seg sg = project_segment(u, a.pos, a.tan, a.P, key >> 8, key & 0xffu, vpf, u->fiberRadius);
   int x0, y0, x1, y1;
   seg_pixel_bbox(sg, x0, y0, x1, y1);
   const auto& R = L.recs;
@@ -1357,6 +1366,26 @@ __device__ __forceinline__ void fine_project(const FineArgs& a, const FineLds<TI
 }
 
 template <int TILE>
+__device__ __forceinline__ void fine_project(const FineArgs& a, const FineLds<TILE>& L, uint32_t j, uint32_t key,
+                                             f2 vpf, uint32_t bx0, uint32_t by0) {
+  const fh_render_uniforms* __restrict__ u = a.u;
+  const Seg sg = project_segment(u, a.pos, a.tan, a.P, key >> 8, key & 0xffu, vpf, u->fiberRadius);
+  fine_record<TILE>(a, L, j, key, sg, bx0, by0);
+}
+
+// The two points and tangents of a segment, loaded ahead of its projection.
+struct SegIn {
+  uint32_t key;
+  float4 p0, t0, p1, t1;
+};
+__device__ __forceinline__ void seg_load(const FineArgs& a, uint32_t key, SegIn& s) {
+  const size_t k = (size_t)(key >> 8) * a.P + (key & 0xffu);
+  s.key = key;
+  s.p0 = a.pos[k]; s.t0 = a.tan[k];
+  s.p1 = a.pos[k + 1]; s.t1 = a.tan[k + 1];
+}
+
+template <int TILE>
 __device__ __forceinline__ void fine_raster_body(FineArgs a, uint32_t block, uint32_t blocks) {
   constexpr uint32_t NPX = TILE * TILE;
   constexpr uint32_t NW = fine_waves<TILE>();
@@ -1367,8 +1396,7 @@ __device__ __forceinline__ void fine_raster_body(FineArgs a, uint32_t block, uin
   L.heads = reinterpret_cast<uint32_t*>(smem + (size_t)RB * kRecFields * 4);
   L.db = reinterpret_cast<float*>(L.heads + (size_t)a.NS * NPX);
   L.done = reinterpret_cast<uint32_t*>(L.db + NPX);
-  L.claim = L.done + NPX;
-  L.ring = L.claim + NPX;
+  L.ring = L.done + NPX;
   L.wsc = L.ring + NW * kFineRing;
   L.poolColor = reinterpret_cast<uint2*>(L.wsc + NW * 6u * 64u);
   L.poolNext = reinterpret_cast<uint32_t*>(L.poolColor + a.ldsPool);
@@ -1395,7 +1423,6 @@ __device__ __forceinline__ void fine_raster_body(FineArgs a, uint32_t block, uin
   uint4* gpool = a.slicePool + (size_t)block * a.perProc;
 
   for (uint32_t i = tid; i < a.NS * NPX; i += blockDim.x) L.heads[i] = kInvalidPtr;
-  if (pixThread) L.claim[tid] = 64u;
 
   while (true) {
     __syncthreads();
@@ -1465,11 +1492,24 @@ __device__ __forceinline__ void fine_raster_body(FineArgs a, uint32_t block, uin
         __syncthreads();
         fine_cheap_batch<TILE>(a, L, gpool, s_binRec[bin], n, bx0, by0, td, invSegs, &s_sliceOff);
       }
+      // Batches of a bin of several: each batch's points and tangents are
+      // loaded while the previous batch is rasterized, and its keys one batch
+      // before that, so a projection waits on no memory.
+      SegIn cur;
+      uint32_t nextKey = 0;
+      if (!whole) {
+        if (tid < min(a.batch, n)) seg_load(a, a.binKey[start + tid], cur);
+        if (tid < a.batch && a.batch + tid < n) nextKey = a.binKey[start + a.batch + tid];
+      }
       for (uint32_t b0 = 0; !whole && b0 < n; b0 += a.batch) {
         const uint32_t nb = min(a.batch, n - b0);
         __syncthreads();
-        if (tid < nb) fine_project<TILE>(a, L, tid, a.binKey[start + b0 + tid], vpf, bx0, by0);
+        if (tid < nb)
+          fine_record<TILE>(a, L, tid, cur.key,
+                            project_segment_v(u, cur.p0, cur.t0, cur.p1, cur.t1, vpf, u->fiberRadius), bx0, by0);
         __syncthreads();
+        if (tid < a.batch && b0 + a.batch + tid < n) seg_load(a, nextKey, cur);
+        if (tid < a.batch && b0 + 2u * a.batch + tid < n) nextKey = a.binKey[start + b0 + 2u * a.batch + tid];
         fine_cheap_batch<TILE>(a, L, gpool, 0u, nb, bx0, by0, td, invSegs, &s_sliceOff);
         __syncthreads();
         if (s_sliceOff >= a.perProc) break;  // slice memory exhausted (:252-264)
@@ -1783,7 +1823,7 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
   const uint32_t nw = T == 8 ? 1u : T / 2u;
   const size_t ldsTarget = std::min<size_t>(T == 8 ? 40 * 1024 : (T == 16 ? 78 * 1024 : 150 * 1024),
                                             ctx->max_lds - 256);
-  const size_t fixed = (size_t)a.batch * kRecFields * 4 + (size_t)npx * 4 * (a.NS + 3) +
+  const size_t fixed = (size_t)a.batch * kRecFields * 4 + (size_t)npx * 4 * (a.NS + 2) +
                        (size_t)nw * (kFineRing + 6 * 64) * 4;
   FH_REQUIRE(fixed + 12 * 64 <= ldsTarget, "fh_fine_raster: slicesPerPixel too large for the tile size");
   a.ldsPool = (uint32_t)std::min<size_t>((ldsTarget - fixed) / 12, a.perProc);

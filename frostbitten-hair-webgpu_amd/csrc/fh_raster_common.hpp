@@ -17,11 +17,8 @@ struct Seg {
 
 // projectHairPoint — swRasterizeHair.wgsl.ts:76-105. The tangent is
 // transformed with w = 1, as in the reference.
-__device__ __forceinline__ void project_point(const fh_render_uniforms* __restrict__ u,
-                                              const float4* __restrict__ pos,
-                                              const float4* __restrict__ tan, size_t k, f2 vp,
-                                              float fiberRadius, f2& v0, f2& v1, f2& dd) {
-  const float4 pk = pos[k], tk = tan[k];
+__device__ __forceinline__ void project_point_v(const fh_render_uniforms* __restrict__ u, float4 pk, float4 tk,
+                                                f2 vp, float fiberRadius, f2& v0, f2& v1, f2& dd) {
   const f4 p0 = mul(u->modelMatrix, mk4(pk.x, pk.y, pk.z, 1.0f));
   const f4 t0 = mul(u->modelMatrix, mk4(tk.x, tk.y, tk.z, 1.0f));
   const f3 cam = mk3(u->cameraPosition[0], u->cameraPosition[1], u->cameraPosition[2]);
@@ -36,6 +33,13 @@ __device__ __forceinline__ void project_point(const fh_render_uniforms* __restri
   dd = mk2(an.z, bn.z);
 }
 
+__device__ __forceinline__ void project_point(const fh_render_uniforms* __restrict__ u,
+                                              const float4* __restrict__ pos,
+                                              const float4* __restrict__ tan, size_t k, f2 vp,
+                                              float fiberRadius, f2& v0, f2& v1, f2& dd) {
+  project_point_v(u, pos[k], tan[k], vp, fiberRadius, v0, v1, dd);
+}
+
 // projectHairSegment — swRasterizeHair.wgsl.ts:39-68
 __device__ __forceinline__ Seg project_segment(const fh_render_uniforms* __restrict__ u,
                                                const float4* __restrict__ pos,
@@ -48,6 +52,19 @@ __device__ __forceinline__ Seg project_segment(const fh_render_uniforms* __restr
   project_point(u, pos, tan, k, vp, fiberRadius, a, b, dd);
   r.v00 = a; r.v01 = b; r.depths.x = dd.x; r.depths.y = dd.y;
   project_point(u, pos, tan, k + 1, vp, fiberRadius, a, b, dd);
+  r.v10 = a; r.v11 = b; r.depths.z = dd.x; r.depths.w = dd.y;
+  return r;
+}
+
+// projectHairSegment from the segment's two points and tangents, loaded by
+// the caller
+__device__ __forceinline__ Seg project_segment_v(const fh_render_uniforms* __restrict__ u, float4 p0, float4 t0,
+                                                 float4 p1, float4 t1, f2 vp, float fiberRadius) {
+  Seg r;
+  f2 a, b, dd;
+  project_point_v(u, p0, t0, vp, fiberRadius, a, b, dd);
+  r.v00 = a; r.v01 = b; r.depths.x = dd.x; r.depths.y = dd.y;
+  project_point_v(u, p1, t1, vp, fiberRadius, a, b, dd);
   r.v10 = a; r.v11 = b; r.depths.z = dd.x; r.depths.w = dd.y;
   return r;
 }
