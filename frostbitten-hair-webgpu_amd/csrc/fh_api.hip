@@ -496,6 +496,13 @@ int fh_scratch_reserve(fh_ctx* ctx, uint64_t bytes) {
   return fh::ensure_scratch(ctx, (size_t)bytes);
 }
 
+int fh_set_raster_band_cap(fh_ctx* ctx, uint32_t bands) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(bands <= (1u << 20), "fh_set_raster_band_cap: at most 2^20 bands");
+  ctx->band_cap = bands;
+  return FH_OK;
+}
+
 int fh_set_persistent_blocks(fh_ctx* ctx, uint32_t blocks) {
   FH_CHECK_CTX(ctx);
   ctx->persist_blocks = blocks;

@@ -48,6 +48,7 @@ struct RasterArgs {
   uint32_t* bigCount;       // queued bands of large triangles
   uint2* big;               // (triangle, band)
   uint32_t W, H, P, nTri, shadow;
+  uint32_t bigCap;          // queued bands at most (<= kBigCap)
 };
 
 // Clip-space corner `side` (0: +thickness, 'right'; 1: -thickness, 'left') of
@@ -134,13 +135,13 @@ struct HairRaster {
         } else if (k == 0u) {
           const uint32_t bands = (uint32_t)((bh + kBandRows - 1) / kBandRows);
           const uint32_t first = atomicAdd(a.bigCount, bands);
-          if (first + bands <= kBigCap) {
+          if (first + bands <= a.bigCap) {
             for (uint32_t q = 0; q < bands; q++) a.big[first + q] = make_uint2(t, q);
           } else {
             // a reservation that crosses the cap leaves its slots below the
             // cap to a sentinel triangle id (skipped by HairRasterBig, which
             // reads every slot below min(count, cap)); this wave draws it
-            for (uint32_t q = first; q < kBigCap; q++) a.big[q] = make_uint2(kBigSentinel, 0u);
+            for (uint32_t q = first; q < a.bigCap; q++) a.big[q] = make_uint2(kBigSentinel, 0u);
             bigMask |= 1u << h;
           }
         }
@@ -169,7 +170,7 @@ struct HairRasterBig {
   static constexpr uint32_t kThreads = 256;
   static constexpr const char* kName = "hair_raster_big";
   static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) {
-    const uint32_t count = min(*a.bigCount, kBigCap);
+    const uint32_t count = min(*a.bigCount, a.bigCap);
     for (uint32_t i = block; i < count; i += blocks) {
       const uint2 e = a.big[i];
       if (e.x >= a.nTri) continue;  // kBigSentinel
@@ -346,6 +347,7 @@ int raster_hair(fh_ctx* ctx, RasterArgs& a) {
   const uint64_t px = (uint64_t)a.W * a.H;
   const size_t bigOff = align_up(px * 8, 256);
   int rc = ensure_scratch(ctx, bigOff + 256 + (size_t)kBigCap * 8);
+  a.bigCap = ctx->band_cap ? ctx->band_cap : kBigCap;
   if (rc) return rc;
   a.key = static_cast<unsigned long long*>(ctx->scratch);
   a.bigCount = reinterpret_cast<uint32_t*>(static_cast<char*>(ctx->scratch) + bigOff);

@@ -57,6 +57,9 @@ struct fh_ctx {
   int num_cus = 0;
   // Workgroups of a grid-stride ("persistent") launch; 0: 8 per CU.
   uint32_t persist_blocks = 0;
+  // Bands of large triangles the software rasterizer queues at most (0: the
+  // default, 2^20); the triangles whose bands do not fit draw themselves.
+  uint32_t band_cap = 0;
   size_t max_lds = 64 * 1024;  // LDS bytes one workgroup may allocate
   // Grow-on-demand scratch owned by the context (fine-pass gather lists,
   // per-bin offsets, prefix-scan temporaries).

@@ -141,6 +141,7 @@ struct MeshArgs {
   const float* ao;
   uint32_t shadowW, aoW, aoH;
   uint32_t W, H, shadowPass;
+  uint32_t bigCap;          // queued bands at most (<= kBigCap)
 };
 
 __device__ __forceinline__ uint32_t mesh_of(const MeshArgs& a, uint32_t t) {
@@ -219,10 +220,10 @@ struct MeshRaster {
           } else if (k == 0u) {
             const uint32_t bands = (uint32_t)((bh + kBandRows - 1) / kBandRows);
             const uint32_t first = atomicAdd(a.bigCount, bands);
-            if (first + bands <= kBigCap) {
+            if (first + bands <= a.bigCap) {
               for (uint32_t q = 0; q < bands; q++) a.big[first + q] = make_uint2(t, q);
             } else {
-              for (uint32_t q = first; q < kBigCap; q++) a.big[q] = make_uint2(kBigSentinel, 0u);
+              for (uint32_t q = first; q < a.bigCap; q++) a.big[q] = make_uint2(kBigSentinel, 0u);
               wave = true;
             }
           }
@@ -252,7 +253,7 @@ struct MeshRasterBig {
   static constexpr uint32_t kThreads = 256;
   static constexpr const char* kName = "mesh_raster_big";
   static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) {
-    const uint32_t count = min(*a.bigCount, kBigCap);
+    const uint32_t count = min(*a.bigCount, a.bigCap);
     for (uint32_t i = block; i < count; i += blocks) {
       const uint2 e = a.big[i];
       if (e.x >= a.nTri) continue;  // kBigSentinel
@@ -507,6 +508,7 @@ int raster_meshes(fh_ctx* ctx, MeshArgs& a) {
   const uint64_t px = (uint64_t)a.W * a.H;
   const size_t bigOff = align_up(px * 8, 256);
   int rc = ensure_scratch(ctx, bigOff + 256 + (size_t)kBigCap * 8);
+  a.bigCap = ctx->band_cap ? ctx->band_cap : kBigCap;
   if (rc) return rc;
   a.key = static_cast<unsigned long long*>(ctx->scratch);
   a.bigCount = reinterpret_cast<uint32_t*>(static_cast<char*>(ctx->scratch) + bigOff);

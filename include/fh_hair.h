@@ -740,6 +740,14 @@ int fh_scratch_reserve(fh_ctx* ctx, uint64_t bytes);
  * depend on this value. */
 int fh_set_persistent_blocks(fh_ctx* ctx, uint32_t blocks);
 
+/* Caps the queue of large-triangle bands of the software rasterizer
+ * (fh_hw_hair, fh_shadow_map_hair, fh_shadow_map_meshes, fh_draw_meshes*)
+ * at `bands` (at most 2^20, the default; 0 restores it). A triangle whose
+ * bands do not fit is drawn by its own wave instead: the results do not
+ * depend on this value. No counterpart in the reference: it exists so that
+ * the overflow path can be exercised without a scene of 2^20 bands. */
+int fh_set_raster_band_cap(fh_ctx* ctx, uint32_t bands);
+
 /* Starts recording: until fh_record_end, the entry points called on `ctx`
  * check their arguments and keep their kernel launches (grid, LDS bytes,
  * arguments) instead of enqueuing them. Nothing runs on the device, so the

@@ -156,3 +156,47 @@ def test_split_pass_equals_fused(gpu):
     r.drawMeshesPass.cmdShadeMeshes(ctx, r.meshes, r.meshVisibility)
     torch.cuda.synchronize()
     assert torch.equal(before, r.hdrRenderTexture)
+
+
+def test_band_queue_cap_same_result(gpu):
+    """A triangle whose bands do not fit the large-triangle queue draws
+    itself (fh_set_raster_band_cap forces that with a queue of 2 bands): the
+    meshes' depth, normals, colour and shadow map and the hair's depth and
+    normals at 4K equal those with the full queue, bit for bit."""
+    import torch
+    wl, r = _renderer("c3", strands=3000, width=3840, height=2160, shadow_size=4096)
+    # the scene has triangles larger than one rasterizer work item (1,024 px boxes)
+    mvp = np.frombuffer(r.render_uniforms["mvpMatrix"].tobytes(), np.float32).reshape(4, 4).T
+    big = 0
+    for m in r.meshList:
+        if m.get("preview"):
+            continue
+        c = np.c_[m["positions"], np.ones(len(m["positions"]))] @ mvp.T
+        ok = c[:, 3] > 0
+        s = np.c_[(c[:, 0] / c[:, 3] * 0.5 + 0.5) * wl.width, (0.5 - c[:, 1] / c[:, 3] * 0.5) * wl.height]
+        tri = m["indices"]
+        sx, sy = s[tri, 0], s[tri, 1]
+        area = (sx.max(1) - sx.min(1)) * (sy.max(1) - sy.min(1))
+        big += int(((area > 1024) & ok[tri].all(1)).sum())
+    print(f"band queue: {big} mesh triangles with boxes over 1,024 px", flush=True)
+    assert big > 50
+
+    def draw():
+        ctx = r.ctx()
+        r.backgroundPass.cmdDraw(ctx)
+        r.shadowMapPass.cmdUpdateShadowMap(ctx, r.hair, meshes=r.meshes)
+        r.drawMeshesPass.cmdDrawMeshes(ctx, r.meshes, r.meshDepthTexture, r.meshNormalsTexture)
+        r.hwHairPass.cmdDrawHair(ctx, r.hair, r.meshDepthTexture, r.meshNormalsTexture)
+        torch.cuda.synchronize()
+        return [t.clone() for t in (r.shadowTexture, r.meshDepthTexture, r.meshNormalsTexture, r.hdrRenderTexture,
+                                     r.depthTexture, r.normalsTexture)]
+
+    full = draw()
+    r.native.call("fh_set_raster_band_cap", 2)
+    try:
+        capped = draw()
+    finally:
+        r.native.call("fh_set_raster_band_cap", 0)
+    for a, b, name in zip(full, capped, ("shadow", "mesh depth", "mesh normals", "hdr", "depth", "normals")):
+        assert torch.equal(a, b), name
+    assert (_np(r.meshDepthTexture) < 1).mean() > 0.01
