@@ -111,21 +111,19 @@ def full_frame(wl, r0, steps: int, dev, branches: bool = True, mesh_passes: bool
     r = PS.HairRenderer(wl, dev, inputs=inputs, hw_passes=True, branches=branches, mesh_passes=mesh_passes)
     ring = torch.from_numpy(np.concatenate([r.sim_uniforms(f).view(np.uint8).reshape(-1)
                                             for f in range(2 * steps + 4)])).to(dev).view(2 * steps + 4, -1)
+    r.use_uniform_ring(ring, r.frameIdx)
     r.beforeFirstFrame()
-    r.frame()
+    r.frame(upload_uniforms=False)
     torch.cuda.synchronize()
     pipelined = branches
     if branches:  # pipelined frames, as the main measurement
-        r.simulationUniforms.copy_(ring[r.frameIdx])
         r.pipeline_begin(upload_uniforms=False)
-        r.simulationUniforms.copy_(ring[r.frameIdx])
         r.frame_pipelined(upload_uniforms=False)
         r.capture_pipelined()
     else:
         r.capture_frames()
 
     def step():
-        r.simulationUniforms.copy_(ring[r.frameIdx])
         if branches:
             r.frame_pipelined_graph(upload_uniforms=False)
         else:
@@ -513,13 +511,15 @@ def plain_run(kind: str, steps: int, warmup: int, args, dev) -> tuple:
     # per-frame simulation uniforms (frameIdx changes), uploaded once up front
     ring = torch.from_numpy(np.concatenate([r.sim_uniforms(f).view(np.uint8).reshape(-1) for f in range(total)]))
     ring = ring.to(dev).view(total, -1)
+    # each simulation step copies its row on the device (fh_copy_indexed), so
+    # no host call sits between two frames' graphs
+    r.use_uniform_ring(ring, r.frameIdx)
     print(f"# setup {time.perf_counter() - t_setup:.1f} s, workload {wl.name}: {wl.strands}x{wl.points} "
           f"{wl.width}x{wl.height}", file=sys.stderr, flush=True)
 
     stages = ["sim", "tiles", "sort", "fine", "post", "shading"]
 
     def frame(events=None):
-        r.simulationUniforms.copy_(ring[r.frameIdx])
         ctx = r.ctx()
         rec = (lambda k: events[k].record()) if events else (lambda k: None)
         rec(0)
@@ -544,7 +544,6 @@ def plain_run(kind: str, steps: int, warmup: int, args, dev) -> tuple:
         r.frameIdx += 1
 
     def graph_frame(events=None):
-        r.simulationUniforms.copy_(ring[r.frameIdx])
         if events:
             events[0].record()
         r.frame_graph(upload_uniforms=False)
@@ -552,7 +551,6 @@ def plain_run(kind: str, steps: int, warmup: int, args, dev) -> tuple:
             events[6].record()
 
     def pipelined_frame(events=None):
-        r.simulationUniforms.copy_(ring[r.frameIdx])
         if events:
             events[0].record()
         r.frame_pipelined_graph(upload_uniforms=False)
@@ -588,9 +586,7 @@ def plain_run(kind: str, steps: int, warmup: int, args, dev) -> tuple:
     # step of frame f beside the render passes of frame f-1.
     unpipelined = None
     if pipelined:
-        r.simulationUniforms.copy_(ring[r.frameIdx])
         r.pipeline_begin(upload_uniforms=False)
-        r.simulationUniforms.copy_(ring[r.frameIdx])
         r.frame_pipelined(upload_uniforms=False)  # every call once before capture
         r.capture_pipelined()
         elapsed, evs = timed_pair(pipelined_frame)

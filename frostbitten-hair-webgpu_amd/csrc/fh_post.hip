@@ -240,6 +240,27 @@ struct CopyWords {
   }
 };
 
+// dst <- record (*counter mod count) of src, then ++*counter (one workgroup).
+struct CopyIndexed {
+  struct Args {
+    uint32_t* dst;
+    const uint32_t* src;
+    uint32_t words, count;
+    uint32_t* counter;
+  };
+  static constexpr uint32_t kThreads = 256;
+  static constexpr const char* kName = "copy_indexed";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) {
+    (void)block;
+    (void)blocks;
+    const uint32_t c = *a.counter;
+    const uint64_t base = (uint64_t)(c % a.count) * a.words;
+    for (uint32_t i = threadIdx.x; i < a.words; i += kThreads) a.dst[i] = a.src[base + i];
+    __syncthreads();  // every thread has read the counter
+    if (threadIdx.x == 0) *a.counter = c + 1u;
+  }
+};
+
 }  // namespace
 
 namespace fh {
@@ -272,6 +293,16 @@ const float* srgb_thresholds() {
 }  // namespace fh
 
 extern "C" {
+
+int fh_copy_indexed(fh_ctx* ctx, void* dst, const void* src, uint64_t bytes, uint32_t count, uint32_t* counter) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(dst && src && counter, "fh_copy_indexed: null buffer");
+  FH_REQUIRE(bytes % 4 == 0 && bytes > 0 && bytes / 4 < (1ull << 32), "fh_copy_indexed: bytes must be a multiple of 4");
+  FH_REQUIRE(count > 0, "fh_copy_indexed: count must be positive");
+  return fh::launch<CopyIndexed>(ctx, 1, 0, CopyIndexed::Args{static_cast<uint32_t*>(dst),
+                                                               static_cast<const uint32_t*>(src),
+                                                               (uint32_t)(bytes / 4), count, counter});
+}
 
 int fh_copy(fh_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
   FH_CHECK_CTX(ctx);

@@ -61,12 +61,6 @@ class HairObject:
 
     def allocShading(self, shading_points: int, device) -> None:
         self.shading = _dev(A.initial_shading(self.strandsCount, shading_points), device)
-        # the buffer a pipelined frame's shading pass writes while the fine
-        # pass still reads `shading` (same layout)
-        self.shadingBack = self.shading.clone()
-
-    def swapShadingBuffers(self) -> None:
-        self.shading, self.shadingBack = self.shadingBack, self.shading
 
     def getRenderedStrandCount(self) -> int:
         """hairObject.ts:52-57"""
@@ -505,7 +499,23 @@ class HairRenderer:
         if self.branches:
             self.native.branch_join()
 
+    def use_uniform_ring(self, ring: torch.Tensor, start: int) -> None:
+        """Per-frame simulation uniforms from a device ring (row k: the bytes
+        of frame k's uniforms): every simulation step then starts by copying
+        row (counter mod rows) into `simulationUniforms` on the device and
+        advancing the counter (fh_copy_indexed), starting at row `start`, so
+        that frames replayed from graphs need no host copy between them."""
+        self.uniformRing = ring.contiguous()
+        self.uniformCounter = torch.tensor([start], dtype=torch.int32, device=self.device)
+
+    def _next_uniforms(self) -> None:
+        ring = getattr(self, "uniformRing", None)
+        if ring is not None:
+            self.native.call("fh_copy_indexed", N.ptr(self.simulationUniforms), N.ptr(ring), ring.shape[1],
+                             ring.shape[0], N.ptr(self.uniformCounter))
+
     def simulateHair(self, ctx: PassCtx) -> None:
+        self._next_uniforms()
         self.gridPreSimPass.cmdUpdateGridsBeforeSim(ctx, self.hair)
         self.simPass.cmdSimulateHairPositions(ctx, self.hair)
         # the grid accumulation feeds only the next frame's grid passes
@@ -530,7 +540,7 @@ class HairRenderer:
         self.shadowMapPass.cmdUpdateShadowMap(ctx, self.hair, meshes=self.meshes)
         self.drawMeshesPass.cmdDrawMeshes(ctx, self.meshes, self.meshDepthTexture, self.meshNormalsTexture)
 
-    def drawHair(self, ctx: PassCtx, shading_first: bool = False) -> None:
+    def drawHair(self, ctx: PassCtx) -> None:
         """cmdDrawScene's hair part (renderer.ts:262-295) and the present pass
         (:205).
 
@@ -541,12 +551,9 @@ class HairRenderer:
         read, so they run on the branch beside the tile, sort and fine passes
         (which read the meshes' depth, not the hair's). The shading pass writes
         the shading points the fine pass reads, so it starts after the fine
-        pass, beside the combine and present passes.
-
-        shading_first: the shading pass writes the hair's back shading buffer
-        on the branch before the tile pass starts, and the buffers swap when
-        the calls are issued (the pipelined frame); the fine pass still reads
-        the previous frame's shading points."""
+        pass, beside the combine and present passes; a shading pass still
+        running when the fine pass starts would keep the fine pass's
+        persistent workgroups from the CUs."""
         # the scene copy and the three passes' clears touch disjoint buffers:
         # one launch for all of them
         self.native.clear_batch_begin()
@@ -563,7 +570,7 @@ class HairRenderer:
             # for the shadow map) onto the branch measured slower: the branch
             # already holds the simulation step, the hair draw and AO.
             self.drawMeshes(ctx)
-        if self.hw_passes or shading_first:
+        if self.hw_passes:
             self._branch_begin()
             if self.mesh_passes:
                 self.hwHairPass.cmdDrawHair(ctx, self.hair, self.meshDepthTexture, self.meshNormalsTexture)
@@ -572,23 +579,13 @@ class HairRenderer:
                 self.shadowMapPass.cmdUpdateShadowMap(ctx, self.hair, self.inputs["shadowMap"])
                 self.hwHairPass.cmdDrawHair(ctx, self.hair, self.inputs["meshDepth"], self.inputs["meshNormals"])
                 self.aoPass.cmdCalcAo(ctx)
-            if shading_first:
-                front = self.hair.shading
-                self.hair.shading = self.hair.shadingBack
-                try:
-                    self.shadingPass.cmdComputeShadingPoints(ctx, self.hair)
-                finally:
-                    self.hair.shading = front
             self._branch_end()
         self.tilesPass.cmdDrawHairToTiles(ctx, self.hair)
         self.sortPass.cmdSortHairTiles(ctx)
         self.finePass.cmdRasterizeSlicesHair(ctx, self.hair, self.tilesPass)
-        if shading_first:
-            self.hair.swapShadingBuffers()
-        else:
-            self._branch_begin()
-            self.shadingPass.cmdComputeShadingPoints(ctx, self.hair)
-            self._branch_end()
+        self._branch_begin()
+        self.shadingPass.cmdComputeShadingPoints(ctx, self.hair)
+        self._branch_end()
         self.hairCombinePass.cmdCombineRasterResults(ctx)
         if self.hw_passes:  # the present pass's debug views read the hair depth, normals and AO
             self.join_branch()
@@ -628,6 +625,7 @@ class HairRenderer:
         ctx = self.ctx()
         drawn = self.hair.current  # frame f-1's positions
         self._branch_begin()
+        self._next_uniforms()
         self.gridPreSimPass.cmdUpdateGridsBeforeSim(ctx, self.hair)
         self.simPass.cmdSimulateHairPositions(ctx, self.hair)  # swaps the pair
         self.gridPostSimPass.cmdUpdateGridsAfterSim(ctx, self.hair)
@@ -635,7 +633,7 @@ class HairRenderer:
         stepped = self.hair.current
         self.hair.current = drawn
         try:
-            self.drawHair(ctx, shading_first=self.branches)
+            self.drawHair(ctx)
         finally:
             self.hair.current = stepped
 
@@ -657,7 +655,6 @@ class HairRenderer:
     def capture_pipelined(self) -> list:
         """capture_frames() for pipelined frames (two graphs, one per parity
         of the position pair); replayed by frame_pipelined_graph()."""
-        self._pshadingBase = self.hair.shading  # both pairs swap once per frame
         return self._capture(self._pipelined_calls, "_pgraphs", "_pgraphBase")
 
     def frame_pipelined_graph(self, upload_uniforms: bool = True) -> None:
@@ -665,9 +662,6 @@ class HairRenderer:
         if upload_uniforms:
             self.upload_sim_uniforms()
         k = self.hair.current ^ self._pgraphBase
-        if self.branches:
-            assert (self.hair.shading is self._pshadingBase) == (k == 0), "shading buffers out of step"
-            self.hair.swapShadingBuffers()
         self._pgraphs[k].launch()
         self.hair.swapPositionBuffersAfterSimIntegration()
         self.frameIdx += 1
@@ -678,7 +672,8 @@ class HairRenderer:
         the frame's GPUCommandEncoder once. Every call must have run once
         before (so no scratch grows while recording); the graphs read the
         simulation uniforms from `simulationUniforms`, whose contents the
-        caller updates per frame."""
+        caller updates per frame, or which each simulation step fills from
+        the device ring of use_uniform_ring()."""
         def calls():
             ctx = self.ctx()
             self.simulateHair(ctx)

@@ -564,6 +564,14 @@ int fh_shadow_map_meshes(fh_ctx* ctx, const fh_render_uniforms* uniforms, const 
  * out-of-scope mesh passes) without memcpy nodes. */
 int fh_copy(fh_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 
+/* Copies record (*counter mod count) of src (count records of `bytes` bytes,
+ * a multiple of 4) to dst and adds one to *counter (a device u32), all on the
+ * device. The per-frame uploads of the reference (queue.writeBuffer of the
+ * simulation uniforms, simulationUniformsBuffer.ts:86-152) become one such
+ * call at the start of a frame over a device ring of the frames' uniforms,
+ * so that replays of a captured frame need no host call between them. */
+int fh_copy_indexed(fh_ctx* ctx, void* dst, const void* src, uint64_t bytes, uint32_t count, uint32_t* counter);
+
 /* ------------------------------------------------------------------------ */
 /* Uniform packing (host side)                                               */
 /* ------------------------------------------------------------------------ */

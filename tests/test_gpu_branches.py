@@ -164,3 +164,45 @@ def test_branch_errors(gpu):
     ctx.sync()
     b = buf.cpu().numpy()
     assert (b[:512] == 0).all() and (b[512:] == 5).all()
+
+
+@pytest.mark.parametrize("pipelined", [False, True])
+def test_uniform_ring_matches_uploads(gpu, pipelined):
+    """Frames whose simulation uniforms come from a device ring
+    (fh_copy_indexed at the start of each simulation step, inside the
+    replayed graphs) leave what frames with a host upload per frame leave."""
+    import torch
+    from frostbitten_hair_webgpu_amd import assets as A
+    from frostbitten_hair_webgpu_amd import passes as PS
+    wl = A.make_workload("c2", seed=79, strands=900, width=400, height=288, points=16)
+    print(f"uniform ring: pipelined={pipelined}", flush=True)
+    plain = PS.HairRenderer(wl, "cuda:0", shadow_size=512, branches=False)
+    inputs = {k: v.cpu().numpy() for k, v in plain.inputs.items()}
+    ringed = PS.HairRenderer(wl, "cuda:0", shadow_size=512, inputs=inputs, branches=True)
+    frames = 6
+    rows = np.stack([ringed.sim_uniforms(f).view(np.uint8).reshape(-1) for f in range(frames + 2)])
+    ringed.use_uniform_ring(torch.from_numpy(rows).to("cuda:0"), 0)
+    plain.beforeFirstFrame()
+    ref = []
+    for _ in range(frames):
+        plain.frame()
+        torch.cuda.synchronize()
+        ref.append(_state(plain))
+    ringed.beforeFirstFrame()
+    if pipelined:
+        ringed.pipeline_begin(upload_uniforms=False)
+        ringed.frame_pipelined(upload_uniforms=False)
+        ringed.capture_pipelined()
+        for f in range(2, frames):
+            ringed.frame_pipelined_graph(upload_uniforms=False)
+            torch.cuda.synchronize()
+            sim, _ = _split(_state(ringed))
+            _same(_split(ref[f])[0], sim, f"simulation state of frame {f}")
+    else:
+        ringed.frame(upload_uniforms=False)
+        ringed.capture_frames()
+        for f in range(1, frames):
+            ringed.frame_graph(upload_uniforms=False)
+            torch.cuda.synchronize()
+            _same(ref[f], _state(ringed), f"frame {f}")
+    assert int(ringed.uniformCounter.item()) == frames
