@@ -62,11 +62,11 @@ def kernel_bytes(wl, n_pairs: int, tiles: int) -> dict:
         "grid_post_sim": 36 * N + 16 * d3,
         # positions + tangents, mesh depth, list element + slot tag per pair, tile headers
         "tiles_draw": 32 * N + 4 * WH + 12 * n_pairs + 16 * tiles * B,
-        "bin_scatter": 12 * n_pairs + 4 * tiles * B,
-        # bin_sort_small reads every range header and sorts the short ranges,
-        # bin_sort_large the long ones; together each key is read and written once
-        "bin_sort_small": 8 * n_pairs + 20 * tiles * B,
-        "bin_sort_large": 8 * n_pairs,
+        # slot tag + list element read, key written per pair; per range the
+        # scan's end and the list length read and the start written
+        "bin_scatter": 12 * n_pairs + 12 * tiles * B,
+        # every range header and start read; each key read and written once
+        "bin_sort": 8 * n_pairs + 20 * tiles * B,
         # tile headers, sorted keys, positions + tangents, mesh depth, shading
         # points, framebuffer read + write, tile list
         "fine_raster": 16 * tiles * B + 4 * n_pairs + 32 * N + 4 * WH + 16 * S * Ps + 32 * WH + 4 * tiles,
@@ -77,6 +77,8 @@ def kernel_bytes(wl, n_pairs: int, tiles: int) -> dict:
         "hair_combine": 32 * WH,
         # HDR texel read, rgba8 texel written
         "present": 12 * WH,
+        # rasterizer pixel and scene texel read, HDR and rgba8 texels written
+        "combine_present": 36 * WH,
     }
 
 
@@ -94,7 +96,7 @@ def stage_bytes(wl, n_pairs: int, tiles: int) -> dict:
         "sort": 12 * tiles + 1024,
         "fine": 16 * tiles * B + 8 * n_pairs + 32 * N + 4 * WH + 16 * S * Ps + 32 * WH + 4 * tiles,
         "shading": 32 * N + 4 * WH + WH // 2 + 2048 * 2048 * 4 + 16 * S * Ps,
-        "post": 16 * WH + 32 * WH + 12 * WH,
+        "post": 36 * WH,
     }
 
 
@@ -525,7 +527,6 @@ def plain_run(kind: str, steps: int, warmup: int, args, dev) -> tuple:
         rec(0)
         r.simulateHair(ctx)
         rec(1)
-        r.drawScene(ctx)
         r.tilesPass.cmdClearBeforeRender(ctx)
         r.finePass.cmdClearBeforeRender(ctx)
         r.sortPass.cmdClearBeforeRender(ctx)
@@ -535,8 +536,7 @@ def plain_run(kind: str, steps: int, warmup: int, args, dev) -> tuple:
         rec(3)
         r.finePass.cmdRasterizeSlicesHair(ctx, r.hair, r.tilesPass)
         rec(4)
-        r.hairCombinePass.cmdCombineRasterResults(ctx)
-        r.presentPass.cmdDraw(ctx, r.screenTexture)
+        r.hairCombinePass.cmdCombineAndPresent(ctx, r.presentPass, r.screenTexture, r.inputs["hdrScene"])
         rec(5)
         r.shadingPass.cmdComputeShadingPoints(ctx, r.hair)
         r.join_branch()  # the grid accumulation ran beside the render passes
@@ -632,9 +632,8 @@ def plain_run(kind: str, steps: int, warmup: int, args, dev) -> tuple:
                          "grid_post_sim"),
                   "tiles": ("fill_many", "tile_depth_bounds", "tiles_draw", "tiles_pieces", "tile_counts"),
                   "sort": ("sort_count", "sort_scatter"),
-                  "fine": ("fill_u32", "bin_scan_local", "bin_scan_add", "bin_scatter", "bin_sort_small",
-                           "bin_sort_large", "fine_raster"),
-                  "post": ("copy_u32", "hair_combine", "present"),
+                  "fine": ("fill_u32", "bin_scan_local", "bin_scan_add", "bin_scatter", "bin_sort", "fine_raster"),
+                  "post": ("copy_u32", "hair_combine", "present", "combine_present"),
                   "shading": ("shading",)}
         stage_ms = {g: sum(kernels[k][1] for k in ks if k in kernels) / steps for g, ks in groups.items()}
         if not kernels:
@@ -692,8 +691,8 @@ def plain_run(kind: str, steps: int, warmup: int, args, dev) -> tuple:
                    "global_batch": 1, "parallelism": "none (1 GPU)",
                    "overlap": "none (one stream)" if args.no_branches else
                               ("pipelined: simulation step of frame f beside the render passes of frame f-1; "
-                               "shading beside combine/present" if pipelined else
-                               "grid accumulation beside tiles/sort/fine, shading beside combine/present")},
+                               "shading beside the combine/present pass" if pipelined else
+                               "grid accumulation beside tiles/sort/fine, shading beside the combine/present pass")},
         "latency_ms": {"p50": round(float(np.percentile(per_frame, 50)), 4),
                        "p99": round(float(np.percentile(per_frame, 99)), 4)},
         "pipelined": pipelined,

@@ -49,6 +49,7 @@ struct CombineArgs {
   const fh_list_element* segs;  // element array (byte 4 of DrawnHairSegments)
   const float4* pix;            // pixel array (byte 16 of the rasterizer result)
   const uint32_t* countPerTile;
+  const uint2* scene;           // the texels blended onto (hdr itself, or a scene texture of its layout)
   uint2* hdr;                   // rgba16float texels, texture memory order
   uint32_t W, H, tile, bins, tilesX;
 };
@@ -56,12 +57,15 @@ struct CombineArgs {
 // One HDR texel (x, ty), ty counted from the top row as the texture is
 // stored. The reference's fragment at pixel centre (x + .5, ty + .5) reads
 // rasterizer row u32(H - (ty + .5)) = H - 1 - ty (hairCombinePass.wgsl.ts:60-63).
-__device__ __forceinline__ void combine_texel(const CombineArgs& a, uint32_t x, uint32_t ty) {
+// Returns the texel as written to hdr.
+__device__ __forceinline__ uint2 combine_texel(const CombineArgs& a, uint32_t x, uint32_t ty) {
   const fh_render_uniforms* __restrict__ u = a.u;
   const uint32_t py = a.H - 1u - ty;
   const uint32_t tx = x / a.tile, tyl = py / a.tile;
   const uint32_t dm = u->displayMode;
   const uint32_t mode = dm & 0xffu, extra = (dm >> 8) & 0xffffu;
+  const size_t at = (size_t)ty * a.W + x;
+  const uint2 d = a.scene[at];
   f4 c;
   if (mode == kModeTiles || mode == kModeTilesPpll) {
     // renderTileSegmentCount :100-128
@@ -83,7 +87,10 @@ __device__ __forceinline__ void combine_texel(const CombineArgs& a, uint32_t x, 
         }
       }
     }
-    if (segments == 0u) return;  // discard
+    if (segments == 0u) {  // discard: the texel keeps the scene's value
+      if (a.scene != a.hdr) a.hdr[at] = d;
+      return d;
+    }
     const float r = (float)segments / (float)maxSeg;
     c = mk4(r, 1.0f - r, 0.0f, 1.0f);
   } else {
@@ -101,12 +108,12 @@ __device__ __forceinline__ void combine_texel(const CombineArgs& a, uint32_t x, 
     }
   }
   // blend state: colour src * srcAlpha + dst * (1 - srcAlpha), alpha src + dst
-  uint2* t = a.hdr + (size_t)ty * a.W + x;
-  const uint2 d = *t;
   const float dr = h2f(d.x & 0xffffu), dg = h2f(d.x >> 16), db = h2f(d.y & 0xffffu), da = h2f(d.y >> 16);
   const float ia = 1.0f - c.w;
   const float r = c.x * c.w + dr * ia, g = c.y * c.w + dg * ia, b = c.z * c.w + db * ia, al = c.w + da;
-  *t = make_uint2(f2h(r) | (f2h(g) << 16), f2h(b) | (f2h(al) << 16));
+  const uint2 o = make_uint2(f2h(r) | (f2h(g) << 16), f2h(b) | (f2h(al) << 16));
+  a.hdr[at] = o;
+  return o;
 }
 
 struct Combine {
@@ -118,7 +125,7 @@ struct Combine {
     const uint32_t i = block * kThreads + threadIdx.x;
     if (i >= a.W * a.H) return;
     const uint32_t ty = i / a.W;
-    combine_texel(a, i - ty * a.W, ty);
+    (void)combine_texel(a, i - ty * a.W, ty);
   }
 };
 
@@ -163,10 +170,10 @@ __device__ __forceinline__ uint32_t srgb8(float c, const float* __restrict__ T) 
   return (uint32_t)k;
 }
 
-__device__ __forceinline__ void present_texel(const PresentArgs& a, uint32_t x, uint32_t y) {
+// One screen texel from its HDR texel h (texel (x, y) of hdr).
+__device__ __forceinline__ void present_texel(const PresentArgs& a, uint32_t x, uint32_t y, uint2 h) {
   const fh_render_uniforms* __restrict__ u = a.u;
   const float gamma = u->colorMgmt[0], exposure = u->colorMgmt[1], ditherStr = u->colorMgmt[2];
-  const uint2 h = a.hdr[(size_t)y * a.W + x];
   f3 c = mk3(h2f(h.x & 0xffffu), h2f(h.x >> 16), h2f(h.y & 0xffffu));
   const uint32_t mode = u->displayMode & 0xffu;
   if (mode == kModeFinal || mode == kModeHwRender) {
@@ -218,7 +225,25 @@ struct Present {
     const uint32_t i = block * kThreads + threadIdx.x;
     if (i >= a.W * a.H) return;
     const uint32_t y = i / a.W;
-    present_texel(a, i - y * a.W, y);
+    present_texel(a, i - y * a.W, y, a.hdr[i]);
+  }
+};
+
+// The combine and present passes in one pass over the texels: each thread
+// blends its texel, writes it to hdr and presents the value it wrote.
+struct CombinePresent {
+  struct Args {
+    CombineArgs c;
+    PresentArgs p;
+  };
+  static constexpr uint32_t kThreads = 256;
+  static constexpr const char* kName = "combine_present";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) {
+    (void)blocks;
+    const uint32_t i = block * kThreads + threadIdx.x;
+    if (i >= a.c.W * a.c.H) return;
+    const uint32_t y = i / a.c.W, x = i - y * a.c.W;
+    present_texel(a.p, x, y, combine_texel(a.c, x, y));
   }
 };
 
@@ -322,39 +347,43 @@ int fh_copy(fh_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
   return fh::launch<K>(ctx, blocks, 0, K::Args{(uint32_t*)dst, (const uint32_t*)src, n});
 }
 
-int fh_hair_combine(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t vp_h,
-                    const fh_render_uniforms* uniforms, const fh_tile_result* tiles, const void* tile_segments,
-                    const void* rasterizer_result, const uint32_t* segment_count_per_tile, void* hdr_texture) {
-  FH_CHECK_CTX(ctx);
-  FH_REQUIRE(cfg && uniforms && tiles && tile_segments && rasterizer_result && segment_count_per_tile && hdr_texture,
-             "fh_hair_combine: null buffer");
-  FH_REQUIRE(cfg->tileSize > 0 && cfg->tileDepthBins > 0, "fh_hair_combine: tileSize and tileDepthBins > 0");
-  FH_REQUIRE((uint64_t)vp_w * vp_h < (1ull << 31), "fh_hair_combine: viewport too large");
-  if (vp_w == 0 || vp_h == 0) return FH_OK;
-  CombineArgs a;
+}  // extern "C"
+
+namespace {
+
+int combine_args(const fh_render_cfg* cfg, uint32_t vp_w, uint32_t vp_h, const fh_render_uniforms* uniforms,
+                 const fh_tile_result* tiles, const void* tile_segments, const void* rasterizer_result,
+                 const uint32_t* segment_count_per_tile, const void* scene, void* hdr_texture, const char* what,
+                 CombineArgs& a) {
+  FH_REQUIRE(cfg && uniforms && tiles && tile_segments && rasterizer_result && segment_count_per_tile && scene &&
+                 hdr_texture,
+             std::string(what) + ": null buffer");
+  FH_REQUIRE(cfg->tileSize > 0 && cfg->tileDepthBins > 0, std::string(what) + ": tileSize and tileDepthBins > 0");
+  FH_REQUIRE((uint64_t)vp_w * vp_h < (1ull << 31), std::string(what) + ": viewport too large");
   a.u = uniforms;
   a.tiles = tiles;
   a.segs = reinterpret_cast<const fh_list_element*>(static_cast<const char*>(tile_segments) + 4);
   a.pix = reinterpret_cast<const float4*>(static_cast<const char*>(rasterizer_result) + 16);
   a.countPerTile = segment_count_per_tile;
+  a.scene = static_cast<const uint2*>(scene);
   a.hdr = static_cast<uint2*>(hdr_texture);
   a.W = vp_w; a.H = vp_h; a.tile = cfg->tileSize; a.bins = cfg->tileDepthBins;
   a.tilesX = fh::div_ceil(vp_w, cfg->tileSize);
-  return fh::launch<Combine>(ctx, fh::div_ceil(vp_w * vp_h, Combine::kThreads), 0, a);
+  return FH_OK;
 }
 
-int fh_present(fh_ctx* ctx, uint32_t vp_w, uint32_t vp_h, const fh_render_uniforms* uniforms,
-               const void* hdr_texture, fh_tex2d depth, fh_tex2d ao, const void* normals_texture,
-               uint32_t dither_per_pixel, void* screen_texture, uint32_t screen_row_bytes) {
-  FH_CHECK_CTX(ctx);
-  FH_REQUIRE(uniforms && hdr_texture && screen_texture && depth.data && ao.data, "fh_present: null buffer");
-  FH_REQUIRE((uint64_t)vp_w * vp_h < (1ull << 31), "fh_present: viewport too large");
-  if (vp_w == 0 || vp_h == 0) return FH_OK;
+int present_args(fh_ctx* ctx, uint32_t vp_w, uint32_t vp_h, const fh_render_uniforms* uniforms,
+                 const void* hdr_texture, fh_tex2d depth, fh_tex2d ao, const void* normals_texture,
+                 uint32_t dither_per_pixel, void* screen_texture, uint32_t screen_row_bytes, const char* what,
+                 PresentArgs& a) {
+  const std::string w(what);
+  FH_REQUIRE(uniforms && hdr_texture && screen_texture && depth.data && ao.data, w + ": null buffer");
+  FH_REQUIRE((uint64_t)vp_w * vp_h < (1ull << 31), w + ": viewport too large");
+  if (vp_w == 0 || vp_h == 0) return FH_OK;  // nothing to draw (the caller launches nothing)
   const uint32_t pitch = screen_row_bytes ? screen_row_bytes : 4u * vp_w;
-  FH_REQUIRE(pitch >= 4u * vp_w && pitch % 4u == 0, "fh_present: screen_row_bytes must be >= 4 * width, a multiple of 4");
-  FH_REQUIRE(depth.width == vp_w && depth.height == vp_h, "fh_present: depth texture must match the viewport");
-  FH_REQUIRE(ao.width > 0 && ao.height > 0, "fh_present: empty AO texture");
-  PresentArgs a;
+  FH_REQUIRE(pitch >= 4u * vp_w && pitch % 4u == 0, w + ": screen_row_bytes must be >= 4 * width, a multiple of 4");
+  FH_REQUIRE(depth.width == vp_w && depth.height == vp_h, w + ": depth texture must match the viewport");
+  FH_REQUIRE(ao.width > 0 && ao.height > 0, w + ": empty AO texture");
   a.u = uniforms;
   a.hdr = static_cast<const uint2*>(hdr_texture);
   a.depth = depth.data;
@@ -363,7 +392,58 @@ int fh_present(fh_ctx* ctx, uint32_t vp_w, uint32_t vp_h, const fh_render_unifor
   a.out = static_cast<uint8_t*>(screen_texture);
   a.W = vp_w; a.H = vp_h; a.outPitch = pitch; a.ditherPerPixel = dither_per_pixel ? 1u : 0u;
   a.srgbT = ctx->srgb_thresholds;
+  return FH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fh_hair_combine(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t vp_h,
+                    const fh_render_uniforms* uniforms, const fh_tile_result* tiles, const void* tile_segments,
+                    const void* rasterizer_result, const uint32_t* segment_count_per_tile, void* hdr_texture) {
+  FH_CHECK_CTX(ctx);
+  CombineArgs a;
+  int rc = combine_args(cfg, vp_w, vp_h, uniforms, tiles, tile_segments, rasterizer_result, segment_count_per_tile,
+                        hdr_texture, hdr_texture, "fh_hair_combine", a);
+  if (rc || vp_w == 0 || vp_h == 0) return rc;
+  return fh::launch<Combine>(ctx, fh::div_ceil(vp_w * vp_h, Combine::kThreads), 0, a);
+}
+
+int fh_present(fh_ctx* ctx, uint32_t vp_w, uint32_t vp_h, const fh_render_uniforms* uniforms,
+               const void* hdr_texture, fh_tex2d depth, fh_tex2d ao, const void* normals_texture,
+               uint32_t dither_per_pixel, void* screen_texture, uint32_t screen_row_bytes) {
+  FH_CHECK_CTX(ctx);
+  PresentArgs a;
+  int rc = present_args(ctx, vp_w, vp_h, uniforms, hdr_texture, depth, ao, normals_texture, dither_per_pixel,
+                        screen_texture, screen_row_bytes, "fh_present", a);
+  if (rc || vp_w == 0 || vp_h == 0) return rc;
   return fh::launch<Present>(ctx, fh::div_ceil(vp_w * vp_h, Present::kThreads), 0, a);
+}
+
+int fh_combine_present(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t vp_h,
+                       const fh_render_uniforms* uniforms, const fh_tile_result* tiles, const void* tile_segments,
+                       const void* rasterizer_result, const uint32_t* segment_count_per_tile,
+                       const void* scene_texture, void* hdr_texture, fh_tex2d depth, fh_tex2d ao,
+                       const void* normals_texture, uint32_t dither_per_pixel, void* screen_texture,
+                       uint32_t screen_row_bytes) {
+  FH_CHECK_CTX(ctx);
+  CombinePresent::Args a;
+  const void* scene = scene_texture ? scene_texture : hdr_texture;
+  int rc = combine_args(cfg, vp_w, vp_h, uniforms, tiles, tile_segments, rasterizer_result, segment_count_per_tile,
+                        scene, hdr_texture, "fh_combine_present", a.c);
+  if (rc) return rc;
+  rc = present_args(ctx, vp_w, vp_h, uniforms, hdr_texture, depth, ao, normals_texture, dither_per_pixel,
+                    screen_texture, screen_row_bytes, "fh_combine_present", a.p);
+  if (rc || vp_w == 0 || vp_h == 0) return rc;
+  if (scene != hdr_texture) {
+    // the scene is read where hdr is written: the two must not overlap
+    const char* s0 = static_cast<const char*>(scene);
+    const char* h0 = static_cast<const char*>(hdr_texture);
+    const size_t bytes = (size_t)vp_w * vp_h * 8;
+    FH_REQUIRE(s0 + bytes <= h0 || h0 + bytes <= s0, "fh_combine_present: scene_texture overlaps hdr_texture");
+  }
+  return fh::launch<CombinePresent>(ctx, fh::div_ceil(vp_w * vp_h, CombinePresent::kThreads), 0, a);
 }
 
 }  // extern "C"

@@ -89,19 +89,21 @@ struct TileDepthBounds {
     float2* bounds;
     uint32_t* zero;  // the tile pass's piece counter (set to 0 here)
   };
+  // one wave per tile: every texel of the tile is loaded at once (T * T / 64
+  // loads per lane, none waiting on another) and reduced across the wave
   static constexpr uint32_t kThreads = 256;
+  static constexpr uint32_t kWaves = kThreads / 64;
   static constexpr const char* kName = "tile_depth_bounds";
   static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) {
     if (block == 0 && threadIdx.x == 0 && a.zero) *a.zero = 0u;
-    __shared__ float smin[kThreads / 64], smax[kThreads / 64];
-    __shared__ uint32_t snan[kThreads / 64];
     const TileGeom tg = tile_geom(mk2(a.u->viewport[0], a.u->viewport[1]), a.T);
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-    for (uint32_t t = block; t < a.nTiles; t += blocks) {
-      const uint32_t ty = t / max(tg.tilesX, 1u), tx = t - ty * max(tg.tilesX, 1u);
+    const uint32_t tilesX = max(tg.tilesX, 1u), nt = a.T * a.T;
+    for (uint32_t t = block * kWaves + w; t < a.nTiles; t += blocks * kWaves) {
+      const uint32_t ty = t / tilesX, tx = t - ty * tilesX;
       float lo = INFINITY, hi = -INFINITY;
       bool nan = false;
-      for (uint32_t i = threadIdx.x; i < a.T * a.T; i += kThreads) {
+      for (uint32_t i = lane; i < nt; i += 64u) {
         const uint32_t x = tx * a.T + i % a.T, y = ty * a.T + i / a.T;
         const float d = tex_load(a.depth, a.depthW, a.depthH, (int32_t)x, (int32_t)(tg.vpH - y));
         nan |= d != d;
@@ -113,15 +115,8 @@ struct TileDepthBounds {
         lo = fminf(lo, __shfl_xor(lo, o));
         hi = fmaxf(hi, __shfl_xor(hi, o));
       }
-      const uint64_t anyNan = __ballot(nan);
-      if (lane == 0) { smin[w] = lo; smax[w] = hi; snan[w] = anyNan != 0; }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        bool n = false;
-        for (uint32_t k = 0; k < kThreads / 64; k++) { lo = fminf(lo, smin[k]); hi = fmaxf(hi, smax[k]); n |= snan[k] != 0; }
-        a.bounds[t] = n ? make_float2(-INFINITY, INFINITY) : make_float2(lo, hi);
-      }
-      __syncthreads();
+      const bool anyNan = __ballot(nan) != 0ull;
+      if (lane == 0) a.bounds[t] = anyNan ? make_float2(-INFINITY, INFINITY) : make_float2(lo, hi);
     }
   }
 };
@@ -802,24 +797,18 @@ struct BinScanAdd {
 };
 
 
-__device__ __forceinline__ void bin_scatter_body(const fh_render_uniforms* __restrict__ u,
-                                                          const uint32_t* __restrict__ segCounter,
-                                                          const fh_list_element* __restrict__ data,
-                                                          uint32_t capacity, const uint2* __restrict__ slotTag,
-                                                          const uint32_t* __restrict__ starts, uint32_t nTB,
-                                                          uint32_t* __restrict__ binSeg, uint32_t block, uint32_t blocks) {
-  const uint32_t n = min(*segCounter, min(capacity, u->maxDrawnHairSegments));
-  for (uint32_t s = block * blockDim.x + threadIdx.x; s < n; s += blocks * blockDim.x) {
-    const uint2 tag = slotTag[s];  // (tile-bin, rank in its list)
-    if (tag.x >= nTB) continue;    // untagged slot: not linked into any list
-    const uint32_t pos = starts[tag.x] + tag.y;
-    if (pos < capacity) {
-      const uint32_t packed = data[s].strandAndSegmentIdxs;  // segment << 24 | strand
-      binSeg[pos] = ((packed & 0x00ffffffu) << 8) | (packed >> 24);
-    }
-  }
-}
+// Ranges of more than this many keys are sorted by a workgroup each; shorter
+// ones by a wave each.
+constexpr uint32_t kWaveSortMax = 64;
+// Scan blocks whose sums the scatter kernel prefixes itself (in LDS); beyond
+// that a separate kernel adds the prefixes first.
+constexpr uint32_t kMaxScanBlocks = 4096;
 
+// Step 2, with the prefix of the scan blocks' sums taken in each workgroup:
+// every range's global start goes to `starts` (and a range of more than
+// kWaveSortMax keys onto the list `large` for the sort), and every tagged
+// slot to its position. `ends` stays as the scan left it, so no workgroup
+// reads a value another one rewrites.
 struct BinScatter {
   struct Args {
     const fh_render_uniforms* u;
@@ -827,13 +816,68 @@ struct BinScatter {
     const fh_list_element* data;
     uint32_t capacity;
     const uint2* slotTag;
-    const uint32_t* starts;
+    const uint32_t* ends;       // exclusive prefix within each scan block
+    const uint32_t* blockSums;  // per scan block; null when `ends` already hold the global prefix
+    uint32_t nSums;
+    const fh_tile_result* tiles;
+    uint32_t* starts;           // out: first element of each (tile, bin) range
     uint32_t nTB;
     uint32_t* binSeg;
+    uint32_t* large;            // out: count in large[0] (zeroed by the scan), then the listed ranges
   };
   static constexpr uint32_t kThreads = 256;
   static constexpr const char* kName = "bin_scatter";
-  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { bin_scatter_body(a.u, a.segCounter, a.data, a.capacity, a.slotTag, a.starts, a.nTB, a.binSeg, block, blocks); }
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) {
+    __shared__ uint32_t pre[kMaxScanBlocks];
+    __shared__ uint32_t warpSums[16];
+    constexpr uint32_t per = kMaxScanBlocks / kThreads;
+    const uint32_t tid = threadIdx.x;
+    if (a.blockSums) {
+      uint32_t v[per], sum = 0;
+#pragma unroll
+      for (uint32_t k = 0; k < per; k++) {
+        const uint32_t j = tid * per + k;
+        v[k] = j < a.nSums ? a.blockSums[j] : 0u;
+        sum += v[k];
+      }
+      uint32_t run = block_inclusive_scan(sum, warpSums) - sum;
+#pragma unroll
+      for (uint32_t k = 0; k < per; k++) {
+        pre[tid * per + k] = run;
+        run += v[k];
+      }
+      __syncthreads();
+    }
+    const uint32_t lane = tid & 63u;
+    for (uint32_t b0 = block * kThreads; b0 < a.nTB; b0 += blocks * kThreads) {
+      const uint32_t tb = b0 + tid;
+      bool big = false;
+      if (tb < a.nTB) {
+        const uint32_t st = a.ends[tb] + (a.blockSums ? pre[tb / kScanBlock] : 0u);
+        a.starts[tb] = st;
+        const uint32_t s0 = min(st, a.capacity);
+        big = min(a.tiles[tb].padding0, a.capacity - s0) > kWaveSortMax;
+      }
+      const uint64_t m = __ballot(big);
+      if (m) {
+        uint32_t at = 0;
+        const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+        if (lane == leader) at = atomicAdd(&a.large[0], (uint32_t)__popcll(m));
+        at = (uint32_t)__builtin_amdgcn_readlane((int)at, (int)leader);
+        if (big) a.large[1u + at + mask_prefix(m)] = tb;
+      }
+    }
+    const uint32_t n = min(*a.segCounter, min(a.capacity, a.u->maxDrawnHairSegments));
+    for (uint32_t sl = block * kThreads + tid; sl < n; sl += blocks * kThreads) {
+      const uint2 tag = a.slotTag[sl];  // (tile-bin, rank in its list)
+      if (tag.x >= a.nTB) continue;     // untagged slot: not linked into any list
+      const uint32_t pos = a.ends[tag.x] + (a.blockSums ? pre[tag.x / kScanBlock] : 0u) + tag.y;
+      if (pos < a.capacity) {
+        const uint32_t packed = a.data[sl].strandAndSegmentIdxs;  // segment << 24 | strand
+        a.binSeg[pos] = ((packed & 0x00ffffffu) << 8) | (packed >> 24);
+      }
+    }
+  }
 };
 
 
@@ -883,18 +927,18 @@ __device__ __forceinline__ uint32_t wave_sort_desc(uint32_t v) {
   return v;
 }
 
-// One workgroup per kSmallRanges consecutive ranges: ranges of up to 64 keys
-// are sorted one per wave in registers (each of the 4 waves takes every 4th
-// short range, so a workgroup's ranges sort 4 at a time and the grid has
-// enough waves to keep every SIMD busy); larger ones are appended to `large`
-// (count in large[0]) for bin_sort_large_kernel, so that the few heavy
-// ranges, which cluster on screen, spread over many workgroups.
+// One workgroup per kSmallRanges consecutive ranges: ranges of up to
+// kWaveSortMax keys are sorted one per wave in registers (each of the 4 waves
+// takes every 4th short range, so a workgroup's ranges sort 4 at a time and
+// the grid has enough waves to keep every SIMD busy); larger ones are on the
+// list the scatter kernel made, so that the few heavy ranges, which cluster
+// on screen, spread over many workgroups.
 constexpr uint32_t kSmallRanges = 64;
 
 __device__ __forceinline__ void bin_sort_small_body(const fh_tile_result* __restrict__ tiles,
                                                              const uint32_t* __restrict__ starts, uint32_t nTB,
                                                              uint32_t capacity, uint32_t* __restrict__ keys,
-                                                             uint32_t* __restrict__ large, uint32_t block, uint32_t blocks) {
+                                                             uint32_t block) {
   __shared__ uint32_t smallList[kSmallRanges];
   __shared__ uint32_t sStart[kSmallRanges], sCount[kSmallRanges];
   __shared__ uint32_t nSmall;
@@ -911,8 +955,7 @@ __device__ __forceinline__ void bin_sort_small_body(const fh_tile_result* __rest
     sStart[tid] = start;
     sCount[tid] = n;
   }
-  if (n >= 2u && n <= 64u) smallList[atomicAdd(&nSmall, 1u)] = tid;
-  if (n > 64u) large[1u + atomicAdd(&large[0], 1u)] = tb;
+  if (n >= 2u && n <= kWaveSortMax) smallList[atomicAdd(&nSmall, 1u)] = tid;
   __syncthreads();
   const uint32_t lane = __lane_id(), w = tid >> 6;
   for (uint32_t i = w; i < nSmall; i += blockDim.x >> 6) {
@@ -923,20 +966,6 @@ __device__ __forceinline__ void bin_sort_small_body(const fh_tile_result* __rest
     if (lane < cnt) keys[st + lane] = v;
   }
 }
-
-struct BinSortSmall {
-  struct Args {
-    const fh_tile_result* tiles;
-    const uint32_t* starts;
-    uint32_t nTB;
-    uint32_t capacity;
-    uint32_t* keys;
-    uint32_t* large;
-  };
-  static constexpr uint32_t kThreads = 256;
-  static constexpr const char* kName = "bin_sort_small";
-  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { bin_sort_small_body(a.tiles, a.starts, a.nTB, a.capacity, a.keys, a.large, block, blocks); }
-};
 
 
 // The same descending network over n2 = 256 * R keys (n2 <= kSortLds) held
@@ -1046,18 +1075,27 @@ __device__ __forceinline__ void bin_sort_large_body(const fh_tile_result* __rest
   }
 }
 
-struct BinSortLarge {
+// Both sorts in one launch: the first nLargeBlocks workgroups take the listed
+// long ranges (dispatched first, so that they start at once), the rest the
+// short ranges, kSmallRanges (tile, bin) entries each.
+struct BinSort {
   struct Args {
     const fh_tile_result* tiles;
     const uint32_t* starts;
+    uint32_t nTB;
     uint32_t capacity;
     uint32_t* keys;
     const uint32_t* large;
     uint32_t* big;
+    uint32_t nLargeBlocks;
   };
   static constexpr uint32_t kThreads = 256;
-  static constexpr const char* kName = "bin_sort_large";
-  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { bin_sort_large_body(a.tiles, a.starts, a.capacity, a.keys, a.large, a.big, block, blocks); }
+  static constexpr const char* kName = "bin_sort";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) {
+    (void)blocks;
+    if (block < a.nLargeBlocks) bin_sort_large_body(a.tiles, a.starts, a.capacity, a.keys, a.large, a.big, block, a.nLargeBlocks);
+    else bin_sort_small_body(a.tiles, a.starts, a.nTB, a.capacity, a.keys, block - a.nLargeBlocks);
+  }
 };
 
 
@@ -1689,7 +1727,7 @@ int fh_tiles_draw(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t
   a.depthBounds = reinterpret_cast<const float2*>(reinterpret_cast<char*>(ctx->scratch) + boundsOff);
   a.nBounds = nTiles;
   // the depth-bounds pass also zeroes the piece counter (nTiles >= 1: it always runs)
-  if ((rc = launch<TileDepthBounds>(ctx, std::min(nTiles, fh::persistent_blocks(ctx)), 0,
+  if ((rc = launch<TileDepthBounds>(ctx, fh::div_ceil(nTiles, TileDepthBounds::kWaves), 0,
                                     {uniforms, depth.data, depth.width, depth.height, cfg->tileSize, nTiles,
                                      const_cast<float2*>(a.depthBounds), a.pieceCount})))
     return rc;
@@ -1767,9 +1805,10 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
   const uint32_t nTB = fh_tile_count(cfg, vp_w, vp_h) * cfg->tileDepthBins;
   const uint32_t limit = segments_capacity;
   const uint32_t scanBlocks = fh::div_ceil(nTB, kScanBlock);
-  // scratch: per-bin range ends/starts, per-block sums, gathered segment ids
+  // scratch: per-bin range ends and starts, per-block sums, gathered segment ids
   const size_t offEnds = 0;
-  const size_t offSums = fh::align_up(offEnds + (size_t)nTB * 4, 256);
+  const size_t offStarts = fh::align_up(offEnds + (size_t)nTB * 4, 256);
+  const size_t offSums = fh::align_up(offStarts + (size_t)nTB * 4, 256);
   const size_t offSeg = fh::align_up(offSums + (size_t)scanBlocks * 4, 256);
   const size_t offBig = fh::align_up(offSeg + (size_t)limit * 4 + 4, 256);
   const size_t offLarge = fh::align_up(offBig + (size_t)limit * 8 + 8, 256);
@@ -1777,6 +1816,7 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
   if ((rc = ensure_scratch(ctx, total))) return rc;
   char* base = reinterpret_cast<char*>(ctx->scratch);
   uint32_t* ends = reinterpret_cast<uint32_t*>(base + offEnds);
+  uint32_t* starts = reinterpret_cast<uint32_t*>(base + offStarts);
   uint32_t* sums = reinterpret_cast<uint32_t*>(base + offSums);
   uint32_t* binSeg = reinterpret_cast<uint32_t*>(base + offSeg);
   uint32_t* big = reinterpret_cast<uint32_t*>(base + offBig);
@@ -1786,23 +1826,26 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
 
   // the scan also zeroes the long-range counter of the bin sort (large[0])
   if ((rc = launch<BinScanLocal>(ctx, scanBlocks, 0, {tiles, nTB, ends, sums, large, ctx->stats}))) return rc;
-  if (scanBlocks > 1 && (rc = launch<BinScanAdd>(ctx, scanBlocks, 0, {nTB, ends, sums}))) return rc;
-  const uint32_t scatterBlocks = std::max(1u, std::min(fh::div_ceil(limit, 256), fh::persistent_blocks(ctx)));
+  // up to kMaxScanBlocks scan blocks the scatter kernel adds their prefixes itself
+  const bool addFirst = scanBlocks > kMaxScanBlocks;
+  if (addFirst && (rc = launch<BinScanAdd>(ctx, scanBlocks, 0, {nTB, ends, sums}))) return rc;
+  const uint32_t scatterBlocks =
+      std::max(1u, std::min(fh::div_ceil(std::max(limit, nTB), 256), fh::persistent_blocks(ctx)));
   if ((rc = launch<BinScatter>(ctx, scatterBlocks, 0,
                                {uniforms, reinterpret_cast<const uint32_t*>(tile_segments), data, limit,
-                                reinterpret_cast<const uint2*>(slot_tile_bin), ends, nTB, binSeg})))
+                                reinterpret_cast<const uint2*>(slot_tile_bin), ends, addFirst ? nullptr : sums,
+                                scanBlocks, tiles, starts, nTB, binSeg, large})))
     return rc;
-  if ((rc = launch<BinSortSmall>(ctx, fh::div_ceil(nTB, kSmallRanges), 0, {tiles, ends, nTB, limit, binSeg, large})))
-    return rc;
-  if ((rc = launch<BinSortLarge>(ctx, std::min(nTB, fh::persistent_blocks(ctx)), 0,
-                                 {tiles, ends, limit, binSeg, large, big})))
+  const uint32_t largeBlocks = std::min(nTB, fh::persistent_blocks(ctx));
+  if ((rc = launch<BinSort>(ctx, largeBlocks + fh::div_ceil(nTB, kSmallRanges), 0,
+                            {tiles, starts, nTB, limit, binSeg, large, big, largeBlocks})))
     return rc;
 
   FineArgs a;
   a.u = uniforms;
   a.pos = reinterpret_cast<const float4*>(positions);
   a.tan = reinterpret_cast<const float4*>(tangents);
-  a.tiles = tiles; a.binStart = ends; a.binKey = binSeg; a.capacity = limit;
+  a.tiles = tiles; a.binStart = starts; a.binKey = binSeg; a.capacity = limit;
   a.slicePool = reinterpret_cast<uint4*>(slice_data);
   a.queue = reinterpret_cast<uint32_t*>(rasterizer_result);
   a.fb = reinterpret_cast<float4*>(reinterpret_cast<char*>(rasterizer_result) + 16);

@@ -125,6 +125,8 @@ SIGNATURES = {
     "fh_debug_raster_barycentric": (i32, [vp, vp, vp, vp, u32, u32, vp]),
     "fh_hair_combine": (i32, [vp, C.POINTER(RenderCfg), u32, u32, vp, vp, vp, vp, vp, vp]),
     "fh_present": (i32, [vp, u32, u32, vp, vp, Tex2D, Tex2D, vp, u32, vp, u32]),
+    "fh_combine_present": (i32, [vp, C.POINTER(RenderCfg), u32, u32, vp, vp, vp, vp, vp, vp, vp, Tex2D, Tex2D, vp,
+                                 u32, vp, u32]),
     "fh_copy": (i32, [vp, vp, vp, C.c_uint64]),
     "fh_copy_indexed": (i32, [vp, vp, vp, C.c_uint64, u32, vp]),
     "fh_hw_hair": (i32, [vp, u32, u32, vp, vp, vp, vp, vp, u32, u32]),

@@ -266,6 +266,19 @@ class HairCombinePass:
                         N.ptr(ctx.hairTileSegmentsBuffer), N.ptr(ctx.hairRasterizerResultsBuffer),
                         N.ptr(ctx.hairSegmentCountPerTileBuffer), N.ptr(ctx.hdrRenderTexture))
 
+    def cmdCombineAndPresent(self, ctx: PassCtx, present: "PresentPass", screenTexture: torch.Tensor,
+                             scene: torch.Tensor | None = None, rowBytes: int = 0) -> None:
+        """cmdCombineRasterResults, then present.cmdDraw(ctx, screenTexture),
+        as one pass over the texels (fh_combine_present), with the same
+        results. `scene` (None: the HDR texture itself) is the HDR scene
+        blended onto, read in place of a copy into the HDR texture."""
+        w, h = ctx.viewport
+        ctx.native.call("fh_combine_present", ctx.cfg, w, h, N.ptr(ctx.globalUniforms), N.ptr(ctx.hairTilesBuffer),
+                        N.ptr(ctx.hairTileSegmentsBuffer), N.ptr(ctx.hairRasterizerResultsBuffer),
+                        N.ptr(ctx.hairSegmentCountPerTileBuffer), N.ptr(scene) if scene is not None else None,
+                        N.ptr(ctx.hdrRenderTexture), ctx.shadingDepthTexture, ctx.aoTexture,
+                        N.ptr(ctx.normalsTexture), 1 if present.ditherPerPixel else 0, N.ptr(screenTexture), rowBytes)
+
 
 class PresentPass:
     """src/passes/presentPass/presentPass.ts:125-210. `ditherPerPixel` False
@@ -525,8 +538,10 @@ class HairRenderer:
 
     def drawScene(self, ctx: PassCtx) -> None:
         """The HDR scene of the background and mesh passes (renderer.ts:187-188,
-        :244-246). Without mesh_passes their output is an input, restored
-        here; with them, drawMeshes() draws it after the clears."""
+        :244-246). Without mesh_passes their output is an input, copied into
+        the HDR texture here for the combine pass alone (drawHair's fused
+        combine and present pass reads the input in place); with them,
+        drawMeshes() draws it after the clears."""
         if self.mesh_passes:
             return
         src = self.inputs["hdrScene"]
@@ -554,11 +569,10 @@ class HairRenderer:
         pass, beside the combine and present passes; a shading pass still
         running when the fine pass starts would keep the fine pass's
         persistent workgroups from the CUs."""
-        # the scene copy and the three passes' clears touch disjoint buffers:
-        # one launch for all of them
+        # the three passes' clears touch disjoint buffers: one launch for all
+        # of them
         self.native.clear_batch_begin()
         try:
-            self.drawScene(ctx)
             self.tilesPass.cmdClearBeforeRender(ctx)
             self.finePass.cmdClearBeforeRender(ctx)
             self.sortPass.cmdClearBeforeRender(ctx)
@@ -583,13 +597,15 @@ class HairRenderer:
         self.tilesPass.cmdDrawHairToTiles(ctx, self.hair)
         self.sortPass.cmdSortHairTiles(ctx)
         self.finePass.cmdRasterizeSlicesHair(ctx, self.hair, self.tilesPass)
+        if self.hw_passes:  # the present pass's debug views read the hair depth, normals and AO
+            self.join_branch()
         self._branch_begin()
         self.shadingPass.cmdComputeShadingPoints(ctx, self.hair)
         self._branch_end()
-        self.hairCombinePass.cmdCombineRasterResults(ctx)
-        if self.hw_passes:  # the present pass's debug views read the hair depth, normals and AO
-            self.join_branch()
-        self.presentPass.cmdDraw(ctx, self.screenTexture)
+        # the combine and present passes in one (the scene blended onto read
+        # where it lies: the input, or the meshes' colour in the HDR texture)
+        self.hairCombinePass.cmdCombineAndPresent(ctx, self.presentPass, self.screenTexture,
+                                                  None if self.mesh_passes else self.inputs["hdrScene"])
         self.join_branch()
 
     def frame(self, upload_uniforms: bool = True) -> None:

@@ -442,6 +442,22 @@ int fh_present(fh_ctx* ctx, uint32_t vp_w, uint32_t vp_h, const fh_render_unifor
                const void* hdr_texture, fh_tex2d depth, fh_tex2d ao, const void* normals_texture,
                uint32_t dither_per_pixel, void* screen_texture, uint32_t screen_row_bytes);
 
+/* fh_hair_combine and then fh_present of the texture it wrote, in one pass
+ * over the texels (each thread presents the texel it blended): leaves
+ * hdr_texture and the screen texture exactly as those two calls do. The
+ * reference records the two passes back to back in one command encoder
+ * (hairCombinePass.ts:228-249, presentPass.ts:182-210); this entry point has
+ * no single counterpart there. scene_texture (NULL: hdr_texture itself) holds
+ * the HDR scene to blend onto, so that a caller whose background and meshes
+ * are an input passes them here instead of copying them into hdr_texture
+ * first; it must not overlap hdr_texture (FH_ERR_INVALID_ARG). */
+int fh_combine_present(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t vp_h,
+                       const fh_render_uniforms* uniforms, const fh_tile_result* tiles, const void* tile_segments,
+                       const void* rasterizer_result, const uint32_t* segment_count_per_tile,
+                       const void* scene_texture, void* hdr_texture, fh_tex2d depth, fh_tex2d ao,
+                       const void* normals_texture, uint32_t dither_per_pixel, void* screen_texture,
+                       uint32_t screen_row_bytes);
+
 /* ------------------------------------------------------------------------ */
 /* Hair depth and normals, shadow map, AO (the reference's hardware-raster   */
 /* passes that feed the shading pass)                                        */
