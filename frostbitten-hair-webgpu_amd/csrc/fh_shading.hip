@@ -18,16 +18,28 @@ constexpr float kHairF0 = (float)((1.0 - 1.55) * (1.0 - 1.55) / ((1.0 + 1.55) * 
 // linearizeDepth constants: CONFIG.camera.projection near / far (constants.ts:131-135)
 constexpr float kNear = 0.01f, kFar = 100.0f;
 
+// The lobes' powers and exponentials as WGSL defines pow (exp2(y * log2(x)))
+// on the hardware's base-2 logarithm and exponential (about 1 ulp each; the
+// relative error of the result grows with |y| by about |y| * 2^-24). The
+// shading points are checked against the oracle's libm powf / expf within a
+// relative 2e-4.
+__device__ __forceinline__ float exp_hw(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+// x >= 0 (NaN passes through); x^0 = 1^e = 1 as powf has them
+__device__ __forceinline__ float pow_hw(float x, float e) {
+  if (e == 0.0f || x == 1.0f) return 1.0f;
+  return __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(x));
+}
 __device__ __forceinline__ float fresnel1(float cosT, float f0) {
-  return f0 + (1.0f - f0) * powf(1.0f - cosT, 5.0f);
+  const float x = 1.0f - cosT, x2 = x * x;  // pow(x, 5)
+  return f0 + (1.0f - f0) * (x2 * x2 * x);
 }
 __device__ __forceinline__ float cos_half(float c) { return sqrtf(satf(0.5f + 0.5f * c)); }
 __device__ __forceinline__ float long_gauss(float si, float sr, float alpha, float beta) {
   const float a = si + sr - alpha;
   const float b = -(a * a) / (2.0f * beta * beta);
-  return expf(b) / (beta * kSqrtTwoPi);
+  return exp_hw(b) / (beta * kSqrtTwoPi);
 }
-__device__ __forceinline__ f3 powv(f3 b, float e) { return mk3(powf(b.x, e), powf(b.y, e), powf(b.z, e)); }
+__device__ __forceinline__ f3 powv(f3 b, float e) { return mk3(pow_hw(b.x, e), pow_hw(b.y, e), pow_hw(b.z, e)); }
 
 struct Marschner {
   f3 base;
@@ -53,13 +65,13 @@ __device__ f3 marschner(const Marschner& p, f3 L, f3 V, f3 T) {
   const float a = 1.0f / eta;
   const float hTT = (1.0f + a * (0.6f - 0.8f * cphi)) * chp;
   const f3 Ttt = powv(p.base, sqrtf(1.0f - hTT * hTT * a * a) / (2.0f * cd));
-  const float Dtt = expf(-3.65f * cphi - 3.98f);
+  const float Dtt = exp_hw(-3.65f * cphi - 3.98f);
   const float f = fresnel1(cd * sqrtf(satf(1.0f - hTT * hTT)), kHairF0);
   const f3 Ntt = (Ttt * ((1.0f - f) * (1.0f - f))) * Dtt;
   // TRT — azimuthalScattering_TRT :437-454
   const float Mtrt = long_gauss(si, sr, 4.0f * p.shift, 2.0f * beta2);
   const f3 Ttrt = powv(p.base, 0.8f / cd);
-  const float Dtrt = expf(17.0f * cphi - 16.78f);
+  const float Dtrt = exp_hw(17.0f * cphi - 16.78f);
   const float f2v = fresnel1(cd * 0.5f, kHairF0);
   const f3 Ntrt = (Ttrt * ((1.0f - f2v) * (1.0f - f2v) * f2v)) * Dtrt;
   const float r = p.specular * Mr * Nr;
