@@ -631,7 +631,7 @@ def plain_run(kind: str, steps: int, warmup: int, args, dev) -> tuple:
         groups = {"sim": ("grid_pre_sim", "sim_integrate", "p2g_clear", "p2g_count", "p2g_scan", "p2g_scatter",
                          "grid_post_sim"),
                   "tiles": ("fill_many", "tile_depth_bounds", "tiles_draw", "tiles_pieces", "tile_counts"),
-                  "sort": ("sort_count", "sort_scatter"),
+                  "sort": ("sort_tiles",),
                   "fine": ("fill_u32", "bin_scan_local", "bin_scan_add", "bin_scatter", "bin_sort", "fine_raster"),
                   "post": ("copy_u32", "hair_combine", "present", "combine_present"),
                   "shading": ("shading",)}

@@ -615,83 +615,20 @@ struct RebuildKeys {
 
 
 // ---------------------------------------------------------------------------
-// K5 — hairTileSortPass.countTiles.wgsl.ts:63-97, with the bucket histogram
-// aggregated in LDS first (one global atomic per bucket per workgroup).
+// K5 + K6 — hairTileSortPass.countTiles.wgsl.ts:63-97 and
+// hairTileSortPass.sort.wgsl.ts:37-75 in one workgroup of 1,024 threads: the
+// bucket histogram in LDS, then the buckets' offsets (heavier buckets first,
+// calculateOffsetsForEachBucket :68-75), then every drawn tile at its
+// bucket's offset plus its rank from an LDS counter. Each thread loads
+// kSortPer tiles' counts at once and issues their LDS atomics back to back.
+// The buckets' tileCount / writeOffset and tileList[0] are added to as the
+// reference's atomics add to them (a cleared buffer ends with each bucket's
+// count in both), and no global atomic is left to wait behind other
+// kernels' atomics in L2.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void sort_count_body(const uint32_t* __restrict__ counts,
-                                                         uint32_t nTiles, uint32_t nBuckets,
-                                                         uint32_t bucketSize,
-                                                         fh_sort_bucket* __restrict__ buckets, uint32_t block, uint32_t blocks) {
-  __shared__ uint32_t hist[64];
-  if (threadIdx.x < 64) hist[threadIdx.x] = 0;
-  __syncthreads();
-  const uint32_t t = block * blockDim.x + threadIdx.x;
-  if (t < nTiles) {
-    const uint32_t c = counts[t];
-    if (c > 0u) atomicAdd(&hist[min(c / bucketSize, nBuckets - 1u)], 1u);
-  }
-  __syncthreads();
-  if (threadIdx.x < nBuckets && hist[threadIdx.x]) atomicAdd(&buckets[threadIdx.x].tileCount, hist[threadIdx.x]);
-}
+constexpr uint32_t kSortPer = 8;
 
-struct SortCount {
-  struct Args {
-    const uint32_t* counts;
-    uint32_t nTiles;
-    uint32_t nBuckets;
-    uint32_t bucketSize;
-    fh_sort_bucket* buckets;
-  };
-  static constexpr uint32_t kThreads = 256;
-  static constexpr const char* kName = "sort_count";
-  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { sort_count_body(a.counts, a.nTiles, a.nBuckets, a.bucketSize, a.buckets, block, blocks); }
-};
-
-
-// K6 — hairTileSortPass.sort.wgsl.ts:37-75: heavier buckets first; each
-// workgroup reserves its in-bucket range with one atomic per bucket.
-__device__ __forceinline__ void sort_scatter_body(const uint32_t* __restrict__ counts,
-                                                           uint32_t nTiles, uint32_t nBuckets,
-                                                           uint32_t bucketSize,
-                                                           fh_sort_bucket* __restrict__ buckets,
-                                                           uint32_t* __restrict__ tileList, uint32_t block, uint32_t blocks) {
-  __shared__ uint32_t bucketOffset[64];
-  __shared__ uint32_t localCount[64];
-  __shared__ uint32_t localBase[64];
-  __shared__ uint32_t total;
-  if (threadIdx.x == 0) {
-    // calculateOffsetsForEachBucket :68-75
-    uint32_t off = 0;
-    for (uint32_t i = 0; i < nBuckets; i++) {
-      const uint32_t idx = nBuckets - 1u - i;
-      bucketOffset[idx] = off;
-      off += buckets[idx].tileCount;
-    }
-    total = 0;
-  }
-  if (threadIdx.x < 64) localCount[threadIdx.x] = 0;
-  __syncthreads();
-  const uint32_t t = block * blockDim.x + threadIdx.x;
-  uint32_t b = 0, rank = 0;
-  const bool has = t < nTiles && counts[t] > 0u;
-  if (has) {
-    b = min(counts[t] / bucketSize, nBuckets - 1u);
-    rank = atomicAdd(&localCount[b], 1u);
-    atomicAdd(&total, 1u);
-  }
-  __syncthreads();
-  if (threadIdx.x < nBuckets && localCount[threadIdx.x])
-    localBase[threadIdx.x] = atomicAdd(&buckets[threadIdx.x].writeOffset, localCount[threadIdx.x]);
-  if (threadIdx.x == 0 && total) atomicAdd(&tileList[0], total);
-  __syncthreads();
-  if (has) {
-    // a caller that skipped the per-frame clear would push offsets past the list
-    const uint32_t o = bucketOffset[b] + localBase[b] + rank;
-    if (o < nTiles) tileList[1 + o] = t;
-  }
-}
-
-struct SortScatter {
+struct SortTiles {
   struct Args {
     const uint32_t* counts;
     uint32_t nTiles;
@@ -700,9 +637,59 @@ struct SortScatter {
     fh_sort_bucket* buckets;
     uint32_t* tileList;
   };
-  static constexpr uint32_t kThreads = 256;
-  static constexpr const char* kName = "sort_scatter";
-  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { sort_scatter_body(a.counts, a.nTiles, a.nBuckets, a.bucketSize, a.buckets, a.tileList, block, blocks); }
+  static constexpr uint32_t kThreads = 1024;
+  static constexpr const char* kName = "sort_tiles";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) {
+    (void)block;
+    (void)blocks;
+    __shared__ uint32_t hist[64], cursor[64], offset[64];
+    const uint32_t tid = threadIdx.x;
+    if (tid < 64u) hist[tid] = 0u;
+    __syncthreads();
+    for (uint32_t t0 = tid; t0 < a.nTiles; t0 += kThreads * kSortPer) {
+      uint32_t c[kSortPer];
+#pragma unroll
+      for (uint32_t k = 0; k < kSortPer; k++) {
+        const uint32_t t = t0 + k * kThreads;
+        c[k] = t < a.nTiles ? a.counts[t] : 0u;
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < kSortPer; k++)
+        if (c[k] > 0u) atomicAdd(&hist[min(c[k] / a.bucketSize, a.nBuckets - 1u)], 1u);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t off = 0, total = 0;
+      for (uint32_t i = 0; i < a.nBuckets; i++) {
+        const uint32_t b = a.nBuckets - 1u - i;
+        const uint32_t count = a.buckets[b].tileCount + hist[b];
+        a.buckets[b].tileCount = count;
+        offset[b] = off;
+        off += count;
+        cursor[b] = a.buckets[b].writeOffset;
+        a.buckets[b].writeOffset = cursor[b] + hist[b];
+        total += hist[b];
+      }
+      a.tileList[0] += total;
+    }
+    __syncthreads();
+    for (uint32_t t0 = tid; t0 < a.nTiles; t0 += kThreads * kSortPer) {
+      uint32_t c[kSortPer];
+#pragma unroll
+      for (uint32_t k = 0; k < kSortPer; k++) {
+        const uint32_t t = t0 + k * kThreads;
+        c[k] = t < a.nTiles ? a.counts[t] : 0u;
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < kSortPer; k++) {
+        if (c[k] == 0u) continue;
+        const uint32_t b = min(c[k] / a.bucketSize, a.nBuckets - 1u);
+        const uint32_t o = offset[b] + atomicAdd(&cursor[b], 1u);
+        // a caller that skipped the per-frame clear would push offsets past the list
+        if (o < a.nTiles) a.tileList[1u + o] = t0 + k * kThreads;
+      }
+    }
+  }
 };
 
 
@@ -713,7 +700,7 @@ struct SortScatter {
 // lengths (HairTileResult.padding0); step 2 writes every slot to the start
 // of its range plus its rank in the list, both tagged by the tile pass.
 // ---------------------------------------------------------------------------
-constexpr int kScanBlock = 1024;
+constexpr int kScanBlock = 256;
 
 __device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t v, uint32_t* warpSums) {
   const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
@@ -1769,13 +1756,9 @@ int fh_tiles_sort(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_t
   FH_REQUIRE(segment_count_per_tile && tile_list && buckets, "fh_tiles_sort: null buffer");
   const uint32_t nt = fh_tile_count(cfg, vp_w, vp_h);
   if (nt == 0) return FH_OK;
-  const uint32_t blocks = fh::div_ceil(nt, 256);
-  if ((rc = launch<SortCount>(ctx, blocks, 0,
-                              {segment_count_per_tile, nt, cfg->sortBuckets, cfg->sortBucketSize, buckets})))
-    return rc;
-  return launch<SortScatter>(ctx, blocks, 0,
-                             {segment_count_per_tile, nt, cfg->sortBuckets, cfg->sortBucketSize, buckets,
-                              reinterpret_cast<uint32_t*>(tile_list)});
+  return launch<SortTiles>(ctx, 1, 0,
+                           {segment_count_per_tile, nt, cfg->sortBuckets, cfg->sortBucketSize, buckets,
+                            reinterpret_cast<uint32_t*>(tile_list)});
 }
 
 int fh_fine_clear(fh_ctx* ctx, uint32_t vp_w, uint32_t vp_h, void* rasterizer_result) {
