@@ -558,9 +558,14 @@ struct P2GScan {
 };
 
 
+// The accumulator of a block's nodes in LDS, component-major, one pad word
+// per 8 nodes so that nodes 8 apart (one step in y) fall in different banks.
+constexpr uint32_t kAccPitch = kBlkNodes + kBlkNodes / 8;
+__device__ __forceinline__ uint32_t acc_idx(uint32_t l, int k) { return (uint32_t)k * kAccPitch + l + (l >> 3); }
+
 // Step 4: one workgroup per chunk of a block's pairs.
 __device__ __forceinline__ void p2g_accum_body(P2GArgs a, uint32_t nBlocks, uint32_t block, uint32_t blocks) {
-  __shared__ int32_t acc[kBlkNodes][4];
+  __shared__ int32_t acc[4 * kAccPitch];
   __shared__ uint32_t s_blk;
   const uint32_t total = a.chunkStart[nBlocks];
   const f3 gmin = mk3(a.u->gridBoundsMin[0], a.u->gridBoundsMin[1], a.u->gridBoundsMin[2]);
@@ -575,7 +580,7 @@ __device__ __forceinline__ void p2g_accum_body(P2GArgs a, uint32_t nBlocks, uint
       }
       s_blk = lo;
     }
-    for (uint32_t i = threadIdx.x; i < kBlkNodes; i += blockDim.x) acc[i][0] = acc[i][1] = acc[i][2] = acc[i][3] = 0;
+    for (uint32_t i = threadIdx.x; i < 4 * kAccPitch; i += blockDim.x) acc[i] = 0;
     __syncthreads();
     const uint32_t b = s_blk;
     const uint32_t bx = b % a.nb, by = (b / a.nb) % a.nb, bz = b / (a.nb * a.nb);
@@ -599,7 +604,7 @@ __device__ __forceinline__ void p2g_accum_body(P2GArgs a, uint32_t nBlocks, uint
             const uint32_t l = ((z - z0) * kBlk + (y - y0)) * kBlk + (x - x0);
 #pragma unroll
             for (int k = 0; k < 4; k++)
-              if (v[k]) atomicAdd(&acc[l][k], v[k]);
+              if (v[k]) atomicAdd(&acc[acc_idx(l, k)], v[k]);
           }
     }
     __syncthreads();
@@ -609,7 +614,7 @@ __device__ __forceinline__ void p2g_accum_body(P2GArgs a, uint32_t nBlocks, uint
       int32_t* cell = reinterpret_cast<int32_t*>(&a.dv[grid_idx(a.d, x, y, z)]);
 #pragma unroll
       for (int k = 0; k < 4; k++)
-        if (acc[l][k]) atomicAdd(cell + k, acc[l][k]);
+        if (acc[acc_idx(l, k)]) atomicAdd(cell + k, acc[acc_idx(l, k)]);
     }
     __syncthreads();
   }
