@@ -141,8 +141,9 @@ constexpr uint32_t kTileQueue = 128;  // ring of queued inside pixels (power of 
 // lie in its convex hull (seg_pixel_bbox), and the hull lies in the band
 // |n . (p - c)| <= R around the axis through the two cap midpoints (n normal
 // to the axis a, c its centre) and in the slab T0 <= a . (p - c) <= T1 along
-// it, with R, T0, T1 taken over the four vertices. Both are widened by 1 px,
-// which covers f32 rounding. On a range of rows each constraint is an x
+// it, with R, T0, T1 taken over the four vertices. Both are widened by the
+// quad's margin (raster_margin), which covers the f32 rounding of the edge
+// tests and of these bounds. On a range of rows each constraint is an x
 // interval with ends linear in y.
 enum SpanField : uint32_t {
   kSpCx = 0, kSpCy, kSpNy, kSpR, kSpINx, kSpAy, kSpT0, kSpT1, kSpIAx, kSpOn, kSpFields
@@ -150,7 +151,7 @@ enum SpanField : uint32_t {
 
 // The span frame of a quad; on = 0 when the bound is not usable (degenerate
 // axis, collinear or far-off vertices, NaN): the caller keeps the whole box.
-__device__ __forceinline__ void span_frame(const Seg& s, bool usable, float* f, uint32_t stride) {
+__device__ __forceinline__ void span_frame(const Seg& s, bool usable, float margin, float* f, uint32_t stride) {
   const f2 m0 = mk2((s.v00.x + s.v01.x) * 0.5f, (s.v00.y + s.v01.y) * 0.5f);
   const f2 m1 = mk2((s.v10.x + s.v11.x) * 0.5f, (s.v10.y + s.v11.y) * 0.5f);
   const f2 ax = m1 - m0;
@@ -171,11 +172,12 @@ __device__ __forceinline__ void span_frame(const Seg& s, bool usable, float* f, 
   f[kSpCx * stride] = c.x;
   f[kSpCy * stride] = c.y;
   f[kSpNy * stride] = n.y;
-  f[kSpR * stride] = R + la;
+  const float w = la * margin;  // n and a have length la
+  f[kSpR * stride] = R + w;
   f[kSpINx * stride] = fabsf(n.x) > 1e-6f * la ? 1.0f / n.x : 0.0f;
   f[kSpAy * stride] = ax.y;
-  f[kSpT0 * stride] = T0 - la;
-  f[kSpT1 * stride] = T1 + la;
+  f[kSpT0 * stride] = T0 - w;
+  f[kSpT1 * stride] = T1 + w;
   f[kSpIAx * stride] = fabsf(ax.x) > 1e-6f * la ? 1.0f / ax.x : 0.0f;
   f[kSpOn * stride] = on ? 1.0f : 0.0f;
 }
@@ -342,13 +344,14 @@ __device__ __forceinline__ void tiles_wave(const TilesArgs& a, TileWaveLds& L, u
     // reject degenerate segments (:109-117); an inverted rectangle has no tile
     if (!(tsx * tsy > a.threshold) && tmaxx >= tminx && tmaxy >= tminy) nTiles = tsx * tsy;
     int cx0, cy0, cx1, cy1;
-    seg_pixel_bbox(ps, cx0, cy0, cx1, cy1);
+    const float margin = raster_margin(ps);
+    seg_pixel_bbox(ps, margin, cx0, cy0, cx1, cy1);
     const float vv[8] = {ps.v00.x, ps.v00.y, ps.v01.x, ps.v01.y, ps.v10.x, ps.v10.y, ps.v11.x, ps.v11.y};
     float amax = 0.0f;
 #pragma unroll
     for (int i = 0; i < 8; i++) { L.v[i][lane] = vv[i]; amax = fmaxf(amax, fabsf(vv[i])); }
     // collinear quads test whole tiles; far-off vertices skip the span
-    span_frame(ps, cx0 > -(1 << 20) && amax < 65536.0f, &L.sp[0][lane], 64);
+    span_frame(ps, cx0 > -(1 << 20) && amax < 65536.0f, margin, &L.sp[0][lane], 64);
     interp_pre(ps, &L.ip[0][lane], 64);
     L.d[0][lane] = ps.depths.x; L.d[1][lane] = ps.depths.y; L.d[2][lane] = ps.depths.z; L.d[3][lane] = ps.depths.w;
     L.box[0][lane] = cx0; L.box[1][lane] = cy0; L.box[2][lane] = cx1; L.box[3][lane] = cy1;
@@ -1362,14 +1365,15 @@ __device__ __forceinline__ void fine_record(const FineArgs& a, const FineLds<TIL
                                             const Seg& sg, uint32_t bx0, uint32_t by0) {
   constexpr uint32_t RB = fine_batch<TILE>();
   int x0, y0, x1, y1;
-  seg_pixel_bbox(sg, x0, y0, x1, y1);
+  const float margin = raster_margin(sg);
+  seg_pixel_bbox(sg, margin, x0, y0, x1, y1);
   const auto& R = L.recs;
   const float vv[8] = {sg.v00.x, sg.v00.y, sg.v01.x, sg.v01.y, sg.v10.x, sg.v10.y, sg.v11.x, sg.v11.y};
   float amax = 0.0f;
 #pragma unroll
   for (int i = 0; i < 8; i++) { R.at(kRecV + i, j) = vv[i]; amax = fmaxf(amax, fabsf(vv[i])); }
   // collinear quads test the whole box; far-off vertices skip the span
-  span_frame(sg, x0 > -(1 << 20) && amax < 65536.0f, &R.at(kRecSpan, j), RB);
+  span_frame(sg, x0 > -(1 << 20) && amax < 65536.0f, margin, &R.at(kRecSpan, j), RB);
   R.at(kRecD + 0, j) = sg.depths.x; R.at(kRecD + 1, j) = sg.depths.y;
   R.at(kRecD + 2, j) = sg.depths.z; R.at(kRecD + 3, j) = sg.depths.w;
   R.at(kRecKey, j) = __uint_as_float(key);
@@ -1641,6 +1645,63 @@ struct DebugRaster {
 };
 
 
+// Test hook: for each quad, every pixel that passes the four edge tests in a
+// window 3 px beyond its pixel box must lie in the box and in the x span of
+// its row (the bounds the tile and fine passes enumerate). Writes per quad
+// (pixels outside the box, pixels outside the span, candidates, inside).
+__device__ __forceinline__ void pixel_bounds_body(const float* __restrict__ quads, uint32_t n,
+                                                  uint4* __restrict__ out, uint32_t block, uint32_t blocks) {
+  const uint32_t i = block * 64u + threadIdx.x;
+  if (i >= n) return;
+  const float* q = quads + (size_t)i * 8u;
+  Seg s;
+  s.v00 = mk2(q[0], q[1]); s.v01 = mk2(q[2], q[3]); s.v10 = mk2(q[4], q[5]); s.v11 = mk2(q[6], q[7]);
+  s.depths = mk4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float margin = raster_margin(s);
+  int x0, y0, x1, y1;
+  seg_pixel_bbox(s, margin, x0, y0, x1, y1);
+  float amax = 0.0f;
+  for (int k = 0; k < 8; k++) amax = fmaxf(amax, fabsf(q[k]));
+  float sp[kSpFields];
+  span_frame(s, x0 > -(1 << 20) && amax < 65536.0f, margin, sp, 1);
+  uint4 r = make_uint4(0u, 0u, 0u, 0u);
+  // quads whose box is the unbounded one (collinear) are not checked
+  if (x0 <= -(1 << 20) || x1 - x0 > 200 || y1 - y0 > 200) { out[i] = make_uint4(0u, 0u, 0u, 0xffffffffu); return; }
+  for (int y = y0 - 3; y <= y1 + 3; y++) {
+    int xa = x0, xb = x1 + 1;
+    if (sp[kSpOn] != 0.0f) {
+      float xlo, xhi;
+      span_x<1>(sp, (float)y, (float)y, xlo, xhi);
+      if (xlo <= xhi) {
+        xa = max(xa, (int)fmaxf(ceilf(xlo), -1.0e9f));
+        xb = min(xb, (int)fminf(floorf(xhi), 1.0e9f) + 1);
+      } else {
+        xb = xa;
+      }
+    }
+    const bool rowIn = y >= y0 && y <= y1;
+    if (rowIn && xb > xa) r.z += (uint32_t)(xb - xa);
+    for (int x = x0 - 3; x <= x1 + 3; x++) {
+      if (!inside_quad(s, mk2((float)x, (float)y))) continue;
+      r.w++;
+      if (!rowIn || x < x0 || x > x1) r.x++;
+      else if (x < xa || x >= xb) r.y++;
+    }
+  }
+  out[i] = r;
+}
+
+struct PixelBounds {
+  struct Args {
+    const float* quads;
+    uint32_t n;
+    uint4* out;
+  };
+  static constexpr uint32_t kThreads = 64;
+  static constexpr const char* kName = "pixel_bounds";
+  static __device__ void run(const Args& a, uint32_t block, uint32_t blocks) { pixel_bounds_body(a.quads, a.n, a.out, block, blocks); }
+};
+
 int check_cfg(const fh_render_cfg* cfg, const char* fn) {
   if (!cfg) return fail(FH_ERR_INVALID_ARG, std::string(fn) + ": null cfg");
   if (!(cfg->tileSize == 8 || cfg->tileSize == 16 || cfg->tileSize == 32))
@@ -1873,6 +1934,13 @@ int fh_debug_raster_barycentric(fh_ctx* ctx, const fh_render_uniforms* uniforms,
   return launch<DebugRaster>(ctx, fh::div_ceil(nSeg, 64), 0,
                              {uniforms, reinterpret_cast<const float4*>(positions),
                               reinterpret_cast<const float4*>(tangents), strands_count, points_per_strand, out});
+}
+
+int fh_debug_pixel_bounds(fh_ctx* ctx, const float* quads, uint32_t count, uint32_t* out) {
+  FH_CHECK_CTX(ctx);
+  FH_REQUIRE(quads && out, "fh_debug_pixel_bounds: null buffer");
+  if (count == 0) return FH_OK;
+  return launch<PixelBounds>(ctx, fh::div_ceil(count, 64), 0, {quads, count, reinterpret_cast<uint4*>(out)});
 }
 
 int fh_fine_exhausted_bins(fh_ctx* ctx, uint32_t* out) {

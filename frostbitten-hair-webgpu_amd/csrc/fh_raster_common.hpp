@@ -156,9 +156,38 @@ __device__ __forceinline__ uint32_t depth_bin(uint32_t n, f2 td, float d) {
   return clampu(f2u(t * (float)n), 0u, n - 1u);
 }
 
+// How far beyond the convex hull of a quad's vertices a pixel may still pass
+// the four f32 edge tests. Each test rounds a difference of two products of
+// differences: an error below 4 units of 2^-24 of |p - a| |b - a| per product,
+// i.e. a distance error below 6e-7 |p - a| from the edge's line; for pixels
+// within the quad's box |p - a| is at most twice the coordinates' magnitude
+// M, so the error is below 2^-19.7 M. Near a convex corner of interior angle
+// t the lines' errors add up to at most 1 / sin(t / 2) times that beyond the
+// vertex. For a convex quad whose corners all have |sin t| >= 1/4 (t between
+// 14.5 and 165.5 degrees) the margin, 1/64 px plus 2^-16 M, is over twice the
+// worst case; any other quad (concave, self-intersecting, a sharp corner, a
+// NaN or infinite vertex, or M so large that the margin would exceed it)
+// keeps a whole pixel.
+__device__ __forceinline__ float raster_margin(const Seg& s) {
+  const float mag = fmaxf(fmaxf(fabsf(s.v00.x) + fabsf(s.v00.y), fabsf(s.v01.x) + fabsf(s.v01.y)),
+                          fmaxf(fabsf(s.v10.x) + fabsf(s.v10.y), fabsf(s.v11.x) + fabsf(s.v11.y)));
+  // the loop v00 -> v10 -> v11 -> v01 of the edge tests
+  const f2 e0 = s.v10 - s.v00, e1 = s.v11 - s.v10, e2 = s.v01 - s.v11, e3 = s.v00 - s.v01;
+  const float c0 = e3.x * e0.y - e3.y * e0.x, c1 = e0.x * e1.y - e0.y * e1.x;
+  const float c2 = e1.x * e2.y - e1.y * e2.x, c3 = e2.x * e3.y - e2.y * e3.x;
+  const float l0 = dot(e0, e0), l1 = dot(e1, e1), l2 = dot(e2, e2), l3 = dot(e3, e3);
+  const bool convex = (c0 > 0.0f && c1 > 0.0f && c2 > 0.0f && c3 > 0.0f) ||
+                      (c0 < 0.0f && c1 < 0.0f && c2 < 0.0f && c3 < 0.0f);
+  const bool open = 16.0f * c0 * c0 >= l3 * l0 && 16.0f * c1 * c1 >= l0 * l1 &&
+                    16.0f * c2 * c2 >= l1 * l2 && 16.0f * c3 * c3 >= l2 * l3;
+  const float m = 1.0f / 64.0f + mag * (1.0f / 65536.0f);
+  return (convex && open && m < 1.0f) ? m : 1.0f;
+}
+
 // Pixel bounding box of a projected quad (getRasterizedHairBounds before the
 // scissor), as integers clamped to a safe range. A pixel can only pass all
-// four edge tests if it lies within [min - 1, max + 1] of this box.
+// four edge tests if it lies within `margin` (raster_margin) of the box of
+// the vertices.
 //
 // Why: the four edge functions E_i(p) are the orientations of the triangles
 // (p, a_i, b_i) over the closed loop v00 -> v10 -> v11 -> v01. For p outside
@@ -166,16 +195,16 @@ __device__ __forceinline__ uint32_t depth_bin(uint32_t n, f2 td, float d) {
 // half-turn, so the signed angles swept along the loop sum to zero; all E_i >= 0
 // then forces every E_i = 0, i.e. p and the four vertices collinear. Any quad
 // (convex, concave or self-intersecting) thus has its inside pixels in the
-// hull; the 1 px margin absorbs f32 rounding near the hull. An exactly
-// collinear quad gets the unbounded box (the caller tests every pixel).
-__device__ __forceinline__ void seg_pixel_bbox(const Seg& s, int& x0, int& y0, int& x1, int& y1) {
+// hull; the margin absorbs f32 rounding near the hull. An exactly collinear
+// quad gets the unbounded box (the caller tests every pixel).
+__device__ __forceinline__ void seg_pixel_bbox(const Seg& s, float margin, int& x0, int& y0, int& x1, int& y1) {
   const f2 mx = vmax(vmax(s.v00, s.v01), vmax(s.v10, s.v11));
   const f2 mn = vmin(vmin(s.v00, s.v01), vmin(s.v10, s.v11));
   const float lo = -1048576.0f, hi = 1048576.0f;
-  x0 = (int)clampf(floorf(mn.x), lo, hi) - 1;
-  y0 = (int)clampf(floorf(mn.y), lo, hi) - 1;
-  x1 = (int)clampf(ceilf(mx.x), lo, hi) + 1;
-  y1 = (int)clampf(ceilf(mx.y), lo, hi) + 1;
+  x0 = (int)clampf(ceilf(mn.x - margin), lo, hi);
+  y0 = (int)clampf(ceilf(mn.y - margin), lo, hi);
+  x1 = (int)clampf(floorf(mx.x + margin), lo, hi);
+  y1 = (int)clampf(floorf(mx.y + margin), lo, hi);
   const f2 e1 = s.v10 - s.v00, e2 = s.v11 - s.v00, e3 = s.v01 - s.v00;
   const float k1 = e1.x * e2.y - e1.y * e2.x;
   const float k2 = e1.x * e3.y - e1.y * e3.x;

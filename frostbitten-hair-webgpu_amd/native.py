@@ -154,6 +154,7 @@ SIGNATURES = {
     "fh_graph_destroy": (i32, [vp]),
     "fh_profiler_read": (i32, [vp, C.POINTER(KernelTime), u32, C.POINTER(u32)]),
     "fh_set_persistent_blocks": (i32, [vp, u32]),
+    "fh_debug_pixel_bounds": (i32, [vp, vp, u32, vp]),
     "fh_set_raster_band_cap": (i32, [vp, u32]),
     "fh_scratch_bytes": (i32, [vp, C.POINTER(C.c_uint64)]),
     "fh_scratch_reserve": (i32, [vp, C.c_uint64]),

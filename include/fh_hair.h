@@ -817,6 +817,17 @@ int fh_debug_raster_barycentric(fh_ctx* ctx, const fh_render_uniforms* uniforms,
                                 uint32_t strands_count, uint32_t points_per_strand,
                                 uint32_t* out);
 
+/* Test hook (no counterpart in the reference): the pixel bounds that the
+ * tile and fine passes enumerate, checked against the edge tests
+ * (swRasterizeHair.wgsl.ts:127-129, processHairSegment.wgsl.ts:14-22). For
+ * quad i (v00, v01, v10, v11 as 8 floats at quads[8 i]) every pixel that
+ * passes the four edge tests within 3 px of the quad's pixel box must lie in
+ * the box and in its row's x span; out[4 i .. 4 i + 3] receives the pixels
+ * outside the box, the pixels outside the span, the candidates the bounds
+ * leave and the inside pixels (0, 0, 0, 0xffffffff for a quad whose box is
+ * unbounded or over 200 px, which is not checked). */
+int fh_debug_pixel_bounds(fh_ctx* ctx, const float* quads, uint32_t count, uint32_t* out);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif
