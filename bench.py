@@ -7,14 +7,19 @@ pre-pass, integration, particle-to-grid), then the tile pass, the tile sort,
 the fine pass and the shading pass, with inputs resident in HBM.
 
 By default frames are pipelined and replayed as HIP graphs (one graph per
-parity of the tick-tock position buffers): each frame runs the simulation
-step of frame f on a second stream beside the render passes of frame f-1, so
-every frame still does one simulation step and one render of the chain.
-Prints one JSON line. `value` comes from a timed region of --steps frames
-with no events between them, the frame latencies from a second region of as
-many frames with events at the frame boundaries; the latency
-from an image's simulation step to its present, and the rate of one graph per
-frame without pipelining (`unpipelined`), are reported beside them. `roofline` is for the kernel that takes the most time in a
+parity of the tick-tock position buffers), in two forms: each frame runs the
+simulation step of frame f on a second stream beside the render passes of
+frame f-1 (`pipelined`), or the simulation step of frame f, the tile and
+sort passes of frame f-1 and the fine, combine/present and shading passes of
+frame f-2 on three streams at once (`deep`); either way every frame does one
+simulation step and one render of the chain, and the images are the plain
+frames'. `value` is the faster form's (named by `frame_form`; `forms` reports
+both). Prints one JSON line. `value` comes from a timed region
+of --steps frames with no events between them, the frame latencies from a
+second region of as many frames with events at the frame boundaries; the
+latency from an image's simulation step to its present, and the rate of one
+graph per frame without pipelining (`unpipelined`), are reported beside
+them. `roofline` is for the kernel that takes the most time in a
 second timed region of as many frames, from its algorithmic bytes per launch
 and its average launch duration, measured with HIP events that the library
 records around each of its kernels on its own stream (per-kernel events cost
@@ -499,6 +504,14 @@ def served(n_req: int, rate: float, budget_gib: float, dev, log) -> dict:
     return out
 
 
+FORM_TEXT = {
+    "pipelined": "pipelined: simulation step of frame f beside the render passes of frame f-1; shading beside "
+                 "the combine/present pass",
+    "deep": "deep-pipelined: simulation step of frame f, tile and sort passes of frame f-1, fine, "
+            "combine/present and shading passes of frame f-2, on three streams at once",
+}
+
+
 def plain_run(kind: str, steps: int, warmup: int, args, dev) -> tuple:
     """Frames of the plain drop-in on workload `kind`: the timed regions, the
     per-kernel durations and the roofline of the dominant kernel. Returns the
@@ -509,7 +522,7 @@ def plain_run(kind: str, steps: int, warmup: int, args, dev) -> tuple:
     t_setup = time.perf_counter()
     wl = A.make_workload(kind)
     r = PS.HairRenderer(wl, dev, branches=not args.no_branches)
-    total = warmup + 5 * steps + 4  # five timed regions, pipeline prologue and sizing frame
+    total = warmup + 7 * steps + 8  # seven timed regions, pipeline prologues and sizing frames
     # per-frame simulation uniforms (frameIdx changes), uploaded once up front
     ring = torch.from_numpy(np.concatenate([r.sim_uniforms(f).view(np.uint8).reshape(-1) for f in range(total)]))
     ring = ring.to(dev).view(total, -1)
@@ -557,6 +570,13 @@ def plain_run(kind: str, steps: int, warmup: int, args, dev) -> tuple:
         if events:
             events[6].record()
 
+    def deep_frame(events=None):
+        if events:
+            events[0].record()
+        r.frame_deep_graph(upload_uniforms=False)
+        if events:
+            events[6].record()
+
     def region(step, events=True):
         evs = [[torch.cuda.Event(enable_timing=True) for _ in range(7)] for _ in range(steps)]
         torch.cuda.synchronize()
@@ -585,15 +605,28 @@ def plain_run(kind: str, steps: int, warmup: int, args, dev) -> tuple:
     # value and the frame latencies. Pipelined, each frame runs the simulation
     # step of frame f beside the render passes of frame f-1.
     unpipelined = None
+    forms = {}
     if pipelined:
         r.pipeline_begin(upload_uniforms=False)
         r.frame_pipelined(upload_uniforms=False)  # every call once before capture
         r.capture_pipelined()
-        elapsed, evs = timed_pair(pipelined_frame)
+        el_p, evs_p = timed_pair(pipelined_frame)
         r.pipeline_end()
         # sim-to-present: from the start of the frame that ran an image's
         # simulation step to the end of the frame that presented it
-        s2p = np.array([evs[k - 1][0].elapsed_time(evs[k][6]) for k in range(1, steps)])
+        forms["pipelined"] = (el_p, evs_p, np.array([evs_p[k - 1][0].elapsed_time(evs_p[k][6])
+                                                     for k in range(1, steps)]))
+        # deep-pipelined: simulation step f, tile stage f-1 and fine stage f-2
+        # at once (three streams)
+        r.deep_pipeline_begin(upload_uniforms=False)
+        r.frame_deep(upload_uniforms=False)  # every call once before capture
+        r.capture_deep()
+        el_d, evs_d = timed_pair(deep_frame)
+        r.deep_pipeline_end()
+        forms["deep"] = (el_d, evs_d, np.array([evs_d[k - 2][0].elapsed_time(evs_d[k][6])
+                                                for k in range(2, steps)]))
+        form = min(forms, key=lambda k: forms[k][0])
+        elapsed, evs, s2p = forms[form]
         r.capture_frames()
         el_u, evs_u = timed_pair(graph_frame)
         per_u = np.array([evs_u[k][0].elapsed_time(evs_u[k][6]) for k in range(steps)])
@@ -690,8 +723,7 @@ def plain_run(kind: str, steps: int, warmup: int, args, dev) -> tuple:
                                f"{wl.sim.constraintIterations} constraint iterations",
                    "global_batch": 1, "parallelism": "none (1 GPU)",
                    "overlap": "none (one stream)" if args.no_branches else
-                              ("pipelined: simulation step of frame f beside the render passes of frame f-1; "
-                               "shading beside the combine/present pass" if pipelined else
+                              (FORM_TEXT[form] if pipelined else
                                "grid accumulation beside tiles/sort/fine, shading beside the combine/present pass")},
         "latency_ms": {"p50": round(float(np.percentile(per_frame, 50)), 4),
                        "p99": round(float(np.percentile(per_frame, 99)), 4)},
@@ -715,9 +747,18 @@ def plain_run(kind: str, steps: int, warmup: int, args, dev) -> tuple:
         "rooflines": rooflines,
     }
     if pipelined:
+        out["frame_form"] = form
         out["latency_ms"]["frame"] = "per pipelined frame (frame period)"
         out["latency_ms"]["sim_to_present_p50"] = round(float(np.percentile(s2p, 50)), 4)
         out["latency_ms"]["sim_to_present_p99"] = round(float(np.percentile(s2p, 99)), 4)
+        out["forms"] = {}
+        for k, (el, ev, sp) in forms.items():
+            per = np.array([ev[i][0].elapsed_time(ev[i][6]) for i in range(steps)])
+            out["forms"][k] = {"value": round(steps / el, 3), "ms_per_step": round(el / steps * 1e3, 4),
+                               "latency_ms": {"p50": round(float(np.percentile(per, 50)), 4),
+                                              "p99": round(float(np.percentile(per, 99)), 4),
+                                              "sim_to_present_p50": round(float(np.percentile(sp, 50)), 4)},
+                               "frame": FORM_TEXT[k]}
         out["unpipelined"] = unpipelined
     return out, r, wl
 
@@ -801,7 +842,8 @@ def main() -> None:
         # the 4K, 100,000-strand workload (the dense fine-pass case) beside the 1080p line
         torch.cuda.empty_cache()
         full, r3, wl3 = plain_run("c3", args.c3_steps, 3, args, dev)
-        c3 = {k: full[k] for k in ("value", "unit", "ms_per_step", "latency_ms", "unpipelined", "throughput",
+        c3 = {k: full[k] for k in ("value", "unit", "ms_per_step", "latency_ms", "frame_form", "forms",
+                                   "unpipelined", "throughput",
                                    "roofline", "rooflines", "stage_ms", "kernel_us") if k in full}
         c3["config"] = full["config"]["workload"]
         c3["steps"] = args.c3_steps

@@ -144,7 +144,8 @@ int ensure_scratch(fh_ctx* ctx, size_t bytes) {
   if (ctx->recording) ctx->rec_grew = true;
   if (ctx->scratch) {
     FH_HIP(hipStreamSynchronize(ctx->stream));
-    if (ctx->branch_stream) FH_HIP(hipStreamSynchronize(ctx->branch_stream));
+    for (hipStream_t b : ctx->branch_stream)
+      if (b) FH_HIP(hipStreamSynchronize(b));
     FH_HIP(hipFree(ctx->scratch));
     ctx->scratch = nullptr;
     ctx->scratch_bytes = 0;
@@ -327,16 +328,19 @@ int fh_destroy(fh_ctx* ctx) {
   hipSetDevice(ctx->device);
   if (ctx->branch_open) fh_branch_end(ctx);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
-  if (ctx->branch_stream) {
-    hipStreamSynchronize(ctx->branch_stream);
-    hipStreamDestroy(ctx->branch_stream);
+  for (int l = 0; l < fh_ctx::kLanes; l++) {
+    if (ctx->branch_stream[l]) {
+      hipStreamSynchronize(ctx->branch_stream[l]);
+      hipStreamDestroy(ctx->branch_stream[l]);
+    }
+    if (ctx->branch_fork[l]) hipEventDestroy(ctx->branch_fork[l]);
+    if (ctx->branch_done[l]) hipEventDestroy(ctx->branch_done[l]);
   }
-  if (ctx->branch_fork) hipEventDestroy(ctx->branch_fork);
-  if (ctx->branch_done) hipEventDestroy(ctx->branch_done);
   for (hipEvent_t e : ctx->prof_events) hipEventDestroy(e);
   ctx->sgen->gen = fh::kGenGone;
   if (ctx->scratch) hipFree(ctx->scratch);
-  if (ctx->branch_scratch) hipFree(ctx->branch_scratch);
+  for (void* b : ctx->branch_scratch)
+    if (b) hipFree(b);
   if (ctx->srgb_thresholds) hipFree(ctx->srgb_thresholds);
   if (ctx->owns_stream && ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -345,7 +349,7 @@ int fh_destroy(fh_ctx* ctx) {
 
 int fh_set_stream(fh_ctx* ctx, void* stream) {
   FH_CHECK_CTX(ctx);
-  FH_REQUIRE(!ctx->branch_open && !ctx->branch_pending, "fh_set_stream: a branch is open or not joined");
+  FH_REQUIRE(!ctx->branch_open && !ctx->any_pending(), "fh_set_stream: a branch is open or not joined");
   if (ctx->owns_stream && ctx->stream) {
     FH_HIP(hipStreamSynchronize(ctx->stream));
     FH_HIP(hipStreamDestroy(ctx->stream));
@@ -360,7 +364,7 @@ void* fh_get_stream(fh_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 int fh_profiler_enable(fh_ctx* ctx, uint32_t max_launches) {
   FH_CHECK_CTX(ctx);
   FH_REQUIRE(max_launches <= (1u << 20), "fh_profiler_enable: max_launches must be <= 2^20");
-  FH_REQUIRE(!ctx->branch_open && !ctx->branch_pending, "fh_profiler_enable: a branch is open or not joined");
+  FH_REQUIRE(!ctx->branch_open && !ctx->any_pending(), "fh_profiler_enable: a branch is open or not joined");
   FH_HIP(hipStreamSynchronize(ctx->stream));
   ctx->prof_names.clear();
   while (ctx->prof_events.size() < 2ull * max_launches) {
@@ -400,7 +404,7 @@ int fh_profiler_read(fh_ctx* ctx, fh_kernel_time* out, uint32_t max_out, uint32_
 int fh_scratch_bytes(fh_ctx* ctx, uint64_t* bytes) {
   FH_CHECK_CTX(ctx);
   FH_REQUIRE(bytes != nullptr, "fh_scratch_bytes: null argument");
-  *bytes = ctx->scratch_bytes + ctx->branch_scratch_bytes;
+  *bytes = ctx->scratch_bytes + ctx->branch_scratch_bytes[0] + ctx->branch_scratch_bytes[1];
   return FH_OK;
 }
 
@@ -440,43 +444,48 @@ int fh_clear_batch_end(fh_ctx* ctx) {
   return FH_OK;
 }
 
-int fh_branch_begin(fh_ctx* ctx) {
+int fh_branch_begin_lane(fh_ctx* ctx, uint32_t lane) {
   FH_CHECK_CTX(ctx);
+  FH_REQUIRE(lane < (uint32_t)fh_ctx::kLanes, "fh_branch_begin_lane: lane must be 0 or 1");
   FH_REQUIRE(!ctx->branch_open, "fh_branch_begin: a branch is already open");
   FH_REQUIRE(!ctx->fill_batch, "fh_branch_begin: a clear batch is open");
   const bool live = !ctx->recording && ctx->prof_cap == 0;
   if (live) {
-    if (!ctx->branch_stream) {
+    if (!ctx->branch_stream[lane]) {
       FH_HIP(hipSetDevice(ctx->device));
-      FH_HIP(hipStreamCreateWithFlags(&ctx->branch_stream, hipStreamNonBlocking));
-      FH_HIP(hipEventCreateWithFlags(&ctx->branch_fork, hipEventDisableTiming));
-      FH_HIP(hipEventCreateWithFlags(&ctx->branch_done, hipEventDisableTiming));
+      FH_HIP(hipStreamCreateWithFlags(&ctx->branch_stream[lane], hipStreamNonBlocking));
+      FH_HIP(hipEventCreateWithFlags(&ctx->branch_fork[lane], hipEventDisableTiming));
+      FH_HIP(hipEventCreateWithFlags(&ctx->branch_done[lane], hipEventDisableTiming));
     }
-    FH_HIP(hipEventRecord(ctx->branch_fork, ctx->stream));
-    FH_HIP(hipStreamWaitEvent(ctx->branch_stream, ctx->branch_fork, 0));
+    FH_HIP(hipEventRecord(ctx->branch_fork[lane], ctx->stream));
+    FH_HIP(hipStreamWaitEvent(ctx->branch_stream[lane], ctx->branch_fork[lane], 0));
     ctx->main_stream = ctx->stream;
-    ctx->stream = ctx->branch_stream;
+    ctx->stream = ctx->branch_stream[lane];
   }
-  std::swap(ctx->scratch, ctx->branch_scratch);
-  std::swap(ctx->scratch_bytes, ctx->branch_scratch_bytes);
+  std::swap(ctx->scratch, ctx->branch_scratch[lane]);
+  std::swap(ctx->scratch_bytes, ctx->branch_scratch_bytes[lane]);
   ctx->branch_open = true;
+  ctx->branch_lane = (int)lane;
   ctx->branch_live = live;
   return FH_OK;
 }
+
+int fh_branch_begin(fh_ctx* ctx) { return fh_branch_begin_lane(ctx, 0); }
 
 int fh_branch_end(fh_ctx* ctx) {
   FH_CHECK_CTX(ctx);
   FH_REQUIRE(ctx->branch_open, "fh_branch_end: no branch is open");
   FH_REQUIRE(!ctx->fill_batch, "fh_branch_end: a clear batch is open");
-  std::swap(ctx->scratch, ctx->branch_scratch);
-  std::swap(ctx->scratch_bytes, ctx->branch_scratch_bytes);
+  const int lane = ctx->branch_lane;
+  std::swap(ctx->scratch, ctx->branch_scratch[lane]);
+  std::swap(ctx->scratch_bytes, ctx->branch_scratch_bytes[lane]);
   ctx->branch_open = false;
   if (!ctx->branch_live) return FH_OK;
   ctx->branch_live = false;
   ctx->stream = ctx->main_stream;
   ctx->main_stream = nullptr;
-  ctx->branch_pending = true;
-  FH_HIP(hipEventRecord(ctx->branch_done, ctx->branch_stream));
+  ctx->branch_pending[lane] = true;
+  FH_HIP(hipEventRecord(ctx->branch_done[lane], ctx->branch_stream[lane]));
   return FH_OK;
 }
 
@@ -484,9 +493,11 @@ int fh_branch_join(fh_ctx* ctx) {
   FH_CHECK_CTX(ctx);
   FH_REQUIRE(!ctx->branch_open, "fh_branch_join: the branch is still open");
   FH_REQUIRE(!ctx->fill_batch, "fh_branch_join: a clear batch is open");
-  if (!ctx->branch_pending) return FH_OK;
-  ctx->branch_pending = false;
-  FH_HIP(hipStreamWaitEvent(ctx->stream, ctx->branch_done, 0));
+  for (int l = 0; l < fh_ctx::kLanes; l++) {
+    if (!ctx->branch_pending[l]) continue;
+    ctx->branch_pending[l] = false;
+    FH_HIP(hipStreamWaitEvent(ctx->stream, ctx->branch_done[l], 0));
+  }
   return FH_OK;
 }
 
@@ -641,7 +652,7 @@ int fh_program_destroy(fh_program* program) {
 int fh_capture_begin(fh_ctx* ctx) {
   FH_CHECK_CTX(ctx);
   FH_REQUIRE(!ctx->capturing, "fh_capture_begin: already capturing");
-  FH_REQUIRE(!ctx->branch_open && !ctx->branch_pending, "fh_capture_begin: a branch is open or not joined");
+  FH_REQUIRE(!ctx->branch_open && !ctx->any_pending(), "fh_capture_begin: a branch is open or not joined");
   FH_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
   ctx->capturing = true;
   ctx->cap_names.clear();
@@ -713,7 +724,8 @@ int fh_sync(fh_ctx* ctx) {
   FH_CHECK_CTX(ctx);
   FH_REQUIRE(!ctx->fill_batch, "fh_sync: a clear batch is open");
   FH_HIP(hipStreamSynchronize(ctx->stream));
-  if (ctx->branch_stream) FH_HIP(hipStreamSynchronize(ctx->branch_stream));
+  for (hipStream_t b : ctx->branch_stream)
+    if (b) FH_HIP(hipStreamSynchronize(b));
   return FH_OK;
 }
 

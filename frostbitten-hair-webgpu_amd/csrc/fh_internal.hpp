@@ -72,14 +72,20 @@ struct fh_ctx {
   // branch has its own scratch, swapped into `scratch` while it is open, so
   // the two streams never share temporaries. While recording or timing
   // kernels the calls stay on the main stream (the scratch is still swapped).
-  hipStream_t branch_stream = nullptr;
+  // Two branch lanes (fh_branch_begin: lane 0, fh_branch_begin_lane), each
+  // with its own stream, events and scratch; at most one is open at a time,
+  // and both may be pending (running) beside the main stream.
+  static constexpr int kLanes = 2;
+  hipStream_t branch_stream[kLanes] = {nullptr, nullptr};
   hipStream_t main_stream = nullptr;  // the context's stream while a branch runs live
-  hipEvent_t branch_fork = nullptr, branch_done = nullptr;
-  void* branch_scratch = nullptr;
-  size_t branch_scratch_bytes = 0;
-  bool branch_open = false;     // between fh_branch_begin and fh_branch_end
-  bool branch_live = false;     // the open branch runs on branch_stream
-  bool branch_pending = false;  // branch work not yet joined into the main stream
+  hipEvent_t branch_fork[kLanes] = {nullptr, nullptr}, branch_done[kLanes] = {nullptr, nullptr};
+  void* branch_scratch[kLanes] = {nullptr, nullptr};
+  size_t branch_scratch_bytes[kLanes] = {0, 0};
+  bool branch_open = false;                     // between fh_branch_begin* and fh_branch_end
+  int branch_lane = 0;                          // the open branch's lane
+  bool branch_live = false;                     // the open branch runs on its lane's stream
+  bool branch_pending[kLanes] = {false, false};  // lane work not yet joined into the main stream
+  bool any_pending() const { return branch_pending[0] || branch_pending[1]; }
   // Between fh_clear_batch_begin and _end the clears and copies of the entry
   // points are collected here and issued as one launch by _end.
   bool fill_batch = false;

@@ -1174,6 +1174,7 @@ struct FineLds {
   static constexpr uint32_t PXW = NPX / NW;  // pixels per wave
   RecStore<fine_batch<TILE>()> recs;
   uint32_t* heads;   // NS * NPX: newest slot of each (slice, pixel) list
+  uint32_t* smask;   // NPX: the slices whose list is not empty (bit s: slice s)
   float* db;         // NPX: mesh depth
   uint32_t* done;    // NPX: pixel saturated
   uint32_t* ring;    // NW * kFineRing
@@ -1259,7 +1260,10 @@ __device__ __forceinline__ void fine_shade_queue(const FineArgs& a, const FineLd
     const uint32_t next = below ? prevSlot : *hp;
     if (slot < a.ldsPool) L.poolNext[slot] = next;
     else gpool[slot - a.ldsPool].z = next;
-    if (above == 0ull) *hp = slot;
+    if (above == 0ull) {
+      *hp = slot;
+      atomicOr(&L.smask[pix], 1u << tag);  // lanes of one pixel may set different slices
+    }
   }
   wave_sync();
 }
@@ -1423,7 +1427,8 @@ __device__ __forceinline__ void fine_raster_body(FineArgs a, uint32_t block, uin
   FineLds<TILE> L;
   L.recs.f = reinterpret_cast<float*>(smem);
   L.heads = reinterpret_cast<uint32_t*>(smem + (size_t)RB * kRecFields * 4);
-  L.db = reinterpret_cast<float*>(L.heads + (size_t)a.NS * NPX);
+  L.smask = L.heads + (size_t)a.NS * NPX;
+  L.db = reinterpret_cast<float*>(L.smask + NPX);
   L.done = reinterpret_cast<uint32_t*>(L.db + NPX);
   L.ring = L.done + NPX;
   L.wsc = L.ring + NW * kFineRing;
@@ -1452,6 +1457,7 @@ __device__ __forceinline__ void fine_raster_body(FineArgs a, uint32_t block, uin
   uint4* gpool = a.slicePool + (size_t)block * a.perProc;
 
   for (uint32_t i = tid; i < a.NS * NPX; i += blockDim.x) L.heads[i] = kInvalidPtr;
+  for (uint32_t i = tid; i < NPX; i += blockDim.x) L.smask[i] = 0u;
 
   while (true) {
     __syncthreads();
@@ -1549,7 +1555,12 @@ __device__ __forceinline__ void fine_raster_body(FineArgs a, uint32_t block, uin
       if (!done) {
         if (!readable) fc = mk4(0.0f, 0.0f, 0.0f, 0.0f);
         uint32_t cnt = isDbg ? f2u(fc.x * (float)dbgMax) : 0u;
-        for (uint32_t s = 0; s < a.NS; s++) {
+        // only the slices with fragments: the others' heads are empty
+        uint32_t sm = L.smask[tid];
+        L.smask[tid] = 0u;
+        while (sm != 0u) {
+          const uint32_t s = (uint32_t)__builtin_ctz(sm);
+          sm &= sm - 1u;
           uint32_t cur = L.heads[s * NPX + tid];
           L.heads[s * NPX + tid] = kInvalidPtr;
           if (!isDbg && fc.w >= 0.999f) continue;
@@ -1910,7 +1921,7 @@ int fh_fine_raster(fh_ctx* ctx, const fh_render_cfg* cfg, uint32_t vp_w, uint32_
   const uint32_t nw = T == 8 ? 1u : T / 2u;
   const size_t ldsTarget = std::min<size_t>(T == 8 ? 40 * 1024 : (T == 16 ? 78 * 1024 : 150 * 1024),
                                             ctx->max_lds - 256);
-  const size_t fixed = (size_t)a.batch * kRecFields * 4 + (size_t)npx * 4 * (a.NS + 2) +
+  const size_t fixed = (size_t)a.batch * kRecFields * 4 + (size_t)npx * 4 * (a.NS + 3) +
                        (size_t)nw * (kFineRing + 6 * 64) * 4;
   FH_REQUIRE(fixed + 12 * 64 <= ldsTarget, "fh_fine_raster: slicesPerPixel too large for the tile size");
   a.ldsPool = (uint32_t)std::min<size_t>((ldsTarget - fixed) / 12, a.perProc);

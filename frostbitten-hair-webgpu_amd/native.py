@@ -145,6 +145,7 @@ SIGNATURES = {
     "fh_profiler_enable": (i32, [vp, u32]),
     "fh_capture_begin": (i32, [vp]),
     "fh_branch_begin": (i32, [vp]),
+    "fh_branch_begin_lane": (i32, [vp, u32]),
     "fh_branch_end": (i32, [vp]),
     "fh_branch_join": (i32, [vp]),
     "fh_clear_batch_begin": (i32, [vp]),
@@ -246,8 +247,8 @@ class Context:
     def sync(self) -> None:
         check(self.lib.fh_sync(self.handle), "fh_sync")
 
-    def branch_begin(self) -> None:
-        check(self.lib.fh_branch_begin(self.handle), "fh_branch_begin")
+    def branch_begin(self, lane: int = 0) -> None:
+        check(self.lib.fh_branch_begin_lane(self.handle, lane), "fh_branch_begin_lane")
 
     def branch_end(self) -> None:
         check(self.lib.fh_branch_end(self.handle), "fh_branch_end")

@@ -84,6 +84,18 @@ class HairObject:
         self.positions[1].copy_(self.initialPositions)
 
 
+class _PositionsView:
+    """A hair object whose `positionsNow` is another buffer (a copy of one
+    frame's positions); every other attribute is the hair object's."""
+
+    def __init__(self, hair: HairObject, now: torch.Tensor):
+        self._hair = hair
+        self.positionsNow = now
+
+    def __getattr__(self, name):
+        return getattr(self._hair, name)
+
+
 class GridData:
     """src/passes/simulation/grids/gridData.ts:14-107"""
 
@@ -536,6 +548,13 @@ class HairRenderer:
         self.gridPostSimPass.cmdUpdateGridsAfterSim(ctx, self.hair)
         self._branch_end()
 
+    def simulateStep(self, ctx: PassCtx) -> None:
+        """simulateHair's passes one after another on the current stream."""
+        self._next_uniforms()
+        self.gridPreSimPass.cmdUpdateGridsBeforeSim(ctx, self.hair)
+        self.simPass.cmdSimulateHairPositions(ctx, self.hair)
+        self.gridPostSimPass.cmdUpdateGridsAfterSim(ctx, self.hair)
+
     def drawScene(self, ctx: PassCtx) -> None:
         """The HDR scene of the background and mesh passes (renderer.ts:187-188,
         :244-246). Without mesh_passes their output is an input, copied into
@@ -679,6 +698,133 @@ class HairRenderer:
             self.upload_sim_uniforms()
         k = self.hair.current ^ self._pgraphBase
         self._pgraphs[k].launch()
+        self.hair.swapPositionBuffersAfterSimIntegration()
+        self.frameIdx += 1
+
+    # Deep-pipelined frames: three frames in flight, one stage each. Frame f's
+    # simulation step runs on branch lane 0, the tile and sort passes of frame
+    # f-1 on lane 1, and the fine, combine-and-present and shading passes of
+    # frame f-2 on the context's stream, all at once. The stages share no
+    # buffer a frame writes: the tile passes of consecutive frames alternate
+    # between two sets of tile buffers (the tile set of frame g is g mod 2),
+    # and the fine and shading passes of frame g read a copy of frame g's
+    # positions that its tile stage took (the simulation step of frame g+2
+    # overwrites the older buffer of the tick-tock pair while they run). The
+    # shading points are written by the fine stage after its fine pass, which
+    # read the previous fine stage's. Each pass computes exactly what it
+    # computes in frame(): after deep frame f the simulation state is frame
+    # f's, the tile set (f-1) mod 2 holds frame f-1's tile lists, and the
+    # image, screen and shading points are frame f-2's.
+
+    def _deep_alloc(self) -> None:
+        if getattr(self, "_deepSets", None) is not None:
+            return
+        assert not self.hw_passes, "deep-pipelined frames take the hair's depth, shadow map and AO as inputs"
+        rc, W, H = self.wl.render, self.wl.width, self.wl.height
+        tiles2 = HairTilesPass()
+        tiles2.onViewportResize(self.cfg, rc, W, H, self.device)
+        sort2 = HairTileSortPass(rc, self.device)
+        sort2.onViewportResize(rc, W, H, self.device)
+        self._deepSets = [(self.tilesPass, self.sortPass), (tiles2, sort2)]
+        self._deepPos = [torch.empty_like(self.hair.positionsNow) for _ in range(2)]
+
+    def deep_tile_set(self, frame: int) -> tuple:
+        """(tiles pass, sort pass) holding frame `frame`'s tile lists."""
+        return self._deepSets[frame % 2]
+
+    def _deep_ctx(self, frame: int) -> PassCtx:
+        ctx = self.ctx()
+        tp, sp = self.deep_tile_set(frame)
+        ctx.hairTilesBuffer = tp.hairTilesBuffer
+        ctx.hairTileSegmentsBuffer = tp.hairTileSegmentsBuffer
+        ctx.hairSegmentCountPerTileBuffer = tp.segmentCountPerTileBuffer
+        ctx.hairTileListBuffer = sp.tileListBuffer
+        return ctx
+
+    def _deep_calls(self, f: int, sim: bool = True, tiles: bool = True, fine: bool = True) -> None:
+        """Simulation step of frame f, tile stage of frame f-1, fine stage of frame f-2."""
+        drawn = self.hair.current  # frame f-1's positions
+        if sim:
+            self.native.branch_begin(0)
+            try:
+                self.simulateStep(self.ctx())  # swaps the pair
+            finally:
+                self.native.branch_end()
+        if tiles:
+            g = f - 1
+            tp, sp = self.deep_tile_set(g)
+            ctx = self._deep_ctx(g)
+            view = _PositionsView(self.hair, self.hair.positions[drawn])
+            self.native.branch_begin(1)
+            try:
+                self.native.clear_batch_begin()
+                try:
+                    tp.cmdClearBeforeRender(ctx)
+                    sp.cmdClearBeforeRender(ctx)
+                finally:
+                    self.native.clear_batch_end()
+                tp.cmdDrawHairToTiles(ctx, view)
+                sp.cmdSortHairTiles(ctx)
+                pos = self._deepPos[g % 2]
+                self.native.call("fh_copy", N.ptr(pos), N.ptr(view.positionsNow), pos.numel() * pos.element_size())
+            finally:
+                self.native.branch_end()
+        if fine:
+            g = f - 2
+            tp, _ = self.deep_tile_set(g)
+            ctx = self._deep_ctx(g)
+            view = _PositionsView(self.hair, self._deepPos[g % 2])
+            self.finePass.cmdClearBeforeRender(ctx)
+            self.finePass.cmdRasterizeSlicesHair(ctx, view, tp)
+            self.hairCombinePass.cmdCombineAndPresent(ctx, self.presentPass, self.screenTexture,
+                                                      self.inputs["hdrScene"])
+            self.shadingPass.cmdComputeShadingPoints(ctx, view)
+        self.native.branch_join()
+
+    def deep_pipeline_begin(self, upload_uniforms: bool = True) -> None:
+        """The prologue: frame 0's simulation step, then frame 1's beside
+        frame 0's tile stage."""
+        self._deep_alloc()
+        for stages in ((True, False, False), (True, True, False)):
+            if upload_uniforms:
+                self.upload_sim_uniforms()
+            self._deep_calls(self.frameIdx, *stages)
+            self.frameIdx += 1
+        self._deepOn = True
+
+    def frame_deep(self, upload_uniforms: bool = True) -> None:
+        """Simulation step of frame frameIdx beside the tile stage of frame
+        frameIdx - 1 and the fine stage of frame frameIdx - 2."""
+        assert getattr(self, "_deepOn", False), "deep_pipeline_begin() first"
+        if upload_uniforms:
+            self.upload_sim_uniforms()
+        self._deep_calls(self.frameIdx)
+        self.frameIdx += 1
+
+    def deep_pipeline_end(self) -> None:
+        """The epilogue: the last two frames' remaining stages."""
+        assert getattr(self, "_deepOn", False), "deep_pipeline_begin() first"
+        f = self.frameIdx
+        self._deep_calls(f, sim=False)
+        self._deep_calls(f + 1, sim=False, tiles=False)
+        self._deepOn = False
+
+    def capture_deep(self) -> list:
+        """Two HIP graphs of deep-pipelined frames (one per frame parity: the
+        position pair and the tile sets alternate together); replayed by
+        frame_deep_graph()."""
+        state = {"f": self.frameIdx}
+
+        def calls():
+            self._deep_calls(state["f"])
+            state["f"] += 1
+        return self._capture(calls, "_dgraphs", "_dgraphBase")
+
+    def frame_deep_graph(self, upload_uniforms: bool = True) -> None:
+        assert getattr(self, "_deepOn", False), "deep_pipeline_begin() first"
+        if upload_uniforms:
+            self.upload_sim_uniforms()
+        self._dgraphs[self.hair.current ^ self._dgraphBase].launch()
         self.hair.swapPositionBuffersAfterSimIntegration()
         self.frameIdx += 1
 

@@ -718,6 +718,12 @@ int fh_graph_destroy(fh_graph* graph);
 int fh_branch_begin(fh_ctx* ctx);
 int fh_branch_end(fh_ctx* ctx);
 int fh_branch_join(fh_ctx* ctx);
+/* A branch on lane 0 (fh_branch_begin's) or lane 1: each lane has its own
+ * stream and scratch, so the work of both lanes and of the context's stream
+ * may run at once (three stages of a pipelined frame). One branch is open at
+ * a time; fh_branch_end leaves its lane pending, and fh_branch_join joins
+ * both lanes. Consecutive branches of one lane run in order. */
+int fh_branch_begin_lane(fh_ctx* ctx, uint32_t lane);
 
 /* Clear batches: the clears and copies of the entry points called between
  * fh_clear_batch_begin and fh_clear_batch_end (fh_tiles_clear,

@@ -6,7 +6,10 @@ eagerly and replayed as HIP graphs, over both parities of the position
 buffers, with the precomputed scene inputs, with the hardware-raster passes
 and with the background and mesh passes drawn every frame. Pipelined frames
 (the simulation step of frame f beside the render passes of frame f-1) leave, frame for frame, the simulation state of frame f
-and the render state of frame f-1 that plain frames leave."""
+and the render state of frame f-1 that plain frames leave, and deep-pipelined
+frames (simulation step f, tile stage f-1, fine stage f-2 on three streams)
+the simulation state of frame f, the tile lists of frame f-1 and the image,
+screen and shading points of frame f-2."""
 import numpy as np
 import pytest
 
@@ -123,6 +126,67 @@ def test_pipelined_frames_match_plain(gpu, passes, use_graph):
     pipe.pipeline_end()
     torch.cuda.synchronize()
     _same(ref[frames - 1][1], _split(_state(pipe))[1], f"render state of frame {frames - 1}")
+
+
+def _tile_state(tp, sp):
+    from gpu_util import walk_lists
+    tiles = _np(tp.hairTilesBuffer).reshape(-1, 4)
+    seg = _np(tp.hairTileSegmentsBuffer).view(np.uint32)
+    lists = walk_lists(tiles.view(np.uint32), seg, tp.segmentsCapacity)
+    return dict(depth=tiles[:, [0, 1, 3]], lists=sorted(lists.items()), counts=_np(tp.segmentCountPerTileBuffer),
+                tile_list=sorted(_np(sp.tileListBuffer).view(np.uint32)[1:1 + int(_np(sp.tileListBuffer)[0])]))
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_deep_pipelined_frames_match_plain(gpu, use_graph):
+    """Deep-pipelined frames (simulation step f, tile stage f-1 and fine
+    stage f-2 at once, on three streams): after frame f the simulation state
+    is plain frame f's, the tile set of frame f-1 holds plain frame f-1's
+    tile lists, and the image, screen and shading points are plain frame
+    f-2's, frame for frame, over both parities of the position pair and of
+    the tile sets."""
+    import torch
+    from frostbitten_hair_webgpu_amd import assets as A
+    from frostbitten_hair_webgpu_amd import passes as PS
+    wl = A.make_workload("c2", seed=79, strands=1100, width=416, height=272, points=24)
+    print(f"deep pipeline: graph={use_graph} {wl.strands}x{wl.points} {wl.width}x{wl.height}", flush=True)
+    plain = PS.HairRenderer(wl, "cuda:0", shadow_size=512, branches=False)
+    inputs = {k: v.cpu().numpy() for k, v in plain.inputs.items()}
+    deep = PS.HairRenderer(wl, "cuda:0", shadow_size=512, inputs=inputs, branches=True)
+    frames = 7
+    plain.beforeFirstFrame()
+    ref = []
+    for _ in range(frames):
+        plain.frame()
+        torch.cuda.synchronize()
+        st = _state(plain)
+        ref.append(dict(sim={k: st[k] for k in SIM_KEYS}, tiles=_tile_state(plain.tilesPass, plain.sortPass),
+                        fine=dict(img=st["img"], shading=st["shading"], screen=st["screen"])))
+
+    def fine_state(r):
+        return dict(img=_np(r.finePass.image(wl.width, wl.height)), shading=_np(r.hair.shading),
+                    screen=_np(r.screenTexture))
+
+    deep.beforeFirstFrame()
+    deep.deep_pipeline_begin()  # frame 0's simulation step, frame 1's beside frame 0's tile stage
+    torch.cuda.synchronize()
+    _same(ref[1]["sim"], {k: _state(deep)[k] for k in SIM_KEYS}, "simulation state of frame 1")
+    _same(ref[0]["tiles"], _tile_state(*deep.deep_tile_set(0)), "tile lists of frame 0")
+    for f in range(2, frames):
+        if use_graph and f >= 3:
+            if f == 3:
+                deep.capture_deep()
+            deep.frame_deep_graph()
+        else:
+            deep.frame_deep()  # frame 2 runs every call once before the capture
+        torch.cuda.synchronize()
+        _same(ref[f]["sim"], {k: _state(deep)[k] for k in SIM_KEYS}, f"simulation state of frame {f}")
+        _same(ref[f - 1]["tiles"], _tile_state(*deep.deep_tile_set(f - 1)), f"tile lists of frame {f - 1}")
+        _same(ref[f - 2]["fine"], fine_state(deep), f"image, screen and shading of frame {f - 2}")
+    deep.deep_pipeline_end()
+    torch.cuda.synchronize()
+    _same(ref[frames - 1]["tiles"], _tile_state(*deep.deep_tile_set(frames - 1)), f"tile lists of frame {frames - 1}")
+    _same(ref[frames - 1]["fine"], fine_state(deep), f"image, screen and shading of frame {frames - 1}")
 
 
 def test_branch_errors(gpu):
