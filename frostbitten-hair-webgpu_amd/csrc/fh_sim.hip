@@ -63,6 +63,28 @@ __device__ __forceinline__ float sample_sdf(const SdfView& s, f3 p) {
   return sample_volume(s.vol, s.dims, t);
 }
 
+// The constraint sweeps' and collisions' divisions and square roots on the
+// hardware reciprocal, square root and reciprocal square root (about 1 ulp
+// each; WGSL allows 2.5 ulp for division and inherits the platform's square
+// root). The integration's positions are compared with the oracle within a
+// relative tolerance, not bit for bit, and the sweeps are chains of dependent
+// steps whose length these set (1080p: 151 -> ~105 us). The grid forces keep
+// correctly rounded arithmetic: the grid velocity is vel / density, which
+// does not vanish as the interpolation weights do, so a weight that rounds to
+// 1e-7 instead of 0 next to an empty node moves a point by a whole grid
+// displacement. The grid passes keep it too: their grids are compared bit for
+// bit.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float flength(f3 a) { return __builtin_amdgcn_sqrtf(dot(a, a)); }
+__device__ __forceinline__ f3 fnormalize(f3 a) { return a * frsq(dot(a, a)); }
+
+__device__ __forceinline__ float sample_sdf_fast(const SdfView& s, f3 p) {
+  const f3 e = s.bmax - s.bmin;
+  f3 t = sat((p - s.bmin) * mk3(frcp(e.x), frcp(e.y), frcp(e.z)));
+  return sample_volume(s.vol, s.dims, t);
+}
+
 // ---------------------------------------------------------------------------
 // K1 — gridPreSimPass.wgsl.ts:38-159. One thread per grid point.
 // ---------------------------------------------------------------------------
@@ -257,8 +279,8 @@ __device__ __forceinline__ void sim_integrate_body(
         float4 p1 = A[(j + 1) * L + lane];
         const float expected = Len[j * L + lane];
         const f3 tg = mk3(p1.x - p0.x, p1.y - p0.y, p1.z - p0.z);
-        const float corr = 1.0f - expected / length(tg);
-        const float df = corr * kLen / (p0.w + p1.w + kEps);
+        const float corr = 1.0f - expected * frsq(dot(tg, tg));
+        const float df = corr * kLen * frcp(p0.w + p1.w + kEps);
         const f3 delta = tg * df;
         p0 = make_float4(p0.x + delta.x * p0.w, p0.y + delta.y * p0.w, p0.z + delta.z * p0.w, p0.w);
         p1 = make_float4(p1.x - delta.x * p1.w, p1.y - delta.y * p1.w, p1.z - delta.z * p1.w, p1.w);
@@ -288,10 +310,9 @@ __device__ __forceinline__ void sim_integrate_body(
       for (; jj + 3 < P; jj++) {
         const float4 c = A[(jj + 2) * L + lane];
         const float4 Kc = K[jj * L + lane];
-        const f3 cc = ((mk3(a.x, a.y, a.z) + mk3(b.x, b.y, b.z)) + mk3(c.x, c.y, c.z)) / 3.0f;
+        const f3 cc = ((mk3(a.x, a.y, a.z) + mk3(b.x, b.y, b.z)) + mk3(c.x, c.y, c.z)) * (1.0f / 3.0f);
         const f3 hv = mk3(b.x, b.y, b.z) - cc;
-        const float h = length(hv);
-        const f3 delta = hv * (1.0f - Kc.x / h);
+        const f3 delta = hv * (1.0f - Kc.x * frsq(dot(hv, hv)));
         const float k0 = kLocal * Kc.y, k1 = kLocal * Kc.z, k2 = kLocal * Kc.w;
         A[jj * L + lane] = make_float4(a.x + delta.x * k0, a.y + delta.y * k0, a.z + delta.z * k0, a.w);
         a = make_float4(b.x + delta.x * k1, b.y + delta.y * k1, b.z + delta.z * k1, b.w);
@@ -321,18 +342,18 @@ __device__ __forceinline__ void sim_integrate_body(
       const float4 p4 = A[o];
       f3 p = mk3(p4.x, p4.y, p4.z);
       const f3 v = p - xyz(sphere);
-      const float toShell = fmx(0.0f, -(length(v) - sphere.w));
-      p = p + (normalize(v) * toShell) * kColl;
+      const float toShell = fmx(0.0f, -(flength(v) - sphere.w));
+      p = p + (fnormalize(v) * toShell) * kColl;
       if (pt > 2u) {
-        const float sd = sample_sdf(sv, p);
+        const float sd = sample_sdf_fast(sv, p);
         const float sdo = sd + sdfOffset;
         if (!(sdo >= 0.0f)) {
           f3 cd;
-          cd.x = sample_sdf(sv, p + mk3(kSdfGradOffset, 0.0f, 0.0f));
-          cd.y = sample_sdf(sv, p + mk3(0.0f, kSdfGradOffset, 0.0f));
-          cd.z = sample_sdf(sv, p + mk3(0.0f, 0.0f, kSdfGradOffset));
+          cd.x = sample_sdf_fast(sv, p + mk3(kSdfGradOffset, 0.0f, 0.0f));
+          cd.y = sample_sdf_fast(sv, p + mk3(0.0f, kSdfGradOffset, 0.0f));
+          cd.z = sample_sdf_fast(sv, p + mk3(0.0f, 0.0f, kSdfGradOffset));
           cd = cd - mk3(sd, sd, sd);
-          p = p + (normalize(cd) * -sdo) * kSdf;
+          p = p + (fnormalize(cd) * -sdo) * kSdf;
         }
       }
       A[o] = make_float4(p.x, p.y, p.z, p4.w);
