@@ -212,6 +212,47 @@ __device__ __forceinline__ void span_x(const float* f, float y0, float y1, float
   hi += cx;
 }
 
+// The view-space depth of a pixel's hair depth z, as the tile pass computes
+// it (projectVertex of (x, y, z, 1) with projMatrixInv) but at x = y = 0:
+// the clip z and w before the division.
+__device__ __forceinline__ f2 clip_zw(const float* m, float z) {
+  const f4 c = mul(m, mk4(0.0f, 0.0f, z, 1.0f));
+  return mk2(c.z, c.w);
+}
+
+// Whether a segment's depth bin can be taken per (segment, tile) pair from
+// the pair's smallest and largest hair depth instead of per pixel.
+// getDepthBin (shaderSnippets.wgls.ts:148-156) is monotone non-decreasing in
+// the view-space depth for any tile depth range (each of its operations is,
+// including the infinite and NaN cases that saturate to 0 or n - 1), so if
+// the view-space depth is monotone in the hair depth over every value a
+// pixel of the segment can take, the closest bin over the pair's pixels
+// (:201, :206) is the bin at one of the two extreme depths. It is when
+//  - the x and y columns of projMatrixInv's z and w rows are exact zeros, so
+//    the pixel position adds only signed zeros to clip z and w, which is
+//    what clip_zw computes at x = y = 0;
+//  - every hair depth lies in [lo, hi] (the segment's four depths widened by
+//    the rounding of interp_f32's two mixes, which satf keeps convex), and
+//    the depths are so small that no mix overflows;
+//  - clip z and w are each a monotone function of the depth (one rounded
+//    product and one rounded sum each); equal values at lo and hi make one
+//    of them constant on [lo, hi], and w keeps one sign there, so the
+//    correctly rounded quotient is monotone too.
+// The per-pixel path handles every other case.
+__device__ __forceinline__ bool pair_bin_ok(const float* m, f4 d) {
+  if (!(m[2] == 0.0f && m[6] == 0.0f && m[3] == 0.0f && m[7] == 0.0f)) return false;
+  const float dn = fminf(fminf(d.x, d.y), fminf(d.z, d.w));
+  const float dx = fmaxf(fmaxf(d.x, d.y), fmaxf(d.z, d.w));
+  // NaN fails both tests; |d| <= 998 also keeps every hair depth inside the
+  // accumulators' +-999 sentinels, so the pair's min and max are pixel values
+  if (!(dn >= -998.0f && dx <= 998.0f)) return false;
+  const float r = 998.0f * 2e-6f + 1e-30f;  // two mixes' rounding, at most ~1e-6 of the magnitude
+  const f2 c0 = clip_zw(m, dn - r), c1 = clip_zw(m, dx + r);
+  if (!(isfinite(c0.x) && isfinite(c1.x))) return false;
+  const bool oneSign = (c0.y > 0.0f && c1.y > 0.0f) || (c0.y < 0.0f && c1.y < 0.0f);
+  return oneSign && (c0.x == c1.x || c0.y == c1.y);
+}
+
 struct TileWaveLds {
   // per segment (lane of the wave), field-major so that lanes reading
   // different segments hit different banks
@@ -220,6 +261,7 @@ struct TileWaveLds {
   float sp[kSpFields][64]; // span frame (span_frame)
   float ip[kIpFields][64]; // per-segment interpolation terms (interp_pre)
   int box[4][64];          // pixel box (seg_pixel_bbox)
+  uint32_t pairBin[64];    // 1: the bin is decided per pair (pair_bin_ok), not per pixel
   uint32_t rect[3][64];    // first tile x, first tile y, tiles per row
   uint32_t gid[64];        // segment index
   uint32_t kBase[64];      // first pair (tile of the rectangle) of this lane
@@ -280,8 +322,10 @@ __device__ __forceinline__ void tiles_shade_queue(const TilesArgs& a, TileWaveLd
     else if (hdp > bd.y) pass = false;
     else pass = !(hdp > tex_load(a.depth, a.depthW, a.depthH, (int32_t)x, (int32_t)(tg.vpH - y)));
     if (pass) {
-      const f3 vs = project(a.u->projMatrixInv, mk4(p.x, p.y, hdp, 1.0f));
-      vbin = (a.B - 1u) - depth_bin(a.B, dbVS, vs.z);
+      if (!L.pairBin[o]) {
+        const f3 vs = project(a.u->projMatrixInv, mk4(p.x, p.y, hdp, 1.0f));
+        vbin = (a.B - 1u) - depth_bin(a.B, dbVS, vs.z);
+      }
       // fminf/fmaxf in the reference skip a NaN depth; its bin still counts
       if (hdp == hdp) vmin = vmax = f32_ord(hdp);
     }
@@ -354,6 +398,7 @@ __device__ __forceinline__ void tiles_wave(const TilesArgs& a, TileWaveLds& L, u
     span_frame(ps, cx0 > -(1 << 20) && amax < 65536.0f, margin, &L.sp[0][lane], 64);
     interp_pre(ps, &L.ip[0][lane], 64);
     L.d[0][lane] = ps.depths.x; L.d[1][lane] = ps.depths.y; L.d[2][lane] = ps.depths.z; L.d[3][lane] = ps.depths.w;
+    L.pairBin[lane] = pair_bin_ok(u->projMatrixInv, ps.depths) ? 1u : 0u;
     L.box[0][lane] = cx0; L.box[1][lane] = cy0; L.box[2][lane] = cx1; L.box[3][lane] = cy1;
     L.rect[0][lane] = tminx; L.rect[1][lane] = tminy; L.rect[2][lane] = tsx;
     if (!mainPass) nTiles = kBase < nTiles ? min(kTilePiece, nTiles - kBase) : 0u;
@@ -487,6 +532,13 @@ __device__ __forceinline__ void tiles_wave(const TilesArgs& a, TileWaveLds& L, u
       dmax = ord_f32(L.accMax[lane]);
       dbin = L.accBin[lane];
       hit = !(dmin > 1.0f);
+      if (hit && L.pairBin[L.owner[lane]]) {
+        // the closest bin of the pair's passing pixels, from its extreme
+        // depths (pair_bin_ok)
+        const f2 c0 = clip_zw(u->projMatrixInv, dmin), c1 = clip_zw(u->projMatrixInv, dmax);
+        const uint32_t b0 = depth_bin(B, dbVS, c0.x / c0.y), b1 = depth_bin(B, dbVS, c1.x / c1.y);
+        dbin = (B - 1u) - max(b0, b1);
+      }
     }
     const uint64_t mask = __ballot(hit);
     if (mask != 0) {

@@ -225,6 +225,60 @@ def test_tiles_bit_exact(gpu, oracle, tile, bins, width, height, radius):
     _check_slot_tags(got_keys, g4[:, 3], n)
 
 
+@pytest.mark.parametrize("form", ["x_in_z", "z_and_w", "affine"])
+def test_tiles_depth_bins_projection_forms(gpu, oracle, form):
+    """The tile pass takes a segment's depth bin per (segment, tile) pair from
+    the pair's extreme depths when the view-space depth is provably monotone
+    in the hair depth (a perspective inverse: clip z constant), and per pixel
+    otherwise (hairTilesPass.wgsl.ts:197-206). Three other inverse
+    projections, uploaded after two frames: one whose z row reads the pixel's
+    x (per pixel), one whose clip z and w both vary with depth (per pixel),
+    and an affine one with w constant (per pair). Lists and depth ranges
+    equal the oracle's, which bins every pixel, and the bins are spread."""
+    import torch
+    wl, r = _renderer("small", 800, 320, 240, tile=16, bins=8)
+    _step_to(r, 2)
+    u = r.render_uniforms
+    m = u["projMatrixInv"][0].reshape(-1)  # column-major 4x4
+    if form == "x_in_z":
+        m[2] = np.float32(2e-4)
+    elif form == "z_and_w":
+        m[10] = np.float32(0.25)
+    else:
+        # view z = alpha * z + beta, spreading the hair's depths over its
+        # view-space bounds (hairTilesPass.wgsl.ts:242-246)
+        mv = u["modelViewMat"][0].reshape(4, 4).T.astype(np.float64)
+        c, rad = r.hair.sphere
+        vz = (mv @ np.array([c[0], c[1], c[2], 1.0]))[2]
+        proj = u["projMatrix"][0].reshape(4, 4).T.astype(np.float64)
+        ndc = lambda v: (proj[2, 2] * v + proj[2, 3]) / (proj[3, 2] * v + proj[3, 3])
+        za, zb = ndc(vz + rad), ndc(vz - rad)
+        alpha = ((vz + rad) - (vz - rad)) / (za - zb)
+        beta = (vz + rad) - alpha * za
+        m[:] = 0.0
+        m[0], m[5], m[10], m[14], m[15] = 1.0, 1.0, np.float32(alpha), np.float32(beta), 1.0
+    r.globalUniforms.copy_(torch.from_numpy(u.view(np.uint8).reshape(-1).copy()).to(gpu))
+    ctx = r.ctx()
+    r.tilesPass.cmdClearBeforeRender(ctx)
+    r.tilesPass.cmdDrawHairToTiles(ctx, r.hair)
+    torch.cuda.synchronize()
+    tp = r.tilesPass
+    got_tiles = _np(tp.hairTilesBuffer).view(np.uint32)
+    got_seg = _np(tp.hairTileSegmentsBuffer).view(np.uint32)
+    i = _tile_inputs(r)
+    et, es, _, _ = oracle.tiles_draw(wl.render, wl.width, wl.height, i["render_u"], i["hair_data"],
+                                     i["positions"], i["tangents"], i["depth"], tp.segmentsCapacity,
+                                     wl.strands, wl.points)
+    assert got_seg[0] == es[0] and es[0] > 0
+    g4, e4 = got_tiles.reshape(-1, 4), et.reshape(-1, 4)
+    assert np.array_equal(g4[:, :2], e4[:, :2]) and np.array_equal(g4[:, 3], e4[:, 3])
+    el = walk_lists(et, es, tp.segmentsCapacity)
+    assert walk_lists(got_tiles, got_seg, tp.segmentsCapacity) == el
+    used = np.bincount(np.array(sorted(el.keys())) % 8, minlength=8)
+    print(f"{form}: pairs per bin {used.tolist()}", flush=True)
+    assert (used > 0).sum() >= 3
+
+
 def test_tile_sort(gpu, oracle):
     import torch
     wl, r = _renderer("small", 800, 320, 240)
