@@ -163,3 +163,47 @@ def test_random_simulation_step(gpu, oracle, seed):
     torch.cuda.synchronize()
     grid = _np(r.grid.densityVelocity)
     assert np.array_equal(grid, oracle.grid_post_sim(wl.sim.gridDims, su, p2, n2, wl.strands, wl.points))
+
+
+SHADE_CASES = 16
+
+
+@pytest.mark.parametrize("seed", range(SHADE_CASES))
+def test_random_shading(gpu, oracle, seed):
+    """The shading pass (hairShadingPass.wgsl.ts:61-282) with the material,
+    the three lights, the shadow filter (PCF radius 0-4 or PCSS), the AO
+    strength and the shading point count drawn from the seed: shading points
+    within rtol 2e-4 of the oracle (the lobes' pow/exp are the hardware's on
+    the device, libm's in the oracle)."""
+    import torch
+    from frostbitten_hair_webgpu_amd import assets as A
+    from frostbitten_hair_webgpu_amd import passes as PS
+    rng = np.random.default_rng(0x5AD0000 + seed)
+    wl = A.make_workload("small", seed=seed, strands=int(rng.integers(100, 800)), width=256, height=192)
+    rc = wl.render
+    rc.shadingPoints = int(rng.choice([2, 3, 8, 16, 64]))
+    rc.material = dict(color0=list(rng.uniform(0.0, 1.0, 3)), color1=list(rng.uniform(0.0, 1.0, 3)),
+                       specular=float(rng.uniform(0.0, 2.0)), weightTT=float(rng.uniform(0.0, 2.0)),
+                       weightTRT=float(rng.uniform(0.0, 2.0)), shift=float(rng.uniform(-0.1, 0.1)),
+                       roughness=float(rng.uniform(0.05, 0.8)), colorRng=float(rng.uniform(0.0, 1.0)),
+                       lumaRng=float(rng.uniform(0.0, 1.0)), attenuation=float(rng.uniform(0.0, 100.0)),
+                       shadows=float(rng.uniform(0.0, 1.0)))
+    rc.lights = [dict(posPhi=float(rng.uniform(-180.0, 180.0)), posTheta=float(rng.uniform(5.0, 175.0)),
+                      color=list(rng.uniform(0.2, 1.0, 3)), energy=float(rng.uniform(0.1, 2.0))) for _ in range(3)]
+    rc.shadows = dict(rc.shadows, usePCSS=bool(rng.uniform() < 0.3), PCF_Radius=int(rng.integers(0, 5)),
+                      bias=float(rng.uniform(0.0001, 0.003)), posPhi=float(rng.uniform(0.0, 90.0)),
+                      posTheta=float(rng.uniform(20.0, 70.0)))
+    rc.ao = dict(rc.ao, strength=float(rng.uniform(0.0, 1.0)))
+    print(f"shading sweep case {seed}: {wl.strands} strands, {rc.shadingPoints} points, "
+          f"PCSS {rc.shadows['usePCSS']}, PCF radius {rc.shadows['PCF_Radius']}", flush=True)
+    r = PS.HairRenderer(wl, "cuda:0", shadow_size=512)
+    _step_to(r, 1)
+    r.shadingPass.cmdComputeShadingPoints(r.ctx(), r.hair)
+    torch.cuda.synchronize()
+    got = _np(r.hair.shading)
+    i = r.inputs
+    exp = oracle.shading(wl.render, r.render_uniforms, _np(r.hair.positionsNow), _np(r.hair.tangents),
+                         np.ascontiguousarray(i["shadowMap"].cpu().numpy()), np.ascontiguousarray(i["ao"].cpu().numpy()),
+                         np.ascontiguousarray(i["hairDepth"].cpu().numpy()), wl.strands, wl.points).reshape(-1, 4)
+    assert np.isfinite(got).all() and np.abs(exp[:, :3]).max() > 0
+    assert np.allclose(got, exp, rtol=2e-4, atol=2e-6)
