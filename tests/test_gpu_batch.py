@@ -171,3 +171,49 @@ def test_batch_generator_instances(gpu):
                 same = a[key] == b[key] if key in ("lists", "drawn") else np.array_equal(a[key], b[key])
                 assert same, f"frame {frame} instance {k}: {key} differs"
             assert a["img"][..., 3].max() > 0.0
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_batch_random_instances(gpu, seed):
+    """Six instances drawn from the seed (strand and point counts,
+    resolution, tile size, bins, slices, fiber width), merged and launched
+    as one batched frame from a captured graph: after each of three frames
+    every instance's state (positions, grid, tile lists, image, shading,
+    screen) equals its plain frame's."""
+    import torch
+    from frostbitten_hair_webgpu_amd import assets as A
+    from frostbitten_hair_webgpu_amd import batch as B
+    from frostbitten_hair_webgpu_amd import passes as PS
+    rng = np.random.default_rng(0xBA7C0 + seed)
+    wls = []
+    for i in range(6):
+        wl = A.make_workload(str(rng.choice(["small", "c2"])), seed=100 * seed + i,
+                             strands=int(rng.integers(32, 1500)), width=int(rng.integers(64, 520)),
+                             height=int(rng.integers(48, 400)), points=int(rng.choice([5, 8, 16, 24, 32])))
+        rc = wl.render
+        rc.tileSize = int(rng.choice([8, 16, 32]))
+        rc.tileDepthBins = int(rng.choice([1, 2, 4, 8, 16]))
+        rc.slicesPerPixel = int(rng.choice([1, 2, 4, 8]))
+        rc.fiberRadius *= float(rng.uniform(0.5, 3.0))
+        rc.avgSegmentsPerTile = 1024
+        wls.append(wl)
+    print(f"batch sweep {seed}: " + "; ".join(f"{wl.strands}x{wl.points} {wl.width}x{wl.height} "
+                                               f"t{wl.render.tileSize} b{wl.render.tileDepthBins} "
+                                               f"s{wl.render.slicesPerPixel}" for wl in wls), flush=True)
+    plain = [PS.HairRenderer(wl, "cuda:0", shadow_size=256) for wl in wls]
+    inputs = [{k: v.cpu().numpy() for k, v in r.inputs.items()} for r in plain]
+    br = B.BatchRenderer(wls, "cuda:0", inputs=inputs, use_graph=True)
+    for r in plain:
+        r.beforeFirstFrame()
+    br.beforeFirstFrame()
+    br.build()
+    for frame in range(3):
+        for r in plain:
+            r.frame()
+        br.frame()
+        torch.cuda.synchronize()
+        for k, (rp, rb) in enumerate(zip(plain, br.renderers)):
+            a, b = _state(rp), _state(rb)
+            for key in a:
+                same = a[key] == b[key] if key in ("lists", "drawn") else np.array_equal(a[key], b[key])
+                assert same, f"frame {frame} instance {k}: {key} differs"
