@@ -143,7 +143,11 @@ int ensure_scratch(fh_ctx* ctx, size_t bytes) {
   // discards such a recording
   if (ctx->recording) ctx->rec_grew = true;
   if (ctx->scratch) {
+    // every stream that may still read it: inside a branch the context's own
+    // stream too (calls made while the context records or times kernels ran
+    // a branch's scratch on it)
     FH_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->main_stream) FH_HIP(hipStreamSynchronize(ctx->main_stream));
     for (hipStream_t b : ctx->branch_stream)
       if (b) FH_HIP(hipStreamSynchronize(b));
     FH_HIP(hipFree(ctx->scratch));
