@@ -180,9 +180,21 @@ def read_traffic(kernel: str, workload: str):
     return None
 
 
-REFERENCE_CPU = ("unavailable: the reference's CPU path is its WGSL under Deno on a software Vulkan adapter "
-                 "(lavapipe); the GPU box has no deno binary, no Vulkan ICD (/usr/share/vulkan/icd.d, "
-                 "/etc/vulkan/icd.d) and no Deno module cache, and nothing can be fetched offline")
+VULKAN_ICD_DIRS = ("/usr/share/vulkan/icd.d", "/etc/vulkan/icd.d")
+
+
+def reference_cpu_status() -> str:
+    """Why the reference's own CPU path (its WGSL under Deno on lavapipe, a
+    software Vulkan adapter) is not timed, from what this machine has."""
+    import shutil
+    deno = shutil.which("deno")
+    icds = sorted(f for d in VULKAN_ICD_DIRS if os.path.isdir(d) for f in os.listdir(d) if f.endswith(".json"))
+    cache = os.environ.get("DENO_DIR") or os.path.expanduser("~/.cache/deno")
+    return ("unavailable: the reference's CPU path is its WGSL under Deno on a software Vulkan adapter "
+            f"(lavapipe); this machine has {'deno at ' + deno if deno else 'no deno binary'}, "
+            f"{'Vulkan ICDs ' + ', '.join(icds) if icds else 'no Vulkan ICD'} (in {', '.join(VULKAN_ICD_DIRS)}) "
+            f"and {'a Deno module cache at ' + cache if os.path.isdir(cache) else 'no Deno module cache'}; "
+            "the reference's modules are remote and nothing can be fetched offline")
 
 
 def host_threads() -> int:
@@ -241,7 +253,7 @@ def cpu_baseline(wl, r, frames: int = 8, budget_s: float = 15.0) -> dict:
             "sample": f"{frames} full frames (sim step + tile + sort + fine + shading) of the same workload, "
                       f"median {t:.3f} s/frame, the C oracle (a restatement of the reference's kernels) with its "
                       f"strand, grid-node and tile loops on {threads} host threads",
-            "reference": REFERENCE_CPU}
+            "reference": reference_cpu_status()}
 
 
 def batched_vs_plain(n: int, steps: int, warmup: int, tune: bool, dev, log, gap: bool = False,
