@@ -43,6 +43,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "sim+raster frames/s at 1080p, batched vs plain; p50/p99 frame latency"
 HBM_PEAK_GBS = 8000.0
+# wave64 f32 VALU instructions per second at full rate: 1,024 SIMDs (256 CUs)
+# at the 2.4 GHz peak engine clock, one non-packed instruction per 4 cycles
+VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4
 
 
 def kernel_bytes(wl, n_pairs: int, tiles: int) -> dict:
@@ -166,6 +169,23 @@ def full_frame(wl, r0, steps: int, dev, branches: bool = True, mesh_passes: bool
             "passes": passes, "pipelined": pipelined,
             "kernel_us": {k: round(t / n * 1e3, 2) for k, (n, t) in sorted(kernels.items(), key=lambda kv: -kv[1][1])
                           if k in extra or k in ("copy_u32", "fill_u32")}}
+
+
+def read_valu_instructions(kernel: str, workload: str):
+    """VALU instructions per launch of `kernel` (waves x SQ_INSTS_VALU per
+    wave) from the rocprofv3 counter table committed under profiles/ (None
+    when absent)."""
+    path = os.path.join(ROOT, "profiles", f"counters_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) > 5 and cells[0] == kernel:
+            try:
+                return float(cells[3]) * float(cells[5])
+            except ValueError:
+                return None
+    return None
 
 
 def read_traffic(kernel: str, workload: str):
@@ -710,6 +730,15 @@ def plain_run(kind: str, steps: int, warmup: int, args, dev) -> tuple:
             rooflines[k] = {"achieved": round(a, 2), "frac": round(a / HBM_PEAK_GBS, 5),
                             "avg_launch_us": round(kernel_ms[k] * 1e3, 2), "algorithmic_bytes": int(kb[k]),
                             "traffic": read_traffic(k, wl.name)}
+            # the issue bound these kernels sit against: their instructions at
+            # full rate (transcendentals take four times as long, so this is
+            # a lower bound of the VALU time and frac a lower bound of the
+            # VALU's use)
+            n_valu = read_valu_instructions(k, wl.name)
+            if n_valu:
+                bound_us = n_valu / VALU_ISSUE_PER_S * 1e6
+                rooflines[k]["valu"] = {"instructions": int(n_valu), "issue_bound_us": round(bound_us, 1),
+                                        "frac": round(bound_us / (kernel_ms[k] * 1e3), 3)}
     img = r.finePass.image(wl.width, wl.height)
     assert torch.isfinite(img).all()
     exhausted_bins = r.native.fine_exhausted_bins()  # of the last frame's fine pass
